@@ -1,0 +1,351 @@
+/*
+ * mpsr.h -- C ABI of the MI355X-native MPS rotating-wall timestep.
+ *
+ * This header replaces the reference's per-kernel launch ABI
+ * (solver/MPS_SOLVER_WITH_ROTATION/PMMultiGPU/Source/DeviceProcess.h:34-89, implemented in
+ * DeviceProcess.cu:29-304) and the per-device manager that drives it
+ * (CCellSet, Source/CGPUCellSet.cpp:1055-1459).  Plain C: POD structs, plain pointers and
+ * sizes, a status return on every call, no exceptions across the boundary.
+ *
+ * Ownership: the handle owns every device allocation.  The caller owns every host buffer it
+ * passes in or out.  A handle is not re-entrant; separate handles may run concurrently.
+ */
+#ifndef MPSR_H
+#define MPSR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CCTStatusType, same order (solver/Common/inc/DataType.h:252-261). */
+typedef enum mpsr_status {
+    MPSR_NOERR = 0,
+    MPSR_FILEERR = 1,
+    MPSR_PARAMERR = 2,
+    MPSR_CALCERR = 3,
+    MPSR_MEMERR = 4,
+    MPSR_GPUERR = 5, /* CCT_CUDAERR */
+    MPSR_ETCERR = 6
+} mpsr_status;
+
+typedef struct mpsr_float3 { float x, y, z; } mpsr_float3;
+typedef struct mpsr_int3 { int x, y, z; } mpsr_int3;
+
+/* CParameter field for field (solver/Common/inc/DataType.h:426-627).  C++ bool members are
+ * carried as int.  The reference spells several names this way; they are kept so a binding
+ * copies field by field. */
+typedef struct mpsr_params {
+    int Dimension;
+    mpsr_float3 GravityConstant;
+    float Density;
+    float ConstantParticleDensity;
+    float SurfaceTensionCoefficient;
+    float InitialParticleDistance;
+    float FreeSurfaceThreshold;
+    float GradientInfluenceRadiusCoefficient;
+    float LaplacianInfluenceRadiusCoefficient;
+    float TemperatureInfluenceRadiusCoefficient;
+    float FruidInitializeialTemperature;
+    float WallInitializeialTemperature;
+    float FruidThermalResistance;
+    float WallThermalResistance;
+    float FruidThermalConductivity;
+    float WallThermalConductivity;
+    float FruidSpecificHeat;
+    float FruidDensity;
+    float Dt;
+    int AnalysisStep;
+    int ThermalStep;
+    int NegativeCalculationStep;
+    int OutputStep;
+    int LambdaID;
+    float SORJacobianRelaxationCoefficient;
+    int SORLoopMax;
+    float SORConvergenceConstant;
+    int bUpdateTemperature;
+    float LambdaValue;
+    mpsr_float3 CellSize;
+    mpsr_float3 MvVelocity;
+    float CourantNumber;
+    float ViscosityUpdateParameterA;
+    float ViscosityUpdateParameterB;
+    float MaxWallTemperature;
+    float MinWallTemperature;
+    float OffSet;
+    int ParticleGenerationStep;
+    int TotalGenerationStep;
+    float Alpha;
+    float AlphaLightWeight;
+    int MaxNeighborhoodNum;
+    int bExplicitCollisonCheck;
+    int bExplicitCollisionOut;
+    int bImplicitCollisonCheck;
+    float CollisionCheckCoefficient;
+    float MinDistanceCoefficient;
+    int bFreeSurfacePressureUpdate;
+    float BasePressure;
+    float MaxPressure;
+    float Time1;
+    int TriggerStep;
+    int EndStep;
+    float AnalysisArea;
+    int bExplicitOut;
+    float WallFrictionCoeffieicnt;
+    float DragCoefficient;
+    int bTwoPhase;
+    int StopWallStep;
+    float Tolerance;
+    float ImplicitSurfaceThreshold;
+    int bCGSolver;
+    int bUpdateInnerPressure;
+    float SurfaceInfluenceRadiusCoefficient;
+    float SurfaceThreshold;
+    float GradientForce;
+    int StartStep;
+    float InitialForce;
+    float VelocityThreshold;
+    int bExplicitPressure;
+    float SonicVelocity;
+    float ArtificialPressure;
+    float r_coeff;
+    int bImplicitVelocity;
+    int bNJPCGSolver;
+    float PressureMax;
+    float RLimitCoefficient;
+    int bTurbulance;
+    float SmagorinskyConstant;
+    float FilterWidth;
+    float InitialTimeForZeroAngle;
+    float FinalTimeToReachAngle;
+    float DragPermiabilityConstant;
+} mpsr_params;
+
+/* CBox / CGridBox (DataType.h:397-410). */
+typedef struct mpsr_box { mpsr_float3 m_MinBound, m_MaxBound; } mpsr_box;
+typedef struct mpsr_gridbox {
+    mpsr_box m_BufferedZone;
+    mpsr_box m_ComputeZone;
+    mpsr_box m_PressureZone;
+    mpsr_box m_ParentZone;
+    mpsr_int3 m_GridSize;
+    float m_CellSize;
+} mpsr_gridbox;
+
+/* CTriangle (DataType.h:296-301). */
+typedef struct mpsr_triangle {
+    int ModelIDNumber;
+    mpsr_float3 Normal;
+    mpsr_float3 Vertex[3];
+} mpsr_triangle;
+
+/* CTriangleParameters (DataType.h:327-352), one per STL model. */
+typedef struct mpsr_tri_params {
+    mpsr_float3 Velocity;
+    float Temperature;
+    int LinearMovableOrNot;
+    int LinearResetInterval;
+    int RotatingOrNot;
+    float AngleOfRotationInDegree;
+    float RotationStartTime;
+    float TimeToReachRPM;
+    mpsr_float3 CenterOfRotation;
+    mpsr_float3 SecondPointOfRotation;
+    float WallFrictionCoefficient;
+    float WallThermalResistivity;
+    float WallThermalConductivity;
+    int isDrag;
+    int DragTriangleID;
+} mpsr_tri_params;
+
+/* DragParameter (DataType.h:302-326), one per drag STL. */
+typedef struct mpsr_drag_params {
+    int DragID;
+    char DragToken;            /* 'E' exponential, 'C' constant velocity, 'F' velocity factor */
+    char InputVelocityToken1;  /* 'V', 'T' or 'B' */
+    char OutputVelocityToken2;
+    float CoefficientofInflueneRegion;
+    float VelocityIndex;
+    float VelocityExponential;
+    float VelocityDragPermiabilityConstant;
+    float TempIndex;
+    float TempExponential;
+    float TempDragPermiabilityConstant;
+    float ConstantVx;
+    float ConstantVy;
+    float ConstantVz;
+    float VelocityMagnitueFactor;
+} mpsr_drag_params;
+
+/* Host view of the particle state, one row per slot (CParticle, DataType.h:368-390).  Vector
+ * fields are xyz-interleaved like the reference's Scalar3 arrays.  Scalars are double on the
+ * host side for both builds; an FP32 handle rounds them to float on upload, which is exact for
+ * values that came from float. */
+typedef struct mpsr_particles {
+    int n;
+    int *id;
+    double *position;          /* [3n] */
+    double *velocity;          /* [3n] */
+    double *pressure;          /* [n] */
+    double *density;           /* [n] particle number density */
+    double *temperature;       /* [n] */
+    double *kinetic_viscosity; /* [n] */
+    double *solid_phase_rate;  /* [n] */
+    int *type;                 /* [n] ParticleType: 0 heavy, 1 heavy free surface, 2 light */
+} mpsr_particles;
+
+typedef enum mpsr_precision { MPSR_FP32 = 0, MPSR_FP64 = 1 } mpsr_precision;
+
+/* How the pressure of the corrector is obtained.  EOS is the reference's live path
+ * (CalcPressureExplicit_Kernel, DeviceProcess_Kernel.h:783-787).  PPE fills the slot the
+ * reference declares but never defines (CalcPoisson, DeviceProcess.h:50-53). */
+typedef enum mpsr_pressure_mode { MPSR_PRESSURE_EOS = 0, MPSR_PRESSURE_PPE = 1 } mpsr_pressure_mode;
+
+typedef enum mpsr_pcg_kind {
+    MPSR_PCG_FP64 = 0,  /* FP64 CSR values, FP64 vectors, Jacobi preconditioner */
+    MPSR_PCG_MIXED = 1  /* FP32 CSR values and SpMV, FP64 dots and norms, FP64 refinement */
+} mpsr_pcg_kind;
+
+typedef enum mpsr_warm_start {
+    MPSR_WARM_ZERO = 0,     /* x0 = 0 */
+    MPSR_WARM_PREVIOUS = 1, /* x0 = the previous step's pressure */
+    MPSR_WARM_LEARNED = 2   /* x0 and preconditioner from the message-passing network */
+} mpsr_warm_start;
+
+typedef struct mpsr_ppe_options {
+    int solver;        /* mpsr_pcg_kind */
+    double rel_tol;    /* stop when ||b - A x||_2 / ||b||_2 <= rel_tol (FP64 residual) */
+    int max_iter;      /* per solve, over all refinement passes */
+    int warm_start;    /* mpsr_warm_start */
+    double n0_lap;     /* PND of the ideal lattice at the Laplacian radius; 0 = computed */
+    double lambda_lap; /* Laplacian model constant at the Laplacian radius; 0 = computed */
+} mpsr_ppe_options;
+
+typedef struct mpsr_create_info {
+    mpsr_params params;          /* LambdaValue must already be set (mpsr_calculate_lambda) */
+    mpsr_box compute_zone;       /* as CMultiGPUHandler::m_ComputeZone (mpsr_auto_compute_zone) */
+    int max_particles;           /* slots to allocate, >= particles->n */
+    const mpsr_particles *particles;
+    const mpsr_triangle *triangles;
+    int num_triangles;
+    const mpsr_tri_params *models;
+    int num_models;
+    const mpsr_drag_params *drag;
+    int num_drag;
+    int precision;               /* mpsr_precision */
+    int pressure_mode;           /* mpsr_pressure_mode */
+    mpsr_ppe_options ppe;
+    int device;
+} mpsr_create_info;
+
+/* Wall-clock of the last step's phases, measured with HIP events on the handle's stream. */
+typedef struct mpsr_timings {
+    double step_ms;
+    double bin_ms;        /* both particle binnings: hash + radix sort + cell ranges + gather */
+    double tri_ms;        /* triangle binning */
+    double stl_ms;        /* both STL distance passes */
+    double drag_ms;
+    double explicit_ms;
+    double relocate_ms;   /* both relocations */
+    double pnd_ms;        /* PND + EOS, or PND alone on the PPE path */
+    double ppe_ms;        /* PPE assembly + solve (0 on the EOS path) */
+    double gradient_ms;
+    double wall_ms;       /* translation + rotation */
+    int ppe_iterations;   /* PCG iterations of the last solve, summed over refinement passes */
+    int ppe_refinements;
+    double ppe_rel_residual; /* FP64 ||b - A x|| / ||b|| after the last solve */
+    double ppe_initial_rel_residual;
+} mpsr_timings;
+
+typedef struct mpsr_handle mpsr_handle;
+
+/* ---- set-up helpers (host only, no GPU) ---------------------------------------------------- */
+
+/* CMultiGPUHandler::CalculateLambdaValueLocal (Source/MultiGPUHandler.cpp:1132-1296). */
+float mpsr_calculate_lambda(float initial_particle_distance);
+
+/* CMultiGPUHandler::SetComputeZone from particles and triangles (MultiGPUHandler.cpp:111-150). */
+mpsr_status mpsr_auto_compute_zone(const mpsr_params *params, const mpsr_particles *particles,
+                                   const mpsr_triangle *triangles, int num_triangles,
+                                   mpsr_box *out);
+
+/* GetSectionComputeZone(zone, (1,1,1), 0,0,0) then CCellSet::SetComputeZone
+ * (solver/Common/inc/Utility_Inline.h:1977-2006, Source/CGPUCellSet.cpp:209-253). */
+mpsr_status mpsr_make_gridbox(const mpsr_params *params, const mpsr_box *compute_zone,
+                              mpsr_gridbox *out);
+
+/* ---- lifetime ------------------------------------------------------------------------------ */
+
+/* CCellSet::InitializeGPU + InitializeDeviceMemConst (CGPUCellSet.cpp:439-659, DeviceProcess.cu:220-264). */
+mpsr_status mpsr_create(const mpsr_create_info *info, mpsr_handle **out);
+void mpsr_destroy(mpsr_handle *h);
+const char *mpsr_last_error(void);
+
+/* ---- the step ------------------------------------------------------------------------------ */
+
+/* One body of CMultiGPUHandler::CalculateNew's loop (MultiGPUHandler.cpp:362-492) without the
+ * output and bucket branches: bin, bin walls, STL distance, drag, predictor, relocate, re-bin,
+ * STL distance, pressure (EOS or PPE), corrector, relocate, translate and rotate walls. */
+mpsr_status mpsr_step(mpsr_handle *h, int step_index);
+/* count consecutive steps first..first+count-1 on the handle's stream, one sync at the end. */
+mpsr_status mpsr_step_many(mpsr_handle *h, int first, int count);
+mpsr_status mpsr_synchronize(mpsr_handle *h);
+
+/* ---- one launcher per reference launcher (DeviceProcess.h:34-89) ---------------------------- */
+
+mpsr_status mpsr_register_particle_topology(mpsr_handle *h);  /* RegisterParticleTopology :36 */
+mpsr_status mpsr_register_triangle_topology(mpsr_handle *h);  /* RegisterTriangleTopology :54 */
+mpsr_status mpsr_calc_stl_distance(mpsr_handle *h);           /* CalcSTLDistance :62 */
+mpsr_status mpsr_calc_drag_effect(mpsr_handle *h);            /* CalcDragEffect :88 */
+mpsr_status mpsr_calc_explicitly(mpsr_handle *h);             /* CalcExplicitly :48 */
+mpsr_status mpsr_relocate_particle_data(mpsr_handle *h);      /* RelocateParticleData :64 */
+mpsr_status mpsr_calc_pressure_explicit(mpsr_handle *h);      /* CalcPressureExplicit :70 */
+mpsr_status mpsr_calc_pressure_poisson(mpsr_handle *h);       /* CalcPoisson :50 (declared only there) */
+mpsr_status mpsr_calc_implicit_explicitly(mpsr_handle *h);    /* CalcImplicitExplicitly :49 */
+mpsr_status mpsr_update_triangle_position(mpsr_handle *h);    /* UpdateTrianglePosition :58 */
+mpsr_status mpsr_rotate_triangle_position(mpsr_handle *h, int analysis_step); /* RotateTrianglePosition :65 */
+
+/* ---- state in and out ---------------------------------------------------------------------- */
+
+/* which = 0: the "in" arrays (c_daParticle*), 1: the "out" arrays (c_daOutputParticle*).
+ * = CCellSet::ParticleDeviceToHost (CGPUCellSet.cpp:833-892). */
+mpsr_status mpsr_get_particles(mpsr_handle *h, int which, mpsr_particles *out);
+/* = ParticleHostToDevice + ParticleNumberToConst (CGPUCellSet.cpp:893-952, DeviceProcess.cu:265-271). */
+mpsr_status mpsr_set_particles(mpsr_handle *h, const mpsr_particles *in);
+mpsr_status mpsr_get_triangles(mpsr_handle *h, mpsr_triangle *out, int n);
+mpsr_status mpsr_get_gridbox(mpsr_handle *h, mpsr_gridbox *out);
+/* = CheckAndUpdateDt's constant upload (CGPUCellSet.cpp:1278). */
+mpsr_status mpsr_set_dt(mpsr_handle *h, float dt, float courant);
+mpsr_status mpsr_get_timings(mpsr_handle *h, mpsr_timings *out);
+mpsr_status mpsr_set_ppe_options(mpsr_handle *h, const mpsr_ppe_options *opt);
+
+/* ---- inspection of intermediate results (parity tests) ---------------------------------------
+ * Binning after the last mpsr_register_particle_topology: sorted keys and sorted-to-slot
+ * permutation ([n] each; key == cell count marks a slot that is not binned), and cellStart /
+ * cellEnd ([cells] each, -1 = empty), as radix DeviceProcess_Kernel.h:39-124. */
+mpsr_status mpsr_get_binning(mpsr_handle *h, int *sorted_keys, int *perm, int *cell_start,
+                             int *cell_end);
+/* Triangle lists per cell after mpsr_register_triangle_topology: start/count per cell and the
+ * ascending triangle ids (capacity cap; *total receives the entry count). */
+mpsr_status mpsr_get_triangle_cells(mpsr_handle *h, int *cell_start, int *cell_count,
+                                    int *tri_ids, int cap, int *total);
+/* Nearest-wall results of the last mpsr_calc_stl_distance, slot order: magnitude [n],
+ * direction [3n], triangle id [n] (CDistance / STLID, DeviceProcess_Kernel.h:635-699). */
+mpsr_status mpsr_get_stl_distance(mpsr_handle *h, double *magnitude, double *direction,
+                                  int *triangle_id);
+/* Neighbour lists in the canonical order (stencil z, y, x with x innermost, then ascending
+ * sorted index) for radius r in units of l0: offsets [n+1] and neighbour slots (cap). */
+mpsr_status mpsr_get_neighbors(mpsr_handle *h, double radius_over_l0, int *offsets,
+                               int *neighbor_slots, int cap, int *total);
+/* The last PPE system in slot order: rhs [n], solution [n], Dirichlet flag [n]. */
+mpsr_status mpsr_get_ppe(mpsr_handle *h, double *rhs, double *solution, int *dirichlet);
+/* The last PPE matrix in slot order: row pointers [n+1], column slots and off-diagonal values
+ * (capacity cap), diagonal [n]; *nnz receives the off-diagonal count. */
+mpsr_status mpsr_get_ppe_matrix(mpsr_handle *h, int *row_ptr, int *col, double *val, double *diag,
+                                int cap, int *nnz);
+/* Per-phase HIP-event timing of the following steps (mpsr_get_timings); off by default. */
+mpsr_status mpsr_set_profiling(mpsr_handle *h, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPSR_H */

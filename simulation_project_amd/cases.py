@@ -1,0 +1,338 @@
+"""Synthetic input cases for the MPS timestep: dam break, rotating drum, small boxes.
+
+Everything a reference input deck carries (parameter.dat, particle.dat, STL models and the
+per-model columns of the configuration file) is produced here in memory, with the reference's
+own conventions:
+  * particle ids are row indices (Source/IO.cpp:61);
+  * triangle normals are recomputed from the winding with the sqrt14 square root
+    (Source/IO.cpp:435-453);
+  * the rotation angle per step is RPM * dt * 6 degrees (Source/MultiGPUHandler.cpp:690);
+  * lambda comes from the reference's piecewise fit (MultiGPUHandler.cpp:1132-1296) and the
+    compute zone from the particle + triangle bounding box (MultiGPUHandler.cpp:111-150), both
+    through the library's host helpers.
+Lattice sites sit a whole number of spacings from the planar walls: the wall weight tables of
+the reference (Utility_Inline.h:199-686) count the missing neighbours as lattice layers at
+distances r, r + l0, ... behind a wall at distance r, so a first fluid layer at l0 from the wall
+starts at the lattice PND n0.  Every site gets a uniform jitter of +-1e-4 l0 per axis from a
+counter-based SplitMix64 stream keyed by (seed, particle, axis).
+"""
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+DEFAULT_SEED = 20261202
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(x):
+    x = (x + _GOLDEN).astype(np.uint64)
+    z = x
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)).astype(np.uint64)
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)).astype(np.uint64)
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed, index, axis):
+    """U[0,1) for each (seed, index, axis); index is an integer array."""
+    with np.errstate(over="ignore"):
+        ctr = (np.asarray(index, np.uint64) * np.uint64(3) + np.uint64(axis) + np.uint64(1)) * _GOLDEN
+        x = splitmix64(np.uint64(seed) ^ ctr)
+    return (x >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def jitter(seed, n, amplitude):
+    idx = np.arange(n, dtype=np.uint64)
+    return np.stack([(2.0 * uniform01(seed, idx, a) - 1.0) * amplitude for a in range(3)], axis=1)
+
+
+def sqrt14_f32(x):
+    """The reference's approximate square root (solver/Common/inc/Utility_Inline.h:13-26)."""
+    x = np.asarray(x, np.float32)
+    xhalf = np.float32(0.5) * x
+    i = x.view(np.int32)
+    y = (np.int32(0x5F3759DF) - (i >> np.int32(1))).astype(np.int32).view(np.float32)
+    return (x * y) * (np.float32(1.5) - (xhalf * y) * y)
+
+
+def triangle_normals(v):
+    """CalcTriangleNormal (Source/IO.cpp:435-453) in float32: cross product of (v1-v0, v2-v0)
+    divided by its sqrt14 length.  v: float32 [T,3,3]."""
+    v = np.asarray(v, np.float32)
+    a = v[:, 1] - v[:, 0]
+    b = v[:, 2] - v[:, 0]
+    nx = a[:, 1] * b[:, 2] - b[:, 1] * a[:, 2]
+    ny = a[:, 2] * b[:, 0] - b[:, 2] * a[:, 0]
+    nz = a[:, 0] * b[:, 1] - b[:, 0] * a[:, 1]
+    ab = sqrt14_f32(nx * nx + ny * ny + nz * nz)
+    out = np.zeros((len(v), 3), np.float32)
+    ok = ab != np.float32(0.0)
+    out[ok, 0] = nx[ok] / ab[ok]
+    out[ok, 1] = ny[ok] / ab[ok]
+    out[ok, 2] = nz[ok] / ab[ok]
+    return out
+
+
+def default_params(l0, dt, *, grad=2.1, lap=3.1, rho=1000.0, nu=1e-6, g=(0.0, 0.0, -9.81), n0=14.418575,
+                   beta=0.97, sonic=20.0, courant=0.2, wall_friction=1.0, explicit_pressure=False,
+                   artificial=0.0, r_coeff=0.0, tolerance=0.0, steps=100):
+    p = abi.Params()
+    p.Dimension = 3
+    p.GravityConstant = abi.Float3(*g)
+    p.Density = rho
+    p.ConstantParticleDensity = n0
+    p.InitialParticleDistance = l0
+    p.FreeSurfaceThreshold = beta
+    p.GradientInfluenceRadiusCoefficient = grad
+    p.LaplacianInfluenceRadiusCoefficient = lap
+    p.TemperatureInfluenceRadiusCoefficient = grad
+    p.FruidSpecificHeat = 1.0
+    p.FruidDensity = rho
+    p.Dt = dt
+    p.AnalysisStep = steps
+    p.ThermalStep = 1
+    p.NegativeCalculationStep = 1
+    p.OutputStep = steps
+    p.CourantNumber = courant
+    p.WallFrictionCoeffieicnt = wall_friction
+    p.StopWallStep = 2 ** 31 - 1
+    p.Tolerance = tolerance
+    p.bExplicitPressure = 1 if explicit_pressure else 0
+    p.SonicVelocity = sonic
+    p.ArtificialPressure = artificial
+    p.r_coeff = r_coeff
+    p.PressureMax = 3.4e38
+    p.RLimitCoefficient = 3.4e38
+    p.DragPermiabilityConstant = 1.0
+    p.MaxNeighborhoodNum = 729
+    return p
+
+
+def tri_params(velocity=(0, 0, 0), rotating=False, rpm=0, dt=0.0, center=(0, 0, 0), second=(0, 0, 0),
+               start=0.0, reach=0.0, friction=0.0, is_drag=False, drag_id=-1, temperature=0.0):
+    m = abi.TriParams()
+    m.Velocity = abi.Float3(*velocity)
+    m.Temperature = temperature
+    m.RotatingOrNot = 1 if rotating else 0
+    # MultiGPUHandler.cpp:690: RotationInDegree = RotationInRPM * Dt * 6 (int * float * int)
+    m.AngleOfRotationInDegree = float(np.float32(np.float32(rpm) * np.float32(dt)) * np.float32(6))
+    m.RotationStartTime = start
+    m.TimeToReachRPM = reach
+    m.CenterOfRotation = abi.Float3(*center)
+    m.SecondPointOfRotation = abi.Float3(*second)
+    m.WallFrictionCoefficient = friction
+    m.isDrag = 1 if is_drag else 0
+    m.DragTriangleID = drag_id
+    return m
+
+
+def box_triangles(lo, hi):
+    """12 triangles of an axis-aligned box (two per face), float32 [12,3,3]."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    c = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], np.float64)
+    faces = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (2, 3, 7, 6), (1, 2, 6, 5), (0, 4, 7, 3)]
+    tris = []
+    for a, b, cc, d in faces:
+        tris.append([c[a], c[b], c[cc]])
+        tris.append([c[a], c[cc], c[d]])
+    return np.asarray(tris, np.float32)
+
+
+def cylinder_triangles(radius, length, center_yz, n_theta=100, n_x=31, n_r=10):
+    """Closed cylinder along x from 0 to length: side rings plus ring-subdivided end caps.
+    Defaults give 100*31*2 + 2*100*(2*10-1) = 10000 triangles."""
+    yc, zc = center_yz
+    th = np.linspace(0.0, 2.0 * math.pi, n_theta + 1)[:-1]
+    ring = np.stack([np.cos(th), np.sin(th)], axis=1)
+    xs = np.linspace(0.0, length, n_x + 1)
+    tris = []
+    for i in range(n_x):
+        for k in range(n_theta):
+            k2 = (k + 1) % n_theta
+            p00 = (xs[i], yc + radius * ring[k, 0], zc + radius * ring[k, 1])
+            p01 = (xs[i], yc + radius * ring[k2, 0], zc + radius * ring[k2, 1])
+            p10 = (xs[i + 1], yc + radius * ring[k, 0], zc + radius * ring[k, 1])
+            p11 = (xs[i + 1], yc + radius * ring[k2, 0], zc + radius * ring[k2, 1])
+            tris.append([p00, p10, p11])
+            tris.append([p00, p11, p01])
+    rs = np.linspace(0.0, radius, n_r + 1)
+    for x in (0.0, length):
+        for j in range(n_r):
+            for k in range(n_theta):
+                k2 = (k + 1) % n_theta
+                a0 = (x, yc + rs[j] * ring[k, 0], zc + rs[j] * ring[k, 1])
+                a1 = (x, yc + rs[j] * ring[k2, 0], zc + rs[j] * ring[k2, 1])
+                b0 = (x, yc + rs[j + 1] * ring[k, 0], zc + rs[j + 1] * ring[k, 1])
+                b1 = (x, yc + rs[j + 1] * ring[k2, 0], zc + rs[j + 1] * ring[k2, 1])
+                if j == 0:
+                    tris.append([a0, b0, b1])
+                else:
+                    tris.append([a0, b0, b1])
+                    tris.append([a0, b1, a1])
+    return np.asarray(tris, np.float32)
+
+
+@dataclass
+class Case:
+    name: str
+    params: abi.Params
+    particles: abi.ParticleArrays
+    vertices: np.ndarray             # float32 [T,3,3]
+    model_of_triangle: np.ndarray    # int32 [T]
+    models: list
+    drag: list = field(default_factory=list)
+    zone: abi.Box = None
+    steps: int = 100
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return self.particles.n
+
+    def triangles(self):
+        T = len(self.vertices)
+        arr = (abi.Triangle * max(1, T))()
+        nrm = triangle_normals(self.vertices) if T else np.zeros((0, 3), np.float32)
+        for t in range(T):
+            arr[t].ModelIDNumber = int(self.model_of_triangle[t])
+            arr[t].Normal = abi.Float3(*map(float, nrm[t]))
+            for k in range(3):
+                arr[t].Vertex[k] = abi.Float3(*map(float, self.vertices[t, k]))
+        return arr
+
+
+def _finish(case, lib):
+    """Lambda fit and automatic compute zone through the library's host helpers."""
+    case.params.LambdaValue = lib.mpsr_calculate_lambda(C.c_float(case.params.InitialParticleDistance))
+    tris = case.triangles()
+    zone = abi.Box()
+    ps = case.particles.struct()
+    st = lib.mpsr_auto_compute_zone(C.byref(case.params), C.byref(ps), tris, len(case.vertices), C.byref(zone))
+    if st != 0:
+        raise RuntimeError("mpsr_auto_compute_zone failed")
+    case.zone = zone
+    return case
+
+
+def _particles(pos, nu):
+    n = len(pos)
+    pa = abi.ParticleArrays(n)
+    pa.id[:] = np.arange(n, dtype=np.int32)
+    pa.position[:] = pos.astype(np.float32).astype(np.float64)
+    pa.kinetic_viscosity[:] = np.float64(np.float32(nu))
+    return pa
+
+
+def dam_break(lib, l0=0.01, seed=DEFAULT_SEED, explicit_pressure=True, steps=100):
+    """Dam break: water column 0.2 x 0.1 x 0.2 m in the corner of a 0.5 x 0.2 x 0.4 m box tank
+    given as 12 triangles; l0 = 0.01 m gives a 20 x 10 x 20 lattice (4000 particles) with sites at
+    l0, 2 l0, ... from the corner walls."""
+    nx, ny, nz = int(round(0.2 / l0)), int(round(0.1 / l0)), int(round(0.2 / l0))
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    pos = (np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1) + 1.0) * l0
+    pos = pos + jitter(seed, len(pos), 1e-4 * l0)
+    p = default_params(l0, 1e-4, explicit_pressure=explicit_pressure,
+                       artificial=0.01 if explicit_pressure else 0.0,
+                       r_coeff=0.7 if explicit_pressure else 0.0, steps=steps)
+    verts = box_triangles((0.0, 0.0, 0.0), (0.5, 0.2, 0.4))
+    case = Case("dam_break", p, _particles(pos, 1e-6), verts, np.zeros(len(verts), np.int32),
+                [tri_params(friction=1.0)], steps=steps, meta={"l0": l0, "seed": seed})
+    return _finish(case, lib)
+
+
+def drum(lib, l0=2.5e-3, fill=0.4, rpm=30, radius=0.15, length=0.3, seed=DEFAULT_SEED, dt=5e-5,
+         n_theta=100, n_x=31, n_r=10, steps=200, explicit_pressure=False):
+    """Rotating drum: cylinder of radius 0.15 m and length 0.3 m along x (10k triangles by
+    default) spinning at rpm about its axis, filled to the given area fraction.  Lattice sites
+    are l0 apart, at l0, 2 l0, ... from the end caps and at least l0 inside the mantle.  l0 = 2.5e-3 m and l0 = 2.039e-3 m are the two sizes
+    quoted for this case (about 0.54M and 1M particles)."""
+    yc = zc = radius
+    # fill level h (relative to the axis) whose circular segment below holds `fill` of the disc
+    lo_h, hi_h = -radius, radius
+    for _ in range(200):
+        h = 0.5 * (lo_h + hi_h)
+        d = -h
+        seg = radius * radius * math.acos(max(-1.0, min(1.0, d / radius))) - d * math.sqrt(max(0.0, radius * radius - d * d))
+        if seg / (math.pi * radius * radius) < fill:
+            lo_h = h
+        else:
+            hi_h = h
+    h = 0.5 * (lo_h + hi_h)
+    nx = int(round(length / l0)) - 1
+    m = int(radius / l0)
+    off = np.arange(-m, m + 1) * l0
+    yy, zz = np.meshgrid(yc + off, zc + off, indexing="ij")
+    rr = (yy - yc) ** 2 + (zz - zc) ** 2
+    inside = (rr <= (radius - l0) ** 2) & ((zz - zc) < h)
+    yz = np.stack([yy[inside], zz[inside]], axis=1)
+    xs = (np.arange(nx) + 1.0) * l0
+    pos = np.empty((nx * len(yz), 3))
+    pos[:, 0] = np.repeat(xs, len(yz))
+    pos[:, 1] = np.tile(yz[:, 0], nx)
+    pos[:, 2] = np.tile(yz[:, 1], nx)
+    pos = pos + jitter(seed, len(pos), 1e-4 * l0)
+    p = default_params(l0, dt, steps=steps, explicit_pressure=explicit_pressure,
+                       artificial=0.01 if explicit_pressure else 0.0,
+                       r_coeff=0.7 if explicit_pressure else 0.0)
+    verts = cylinder_triangles(radius, length, (yc, zc), n_theta, n_x, n_r)
+    m = tri_params(rotating=True, rpm=rpm, dt=dt, center=(0.0, yc, zc), second=(length, yc, zc),
+                   friction=1.0)
+    case = Case("drum", p, _particles(pos, 1e-6), verts, np.zeros(len(verts), np.int32), [m],
+                steps=steps, meta={"l0": l0, "fill": fill, "rpm": rpm, "seed": seed})
+    return _finish(case, lib)
+
+
+def small_box(lib, nx=8, ny=6, nz=8, l0=0.01, seed=DEFAULT_SEED, rotating=False, rpm=60, dt=1e-4,
+              explicit_pressure=True, velocity_jitter=0.0):
+    """A small water block in a closed box tank, for fast parity tests.  With rotating=True the
+    tank spins about the x axis through its centre."""
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    pos = (np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1) + 1.0) * l0
+    pos = pos + jitter(seed, len(pos), 1e-4 * l0)
+    lo = (0.0, 0.0, 0.0)
+    hi = ((nx + 6) * l0, (ny + 4) * l0, (nz + 6) * l0)
+    p = default_params(l0, dt, explicit_pressure=explicit_pressure,
+                       artificial=0.01 if explicit_pressure else 0.0,
+                       r_coeff=0.7 if explicit_pressure else 0.0)
+    verts = box_triangles(lo, hi)
+    cy, cz = 0.5 * hi[1], 0.5 * hi[2]
+    m = tri_params(rotating=rotating, rpm=rpm, dt=dt, center=(0.0, cy, cz), second=(hi[0], cy, cz),
+                   friction=1.0)
+    pa = _particles(pos, 1e-6)
+    if velocity_jitter:
+        pa.velocity[:] = (jitter(seed + 1, len(pos), velocity_jitter).astype(np.float32)).astype(np.float64)
+    case = Case("small_box", p, pa, verts, np.zeros(len(verts), np.int32), [m], steps=20,
+                meta={"l0": l0, "seed": seed})
+    return _finish(case, lib)
+
+
+def create_info(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_EOS, ppe=None,
+                max_particles=None, device=0):
+    """Build an mpsr_create_info; returns (info, keepalive)."""
+    tris = case.triangles()
+    models = (abi.TriParams * max(1, len(case.models)))(*case.models)
+    drag = (abi.DragParams * max(1, len(case.drag)))(*case.drag) if case.drag else None
+    ps = case.particles.struct()
+    info = abi.CreateInfo()
+    info.params = case.params
+    info.compute_zone = case.zone
+    info.max_particles = max_particles or case.n
+    info.particles = C.pointer(ps)
+    info.triangles = C.cast(tris, C.POINTER(abi.Triangle))
+    info.num_triangles = len(case.vertices)
+    info.models = C.cast(models, C.POINTER(abi.TriParams))
+    info.num_models = len(case.models)
+    info.drag = C.cast(drag, C.POINTER(abi.DragParams)) if drag is not None else None
+    info.num_drag = len(case.drag)
+    info.precision = precision
+    info.pressure_mode = pressure_mode
+    if ppe is None:
+        ppe = abi.PpeOptions(abi.MPSR_PCG_FP64, 1e-8, 20000, abi.MPSR_WARM_ZERO, 0.0, 0.0)
+    info.ppe = ppe
+    info.device = device
+    return info, (tris, models, drag, ps, case)

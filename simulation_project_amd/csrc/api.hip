@@ -1,0 +1,839 @@
+// api.hip -- the handle, the step driver and the C ABI of include/mpsr.h.
+//
+// mpsr_step is one body of CMultiGPUHandler::CalculateNew's loop
+// (Source/MultiGPUHandler.cpp:362-492) with the output and bucket branches left to the caller:
+//   RegisterParticleTopology  -> hash, radix sort, cell ranges, gather
+//   RegisterTriangleTopology  -> triangle binning
+//   CalcSTLDistance
+//   CalculateDragEffect       (when drag STLs exist)
+//   CalcExplicitly
+//   RelocateAllParticleTopology
+//   RegisterParticleTopology
+//   CalcSTLDistance
+//   CalcPressureExplicit      (EOS) or PND + PPE solve
+//   CalcPressureExplicitGradient
+//   RelocateAllParticleTopology
+//   UpdateTrianglePosition, RotateTrianglePosition(i)
+// All launches go to one HIP stream; there is no host synchronisation inside a step (the
+// reference synchronises every step, MultiGPUHandler.cpp:1009-1018), so mpsr_step_many queues
+// `count` steps back to back and waits once.
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "mpsr_internal.h"
+#include "ppe.h"
+
+namespace mpsr {
+
+static thread_local std::string g_err;
+static mpsr_status fail(mpsr_status s, const std::string& m) {
+    g_err = m;
+    return s;
+}
+#define HIPC(x)                                                                                      \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return fail(MPSR_GPUERR, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+enum Phase { PH_BIN, PH_TRI, PH_STL, PH_DRAG, PH_EXPL, PH_RELOC, PH_PND, PH_PPE, PH_GRAD, PH_WALL, PH_COUNT };
+
+struct EngineBase {
+    virtual ~EngineBase() {}
+    virtual mpsr_status step_many(int first, int count) = 0;
+    virtual mpsr_status phase(int which, int arg) = 0;
+    virtual mpsr_status get_particles(int which, mpsr_particles* o) = 0;
+    virtual mpsr_status set_particles(const mpsr_particles* p) = 0;
+    virtual mpsr_status get_triangles(mpsr_triangle* o, int n) = 0;
+    virtual mpsr_status get_binning(int* k, int* p, int* cs, int* ce) = 0;
+    virtual mpsr_status get_triangle_cells(int* cs, int* cc, int* ids, int cap, int* total) = 0;
+    virtual mpsr_status get_stl(double* m, double* dir, int* id) = 0;
+    virtual mpsr_status get_neighbors(double r, int* off, int* nbr, int cap, int* total) = 0;
+    virtual mpsr_status get_ppe(double* rhs, double* x, int* dir) = 0;
+    virtual mpsr_status get_ppe_matrix(int* ptr, int* col, double* val, double* diag, int cap, int* nnz) = 0;
+    virtual mpsr_status set_dt(float dt, float courant) = 0;
+    virtual mpsr_status set_ppe(const mpsr_ppe_options* o) = 0;
+    virtual mpsr_status sync() = 0;
+    mpsr_gridbox gb{};
+    mpsr_timings tm{};
+    int profiling = 0;
+    int device = 0;
+};
+
+template <typename R> static Consts<R> make_consts(const mpsr_params& P, const mpsr_gridbox& gb, int ntri, int ndrag) {
+    Consts<R> c{};
+    c.bminx = (R)gb.m_BufferedZone.m_MinBound.x;
+    c.bminy = (R)gb.m_BufferedZone.m_MinBound.y;
+    c.bminz = (R)gb.m_BufferedZone.m_MinBound.z;
+    c.cminx = (R)gb.m_ComputeZone.m_MinBound.x;
+    c.cminy = (R)gb.m_ComputeZone.m_MinBound.y;
+    c.cminz = (R)gb.m_ComputeZone.m_MinBound.z;
+    c.cmaxx = (R)gb.m_ComputeZone.m_MaxBound.x;
+    c.cmaxy = (R)gb.m_ComputeZone.m_MaxBound.y;
+    c.cmaxz = (R)gb.m_ComputeZone.m_MaxBound.z;
+    c.cell = (R)gb.m_CellSize;
+    c.gx = gb.m_GridSize.x;
+    c.gy = gb.m_GridSize.y;
+    c.gz = gb.m_GridSize.z;
+    c.ncell = c.gx * c.gy * c.gz;
+    c.tol = (R)P.Tolerance;
+    c.l0 = (R)P.InitialParticleDistance;
+    c.dt = (R)P.Dt;
+    c.re_lap = (R)P.LaplacianInfluenceRadiusCoefficient * (R)P.InitialParticleDistance;
+    c.mind_lap = (R)((double)c.re_lap * 1e-3);
+    c.lapc = (R)((2.0 * P.Dimension) / (double)((R)P.LambdaValue * (R)P.ConstantParticleDensity));
+    c.rc = (R)P.r_coeff * c.l0;
+    c.art_p = (R)P.ArtificialPressure;
+    c.art_coeff = -(R)P.Dimension / ((R)P.ConstantParticleDensity * (R)P.Density);
+    c.mu_wall = (R)P.WallFrictionCoeffieicnt;
+    c.gx_acc = (R)P.GravityConstant.x;
+    c.gy_acc = (R)P.GravityConstant.y;
+    c.gz_acc = (R)P.GravityConstant.z;
+    c.expl = P.bExplicitPressure ? 1 : 0;
+    c.has_drag = ndrag > 0 ? 1 : 0;
+    c.re_grad = (R)P.GradientInfluenceRadiusCoefficient * (R)P.InitialParticleDistance;
+    c.mind_grad = (R)((double)c.re_grad * 1e-3);
+    c.eos_coef = (R)P.SonicVelocity * (R)P.SonicVelocity * (R)P.Density / (R)P.ConstantParticleDensity;
+    c.n0 = (R)P.ConstantParticleDensity;
+    c.beta_n0 = (R)P.ConstantParticleDensity * (R)P.FreeSurfaceThreshold;
+    c.c1 = (R)(2 * P.Dimension) / (R)P.ConstantParticleDensity;
+    c.c2 = (R)P.Density / ((R)P.Dt * (R)P.Dt);
+    c.c3 = -(R)P.Dt / (R)P.Density;
+    c.vmax = ((R)P.InitialParticleDistance * (R)P.CourantNumber / (R)P.Dt) * build_sqrt((R)(1.0 / 3.0));
+    c.density = (R)P.Density;
+    c.cp = (R)P.FruidSpecificHeat;
+    c.thermal_step = (R)P.ThermalStep;
+    c.ntri = ntri;
+    return c;
+}
+
+// RotateTrianglePosition_Kernel's per-model angle logic (Kern :945-971) and the axis of
+// CalcNormal(Center, Second) (Utility_Inline.h:58-67), evaluated once per model and step.
+template <typename R> static RotCoef<R> rot_coef(const mpsr_tri_params& m, const mpsr_params& P, int step) {
+    RotCoef<R> r{};
+    r.active = 0;
+    const R dt = (R)P.Dt;
+    if (!m.RotatingOrNot) return r;
+    if ((R)step * dt < (R)m.RotationStartTime) return r;
+    const R start = (R)m.RotationStartTime, reach = (R)m.TimeToReachRPM, deg = (R)m.AngleOfRotationInDegree;
+    const int req = (int)((reach - start) / dt);
+    const R passed = dt * (R)step;
+    R ang = 0;
+    if (passed < start || deg == 0) ang = 0;
+    else if (passed >= start && passed < reach) ang = (deg / (R)(req + 1)) * ((R)step - (start / dt));
+    else ang = deg;
+    if (ang == 0) return r;
+    const R cx = (R)m.CenterOfRotation.x, cy = (R)m.CenterOfRotation.y, cz = (R)m.CenterOfRotation.z;
+    const R qx = (R)m.SecondPointOfRotation.x, qy = (R)m.SecondPointOfRotation.y, qz = (R)m.SecondPointOfRotation.z;
+    const R mg = build_sqrt((qx - cx) * (qx - cx) + (qy - cy) * (qy - cy) + (qz - cz) * (qz - cz));
+    R ux = 0, uy = 0, uz = 0;
+    if (mg > 0) { ux = (qx - cx) / mg; uy = (qy - cy) / mg; uz = (qz - cz) / mg; }
+    const R th = (R)(((double)ang * 3.141592653589793238) / 180.0);
+    const R cs = std::cos(th), sn = std::sin(th);
+    r.a = cx; r.b = cy; r.c = cz; r.u = ux; r.v = uy; r.w = uz;
+    r.cs = cs; r.sn = sn; r.omc = 1 - cs;
+    r.active = 1;
+    return r;
+}
+
+// PND and lambda of the ideal cubic lattice at radius re (in l0), exact square roots.
+static void lattice_n0_lambda(double re, double* n0, double* lam) {
+    const int m = (int)std::ceil(re) + 1;
+    double sw = 0.0, swr2 = 0.0;
+    for (int i = -m; i <= m; ++i)
+        for (int j = -m; j <= m; ++j)
+            for (int k = -m; k <= m; ++k) {
+                if (!i && !j && !k) continue;
+                const double r = std::sqrt((double)(i * i + j * j + k * k));
+                if (r < re) {
+                    const double w = re / r - 1.0;
+                    sw += w;
+                    swr2 += w * r * r;
+                }
+            }
+    *n0 = sw;
+    *lam = swr2 / sw;
+}
+
+// PND of the ideal cubic lattice seen from a site, with the build's distance and weight at re:
+// the reference density of the PPE source term.
+template <typename R> static double lattice_pnd(R l0, R re) {
+    const int m = (int)std::ceil((double)re / (double)l0) + 1;
+    const R mind = (R)((double)re * 1e-3);
+    double sum = 0.0;
+    for (int i = -m; i <= m; ++i)
+        for (int j = -m; j <= m; ++j)
+            for (int k = -m; k <= m; ++k) {
+                if (!i && !j && !k) continue;
+                const R x = (R)i * l0, y = (R)j * l0, z = (R)k * l0;
+                R d = build_sqrt((R(0) - x) * (R(0) - x) + (R(0) - y) * (R(0) - y) + (R(0) - z) * (R(0) - z));
+                if (d < re) {
+                    if (d < mind) d = mind;
+                    sum += (double)weight(re, d);
+                }
+            }
+    return sum;
+}
+
+template <typename T> static hipError_t dalloc(T** p, size_t n) {
+    return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+template <typename R> struct Engine : EngineBase {
+    Dev<R> d;
+    mpsr_params P{};
+    hipStream_t st = nullptr;
+    std::vector<void*> allocs;
+    RotCoef<R>* rot_table = nullptr;   // device [rot_steps * nmodel]
+    int rot_steps = 0;
+    std::vector<mpsr_tri_params> models_h;
+    std::vector<RotCoef<R>> rot_host;
+    int pressure_mode = 0;
+    mpsr_ppe_options ppe_opt{};
+    PpeSolver<R>* ppe = nullptr;
+    double n0_lap = 0, lambda_lap = 0, n0_src = 0;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    std::vector<hipEvent_t> pev;      // profiling events, two per phase launch
+    std::vector<int> pev_phase;
+    size_t pev_used = 0;
+
+    ~Engine() override {
+        if (ppe) ppe_destroy(ppe);
+        for (void* p : allocs) (void)hipFree(p);
+        for (hipEvent_t e : pev) (void)hipEventDestroy(e);
+        if (ev_a) (void)hipEventDestroy(ev_a);
+        if (ev_b) (void)hipEventDestroy(ev_b);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    template <typename T> mpsr_status alloc(T** p, size_t n) {
+        HIPC(dalloc(p, n));
+        allocs.push_back((void*)*p);
+        return MPSR_NOERR;
+    }
+#define ALLOC(p, n)                                  \
+    do {                                             \
+        mpsr_status s_ = alloc(&(p), (size_t)(n));   \
+        if (s_ != MPSR_NOERR) return s_;             \
+    } while (0)
+
+    mpsr_status alloc_state(PState<R>& s, int cap) {
+        ALLOC(s.id, cap); ALLOC(s.type, cap);
+        ALLOC(s.px, cap); ALLOC(s.py, cap); ALLOC(s.pz, cap);
+        ALLOC(s.vx, cap); ALLOC(s.vy, cap); ALLOC(s.vz, cap);
+        ALLOC(s.p, cap); ALLOC(s.n, cap); ALLOC(s.temp, cap); ALLOC(s.nu, cap); ALLOC(s.solid, cap);
+        return MPSR_NOERR;
+    }
+    mpsr_status alloc_sort(int cap) {
+        ALLOC(d.sort.k1, cap);
+        ALLOC(d.sort.v1, cap);
+        d.sort.hist_cap = (int)radix_hist_ints(cap);
+        ALLOC(d.sort.hist, d.sort.hist_cap);
+        d.sort.scan_tmp_cap = (int)scan_tmp_ints(cap);
+        ALLOC(d.sort.scan_tmp, d.sort.scan_tmp_cap);
+        return MPSR_NOERR;
+    }
+
+    mpsr_status upload_state(const mpsr_particles* pp) {
+        const int n = pp->n;
+        std::vector<int> ids(n), type(n);
+        std::vector<R> f[11];
+        for (auto& v : f) v.assign(n, R(0));
+        for (int s = 0; s < n; ++s) {
+            ids[s] = pp->id ? pp->id[s] : s;
+            type[s] = pp->type ? pp->type[s] : 0;
+            for (int a = 0; a < 3; ++a) {
+                f[a][s] = (R)pp->position[3 * s + a];
+                f[3 + a][s] = (R)pp->velocity[3 * s + a];
+            }
+            f[6][s] = pp->pressure ? (R)pp->pressure[s] : R(0);
+            f[7][s] = pp->density ? (R)pp->density[s] : R(0);
+            f[8][s] = pp->temperature ? (R)pp->temperature[s] : R(0);
+            f[9][s] = pp->kinetic_viscosity ? (R)pp->kinetic_viscosity[s] : R(0);
+            f[10][s] = pp->solid_phase_rate ? (R)pp->solid_phase_rate[s] : R(0);
+        }
+        for (PState<R>* S : {&d.in, &d.out}) {
+            HIPC(hipMemcpy(S->id, ids.data(), n * sizeof(int), hipMemcpyHostToDevice));
+            HIPC(hipMemcpy(S->type, type.data(), n * sizeof(int), hipMemcpyHostToDevice));
+            R* dst[11] = {S->px, S->py, S->pz, S->vx, S->vy, S->vz, S->p, S->n, S->temp, S->nu, S->solid};
+            for (int a = 0; a < 11; ++a) HIPC(hipMemcpy(dst[a], f[a].data(), n * sizeof(R), hipMemcpyHostToDevice));
+        }
+        d.N = n;
+        return MPSR_NOERR;
+    }
+
+    mpsr_status init(const mpsr_create_info* ci) {
+        P = ci->params;
+        if (P.Dimension != 3) return fail(MPSR_PARAMERR, "only the 3D operators are built (Dimension must be 3)");
+        if (P.bUpdateTemperature) return fail(MPSR_PARAMERR, "the thermal model (CalcTemperatureFactor) is not built");
+        if (!ci->particles || ci->particles->n < 0) return fail(MPSR_PARAMERR, "particles missing");
+        if (ci->num_triangles > 0 && (!ci->triangles || ci->num_models <= 0 || !ci->models))
+            return fail(MPSR_PARAMERR, "triangles need their model parameters");
+        for (int t = 0; t < ci->num_triangles; ++t) {
+            const int m = ci->triangles[t].ModelIDNumber;
+            if (m < 0 || m >= ci->num_models) return fail(MPSR_PARAMERR, "triangle model id out of range");
+            const mpsr_tri_params& mp = ci->models[m];
+            if (mp.isDrag && (mp.DragTriangleID < 0 || mp.DragTriangleID >= ci->num_drag))
+                return fail(MPSR_PARAMERR, "drag model without drag parameters");
+        }
+        mpsr_status s = mpsr_make_gridbox(&P, &ci->compute_zone, &gb);
+        if (s != MPSR_NOERR) return fail(s, "invalid compute zone");
+        device = ci->device;
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPC(hipEventCreate(&ev_a));
+        HIPC(hipEventCreate(&ev_b));
+        const int N = ci->particles->n;
+        const int cap = std::max(ci->max_particles, N);
+        d.cap = cap;
+        d.ntri = ci->num_triangles;
+        d.nmodel = ci->num_models;
+        d.ndrag = ci->num_drag;
+        d.c = make_consts<R>(P, gb, d.ntri, d.ndrag);
+        pressure_mode = ci->pressure_mode;
+        ppe_opt = ci->ppe;
+        int bits = 1;
+        while ((1ll << bits) <= (long long)d.c.ncell) ++bits;  // keys 0..ncell (sentinel = ncell)
+        d.key_bits = bits;
+        if (alloc_state(d.in, cap) != MPSR_NOERR || alloc_state(d.out, cap) != MPSR_NOERR) return MPSR_GPUERR;
+        ALLOC(d.keys, cap); ALLOC(d.vals, cap);
+        ALLOC(d.cell_start, d.c.ncell); ALLOC(d.cell_end, d.c.ncell);
+        ALLOC(d.spos, cap); ALLOC(d.svel, cap); ALLOC(d.spres, cap);
+        ALLOC(d.stl_mag, cap); ALLOC(d.stl_dir, cap); ALLOC(d.stl_id, cap);
+        ALLOC(d.drag_acc, cap); ALLOC(d.drag_temp, cap);
+        HIPC(hipMemset(d.drag_acc, 0, sizeof(R4<R>) * cap));
+        HIPC(hipMemset(d.drag_temp, 0, sizeof(R) * cap));
+        s = upload_state(ci->particles);
+        if (s != MPSR_NOERR) return s;
+        // walls
+        models_h.assign(ci->models, ci->models + ci->num_models);
+        ALLOC(d.models, std::max(1, d.nmodel));
+        if (d.nmodel) HIPC(hipMemcpy(d.models, ci->models, sizeof(mpsr_tri_params) * d.nmodel, hipMemcpyHostToDevice));
+        ALLOC(d.drag, std::max(1, d.ndrag));
+        if (d.ndrag) HIPC(hipMemcpy(d.drag, ci->drag, sizeof(mpsr_drag_params) * d.ndrag, hipMemcpyHostToDevice));
+        std::vector<TriD<R>> th(std::max(1, d.ntri));
+        for (int t = 0; t < d.ntri; ++t) {
+            const mpsr_triangle& T = ci->triangles[t];
+            TriD<R> q{};
+            for (int k = 0; k < 3; ++k) {
+                q.v[3 * k] = (R)T.Vertex[k].x;
+                q.v[3 * k + 1] = (R)T.Vertex[k].y;
+                q.v[3 * k + 2] = (R)T.Vertex[k].z;
+            }
+            q.nrm[0] = (R)T.Normal.x; q.nrm[1] = (R)T.Normal.y; q.nrm[2] = (R)T.Normal.z;
+            q.model = T.ModelIDNumber;
+            q.is_drag = ci->models[T.ModelIDNumber].isDrag ? 1 : 0;
+            th[t] = q;
+        }
+        ALLOC(d.tris, std::max(1, d.ntri));
+        HIPC(hipMemcpy(d.tris, th.data(), sizeof(TriD<R>) * std::max(1, d.ntri), hipMemcpyHostToDevice));
+        ALLOC(d.tri_emit, d.ntri + 1); ALLOC(d.tri_off, d.ntri + 1);
+        ALLOC(d.pair_n, 1); ALLOC(d.uniq_n, 1); ALLOC(d.overflow, 1);
+        HIPC(hipMemset(d.overflow, 0, sizeof(int)));
+        HIPC(hipMemset(d.uniq_n, 0, sizeof(int)));
+        ALLOC(d.cell_tri_start, d.c.ncell); ALLOC(d.cell_tri_count, d.c.ncell);
+        HIPC(hipMemset(d.cell_tri_count, 0, sizeof(int) * d.c.ncell));
+        // capacity of the (cell, triangle) pair buffer from the walls as given
+        int pair_total = 0;
+        if (d.ntri > 0) {
+            // count pass alone, with a scan scratch sized for the triangles
+            if (alloc_sort(std::max(N, d.ntri) + 1) != MPSR_NOERR) return MPSR_GPUERR;
+            launch_tri_count_total(d, st);
+            HIPC(hipStreamSynchronize(st));
+            HIPC(hipMemcpy(&pair_total, d.pair_n, sizeof(int), hipMemcpyDeviceToHost));
+        }
+        d.pair_cap = std::max(64, pair_total * 2 + 65536);
+        ALLOC(d.pair_cell, d.pair_cap); ALLOC(d.pair_tri, d.pair_cap);
+        ALLOC(d.pair_flag, d.pair_cap); ALLOC(d.pair_pos, d.pair_cap);
+        ALLOC(d.uniq_cell, d.pair_cap); ALLOC(d.uniq_tri, d.pair_cap);
+        if (alloc_sort(std::max(cap, d.pair_cap)) != MPSR_NOERR) return MPSR_GPUERR;
+        // rotation coefficients table
+        rot_steps = 64;
+        ALLOC(rot_table, (size_t)rot_steps * std::max(1, d.nmodel));
+        // Laplacian-radius lattice constants for the PPE
+        double n0l, laml;
+        lattice_n0_lambda((double)(R)P.LaplacianInfluenceRadiusCoefficient, &n0l, &laml);
+        const double l0 = (double)(R)P.InitialParticleDistance;
+        n0_lap = ppe_opt.n0_lap > 0 ? ppe_opt.n0_lap : n0l;
+        lambda_lap = ppe_opt.lambda_lap > 0 ? ppe_opt.lambda_lap : laml * l0 * l0;
+        n0_src = lattice_pnd<R>(d.c.l0, d.c.re_grad);
+        if (pressure_mode == MPSR_PRESSURE_PPE) {
+            ppe = ppe_create<R>(d, P, ppe_opt, n0_lap, lambda_lap, n0_src);
+            if (!ppe) return fail(MPSR_MEMERR, "PPE solver allocation failed");
+        }
+        HIPC(hipDeviceSynchronize());
+        return MPSR_NOERR;
+    }
+
+    // ---- timing -----------------------------------------------------------------------------
+    void mark_begin(int ph) {
+        if (!profiling) return;
+        if (pev_used + 2 > pev.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return;
+                pev.push_back(e);
+                pev_phase.push_back(0);
+            }
+        }
+        pev_phase[pev_used] = ph;
+        (void)hipEventRecord(pev[pev_used], st);
+    }
+    void mark_end() {
+        if (!profiling) return;
+        (void)hipEventRecord(pev[pev_used + 1], st);
+        pev_used += 2;
+    }
+
+    // ---- phases -------------------------------------------------------------------------------
+    void do_bin() {
+        mark_begin(PH_BIN);
+        launch_bin_hash(d, st);
+        launch_bin_ranges(d, st);
+        mark_end();
+    }
+    void do_tri() {
+        mark_begin(PH_TRI);
+        launch_tri_bin(d, st);
+        mark_end();
+    }
+    void do_stl() {
+        mark_begin(PH_STL);
+        launch_stl_distance(d, st);
+        mark_end();
+    }
+    void do_drag() {
+        mark_begin(PH_DRAG);
+        launch_drag(d, st);
+        mark_end();
+    }
+    void do_explicit() {
+        mark_begin(PH_EXPL);
+        launch_explicit(d, st);
+        mark_end();
+    }
+    void do_relocate() {
+        mark_begin(PH_RELOC);
+        launch_relocate(d, st);
+        mark_end();
+    }
+    mpsr_status do_pressure() {
+        if (pressure_mode == MPSR_PRESSURE_PPE) {
+            mark_begin(PH_PND);
+            launch_pnd(d, 0, st);
+            mark_end();
+            mark_begin(PH_PPE);
+            mpsr_status s = ppe_solve(ppe, d, ppe_opt, st);
+            mark_end();
+            return s;
+        }
+        mark_begin(PH_PND);
+        launch_pnd(d, 1, st);
+        mark_end();
+        return MPSR_NOERR;
+    }
+    void do_gradient() {
+        mark_begin(PH_GRAD);
+        launch_gradient(d, st);
+        mark_end();
+    }
+    mpsr_status upload_rot(int first, int count) {
+        if (count > rot_steps) {
+            rot_steps = count;
+            RotCoef<R>* p = nullptr;
+            ALLOC(p, (size_t)rot_steps * std::max(1, d.nmodel));
+            rot_table = p;
+        }
+        rot_host.assign((size_t)count * std::max(1, d.nmodel), RotCoef<R>{});
+        for (int k = 0; k < count; ++k)
+            for (int m = 0; m < d.nmodel; ++m) rot_host[(size_t)k * d.nmodel + m] = rot_coef<R>(models_h[m], P, first + k);
+        HIPC(hipMemcpyAsync(rot_table, rot_host.data(), sizeof(RotCoef<R>) * rot_host.size(), hipMemcpyHostToDevice, st));
+        return MPSR_NOERR;
+    }
+    void do_walls(int k) {
+        mark_begin(PH_WALL);
+        launch_translate(d, st);
+        d.rot = rot_table + (size_t)k * std::max(1, d.nmodel);
+        launch_rotate(d, st);
+        mark_end();
+    }
+
+    mpsr_status one_step(int k) {
+        do_bin();
+        do_tri();
+        do_stl();
+        if (d.ndrag > 0) do_drag();
+        do_explicit();
+        do_relocate();
+        do_bin();
+        do_stl();
+        mpsr_status s = do_pressure();
+        if (s != MPSR_NOERR) return s;
+        do_gradient();
+        do_relocate();
+        do_walls(k);
+        return MPSR_NOERR;
+    }
+
+    mpsr_status step_many(int first, int count) override {
+        HIPC(hipSetDevice(device));
+        if (count <= 0) return MPSR_NOERR;
+        pev_used = 0;
+        mpsr_status s = upload_rot(first, count);
+        if (s != MPSR_NOERR) return s;
+        // the host copy above must finish before rot_host is refilled by a later call
+        HIPC(hipEventRecord(ev_a, st));
+        for (int k = 0; k < count; ++k) {
+            s = one_step(k);
+            if (s != MPSR_NOERR) return s;
+            HIPC(hipGetLastError());
+        }
+        HIPC(hipEventRecord(ev_b, st));
+        HIPC(hipEventSynchronize(ev_b));
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, ev_a, ev_b));
+        collect_timings(ms, count);
+        int ovf = 0;
+        HIPC(hipMemcpy(&ovf, d.overflow, sizeof(int), hipMemcpyDeviceToHost));
+        if (ovf) return fail(MPSR_MEMERR, "triangle/cell pair buffer overflow: walls grew beyond the set-up capacity");
+        if (ppe) {
+            mpsr_status ps = ppe_check(ppe, &tm);
+            if (ps != MPSR_NOERR) return fail(ps, "PPE solve did not converge");
+        }
+        return MPSR_NOERR;
+    }
+
+    void collect_timings(float total_ms, int count) {
+        mpsr_timings t{};
+        t.step_ms = total_ms / count;
+        if (profiling) {
+            double acc[PH_COUNT] = {0};
+            for (size_t e = 0; e + 1 < pev_used; e += 2) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, pev[e], pev[e + 1]) == hipSuccess) acc[pev_phase[e]] += ms;
+            }
+            t.bin_ms = acc[PH_BIN] / count;
+            t.tri_ms = acc[PH_TRI] / count;
+            t.stl_ms = acc[PH_STL] / count;
+            t.drag_ms = acc[PH_DRAG] / count;
+            t.explicit_ms = acc[PH_EXPL] / count;
+            t.relocate_ms = acc[PH_RELOC] / count;
+            t.pnd_ms = acc[PH_PND] / count;
+            t.ppe_ms = acc[PH_PPE] / count;
+            t.gradient_ms = acc[PH_GRAD] / count;
+            t.wall_ms = acc[PH_WALL] / count;
+        }
+        t.ppe_iterations = tm.ppe_iterations;
+        t.ppe_refinements = tm.ppe_refinements;
+        t.ppe_rel_residual = tm.ppe_rel_residual;
+        t.ppe_initial_rel_residual = tm.ppe_initial_rel_residual;
+        tm = t;
+    }
+
+    mpsr_status phase(int which, int arg) override {
+        HIPC(hipSetDevice(device));
+        pev_used = 0;
+        const int saved = profiling;
+        profiling = 0;
+        mpsr_status s = MPSR_NOERR;
+        switch (which) {
+            case 0: do_bin(); break;
+            case 1: do_tri(); break;
+            case 2: do_stl(); break;
+            case 3: if (d.ndrag > 0) do_drag(); break;
+            case 4: do_explicit(); break;
+            case 5: do_relocate(); break;
+            case 6: launch_pnd(d, 1, st); break;
+            case 7:
+                if (!ppe) {
+                    ppe = ppe_create<R>(d, P, ppe_opt, n0_lap, lambda_lap, n0_src);
+                    if (!ppe) { s = fail(MPSR_MEMERR, "PPE solver allocation failed"); break; }
+                }
+                launch_pnd(d, 0, st);
+                s = ppe_solve(ppe, d, ppe_opt, st);
+                break;
+            case 8: do_gradient(); break;
+            case 9: launch_translate(d, st); break;
+            case 10:
+                s = upload_rot(arg, 1);
+                if (s == MPSR_NOERR) { d.rot = rot_table; launch_rotate(d, st); }
+                break;
+            default: s = fail(MPSR_PARAMERR, "unknown phase");
+        }
+        profiling = saved;
+        if (s != MPSR_NOERR) return s;
+        HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(st));
+        if (which == 7 && ppe) {
+            mpsr_status ps = ppe_check(ppe, &tm);
+            if (ps != MPSR_NOERR) return fail(ps, "PPE solve did not converge");
+        }
+        return MPSR_NOERR;
+    }
+
+    template <typename T> mpsr_status d2h(std::vector<T>& v, const T* p, size_t n) {
+        v.resize(n);
+        if (n) HIPC(hipMemcpy(v.data(), p, n * sizeof(T), hipMemcpyDeviceToHost));
+        return MPSR_NOERR;
+    }
+#define D2H(v, p, n)                                   \
+    do {                                               \
+        mpsr_status s_ = d2h(v, p, (size_t)(n));       \
+        if (s_ != MPSR_NOERR) return s_;               \
+    } while (0)
+
+    mpsr_status sync() override {
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamSynchronize(st));
+        return MPSR_NOERR;
+    }
+
+    mpsr_status get_particles(int which, mpsr_particles* o) override {
+        HIPC(hipStreamSynchronize(st));
+        const PState<R>& S = which ? d.out : d.in;
+        const int n = std::min(o->n, d.N);
+        std::vector<int> ids, type;
+        std::vector<R> f[11];
+        D2H(ids, S.id, n); D2H(type, S.type, n);
+        const R* src[11] = {S.px, S.py, S.pz, S.vx, S.vy, S.vz, S.p, S.n, S.temp, S.nu, S.solid};
+        for (int a = 0; a < 11; ++a) D2H(f[a], src[a], n);
+        for (int s = 0; s < n; ++s) {
+            if (o->id) o->id[s] = ids[s];
+            if (o->type) o->type[s] = type[s];
+            for (int a = 0; a < 3; ++a) {
+                if (o->position) o->position[3 * s + a] = (double)f[a][s];
+                if (o->velocity) o->velocity[3 * s + a] = (double)f[3 + a][s];
+            }
+            if (o->pressure) o->pressure[s] = f[6][s];
+            if (o->density) o->density[s] = f[7][s];
+            if (o->temperature) o->temperature[s] = f[8][s];
+            if (o->kinetic_viscosity) o->kinetic_viscosity[s] = f[9][s];
+            if (o->solid_phase_rate) o->solid_phase_rate[s] = f[10][s];
+        }
+        return MPSR_NOERR;
+    }
+    mpsr_status set_particles(const mpsr_particles* p) override {
+        if (!p || p->n < 0 || p->n > d.cap) return fail(MPSR_PARAMERR, "particle count exceeds the allocated slots");
+        HIPC(hipStreamSynchronize(st));
+        return upload_state(p);
+    }
+    mpsr_status get_triangles(mpsr_triangle* o, int n) override {
+        HIPC(hipStreamSynchronize(st));
+        std::vector<TriD<R>> th;
+        D2H(th, d.tris, std::min(n, d.ntri));
+        for (size_t t = 0; t < th.size(); ++t) {
+            o[t].ModelIDNumber = th[t].model;
+            o[t].Normal = {(float)th[t].nrm[0], (float)th[t].nrm[1], (float)th[t].nrm[2]};
+            for (int k = 0; k < 3; ++k)
+                o[t].Vertex[k] = {(float)th[t].v[3 * k], (float)th[t].v[3 * k + 1], (float)th[t].v[3 * k + 2]};
+        }
+        return MPSR_NOERR;
+    }
+    mpsr_status get_binning(int* k, int* p, int* cs, int* ce) override {
+        HIPC(hipStreamSynchronize(st));
+        if (!d.perm) return fail(MPSR_CALCERR, "no binning yet");
+        std::vector<uint32_t> kk;
+        std::vector<int> pp, a, b;
+        D2H(kk, d.skeys, d.N); D2H(pp, d.perm, d.N);
+        D2H(a, d.cell_start, d.c.ncell); D2H(b, d.cell_end, d.c.ncell);
+        for (int t = 0; t < d.N; ++t) {
+            if (k) k[t] = (int)kk[t];
+            if (p) p[t] = pp[t];
+        }
+        if (cs) std::memcpy(cs, a.data(), sizeof(int) * a.size());
+        if (ce) std::memcpy(ce, b.data(), sizeof(int) * b.size());
+        return MPSR_NOERR;
+    }
+    mpsr_status get_triangle_cells(int* cs, int* cc, int* ids, int cap, int* total) override {
+        HIPC(hipStreamSynchronize(st));
+        std::vector<int> a, b, u;
+        D2H(a, d.cell_tri_start, d.c.ncell); D2H(b, d.cell_tri_count, d.c.ncell);
+        int un = 0;
+        HIPC(hipMemcpy(&un, d.uniq_n, sizeof(int), hipMemcpyDeviceToHost));
+        if (d.ntri > 0 && un > 0) D2H(u, d.uniq_tri, std::min(un, d.pair_cap));
+        // compact to the capped per-cell lists in cell order
+        int e = 0;
+        for (int c = 0; c < d.c.ncell; ++c) {
+            if (cs) cs[c] = e;
+            const int n = b[c];
+            if (cc) cc[c] = n;
+            for (int q = 0; q < n; ++q, ++e)
+                if (ids && e < cap) ids[e] = u[a[c] + q];
+        }
+        if (total) *total = e;
+        return MPSR_NOERR;
+    }
+    mpsr_status get_stl(double* m, double* dir, int* id) override {
+        HIPC(hipStreamSynchronize(st));
+        std::vector<R> mm;
+        std::vector<R4<R>> dd;
+        std::vector<int> ii, pp;
+        D2H(mm, d.stl_mag, d.N); D2H(dd, d.stl_dir, d.N); D2H(ii, d.stl_id, d.N); D2H(pp, d.perm, d.N);
+        for (int t = 0; t < d.N; ++t) {
+            const int s = pp[t];
+            if (m) m[s] = mm[t];
+            if (dir) { dir[3 * s] = dd[t].x; dir[3 * s + 1] = dd[t].y; dir[3 * s + 2] = dd[t].z; }
+            if (id) id[s] = ii[t];
+        }
+        return MPSR_NOERR;
+    }
+    mpsr_status get_neighbors(double r, int* off, int* nbr, int cap, int* total) override {
+        HIPC(hipStreamSynchronize(st));
+        if (!d.perm) return fail(MPSR_CALCERR, "no binning yet");
+        const R re = (R)r * d.c.l0;
+        int *cnt = nullptr, *doff = nullptr, *dn = nullptr;
+        HIPC(hipMalloc(&cnt, sizeof(int) * (d.N + 1)));
+        HIPC(hipMalloc(&doff, sizeof(int) * (d.N + 1)));
+        launch_neighbor_count(d, re, cnt, st);
+        std::vector<int> c(d.N);
+        HIPC(hipStreamSynchronize(st));
+        HIPC(hipMemcpy(c.data(), cnt, sizeof(int) * d.N, hipMemcpyDeviceToHost));
+        std::vector<int> o(d.N + 1, 0);
+        for (int t = 0; t < d.N; ++t) o[t + 1] = o[t] + c[t];
+        const int tot = o[d.N];
+        HIPC(hipMemcpy(doff, o.data(), sizeof(int) * (d.N + 1), hipMemcpyHostToDevice));
+        HIPC(hipMalloc(&dn, sizeof(int) * std::max(1, tot)));
+        launch_neighbor_fill(d, re, doff, dn, st);
+        std::vector<int> nb(tot), pp;
+        HIPC(hipStreamSynchronize(st));
+        if (tot) HIPC(hipMemcpy(nb.data(), dn, sizeof(int) * tot, hipMemcpyDeviceToHost));
+        D2H(pp, d.perm, d.N);
+        (void)hipFree(cnt); (void)hipFree(doff); (void)hipFree(dn);
+        // rows in slot order
+        std::vector<int> row_of_slot(d.N);
+        for (int t = 0; t < d.N; ++t) row_of_slot[pp[t]] = t;
+        int e = 0;
+        off[0] = 0;
+        for (int s = 0; s < d.N; ++s) {
+            const int t = row_of_slot[s];
+            for (int q = o[t]; q < o[t + 1]; ++q, ++e)
+                if (e < cap) nbr[e] = nb[q];
+            off[s + 1] = e;
+        }
+        if (total) *total = tot;
+        return MPSR_NOERR;
+    }
+    mpsr_status get_ppe(double* rhs, double* x, int* dir) override {
+        HIPC(hipStreamSynchronize(st));
+        if (!ppe) return fail(MPSR_CALCERR, "no PPE solve yet");
+        return ppe_get(ppe, d, rhs, x, dir);
+    }
+    mpsr_status get_ppe_matrix(int* ptr, int* col, double* val, double* dg, int cap, int* nnz) override {
+        HIPC(hipStreamSynchronize(st));
+        if (!ppe) return fail(MPSR_CALCERR, "no PPE solve yet");
+        return ppe_get_matrix(ppe, d, ptr, col, val, dg, cap, nnz);
+    }
+    mpsr_status set_dt(float dt, float courant) override {
+        HIPC(hipStreamSynchronize(st));
+        P.Dt = dt;
+        P.CourantNumber = courant;
+        d.c = make_consts<R>(P, gb, d.ntri, d.ndrag);
+        if (ppe) ppe_update_params(ppe, P);
+        return MPSR_NOERR;
+    }
+    mpsr_status set_ppe(const mpsr_ppe_options* o) override {
+        ppe_opt = *o;
+        return MPSR_NOERR;
+    }
+};
+
+}  // namespace mpsr
+
+using namespace mpsr;
+
+struct mpsr_handle {
+    EngineBase* e;
+};
+
+#define GUARD(h)                                                       \
+    do {                                                               \
+        if (!(h) || !(h)->e) return fail(MPSR_PARAMERR, "null handle"); \
+    } while (0)
+
+extern "C" {
+
+const char* mpsr_last_error(void) { return g_err.c_str(); }
+
+mpsr_status mpsr_create(const mpsr_create_info* info, mpsr_handle** out) {
+    if (!info || !out) return fail(MPSR_PARAMERR, "null argument");
+    *out = nullptr;
+    EngineBase* e = nullptr;
+    mpsr_status s;
+    if (info->precision == MPSR_FP64) {
+        auto* en = new Engine<double>();
+        s = en->init(info);
+        e = en;
+    } else if (info->precision == MPSR_FP32) {
+        auto* en = new Engine<float>();
+        s = en->init(info);
+        e = en;
+    } else {
+        return fail(MPSR_PARAMERR, "unknown precision");
+    }
+    if (s != MPSR_NOERR) {
+        delete e;
+        return s;
+    }
+    *out = new mpsr_handle{e};
+    return MPSR_NOERR;
+}
+void mpsr_destroy(mpsr_handle* h) {
+    if (!h) return;
+    delete h->e;
+    delete h;
+}
+mpsr_status mpsr_step(mpsr_handle* h, int i) { GUARD(h); return h->e->step_many(i, 1); }
+mpsr_status mpsr_step_many(mpsr_handle* h, int first, int count) { GUARD(h); return h->e->step_many(first, count); }
+mpsr_status mpsr_synchronize(mpsr_handle* h) { GUARD(h); return h->e->sync(); }
+mpsr_status mpsr_register_particle_topology(mpsr_handle* h) { GUARD(h); return h->e->phase(0, 0); }
+mpsr_status mpsr_register_triangle_topology(mpsr_handle* h) { GUARD(h); return h->e->phase(1, 0); }
+mpsr_status mpsr_calc_stl_distance(mpsr_handle* h) { GUARD(h); return h->e->phase(2, 0); }
+mpsr_status mpsr_calc_drag_effect(mpsr_handle* h) { GUARD(h); return h->e->phase(3, 0); }
+mpsr_status mpsr_calc_explicitly(mpsr_handle* h) { GUARD(h); return h->e->phase(4, 0); }
+mpsr_status mpsr_relocate_particle_data(mpsr_handle* h) { GUARD(h); return h->e->phase(5, 0); }
+mpsr_status mpsr_calc_pressure_explicit(mpsr_handle* h) { GUARD(h); return h->e->phase(6, 0); }
+mpsr_status mpsr_calc_pressure_poisson(mpsr_handle* h) { GUARD(h); return h->e->phase(7, 0); }
+mpsr_status mpsr_calc_implicit_explicitly(mpsr_handle* h) { GUARD(h); return h->e->phase(8, 0); }
+mpsr_status mpsr_update_triangle_position(mpsr_handle* h) { GUARD(h); return h->e->phase(9, 0); }
+mpsr_status mpsr_rotate_triangle_position(mpsr_handle* h, int step) { GUARD(h); return h->e->phase(10, step); }
+mpsr_status mpsr_get_particles(mpsr_handle* h, int which, mpsr_particles* o) {
+    GUARD(h);
+    if (!o) return fail(MPSR_PARAMERR, "null output");
+    return h->e->get_particles(which, o);
+}
+mpsr_status mpsr_set_particles(mpsr_handle* h, const mpsr_particles* p) { GUARD(h); return h->e->set_particles(p); }
+mpsr_status mpsr_get_triangles(mpsr_handle* h, mpsr_triangle* o, int n) { GUARD(h); return h->e->get_triangles(o, n); }
+mpsr_status mpsr_get_gridbox(mpsr_handle* h, mpsr_gridbox* o) {
+    GUARD(h);
+    *o = h->e->gb;
+    return MPSR_NOERR;
+}
+mpsr_status mpsr_set_dt(mpsr_handle* h, float dt, float courant) { GUARD(h); return h->e->set_dt(dt, courant); }
+mpsr_status mpsr_get_timings(mpsr_handle* h, mpsr_timings* o) {
+    GUARD(h);
+    *o = h->e->tm;
+    return MPSR_NOERR;
+}
+mpsr_status mpsr_set_ppe_options(mpsr_handle* h, const mpsr_ppe_options* o) { GUARD(h); return h->e->set_ppe(o); }
+mpsr_status mpsr_set_profiling(mpsr_handle* h, int on) {
+    GUARD(h);
+    h->e->profiling = on;
+    return MPSR_NOERR;
+}
+mpsr_status mpsr_get_binning(mpsr_handle* h, int* k, int* p, int* cs, int* ce) { GUARD(h); return h->e->get_binning(k, p, cs, ce); }
+mpsr_status mpsr_get_triangle_cells(mpsr_handle* h, int* cs, int* cc, int* ids, int cap, int* total) {
+    GUARD(h);
+    return h->e->get_triangle_cells(cs, cc, ids, cap, total);
+}
+mpsr_status mpsr_get_stl_distance(mpsr_handle* h, double* m, double* dir, int* id) { GUARD(h); return h->e->get_stl(m, dir, id); }
+mpsr_status mpsr_get_neighbors(mpsr_handle* h, double r, int* off, int* nbr, int cap, int* total) {
+    GUARD(h);
+    return h->e->get_neighbors(r, off, nbr, cap, total);
+}
+mpsr_status mpsr_get_ppe(mpsr_handle* h, double* rhs, double* x, int* dir) { GUARD(h); return h->e->get_ppe(rhs, x, dir); }
+mpsr_status mpsr_get_ppe_matrix(mpsr_handle* h, int* ptr, int* col, double* val, double* diag, int cap, int* nnz) {
+    GUARD(h);
+    return h->e->get_ppe_matrix(ptr, col, val, diag, cap, nnz);
+}
+
+}  // extern "C"

@@ -1,0 +1,268 @@
+// mpsr_internal.h -- types shared by the HIP translation units of libmpsr.
+//
+// Everything here is MI355X (gfx950, wave64) code.  Kernels are templated on the build
+// precision R (float: the reference's Scalar, with its sqrt14 distances; double: the FP64 build
+// with exact square roots).  Particle state lives in structure-of-arrays form in HBM; the
+// neighbour kernels read 16 B (float4) or 32 B (double4) position/velocity records gathered
+// into cell-sorted order so that one load brings a whole neighbour.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+#include "../../include/mpsr.h"
+
+namespace mpsr {
+
+template <typename R> struct Vec4;
+template <> struct Vec4<float> { using T = float4; };
+template <> struct Vec4<double> { using T = double4; };
+template <typename R> using R4 = typename Vec4<R>::T;
+
+// ---- the reference's scalar helpers, restated for the device --------------------------------
+
+// solver/Common/inc/Utility_Inline.h:13-26 (sqrt14): one Newton step from the 0x5f3759df guess
+__host__ __device__ inline float sqrt14(float x) {
+    const float xhalf = 0.5f * x;
+    int32_t i = __builtin_bit_cast(int32_t, x);
+    i = 0x5f3759df - (i >> 1);
+    const float y = __builtin_bit_cast(float, i);
+    return x * y * (1.5f - xhalf * y * y);
+}
+__host__ __device__ inline float build_sqrt(float x) { return sqrt14(x); }
+__host__ __device__ inline double build_sqrt(double x) { return sqrt(x); }
+
+// Utility_Inline.h:171-183 GetWeight
+template <typename R> __host__ __device__ inline R weight(R re, R r) {
+    if (r > re) return R(0);
+    if (r > 0.0) return re / r - 1;
+    return R(1e6);
+}
+
+// Utility_Inline.h:199-336 (gradient) and :513-686 (Laplacian) wall weight tables.  The
+// reference compares a float ratio against double literals, so the knots are compared in double.
+struct ZTable { int n; double knot[25]; double a[25]; double c[25]; };
+
+template <typename R> __host__ __device__ inline R ztable_eval(const ZTable& t, R r, R l0) {
+    const R x = r / l0;
+    const double xd = (double)x;
+    for (int s = 0; s < t.n; ++s) {
+        if (xd <= t.knot[s]) {
+            const R a = (R)t.a[s], b = (R)t.knot[s], c = (R)t.c[s];
+            return a * (x - b) + c;
+        }
+    }
+    return R(0);
+}
+
+// Per-launch constants, computed on the host from mpsr_params with the reference's own
+// expressions (Kern = Source/DeviceProcess_Kernel.h).
+template <typename R> struct Consts {
+    // grid (CGridBox)
+    R bminx, bminy, bminz;
+    R cminx, cminy, cminz, cmaxx, cmaxy, cmaxz;
+    R cell;
+    int gx, gy, gz, ncell;
+    R tol, l0, dt;
+    // explicit predictor, Kern :283-441
+    R re_lap, mind_lap, lapc, rc, art_p, art_coeff, mu_wall;
+    R gx_acc, gy_acc, gz_acc;
+    int expl, has_drag;
+    // PND / EOS, Kern :702-801
+    R re_grad, mind_grad, eos_coef, n0, beta_n0;
+    // gradient corrector, Kern :804-912
+    R c1, c2, c3, vmax;
+    // drag, Kern :106-281
+    R density, cp, thermal_step;
+    int ntri;
+};
+
+// position -> cell coordinates, Utility_Inline.h:1636-1674 (C truncation toward zero)
+template <typename R>
+__device__ inline bool cell_coords(const Consts<R>& c, R x, R y, R z, int& i, int& j, int& k) {
+    R rx = x - c.bminx, ry = y - c.bminy, rz = z - c.bminz;
+    rx /= c.cell; ry /= c.cell; rz /= c.cell;
+    if (!(fabs((double)rx) < 1.0e9 && fabs((double)ry) < 1.0e9 && fabs((double)rz) < 1.0e9)) return false;
+    i = (int)rx; j = (int)ry; k = (int)rz;
+    return true;
+}
+template <typename R> __device__ inline long long cell_of(const Consts<R>& c, int i, int j, int k) {
+    return (long long)c.gy * c.gz * i + (long long)c.gz * j + k;
+}
+// Utility_Inline.h:1936-1975 IsInclude
+template <typename R> __device__ inline bool include(const Consts<R>& c, R x, R y, R z) {
+    if (!isfinite(x) || !isfinite(y) || !isfinite(z)) return false;
+    if (x < c.cminx - c.tol) return false;
+    if (y < c.cminy - c.tol) return false;
+    if (z < c.cminz - c.tol) return false;
+    if (x >= c.cmaxx + c.tol) return false;
+    if (y >= c.cmaxy + c.tol) return false;
+    if (z >= c.cmaxz + c.tol) return false;
+    return true;
+}
+
+// Slot-order particle state (the reference's c_daParticle* / c_daOutputParticle* arrays).
+template <typename R> struct PState {
+    int* id;
+    int* type;
+    R *px, *py, *pz, *vx, *vy, *vz;
+    R *p, *n, *temp, *nu, *solid;
+};
+
+// Triangle record on the device: three vertices, the stored normal, the model, the model's
+// drag flag.  64 B for float, 112 B for double.
+template <typename R> struct alignas(16) TriD {
+    R v[9];
+    R nrm[3];
+    int model;
+    int is_drag;
+    int pad[2];
+};
+
+// Rotation of one model for one step (Kern :930-998), cos/sin evaluated on the host.
+template <typename R> struct RotCoef {
+    R a, b, c, u, v, w, cs, sn, omc;
+    int active;
+};
+
+// ---- wave64 / block primitives --------------------------------------------------------------
+__device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ inline int wave_inclusive_scan(int v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline int wave_sum(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Exclusive scan over a block of NW waves; `lds` holds NW + 1 ints.  Returns the exclusive
+// prefix of v and stores the block total in *total.
+template <int NW> __device__ inline int block_exclusive_scan(int v, int* lds, int* total) {
+    const int lane = lane_id(), w = (int)(threadIdx.x >> 6);
+    const int inc = wave_inclusive_scan(v);
+    if (lane == 63) lds[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        int s = lane < NW ? lds[lane] : 0;
+        int si = wave_inclusive_scan(s);
+        if (lane < NW) lds[lane] = si - s;
+        if (lane == NW - 1) lds[NW] = si;
+    }
+    __syncthreads();
+    const int r = inc - v + lds[w];
+    *total = lds[NW];
+    __syncthreads();
+    return r;
+}
+
+// ---- launch helpers shared by the translation units ------------------------------------------
+inline unsigned grid_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// sort.hip
+struct SortScratch {
+    uint32_t* k1;
+    int* v1;
+    int* hist;
+    int hist_cap;      // ints
+    int* scan_tmp;     // block sums for the generic scan
+    int scan_tmp_cap;
+};
+size_t radix_hist_ints(int n_cap);
+size_t scan_tmp_ints(int n_cap);
+// Stable LSD radix sort of (key, value) pairs on `bits` low key bits.  n is *d_n when d_n is
+// non-null (device-resident count, n_cap bounds it), else n_host.  The result ends in (k0, v0)
+// or in (s.k1, s.v1); the return value says which (0 or 1).
+int radix_sort_pairs(uint32_t* k0, int* v0, const int* d_n, int n_host, int n_cap, int bits,
+                     const SortScratch& s, hipStream_t st);
+// Exclusive scan of n ints (n = *d_n or n_host); *d_total (if non-null) receives the sum.
+void exclusive_scan(const int* in, int* out, const int* d_n, int n_host, int n_cap, int* d_total,
+                    const SortScratch& s, hipStream_t st);
+void fill_int(int* p, int value, long long n, hipStream_t st);
+
+}  // namespace mpsr
+
+namespace mpsr {
+
+// All device pointers of one handle.  Slot-order state (in/out) is the reference's layout;
+// everything prefixed s* is in cell-sorted order (index t), valid between two binnings.
+template <typename R> struct Dev {
+    int N = 0, cap = 0;
+    int ntri = 0, nmodel = 0, ndrag = 0;
+    Consts<R> c{};
+    PState<R> in{}, out{};
+    // particle binning
+    uint32_t* keys = nullptr;
+    int* vals = nullptr;
+    const uint32_t* skeys = nullptr;  // sorted keys (points into keys or sort.k1)
+    const int* perm = nullptr;        // sorted index -> slot
+    int* cell_start = nullptr;
+    int* cell_end = nullptr;
+    R4<R>* spos = nullptr;
+    R4<R>* svel = nullptr;
+    R* spres = nullptr;
+    int key_bits = 0;
+    // walls
+    TriD<R>* tris = nullptr;
+    mpsr_tri_params* models = nullptr;
+    mpsr_drag_params* drag = nullptr;
+    RotCoef<R>* rot = nullptr;        // [nmodel] for the current step
+    int* tri_emit = nullptr;          // [ntri + 1] emitted cell count per triangle
+    int* tri_off = nullptr;           // [ntri + 1]
+    int* pair_n = nullptr;            // device count of emitted (cell, triangle) pairs
+    int* uniq_n = nullptr;            // device count after de-duplication
+    uint32_t* pair_cell = nullptr;    // [pair_cap]
+    int* pair_tri = nullptr;          // [pair_cap]
+    uint32_t* pair_cell_tmp = nullptr;
+    int* pair_tri_tmp = nullptr;
+    int* pair_flag = nullptr;         // [pair_cap]
+    int* pair_pos = nullptr;          // [pair_cap]
+    uint32_t* uniq_cell = nullptr;    // [pair_cap]
+    int* uniq_tri = nullptr;          // [pair_cap] -> triangle ids listed per cell
+    int pair_cap = 0;
+    int* overflow = nullptr;          // set when a march emits more than pair_cap pairs
+    int* cell_tri_start = nullptr;    // [ncell]
+    int* cell_tri_count = nullptr;    // [ncell], at most 100
+    // nearest wall and drag, sorted order
+    R* stl_mag = nullptr;
+    R4<R>* stl_dir = nullptr;
+    int* stl_id = nullptr;
+    R4<R>* drag_acc = nullptr;
+    R* drag_temp = nullptr;
+    SortScratch sort{};
+};
+
+// particles.hip
+template <typename R> void launch_bin_hash(const Dev<R>& d, hipStream_t st);
+template <typename R> void launch_bin_ranges(Dev<R>& d, hipStream_t st);
+template <typename R> void launch_explicit(const Dev<R>& d, hipStream_t st);
+template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st);
+template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st);
+template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st);
+template <typename R> void launch_neighbor_count(const Dev<R>& d, R re, int* counts, hipStream_t st);
+template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int* offsets, int* nbr, hipStream_t st);
+// walls.hip
+template <typename R> void launch_tri_bin(Dev<R>& d, hipStream_t st);
+template <typename R> void launch_tri_count_total(Dev<R>& d, hipStream_t st);
+template <typename R> void launch_stl_distance(const Dev<R>& d, hipStream_t st);
+template <typename R> void launch_drag(const Dev<R>& d, hipStream_t st);
+template <typename R> void launch_translate(const Dev<R>& d, hipStream_t st);
+template <typename R> void launch_rotate(const Dev<R>& d, hipStream_t st);
+
+}  // namespace mpsr

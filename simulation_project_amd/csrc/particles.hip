@@ -1,0 +1,419 @@
+// particles.hip -- particle binning and the three MPS neighbour operators.
+//
+//   k_bin_hash      RegisterParticleTopology_Kernel (Source/DeviceProcess_Kernel.h:6-58):
+//                   zone filter, in -> out copy, cell key per slot
+//   (radix sort)    thrust::sort_by_key (radix/.../DeviceProcess.cu:207-220)
+//   k_bin_ranges    reorderDataAndFindCellStartD (radix/.../DeviceProcess_Kernel.h:54-124):
+//                   cellStart/cellEnd, gather of position and velocity into sorted order
+//   k_explicit      CalcExplicitly_Kernel (Kern :283-441)
+//   k_relocate      RelocateParticleData_Kernel (Kern :61-92)
+//   k_pnd           CalcPressureExplicit_Kernel (Kern :702-801)
+//   k_gradient      CalcImplicitExplicitly_Kernel (Kern :804-912)
+//
+// The operator kernels run one thread per sorted index t.  Neighbours are visited in the
+// canonical order: stencil z outer, x inner (Kern :317-330), ascending sorted index inside a
+// cell.  Consecutive threads share cells, so a wave walks the same 27 ranges and its neighbour
+// records come from L2 in whole 16 B (float4) or 32 B (double4) loads.  Results go to the
+// slot-order arrays through the sorted->slot permutation.
+#include "mpsr_internal.h"
+#include "neighbors.h"
+
+namespace mpsr {
+
+__constant__ ZTable kZgrad = {
+    21,
+    {0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1, 1.2, 1.3, 1.4, 1.5, 1.6, 1.7, 1.8, 1.9, 2.0, 2.1},
+    {-5.0474610E+03, -1.1272200E+02, -4.3342000E+01, -2.6328000E+01, -1.9648400E+01, -1.5636200E+01,
+     -1.2591500E+01, -1.0988800E+01, -9.0692000E+00, -7.5385000E+00, -6.9148000E+00, -5.9144000E+00,
+     -5.4670000E+00, -5.0626000E+00, -4.6948000E+00, -3.6959000E+00, -2.7022800E+00, -2.4819500E+00,
+     -1.4080400E+00, -5.5263200E-01, -4.9999800E-01},
+    {2.9725900E+01, 1.8453700E+01, 1.4119500E+01, 1.1486700E+01, 9.5218600E+00, 7.9582400E+00,
+     6.6990900E+00, 5.6002100E+00, 4.6932900E+00, 3.9394400E+00, 3.2479600E+00, 2.6565200E+00,
+     2.1098200E+00, 1.6035600E+00, 1.1340800E+00, 7.6449000E-01, 4.9426200E-01, 2.4606700E-01,
+     1.0526300E-01, 4.9999800E-02, 0.0000000E+00}};
+
+__constant__ ZTable kZlap = {
+    25,
+    {0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1, 1.2, 1.3, 1.4, 1.5, 1.6, 1.7, 1.8, 1.9, 2.0,
+     2.4, 2.8, 3.2, 3.6, 4.0},
+    {-9.6379E+03, -2.3979E+02, -1.0777E+02, -7.4088E+01, -6.0259E+01, -5.2803E+01, -4.8681E+01,
+     -4.4630E+01, -4.1171E+01, -3.8583E+01, -3.6846E+01, -3.5476E+01, -3.4183E+01, -3.1589E+01,
+     -2.9258E+01, -2.7047E+01, -2.5690E+01, -2.3305E+01, -2.0982E+01, -1.9268E+01, -1.6887E+01,
+     -1.1091E+01, -6.5068E+00, -3.7565E+00, -1.3253E+00},
+    {1.1497E+02, 9.0990E+01, 8.0213E+01, 7.2804E+01, 6.6778E+01, 6.1498E+01, 5.6630E+01, 5.2167E+01,
+     4.8050E+01, 4.4191E+01, 4.0507E+01, 3.6959E+01, 3.3541E+01, 3.0382E+01, 2.7456E+01, 2.4751E+01,
+     2.2182E+01, 1.9852E+01, 1.7754E+01, 1.5827E+01, 9.0719E+00, 4.6355E+00, 2.0327E+00, 5.3011E-01,
+     0.0000E+00}};
+
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_bin_hash(int N, Consts<R> c, PState<R> in, PState<R> out, uint32_t* keys,
+                                                   int* vals) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    uint32_t key = (uint32_t)c.ncell;
+    if (in.id[s] < 0) {
+        out.id[s] = -1;
+    } else {
+        const R x = in.px[s], y = in.py[s], z = in.pz[s];
+        if (!include(c, x, y, z)) {
+            out.id[s] = -1;
+        } else {
+            out.id[s] = in.id[s];
+            out.px[s] = x; out.py[s] = y; out.pz[s] = z;
+            out.vx[s] = in.vx[s]; out.vy[s] = in.vy[s]; out.vz[s] = in.vz[s];
+            out.p[s] = in.p[s]; out.n[s] = in.n[s]; out.temp[s] = in.temp[s];
+            out.nu[s] = in.nu[s]; out.solid[s] = in.solid[s]; out.type[s] = in.type[s];
+        }
+        int i, j, k;
+        if (cell_coords(c, x, y, z, i, j, k)) {
+            const long long m = cell_of(c, i, j, k);
+            if (m >= 0 && m < c.ncell) key = (uint32_t)m;
+        }
+    }
+    keys[s] = key;
+    vals[s] = s;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_bin_ranges(int N, int ncell, const uint32_t* __restrict__ sk,
+                                                     const int* __restrict__ perm, int* cs, int* ce, PState<R> in,
+                                                     R4<R>* __restrict__ spos, R4<R>* __restrict__ svel) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const uint32_t k = sk[t];
+    const uint32_t kp = t > 0 ? sk[t - 1] : 0xffffffffu;
+    const uint32_t sent = (uint32_t)ncell;
+    if (k != kp) {
+        if (k != sent) cs[k] = t;
+        if (t > 0 && kp != sent) ce[kp] = t;
+    }
+    if (t == N - 1 && k != sent) ce[k] = N;
+    const int s = perm[t];
+    spos[t] = mk4<R>(in.px[s], in.py[s], in.pz[s], R(0));
+    svel[t] = mk4<R>(in.vx[s], in.vy[s], in.vz[s], R(0));
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_explicit(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                                   PState<R> out, const R4<R>* __restrict__ spos,
+                                                   const R4<R>* __restrict__ svel, const int* __restrict__ cs,
+                                                   const int* __restrict__ ce, const R* __restrict__ stl_mag,
+                                                   const R4<R>* __restrict__ stl_dir, const int* __restrict__ stl_id,
+                                                   const R4<R>* __restrict__ drag_acc, const TriD<R>* __restrict__ tris,
+                                                   const mpsr_tri_params* __restrict__ models) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int s = perm[t];
+    if (out.id[s] < 0) return;
+    const R4<R> P = spos[t];
+    const R x = P.x, y = P.y, z = P.z;
+    if (!include(c, x, y, z)) return;
+    const R4<R> V0 = svel[t];
+    R lx = 0, ly = 0, lz = 0, ax = 0, ay = 0, az = 0, qx = 0, qy = 0, qz = 0;
+    const R re = c.re_lap, mind = c.mind_lap;
+    for_candidates(c, cs, ce, x, y, z, t, [&](int q) {
+        const R4<R> Q = spos[q];
+        R d = dist3(x, y, z, Q.x, Q.y, Q.z);
+        if (d < re) {
+            if (d < mind) d = mind;
+            const R w = weight(re, d);
+            const R4<R> V1 = svel[q];
+            lx += (V1.x - V0.x) * w;
+            ly += (V1.y - V0.y) * w;
+            lz += (V1.z - V0.z) * w;
+            if (c.expl) {
+                const R a = c.art_p * w / (d * d);
+                ax = ax + (Q.x - x) * a;
+                ay = ay + (Q.y - y) * a;
+                az = az + (Q.z - z) * a;
+                if (d < c.rc) {
+                    const R cc = (c.rc - d) / d;
+                    qx = qx + (Q.x - x) * cc;
+                    qy = qy + (Q.y - y) * cc;
+                    qz = qz + (Q.z - z) * cc;
+                }
+            }
+        }
+    });
+    const R dw = stl_mag[t];
+    const int tid = stl_id[t];
+    if (tid >= 0 && tid < c.ntri) {
+        if (dw > 0 && dw < re) {
+            const R ww = ztable_eval(kZlap, dw, c.l0);
+            const mpsr_float3 mv = models[tris[tid].model].Velocity;
+            const R v1x = (R)mv.x, v1y = (R)mv.y, v1z = (R)mv.z;
+            lx += c.mu_wall * (v1x - V0.x) * ww;
+            ly += c.mu_wall * (v1y - V0.y) * ww;
+            lz += c.mu_wall * (v1z - V0.z) * ww;
+            if (c.expl) {
+                const R4<R> D = stl_dir[t];
+                const R a = c.art_p * ww / (dw * dw);
+                ax = ax + D.x * a;
+                ay = ay + D.y * a;
+                az = az + D.z * a;
+                if (dw < c.rc) {
+                    const R cc = (c.rc - dw) / dw;
+                    qx = qx + D.x * cc;
+                    qy = qy + D.y * cc;
+                    qz = qz + D.z * cc;
+                }
+            }
+        }
+    }
+    lx *= c.lapc; ly *= c.lapc; lz *= c.lapc;
+    R dvx = 0, dvy = 0, dvz = 0;
+    R accx = R(0) + c.gx_acc, accy = R(0) + c.gy_acc, accz = R(0) + c.gz_acc;
+    if (c.has_drag) {
+        const R4<R> Dg = drag_acc[t];
+        dvx = Dg.x; dvy = Dg.y; dvz = Dg.z;
+    }
+    const R nu = in.nu[s];
+    lx = lx * nu; ly = ly * nu; lz = lz * nu;
+    accx = accx + lx; accy = accy + ly; accz = accz + lz;
+    R vx = (V0.x + accx * c.dt) - dvx;
+    R vy = (V0.y + accy * c.dt) - dvy;
+    R vz = (V0.z + accz * c.dt) - dvz;
+    if (c.expl) {
+        accx = ax * c.art_coeff + qx * (R)0.01;
+        accy = ay * c.art_coeff + qy * (R)0.01;
+        accz = az * c.art_coeff + qz * (R)0.01;
+        vx = vx + accx * c.dt;
+        vy = vy + accy * c.dt;
+        vz = vz + accz * c.dt;
+    }
+    if (isfinite(vx) && isfinite(vy) && isfinite(vz)) {
+        out.px[s] = x + c.dt * vx;
+        out.py[s] = y + c.dt * vy;
+        out.pz[s] = z + c.dt * vz;
+        out.vx[s] = vx; out.vy[s] = vy; out.vz[s] = vz;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_relocate(int N, Consts<R> c, PState<R> in, PState<R> out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    if (out.id[s] < 0) return;
+    const R x = out.px[s], y = out.py[s], z = out.pz[s];
+    if (!include(c, x, y, z)) {
+        in.id[s] = -1;
+        in.px[s] = c.cmaxx + c.tol;
+        in.py[s] = c.cmaxy + c.tol;
+        in.pz[s] = c.cmaxz + c.tol;
+        return;
+    }
+    in.id[s] = out.id[s];
+    in.px[s] = x; in.py[s] = y; in.pz[s] = z;
+    in.vx[s] = out.vx[s]; in.vy[s] = out.vy[s]; in.vz[s] = out.vz[s];
+    in.p[s] = out.p[s]; in.n[s] = out.n[s]; in.temp[s] = out.temp[s];
+    in.nu[s] = out.nu[s]; in.solid[s] = out.solid[s]; in.type[s] = out.type[s];
+}
+
+// ---------------------------------------------------------------------------------------------
+// PND (and the EOS pressure when eos != 0).  spres[t] receives the slot's out-pressure after the
+// kernel, which is what the gradient kernel reads for neighbours (Kern :860).
+template <typename R>
+__global__ __launch_bounds__(256) void k_pnd(int N, Consts<R> c, int eos, const int* __restrict__ perm, PState<R> in,
+                                              PState<R> out, const R4<R>* __restrict__ spos, R* __restrict__ spres,
+                                              const int* __restrict__ cs, const int* __restrict__ ce,
+                                              const R* __restrict__ stl_mag, const int* __restrict__ stl_id) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int s = perm[t];
+    const R4<R> P = spos[t];
+    const R x = P.x, y = P.y, z = P.z;
+    if (!include(c, x, y, z)) {
+        in.p[s] = 0;
+        spres[t] = out.p[s];
+        return;
+    }
+    int ci, cj, ck;
+    long long cid = -1;
+    if (cell_coords(c, x, y, z, ci, cj, ck)) cid = cell_of(c, ci, cj, ck);
+    if (cid < 0 || cid >= c.ncell) {
+        in.p[s] = 0;
+        spres[t] = out.p[s];
+        return;
+    }
+    const R re = c.re_grad, mind = c.mind_grad;
+    R dens = 0;
+    for_candidates(c, cs, ce, x, y, z, t, [&](int q) {
+        const R4<R> Q = spos[q];
+        R d = dist3(x, y, z, Q.x, Q.y, Q.z);
+        if (d < re) {
+            if (d < mind) d = mind;
+            dens = dens + weight(re, d);
+        }
+    });
+    const R riw = stl_mag[t];
+    R ww = 0;
+    const int tid = stl_id[t];
+    if (tid >= 0 && tid < c.ntri)
+        if (riw >= 0 && riw <= re) ww = ztable_eval(kZgrad, riw, c.l0);
+    dens += ww;
+    out.n[s] = dens;
+    R pout = out.p[s];
+    if (eos) {
+        R pr = 0;
+        if (dens > c.beta_n0) pr = c.eos_coef * (dens - c.n0);
+        if (isfinite(pr)) {
+            out.p[s] = pr;
+            in.p[s] = pr;
+            pout = pr;
+        }
+    }
+    spres[t] = pout;
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_gradient(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                                   PState<R> out, const R4<R>* __restrict__ spos,
+                                                   const R4<R>* __restrict__ svel, const R* __restrict__ spres,
+                                                   const int* __restrict__ cs, const int* __restrict__ ce,
+                                                   const R* __restrict__ stl_mag, const R4<R>* __restrict__ stl_dir,
+                                                   const int* __restrict__ stl_id) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int s = perm[t];
+    if (out.id[s] < 0) return;
+    const R4<R> P = spos[t];
+    const R x = P.x, y = P.y, z = P.z;
+    if (!include(c, x, y, z)) return;
+    R gx = 0, gy = 0, gz = 0, wx = 0, wy = 0, wz = 0;
+    const R pi = in.p[s];
+    const R re = c.re_grad, mind = c.mind_grad;
+    for_candidates(c, cs, ce, x, y, z, t, [&](int q) {
+        const R4<R> Q = spos[q];
+        R d = dist3(x, y, z, Q.x, Q.y, Q.z);
+        if (d < re) {
+            if (d < mind) d = mind;
+            const R w = weight(re, d);
+            const R pc = (R)(0.5 * (double)(spres[q] + pi) * (double)w / (double)(d * d));
+            gx = gx + (Q.x - x) * pc;
+            gy = gy + (Q.y - y) * pc;
+            gz = gz + (Q.z - z) * pc;
+        }
+    });
+    gx *= c.c1; gy *= c.c1; gz *= c.c1;
+    const int tid = stl_id[t];
+    if (tid >= 0 && tid < c.ntri) {
+        const R r = stl_mag[t];
+        if (r <= c.l0 && r > 0) {
+            const R dr = c.l0 - r;
+            const R mc = c.c2 * dr / r;
+            const R4<R> D = stl_dir[t];
+            wx = mc * (-D.x);
+            wy = mc * (-D.y);
+            wz = mc * (-D.z);
+        }
+    }
+    const R grx = gx + wx, gry = gy + wy, grz = gz + wz;
+    const R ix = c.c3 * grx, iy = c.c3 * gry, iz = c.c3 * grz;
+    if (isfinite(ix) && isfinite(iy) && isfinite(iz)) {
+        out.px[s] = x + ix * c.dt;
+        out.py[s] = y + iy * c.dt;
+        out.pz[s] = z + iz * c.dt;
+        const R4<R> V = svel[t];
+        R vx = V.x + ix, vy = V.y + iy, vz = V.z + iz;
+        // Utility_Inline.h:2008-2027 CorrectVelocity, per component
+        if (fabs(vx) > c.vmax) vx = c.vmax * vx / fabs(vx);
+        if (fabs(vy) > c.vmax) vy = c.vmax * vy / fabs(vy);
+        if (fabs(vz) > c.vmax) vz = c.vmax * vz / fabs(vz);
+        out.vx[s] = vx; out.vy[s] = vy; out.vz[s] = vz;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// neighbour lists (inspection and PPE structure): count, then fill with slot ids
+template <typename R>
+__global__ __launch_bounds__(256) void k_nbr_count(int N, Consts<R> c, R re, const R4<R>* __restrict__ spos,
+                                                    const int* __restrict__ cs, const int* __restrict__ ce,
+                                                    int* counts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const R4<R> P = spos[t];
+    int n = 0;
+    for_candidates(c, cs, ce, P.x, P.y, P.z, t, [&](int q) {
+        const R4<R> Q = spos[q];
+        if (dist3(P.x, P.y, P.z, Q.x, Q.y, Q.z) < re) ++n;
+    });
+    counts[t] = n;
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_nbr_fill(int N, Consts<R> c, R re, const int* __restrict__ perm,
+                                                   const R4<R>* __restrict__ spos, const int* __restrict__ cs,
+                                                   const int* __restrict__ ce, const int* __restrict__ off, int* nbr) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const R4<R> P = spos[t];
+    int e = off[t];
+    for_candidates(c, cs, ce, P.x, P.y, P.z, t, [&](int q) {
+        const R4<R> Q = spos[q];
+        if (dist3(P.x, P.y, P.z, Q.x, Q.y, Q.z) < re) nbr[e++] = perm[q];
+    });
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename R> void launch_bin_hash(const Dev<R>& d, hipStream_t st) {
+    if (d.N <= 0) return;
+    hipLaunchKernelGGL(k_bin_hash<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.in, d.out, d.keys,
+                       d.vals);
+}
+template <typename R> void launch_bin_ranges(Dev<R>& d, hipStream_t st) {
+    if (d.N <= 0) return;
+    const int which = radix_sort_pairs(d.keys, d.vals, nullptr, d.N, d.N, d.key_bits, d.sort, st);
+    d.skeys = which ? d.sort.k1 : d.keys;
+    d.perm = which ? d.sort.v1 : d.vals;
+    fill_int(d.cell_start, -1, d.c.ncell, st);
+    fill_int(d.cell_end, -1, d.c.ncell, st);
+    hipLaunchKernelGGL(k_bin_ranges<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c.ncell, d.skeys,
+                       d.perm, d.cell_start, d.cell_end, d.in, d.spos, d.svel);
+}
+template <typename R> void launch_explicit(const Dev<R>& d, hipStream_t st) {
+    if (d.N <= 0) return;
+    hipLaunchKernelGGL(k_explicit<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.perm, d.in, d.out,
+                       d.spos, d.svel, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id, d.drag_acc, d.tris,
+                       d.models);
+}
+template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st) {
+    if (d.N <= 0) return;
+    hipLaunchKernelGGL(k_relocate<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.in, d.out);
+}
+template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st) {
+    if (d.N <= 0) return;
+    hipLaunchKernelGGL(k_pnd<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, eos, d.perm, d.in, d.out,
+                       d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id);
+}
+template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st) {
+    if (d.N <= 0) return;
+    hipLaunchKernelGGL(k_gradient<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.perm, d.in, d.out,
+                       d.spos, d.svel, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id);
+}
+template <typename R> void launch_neighbor_count(const Dev<R>& d, R re, int* counts, hipStream_t st) {
+    if (d.N <= 0) return;
+    hipLaunchKernelGGL(k_nbr_count<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, re, d.spos,
+                       d.cell_start, d.cell_end, counts);
+}
+template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int* offsets, int* nbr, hipStream_t st) {
+    if (d.N <= 0) return;
+    hipLaunchKernelGGL(k_nbr_fill<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, re, d.perm, d.spos,
+                       d.cell_start, d.cell_end, offsets, nbr);
+}
+
+#define MPSR_INSTANTIATE(R)                                                                   \
+    template void launch_bin_hash<R>(const Dev<R>&, hipStream_t);                             \
+    template void launch_bin_ranges<R>(Dev<R>&, hipStream_t);                                 \
+    template void launch_explicit<R>(const Dev<R>&, hipStream_t);                             \
+    template void launch_relocate<R>(const Dev<R>&, hipStream_t);                             \
+    template void launch_pnd<R>(const Dev<R>&, int, hipStream_t);                             \
+    template void launch_gradient<R>(const Dev<R>&, hipStream_t);                             \
+    template void launch_neighbor_count<R>(const Dev<R>&, R, int*, hipStream_t);              \
+    template void launch_neighbor_fill<R>(const Dev<R>&, R, const int*, int*, hipStream_t);
+MPSR_INSTANTIATE(float)
+MPSR_INSTANTIATE(double)
+
+}  // namespace mpsr

@@ -1,0 +1,26 @@
+// ppe.h -- pressure Poisson equation: assembly and PCG solve on the device.
+#pragma once
+#include "mpsr_internal.h"
+
+namespace mpsr {
+
+template <typename R> struct PpeSolver;
+
+template <typename R>
+PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_options& opt, double n0_lap,
+                         double lambda_lap, double n0_src);
+template <typename R> void ppe_destroy(PpeSolver<R>* s);
+// Assembles A p = b from the current binning and PND (out.n) and solves it; writes the
+// pressure to in.p / out.p and to the sorted pressure array used by the gradient kernel.
+template <typename R> mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, hipStream_t st);
+// Iteration counts and residuals of the last solve (synchronises); error when it failed.
+template <typename R> mpsr_status ppe_check(PpeSolver<R>* s, mpsr_timings* tm);
+template <typename R> mpsr_status ppe_get(PpeSolver<R>* s, const Dev<R>& d, double* rhs, double* x, int* dir);
+// The assembled matrix in slot order: row pointers [n+1], column slots, off-diagonal values,
+// diagonal [n] (Dirichlet rows: no entries, diagonal 1).
+template <typename R>
+mpsr_status ppe_get_matrix(PpeSolver<R>* s, const Dev<R>& d, int* ptr, int* col, double* val, double* diag, int cap,
+                           int* nnz);
+template <typename R> void ppe_update_params(PpeSolver<R>* s, const mpsr_params& P);
+
+}  // namespace mpsr

@@ -1,0 +1,679 @@
+// ppe.hip -- pressure Poisson equation (PPE) of the semi-implicit MPS step, assembled and
+// solved on the device.
+//
+// The reference declares this stage (CalcPoisson, Source/DeviceProcess.h:50-53;
+// CalculatePressureCG / NJPCG, Source/CGPUCellSet.h:218-225) and keeps its coefficients
+// (CoefficientCIterate = 2d/(lambda n0), PressureCoefficient = lambda n0 rho / (2 d dt^2),
+// radix/SPVSOLVER/MultiGPUSolver/MultiGPUHandler.cpp:1303-1314) but never defines it.  Its live
+// step uses an equation of state instead (DeviceProcess_Kernel.h:783-787).
+//
+// System, one row per sorted particle t:
+//   sum_j w_ij (p_t - p_j) = alpha (n*_t - n0s) / n0s,   alpha = lambda_lap n0_lap rho / (2 d dt^2)
+// with w_ij = GetWeight(re_lap, max(r_ij, re_lap 1e-3)) from the build-precision distance.  n0s
+// is the PND of the ideal lattice under this build's distance and weight (zero source for a
+// perfect lattice); lambda_lap and n0_lap are the exact-sqrt lattice values at re_lap.  Rows of
+// particles that are out of the zone, not binned, or at the free surface (n* <= beta n0) are
+// Dirichlet p = 0; the walls contribute no Laplacian term (Neumann), only their PND to n*.  The
+// pressure handed to the corrector is max(p, 0): the corrector is the reference's symmetric
+// (P_i + P_j) gradient, which would turn a suction pressure into an attraction.
+//
+// Storage: SELL-64 -- slices of 64 consecutive rows (one wave), columns stored slice-major so
+// lane i of a wave reads entry k of row i at slice_off + 64 k + i: every matrix load is a
+// coalesced 256 B (int32 columns, FP32 values) or 512 B (FP64 values) wave access.  The diagonal
+// is a separate FP64 vector.  Rows keep the canonical neighbour order, so the assembled matrix
+// equals the CPU restatement's bit for bit.
+//
+// Solvers:
+//   FP64   Jacobi-preconditioned CG, FP64 values, vectors and dots.
+//   MIXED  iterative refinement: FP64 residual r = b - A x (FP32 values, FP64 accumulation),
+//          inner Jacobi-PCG with FP32 values, FP32 vectors and SpMV, FP64 dot products, then
+//          x += e in FP64.  Repeats until the FP64 residual meets the tolerance.
+// Each PCG iteration is four launches (SpMV+dot, alpha, update+dots, beta); every kernel reads
+// a device-side "done" flag and returns at once after convergence, so the host queues fixed
+// chunks of iterations and looks at the flag once per chunk.
+#include <cstdio>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+#include <algorithm>
+#include "mpsr_internal.h"
+#include "neighbors.h"
+#include "ppe.h"
+
+namespace mpsr {
+
+static constexpr int PB = 256;          // block of the vector kernels (4 waves = 4 slices)
+static constexpr int CHUNK = 32;        // PCG iterations queued between two flag reads
+enum Scal { S_RZ, S_PQ, S_ALPHA, S_BETA, S_RR, S_TARGET, S_DONE, S_ITERS, S_MAXIT, S_BB, S_RR0, S_COUNT };
+
+template <typename R> struct PpeSolver {
+    int cap = 0;
+    int nslices = 0;
+    double alpha = 0, n0 = 0, n0_lap = 0, lambda_lap = 0;
+    int* active = nullptr;
+    int* dir = nullptr;
+    int* cnt = nullptr;
+    int* slice_w = nullptr;
+    int* slice_off = nullptr;
+    int* total = nullptr;
+    int* cols = nullptr;
+    double* vals64 = nullptr;
+    float* vals32 = nullptr;
+    long long nnz_cap = 0;
+    double* diag = nullptr;
+    double* rhs = nullptr;
+    double* x = nullptr;
+    double* prev = nullptr;      // last solution, slot order
+    int have_prev = 0;
+    double *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
+    float *rf = nullptr, *zf = nullptr, *p0f = nullptr, *p1f = nullptr, *qf = nullptr, *ef = nullptr;
+    double* part = nullptr;      // [2 * nblocks]
+    double* scal = nullptr;      // device scalars
+    double* scal_h = nullptr;    // pinned host mirror
+    int mixed = 0;
+    SortScratch scan{};
+    // last solve
+    int iters = 0, passes = 0, status = 0;
+    double relres = 0, relres0 = 0;
+    long long nnz_real = 0;
+};
+
+template <int NW> __device__ inline double block_sum(double v, double* lds) {
+    v = wave_sum(v);
+    const int lane = lane_id(), w = (int)(threadIdx.x >> 6);
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (w == 0) {
+        r = lane < NW ? lds[lane] : 0.0;
+        r = wave_sum(r);
+    }
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// assembly
+// ---------------------------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(PB) void k_ppe_active(int N, Consts<R> c, const uint32_t* __restrict__ skeys,
+                                                    const int* __restrict__ perm, PState<R> out,
+                                                    const R4<R>* __restrict__ spos, int* active) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int s = perm[t];
+    int a = 0;
+    if (out.id[s] >= 0 && skeys[t] != (uint32_t)c.ncell) {
+        const R4<R> P = spos[t];
+        if (include(c, P.x, P.y, P.z)) a = ((double)out.n[s] > (double)c.beta_n0) ? 1 : 0;
+    }
+    active[t] = a;
+}
+
+template <typename R>
+__global__ __launch_bounds__(PB) void k_ppe_count(int N, int nslices, Consts<R> c, const int* __restrict__ active,
+                                                   const R4<R>* __restrict__ spos, const int* __restrict__ cs,
+                                                   const int* __restrict__ ce, int* cnt, double* diag, int* dir,
+                                                   int* slice_w) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = 0;
+    double sum = 0.0;
+    const bool act = t < N && active[t];
+    if (act) {
+        const R4<R> P = spos[t];
+        const R re = c.re_lap, mind = c.mind_lap;
+        for_candidates(c, cs, ce, P.x, P.y, P.z, t, [&](int q) {
+            const R4<R> Q = spos[q];
+            R d = dist3(P.x, P.y, P.z, Q.x, Q.y, Q.z);
+            if (d < re) {
+                if (d < mind) d = mind;
+                const R w = weight(re, d);
+                sum += (double)w;
+                if (active[q]) ++n;
+            }
+        });
+    }
+    const bool dr = !(act && sum > 0.0);
+    if (dr) n = 0;
+    if (t < N) {
+        cnt[t] = n;
+        diag[t] = dr ? 1.0 : sum;
+        dir[t] = dr ? 1 : 0;
+    }
+    int m = n;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    const int slice = t >> 6;
+    if (lane_id() == 0 && slice < nslices) slice_w[slice] = m * 64;
+}
+
+template <typename R, typename VA>
+__global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c, const int* __restrict__ active,
+                                                  const int* __restrict__ dir, const R4<R>* __restrict__ spos,
+                                                  const int* __restrict__ cs, const int* __restrict__ ce,
+                                                  const int* __restrict__ slice_off, const int* __restrict__ slice_w,
+                                                  const int* __restrict__ perm, PState<R> out, double alpha, double n0,
+                                                  int* cols, VA* vals, double* rhs) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = t >> 6, lane = lane_id();
+    if (slice >= nslices) return;
+    const int base = slice_off[slice], width = slice_w[slice] >> 6;
+    int k = 0;
+    if (t < N && !dir[t]) {
+        const R4<R> P = spos[t];
+        const R re = c.re_lap, mind = c.mind_lap;
+        for_candidates(c, cs, ce, P.x, P.y, P.z, t, [&](int q) {
+            const R4<R> Q = spos[q];
+            R d = dist3(P.x, P.y, P.z, Q.x, Q.y, Q.z);
+            if (d < re) {
+                if (d < mind) d = mind;
+                const R w = weight(re, d);
+                if (active[q] && k < width) {
+                    cols[base + k * 64 + lane] = q;
+                    vals[base + k * 64 + lane] = (VA)(-(double)w);
+                    ++k;
+                }
+            }
+        });
+    }
+    const int self = t < N ? t : 0;
+    for (; k < width; ++k) {
+        cols[base + k * 64 + lane] = self;
+        vals[base + k * 64 + lane] = (VA)0;
+    }
+    if (t < N) rhs[t] = dir[t] ? 0.0 : alpha * ((double)out.n[perm[t]] - n0) / n0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// solver kernels
+// ---------------------------------------------------------------------------------------------
+// r = b - A x with FP64 accumulation (x, b, r FP64; values VA); partial sums of r.r and b.b
+template <typename VA>
+__global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __restrict__ slice_off,
+                                               const int* __restrict__ slice_w, const int* __restrict__ cols,
+                                               const VA* __restrict__ vals, const double* __restrict__ diag,
+                                               const double* __restrict__ x, const double* __restrict__ b,
+                                               double* r, double* part) {
+    __shared__ double lds[PB / 64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = t >> 6, lane = lane_id();
+    double rr = 0.0, bb = 0.0;
+    if (t < N) {
+        const int base = slice_off[slice], width = slice_w[slice] >> 6;
+        double a = diag[t] * x[t];
+        for (int k = 0; k < width; ++k) {
+            const int e = base + k * 64 + lane;
+            a += (double)vals[e] * x[cols[e]];
+        }
+        const double rv = b[t] - a;
+        r[t] = rv;
+        rr = rv * rv;
+        bb = b[t] * b[t];
+    }
+    rr = block_sum<PB / 64>(rr, lds);
+    bb = block_sum<PB / 64>(bb, lds);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = rr;
+        part[2 * blockIdx.x + 1] = bb;
+    }
+    (void)nslices;
+}
+
+// start of a PCG on A e = r0 (vectors V): r = r0, z = r / diag, p = 0, partial r.z and r.r
+template <typename V>
+__global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restrict__ r0, const double* __restrict__ diag,
+                                                   V* r, V* z, V* p, V* e, int zero_e, double* part) {
+    __shared__ double lds[PB / 64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (t < N) {
+        const V rv = (V)r0[t];
+        const V zv = rv / (V)diag[t];
+        r[t] = rv;
+        z[t] = zv;
+        p[t] = (V)0;
+        if (zero_e) e[t] = (V)0;
+        rz = (double)rv * (double)zv;
+        rr = (double)rv * (double)rv;
+    }
+    rz = block_sum<PB / 64>(rz, lds);
+    rr = block_sum<PB / 64>(rr, lds);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = rz;
+        part[2 * blockIdx.x + 1] = rr;
+    }
+}
+
+// p_new = z + beta p_old;  q = A p_new  (values VA, arithmetic V);  partial p_new.q
+template <typename VA, typename V>
+__global__ __launch_bounds__(PB) void k_pcg_spmv(int N, const int* __restrict__ slice_off,
+                                                  const int* __restrict__ slice_w, const int* __restrict__ cols,
+                                                  const VA* __restrict__ vals, const double* __restrict__ diag,
+                                                  const V* __restrict__ z, const V* __restrict__ pold, V* pnew, V* q,
+                                                  const double* __restrict__ scal, double* part) {
+    if (scal[S_DONE] != 0.0) return;
+    __shared__ double lds[PB / 64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = t >> 6, lane = lane_id();
+    const V beta = (V)scal[S_BETA];
+    double pq = 0.0;
+    if (t < N) {
+        const V pn = z[t] + beta * pold[t];
+        V a = (V)diag[t] * pn;
+        const int base = slice_off[slice], width = slice_w[slice] >> 6;
+        for (int k = 0; k < width; ++k) {
+            const int e = base + k * 64 + lane;
+            const int cidx = cols[e];
+            a += (V)vals[e] * (z[cidx] + beta * pold[cidx]);
+        }
+        pnew[t] = pn;
+        q[t] = a;
+        pq = (double)pn * (double)a;
+    }
+    pq = block_sum<PB / 64>(pq, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = pq;
+}
+
+// x += alpha p; r -= alpha q; z = r / diag; partial r.z and r.r
+template <typename V>
+__global__ __launch_bounds__(PB) void k_pcg_update(int N, V* x, V* r, V* z, const V* __restrict__ p,
+                                                    const V* __restrict__ q, const double* __restrict__ diag,
+                                                    const double* __restrict__ scal, double* part) {
+    if (scal[S_DONE] != 0.0) return;
+    __shared__ double lds[PB / 64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const V alpha = (V)scal[S_ALPHA];
+    double rz = 0.0, rr = 0.0;
+    if (t < N) {
+        x[t] = x[t] + alpha * p[t];
+        const V rv = r[t] - alpha * q[t];
+        const V zv = rv / (V)diag[t];
+        r[t] = rv;
+        z[t] = zv;
+        rz = (double)rv * (double)zv;
+        rr = (double)rv * (double)rv;
+    }
+    rz = block_sum<PB / 64>(rz, lds);
+    rr = block_sum<PB / 64>(rr, lds);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = rz;
+        part[2 * blockIdx.x + 1] = rr;
+    }
+}
+
+enum { RED_START = 0, RED_ALPHA = 1, RED_BETA = 2, RED_RESID = 3 };
+
+// one workgroup: sums the block partials (1 or 2 interleaved sums) and advances the scalars
+__global__ __launch_bounds__(1024) void k_pcg_reduce(const double* __restrict__ part, int nb, int nsum, double* scal,
+                                                      int mode) {
+    if (mode == RED_ALPHA || mode == RED_BETA)
+        if (scal[S_DONE] != 0.0) return;
+    __shared__ double lds[16];
+    double s0 = 0.0, s1 = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 1024) {
+        if (nsum == 2) {
+            s0 += part[2 * b];
+            s1 += part[2 * b + 1];
+        } else {
+            s0 += part[b];
+        }
+    }
+    s0 = block_sum<16>(s0, lds);
+    s1 = block_sum<16>(s1, lds);
+    if (threadIdx.x != 0) return;
+    if (mode == RED_START) {
+        scal[S_RZ] = s0;
+        scal[S_RR] = s1;
+        scal[S_RR0] = s1;
+        scal[S_BETA] = 0.0;
+        scal[S_ITERS] = 0.0;
+        scal[S_DONE] = (s1 <= scal[S_TARGET]) ? 1.0 : 0.0;
+    } else if (mode == RED_ALPHA) {
+        if (!(s0 > 0.0) || !isfinite(s0)) {
+            scal[S_DONE] = 3.0;
+            return;
+        }
+        scal[S_PQ] = s0;
+        scal[S_ALPHA] = scal[S_RZ] / s0;
+    } else if (mode == RED_BETA) {
+        scal[S_ITERS] += 1.0;
+        scal[S_BETA] = s0 / scal[S_RZ];
+        scal[S_RZ] = s0;
+        scal[S_RR] = s1;
+        if (!isfinite(s1)) scal[S_DONE] = 3.0;
+        else if (s1 <= scal[S_TARGET]) scal[S_DONE] = 1.0;
+        else if (scal[S_ITERS] >= scal[S_MAXIT]) scal[S_DONE] = 2.0;
+    } else {  // RED_RESID: residual norm and rhs norm of an FP64 residual pass
+        scal[S_RR] = s0;
+        scal[S_BB] = s1;
+    }
+}
+
+__global__ __launch_bounds__(PB) void k_add_f2d(int N, double* x, const float* e) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) x[t] = x[t] + (double)e[t];
+}
+
+// x0 from the previous solution (slot order) or zero; Dirichlet rows are 0
+__global__ __launch_bounds__(PB) void k_warm(int N, const int* __restrict__ perm, const int* __restrict__ dir,
+                                              const double* __restrict__ prev, int use_prev, double* x) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    x[t] = (use_prev && !dir[t]) ? prev[perm[t]] : 0.0;
+}
+
+// pressure to the particle state (the EOS kernel's writes, Kern :796-800) and to the sorted
+// pressure array; the solution is kept in slot order for the next warm start
+template <typename R>
+__global__ __launch_bounds__(PB) void k_ppe_writeback(int N, Consts<R> c, const int* __restrict__ perm,
+                                                       const int* __restrict__ dir, const double* __restrict__ x,
+                                                       PState<R> in, PState<R> out, const R4<R>* __restrict__ spos,
+                                                       R* spres, double* prev) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int s = perm[t];
+    const R4<R> P = spos[t];
+    prev[s] = dir[t] ? 0.0 : x[t];
+    if (!include(c, P.x, P.y, P.z)) {
+        in.p[s] = 0;
+        spres[t] = out.p[s];
+        return;
+    }
+    R pr = dir[t] ? R(0) : (R)x[t];
+    if (pr < 0) pr = 0;
+    out.p[s] = pr;
+    in.p[s] = pr;
+    spres[t] = pr;
+}
+
+// ---------------------------------------------------------------------------------------------
+template <typename T> static bool dev_alloc(T** p, size_t n) {
+    return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)) == hipSuccess;
+}
+
+template <typename R> static double ppe_alpha(const mpsr_params& P, double n0_lap, double lambda_lap) {
+    return lambda_lap * n0_lap * (double)(R)P.Density / (2.0 * P.Dimension * (double)(R)P.Dt * (double)(R)P.Dt);
+}
+
+template <typename R>
+PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_options& opt, double n0_lap,
+                         double lambda_lap, double n0_src) {
+    auto* s = new PpeSolver<R>();
+    s->cap = d.cap;
+    s->nslices = (d.cap + 63) / 64;
+    s->n0_lap = n0_lap;
+    s->lambda_lap = lambda_lap;
+    s->alpha = ppe_alpha<R>(P, n0_lap, lambda_lap);
+    s->n0 = n0_src;
+    const int rows = s->nslices * 64;
+    const int nb = (rows + PB - 1) / PB;
+    bool ok = true;
+    ok &= dev_alloc(&s->active, rows) && dev_alloc(&s->dir, rows) && dev_alloc(&s->cnt, rows);
+    ok &= dev_alloc(&s->slice_w, s->nslices + 1) && dev_alloc(&s->slice_off, s->nslices + 1) && dev_alloc(&s->total, 1);
+    ok &= dev_alloc(&s->diag, rows) && dev_alloc(&s->rhs, rows) && dev_alloc(&s->x, rows) && dev_alloc(&s->prev, rows);
+    ok &= dev_alloc(&s->r, rows) && dev_alloc(&s->z, rows) && dev_alloc(&s->p0, rows) && dev_alloc(&s->p1, rows) &&
+          dev_alloc(&s->q, rows);
+    ok &= dev_alloc(&s->rf, rows) && dev_alloc(&s->zf, rows) && dev_alloc(&s->p0f, rows) && dev_alloc(&s->p1f, rows) &&
+          dev_alloc(&s->qf, rows) && dev_alloc(&s->ef, rows);
+    ok &= dev_alloc(&s->part, 2 * nb + 2) && dev_alloc(&s->scal, S_COUNT);
+    s->scan.scan_tmp_cap = (int)scan_tmp_ints(s->nslices + 1);
+    ok &= dev_alloc(&s->scan.scan_tmp, s->scan.scan_tmp_cap);
+    ok &= hipHostMalloc((void**)&s->scal_h, sizeof(double) * S_COUNT) == hipSuccess;
+    if (!ok) {
+        ppe_destroy(s);
+        return nullptr;
+    }
+    (void)hipMemset(s->prev, 0, sizeof(double) * rows);
+    (void)hipMemset(s->scal, 0, sizeof(double) * S_COUNT);
+    s->mixed = opt.solver == MPSR_PCG_MIXED;
+    return s;
+}
+
+template <typename R> void ppe_destroy(PpeSolver<R>* s) {
+    if (!s) return;
+    void* ps[] = {s->active, s->dir, s->cnt, s->slice_w, s->slice_off, s->total, s->cols, s->vals64, s->vals32,
+                  s->diag, s->rhs, s->x, s->prev, s->r, s->z, s->p0, s->p1, s->q, s->rf, s->zf, s->p0f,
+                  s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    if (s->scal_h) (void)hipHostFree(s->scal_h);
+    delete s;
+}
+
+template <typename R> void ppe_update_params(PpeSolver<R>* s, const mpsr_params& P) {
+    s->alpha = ppe_alpha<R>(P, s->n0_lap, s->lambda_lap);
+}
+
+// read the device scalars (synchronises the stream)
+template <typename R> static hipError_t read_scal(PpeSolver<R>* s, hipStream_t st) {
+    hipError_t e = hipMemcpyAsync(s->scal_h, s->scal, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+}
+
+// Runs one PCG on A e = r0 from e = 0 (V = FP32) or on A x = b from the x in s->x (V = FP64)
+// until r.r <= target.  Returns iterations, or -1 on breakdown.
+template <typename R, typename VA, typename V>
+static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* pa, V* pb, V* q,
+                   const double* r0, double target, int maxit, hipStream_t st) {
+    const int rows = s->nslices * 64;
+    const int nb = (rows + PB - 1) / PB;
+    s->scal_h[S_TARGET] = target;
+    s->scal_h[S_MAXIT] = (double)maxit;
+    if (hipMemcpyAsync(s->scal + S_TARGET, s->scal_h + S_TARGET, sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+        return -1;
+    if (hipMemcpyAsync(s->scal + S_MAXIT, s->scal_h + S_MAXIT, sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+        return -1;
+    // the PCG start zeroes x only on the refinement path (x is the correction e there)
+    const int zero_x = std::is_same<V, float>::value ? 1 : 0;
+    hipLaunchKernelGGL(k_pcg_start<V>, dim3(nb), dim3(PB), 0, st, N, r0, s->diag, r, z, pa, x, zero_x, s->part);
+    hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_START);
+    int done_iters = 0;
+    for (int it = 0; it < maxit; it += CHUNK) {
+        for (int k = 0; k < CHUNK; ++k) {
+            V* pold = (k & 1) ? pb : pa;
+            V* pnew = (k & 1) ? pa : pb;
+            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, N, s->slice_off, s->slice_w, s->cols,
+                               vals, s->diag, z, pold, pnew, q, s->scal, s->part);
+            hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 1, s->scal, (int)RED_ALPHA);
+            hipLaunchKernelGGL(k_pcg_update<V>, dim3(nb), dim3(PB), 0, st, N, x, r, z, pnew, q, s->diag, s->scal,
+                               s->part);
+            hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_BETA);
+        }
+        if (read_scal(s, st) != hipSuccess) return -1;
+        done_iters = (int)s->scal_h[S_ITERS];
+        if (s->scal_h[S_DONE] != 0.0) break;
+    }
+    if (s->scal_h[S_DONE] == 3.0) return -1;
+    return done_iters;
+}
+
+template <typename R> static double resid_pass(PpeSolver<R>* s, int N, hipStream_t st, double* bb_out) {
+    const int rows = s->nslices * 64;
+    const int nb = (rows + PB - 1) / PB;
+    if (s->mixed)
+        hipLaunchKernelGGL(k_resid<float>, dim3(nb), dim3(PB), 0, st, N, s->nslices, s->slice_off, s->slice_w, s->cols,
+                           s->vals32, s->diag, s->x, s->rhs, s->r, s->part);
+    else
+        hipLaunchKernelGGL(k_resid<double>, dim3(nb), dim3(PB), 0, st, N, s->nslices, s->slice_off, s->slice_w,
+                           s->cols, s->vals64, s->diag, s->x, s->rhs, s->r, s->part);
+    hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_RESID);
+    if (read_scal(s, st) != hipSuccess) return -1.0;
+    *bb_out = s->scal_h[S_BB];
+    return s->scal_h[S_RR];
+}
+
+template <typename R> mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, hipStream_t st) {
+    const int N = d.N;
+    const int nsl = (N + 63) / 64;
+    const int rows = nsl * 64;
+    const int nb = (rows + PB - 1) / PB;
+    s->mixed = opt.solver == MPSR_PCG_MIXED;
+    s->status = 0;
+    // ---- assembly ----
+    hipLaunchKernelGGL(k_ppe_active<R>, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.c, d.skeys, d.perm, d.out,
+                       d.spos, s->active);
+    hipLaunchKernelGGL(k_ppe_count<R>, dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, d.spos, d.cell_start,
+                       d.cell_end, s->cnt, s->diag, s->dir, s->slice_w);
+    exclusive_scan(s->slice_w, s->slice_off, nullptr, nsl, nsl, s->total, s->scan, st);
+    int total = 0;
+    if (hipMemcpyAsync(&s->scal_h[0], s->total, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return MPSR_GPUERR;
+    if (hipStreamSynchronize(st) != hipSuccess) return MPSR_GPUERR;
+    std::memcpy(&total, &s->scal_h[0], sizeof(int));
+    if ((long long)total > s->nnz_cap) {
+        if (s->cols) (void)hipFree(s->cols);
+        if (s->vals64) (void)hipFree(s->vals64);
+        if (s->vals32) (void)hipFree(s->vals32);
+        s->cols = nullptr; s->vals64 = nullptr; s->vals32 = nullptr;
+        s->nnz_cap = (long long)(total * 1.25) + 4096;
+        if (!dev_alloc(&s->cols, (size_t)s->nnz_cap)) return MPSR_MEMERR;
+        if (!dev_alloc(&s->vals64, (size_t)s->nnz_cap)) return MPSR_MEMERR;
+        if (!dev_alloc(&s->vals32, (size_t)s->nnz_cap)) return MPSR_MEMERR;
+    }
+    if (s->mixed)
+        hipLaunchKernelGGL((k_ppe_fill<R, float>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
+                           d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, s->alpha, s->n0,
+                           s->cols, s->vals32, s->rhs);
+    else
+        hipLaunchKernelGGL((k_ppe_fill<R, double>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
+                           d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, s->alpha, s->n0,
+                           s->cols, s->vals64, s->rhs);
+    s->nslices = nsl;
+    // ---- solve ----
+    const double tol = opt.rel_tol > 0 ? opt.rel_tol : 1e-8;
+    const int maxit = opt.max_iter > 0 ? opt.max_iter : 20000;
+    const int use_prev = (opt.warm_start == MPSR_WARM_PREVIOUS && s->have_prev) ? 1 : 0;
+    hipLaunchKernelGGL(k_warm, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.perm, s->dir, s->prev, use_prev, s->x);
+    double bb = 0.0;
+    double rr = resid_pass(s, N, st, &bb);
+    if (rr < 0) return MPSR_GPUERR;
+    s->relres0 = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    const double target = tol * tol * bb;
+    s->iters = 0;
+    s->passes = 0;
+    if (!s->mixed) {
+        if (bb > 0 && rr > target) {
+            // FP64 Jacobi-PCG from the warm start; r already holds b - A x
+            const int it = run_pcg<R, double, double>(s, N, s->vals64, s->x, s->r, s->z, s->p0, s->p1, s->q, s->r,
+                                                      target, maxit, st);
+            if (it < 0) s->status = 1;
+            else s->iters = it;
+            s->passes = 1;
+            rr = resid_pass(s, N, st, &bb);
+        }
+    } else {
+        // iterative refinement: FP64 residual, FP32 inner PCG with FP64 dots, FP64 update
+        while (bb > 0 && rr > target && s->passes < 16 && s->iters < maxit && s->status == 0) {
+            const double inner_rel = std::max(1e-6, 0.5 * tol * std::sqrt(bb / rr));
+            const int it = run_pcg<R, float, float>(s, N, s->vals32, s->ef, s->rf, s->zf, s->p0f, s->p1f, s->qf, s->r,
+                                                    inner_rel * inner_rel * rr, maxit - s->iters, st);
+            if (it < 0) {
+                s->status = 1;
+                break;
+            }
+            hipLaunchKernelGGL(k_add_f2d, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, s->x, s->ef);
+            s->iters += it;
+            s->passes += 1;
+            rr = resid_pass(s, N, st, &bb);
+            if (rr < 0) return MPSR_GPUERR;
+            if (it == 0) break;
+        }
+    }
+    s->relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    if (!std::isfinite(s->relres)) s->status = 1;
+    hipLaunchKernelGGL(k_ppe_writeback<R>, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.c, d.perm, s->dir, s->x, d.in,
+                       d.out, d.spos, d.spres, s->prev);
+    s->have_prev = 1;
+    return MPSR_NOERR;
+}
+
+template <typename R> mpsr_status ppe_check(PpeSolver<R>* s, mpsr_timings* tm) {
+    tm->ppe_iterations = s->iters;
+    tm->ppe_refinements = s->passes;
+    tm->ppe_rel_residual = s->relres;
+    tm->ppe_initial_rel_residual = s->relres0;
+    return s->status == 0 ? MPSR_NOERR : MPSR_CALCERR;
+}
+
+template <typename R> mpsr_status ppe_get(PpeSolver<R>* s, const Dev<R>& d, double* rhs, double* x, int* dir) {
+    const int N = d.N;
+    std::vector<double> b(N), xx(N);
+    std::vector<int> dd(N), pp(N);
+    if (hipMemcpy(b.data(), s->rhs, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    if (hipMemcpy(xx.data(), s->x, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    if (hipMemcpy(dd.data(), s->dir, sizeof(int) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    if (hipMemcpy(pp.data(), d.perm, sizeof(int) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    for (int t = 0; t < N; ++t) {
+        const int sl = pp[t];
+        if (rhs) rhs[sl] = b[t];
+        if (x) x[sl] = xx[t];
+        if (dir) dir[sl] = dd[t];
+    }
+    return MPSR_NOERR;
+}
+
+template <typename R>
+mpsr_status ppe_get_matrix(PpeSolver<R>* s, const Dev<R>& d, int* ptr, int* col, double* val, double* diag, int cap,
+                           int* nnz) {
+    const int N = d.N;
+    const int nsl = (N + 63) / 64;
+    std::vector<int> sw(nsl), so(nsl), cn(N), pp(N), dd(N);
+    std::vector<double> dg(N);
+    if (hipMemcpy(sw.data(), s->slice_w, sizeof(int) * nsl, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    if (hipMemcpy(so.data(), s->slice_off, sizeof(int) * nsl, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    if (hipMemcpy(cn.data(), s->cnt, sizeof(int) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    if (hipMemcpy(pp.data(), d.perm, sizeof(int) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    if (hipMemcpy(dd.data(), s->dir, sizeof(int) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    if (hipMemcpy(dg.data(), s->diag, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    const long long total = nsl ? (long long)so[nsl - 1] + sw[nsl - 1] : 0;
+    std::vector<int> cc(total);
+    std::vector<double> vv(total);
+    if (total) {
+        if (hipMemcpy(cc.data(), s->cols, sizeof(int) * total, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+        if (s->mixed) {
+            std::vector<float> vf(total);
+            if (hipMemcpy(vf.data(), s->vals32, sizeof(float) * total, hipMemcpyDeviceToHost) != hipSuccess)
+                return MPSR_GPUERR;
+            for (long long e = 0; e < total; ++e) vv[e] = vf[e];
+        } else if (hipMemcpy(vv.data(), s->vals64, sizeof(double) * total, hipMemcpyDeviceToHost) != hipSuccess) {
+            return MPSR_GPUERR;
+        }
+    }
+    std::vector<int> row_of(N);
+    for (int t = 0; t < N; ++t) row_of[pp[t]] = t;
+    int e = 0;
+    ptr[0] = 0;
+    for (int sl = 0; sl < N; ++sl) {
+        const int t = row_of[sl];
+        if (diag) diag[sl] = dg[t];
+        const int n = dd[t] ? 0 : cn[t];
+        for (int k = 0; k < n; ++k, ++e) {
+            const long long idx = (long long)so[t >> 6] + (long long)k * 64 + (t & 63);
+            if (e < cap) {
+                col[e] = pp[cc[idx]];
+                val[e] = vv[idx];
+            }
+        }
+        ptr[sl + 1] = e;
+    }
+    if (nnz) *nnz = e;
+    return MPSR_NOERR;
+}
+
+}  // namespace mpsr
+
+namespace mpsr {
+#define MPSR_INSTANTIATE_P(R)                                                                                        \
+    template mpsr_status ppe_get_matrix<R>(PpeSolver<R>*, const Dev<R>&, int*, int*, double*, double*, int, int*);   \
+    template PpeSolver<R>* ppe_create<R>(const Dev<R>&, const mpsr_params&, const mpsr_ppe_options&, double, double, \
+                                         double);                                                                    \
+    template void ppe_destroy<R>(PpeSolver<R>*);                                                                     \
+    template mpsr_status ppe_solve<R>(PpeSolver<R>*, Dev<R>&, const mpsr_ppe_options&, hipStream_t);                 \
+    template mpsr_status ppe_check<R>(PpeSolver<R>*, mpsr_timings*);                                                 \
+    template mpsr_status ppe_get<R>(PpeSolver<R>*, const Dev<R>&, double*, double*, int*);                           \
+    template void ppe_update_params<R>(PpeSolver<R>*, const mpsr_params&);
+MPSR_INSTANTIATE_P(float)
+MPSR_INSTANTIATE_P(double)
+}  // namespace mpsr

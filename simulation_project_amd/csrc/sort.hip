@@ -1,0 +1,277 @@
+// sort.hip -- LDS-staged stable LSD radix sort of (key, value) pairs and a device-wide
+// exclusive scan, both with an optional device-resident element count.
+//
+// Replaces thrust::sort_by_key of the reference's sorted-binning variant
+// (radix/SPVSOLVER/MultiGPUSolver/DeviceProcess.cu:207-220).  The sort is stable, so sorting
+// (cell, slot) pairs produced in slot order yields cells in ascending order and, inside a cell,
+// ascending slots: the canonical permutation.
+//
+// One pass over a digit of <= 8 bits:
+//   upsweep   per 2048-key tile, a digit histogram in LDS -> hist[digit * ntiles + tile]
+//   scan      exclusive scan of hist (digit-major), one workgroup
+//   downsweep per tile: wave64 ballot matching ranks every key among equal digits of its
+//             wave, an LDS table turns that into a rank inside the tile, the tile is staged in
+//             LDS in digit order and written out so that equal digits go to consecutive
+//             addresses.
+#include "mpsr_internal.h"
+
+namespace mpsr {
+
+static constexpr int RS_BLOCK = 256;  // 4 waves
+static constexpr int RS_WAVES = RS_BLOCK / 64;
+static constexpr int RS_ITEMS = 8;    // rounds of 64 keys per wave
+static constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
+static constexpr int SCAN_BLOCK = 1024;
+static constexpr int SCAN_ITEMS = 4;
+static constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+
+size_t radix_hist_ints(int n_cap) { return (size_t)256 * ((n_cap + RS_TILE - 1) / RS_TILE + 1); }
+size_t scan_tmp_ints(int n_cap) { return (size_t)((n_cap + SCAN_TILE - 1) / SCAN_TILE + 2) + 64; }
+
+// element count: the device-resident count clamped to the capacity, or the host count
+__device__ inline int dev_n(const int* d_n, int n_host, int n_cap) { return d_n ? min(*d_n, n_cap) : n_host; }
+
+__global__ void k_fill_int(int* p, int value, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+void fill_int(int* p, int value, long long n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(n, 256)), dim3(256), 0, st, p, value, n);
+}
+
+// ---------------------------------------------------------------------------------------------
+// generic exclusive scan: per-tile sums -> scan of tile sums in one workgroup -> tile scans
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile_sums(const int* in, const int* d_n, int n_host, int n_cap, int* sums) {
+    const int n = dev_n(d_n, n_host, n_cap);
+    const int base = blockIdx.x * SCAN_TILE;
+    if (base >= n) return;
+    __shared__ int lds[SCAN_BLOCK / 64 + 1];
+    int s = 0;
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        int i = base + r * SCAN_BLOCK + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    s = wave_sum(s);
+    if (lane_id() == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < SCAN_BLOCK / 64; ++w) t += lds[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+// In-place exclusive scan of m = ceil(n / tile) ints in one workgroup; total to sums[m] and to
+// *d_total.  Used for tile sums (tile = SCAN_TILE) and for radix histograms (tile = 1 with the
+// element count given directly).
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(int* a, const int* d_n, int n_host, int n_cap, int tile, int mult,
+                                                            int* d_total) {
+    const int n = dev_n(d_n, n_host, n_cap);
+    const int m = ((n + tile - 1) / tile) * mult;
+    __shared__ int lds[SCAN_BLOCK / 64 + 1];
+    int carry = 0;
+    for (int base = 0; base < m; base += SCAN_BLOCK * SCAN_ITEMS) {
+        // each thread owns SCAN_ITEMS consecutive elements
+        int v[SCAN_ITEMS];
+        int local = 0;
+        const int i0 = base + threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+        for (int r = 0; r < SCAN_ITEMS; ++r) {
+            v[r] = (i0 + r < m) ? a[i0 + r] : 0;
+            local += v[r];
+        }
+        int total;
+        int pre = block_exclusive_scan<SCAN_BLOCK / 64>(local, lds, &total);
+        int run = carry + pre;
+#pragma unroll
+        for (int r = 0; r < SCAN_ITEMS; ++r) {
+            if (i0 + r < m) a[i0 + r] = run;
+            run += v[r];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) {
+        if (d_total) *d_total = carry;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const int* in, int* out, const int* d_n, int n_host,
+                                                           int n_cap, const int* sums) {
+    const int n = dev_n(d_n, n_host, n_cap);
+    const int base = blockIdx.x * SCAN_TILE;
+    if (base >= n) return;
+    __shared__ int lds[SCAN_BLOCK / 64 + 1];
+    int v[SCAN_ITEMS];
+    int local = 0;
+    const int i0 = base + threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        v[r] = (i0 + r < n) ? in[i0 + r] : 0;
+        local += v[r];
+    }
+    int total;
+    int run = block_exclusive_scan<SCAN_BLOCK / 64>(local, lds, &total) + sums[blockIdx.x];
+#pragma unroll
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+        if (i0 + r < n) out[i0 + r] = run;
+        run += v[r];
+    }
+}
+
+void exclusive_scan(const int* in, int* out, const int* d_n, int n_host, int n_cap, int* d_total,
+                    const SortScratch& s, hipStream_t st) {
+    const int cap = d_n ? n_cap : n_host;
+    if (cap <= 0) {
+        if (d_total) fill_int(d_total, 0, 1, st);
+        return;
+    }
+    const int tiles = (cap + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(tiles), dim3(SCAN_BLOCK), 0, st, in, d_n, n_host, cap, s.scan_tmp);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(SCAN_BLOCK), 0, st, s.scan_tmp, d_n, n_host, cap, SCAN_TILE, 1,
+                       d_total);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(SCAN_BLOCK), 0, st, in, out, d_n, n_host, cap, s.scan_tmp);
+}
+
+// ---------------------------------------------------------------------------------------------
+// radix sort
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_upsweep(const uint32_t* keys, const int* d_n, int n_host, int n_cap, int shift,
+                                                          int nbits, int* hist) {
+    const int n = dev_n(d_n, n_host, n_cap);
+    const int ntiles = (n + RS_TILE - 1) / RS_TILE;
+    const int b = blockIdx.x;
+    if (b >= ntiles) return;
+    const int radix = 1 << nbits;
+    const uint32_t mask = (uint32_t)radix - 1u;
+    __shared__ int h[256];
+    for (int d = threadIdx.x; d < radix; d += RS_BLOCK) h[d] = 0;
+    __syncthreads();
+    const int base = b * RS_TILE;
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        int i = base + r * RS_BLOCK + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < radix; d += RS_BLOCK) hist[d * ntiles + b] = h[d];
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_downsweep(const uint32_t* __restrict__ kin, const int* __restrict__ vin,
+                                                            uint32_t* __restrict__ kout, int* __restrict__ vout,
+                                                            const int* d_n, int n_host, int n_cap, int shift, int nbits,
+                                                            const int* __restrict__ hist) {
+    const int n = dev_n(d_n, n_host, n_cap);
+    const int ntiles = (n + RS_TILE - 1) / RS_TILE;
+    const int b = blockIdx.x;
+    if (b >= ntiles) return;
+    const int radix = 1 << nbits;
+    const uint32_t mask = (uint32_t)radix - 1u;
+    const int lane = lane_id(), w = (int)(threadIdx.x >> 6);
+
+    __shared__ int wh[RS_WAVES][256];      // per-wave running digit counts, later wave prefixes
+    __shared__ int tile_off[256];          // digit offset inside the sorted tile
+    __shared__ int gbase[256];             // global offset of this tile's first key of a digit
+    __shared__ uint32_t sk[RS_TILE];
+    __shared__ int sv[RS_TILE];
+    __shared__ int lds_scan[RS_WAVES + 1];
+
+    for (int d = threadIdx.x; d < 256; d += RS_BLOCK) {
+#pragma unroll
+        for (int q = 0; q < RS_WAVES; ++q) wh[q][d] = 0;
+    }
+    __syncthreads();
+
+    const int base = b * RS_TILE + w * (RS_ITEMS * 64);
+    uint32_t key[RS_ITEMS];
+    int val[RS_ITEMS], dig[RS_ITEMS], rank[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const int i = base + r * 64 + lane;
+        const bool valid = i < n;
+        key[r] = valid ? kin[i] : 0xffffffffu;
+        val[r] = valid ? vin[i] : 0;
+        const int d = (int)((key[r] >> shift) & mask);
+        dig[r] = valid ? d : -1;
+        // lanes of this round holding the same digit
+        unsigned long long peers = __ballot(valid);
+        for (int bit = 0; bit < nbits; ++bit) {
+            const unsigned long long bb = __ballot((d >> bit) & 1);
+            peers &= ((d >> bit) & 1) ? bb : ~bb;
+        }
+        const unsigned long long lower = peers & ((1ull << lane) - 1ull);
+        int cnt = 0;
+        if (valid) cnt = wh[w][d];
+        rank[r] = cnt + __popcll(lower);
+        if (valid && lower == 0ull) wh[w][d] = cnt + __popcll(peers);
+    }
+    __syncthreads();
+    // per digit: prefix over waves, tile total, global base
+    int tot = 0;
+    const int d = threadIdx.x;
+    if (d < radix) {
+        int run = 0;
+#pragma unroll
+        for (int q = 0; q < RS_WAVES; ++q) {
+            int t = wh[q][d];
+            wh[q][d] = run;
+            run += t;
+        }
+        tot = run;
+        gbase[d] = hist[d * ntiles + b];
+    }
+    int total;
+    int off = block_exclusive_scan<RS_WAVES>(d < radix ? tot : 0, lds_scan, &total);
+    if (d < radix) tile_off[d] = off;
+    __syncthreads();
+    // stage the tile in digit order
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        if (dig[r] >= 0) {
+            const int pos = tile_off[dig[r]] + wh[w][dig[r]] + rank[r];
+            sk[pos] = key[r];
+            sv[pos] = val[r];
+        }
+    }
+    __syncthreads();
+    const int m = min(RS_TILE, n - b * RS_TILE);
+    for (int i = threadIdx.x; i < m; i += RS_BLOCK) {
+        const uint32_t k = sk[i];
+        const int dd = (int)((k >> shift) & mask);
+        const int g = gbase[dd] + (i - tile_off[dd]);
+        kout[g] = k;
+        vout[g] = sv[i];
+    }
+}
+
+int radix_sort_pairs(uint32_t* k0, int* v0, const int* d_n, int n_host, int n_cap, int bits,
+                     const SortScratch& s, hipStream_t st) {
+    const int cap = d_n ? n_cap : n_host;
+    if (cap <= 0 || bits <= 0) return 0;
+    const int tiles = (cap + RS_TILE - 1) / RS_TILE;
+    const int passes = (bits + 7) / 8;
+    const int width = (bits + passes - 1) / passes;
+    uint32_t* ka = k0;
+    int* va = v0;
+    uint32_t* kb = s.k1;
+    int* vb = s.v1;
+    int which = 0;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = p * width;
+        const int nb = min(width, bits - shift);
+        hipLaunchKernelGGL(k_rs_upsweep, dim3(tiles), dim3(RS_BLOCK), 0, st, ka, d_n, n_host, cap, shift, nb, s.hist);
+        // exclusive scan of the digit-major histogram: (ntiles * radix) entries
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(SCAN_BLOCK), 0, st, s.hist, d_n, n_host, cap, RS_TILE, 1 << nb,
+                           (int*)nullptr);
+        hipLaunchKernelGGL(k_rs_downsweep, dim3(tiles), dim3(RS_BLOCK), 0, st, ka, va, kb, vb, d_n, n_host, cap, shift, nb,
+                           s.hist);
+        uint32_t* tk = ka; ka = kb; kb = tk;
+        int* tv = va; va = vb; vb = tv;
+        which ^= 1;
+    }
+    return which;
+}
+
+}  // namespace mpsr

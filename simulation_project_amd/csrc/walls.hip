@@ -1,0 +1,421 @@
+// walls.hip -- STL wall kernels.
+//
+//   triangle binning   RegisterTriangleTopology_Kernel / RegisterTriangle / RegisterLine
+//                      (Source/DeviceProcess_Kernel.h:443-460, Utility_Inline.h:1747-1857).
+//                      The reference appends ids with atomicCAS/atomicInc in arrival order and
+//                      wraps at 100 per cell.  Here: every triangle marches its edges and its
+//                      shrunken copies twice (count, then emit (cell, triangle) pairs), the pairs
+//                      are radix-sorted by cell (stable, so ids ascend inside a cell), duplicates
+//                      dropped, and each cell lists its first 100 ids.  Deterministic.
+//   k_stl_distance     CalcSTLDistance_Kernel (Kern :635-699), nearest non-drag triangle
+//   k_drag             CalcDragEffect_Kernel (Kern :106-281), nearest drag triangle
+//   k_translate        UpdateTrianglePosition_Kernel (Kern :463-482)
+//   k_rotate           RotateTrianglePosition_Kernel (Kern :930-998); cos/sin of the step's
+//                      angle come from the host so the rotation is bit-reproducible
+#include "mpsr_internal.h"
+
+namespace mpsr {
+
+static constexpr int kCellMaxTriangles = 100;
+static constexpr int kMaxShrink = 4096;
+static constexpr int kMaxLineSteps = 1 << 20;
+
+template <typename R> struct P3 { R x, y, z; };
+template <typename R> __device__ inline P3<R> p3(R x, R y, R z) { return P3<R>{x, y, z}; }
+template <typename R> __device__ inline P3<R> psub(P3<R> a, P3<R> b) { return p3(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <typename R> __device__ inline P3<R> padd(P3<R> a, P3<R> b) { return p3(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <typename R> __device__ inline R pdot(P3<R> a, P3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename R> __device__ inline R pmag(P3<R> a) { return build_sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+// ---------------------------------------------------------------------------------------------
+// triangle binning
+// ---------------------------------------------------------------------------------------------
+// One RegisterLine march: sample V + TAB*D for TAB = 0, cell, 2*cell, ... while TAB <= Mag.
+// Emits a cell only when it differs from the previous emission of this triangle.
+template <typename R, typename F>
+__device__ inline void march_line(const Consts<R>& c, P3<R> v, P3<R> d, R m, int& last, F&& emit) {
+    R tab = 0;
+    int steps = 0;
+    do {
+        const R tx = v.x + tab * d.x, ty = v.y + tab * d.y, tz = v.z + tab * d.z;
+        int i, j, k;
+        if (cell_coords(c, tx, ty, tz, i, j, k)) {
+            const long long cid = cell_of(c, i, j, k);
+            if (cid >= 0 && cid < c.ncell && (int)cid != last) {
+                last = (int)cid;
+                emit((int)cid);
+            }
+        }
+        tab += c.cell;
+    } while (tab <= m && ++steps < kMaxLineSteps);
+}
+
+template <typename R, typename F> __device__ inline void march_triangle(const Consts<R>& c, const TriD<R>& tr, F&& emit) {
+    P3<R> A = p3(tr.v[0], tr.v[1], tr.v[2]), B = p3(tr.v[3], tr.v[4], tr.v[5]), C = p3(tr.v[6], tr.v[7], tr.v[8]);
+    for (int q = 0; q < 9; ++q)
+        if (!isfinite(tr.v[q])) return;
+    const R cs = c.cell;
+    R mBC, mAB, mCA;
+    int iter = 0, last = -1;
+    do {
+        const P3<R> AB = psub(B, A), BA = psub(A, B), BC = psub(C, B), CB = psub(B, C), CA = psub(A, C), AC = psub(C, A);
+        mBC = pmag(BC); mAB = pmag(AB); mCA = pmag(CA);
+        if (mAB > 0.0) march_line(c, A, p3(AB.x / mAB, AB.y / mAB, AB.z / mAB), mAB, last, emit);
+        if (mBC > 0.0) march_line(c, B, p3(BC.x / mBC, BC.y / mBC, BC.z / mBC), mBC, last, emit);
+        if (mCA > 0.0) march_line(c, C, p3(CA.x / mCA, CA.y / mCA, CA.z / mCA), mCA, last, emit);
+        P3<R> AM = padd(AB, AC), BM = padd(BA, BC), CM = padd(CA, CB);
+        const R mAM = pmag(AM), mBM = pmag(BM), mCM = pmag(CM);
+        if (mAM > 0.0) { AM.x /= mAM; AM.y /= mAM; AM.z /= mAM; A.x += AM.x * cs; A.y += AM.y * cs; A.z += AM.z * cs; }
+        if (mBM > 0.0) { BM.x /= mBM; BM.y /= mBM; BM.z /= mBM; B.x += BM.x * cs; B.y += BM.y * cs; B.z += BM.z * cs; }
+        if (mCM > 0.0) { CM.x /= mCM; CM.y /= mCM; CM.z /= mCM; C.x += CM.x * cs; C.y += CM.y * cs; C.z += CM.z * cs; }
+    } while (mBC >= cs && mAB >= cs && mCA >= cs && ++iter < kMaxShrink);
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_tri_count(int T, Consts<R> c, const TriD<R>* __restrict__ tris, int* emit) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const TriD<R> tr = tris[t];
+    int n = 0;
+    march_triangle(c, tr, [&](int) { ++n; });
+    emit[t] = n;
+}
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_tri_fill(int T, Consts<R> c, const TriD<R>* __restrict__ tris,
+                                                 const int* __restrict__ off, const int* __restrict__ pair_n,
+                                                 int pair_cap, uint32_t* pair_cell, int* pair_tri, int* overflow) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    if (*pair_n > pair_cap) {
+        if (t == 0) *overflow = 1;
+        return;
+    }
+    const TriD<R> tr = tris[t];
+    int e = off[t];
+    march_triangle(c, tr, [&](int cid) {
+        pair_cell[e] = (uint32_t)cid;
+        pair_tri[e] = t;
+        ++e;
+    });
+}
+
+// 1 where a sorted pair differs from its predecessor
+__global__ __launch_bounds__(256) void k_pair_flags(const int* pair_n, int pair_cap, const uint32_t* __restrict__ cell,
+                                                     const int* __restrict__ tri, int* flag) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = min(*pair_n, pair_cap);
+    if (e >= n) return;
+    flag[e] = (e == 0 || cell[e] != cell[e - 1] || tri[e] != tri[e - 1]) ? 1 : 0;
+}
+__global__ __launch_bounds__(256) void k_pair_compact(const int* pair_n, int pair_cap, const uint32_t* __restrict__ cell,
+                                                       const int* __restrict__ tri, const int* __restrict__ flag,
+                                                       const int* __restrict__ pos, uint32_t* ucell, int* utri) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = min(*pair_n, pair_cap);
+    if (e >= n) return;
+    if (flag[e]) {
+        ucell[pos[e]] = cell[e];
+        utri[pos[e]] = tri[e];
+    }
+}
+// first index and (capped) length of every cell's run in the unique list
+__global__ __launch_bounds__(256) void k_pair_ranges(const int* uniq_n, const uint32_t* __restrict__ ucell, int* cstart,
+                                                      int* ccount) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = *uniq_n;
+    if (e >= n) return;
+    const uint32_t k = ucell[e];
+    if (e == 0 || ucell[e - 1] != k) {
+        int f = e + 1;
+        while (f < n && ucell[f] == k && f - e < kCellMaxTriangles) ++f;
+        cstart[k] = e;
+        ccount[k] = f - e;
+    }
+}
+
+template <typename R> void launch_tri_count_total(Dev<R>& d, hipStream_t st) {
+    if (d.ntri <= 0) {
+        fill_int(d.pair_n, 0, 1, st);
+        return;
+    }
+    hipLaunchKernelGGL(k_tri_count<R>, dim3(grid_for(d.ntri, 64)), dim3(64), 0, st, d.ntri, d.c, d.tris, d.tri_emit);
+    exclusive_scan(d.tri_emit, d.tri_off, nullptr, d.ntri, d.ntri, d.pair_n, d.sort, st);
+}
+
+template <typename R> void launch_tri_bin(Dev<R>& d, hipStream_t st) {
+    fill_int(d.cell_tri_count, 0, d.c.ncell, st);
+    if (d.ntri <= 0) {
+        fill_int(d.uniq_n, 0, 1, st);
+        return;
+    }
+    const int T = d.ntri;
+    hipLaunchKernelGGL(k_tri_count<R>, dim3(grid_for(T, 64)), dim3(64), 0, st, T, d.c, d.tris, d.tri_emit);
+    exclusive_scan(d.tri_emit, d.tri_off, nullptr, T, T, d.pair_n, d.sort, st);
+    hipLaunchKernelGGL(k_tri_fill<R>, dim3(grid_for(T, 64)), dim3(64), 0, st, T, d.c, d.tris, d.tri_off, d.pair_n,
+                       d.pair_cap, d.pair_cell, d.pair_tri, d.overflow);
+    int bits = 1;
+    while ((1ll << bits) < (long long)d.c.ncell) ++bits;
+    const int which = radix_sort_pairs(d.pair_cell, d.pair_tri, d.pair_n, 0, d.pair_cap, bits, d.sort, st);
+    const uint32_t* sc = which ? d.sort.k1 : d.pair_cell;
+    const int* stri = which ? d.sort.v1 : d.pair_tri;
+    const unsigned g = grid_for(d.pair_cap, 256);
+    hipLaunchKernelGGL(k_pair_flags, dim3(g), dim3(256), 0, st, d.pair_n, d.pair_cap, sc, stri, d.pair_flag);
+    exclusive_scan(d.pair_flag, d.pair_pos, d.pair_n, 0, d.pair_cap, d.uniq_n, d.sort, st);
+    hipLaunchKernelGGL(k_pair_compact, dim3(g), dim3(256), 0, st, d.pair_n, d.pair_cap, sc, stri, d.pair_flag,
+                       d.pair_pos, d.uniq_cell, d.uniq_tri);
+    hipLaunchKernelGGL(k_pair_ranges, dim3(g), dim3(256), 0, st, d.uniq_n, d.uniq_cell, d.cell_tri_start,
+                       d.cell_tri_count);
+}
+
+// ---------------------------------------------------------------------------------------------
+// distance to a triangle, Utility_Inline.h:46-170
+// ---------------------------------------------------------------------------------------------
+template <typename R> __device__ inline R segment_ratio(P3<R> p1, P3<R> p2, P3<R> p3_) {
+    const P3<R> dv = psub(p2, p1);
+    R den = dv.x * dv.x + dv.y * dv.y + dv.z * dv.z;
+    const R num = (p3_.x - p1.x) * (p2.x - p1.x) + (p3_.y - p1.y) * (p2.y - p1.y) + (p3_.z - p1.z) * (p2.z - p1.z);
+    if (den == 0.0) den = (R)1e-5;
+    return num / den;
+}
+// CalcDistanceVectorLine: beyond the ends it returns L1 - P (u < 0) and L0 - P (u > 1)
+template <typename R> __device__ inline P3<R> line_vec(P3<R> l0, P3<R> l1, P3<R> p) {
+    const R u = segment_ratio(l0, l1, p);
+    P3<R> d = p3<R>(0, 0, 0);
+    if (u >= 0 && u <= 1) {
+        const P3<R> m = psub(l1, l0);
+        const P3<R> q = p3(l0.x + u * m.x, l0.y + u * m.y, l0.z + u * m.z);
+        d = psub(p, q);
+    }
+    if (u < 0) d = psub(l1, p);
+    if (u > 1) d = psub(l0, p);
+    return d;
+}
+template <typename R> __device__ inline R tri_distance(const TriD<R>& t, P3<R> p, P3<R>* out) {
+    const P3<R> v0 = p3(t.v[0], t.v[1], t.v[2]), v1 = p3(t.v[3], t.v[4], t.v[5]), v2 = p3(t.v[6], t.v[7], t.v[8]);
+    // CalcNormalDistance with the stored normal and the centroid divided by the double 3.0
+    const P3<R> n = p3(t.nrm[0], t.nrm[1], t.nrm[2]);
+    P3<R> cc = p3<R>(0, 0, 0);
+    cc.x += v0.x; cc.y += v0.y; cc.z += v0.z;
+    cc.x += v1.x; cc.y += v1.y; cc.z += v1.z;
+    cc.x += v2.x; cc.y += v2.y; cc.z += v2.z;
+    cc.x = (R)((double)cc.x / 3.0);
+    cc.y = (R)((double)cc.y / 3.0);
+    cc.z = (R)((double)cc.z / 3.0);
+    const R ln = n.x * cc.x + n.y * cc.y + n.z * cc.z;
+    const R rn = n.x * p.x + n.y * p.y + n.z * p.z;
+    const R den = n.x * n.x + n.y * n.y + n.z * n.z;
+    const R sden = build_sqrt(den);
+    P3<R> dv = p3<R>(0, 0, 0);
+    if (!(sden == 0.0)) {
+        const R coef = (rn - ln) / sden;
+        dv.x = coef / sden * n.x;
+        dv.y = coef / sden * n.y;
+        dv.z = coef / sden * n.z;
+    }
+    const P3<R> P = p3(p.x - dv.x, p.y - dv.y, p.z - dv.z);
+    const P3<R> e0 = psub(v2, v0), e1 = psub(v1, v0), e2 = psub(P, v0);
+    const R d00 = pdot(e0, e0), d01 = pdot(e0, e1), d02 = pdot(e0, e2), d11 = pdot(e1, e1), d12 = pdot(e1, e2);
+    const R inv = 1 / (d00 * d11 - d01 * d01);
+    const R u = (d11 * d02 - d01 * d12) * inv;
+    const R v = (d00 * d12 - d01 * d02) * inv;
+    if (u >= 0 && v >= 0 && u + v > 1) dv = line_vec(v1, v2, p);
+    if (u < 0) dv = line_vec(v0, v1, p);
+    if (v < 0) dv = line_vec(v0, v2, p);
+    *out = dv;
+    return pmag(dv);
+}
+
+// nearest triangle of the wanted kind (drag or not) over the 27-cell stencil
+template <typename R>
+__device__ inline int nearest_triangle(const Consts<R>& c, R x, R y, R z, const int* __restrict__ tstart,
+                                       const int* __restrict__ tcount, const int* __restrict__ tids,
+                                       const TriD<R>* __restrict__ tris, int want_drag, R* best, P3<R>* bdir) {
+    *best = (R)1e8;
+    *bdir = p3<R>(0, 0, 0);
+    int bid = -1;
+    int ci, cj, ck;
+    if (!cell_coords(c, x, y, z, ci, cj, ck)) return bid;
+    const long long cid = cell_of(c, ci, cj, ck);
+    if (cid < 0 || cid >= c.ncell) return bid;
+    const P3<R> p = p3(x, y, z);
+    for (int k = -1; k <= 1; ++k)
+        for (int j = -1; j <= 1; ++j)
+            for (int i = -1; i <= 1; ++i) {
+                const long long m = cell_of(c, ci + i, cj + j, ck + k);
+                if (m < 0 || m >= c.ncell) continue;
+                const int n = tcount[m];
+                const int a = tstart[m];
+                for (int e = 0; e < n; ++e) {
+                    const int tid = tids[a + e];
+                    const TriD<R>& tr = tris[tid];
+                    if (tr.is_drag != want_drag) continue;
+                    if (tid != bid) {
+                        P3<R> dv;
+                        const R dm = tri_distance(tr, p, &dv);
+                        if (dm < *best) {
+                            *best = dm;
+                            *bdir = dv;
+                            bid = tid;
+                        }
+                    }
+                }
+            }
+    return bid;
+}
+
+template <typename R> __device__ inline R4<R> mk4w(R x, R y, R z);
+template <> __device__ inline float4 mk4w<float>(float x, float y, float z) { return make_float4(x, y, z, 0.f); }
+template <> __device__ inline double4 mk4w<double>(double x, double y, double z) { return make_double4(x, y, z, 0.0); }
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_stl_distance(int N, Consts<R> c, const R4<R>* __restrict__ spos,
+                                                       const int* __restrict__ tstart, const int* __restrict__ tcount,
+                                                       const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
+                                                       R* stl_mag, R4<R>* stl_dir, int* stl_id) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const R4<R> P = spos[t];
+    R best;
+    P3<R> dir;
+    const int bid = nearest_triangle(c, P.x, P.y, P.z, tstart, tcount, tids, tris, 0, &best, &dir);
+    stl_mag[t] = best;
+    stl_dir[t] = mk4w<R>(dir.x, dir.y, dir.z);
+    stl_id[t] = bid;
+}
+
+// deterministic pow: exp(y log x) in double, rounded once
+template <typename R> __device__ inline R powd(R x, R y) {
+    if (y == R(0)) return R(1);
+    if (x == R(0)) return y > R(0) ? R(0) : (R)INFINITY;
+    return (R)exp((double)y * log((double)x));
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_drag(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                               PState<R> out, R4<R>* __restrict__ spos, R4<R>* __restrict__ svel,
+                                               const int* __restrict__ tstart, const int* __restrict__ tcount,
+                                               const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
+                                               const mpsr_tri_params* __restrict__ models,
+                                               const mpsr_drag_params* __restrict__ drag, R4<R>* drag_acc,
+                                               R* drag_temp) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int s = perm[t];
+    if (out.id[s] < 0) return;
+    const R4<R> P = spos[t];
+    R best;
+    P3<R> dir;
+    const int bid = nearest_triangle(c, P.x, P.y, P.z, tstart, tcount, tids, tris, 1, &best, &dir);
+    drag_acc[t] = mk4w<R>(0, 0, 0);
+    drag_temp[t] = 0;
+    out.type[s] = 0;
+    if (!(bid != -1 && best != (R)1e8)) return;
+    const mpsr_drag_params dp = drag[models[tris[bid].model].DragTriangleID];
+    if (!(best <= (R)dp.CoefficientofInflueneRegion * c.l0)) return;
+    out.type[s] = 2;
+    const R4<R> V = svel[t];
+    if (dp.DragToken == 'E') {
+        const char k1 = dp.InputVelocityToken1;
+        if (k1 == 'V' || k1 == 'B') {
+            const R sv = build_sqrt(V.x * V.x + V.y * V.y + V.z * V.z);
+            R pr = 0;
+            pr += (R)dp.VelocityIndex * powd(sv, (R)dp.VelocityExponential);
+            const TriD<R>& tr = tris[bid];
+            const R kp = (R)dp.VelocityDragPermiabilityConstant;
+            if (kp != 0) drag_acc[t] = mk4w<R>(pr * tr.nrm[0] / kp, pr * tr.nrm[1] / kp, pr * tr.nrm[2] / kp);
+        }
+        if (k1 == 'T' || k1 == 'B') {
+            const R sv = build_sqrt(V.x * V.x + V.y * V.y + V.z * V.z);
+            const R en = (R)dp.TempIndex * powd(sv, (R)dp.TempExponential);
+            const R den = build_sqrt(V.x * V.x + V.y * V.y + V.z * V.z) * c.density * c.cp;
+            R dtmp = 0;
+            if (den != 0) dtmp = (en * (R)dp.TempDragPermiabilityConstant * c.dt * c.thermal_step) / den;
+            drag_temp[t] = dtmp;
+            out.solid[s] = dtmp;
+        }
+    } else if (dp.DragToken == 'C') {
+        if (dp.InputVelocityToken1 == 'V' || dp.InputVelocityToken1 == 'v') {
+            const R cx = (R)dp.ConstantVx, cy = (R)dp.ConstantVy, cz = (R)dp.ConstantVz;
+            out.vx[s] = cx; out.vy[s] = cy; out.vz[s] = cz;
+            in.vx[s] = cx; in.vy[s] = cy; in.vz[s] = cz;
+            svel[t] = mk4w<R>(cx, cy, cz);
+        }
+    } else if (dp.DragToken == 'F') {
+        if (dp.InputVelocityToken1 == 'V' || dp.InputVelocityToken1 == 'v') {
+            const R f = (R)dp.VelocityMagnitueFactor;
+            const R nx = V.x * f, ny = V.y * f, nz = V.z * f;
+            in.vx[s] = nx; in.vy[s] = ny; in.vz[s] = nz;
+            out.vx[s] = out.vx[s] * f; out.vy[s] = out.vy[s] * f; out.vz[s] = out.vz[s] * f;
+            svel[t] = mk4w<R>(nx, ny, nz);
+        }
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_translate(int T, R dt, TriD<R>* tris, const mpsr_tri_params* __restrict__ models) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    TriD<R> tr = tris[t];
+    const mpsr_float3 mv = models[tr.model].Velocity;
+    R vx = (R)mv.x, vy = (R)mv.y, vz = (R)mv.z;
+    vx = vx * dt; vy = vy * dt; vz = vz * dt;
+    for (int k = 0; k < 3; ++k) {
+        tr.v[3 * k] = tr.v[3 * k] + vx;
+        tr.v[3 * k + 1] = tr.v[3 * k + 1] + vy;
+        tr.v[3 * k + 2] = tr.v[3 * k + 2] + vz;
+    }
+    tris[t] = tr;
+}
+
+template <typename R> __global__ __launch_bounds__(256) void k_rotate(int T, TriD<R>* tris, const RotCoef<R>* __restrict__ rot) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    TriD<R> tr = tris[t];
+    const RotCoef<R> r = rot[tr.model];
+    if (!r.active) return;
+    const R a = r.a, b = r.b, cc = r.c, u = r.u, v = r.v, w = r.w, cs = r.cs, sn = r.sn, omc = r.omc;
+    for (int k = 0; k < 3; ++k) {
+        const R x = tr.v[3 * k], y = tr.v[3 * k + 1], z = tr.v[3 * k + 2];
+        tr.v[3 * k] = (a * (v * v + w * w) - (u * (b * v + cc * w - u * x - v * y - w * z))) * omc + (x * cs) +
+                      (((-cc * v) + b * w - w * y + v * z) * sn);
+        tr.v[3 * k + 1] = (b * (u * u + w * w) - (v * (a * u + cc * w - u * x - v * y - w * z))) * omc + (y * cs) +
+                          (((cc * u) - a * w + w * x - u * z) * sn);
+        tr.v[3 * k + 2] = (cc * (u * u + v * v) - (w * (a * u + b * v - u * x - v * y - w * z))) * omc + (z * cs) +
+                          (((-b * u) + a * v - v * x + u * y) * sn);
+    }
+    tris[t] = tr;
+}
+
+template <typename R> void launch_stl_distance(const Dev<R>& d, hipStream_t st) {
+    if (d.N <= 0) return;
+    hipLaunchKernelGGL(k_stl_distance<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.spos,
+                       d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.stl_mag, d.stl_dir, d.stl_id);
+}
+template <typename R> void launch_drag(const Dev<R>& d, hipStream_t st) {
+    if (d.N <= 0) return;
+    hipLaunchKernelGGL(k_drag<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.perm, d.in, d.out, d.spos,
+                       d.svel, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.models, d.drag, d.drag_acc,
+                       d.drag_temp);
+}
+template <typename R> void launch_translate(const Dev<R>& d, hipStream_t st) {
+    if (d.ntri <= 0) return;
+    hipLaunchKernelGGL(k_translate<R>, dim3(grid_for(d.ntri, 256)), dim3(256), 0, st, d.ntri, d.c.dt, d.tris,
+                       d.models);
+}
+template <typename R> void launch_rotate(const Dev<R>& d, hipStream_t st) {
+    if (d.ntri <= 0) return;
+    hipLaunchKernelGGL(k_rotate<R>, dim3(grid_for(d.ntri, 256)), dim3(256), 0, st, d.ntri, d.tris, d.rot);
+}
+
+#define MPSR_INSTANTIATE_W(R)                                               \
+    template void launch_tri_bin<R>(Dev<R>&, hipStream_t);                  \
+    template void launch_tri_count_total<R>(Dev<R>&, hipStream_t);          \
+    template void launch_stl_distance<R>(const Dev<R>&, hipStream_t);       \
+    template void launch_drag<R>(const Dev<R>&, hipStream_t);               \
+    template void launch_translate<R>(const Dev<R>&, hipStream_t);          \
+    template void launch_rotate<R>(const Dev<R>&, hipStream_t);
+MPSR_INSTANTIATE_W(float)
+MPSR_INSTANTIATE_W(double)
+
+}  // namespace mpsr
