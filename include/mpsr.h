@@ -254,6 +254,14 @@ typedef struct mpsr_timings {
     int ppe_refinements;
     double ppe_rel_residual; /* FP64 ||b - A x|| / ||b|| after the last solve */
     double ppe_initial_rel_residual;
+    /* PCG SpMV launches of the last step (profiling on): mean duration from HIP events around
+     * each launch, launch count, and algorithmic bytes per launch (matrix entries and the
+     * row-wise vectors once; neighbour gathers not counted). */
+    double spmv_ms;
+    int spmv_launches;
+    double spmv_bytes;
+    long long ppe_nnz;       /* off-diagonal entries of the last PPE matrix */
+    int ppe_rows;            /* non-Dirichlet rows of the last PPE matrix */
 } mpsr_timings;
 
 typedef struct mpsr_handle mpsr_handle;

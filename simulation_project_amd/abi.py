@@ -128,7 +128,9 @@ class Timings(C.Structure):
                 ("relocate_ms", C.c_double), ("pnd_ms", C.c_double), ("ppe_ms", C.c_double),
                 ("gradient_ms", C.c_double), ("wall_ms", C.c_double), ("ppe_iterations", C.c_int),
                 ("ppe_refinements", C.c_int), ("ppe_rel_residual", C.c_double),
-                ("ppe_initial_rel_residual", C.c_double)]
+                ("ppe_initial_rel_residual", C.c_double), ("spmv_ms", C.c_double),
+                ("spmv_launches", C.c_int), ("spmv_bytes", C.c_double), ("ppe_nnz", C.c_longlong),
+                ("ppe_rows", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -336,3 +336,45 @@ def create_info(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_E
     info.ppe = ppe
     info.device = device
     return info, (tris, models, drag, ps, case)
+
+
+def drag_box(lib, token="E", nx=8, ny=6, nz=8, l0=0.01, seed=DEFAULT_SEED, dt=1e-4):
+    """A small box with a drag plate (an extra two-triangle STL model marked as drag, the
+    reference's 'DE V V' / 'DC V' / 'DF V' configuration lines) crossing the water block.  The
+    block starts moving with a uniform velocity so the drag terms have a speed to act on."""
+    case = small_box(lib, nx=nx, ny=ny, nz=nz, l0=l0, seed=seed, dt=dt)
+    x0, x1 = 0.0, (nx + 6) * l0
+    zp = (0.5 * nz + 3.3) * l0
+    plate = np.asarray([[[x0, 0.0, zp], [x1, 0.0, zp], [x1, (ny + 4) * l0, zp]],
+                        [[x0, 0.0, zp], [x1, (ny + 4) * l0, zp], [x0, (ny + 4) * l0, zp]]], np.float32)
+    case.vertices = np.concatenate([case.vertices, plate])
+    case.model_of_triangle = np.concatenate([case.model_of_triangle, np.array([1, 1], np.int32)])
+    case.models = [case.models[0], tri_params(is_drag=True, drag_id=0)]
+    d = abi.DragParams()
+    d.DragID = 0
+    d.DragToken = token.encode()
+    d.InputVelocityToken1 = b"V"
+    d.OutputVelocityToken2 = b"V"
+    d.CoefficientofInflueneRegion = 1.5
+    if token == "E":
+        d.VelocityIndex = 0.0007
+        d.VelocityExponential = 1.5061
+        d.VelocityDragPermiabilityConstant = -0.75
+    elif token == "B":
+        d.DragToken = b"E"
+        d.InputVelocityToken1 = b"B"
+        d.OutputVelocityToken2 = b"B"
+        d.VelocityIndex = 0.0007
+        d.VelocityExponential = 1.5061
+        d.VelocityDragPermiabilityConstant = -0.75
+        d.TempIndex = 0.0005
+        d.TempExponential = 0.6668
+        d.TempDragPermiabilityConstant = -55.0
+    elif token == "C":
+        d.ConstantVx, d.ConstantVy, d.ConstantVz = 0.01, 0.0, -0.02
+    elif token == "F":
+        d.VelocityMagnitueFactor = 0.9
+    case.drag = [d]
+    case.particles.velocity[:] = np.array([0.05, -0.02, 0.03], np.float32).astype(np.float64)
+    case.name = f"drag_box_{token}"
+    return _finish(case, lib)

@@ -423,7 +423,7 @@ template <typename R> struct Engine : EngineBase {
             launch_pnd(d, 0, st);
             mark_end();
             mark_begin(PH_PPE);
-            mpsr_status s = ppe_solve(ppe, d, ppe_opt, st);
+            mpsr_status s = ppe_solve(ppe, d, ppe_opt, profiling, st);
             mark_end();
             return s;
         }
@@ -527,6 +527,11 @@ template <typename R> struct Engine : EngineBase {
         t.ppe_refinements = tm.ppe_refinements;
         t.ppe_rel_residual = tm.ppe_rel_residual;
         t.ppe_initial_rel_residual = tm.ppe_initial_rel_residual;
+        t.spmv_ms = tm.spmv_ms;
+        t.spmv_launches = tm.spmv_launches;
+        t.spmv_bytes = tm.spmv_bytes;
+        t.ppe_nnz = tm.ppe_nnz;
+        t.ppe_rows = tm.ppe_rows;
         tm = t;
     }
 
@@ -550,7 +555,7 @@ template <typename R> struct Engine : EngineBase {
                     if (!ppe) { s = fail(MPSR_MEMERR, "PPE solver allocation failed"); break; }
                 }
                 launch_pnd(d, 0, st);
-                s = ppe_solve(ppe, d, ppe_opt, st);
+                s = ppe_solve(ppe, d, ppe_opt, 0, st);
                 break;
             case 8: do_gradient(); break;
             case 9: launch_translate(d, st); break;

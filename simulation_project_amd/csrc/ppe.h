@@ -12,7 +12,8 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
 template <typename R> void ppe_destroy(PpeSolver<R>* s);
 // Assembles A p = b from the current binning and PND (out.n) and solves it; writes the
 // pressure to in.p / out.p and to the sorted pressure array used by the gradient kernel.
-template <typename R> mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, hipStream_t st);
+template <typename R>
+mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, int profile, hipStream_t st);
 // Iteration counts and residuals of the last solve (synchronises); error when it failed.
 template <typename R> mpsr_status ppe_check(PpeSolver<R>* s, mpsr_timings* tm);
 template <typename R> mpsr_status ppe_get(PpeSolver<R>* s, const Dev<R>& d, double* rhs, double* x, int* dir);

@@ -76,7 +76,16 @@ template <typename R> struct PpeSolver {
     // last solve
     int iters = 0, passes = 0, status = 0;
     double relres = 0, relres0 = 0;
+    int* nnz_dev = nullptr;      // [2]: off-diagonal entries, non-Dirichlet rows
     long long nnz_real = 0;
+    int rows_real = 0;
+    // SpMV timing (profiling)
+    int profile = 0;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    int spmv_launches = 0;
+    double spmv_ms = 0.0;
+    double spmv_bytes = 0.0;
 };
 
 template <int NW> __device__ inline double block_sum(double v, double* lds) {
@@ -115,7 +124,7 @@ template <typename R>
 __global__ __launch_bounds__(PB) void k_ppe_count(int N, int nslices, Consts<R> c, const int* __restrict__ active,
                                                    const R4<R>* __restrict__ spos, const int* __restrict__ cs,
                                                    const int* __restrict__ ce, int* cnt, double* diag, int* dir,
-                                                   int* slice_w) {
+                                                   int* slice_w, int* nnz) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int n = 0;
     double sum = 0.0;
@@ -144,8 +153,14 @@ __global__ __launch_bounds__(PB) void k_ppe_count(int N, int nslices, Consts<R> 
     int m = n;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    const int tot = wave_sum(n);
+    const int rows = wave_sum((t < N && !dr) ? 1 : 0);
     const int slice = t >> 6;
-    if (lane_id() == 0 && slice < nslices) slice_w[slice] = m * 64;
+    if (lane_id() == 0 && slice < nslices) {
+        slice_w[slice] = m * 64;
+        atomicAdd(&nnz[0], tot);
+        atomicAdd(&nnz[1], rows);
+    }
 }
 
 template <typename R, typename VA>
@@ -420,6 +435,7 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     s->scan.scan_tmp_cap = (int)scan_tmp_ints(s->nslices + 1);
     ok &= dev_alloc(&s->scan.scan_tmp, s->scan.scan_tmp_cap);
     ok &= hipHostMalloc((void**)&s->scal_h, sizeof(double) * S_COUNT) == hipSuccess;
+    ok &= dev_alloc(&s->nnz_dev, 2);
     if (!ok) {
         ppe_destroy(s);
         return nullptr;
@@ -434,10 +450,11 @@ template <typename R> void ppe_destroy(PpeSolver<R>* s) {
     if (!s) return;
     void* ps[] = {s->active, s->dir, s->cnt, s->slice_w, s->slice_off, s->total, s->cols, s->vals64, s->vals32,
                   s->diag, s->rhs, s->x, s->prev, s->r, s->z, s->p0, s->p1, s->q, s->rf, s->zf, s->p0f,
-                  s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp};
+                  s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp, s->nnz_dev};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (s->scal_h) (void)hipHostFree(s->scal_h);
+    for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     delete s;
 }
 
@@ -471,17 +488,29 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_START);
     int done_iters = 0;
     for (int it = 0; it < maxit; it += CHUNK) {
+        // SpMV launches are bracketed by events while the pool lasts; after the chunk, the
+        // events of launches that found the solve already converged (and returned at once) are
+        // dropped, so the mean covers working launches only
+        const size_t chunk_ev0 = s->ev_used;
+        const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
         for (int k = 0; k < CHUNK; ++k) {
             V* pold = (k & 1) ? pb : pa;
             V* pnew = (k & 1) ? pa : pb;
+            if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
             hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, N, s->slice_off, s->slice_w, s->cols,
                                vals, s->diag, z, pold, pnew, q, s->scal, s->part);
+            if (timed) {
+                (void)hipEventRecord(s->ev[s->ev_used + 1], st);
+                s->ev_used += 2;
+            }
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 1, s->scal, (int)RED_ALPHA);
             hipLaunchKernelGGL(k_pcg_update<V>, dim3(nb), dim3(PB), 0, st, N, x, r, z, pnew, q, s->diag, s->scal,
                                s->part);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_BETA);
         }
         if (read_scal(s, st) != hipSuccess) return -1;
+        const int live = std::max(0, std::min(CHUNK, (int)s->scal_h[S_ITERS] - done_iters));
+        if (timed) s->ev_used = chunk_ev0 + 2 * (size_t)live;
         done_iters = (int)s->scal_h[S_ITERS];
         if (s->scal_h[S_DONE] != 0.0) break;
     }
@@ -504,8 +533,16 @@ template <typename R> static double resid_pass(PpeSolver<R>* s, int N, hipStream
     return s->scal_h[S_RR];
 }
 
-template <typename R> mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, hipStream_t st) {
+template <typename R>
+mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, int profile, hipStream_t st) {
     const int N = d.N;
+    s->profile = profile;
+    s->ev_used = 0;
+    if (profile && s->ev.empty()) {
+        s->ev.resize(2 * 4096);
+        for (auto& e : s->ev)
+            if (hipEventCreate(&e) != hipSuccess) return MPSR_GPUERR;
+    }
     const int nsl = (N + 63) / 64;
     const int rows = nsl * 64;
     const int nb = (rows + PB - 1) / PB;
@@ -514,14 +551,21 @@ template <typename R> mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mp
     // ---- assembly ----
     hipLaunchKernelGGL(k_ppe_active<R>, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.c, d.skeys, d.perm, d.out,
                        d.spos, s->active);
+    fill_int(s->nnz_dev, 0, 2, st);
     hipLaunchKernelGGL(k_ppe_count<R>, dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, d.spos, d.cell_start,
-                       d.cell_end, s->cnt, s->diag, s->dir, s->slice_w);
+                       d.cell_end, s->cnt, s->diag, s->dir, s->slice_w, s->nnz_dev);
     exclusive_scan(s->slice_w, s->slice_off, nullptr, nsl, nsl, s->total, s->scan, st);
     int total = 0;
+    int counts[2] = {0, 0};
     if (hipMemcpyAsync(&s->scal_h[0], s->total, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return MPSR_GPUERR;
+    if (hipMemcpyAsync(&s->scal_h[1], s->nnz_dev, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
         return MPSR_GPUERR;
     if (hipStreamSynchronize(st) != hipSuccess) return MPSR_GPUERR;
     std::memcpy(&total, &s->scal_h[0], sizeof(int));
+    std::memcpy(counts, &s->scal_h[1], 2 * sizeof(int));
+    s->nnz_real = counts[0];
+    s->rows_real = counts[1];
     if ((long long)total > s->nnz_cap) {
         if (s->cols) (void)hipFree(s->cols);
         if (s->vals64) (void)hipFree(s->vals64);
@@ -583,6 +627,28 @@ template <typename R> mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mp
     }
     s->relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     if (!std::isfinite(s->relres)) s->status = 1;
+    // SpMV launch statistics (events already complete: resid_pass synchronised the stream)
+    s->spmv_launches = 0;
+    s->spmv_ms = 0.0;
+    if (profile) {
+        double tot_ms = 0.0;
+        int cnt = 0;
+        for (size_t e = 0; e + 1 < s->ev_used; e += 2) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, s->ev[e], s->ev[e + 1]) == hipSuccess) {
+                tot_ms += ms;
+                ++cnt;
+            }
+        }
+        s->spmv_launches = cnt;
+        s->spmv_ms = cnt ? tot_ms / cnt : 0.0;
+    }
+    {
+        // algorithmic bytes of one SpMV launch: every stored entry's value and column once, and
+        // per row z, p_old, diag read and p_new, q written; gathers of z/p at columns not counted
+        const double sv = s->mixed ? 4.0 : 8.0;
+        s->spmv_bytes = (double)s->nnz_real * (sv + 4.0) + (double)N * (4.0 * sv + 8.0);
+    }
     hipLaunchKernelGGL(k_ppe_writeback<R>, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.c, d.perm, s->dir, s->x, d.in,
                        d.out, d.spos, d.spres, s->prev);
     s->have_prev = 1;
@@ -590,6 +656,11 @@ template <typename R> mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mp
 }
 
 template <typename R> mpsr_status ppe_check(PpeSolver<R>* s, mpsr_timings* tm) {
+    tm->spmv_ms = s->spmv_ms;
+    tm->spmv_launches = s->spmv_launches;
+    tm->spmv_bytes = s->spmv_bytes;
+    tm->ppe_nnz = s->nnz_real;
+    tm->ppe_rows = s->rows_real;
     tm->ppe_iterations = s->iters;
     tm->ppe_refinements = s->passes;
     tm->ppe_rel_residual = s->relres;
@@ -670,7 +741,7 @@ namespace mpsr {
     template PpeSolver<R>* ppe_create<R>(const Dev<R>&, const mpsr_params&, const mpsr_ppe_options&, double, double, \
                                          double);                                                                    \
     template void ppe_destroy<R>(PpeSolver<R>*);                                                                     \
-    template mpsr_status ppe_solve<R>(PpeSolver<R>*, Dev<R>&, const mpsr_ppe_options&, hipStream_t);                 \
+    template mpsr_status ppe_solve<R>(PpeSolver<R>*, Dev<R>&, const mpsr_ppe_options&, int, hipStream_t);            \
     template mpsr_status ppe_check<R>(PpeSolver<R>*, mpsr_timings*);                                                 \
     template mpsr_status ppe_get<R>(PpeSolver<R>*, const Dev<R>&, double*, double*, int*);                           \
     template void ppe_update_params<R>(PpeSolver<R>*, const mpsr_params&);
