@@ -143,37 +143,49 @@ def box_triangles(lo, hi):
     return np.asarray(tris, np.float32)
 
 
-def cylinder_triangles(radius, length, center_yz, n_theta=100, n_x=31, n_r=10):
-    """Closed cylinder along x from 0 to length: side rings plus ring-subdivided end caps.
-    Defaults give 100*31*2 + 2*100*(2*10-1) = 10000 triangles."""
-    yc, zc = center_yz
-    th = np.linspace(0.0, 2.0 * math.pi, n_theta + 1)[:-1]
-    ring = np.stack([np.cos(th), np.sin(th)], axis=1)
-    xs = np.linspace(0.0, length, n_x + 1)
+def _zip_rings(inner, outer):
+    """Triangles between two closed rings of points (angles ascending from 0)."""
+    ma, mb = len(inner), len(outer)
     tris = []
+    i = j = 0
+    while i < ma or j < mb:
+        a_next = 2.0 * math.pi * (i + 1) / ma
+        b_next = 2.0 * math.pi * (j + 1) / mb
+        if j == mb or (i < ma and a_next < b_next):
+            tris.append([inner[i % ma], outer[j % mb], inner[(i + 1) % ma]])
+            i += 1
+        else:
+            tris.append([inner[i % ma], outer[j % mb], outer[(j + 1) % mb]])
+            j += 1
+    return tris
+
+
+def cylinder_triangles(radius, length, center_yz, n_r=16, n_x=36):
+    """Closed cylinder along x from 0 to length with near-uniform triangles: each end cap is a
+    hexagonal disk (ring j of n_r carries 6 j vertices, 6 n_r^2 triangles per cap) and the mantle
+    has 6 n_r segments around and n_x along.  Defaults: 2*1536 + 6912 = 9984 triangles with
+    edges of about R / n_r (9.4 mm for R = 0.15 m)."""
+    yc, zc = center_yz
+    n_theta = 6 * n_r
+
+    def ring(x, r, m):
+        th = 2.0 * math.pi * np.arange(m) / m
+        return [(x, yc + r * math.cos(t), zc + r * math.sin(t)) for t in th]
+    tris = []
+    xs = np.linspace(0.0, length, n_x + 1)
     for i in range(n_x):
+        a, b = ring(xs[i], radius, n_theta), ring(xs[i + 1], radius, n_theta)
         for k in range(n_theta):
             k2 = (k + 1) % n_theta
-            p00 = (xs[i], yc + radius * ring[k, 0], zc + radius * ring[k, 1])
-            p01 = (xs[i], yc + radius * ring[k2, 0], zc + radius * ring[k2, 1])
-            p10 = (xs[i + 1], yc + radius * ring[k, 0], zc + radius * ring[k, 1])
-            p11 = (xs[i + 1], yc + radius * ring[k2, 0], zc + radius * ring[k2, 1])
-            tris.append([p00, p10, p11])
-            tris.append([p00, p11, p01])
-    rs = np.linspace(0.0, radius, n_r + 1)
+            tris.append([a[k], b[k], b[k2]])
+            tris.append([a[k], b[k2], a[k2]])
     for x in (0.0, length):
-        for j in range(n_r):
-            for k in range(n_theta):
-                k2 = (k + 1) % n_theta
-                a0 = (x, yc + rs[j] * ring[k, 0], zc + rs[j] * ring[k, 1])
-                a1 = (x, yc + rs[j] * ring[k2, 0], zc + rs[j] * ring[k2, 1])
-                b0 = (x, yc + rs[j + 1] * ring[k, 0], zc + rs[j + 1] * ring[k, 1])
-                b1 = (x, yc + rs[j + 1] * ring[k2, 0], zc + rs[j + 1] * ring[k2, 1])
-                if j == 0:
-                    tris.append([a0, b0, b1])
-                else:
-                    tris.append([a0, b0, b1])
-                    tris.append([a0, b1, a1])
+        centre = (x, yc, zc)
+        r1 = ring(x, radius / n_r, 6)
+        for k in range(6):
+            tris.append([centre, r1[k], r1[(k + 1) % 6]])
+        for j in range(2, n_r + 1):
+            tris.extend(_zip_rings(ring(x, radius * (j - 1) / n_r, 6 * (j - 1)), ring(x, radius * j / n_r, 6 * j)))
     return np.asarray(tris, np.float32)
 
 
@@ -246,8 +258,8 @@ def dam_break(lib, l0=0.01, seed=DEFAULT_SEED, explicit_pressure=True, steps=100
 
 
 def drum(lib, l0=2.5e-3, fill=0.4, rpm=30, radius=0.15, length=0.3, seed=DEFAULT_SEED, dt=5e-5,
-         n_theta=100, n_x=31, n_r=10, steps=200, explicit_pressure=False):
-    """Rotating drum: cylinder of radius 0.15 m and length 0.3 m along x (10k triangles by
+         n_r=16, n_x=36, steps=200, explicit_pressure=False):
+    """Rotating drum: cylinder of radius 0.15 m and length 0.3 m along x (9984 triangles by
     default) spinning at rpm about its axis, filled to the given area fraction.  Lattice sites
     are l0 apart, at l0, 2 l0, ... from the end caps and at least l0 inside the mantle.  l0 = 2.5e-3 m and l0 = 2.039e-3 m are the two sizes
     quoted for this case (about 0.54M and 1M particles)."""
@@ -279,7 +291,7 @@ def drum(lib, l0=2.5e-3, fill=0.4, rpm=30, radius=0.15, length=0.3, seed=DEFAULT
     p = default_params(l0, dt, steps=steps, explicit_pressure=explicit_pressure,
                        artificial=0.01 if explicit_pressure else 0.0,
                        r_coeff=0.7 if explicit_pressure else 0.0)
-    verts = cylinder_triangles(radius, length, (yc, zc), n_theta, n_x, n_r)
+    verts = cylinder_triangles(radius, length, (yc, zc), n_r, n_x)
     m = tri_params(rotating=True, rpm=rpm, dt=dt, center=(0.0, yc, zc), second=(length, yc, zc),
                    friction=1.0)
     case = Case("drum", p, _particles(pos, 1e-6), verts, np.zeros(len(verts), np.int32), [m],

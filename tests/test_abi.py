@@ -40,7 +40,7 @@ def test_missing_library_raises():
 
 @pytest.mark.parametrize("make", ["dam_break", "drum_small"])
 def test_host_helpers_match_oracle(lib, orc, make):
-    case = cases.dam_break(lib) if make == "dam_break" else cases.drum(lib, l0=0.01, n_theta=40, n_x=10, n_r=4)
+    case = cases.dam_break(lib) if make == "dam_break" else cases.drum(lib, l0=0.01, n_r=6, n_x=10)
     z1, z2 = abi.Box(), abi.Box()
     ps = case.particles.struct()
     tris = case.triangles()

@@ -324,7 +324,7 @@ def test_triangle_distance_face_and_endpoint_swap(orc):
 
 
 def test_triangle_binning_invariants(lib):
-    case = cases.drum(lib, l0=0.01, n_theta=40, n_x=10, n_r=4)
+    case = cases.drum(lib, l0=0.01, n_r=6, n_x=10)
     o = Oracle(case)
     o.RegisterTriangleTopology()
     cs, cc, ids = o.triangle_cells()
