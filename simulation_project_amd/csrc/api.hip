@@ -224,13 +224,13 @@ template <typename R> struct Engine : EngineBase {
         ALLOC(s.p, cap); ALLOC(s.n, cap); ALLOC(s.temp, cap); ALLOC(s.nu, cap); ALLOC(s.solid, cap);
         return MPSR_NOERR;
     }
-    mpsr_status alloc_sort(int cap) {
-        ALLOC(d.sort.k1, cap);
-        ALLOC(d.sort.v1, cap);
-        d.sort.hist_cap = (int)radix_hist_ints(cap);
-        ALLOC(d.sort.hist, d.sort.hist_cap);
-        d.sort.scan_tmp_cap = (int)scan_tmp_ints(cap);
-        ALLOC(d.sort.scan_tmp, d.sort.scan_tmp_cap);
+    mpsr_status alloc_sort(SortScratch& ss, int cap) {
+        ALLOC(ss.k1, cap);
+        ALLOC(ss.v1, cap);
+        ss.hist_cap = (int)radix_hist_ints(cap);
+        ALLOC(ss.hist, ss.hist_cap);
+        ss.scan_tmp_cap = (int)scan_tmp_ints(cap);
+        ALLOC(ss.scan_tmp, ss.scan_tmp_cap);
         return MPSR_NOERR;
     }
 
@@ -337,7 +337,7 @@ template <typename R> struct Engine : EngineBase {
         int pair_total = 0;
         if (d.ntri > 0) {
             // count pass alone, with a scan scratch sized for the triangles
-            if (alloc_sort(std::max(N, d.ntri) + 1) != MPSR_NOERR) return MPSR_GPUERR;
+            if (alloc_sort(d.tsort, d.ntri + 1) != MPSR_NOERR) return MPSR_GPUERR;
             launch_tri_count_total(d, st);
             HIPC(hipStreamSynchronize(st));
             HIPC(hipMemcpy(&pair_total, d.pair_n, sizeof(int), hipMemcpyDeviceToHost));
@@ -346,7 +346,8 @@ template <typename R> struct Engine : EngineBase {
         ALLOC(d.pair_cell, d.pair_cap); ALLOC(d.pair_tri, d.pair_cap);
         ALLOC(d.pair_flag, d.pair_cap); ALLOC(d.pair_pos, d.pair_cap);
         ALLOC(d.uniq_cell, d.pair_cap); ALLOC(d.uniq_tri, d.pair_cap);
-        if (alloc_sort(std::max(cap, d.pair_cap)) != MPSR_NOERR) return MPSR_GPUERR;
+        if (alloc_sort(d.sort, cap) != MPSR_NOERR) return MPSR_GPUERR;
+        if (alloc_sort(d.tsort, d.pair_cap) != MPSR_NOERR) return MPSR_GPUERR;
         // rotation coefficients table
         rot_steps = 64;
         ALLOC(rot_table, (size_t)rot_steps * std::max(1, d.nmodel));

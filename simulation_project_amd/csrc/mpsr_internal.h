@@ -245,7 +245,11 @@ template <typename R> struct Dev {
     int* stl_id = nullptr;
     R4<R>* drag_acc = nullptr;
     R* drag_temp = nullptr;
+    // radix-sort ping-pong buffers: one set for the particle keys (the sorted keys and the
+    // permutation may end up in them and must live until the next binning), one for the
+    // triangle pairs
     SortScratch sort{};
+    SortScratch tsort{};
 };
 
 // particles.hip

@@ -140,7 +140,7 @@ template <typename R> void launch_tri_count_total(Dev<R>& d, hipStream_t st) {
         return;
     }
     hipLaunchKernelGGL(k_tri_count<R>, dim3(grid_for(d.ntri, 64)), dim3(64), 0, st, d.ntri, d.c, d.tris, d.tri_emit);
-    exclusive_scan(d.tri_emit, d.tri_off, nullptr, d.ntri, d.ntri, d.pair_n, d.sort, st);
+    exclusive_scan(d.tri_emit, d.tri_off, nullptr, d.ntri, d.ntri, d.pair_n, d.tsort, st);
 }
 
 template <typename R> void launch_tri_bin(Dev<R>& d, hipStream_t st) {
@@ -151,17 +151,17 @@ template <typename R> void launch_tri_bin(Dev<R>& d, hipStream_t st) {
     }
     const int T = d.ntri;
     hipLaunchKernelGGL(k_tri_count<R>, dim3(grid_for(T, 64)), dim3(64), 0, st, T, d.c, d.tris, d.tri_emit);
-    exclusive_scan(d.tri_emit, d.tri_off, nullptr, T, T, d.pair_n, d.sort, st);
+    exclusive_scan(d.tri_emit, d.tri_off, nullptr, T, T, d.pair_n, d.tsort, st);
     hipLaunchKernelGGL(k_tri_fill<R>, dim3(grid_for(T, 64)), dim3(64), 0, st, T, d.c, d.tris, d.tri_off, d.pair_n,
                        d.pair_cap, d.pair_cell, d.pair_tri, d.overflow);
     int bits = 1;
     while ((1ll << bits) < (long long)d.c.ncell) ++bits;
-    const int which = radix_sort_pairs(d.pair_cell, d.pair_tri, d.pair_n, 0, d.pair_cap, bits, d.sort, st);
-    const uint32_t* sc = which ? d.sort.k1 : d.pair_cell;
-    const int* stri = which ? d.sort.v1 : d.pair_tri;
+    const int which = radix_sort_pairs(d.pair_cell, d.pair_tri, d.pair_n, 0, d.pair_cap, bits, d.tsort, st);
+    const uint32_t* sc = which ? d.tsort.k1 : d.pair_cell;
+    const int* stri = which ? d.tsort.v1 : d.pair_tri;
     const unsigned g = grid_for(d.pair_cap, 256);
     hipLaunchKernelGGL(k_pair_flags, dim3(g), dim3(256), 0, st, d.pair_n, d.pair_cap, sc, stri, d.pair_flag);
-    exclusive_scan(d.pair_flag, d.pair_pos, d.pair_n, 0, d.pair_cap, d.uniq_n, d.sort, st);
+    exclusive_scan(d.pair_flag, d.pair_pos, d.pair_n, 0, d.pair_cap, d.uniq_n, d.tsort, st);
     hipLaunchKernelGGL(k_pair_compact, dim3(g), dim3(256), 0, st, d.pair_n, d.pair_cap, sc, stri, d.pair_flag,
                        d.pair_pos, d.uniq_cell, d.uniq_tri);
     hipLaunchKernelGGL(k_pair_ranges, dim3(g), dim3(256), 0, st, d.uniq_n, d.uniq_cell, d.cell_tri_start,

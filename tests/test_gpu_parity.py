@@ -177,3 +177,23 @@ def test_large_drum_binning_properties(lib):
     g.step_many(0, 2)
     p = g.get_particles(0)
     assert np.isfinite(p.position).all() and (p.id >= 0).all()
+
+
+def test_large_drum_identical(lib):
+    # 10k triangles and 17-bit cell keys (three radix passes): the wall pairs and the particle
+    # permutation both go through the multi-tile, odd-pass sort path
+    case = cases.drum(lib, l0=2.5e-3)
+    g, o = pair(case)
+    assert g.cells > 65536
+    run_phase("RegisterParticleTopology", g, o)
+    run_phase("RegisterTriangleTopology", g, o)
+    for a, b in zip(g.triangle_cells(), o.triangle_cells()):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(g.binning(), o.binning()):
+        np.testing.assert_array_equal(a, b)
+    run_phase("CalcSTLDistance", g, o)
+    for a, b in zip(g.stl_distance(), o.stl_distance()):
+        np.testing.assert_array_equal(a, b)
+    g.step_many(0, 3)
+    o.step_many(0, 3)
+    same_state(g, o, label="drum 3 steps")
