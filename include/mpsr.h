@@ -262,6 +262,7 @@ typedef struct mpsr_timings {
     double spmv_bytes;
     long long ppe_nnz;       /* off-diagonal entries of the last PPE matrix */
     int ppe_rows;            /* non-Dirichlet rows of the last PPE matrix */
+    int ppe_fallbacks;       /* mixed solves that finished with the plain FP64 PCG */
 } mpsr_timings;
 
 typedef struct mpsr_handle mpsr_handle;

@@ -499,7 +499,7 @@ template <typename R> struct Engine : EngineBase {
         if (ovf) return fail(MPSR_MEMERR, "triangle/cell pair buffer overflow: walls grew beyond the set-up capacity");
         if (ppe) {
             mpsr_status ps = ppe_check(ppe, &tm);
-            if (ps != MPSR_NOERR) return fail(ps, "PPE solve did not converge");
+            if (ps != MPSR_NOERR) return fail(ps, std::string("PPE solve failed: ") + ppe_why(ppe));
         }
         return MPSR_NOERR;
     }
@@ -533,6 +533,7 @@ template <typename R> struct Engine : EngineBase {
         t.spmv_bytes = tm.spmv_bytes;
         t.ppe_nnz = tm.ppe_nnz;
         t.ppe_rows = tm.ppe_rows;
+        t.ppe_fallbacks = tm.ppe_fallbacks;
         tm = t;
     }
 
@@ -572,7 +573,7 @@ template <typename R> struct Engine : EngineBase {
         HIPC(hipStreamSynchronize(st));
         if (which == 7 && ppe) {
             mpsr_status ps = ppe_check(ppe, &tm);
-            if (ps != MPSR_NOERR) return fail(ps, "PPE solve did not converge");
+            if (ps != MPSR_NOERR) return fail(ps, std::string("PPE solve failed: ") + ppe_why(ppe));
         }
         return MPSR_NOERR;
     }

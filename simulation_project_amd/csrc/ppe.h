@@ -16,6 +16,7 @@ template <typename R>
 mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, int profile, hipStream_t st);
 // Iteration counts and residuals of the last solve (synchronises); error when it failed.
 template <typename R> mpsr_status ppe_check(PpeSolver<R>* s, mpsr_timings* tm);
+template <typename R> const char* ppe_why(PpeSolver<R>* s);
 template <typename R> mpsr_status ppe_get(PpeSolver<R>* s, const Dev<R>& d, double* rhs, double* x, int* dir);
 // The assembled matrix in slot order: row pointers [n+1], column slots, off-diagonal values,
 // diagonal [n] (Dirichlet rows: no entries, diagonal 1).

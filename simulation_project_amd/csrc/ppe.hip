@@ -74,8 +74,9 @@ template <typename R> struct PpeSolver {
     int mixed = 0;
     SortScratch scan{};
     // last solve
-    int iters = 0, passes = 0, status = 0;
+    int iters = 0, passes = 0, status = 0, fallbacks = 0;
     double relres = 0, relres0 = 0;
+    char why[256] = {0};         // what went wrong in the last solve
     int* nnz_dev = nullptr;      // [2]: off-diagonal entries, non-Dirichlet rows
     long long nnz_real = 0;
     int rows_real = 0;
@@ -508,13 +509,21 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
                                s->part);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_BETA);
         }
-        if (read_scal(s, st) != hipSuccess) return -1;
+        if (read_scal(s, st) != hipSuccess) {
+            std::snprintf(s->why, sizeof(s->why), "device error while reading the PCG scalars");
+            return -1;
+        }
         const int live = std::max(0, std::min(CHUNK, (int)s->scal_h[S_ITERS] - done_iters));
         if (timed) s->ev_used = chunk_ev0 + 2 * (size_t)live;
         done_iters = (int)s->scal_h[S_ITERS];
         if (s->scal_h[S_DONE] != 0.0) break;
     }
-    if (s->scal_h[S_DONE] == 3.0) return -1;
+    if (s->scal_h[S_DONE] == 3.0) {
+        std::snprintf(s->why, sizeof(s->why),
+                      "PCG breakdown after %d iterations (p.Ap = %.3e, r.z = %.3e, r.r = %.3e, target %.3e)",
+                      done_iters, s->scal_h[S_PQ], s->scal_h[S_RZ], s->scal_h[S_RR], s->scal_h[S_TARGET]);
+        return -1;
+    }
     return done_iters;
 }
 
@@ -597,6 +606,8 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     const double target = tol * tol * bb;
     s->iters = 0;
     s->passes = 0;
+    s->fallbacks = 0;
+    s->why[0] = 0;
     if (!s->mixed) {
         if (bb > 0 && rr > target) {
             // FP64 Jacobi-PCG from the warm start; r already holds b - A x
@@ -606,27 +617,49 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
             else s->iters = it;
             s->passes = 1;
             rr = resid_pass(s, N, st, &bb);
+            if (rr < 0) return MPSR_GPUERR;
         }
     } else {
         // iterative refinement: FP64 residual, FP32 inner PCG with FP64 dots, FP64 update
-        while (bb > 0 && rr > target && s->passes < 16 && s->iters < maxit && s->status == 0) {
+        bool fallback = false;
+        while (bb > 0 && rr > target && s->passes < 16 && s->iters < maxit) {
             const double inner_rel = std::max(1e-6, 0.5 * tol * std::sqrt(bb / rr));
             const int it = run_pcg<R, float, float>(s, N, s->vals32, s->ef, s->rf, s->zf, s->p0f, s->p1f, s->qf, s->r,
                                                     inner_rel * inner_rel * rr, maxit - s->iters, st);
             if (it < 0) {
-                s->status = 1;
+                fallback = true;
                 break;
             }
             hipLaunchKernelGGL(k_add_f2d, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, s->x, s->ef);
             s->iters += it;
             s->passes += 1;
+            const double prev_rr = rr;
             rr = resid_pass(s, N, st, &bb);
             if (rr < 0) return MPSR_GPUERR;
-            if (it == 0) break;
+            if (it == 0 || !(rr < 0.5 * prev_rr)) {
+                // the FP32 inner solve no longer reduces the FP64 residual
+                fallback = rr > target;
+                break;
+            }
+        }
+        if (fallback && bb > 0 && rr > target) {
+            // plain PCG in FP64 vectors on the same (FP32-stored) matrix from the current x;
+            // r holds b - A x from the last residual pass
+            s->fallbacks += 1;
+            const int it = run_pcg<R, float, double>(s, N, s->vals32, s->x, s->r, s->z, s->p0, s->p1, s->q, s->r,
+                                                     target, std::max(1, maxit - s->iters), st);
+            if (it < 0) s->status = 1;
+            else s->iters += it;
+            s->passes += 1;
+            rr = resid_pass(s, N, st, &bb);
+            if (rr < 0) return MPSR_GPUERR;
         }
     }
     s->relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
-    if (!std::isfinite(s->relres)) s->status = 1;
+    if (!std::isfinite(s->relres)) {
+        s->status = 1;
+        std::snprintf(s->why, sizeof(s->why), "non-finite residual (r.r = %.3e, b.b = %.3e)", rr, bb);
+    }
     // SpMV launch statistics (events already complete: resid_pass synchronised the stream)
     s->spmv_launches = 0;
     s->spmv_ms = 0.0;
@@ -665,8 +698,11 @@ template <typename R> mpsr_status ppe_check(PpeSolver<R>* s, mpsr_timings* tm) {
     tm->ppe_refinements = s->passes;
     tm->ppe_rel_residual = s->relres;
     tm->ppe_initial_rel_residual = s->relres0;
+    tm->ppe_fallbacks = s->fallbacks;
     return s->status == 0 ? MPSR_NOERR : MPSR_CALCERR;
 }
+
+template <typename R> const char* ppe_why(PpeSolver<R>* s) { return s->why; }
 
 template <typename R> mpsr_status ppe_get(PpeSolver<R>* s, const Dev<R>& d, double* rhs, double* x, int* dir) {
     const int N = d.N;
@@ -743,6 +779,7 @@ namespace mpsr {
     template void ppe_destroy<R>(PpeSolver<R>*);                                                                     \
     template mpsr_status ppe_solve<R>(PpeSolver<R>*, Dev<R>&, const mpsr_ppe_options&, int, hipStream_t);            \
     template mpsr_status ppe_check<R>(PpeSolver<R>*, mpsr_timings*);                                                 \
+    template const char* ppe_why<R>(PpeSolver<R>*);                                                                 \
     template mpsr_status ppe_get<R>(PpeSolver<R>*, const Dev<R>&, double*, double*, int*);                           \
     template void ppe_update_params<R>(PpeSolver<R>*, const mpsr_params&);
 MPSR_INSTANTIATE_P(float)
