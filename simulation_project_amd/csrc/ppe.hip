@@ -236,7 +236,7 @@ __global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __r
     (void)nslices;
 }
 
-// start of a PCG on A e = r0 (vectors V): r = r0, z = r / diag, p = 0, partial r.z and r.r
+// start of a PCG on A e = r0 (vectors V): r = r0, z = r / diag, p = z, partial r.z and r.r
 template <typename V>
 __global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restrict__ r0, const double* __restrict__ diag,
                                                    V* r, V* z, V* p, V* e, int zero_e, double* part) {
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restric
         const V zv = rv / (V)diag[t];
         r[t] = rv;
         z[t] = zv;
-        p[t] = (V)0;
+        p[t] = zv;
         if (zero_e) e[t] = (V)0;
         rz = (double)rv * (double)zv;
         rr = (double)rv * (double)rv;
@@ -261,31 +261,46 @@ __global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restric
     }
 }
 
-// p_new = z + beta p_old;  q = A p_new  (values VA, arithmetic V);  partial p_new.q
+// q = A p (values VA, arithmetic V), partial p.q.  One lane per row of a 64-row slice; the entry
+// loop is unrolled by 8 so every lane keeps 8 column/value loads and then 8 gathers of p in
+// flight (the loop is latency-bound otherwise).  Matrix streams are loaded non-temporally: they
+// are read once per SpMV and would only evict p from L2.
 template <typename VA, typename V>
 __global__ __launch_bounds__(PB) void k_pcg_spmv(int N, const int* __restrict__ slice_off,
                                                   const int* __restrict__ slice_w, const int* __restrict__ cols,
                                                   const VA* __restrict__ vals, const double* __restrict__ diag,
-                                                  const V* __restrict__ z, const V* __restrict__ pold, V* pnew, V* q,
-                                                  const double* __restrict__ scal, double* part) {
+                                                  const V* __restrict__ p, V* q, const double* __restrict__ scal,
+                                                  double* part) {
     if (scal[S_DONE] != 0.0) return;
     __shared__ double lds[PB / 64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int slice = t >> 6, lane = lane_id();
-    const V beta = (V)scal[S_BETA];
     double pq = 0.0;
     if (t < N) {
-        const V pn = z[t] + beta * pold[t];
-        V a = (V)diag[t] * pn;
-        const int base = slice_off[slice], width = slice_w[slice] >> 6;
-        for (int k = 0; k < width; ++k) {
-            const int e = base + k * 64 + lane;
-            const int cidx = cols[e];
-            a += (V)vals[e] * (z[cidx] + beta * pold[cidx]);
+        const V pt = p[t];
+        V a = (V)diag[t] * pt;
+        const int base = slice_off[slice] + lane, width = slice_w[slice] >> 6;
+        int k = 0;
+        for (; k + 8 <= width; k += 8) {
+            int c[8];
+            VA v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                c[u] = __builtin_nontemporal_load(&cols[base + (k + u) * 64]);
+                v[u] = __builtin_nontemporal_load(&vals[base + (k + u) * 64]);
+            }
+            V g[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) g[u] = p[c[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a += (V)v[u] * g[u];
         }
-        pnew[t] = pn;
+        for (; k < width; ++k) {
+            const int e = base + k * 64;
+            a += (V)__builtin_nontemporal_load(&vals[e]) * p[__builtin_nontemporal_load(&cols[e])];
+        }
         q[t] = a;
-        pq = (double)pn * (double)a;
+        pq = (double)pt * (double)a;
     }
     pq = block_sum<PB / 64>(pq, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = pq;
@@ -316,6 +331,15 @@ __global__ __launch_bounds__(PB) void k_pcg_update(int N, V* x, V* r, V* z, cons
         part[2 * blockIdx.x] = rz;
         part[2 * blockIdx.x + 1] = rr;
     }
+}
+
+// p = z + beta p (after the beta reduction; skipped once the solve has converged)
+template <typename V>
+__global__ __launch_bounds__(PB) void k_pcg_direction(int N, const V* __restrict__ z, V* p,
+                                                       const double* __restrict__ scal) {
+    if (scal[S_DONE] != 0.0) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) p[t] = z[t] + (V)scal[S_BETA] * p[t];
 }
 
 enum { RED_START = 0, RED_ALPHA = 1, RED_BETA = 2, RED_RESID = 3 };
@@ -495,19 +519,18 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
         const size_t chunk_ev0 = s->ev_used;
         const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
         for (int k = 0; k < CHUNK; ++k) {
-            V* pold = (k & 1) ? pb : pa;
-            V* pnew = (k & 1) ? pa : pb;
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
             hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, N, s->slice_off, s->slice_w, s->cols,
-                               vals, s->diag, z, pold, pnew, q, s->scal, s->part);
+                               vals, s->diag, pa, q, s->scal, s->part);
             if (timed) {
                 (void)hipEventRecord(s->ev[s->ev_used + 1], st);
                 s->ev_used += 2;
             }
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 1, s->scal, (int)RED_ALPHA);
-            hipLaunchKernelGGL(k_pcg_update<V>, dim3(nb), dim3(PB), 0, st, N, x, r, z, pnew, q, s->diag, s->scal,
+            hipLaunchKernelGGL(k_pcg_update<V>, dim3(nb), dim3(PB), 0, st, N, x, r, z, pa, q, s->diag, s->scal,
                                s->part);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_BETA);
+            hipLaunchKernelGGL(k_pcg_direction<V>, dim3(nb), dim3(PB), 0, st, N, z, pa, s->scal);
         }
         if (read_scal(s, st) != hipSuccess) {
             std::snprintf(s->why, sizeof(s->why), "device error while reading the PCG scalars");
@@ -678,9 +701,9 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     }
     {
         // algorithmic bytes of one SpMV launch: every stored entry's value and column once, and
-        // per row z, p_old, diag read and p_new, q written; gathers of z/p at columns not counted
+        // per row p and diag read and q written; gathers of p at the columns not counted
         const double sv = s->mixed ? 4.0 : 8.0;
-        s->spmv_bytes = (double)s->nnz_real * (sv + 4.0) + (double)N * (4.0 * sv + 8.0);
+        s->spmv_bytes = (double)s->nnz_real * (sv + 4.0) + (double)N * (2.0 * sv + 8.0);
     }
     hipLaunchKernelGGL(k_ppe_writeback<R>, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.c, d.perm, s->dir, s->x, d.in,
                        d.out, d.spos, d.spres, s->prev);
