@@ -211,6 +211,11 @@ typedef enum mpsr_warm_start {
     MPSR_WARM_LEARNED = 2   /* x0 and preconditioner from the message-passing network */
 } mpsr_warm_start;
 
+typedef enum mpsr_precond {
+    MPSR_PRECOND_JACOBI = 0,   /* z = D^-1 r */
+    MPSR_PRECOND_TWO_LEVEL = 1 /* z = D^-1 r + P Ac^-1 P^T r, P = aggregates of cells */
+} mpsr_precond;
+
 typedef struct mpsr_ppe_options {
     int solver;        /* mpsr_pcg_kind */
     double rel_tol;    /* stop when ||b - A x||_2 / ||b||_2 <= rel_tol (FP64 residual) */
@@ -218,6 +223,8 @@ typedef struct mpsr_ppe_options {
     int warm_start;    /* mpsr_warm_start */
     double n0_lap;     /* PND of the ideal lattice at the Laplacian radius; 0 = computed */
     double lambda_lap; /* Laplacian model constant at the Laplacian radius; 0 = computed */
+    int precond;       /* mpsr_precond */
+    int coarse_cells;  /* aggregate edge in cells for the two-level preconditioner; 0 = 4 */
 } mpsr_ppe_options;
 
 typedef struct mpsr_create_info {

@@ -109,9 +109,13 @@ class ParticlesC(C.Structure):
                 ("solid_phase_rate", _dp), ("type", _ip)]
 
 
+MPSR_PRECOND_JACOBI, MPSR_PRECOND_TWO_LEVEL = 0, 1
+
+
 class PpeOptions(C.Structure):
     _fields_ = [("solver", C.c_int), ("rel_tol", C.c_double), ("max_iter", C.c_int),
-                ("warm_start", C.c_int), ("n0_lap", C.c_double), ("lambda_lap", C.c_double)]
+                ("warm_start", C.c_int), ("n0_lap", C.c_double), ("lambda_lap", C.c_double),
+                ("precond", C.c_int), ("coarse_cells", C.c_int)]
 
 
 class CreateInfo(C.Structure):

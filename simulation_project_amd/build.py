@@ -71,7 +71,8 @@ def build(force=False, verbose=False):
     if failed:
         raise RuntimeError("libmpsr build failed: " + " ".join(failed[0][0]))
     if jobs or force or not os.path.exists(LIB):
-        cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", LIB] + objs
+        cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", LIB] + objs + \
+            [f"-L{ROCM}/lib", "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{ROCM}/lib"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("libmpsr link failed:\n" + r.stdout + r.stderr)

@@ -40,6 +40,8 @@
 #include "mpsr_internal.h"
 #include "neighbors.h"
 #include "ppe.h"
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 namespace mpsr {
 
@@ -84,6 +86,23 @@ template <typename R> struct PpeSolver {
     int profile = 0;
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
+    // two-level preconditioner: aggregates of a x a x a cells, dense inverse of Ac = P^T A P
+    int precond = 0;
+    int agg_edge = 4;
+    int cgx = 0, cgy = 0, cgz = 0, nfull = 0;
+    int* agg_flag = nullptr;     // [nfull]
+    int* agg_map = nullptr;      // [nfull] compact index (valid where flagged)
+    int* agg_full = nullptr;     // [nfull] compact -> full
+    int* agg_of_row = nullptr;   // [rows] compact aggregate of a non-Dirichlet row, else -1
+    int* nc_dev = nullptr;
+    int nc = 0, nc_cap = 0;
+    double* Ac = nullptr;        // [nc_cap^2] coarse matrix, then its inverse (column-major)
+    double* rc = nullptr;        // [nc_cap] restricted residual
+    double* cc = nullptr;        // [nc_cap] coarse correction
+    int* info_dev = nullptr;
+    rocblas_handle blas = nullptr;
+    SortScratch agg_scan{};
+    int two_level_ok = 0;        // the coarse factorisation of this solve succeeded
     int spmv_launches = 0;
     double spmv_ms = 0.0;
     double spmv_bytes = 0.0;
@@ -428,6 +447,197 @@ __global__ __launch_bounds__(PB) void k_ppe_writeback(int N, Consts<R> c, const 
 }
 
 // ---------------------------------------------------------------------------------------------
+// two-level preconditioner: z = D^-1 r + P Ac^-1 P^T r with P the indicator of aggregates of
+// a x a x a cells (non-Dirichlet rows only) and Ac = P^T A P.  Every aggregate is handled by one
+// wave whose lanes take its cells, so restriction and coarse assembly are fixed-order sums.
+// ---------------------------------------------------------------------------------------------
+struct AggGeom {
+    int gx, gy, gz;      // cell grid
+    int a;               // aggregate edge in cells
+    int cgx, cgy, cgz;   // aggregate grid
+};
+
+__device__ inline int agg_full_of_cell(const AggGeom& g, int cell) {
+    const int ci = cell / (g.gy * g.gz), cj = (cell / g.gz) % g.gy, ck = cell % g.gz;
+    return ((ci / g.a) * g.cgy + cj / g.a) * g.cgz + ck / g.a;
+}
+
+__global__ __launch_bounds__(PB) void k_agg_flag(int N, AggGeom g, const uint32_t* __restrict__ skeys,
+                                                  const int* __restrict__ dir, int* flag) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N || dir[t]) return;
+    flag[agg_full_of_cell(g, (int)skeys[t])] = 1;
+}
+__global__ __launch_bounds__(PB) void k_agg_rows(int N, AggGeom g, const uint32_t* __restrict__ skeys,
+                                                  const int* __restrict__ dir, const int* __restrict__ map,
+                                                  int* agg_of_row) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    agg_of_row[t] = dir[t] ? -1 : map[agg_full_of_cell(g, (int)skeys[t])];
+}
+__global__ __launch_bounds__(PB) void k_agg_list(int nfull, const int* __restrict__ flag, const int* __restrict__ map,
+                                                  int* full_of) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nfull && flag[f]) full_of[map[f]] = f;
+}
+
+// visit the cells of aggregate (full id f), lane-strided; F(cell)
+template <typename F> __device__ inline void for_agg_cells(const AggGeom& g, int f, F&& fn) {
+    const int ax = f / (g.cgy * g.cgz), ay = (f / g.cgz) % g.cgy, az = f % g.cgz;
+    const int n = g.a * g.a * g.a;
+    for (int c = lane_id(); c < n; c += 64) {
+        const int i = ax * g.a + c / (g.a * g.a), j = ay * g.a + (c / g.a) % g.a, k = az * g.a + c % g.a;
+        if (i < g.gx && j < g.gy && k < g.gz) fn((i * g.gy + j) * g.gz + k);
+    }
+}
+
+// Ac (column-major, nc x nc, zeroed before): one wave per aggregate I sums, per neighbour slot
+// (dx, dy, dz) in {-1,0,1}^3, the diagonal and off-diagonal entries of its rows
+template <typename VA>
+__global__ __launch_bounds__(64) void k_coarse_assemble(int nc, AggGeom g, const int* __restrict__ full_of,
+                                                         const int* __restrict__ flag, const int* __restrict__ map,
+                                                         const int* __restrict__ cs, const int* __restrict__ ce,
+                                                         const int* __restrict__ dir, const int* __restrict__ cnt,
+                                                         const double* __restrict__ diag,
+                                                         const int* __restrict__ slice_off,
+                                                         const int* __restrict__ cols, const VA* __restrict__ vals,
+                                                         const int* __restrict__ agg_of_row, double* Ac) {
+    const int I = blockIdx.x;
+    if (I >= nc) return;
+    const int f = full_of[I];
+    const int ax = f / (g.cgy * g.cgz), ay = (f / g.cgz) % g.cgy, az = f % g.cgz;
+    __shared__ double acc[27][64];   // per-lane running sums, one column per lane
+    const int lane = lane_id();
+    for (int q = 0; q < 27; ++q) acc[q][lane] = 0.0;
+    for_agg_cells(g, f, [&](int cell) {
+        const int a0 = cs[cell];
+        if (a0 < 0) return;
+        const int a1 = ce[cell];
+        for (int t = a0; t < a1; ++t) {
+            if (dir[t]) continue;
+            acc[13][lane] += diag[t];
+            const int base = slice_off[t >> 6] + (t & 63), n = cnt[t];
+            for (int k = 0; k < n; ++k) {
+                const int e = base + k * 64;
+                const int fj = full_of[agg_of_row[cols[e]]];
+                const int dx = fj / (g.cgy * g.cgz) - ax, dy = (fj / g.cgz) % g.cgy - ay, dz = fj % g.cgz - az;
+                // entries reach at most one aggregate away; anything else came from a cell-id
+                // wrap at the grid edge and is left out of the preconditioner
+                if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;
+                acc[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)][lane] += (double)vals[e];
+            }
+        }
+    });
+    __syncthreads();
+    if (lane < 27) {
+        double sum = 0.0;
+        for (int l = 0; l < 64; ++l) sum += acc[lane][l];
+        const int q = lane;
+        const int dx = q / 9 - 1, dy = (q / 3) % 3 - 1, dz = q % 3 - 1;
+        const int bx = ax + dx, by = ay + dy, bz = az + dz;
+        if (bx >= 0 && by >= 0 && bz >= 0 && bx < g.cgx && by < g.cgy && bz < g.cgz) {
+            const int fb = (bx * g.cgy + by) * g.cgz + bz;
+            if (flag[fb]) Ac[(size_t)map[fb] * nc + I] = sum;
+        }
+    }
+}
+
+// rocsolver_dpotri leaves the inverse in the lower triangle (column-major): mirror it
+__global__ __launch_bounds__(PB) void k_symmetrize(int nc, double* A) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nc * nc) return;
+    const int j = (int)(idx / nc), i = (int)(idx % nc);   // element (i, j) at A[j * nc + i]
+    if (i < j) A[(size_t)j * nc + i] = A[(size_t)i * nc + j];
+}
+
+__global__ __launch_bounds__(PB) void k_fill_d(double* a, long long n, double v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// rc[I] = sum of r over the non-Dirichlet rows of aggregate I (one wave each)
+template <typename V>
+__global__ __launch_bounds__(64) void k_restrict(int nc, AggGeom g, const int* __restrict__ full_of,
+                                                  const int* __restrict__ cs, const int* __restrict__ ce,
+                                                  const int* __restrict__ dir, const V* __restrict__ r, double* rc,
+                                                  const double* __restrict__ scal, int check_done) {
+    if (check_done && scal[S_DONE] != 0.0) return;
+    const int I = blockIdx.x;
+    if (I >= nc) return;
+    double acc = 0.0;
+    for_agg_cells(g, full_of[I], [&](int cell) {
+        const int a0 = cs[cell];
+        if (a0 < 0) return;
+        const int a1 = ce[cell];
+        for (int t = a0; t < a1; ++t)
+            if (!dir[t]) acc += (double)r[t];
+    });
+    acc = wave_sum(acc);
+    if (lane_id() == 0) rc[I] = acc;
+}
+
+// cc = Ainv rc (one wave per row; Ainv symmetric so row I = column I, contiguous)
+__global__ __launch_bounds__(64) void k_coarse_mv(int nc, const double* __restrict__ Ainv, const double* __restrict__ rc,
+                                                   double* cc, const double* __restrict__ scal, int check_done) {
+    if (check_done && scal[S_DONE] != 0.0) return;
+    const int I = blockIdx.x;
+    if (I >= nc) return;
+    const double* row = Ainv + (size_t)I * nc;
+    double acc = 0.0;
+    for (int J = lane_id(); J < nc; J += 64) acc += row[J] * rc[J];
+    acc = wave_sum(acc);
+    if (lane_id() == 0) cc[I] = acc;
+}
+
+// z = r / diag + cc[agg]; optionally p = z; partial r.z and r.r
+template <typename V>
+__global__ __launch_bounds__(PB) void k_zcomp(int N, const V* __restrict__ r, const double* __restrict__ diag,
+                                               const int* __restrict__ agg_of_row, const double* __restrict__ cc,
+                                               V* z, V* p, int set_p, double* part, const double* __restrict__ scal,
+                                               int check_done) {
+    if (check_done && scal[S_DONE] != 0.0) return;
+    __shared__ double lds[PB / 64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (t < N) {
+        const V rv = r[t];
+        const int I = agg_of_row[t];
+        const V zv = (V)((double)(rv / (V)diag[t]) + (I >= 0 ? cc[I] : 0.0));
+        z[t] = zv;
+        if (set_p) p[t] = zv;
+        rz = (double)rv * (double)zv;
+        rr = (double)rv * (double)rv;
+    }
+    rz = block_sum<PB / 64>(rz, lds);
+    rr = block_sum<PB / 64>(rr, lds);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = rz;
+        part[2 * blockIdx.x + 1] = rr;
+    }
+}
+
+// r = r0 (V) and, where asked, e = 0
+template <typename V>
+__global__ __launch_bounds__(PB) void k_pcg_start2(int N, const double* __restrict__ r0, V* r, V* e, int zero_e) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    r[t] = (V)r0[t];
+    if (zero_e) e[t] = (V)0;
+}
+
+// x += alpha p; r -= alpha q (the preconditioner and the dots follow in k_zcomp)
+template <typename V>
+__global__ __launch_bounds__(PB) void k_pcg_update2(int N, V* x, V* r, const V* __restrict__ p, const V* __restrict__ q,
+                                                     const double* __restrict__ scal) {
+    if (scal[S_DONE] != 0.0) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const V alpha = (V)scal[S_ALPHA];
+    x[t] = x[t] + alpha * p[t];
+    r[t] = r[t] - alpha * q[t];
+}
+
+// ---------------------------------------------------------------------------------------------
 template <typename T> static bool dev_alloc(T** p, size_t n) {
     return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)) == hipSuccess;
 }
@@ -468,18 +678,152 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     (void)hipMemset(s->prev, 0, sizeof(double) * rows);
     (void)hipMemset(s->scal, 0, sizeof(double) * S_COUNT);
     s->mixed = opt.solver == MPSR_PCG_MIXED;
+    // two-level preconditioner geometry and buffers
+    s->agg_edge = opt.coarse_cells > 0 ? opt.coarse_cells : 4;
+    s->cgx = (d.c.gx + s->agg_edge - 1) / s->agg_edge;
+    s->cgy = (d.c.gy + s->agg_edge - 1) / s->agg_edge;
+    s->cgz = (d.c.gz + s->agg_edge - 1) / s->agg_edge;
+    s->nfull = s->cgx * s->cgy * s->cgz;
+    ok &= dev_alloc(&s->agg_flag, s->nfull) && dev_alloc(&s->agg_map, s->nfull) && dev_alloc(&s->agg_full, s->nfull);
+    ok &= dev_alloc(&s->agg_of_row, rows) && dev_alloc(&s->nc_dev, 1) && dev_alloc(&s->info_dev, 2);
+    s->agg_scan.scan_tmp_cap = (int)scan_tmp_ints(s->nfull + 1);
+    ok &= dev_alloc(&s->agg_scan.scan_tmp, s->agg_scan.scan_tmp_cap);
+    if (!ok) {
+        ppe_destroy(s);
+        return nullptr;
+    }
     return s;
+}
+
+static constexpr int NC_MAX = 3072;   // largest coarse system factorised densely
+
+// Builds the aggregates of this solve's non-Dirichlet rows, Ac = P^T A P and its inverse.
+// Returns false (and the solve uses the Jacobi preconditioner) when there are no aggregates,
+// too many, or the factorisation fails.
+template <typename R, typename VA>
+static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hipStream_t st) {
+    const int N = d.N;
+    const AggGeom g{d.c.gx, d.c.gy, d.c.gz, s->agg_edge, s->cgx, s->cgy, s->cgz};
+    fill_int(s->agg_flag, 0, s->nfull, st);
+    hipLaunchKernelGGL(k_agg_flag, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, g, d.skeys, s->dir, s->agg_flag);
+    exclusive_scan(s->agg_flag, s->agg_map, nullptr, s->nfull, s->nfull, s->nc_dev, s->agg_scan, st);
+    hipLaunchKernelGGL(k_agg_list, dim3(grid_for(s->nfull, PB)), dim3(PB), 0, st, s->nfull, s->agg_flag, s->agg_map,
+                       s->agg_full);
+    int nc = 0;
+    if (hipMemcpyAsync(&s->scal_h[S_COUNT - 1], s->nc_dev, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return false;
+    if (hipStreamSynchronize(st) != hipSuccess) return false;
+    std::memcpy(&nc, &s->scal_h[S_COUNT - 1], sizeof(int));
+    s->nc = nc;
+    if (nc <= 0 || nc > NC_MAX) return false;
+    if (nc > s->nc_cap) {
+        if (s->Ac) (void)hipFree(s->Ac);
+        if (s->rc) (void)hipFree(s->rc);
+        if (s->cc) (void)hipFree(s->cc);
+        s->Ac = nullptr; s->rc = nullptr; s->cc = nullptr;
+        s->nc_cap = std::min(NC_MAX, nc + nc / 4 + 64);
+        if (!dev_alloc(&s->Ac, (size_t)s->nc_cap * s->nc_cap) || !dev_alloc(&s->rc, s->nc_cap) ||
+            !dev_alloc(&s->cc, s->nc_cap)) {
+            s->nc_cap = 0;
+            return false;
+        }
+    }
+    if (!s->blas) {
+        if (rocblas_create_handle(&s->blas) != rocblas_status_success) return false;
+    }
+    if (rocblas_set_stream(s->blas, st) != rocblas_status_success) return false;
+    hipLaunchKernelGGL(k_agg_rows, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, g, d.skeys, s->dir, s->agg_map,
+                       s->agg_of_row);
+    const long long n2 = (long long)nc * nc;
+    hipLaunchKernelGGL(k_fill_d, dim3(grid_for(n2, PB)), dim3(PB), 0, st, s->Ac, n2, 0.0);
+    hipLaunchKernelGGL((k_coarse_assemble<VA>), dim3(nc), dim3(64), 0, st, nc, g, s->agg_full, s->agg_flag, s->agg_map,
+                       d.cell_start, d.cell_end, s->dir, s->cnt, s->diag, s->slice_off, s->cols, vals, s->agg_of_row,
+                       s->Ac);
+    if (rocsolver_dpotrf(s->blas, rocblas_fill_lower, nc, s->Ac, nc, s->info_dev) != rocblas_status_success) return false;
+    if (rocsolver_dpotri(s->blas, rocblas_fill_lower, nc, s->Ac, nc, s->info_dev + 1) != rocblas_status_success)
+        return false;
+    int info[2] = {0, 0};
+    if (hipMemcpyAsync(info, s->info_dev, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return false;
+    if (hipStreamSynchronize(st) != hipSuccess) return false;
+    if (info[0] != 0 || info[1] != 0) return false;
+    hipLaunchKernelGGL(k_symmetrize, dim3(grid_for(n2, PB)), dim3(PB), 0, st, nc, s->Ac);
+    return true;
+}
+
+// PCG on A e = r0 (V = FP32, e from 0) or A x = b (V = FP64, x = warm start, r0 = b - A x) with
+// the two-level preconditioner.  Returns iterations or -1 on breakdown.
+template <typename R, typename VA, typename V>
+static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* x, V* r, V* z, V* p, V* q,
+                    const double* r0, double target, int maxit, int zero_x, hipStream_t st) {
+    const int rows = s->nslices * 64;
+    const int nb = (rows + PB - 1) / PB;
+    const int nc = s->nc;
+    const AggGeom g{d.c.gx, d.c.gy, d.c.gz, s->agg_edge, s->cgx, s->cgy, s->cgz};
+    s->scal_h[S_TARGET] = target;
+    s->scal_h[S_MAXIT] = (double)maxit;
+    if (hipMemcpyAsync(s->scal + S_TARGET, s->scal_h + S_TARGET, sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+        return -1;
+    if (hipMemcpyAsync(s->scal + S_MAXIT, s->scal_h + S_MAXIT, sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+        return -1;
+    hipLaunchKernelGGL(k_pcg_start2<V>, dim3(nb), dim3(PB), 0, st, N, r0, r, x, zero_x);
+    hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, g, s->agg_full, d.cell_start, d.cell_end, s->dir,
+                       r, s->rc, s->scal, 0);
+    hipLaunchKernelGGL(k_coarse_mv, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 0);
+    hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->diag, s->agg_of_row, s->cc, z, p, 1, s->part,
+                       s->scal, 0);
+    hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_START);
+    int done_iters = 0;
+    for (int it = 0; it < maxit; it += CHUNK) {
+        const size_t chunk_ev0 = s->ev_used;
+        const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
+        for (int k = 0; k < CHUNK; ++k) {
+            if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
+            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, N, s->slice_off, s->slice_w, s->cols,
+                               vals, s->diag, p, q, s->scal, s->part);
+            if (timed) {
+                (void)hipEventRecord(s->ev[s->ev_used + 1], st);
+                s->ev_used += 2;
+            }
+            hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 1, s->scal, (int)RED_ALPHA);
+            hipLaunchKernelGGL(k_pcg_update2<V>, dim3(nb), dim3(PB), 0, st, N, x, r, p, q, s->scal);
+            hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, g, s->agg_full, d.cell_start, d.cell_end,
+                               s->dir, r, s->rc, s->scal, 1);
+            hipLaunchKernelGGL(k_coarse_mv, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 1);
+            hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->diag, s->agg_of_row, s->cc, z, p, 0,
+                               s->part, s->scal, 1);
+            hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_BETA);
+            hipLaunchKernelGGL(k_pcg_direction<V>, dim3(nb), dim3(PB), 0, st, N, z, p, s->scal);
+        }
+        if (read_scal(s, st) != hipSuccess) {
+            std::snprintf(s->why, sizeof(s->why), "device error while reading the PCG scalars");
+            return -1;
+        }
+        const int live = std::max(0, std::min(CHUNK, (int)s->scal_h[S_ITERS] - done_iters));
+        if (timed) s->ev_used = chunk_ev0 + 2 * (size_t)live;
+        done_iters = (int)s->scal_h[S_ITERS];
+        if (s->scal_h[S_DONE] != 0.0) break;
+    }
+    if (s->scal_h[S_DONE] == 3.0) {
+        std::snprintf(s->why, sizeof(s->why),
+                      "two-level PCG breakdown after %d iterations (p.Ap = %.3e, r.z = %.3e, r.r = %.3e)", done_iters,
+                      s->scal_h[S_PQ], s->scal_h[S_RZ], s->scal_h[S_RR]);
+        return -1;
+    }
+    return done_iters;
 }
 
 template <typename R> void ppe_destroy(PpeSolver<R>* s) {
     if (!s) return;
     void* ps[] = {s->active, s->dir, s->cnt, s->slice_w, s->slice_off, s->total, s->cols, s->vals64, s->vals32,
                   s->diag, s->rhs, s->x, s->prev, s->r, s->z, s->p0, s->p1, s->q, s->rf, s->zf, s->p0f,
-                  s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp, s->nnz_dev};
+                  s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp, s->nnz_dev, s->agg_flag,
+                  s->agg_map, s->agg_full, s->agg_of_row, s->nc_dev, s->info_dev, s->Ac, s->rc, s->cc,
+                  s->agg_scan.scan_tmp};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (s->scal_h) (void)hipHostFree(s->scal_h);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+    if (s->blas) (void)rocblas_destroy_handle(s->blas);
     delete s;
 }
 
@@ -631,11 +975,18 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->passes = 0;
     s->fallbacks = 0;
     s->why[0] = 0;
+    s->two_level_ok = 0;
+    if (opt.precond == MPSR_PRECOND_TWO_LEVEL && bb > 0 && rr > target)
+        s->two_level_ok = s->mixed ? setup_two_level<R, float>(s, d, s->vals32, st)
+                                   : setup_two_level<R, double>(s, d, s->vals64, st);
+    const bool two = s->two_level_ok != 0;
     if (!s->mixed) {
         if (bb > 0 && rr > target) {
-            // FP64 Jacobi-PCG from the warm start; r already holds b - A x
-            const int it = run_pcg<R, double, double>(s, N, s->vals64, s->x, s->r, s->z, s->p0, s->p1, s->q, s->r,
-                                                      target, maxit, st);
+            // FP64 PCG from the warm start; r already holds b - A x
+            const int it = two ? run_pcg2<R, double, double>(s, d, N, s->vals64, s->x, s->r, s->z, s->p0, s->q, s->r,
+                                                             target, maxit, 0, st)
+                               : run_pcg<R, double, double>(s, N, s->vals64, s->x, s->r, s->z, s->p0, s->p1, s->q,
+                                                            s->r, target, maxit, st);
             if (it < 0) s->status = 1;
             else s->iters = it;
             s->passes = 1;
@@ -647,8 +998,11 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
         bool fallback = false;
         while (bb > 0 && rr > target && s->passes < 16 && s->iters < maxit) {
             const double inner_rel = std::max(1e-6, 0.5 * tol * std::sqrt(bb / rr));
-            const int it = run_pcg<R, float, float>(s, N, s->vals32, s->ef, s->rf, s->zf, s->p0f, s->p1f, s->qf, s->r,
-                                                    inner_rel * inner_rel * rr, maxit - s->iters, st);
+            const int it =
+                two ? run_pcg2<R, float, float>(s, d, N, s->vals32, s->ef, s->rf, s->zf, s->p0f, s->qf, s->r,
+                                                inner_rel * inner_rel * rr, maxit - s->iters, 1, st)
+                    : run_pcg<R, float, float>(s, N, s->vals32, s->ef, s->rf, s->zf, s->p0f, s->p1f, s->qf, s->r,
+                                               inner_rel * inner_rel * rr, maxit - s->iters, st);
             if (it < 0) {
                 fallback = true;
                 break;
@@ -669,8 +1023,10 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
             // plain PCG in FP64 vectors on the same (FP32-stored) matrix from the current x;
             // r holds b - A x from the last residual pass
             s->fallbacks += 1;
-            const int it = run_pcg<R, float, double>(s, N, s->vals32, s->x, s->r, s->z, s->p0, s->p1, s->q, s->r,
-                                                     target, std::max(1, maxit - s->iters), st);
+            const int it = two ? run_pcg2<R, float, double>(s, d, N, s->vals32, s->x, s->r, s->z, s->p0, s->q, s->r,
+                                                            target, std::max(1, maxit - s->iters), 0, st)
+                               : run_pcg<R, float, double>(s, N, s->vals32, s->x, s->r, s->z, s->p0, s->p1, s->q,
+                                                           s->r, target, std::max(1, maxit - s->iters), st);
             if (it < 0) s->status = 1;
             else s->iters += it;
             s->passes += 1;

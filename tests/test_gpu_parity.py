@@ -197,3 +197,27 @@ def test_large_drum_identical(lib):
     g.step_many(0, 3)
     o.step_many(0, 3)
     same_state(g, o, label="drum 3 steps")
+
+
+@pytest.mark.parametrize("solver", [abi.MPSR_PCG_FP64, abi.MPSR_PCG_MIXED])
+def test_two_level_preconditioner_same_solution_fewer_iterations(lib, solver):
+    # The two-level preconditioner (aggregates of 4^3 cells, Ac = P^T A P inverted densely) must
+    # solve the same system to the same tolerance as Jacobi-PCG and the oracle, in fewer iterations.
+    case = cases.drum(lib, l0=6e-3)
+    base = dict(pressure_mode=abi.MPSR_PRESSURE_PPE)
+    jac = abi.PpeOptions(abi.MPSR_PCG_FP64, 1e-8, 20000, abi.MPSR_WARM_ZERO, 0.0, 0.0, abi.MPSR_PRECOND_JACOBI, 0)
+    two = abi.PpeOptions(solver, 1e-8, 20000, abi.MPSR_WARM_ZERO, 0.0, 0.0, abi.MPSR_PRECOND_TWO_LEVEL, 4)
+    g_j = Solver(case, ppe=jac, **base)
+    g_t = Solver(case, ppe=two, **base)
+    o = Oracle(case, ppe=jac, **base)
+    for s in (g_j, g_t, o):
+        for name in ("RegisterParticleTopology", "RegisterTriangleTopology", "CalcSTLDistance", "CalcExplicitly",
+                     "RelocateAllParticleTopology", "RegisterParticleTopology", "CalcSTLDistance",
+                     "CalcPressurePoisson"):
+            getattr(s, name)()
+    tj, tt = g_j.timings(), g_t.timings()
+    assert tj["ppe_rel_residual"] <= 1e-8 and tt["ppe_rel_residual"] <= 1e-8
+    assert tt["ppe_iterations"] < tj["ppe_iterations"]
+    _, xo, _, _ = o.ppe_system()
+    _, xt, _ = g_t.ppe_system()
+    assert np.linalg.norm(xt - xo) <= 1e-6 * np.linalg.norm(xo)
