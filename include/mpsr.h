@@ -270,6 +270,7 @@ typedef struct mpsr_timings {
     long long ppe_nnz;       /* off-diagonal entries of the last PPE matrix */
     int ppe_rows;            /* non-Dirichlet rows of the last PPE matrix */
     int ppe_fallbacks;       /* mixed solves that finished with the plain FP64 PCG */
+    int ppe_coarse_rows;     /* aggregates of the two-level preconditioner, 0 = Jacobi was used */
 } mpsr_timings;
 
 typedef struct mpsr_handle mpsr_handle;
@@ -328,6 +329,24 @@ mpsr_status mpsr_get_particles(mpsr_handle *h, int which, mpsr_particles *out);
 /* = ParticleHostToDevice + ParticleNumberToConst (CGPUCellSet.cpp:893-952, DeviceProcess.cu:265-271). */
 mpsr_status mpsr_set_particles(mpsr_handle *h, const mpsr_particles *in);
 mpsr_status mpsr_get_triangles(mpsr_handle *h, mpsr_triangle *out, int n);
+/* Current slot count (m_ParticleNum): grows when bucket particles are appended. */
+mpsr_status mpsr_get_particle_count(mpsr_handle *h, int *n);
+/* Bucket inflow = the reference's multi-bucket set-up (LoadMultiBucket,
+ * Src/MultiGPUHandler.cpp:1068-1083): bucket j (buckets[j], slot rows as in mpsr_create) is added
+ * after step i when (i + 1) % interval[j] == 0, at most total[j] times.  An addition
+ * (BucketAddition + InsertBucket, MultiGPUHandler.cpp:943-981, 1085-1123) fills the first dead
+ * slots (ID < 0) in ascending slot order, appends the rest after the last slot while the capacity
+ * given at mpsr_create (max_particles) allows and drops it otherwise; ID = slot index, the pressure
+ * of a reused slot is kept and that of an appended slot is 0.  In a step with an addition the
+ * walls of linearly moving models are reset (ResetWallPosition_Kernel, Kern :913-925), the
+ * particles are relocated again and the wall translation is skipped, as in CalculateNew
+ * (MultiGPUHandler.cpp:462-484).  Replaces any earlier buckets and restarts their counters. */
+mpsr_status mpsr_set_buckets(mpsr_handle *h, const mpsr_particles *buckets, const int *interval,
+                             const int *total, int count);
+/* The addition of one step alone, for drivers that call the phases one by one
+ * (CMultiGPUHandler::BucketAddition, MultiGPUHandler.cpp:1085-1123); *added = 1 when a bucket
+ * was due. */
+mpsr_status mpsr_bucket_addition(mpsr_handle *h, int step, int *added);
 mpsr_status mpsr_get_gridbox(mpsr_handle *h, mpsr_gridbox *out);
 /* = CheckAndUpdateDt's constant upload (CGPUCellSet.cpp:1278). */
 mpsr_status mpsr_set_dt(mpsr_handle *h, float dt, float courant);
@@ -360,6 +379,47 @@ mpsr_status mpsr_get_ppe_matrix(mpsr_handle *h, int *row_ptr, int *col, double *
                                 int cap, int *nnz);
 /* Per-phase HIP-event timing of the following steps (mpsr_get_timings); off by default. */
 mpsr_status mpsr_set_profiling(mpsr_handle *h, int on);
+
+/* ---- input decks and output frames (host only, no device use) --------------------------------
+ * A deck is the reference's configuration file and the files it names, parsed as
+ *   CConfiguration::ReadConfigurationFileWithDrag (Src/Configuration.cpp:21-421): input directory,
+ *     parameter file, particle file, bucket lines, STL lines (moving walls or drag walls D/DE/DC/DF),
+ *     output directory and name, output interval, compute zone, section division, start step,
+ *     ASCII/BINARY output;
+ *   IO::Load of the positional parameter file with its defaults (Src/IO.cpp:66-432);
+ *   IO::Load of the 12-column particle file (Src/IO.cpp:7-64);
+ *   STL models, ASCII or binary (solver/Common/inc/STLSerializer.cpp:226-351), normals recomputed
+ *     as CalcTriangleNormal (Src/IO.cpp:435-453), ModelIDNumber = model index;
+ *   the model set-up of CMultiGPUHandler::Run (Src/MultiGPUHandler.cpp:640-786): model offsets
+ *     added to the vertices, degrees per step = RPM * Dt * 6, lambda from l0, compute zone from
+ *     the configuration or computed from particles and triangles, capacity = particles + the sum
+ *     over bucket lines of bucket particles * total additions (the multi-bucket build,
+ *     Src/MultiGPUHandler.cpp:758-765).
+ * More than one section is rejected (MPSR_PARAMERR): one GPU holds the whole domain. */
+typedef struct mpsr_deck mpsr_deck;
+mpsr_status mpsr_deck_load(const char *config_path, mpsr_deck **out);
+void mpsr_deck_free(mpsr_deck *d);
+const char *mpsr_deck_last_error(void);
+/* Fills *info for mpsr_create (FP32, EOS pressure); its pointers point into the deck, which must
+ * outlive the mpsr_create call. */
+mpsr_status mpsr_deck_create_info(const mpsr_deck *d, mpsr_create_info *info);
+/* Bucket lines (LoadMultiBucket, Src/MultiGPUHandler.cpp:1068-1083): particle count, addition
+ * interval in steps and total number of additions of bucket j. */
+int mpsr_deck_num_buckets(const mpsr_deck *d);
+mpsr_status mpsr_deck_bucket(const mpsr_deck *d, int j, int *count, int *interval, int *total);
+/* Rows of bucket j into caller arrays of out->n >= count slots (out->n is set to count). */
+mpsr_status mpsr_deck_get_bucket(const mpsr_deck *d, int j, mpsr_particles *out);
+int mpsr_deck_start_step(const mpsr_deck *d);
+int mpsr_deck_output_interval(const mpsr_deck *d);
+int mpsr_deck_output_ascii(const mpsr_deck *d);
+const char *mpsr_deck_output_name(const mpsr_deck *d);
+/* Per-model offsets from the configuration (cap entries); returns the model count. */
+int mpsr_deck_model_offsets(const mpsr_deck *d, mpsr_float3 *out, int cap);
+/* One output frame as IO::SaveAscii / IO::SaveBinary (Src/IO.cpp:635-742): particle count,
+ * 12 values per particle (position, velocity, pressure, density, temperature, viscosity, solid
+ * rate, type), time, model count, per model position and rotation angle. */
+mpsr_status mpsr_write_frame(const char *path, int ascii, const mpsr_particles *p, float time,
+                             int nmodels, const mpsr_float3 *model_pos, const float *angles);
 
 #ifdef __cplusplus
 }

@@ -321,6 +321,12 @@ template <typename R> struct Fields {
         pos.assign(n, V3<R>{0, 0, 0}); vel.assign(n, V3<R>{0, 0, 0});
         p.assign(n, 0); dens.assign(n, 0); temp.assign(n, 0); nu.assign(n, 0); solid.assign(n, 0);
     }
+    // keeps the first slots, new ones as resize() leaves them
+    void grow(int n) {
+        id.resize(n, -1); type.resize(n, 0);
+        pos.resize(n, V3<R>{0, 0, 0}); vel.resize(n, V3<R>{0, 0, 0});
+        p.resize(n, 0); dens.resize(n, 0); temp.resize(n, 0); nu.resize(n, 0); solid.resize(n, 0);
+    }
     void copy_slot(const Fields& o, int s) {
         id[s] = o.id[s]; type[s] = o.type[s]; pos[s] = o.pos[s]; vel[s] = o.vel[s];
         p[s] = o.p[s]; dens[s] = o.dens[s]; temp[s] = o.temp[s]; nu[s] = o.nu[s]; solid[s] = o.solid[s];
@@ -332,8 +338,14 @@ template <typename R> struct Sim {
     Grid<R> g;
     int n = 0;
     Fields<R> in, out;
-    std::vector<Tri<R>> tris;
+    std::vector<Tri<R>> tris, tris0;
     std::vector<mpsr_tri_params> models;
+    int cap = 0;
+    struct Bucket {
+        Fields<R> f;
+        int count = 0, interval = 1, total = 0, added = 0;
+    };
+    std::vector<Bucket> buckets;
     std::vector<mpsr_drag_params> drag;
     // binning
     std::vector<int> key, perm, cstart, cend;
@@ -803,8 +815,10 @@ template <typename R> struct Sim {
         const double tolr = ppe.rel_tol > 0 ? ppe.rel_tol : 1e-8;
         const int maxit = ppe.max_iter > 0 ? ppe.max_iter : 20000;
         ppe_x.assign(n, 0.0);
-        if (ppe.warm_start == MPSR_WARM_PREVIOUS && (int)ppe_prev.size() == n)
+        if (ppe.warm_start == MPSR_WARM_PREVIOUS && !ppe_prev.empty()) {
+            ppe_prev.resize(n, 0.0);  // slots appended since the last solve start from 0
             for (int s = 0; s < n; ++s) ppe_x[s] = ppe_dir[s] ? 0.0 : ppe_prev[s];
+        }
         std::vector<double> r(n), z(n), p(n), q(n);
         spmv(ppe_x, q);
         double bb = 0.0;
@@ -852,7 +866,56 @@ template <typename R> struct Sim {
         }
     }
 
-    // one body of CalculateNew's loop (MultiGPUHandler.cpp:362-492) without output/buckets
+    // Kern :913-925 ResetWallPosition: linearly moving models back to their initial triangles
+    void reset_walls(int step) {
+        for (size_t t = 0; t < tris.size(); ++t) {
+            const mpsr_tri_params& m = models[tris[t].model];
+            if (!m.LinearMovableOrNot || m.LinearResetInterval <= 0) continue;
+            if ((step + 1) % m.LinearResetInterval == 0) tris[t] = tris0[t];
+        }
+    }
+
+    // MultiGPUHandler.cpp:949-981 InsertBucket on the output arrays, serial slot loop
+    void insert_bucket(const Bucket& b) {
+        auto put = [&](int i, int j) {
+            out.id[i] = i;
+            out.pos[i] = b.f.pos[j]; out.vel[i] = b.f.vel[j];
+            out.temp[i] = b.f.temp[j]; out.dens[i] = b.f.dens[j]; out.nu[i] = b.f.nu[j];
+            out.solid[i] = b.f.solid[j]; out.type[i] = b.f.type[j];
+        };
+        int j = 0, i = 0;
+        if (b.count == 0) return;
+        for (; i < n; ++i) {
+            if (out.id[i] < 0) {
+                put(i, j);
+                ++j;
+                if (j >= b.count) return;
+            }
+        }
+        if (n + b.count - j <= cap) {
+            in.grow(n + b.count - j); out.grow(n + b.count - j);
+            for (; j < b.count; ++j, ++i) {
+                put(i, j);
+                out.p[i] = 0;  // a slot never written before
+            }
+            n = i;
+        }
+    }
+
+    // MultiGPUHandler.cpp:1085-1123 BucketAddition; returns whether a bucket was due
+    bool bucket_addition(int step) {
+        bool added = false;
+        for (Bucket& b : buckets) {
+            if (!(b.added < b.total && (step + 1) % b.interval == 0)) continue;
+            added = true;
+            insert_bucket(b);
+            reset_walls(step);
+            ++b.added;
+        }
+        return added;
+    }
+
+    // one body of CalculateNew's loop (MultiGPUHandler.cpp:362-492) without output
     void step(int i) {
         register_particles();
         register_triangles();
@@ -866,7 +929,8 @@ template <typename R> struct Sim {
         else pnd_pressure(true);
         gradient();
         relocate();
-        translate();
+        if (bucket_addition(i)) relocate();
+        else translate();
         rotate(i);
     }
 };
@@ -978,6 +1042,8 @@ template <typename R> static Sim<R>* build(const mpsr_create_info* ci, const mps
         for (int k = 0; k < 3; ++k) q.v[k] = mk((R)T.Vertex[k].x, (R)T.Vertex[k].y, (R)T.Vertex[k].z);
         S->tris.push_back(q);
     }
+    S->tris0 = S->tris;
+    S->cap = std::max(ci->max_particles, S->n);
     S->models.assign(ci->models, ci->models + ci->num_models);
     if (ci->num_drag > 0) S->drag.assign(ci->drag, ci->drag + ci->num_drag);
     S->pressure_mode = ci->pressure_mode;
@@ -1016,6 +1082,29 @@ using namespace orc;
 
 #define DISPATCH(h, call) ((h)->prec == MPSR_FP64 ? ((h)->d->call) : ((h)->f->call))
 
+template <typename R> static void set_buckets(Sim<R>* S, const mpsr_particles* b, const int* itv, const int* tot, int count) {
+    S->buckets.clear();
+    for (int j = 0; j < count; ++j) {
+        typename Sim<R>::Bucket B;
+        const mpsr_particles& q = b[j];
+        B.count = q.n;
+        B.interval = itv[j];
+        B.total = tot[j];
+        B.f.resize(q.n);
+        for (int s = 0; s < q.n; ++s) {
+            B.f.id[s] = q.id ? q.id[s] : s;
+            B.f.pos[s] = mk((R)q.position[3 * s], (R)q.position[3 * s + 1], (R)q.position[3 * s + 2]);
+            B.f.vel[s] = mk((R)q.velocity[3 * s], (R)q.velocity[3 * s + 1], (R)q.velocity[3 * s + 2]);
+            B.f.p[s] = q.pressure ? (R)q.pressure[s] : 0;
+            B.f.dens[s] = q.density ? (R)q.density[s] : 0;
+            B.f.temp[s] = q.temperature ? (R)q.temperature[s] : 0;
+            B.f.nu[s] = q.kinetic_viscosity ? (R)q.kinetic_viscosity[s] : 0;
+            B.f.solid[s] = q.solid_phase_rate ? (R)q.solid_phase_rate[s] : 0;
+            B.f.type[s] = q.type ? q.type[s] : 0;
+        }
+        S->buckets.push_back(B);
+    }
+}
 extern "C" {
 
 float orc_sqrt14(float x) { return sqrt14(x); }
@@ -1075,6 +1164,18 @@ void orc_pressure_ppe(void* p) { Handle* h = (Handle*)p; DISPATCH(h, pressure_pp
 void orc_gradient(void* p) { Handle* h = (Handle*)p; DISPATCH(h, gradient()); }
 void orc_translate(void* p) { Handle* h = (Handle*)p; DISPATCH(h, translate()); }
 void orc_rotate(void* p, int i) { Handle* h = (Handle*)p; DISPATCH(h, rotate(i)); }
+void orc_set_buckets(void* p, const mpsr_particles* b, const int* itv, const int* tot, int count) {
+    Handle* h = (Handle*)p;
+    if (h->d) set_buckets(h->d, b, itv, tot, count);
+    else set_buckets(h->f, b, itv, tot, count);
+}
+int orc_bucket_addition(void* p, int step) {
+    Handle* h = (Handle*)p;
+    return h->d ? (int)h->d->bucket_addition(step) : (int)h->f->bucket_addition(step);
+}
+int orc_count(void* p) { Handle* h = (Handle*)p; return h->d ? h->d->n : h->f->n; }
+void orc_reset_walls(void* p, int step) { Handle* h = (Handle*)p; DISPATCH(h, reset_walls(step)); }
+
 void orc_set_ppe_options(void* p, const mpsr_ppe_options* o) { Handle* h = (Handle*)p; if (h->d) h->d->ppe = *o; if (h->f) h->f->ppe = *o; }
 
 void orc_get_particles(void* p, int which, mpsr_particles* o) {

@@ -74,6 +74,10 @@ def load():
         "orc_get_ppe_csr": (C.c_int, [vp, ip, ip, dp, dp, C.c_int]),
         "orc_lap_coeffs": (None, [vp, dp, dp]),
         "orc_n0_source": (C.c_double, [vp]),
+        "orc_set_buckets": (None, [vp, P(abi.ParticlesC), ip, ip, C.c_int]),
+        "orc_bucket_addition": (C.c_int, [vp, C.c_int]),
+        "orc_count": (C.c_int, [vp]),
+        "orc_reset_walls": (None, [vp, C.c_int]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -94,12 +98,13 @@ def _dp(a):
 class Oracle:
     """CPU restatement of one domain; same phase names as simulation_project_amd.Solver."""
 
-    def __init__(self, case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_EOS, ppe=None):
+    def __init__(self, case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_EOS, ppe=None,
+                 max_particles=None):
         self.lib = load()
         self.case = case
         self.n = case.n
         self.ntri = len(case.vertices)
-        info, self._keep = cases.create_info(case, precision, pressure_mode, ppe)
+        info, self._keep = cases.create_info(case, precision, pressure_mode, ppe, max_particles)
         self.h = self.lib.orc_create(C.byref(info))
         gb = abi.GridBox()
         self.lib.orc_get_gridbox(self.h, C.byref(gb))
@@ -119,10 +124,28 @@ class Oracle:
 
     def step(self, i):
         self.lib.orc_step(self.h, i)
+        self.n = self.lib.orc_count(self.h)
 
     def step_many(self, first, count):
         for i in range(first, first + count):
             self.lib.orc_step(self.h, i)
+        self.n = self.lib.orc_count(self.h)
+
+    def set_buckets(self, buckets):
+        """buckets: list of (ParticleArrays, interval, total)."""
+        arr, keep = cases.bucket_structs(buckets)
+        self._bkeep = keep
+        itv = (C.c_int * max(1, len(buckets)))(*[b[1] for b in buckets])
+        tot = (C.c_int * max(1, len(buckets)))(*[b[2] for b in buckets])
+        self.lib.orc_set_buckets(self.h, arr, itv, tot, len(buckets))
+
+    def bucket_addition(self, step):
+        added = self.lib.orc_bucket_addition(self.h, step)
+        self.n = self.lib.orc_count(self.h)
+        return bool(added)
+
+    def ResetWallPosition(self, step):
+        self.lib.orc_reset_walls(self.h, step)
 
     def RegisterParticleTopology(self):
         self.lib.orc_register(self.h)

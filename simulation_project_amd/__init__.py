@@ -73,6 +73,23 @@ def load_library(path=None):
         "mpsr_get_ppe": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_int)]),
         "mpsr_get_ppe_matrix": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_double), P(C.c_double), C.c_int,
                                           P(C.c_int)]),
+        "mpsr_get_particle_count": (C.c_int, [vp, P(C.c_int)]),
+        "mpsr_set_buckets": (C.c_int, [vp, P(abi.ParticlesC), P(C.c_int), P(C.c_int), C.c_int]),
+        "mpsr_bucket_addition": (C.c_int, [vp, C.c_int, P(C.c_int)]),
+        "mpsr_deck_load": (C.c_int, [C.c_char_p, P(vp)]),
+        "mpsr_deck_free": (None, [vp]),
+        "mpsr_deck_last_error": (C.c_char_p, []),
+        "mpsr_deck_create_info": (C.c_int, [vp, P(abi.CreateInfo)]),
+        "mpsr_deck_num_buckets": (C.c_int, [vp]),
+        "mpsr_deck_bucket": (C.c_int, [vp, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "mpsr_deck_get_bucket": (C.c_int, [vp, C.c_int, P(abi.ParticlesC)]),
+        "mpsr_deck_start_step": (C.c_int, [vp]),
+        "mpsr_deck_output_interval": (C.c_int, [vp]),
+        "mpsr_deck_output_ascii": (C.c_int, [vp]),
+        "mpsr_deck_output_name": (C.c_char_p, [vp]),
+        "mpsr_deck_model_offsets": (C.c_int, [vp, P(abi.Float3), C.c_int]),
+        "mpsr_write_frame": (C.c_int, [C.c_char_p, C.c_int, P(abi.ParticlesC), C.c_float, C.c_int, P(abi.Float3),
+                                       P(C.c_float)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -101,6 +118,7 @@ class Solver:
         self.case = case
         self.n = case.n
         self.ntri = len(case.vertices)
+        self._buckets = False
         info, self._keep = cases.create_info(case, precision, pressure_mode, ppe, max_particles, device)
         h = C.c_void_p()
         self._check(self.lib.mpsr_create(C.byref(info), C.byref(h)), "mpsr_create")
@@ -126,11 +144,35 @@ class Solver:
             pass
 
     # ---- the step (CalculateNew loop body) ----
+    def _count(self):
+        n = C.c_int()
+        self._check(self.lib.mpsr_get_particle_count(self.h, C.byref(n)), "mpsr_get_particle_count")
+        self.n = n.value
+
     def step(self, i):
         self._check(self.lib.mpsr_step(self.h, i), "mpsr_step")
+        if self._buckets:
+            self._count()
 
     def step_many(self, first, count):
         self._check(self.lib.mpsr_step_many(self.h, first, count), "mpsr_step_many")
+        if self._buckets:
+            self._count()
+
+    def set_buckets(self, buckets):
+        """Bucket inflow: buckets is a list of (ParticleArrays, interval, total)."""
+        from . import cases
+        arr, keep = cases.bucket_structs(buckets)
+        itv = (C.c_int * max(1, len(buckets)))(*[b[1] for b in buckets])
+        tot = (C.c_int * max(1, len(buckets)))(*[b[2] for b in buckets])
+        self._check(self.lib.mpsr_set_buckets(self.h, arr, itv, tot, len(buckets)), "mpsr_set_buckets")
+        self._buckets = bool(buckets)
+
+    def bucket_addition(self, step):
+        added = C.c_int()
+        self._check(self.lib.mpsr_bucket_addition(self.h, step, C.byref(added)), "mpsr_bucket_addition")
+        self._count()
+        return bool(added.value)
 
     def synchronize(self):
         self._check(self.lib.mpsr_synchronize(self.h), "mpsr_synchronize")

@@ -134,7 +134,8 @@ class Timings(C.Structure):
                 ("ppe_refinements", C.c_int), ("ppe_rel_residual", C.c_double),
                 ("ppe_initial_rel_residual", C.c_double), ("spmv_ms", C.c_double),
                 ("spmv_launches", C.c_int), ("spmv_bytes", C.c_double), ("ppe_nnz", C.c_longlong),
-                ("ppe_rows", C.c_int), ("ppe_fallbacks", C.c_int)]
+                ("ppe_rows", C.c_int), ("ppe_fallbacks", C.c_int),
+                ("ppe_coarse_rows", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

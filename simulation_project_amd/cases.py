@@ -17,6 +17,7 @@ starts at the lattice PND n0.  Every site gets a uniform jitter of +-1e-4 l0 per
 counter-based SplitMix64 stream keyed by (seed, particle, axis).
 """
 import ctypes as C
+import os
 import math
 from dataclasses import dataclass, field
 
@@ -350,6 +351,14 @@ def create_info(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_E
     return info, (tris, models, drag, ps, case)
 
 
+def bucket_structs(buckets):
+    """ctypes array of mpsr_particles for [(ParticleArrays, interval, total), ...] and the objects
+    that keep its buffers alive."""
+    structs = [b[0].struct() for b in buckets]
+    arr = (abi.ParticlesC * max(1, len(structs)))(*structs)
+    return arr, (structs, [b[0] for b in buckets])
+
+
 def drag_box(lib, token="E", nx=8, ny=6, nz=8, l0=0.01, seed=DEFAULT_SEED, dt=1e-4):
     """A small box with a drag plate (an extra two-triangle STL model marked as drag, the
     reference's 'DE V V' / 'DC V' / 'DF V' configuration lines) crossing the water block.  The
@@ -390,3 +399,62 @@ def drag_box(lib, token="E", nx=8, ny=6, nz=8, l0=0.01, seed=DEFAULT_SEED, dt=1e
     case.particles.velocity[:] = np.array([0.05, -0.02, 0.03], np.float32).astype(np.float64)
     case.name = f"drag_box_{token}"
     return _finish(case, lib)
+
+
+def from_deck(lib, config_path, steps=None):
+    """Load one of the reference's input decks (configuration file, parameter file, particle and
+    bucket files, STL models) through mpsr_deck_load and return it as a Case.  The deck's own
+    set-up is kept as loaded: parameters from the parameter file, lambda from l0, model offsets
+    added to the vertices, the compute zone of the configuration or the automatic one.  The
+    capacity the reference reserves for bucket inflow is in case.meta["max_particles"], the buckets
+    as [(ParticleArrays, interval, total), ...] in case.meta["buckets"]."""
+    h = C.c_void_p()
+    st = lib.mpsr_deck_load(os.fsencode(config_path), C.byref(h))
+    if st != 0:
+        raise RuntimeError(f"mpsr_deck_load({config_path}): status {st}: "
+                           f"{lib.mpsr_deck_last_error().decode(errors='replace')}")
+    try:
+        info = abi.CreateInfo()
+        if lib.mpsr_deck_create_info(h, C.byref(info)) != 0:
+            raise RuntimeError(lib.mpsr_deck_last_error().decode(errors="replace"))
+        src = info.particles.contents
+        n = src.n
+        pa = abi.ParticleArrays(n)
+        if n:
+            pa.id[:] = np.ctypeslib.as_array(src.id, (n,))
+            pa.position[:] = np.ctypeslib.as_array(src.position, (n * 3,)).reshape(n, 3)
+            pa.velocity[:] = np.ctypeslib.as_array(src.velocity, (n * 3,)).reshape(n, 3)
+            for k in ("pressure", "density", "temperature", "kinetic_viscosity", "solid_phase_rate"):
+                getattr(pa, k)[:] = np.ctypeslib.as_array(getattr(src, k), (n,))
+            pa.type[:] = np.ctypeslib.as_array(src.type, (n,))
+        T = info.num_triangles
+        verts = np.zeros((T, 3, 3), np.float32)
+        model = np.zeros(T, np.int32)
+        for t in range(T):
+            tr = info.triangles[t]
+            model[t] = tr.ModelIDNumber
+            for k in range(3):
+                verts[t, k] = (tr.Vertex[k].x, tr.Vertex[k].y, tr.Vertex[k].z)
+        models = [abi.TriParams.from_buffer_copy(info.models[m]) for m in range(info.num_models)]
+        drag = [abi.DragParams.from_buffer_copy(info.drag[m]) for m in range(info.num_drag)]
+        params = abi.Params.from_buffer_copy(info.params)
+        zone = abi.Box.from_buffer_copy(info.compute_zone)
+        buckets = []
+        for j in range(lib.mpsr_deck_num_buckets(h)):
+            cnt, itv, tot = C.c_int(), C.c_int(), C.c_int()
+            lib.mpsr_deck_bucket(h, j, C.byref(cnt), C.byref(itv), C.byref(tot))
+            bp = abi.ParticleArrays(cnt.value)
+            bs = bp.struct()
+            if lib.mpsr_deck_get_bucket(h, j, C.byref(bs)) != 0:
+                raise RuntimeError(lib.mpsr_deck_last_error().decode(errors="replace"))
+            buckets.append((bp, itv.value, tot.value))
+        meta = {"max_particles": info.max_particles, "buckets": buckets,
+                "start_step": lib.mpsr_deck_start_step(h),
+                "output_interval": lib.mpsr_deck_output_interval(h),
+                "output_ascii": bool(lib.mpsr_deck_output_ascii(h)),
+                "output_name": lib.mpsr_deck_output_name(h).decode(errors="replace")}
+    finally:
+        lib.mpsr_deck_free(h)
+    name = os.path.splitext(os.path.basename(config_path))[0]
+    return Case(name, params, pa, verts, model, models, drag, zone,
+                steps if steps is not None else params.AnalysisStep, meta)

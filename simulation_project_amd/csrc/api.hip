@@ -55,6 +55,9 @@ struct EngineBase {
     virtual mpsr_status set_dt(float dt, float courant) = 0;
     virtual mpsr_status set_ppe(const mpsr_ppe_options* o) = 0;
     virtual mpsr_status sync() = 0;
+    virtual mpsr_status set_buckets(const mpsr_particles* b, const int* interval, const int* total, int count) = 0;
+    virtual mpsr_status bucket_phase(int step, int* added) = 0;
+    virtual int count() const = 0;
     mpsr_gridbox gb{};
     mpsr_timings tm{};
     int profiling = 0;
@@ -194,12 +197,24 @@ template <typename R> struct Engine : EngineBase {
     PpeSolver<R>* ppe = nullptr;
     double n0_lap = 0, lambda_lap = 0, n0_src = 0;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // bucket inflow (multi-bucket build of the reference)
+    struct BucketD {
+        PState<R> s{};
+        int count = 0, interval = 1, total = 0, added = 0;
+    };
+    std::vector<BucketD> buckets;
+    int* bk_flag = nullptr;            // [cap] dead-slot flags
+    int* bk_rank = nullptr;            // [cap] exclusive scan of the flags
+    int* bk_dead = nullptr;            // [1] dead-slot count
+    int* bk_dead_h = nullptr;          // pinned copy of it
+    TriD<R>* tris0 = nullptr;          // initial triangles (ResetWallPosition)
     std::vector<hipEvent_t> pev;      // profiling events, two per phase launch
     std::vector<int> pev_phase;
     size_t pev_used = 0;
 
     ~Engine() override {
         if (ppe) ppe_destroy(ppe);
+        if (bk_dead_h) (void)hipHostFree(bk_dead_h);
         for (void* p : allocs) (void)hipFree(p);
         for (hipEvent_t e : pev) (void)hipEventDestroy(e);
         if (ev_a) (void)hipEventDestroy(ev_a);
@@ -234,7 +249,7 @@ template <typename R> struct Engine : EngineBase {
         return MPSR_NOERR;
     }
 
-    mpsr_status upload_state(const mpsr_particles* pp) {
+    mpsr_status upload_fields(const mpsr_particles* pp, std::initializer_list<PState<R>*> dst_states) {
         const int n = pp->n;
         std::vector<int> ids(n), type(n);
         std::vector<R> f[11];
@@ -252,13 +267,79 @@ template <typename R> struct Engine : EngineBase {
             f[9][s] = pp->kinetic_viscosity ? (R)pp->kinetic_viscosity[s] : R(0);
             f[10][s] = pp->solid_phase_rate ? (R)pp->solid_phase_rate[s] : R(0);
         }
-        for (PState<R>* S : {&d.in, &d.out}) {
+        if (n == 0) return MPSR_NOERR;
+        for (PState<R>* S : dst_states) {
             HIPC(hipMemcpy(S->id, ids.data(), n * sizeof(int), hipMemcpyHostToDevice));
             HIPC(hipMemcpy(S->type, type.data(), n * sizeof(int), hipMemcpyHostToDevice));
             R* dst[11] = {S->px, S->py, S->pz, S->vx, S->vy, S->vz, S->p, S->n, S->temp, S->nu, S->solid};
             for (int a = 0; a < 11; ++a) HIPC(hipMemcpy(dst[a], f[a].data(), n * sizeof(R), hipMemcpyHostToDevice));
         }
-        d.N = n;
+        return MPSR_NOERR;
+    }
+    mpsr_status upload_state(const mpsr_particles* pp) {
+        mpsr_status s = upload_fields(pp, {&d.in, &d.out});
+        if (s != MPSR_NOERR) return s;
+        d.N = pp->n;
+        return MPSR_NOERR;
+    }
+
+    // Bucket inflow: InsertBucket + SetAddedBucket + ResetWallPosition of BucketAddition
+    // (MultiGPUHandler.cpp:943-981, 1085-1123), for every bucket due at this step.
+    mpsr_status set_buckets(const mpsr_particles* b, const int* interval, const int* total, int count) override {
+        HIPC(hipSetDevice(device));
+        if (count < 0 || (count > 0 && (!b || !interval || !total))) return fail(MPSR_PARAMERR, "invalid bucket arguments");
+        for (int j = 0; j < count; ++j) {
+            if (interval[j] <= 0) return fail(MPSR_PARAMERR, "bucket addition interval must be positive");
+            if (b[j].n < 0 || (b[j].n > 0 && !b[j].position)) return fail(MPSR_PARAMERR, "bucket particles missing");
+        }
+        HIPC(hipStreamSynchronize(st));
+        buckets.clear();
+        for (int j = 0; j < count; ++j) {
+            BucketD B;
+            if (alloc_state(B.s, std::max(1, b[j].n)) != MPSR_NOERR) return MPSR_GPUERR;
+            mpsr_status s = upload_fields(&b[j], {&B.s});
+            if (s != MPSR_NOERR) return s;
+            B.count = b[j].n;
+            B.interval = interval[j];
+            B.total = total[j];
+            buckets.push_back(B);
+        }
+        if (count > 0 && !bk_flag) {
+            ALLOC(bk_flag, d.cap);
+            ALLOC(bk_rank, d.cap);
+            ALLOC(bk_dead, 1);
+            HIPC(hipHostMalloc((void**)&bk_dead_h, sizeof(int)));
+        }
+        return MPSR_NOERR;
+    }
+    int count() const override { return d.N; }
+    mpsr_status bucket_phase(int step, int* added) override {
+        HIPC(hipSetDevice(device));
+        bool a = false;
+        mpsr_status s = bucket_addition(step, a);
+        if (s != MPSR_NOERR) return s;
+        HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(st));
+        if (added) *added = a ? 1 : 0;
+        return MPSR_NOERR;
+    }
+    // Returns whether a bucket was added at this step (the wall translation is then skipped).
+    mpsr_status bucket_addition(int step, bool& added) {
+        added = false;
+        for (BucketD& B : buckets) {
+            if (!(B.added < B.total && (step + 1) % B.interval == 0)) continue;
+            added = true;
+            launch_bucket_reuse(d, B.s, B.count, bk_flag, bk_rank, bk_dead, st);
+            HIPC(hipMemcpyAsync(bk_dead_h, bk_dead, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPC(hipStreamSynchronize(st));
+            const int used = std::min(*bk_dead_h, B.count);
+            if (used < B.count && (long long)d.N + B.count - used <= (long long)d.cap) {
+                launch_bucket_append(d, B.s, used, B.count - used, st);
+                d.N += B.count - used;
+            }
+            launch_reset_walls(d, tris0, step, st);
+            ++B.added;
+        }
         return MPSR_NOERR;
     }
 
@@ -327,6 +408,8 @@ template <typename R> struct Engine : EngineBase {
         }
         ALLOC(d.tris, std::max(1, d.ntri));
         HIPC(hipMemcpy(d.tris, th.data(), sizeof(TriD<R>) * std::max(1, d.ntri), hipMemcpyHostToDevice));
+        ALLOC(tris0, std::max(1, d.ntri));
+        HIPC(hipMemcpy(tris0, th.data(), sizeof(TriD<R>) * std::max(1, d.ntri), hipMemcpyHostToDevice));
         ALLOC(d.tri_emit, d.ntri + 1); ALLOC(d.tri_off, d.ntri + 1);
         ALLOC(d.pair_n, 1); ALLOC(d.uniq_n, 1); ALLOC(d.overflow, 1);
         HIPC(hipMemset(d.overflow, 0, sizeof(int)));
@@ -451,15 +534,15 @@ template <typename R> struct Engine : EngineBase {
         HIPC(hipMemcpyAsync(rot_table, rot_host.data(), sizeof(RotCoef<R>) * rot_host.size(), hipMemcpyHostToDevice, st));
         return MPSR_NOERR;
     }
-    void do_walls(int k) {
+    void do_walls(int k, bool translate) {
         mark_begin(PH_WALL);
-        launch_translate(d, st);
+        if (translate) launch_translate(d, st);
         d.rot = rot_table + (size_t)k * std::max(1, d.nmodel);
         launch_rotate(d, st);
         mark_end();
     }
 
-    mpsr_status one_step(int k) {
+    mpsr_status one_step(int k, int step) {
         do_bin();
         do_tri();
         do_stl();
@@ -472,7 +555,13 @@ template <typename R> struct Engine : EngineBase {
         if (s != MPSR_NOERR) return s;
         do_gradient();
         do_relocate();
-        do_walls(k);
+        bool added = false;
+        if (!buckets.empty()) {
+            s = bucket_addition(step, added);
+            if (s != MPSR_NOERR) return s;
+            if (added) do_relocate();
+        }
+        do_walls(k, !added);
         return MPSR_NOERR;
     }
 
@@ -485,7 +574,7 @@ template <typename R> struct Engine : EngineBase {
         // the host copy above must finish before rot_host is refilled by a later call
         HIPC(hipEventRecord(ev_a, st));
         for (int k = 0; k < count; ++k) {
-            s = one_step(k);
+            s = one_step(k, first + k);
             if (s != MPSR_NOERR) return s;
             HIPC(hipGetLastError());
         }
@@ -534,6 +623,7 @@ template <typename R> struct Engine : EngineBase {
         t.ppe_nnz = tm.ppe_nnz;
         t.ppe_rows = tm.ppe_rows;
         t.ppe_fallbacks = tm.ppe_fallbacks;
+        t.ppe_coarse_rows = tm.ppe_coarse_rows;
         tm = t;
     }
 
@@ -809,6 +899,17 @@ mpsr_status mpsr_get_particles(mpsr_handle* h, int which, mpsr_particles* o) {
     return h->e->get_particles(which, o);
 }
 mpsr_status mpsr_set_particles(mpsr_handle* h, const mpsr_particles* p) { GUARD(h); return h->e->set_particles(p); }
+mpsr_status mpsr_get_particle_count(mpsr_handle* h, int* n) {
+    GUARD(h);
+    if (!n) return MPSR_PARAMERR;
+    *n = h->e->count();
+    return MPSR_NOERR;
+}
+mpsr_status mpsr_set_buckets(mpsr_handle* h, const mpsr_particles* b, const int* interval, const int* total, int count) {
+    GUARD(h);
+    return h->e->set_buckets(b, interval, total, count);
+}
+mpsr_status mpsr_bucket_addition(mpsr_handle* h, int step, int* added) { GUARD(h); return h->e->bucket_phase(step, added); }
 mpsr_status mpsr_get_triangles(mpsr_handle* h, mpsr_triangle* o, int n) { GUARD(h); return h->e->get_triangles(o, n); }
 mpsr_status mpsr_get_gridbox(mpsr_handle* h, mpsr_gridbox* o) {
     GUARD(h);

@@ -268,5 +268,10 @@ template <typename R> void launch_stl_distance(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_drag(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_translate(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_rotate(const Dev<R>& d, hipStream_t st);
+// inflow.hip
+template <typename R>
+void launch_bucket_reuse(const Dev<R>& d, const PState<R>& b, int B, int* flag, int* rank, int* dead, hipStream_t st);
+template <typename R> void launch_bucket_append(const Dev<R>& d, const PState<R>& b, int first, int count, hipStream_t st);
+template <typename R> void launch_reset_walls(const Dev<R>& d, const TriD<R>* tris0, int step, hipStream_t st);
 
 }  // namespace mpsr

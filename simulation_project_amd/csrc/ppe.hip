@@ -1078,6 +1078,7 @@ template <typename R> mpsr_status ppe_check(PpeSolver<R>* s, mpsr_timings* tm) {
     tm->ppe_rel_residual = s->relres;
     tm->ppe_initial_rel_residual = s->relres0;
     tm->ppe_fallbacks = s->fallbacks;
+    tm->ppe_coarse_rows = s->two_level_ok ? s->nc : 0;
     return s->status == 0 ? MPSR_NOERR : MPSR_CALCERR;
 }
 

@@ -221,3 +221,51 @@ def test_two_level_preconditioner_same_solution_fewer_iterations(lib, solver):
     _, xo, _, _ = o.ppe_system()
     _, xt, _ = g_t.ppe_system()
     assert np.linalg.norm(xt - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+DECK_CONFIG = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "velocity_deck",
+                                         "configurationDragTemp.txt")
+
+
+def test_velocity_deck_with_buckets_identical(lib):
+    """The reference's own test deck (1428 particles, four STL models of which two are
+    exponential drag walls, EOS pressure, no gravity) for 40 steps with its bucket line
+    (1428 particles every 16 steps): additions after steps 15 and 31, batched in one call."""
+    case = cases.from_deck(lib, DECK_CONFIG)
+    cap = case.n + 3 * 1428
+    g = Solver(case, max_particles=cap)
+    o = Oracle(case, max_particles=cap)
+    for s in (g, o):
+        s.set_buckets(case.meta["buckets"])
+    g.step_many(0, 40)
+    o.step_many(0, 40)
+    assert g.n == o.n == case.n + 2 * 1428
+    same_state(g, o, label="velocity deck")
+    np.testing.assert_array_equal(g.get_triangles(), o.get_triangles())
+
+
+@pytest.mark.parametrize("precision", [abi.MPSR_FP32, abi.MPSR_FP64])
+@pytest.mark.parametrize("cap_extra", [100, 3])
+def test_bucket_reuse_append_drop_identical(lib, precision, cap_extra):
+    """Dead slots refilled first, the rest appended or (small capacity) dropped; then ten steps
+    with a bucket every second step, three additions at most."""
+    from tests.test_deck import make_bucket
+    case = cases.small_box(lib)
+    l0 = case.params.InitialParticleDistance
+    case.zone.m_MaxBound.x = float(np.float32(7.5 * l0))
+    cap = case.n + cap_extra
+    g = Solver(case, precision=precision, max_particles=cap)
+    o = Oracle(case, precision=precision, max_particles=cap)
+    bucket = make_bucket(60, l0)
+    for s in (g, o):
+        s.set_buckets([(bucket, 1, 1)])
+    run_phase("RegisterParticleTopology", g, o)
+    assert g.bucket_addition(0) and o.bucket_addition(0)
+    assert g.n == o.n
+    same_state(g, o, which=(1,), label="after insertion")
+    for s in (g, o):
+        s.set_buckets([(make_bucket(7, l0, seed=5), 2, 3)])
+    g.step_many(1, 10)
+    o.step_many(1, 10)
+    assert g.n == o.n
+    same_state(g, o, label="after steps")
