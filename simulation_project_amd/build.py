@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmpsr.so")
+RUNNER = os.path.join(HERE, "mpsr_run")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("MPSR_OFFLOAD_ARCH", "gfx950")
@@ -76,6 +77,14 @@ def build(force=False, verbose=False):
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("libmpsr link failed:\n" + r.stdout + r.stderr)
+    # the command-line driver: host code over the C ABI, linked against the library next to it
+    src = os.path.join(CSRC, "mpsr_run.cpp")
+    if force or not os.path.exists(RUNNER) or os.path.getmtime(RUNNER) < max(os.path.getmtime(src), os.path.getmtime(LIB)):
+        cmd = ["g++"] + CXX_FLAGS + [src, "-o", RUNNER, f"-L{HERE}", "-lmpsr", "-Wl,-rpath,$ORIGIN",
+                                     f"-Wl,-rpath-link,{ROCM}/lib", f"-Wl,-rpath,{ROCM}/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("mpsr_run build failed:\n" + r.stdout + r.stderr)
     return LIB
 
 
