@@ -1,19 +1,26 @@
-"""Benchmark: MPS rotating-drum timestep at ~1M particles on one MI355X.
+"""Benchmark: MPS rotating-drum timestep at 1M particles on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--l0 2.016e-3] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--l0 2.012e-3] [--solver learned|mixed|fp64]
+                    [--precond two|jacobi] [--no-cpu-baseline] [--no-plain]
 
-Workload: the rotating drum (cylinder R = 0.15 m, L = 0.3 m, 10,000-triangle STL, 30 rpm, 40 %
-fill) at l0 = 2.016e-3 m, about 1.0M fluid particles, FP32 particle operators, pressure from the
-pressure Poisson equation solved each step to an FP64 relative residual of 1e-8.  A step is one
-body of the reference's CalculateNew loop: two binnings, wall binning, two STL distance passes,
+Workload: the rotating drum (cylinder R = 0.15 m, L = 0.3 m, 9984-triangle STL, 30 rpm, 40 % fill)
+at l0 = 2.012e-3 m: 1,008,472 fluid particles, FP32 particle operators, pressure from the pressure
+Poisson equation solved every step to an FP64 relative residual of 1e-8.  A step is one body of
+the reference's CalculateNew loop: two binnings, wall binning, two STL distance passes,
 predictor, PND, PPE assembly + PCG, corrector, two relocations, wall rotation.  Inputs are
-resident in HBM when the timed region starts (only particle state lives on the device; nothing
-crosses PCIe inside the timed region except the PCG's per-chunk convergence flag).
+resident in HBM when the timed region starts; inside it only the PCG's convergence flag (once per
+32 iterations) and the per-step timings cross PCIe.
 
-Printed JSON line: metric / value = particle-updates per second over the K timed steps; PPE
-ms/step for the production solver and for plain FP64 PCG from x0 = 0 (same case, separate run);
+Production solver: mixed precision (FP32 matrix and SpMV, FP64 dots and residuals, FP64
+refinement) with the two-level preconditioner (3^3-cell aggregates, coarse operator rebuilt every
+10 solves), warm-started by the learned network when its trained weights are present
+(simulation_project_amd/weights/mpnn_drum.npy), else by the previous pressure.  The plain
+comparison solves the same steps with FP64 Jacobi-PCG from the previous pressure.
+
+One JSON line: value = particle-updates/s over the K timed steps; PPE ms/step of both solvers;
 roofline of the dominant kernel (the PCG SpMV) from HIP events around every SpMV launch of the
-timed region; the CPU oracle timed on the host cores on a bounded sample.
+timed region, on the solver's stream; a whole-step roofline from the algorithmic bytes of a step;
+the CPU oracle timed on the host cores on one step of the same drum.
 """
 import argparse
 import json
@@ -21,14 +28,35 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+WEIGHTS = os.path.join(ROOT, "simulation_project_amd", "weights", "mpnn_drum.npy")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def ppe_options(abi, kind, precond, warm):
+    solver = abi.MPSR_PCG_FP64 if kind == "fp64" else abi.MPSR_PCG_MIXED
+    pre = abi.MPSR_PRECOND_TWO_LEVEL if precond == "two" else abi.MPSR_PRECOND_JACOBI
+    return abi.PpeOptions(solver, 1e-8, 20000, warm, 0.0, 0.0, pre, 3, 10)
+
+
+def run_steps(s, first, count):
+    """Per-step timings of `count` steps; returns (elapsed s, list of timing dicts)."""
+    out = []
+    s.synchronize()
+    t0 = time.perf_counter()
+    for k in range(count):
+        s.step(first + k)
+        out.append(s.timings())
+    s.synchronize()
+    return time.perf_counter() - t0, out
 
 
 def main():
@@ -36,79 +64,78 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--l0", type=float, default=2.016e-3)
-    ap.add_argument("--solver", choices=["mixed", "fp64"], default="mixed")
-    ap.add_argument("--warm", choices=["previous", "zero"], default="previous")
+    ap.add_argument("--l0", type=float, default=2.012e-3)
+    ap.add_argument("--solver", choices=["learned", "mixed", "fp64"], default=None)
+    ap.add_argument("--precond", choices=["two", "jacobi"], default="two")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 PCG comparison run")
-    ap.add_argument("--plain-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build runs one domain per GPU and no multi-GPU "
-              "path is built (see DESIGN.md for how the domain would shard); use --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: this build runs one domain per GPU and no multi-GPU path is "
+              "built; run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from simulation_project_amd import Solver, abi, cases, load_library
     lib = load_library()
+    solver = args.solver or ("learned" if os.path.exists(WEIGHTS) else "mixed")
     t0 = time.time()
     case = cases.drum(lib, l0=args.l0, steps=args.warmup + args.steps)
-    log(f"[bench] drum l0={args.l0} n={case.n} triangles={len(case.vertices)} built in {time.time() - t0:.1f}s")
-
-    solver_kind = abi.MPSR_PCG_MIXED if args.solver == "mixed" else abi.MPSR_PCG_FP64
-    warm = abi.MPSR_WARM_PREVIOUS if args.warm == "previous" else abi.MPSR_WARM_ZERO
-    ppe = abi.PpeOptions(solver_kind, 1e-8, 20000, warm, 0.0, 0.0)
-    s = Solver(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE, ppe=ppe)
-    log("[bench] solver created; warm-up")
-    if args.warmup:
-        s.step_many(0, args.warmup)
-    s.synchronize()
-    s.set_profiling(True)
-    log(f"[bench] timed region: {args.steps} steps")
-    per_step = []
-    ppe_ms, iters, spmv_ms, spmv_n, spmv_bytes, relres = [], [], [], [], [], []
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        s.step(args.warmup + k)
-        t = s.timings()
-        per_step.append(t)
-        ppe_ms.append(t["ppe_ms"])
-        iters.append(t["ppe_iterations"])
-        relres.append(t["ppe_rel_residual"])
-        if t["spmv_launches"]:
-            spmv_ms.append(t["spmv_ms"] * t["spmv_launches"])
-            spmv_n.append(t["spmv_launches"])
-            spmv_bytes.append(t["spmv_bytes"])
-    s.synchronize()
-    elapsed = time.perf_counter() - t_start
     n = case.n
+    log(f"[bench] drum l0={args.l0} n={n} triangles={len(case.vertices)} built in {time.time() - t0:.1f}s; "
+        f"solver {solver}, preconditioner {args.precond}")
+
+    warm = abi.MPSR_WARM_LEARNED if solver == "learned" else abi.MPSR_WARM_PREVIOUS
+    kind = "fp64" if solver == "fp64" else "mixed"
+    s = Solver(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE,
+               ppe=ppe_options(abi, kind, args.precond, warm))
+    if solver == "learned":
+        s.mpnn_set_weights(np.load(WEIGHTS).astype(np.float32))
+    s.set_profiling(True)
+    log("[bench] warm-up")
+    if args.warmup:
+        run_steps(s, 0, args.warmup)
+    log(f"[bench] timed region: {args.steps} steps")
+    elapsed, per_step = run_steps(s, args.warmup, args.steps)
     value = n * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    spmv_avg_ms = sum(spmv_ms) / max(1, sum(spmv_n))
-    spmv_b = sum(spmv_bytes) / max(1, len(spmv_bytes))
+    ppe_ms = [t["ppe_ms"] for t in per_step]
+    iters = [t["ppe_iterations"] for t in per_step]
+    relres = [t["ppe_rel_residual"] for t in per_step]
+    launches = sum(t["spmv_launches"] for t in per_step)
+    spmv_total_ms = sum(t["spmv_ms"] * t["spmv_launches"] for t in per_step)
+    spmv_avg_ms = spmv_total_ms / max(1, launches)
+    spmv_b = float(np.mean([t["spmv_bytes"] for t in per_step]))
     achieved = spmv_b / (spmv_avg_ms * 1e-3) / 1e9 if spmv_avg_ms > 0 else 0.0
-    phases = {k: round(sum(t[k] for t in per_step) / len(per_step), 4) for k in
-              ("bin_ms", "tri_ms", "stl_ms", "explicit_ms", "relocate_ms", "pnd_ms", "ppe_ms", "gradient_ms", "wall_ms")}
-    log(f"[bench] {ms_per_step:.2f} ms/step, PPE {sum(ppe_ms) / len(ppe_ms):.2f} ms, iters {iters}, "
-        f"spmv {spmv_avg_ms * 1e3:.1f} us x {sum(spmv_n)}")
+    phases = {k: round(float(np.mean([t[k] for t in per_step])), 4) for k in
+              ("bin_ms", "tri_ms", "stl_ms", "explicit_ms", "relocate_ms", "pnd_ms", "ppe_ms", "gradient_ms",
+               "wall_ms", "ppe_setup_ms")}
+    # whole-step algorithmic bytes: the particle phases (FP32: 442 B per particle) plus the PPE
+    # assembly and every PCG iteration (SpMV values and columns, 19 vector words per row)
+    nnz = float(np.mean([t["ppe_nnz"] for t in per_step]))
+    rows = float(np.mean([t["ppe_rows"] for t in per_step]))
+    sv, sx = (4.0, 4.0) if kind == "mixed" else (8.0, 8.0)
+    step_bytes = n * 442.0 + rows * 5 * 4 + nnz * (sv + 4) + float(np.mean(iters)) * (nnz * (sv + 4) + rows * 19 * sx)
+    step_achieved = step_bytes / (ms_per_step * 1e-3) / 1e9
+    log(f"[bench] {ms_per_step:.2f} ms/step, PPE {np.mean(ppe_ms):.2f} ms, iterations {iters}, "
+        f"SpMV {spmv_avg_ms * 1e3:.1f} us x {launches}")
+    learned_fb = sum(t.get("ppe_learned_fallback", 0) for t in per_step)
     s.close()
 
     plain = None
     if not args.no_plain:
-        log("[bench] plain FP64 PCG from x0 = 0, same case")
-        pp = abi.PpeOptions(abi.MPSR_PCG_FP64, 1e-8, 20000, abi.MPSR_WARM_ZERO, 0.0, 0.0)
-        s2 = Solver(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE, ppe=pp)
+        log("[bench] plain FP64 Jacobi-PCG, same steps")
+        p = Solver(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE,
+                   ppe=ppe_options(abi, "fp64", "jacobi", abi.MPSR_WARM_PREVIOUS))
+        p.set_profiling(True)
         if args.warmup:
-            s2.step_many(0, args.warmup)
-        s2.set_profiling(True)
-        pm, pit = [], []
-        for k in range(args.plain_steps):
-            s2.step(args.warmup + k)
-            t = s2.timings()
-            pm.append(t["ppe_ms"])
-            pit.append(t["ppe_iterations"])
-        s2.close()
-        plain = {"ppe_ms_per_step": round(sum(pm) / len(pm), 3), "iterations": pit}
-        log(f"[bench] plain FP64 PCG: {plain}")
+            run_steps(p, 0, args.warmup)
+        pe, pt = run_steps(p, args.warmup, args.steps)
+        p.close()
+        pm = float(np.mean([t["ppe_ms"] for t in pt]))
+        plain = {"ppe_ms_per_step": round(pm, 3), "iterations": [t["ppe_iterations"] for t in pt],
+                 "ms_per_step": round(1e3 * pe / args.steps, 3),
+                 "ppe_speedup": round(pm / float(np.mean(ppe_ms)), 3)}
+        log(f"[bench] plain: {plain}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -117,14 +144,13 @@ def main():
             threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
             o = Oracle(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE,
                        ppe=abi.PpeOptions(abi.MPSR_PCG_FP64, 1e-8, 20000, abi.MPSR_WARM_PREVIOUS, 0.0, 0.0))
-            log(f"[bench] CPU oracle: 2 steps of the same drum on {threads} threads")
+            log(f"[bench] CPU oracle: 1 step of the same drum on {threads} threads")
             tc = time.perf_counter()
-            o.step_many(0, 2)
+            o.step_many(0, 1)
             tcpu = time.perf_counter() - tc
-            cpu = {"value": round(n * 2 / tcpu, 1), "unit": "particle-updates/s", "cores": threads,
-                   "kind": "port",
-                   "sample": f"2 steps (steps 0-1, FP32 operators, FP64 Jacobi-PCG to 1e-8) of the same "
-                             f"{n}-particle drum, {tcpu:.1f} s, OpenMP over particles"}
+            cpu = {"value": round(n / tcpu, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
+                   "sample": f"step 0 of the same {n}-particle drum (FP32 operators, FP64 Jacobi-PCG to 1e-8), "
+                             f"{tcpu:.1f} s, OpenMP over particles"}
             o.close()
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] CPU baseline skipped: {e}")
@@ -142,18 +168,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"rotating drum, {n} particles (l0={args.l0} m), 10000-triangle STL, 30 rpm, "
-                               f"40% fill, PPE {args.solver} PCG to 1e-8, warm start {args.warm}",
+        "config": {"workload": f"rotating drum, {n} particles (l0={args.l0} m), 9984-triangle STL, 30 rpm, "
+                               f"40% fill, PPE to 1e-8 ({solver}, {args.precond} preconditioner)",
                    "particles": n, "triangles": len(case.vertices), "dt": case.params.Dt},
-        "ppe_ms_per_step": round(sum(ppe_ms) / len(ppe_ms), 3),
+        "ppe_ms_per_step": round(float(np.mean(ppe_ms)), 3),
         "ppe_iterations": iters,
         "ppe_rel_residual_max": max(relres),
+        "ppe_learned_fallbacks": learned_fb,
         "plain_fp64_pcg": plain,
         "phase_ms": phases,
-        "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": spmv_b, "avg_launch_ms": spmv_avg_ms,
-                     "launches": sum(spmv_n)},
+        "step_roofline": {"achieved": round(step_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(step_achieved / HBM_PEAK_GBS, 4), "bytes_per_step": step_bytes},
+        "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": spmv_b, "avg_launch_ms": spmv_avg_ms, "launches": launches},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

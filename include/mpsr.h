@@ -225,6 +225,10 @@ typedef struct mpsr_ppe_options {
     double lambda_lap; /* Laplacian model constant at the Laplacian radius; 0 = computed */
     int precond;       /* mpsr_precond */
     int coarse_cells;  /* aggregate edge in cells for the two-level preconditioner; 0 = 4 */
+    int coarse_rebuild;/* rebuild the coarse operator every this many solves (or when the iteration
+                          count grows by a quarter over the one after the last rebuild); 0 = 10,
+                          1 = every solve.  In between, the last coarse inverse is applied to the
+                          current aggregates: the preconditioner stays symmetric positive definite. */
 } mpsr_ppe_options;
 
 typedef struct mpsr_create_info {
@@ -271,6 +275,10 @@ typedef struct mpsr_timings {
     int ppe_rows;            /* non-Dirichlet rows of the last PPE matrix */
     int ppe_fallbacks;       /* mixed solves that finished with the plain FP64 PCG */
     int ppe_coarse_rows;     /* aggregates of the two-level preconditioner, 0 = Jacobi was used */
+    int ppe_coarse_rebuilt;  /* 1 when this solve rebuilt the coarse operator */
+    double ppe_setup_ms;     /* coarse operator set-up of this solve (0 when reused; profiling on) */
+    int ppe_learned;         /* 1 when the network gave the warm start and preconditioner weights */
+    int ppe_learned_fallback;/* 1 when it lost to the plain solver after 5 iterations (plain restart) */
 } mpsr_timings;
 
 typedef struct mpsr_handle mpsr_handle;
@@ -379,6 +387,27 @@ mpsr_status mpsr_get_ppe_matrix(mpsr_handle *h, int *row_ptr, int *col, double *
                                 int cap, int *nnz);
 /* Per-phase HIP-event timing of the following steps (mpsr_get_timings); off by default. */
 mpsr_status mpsr_set_profiling(mpsr_handle *h, int on);
+
+/* ---- learned PPE warm start and preconditioner (no counterpart in the reference) ---------------
+ * A three-layer message-passing network over the PPE matrix graph (hidden width 128, message
+ * width 32, 16 input features), BF16 operands on MFMA with FP32 accumulation.  With
+ * ppe.warm_start = MPSR_WARM_LEARNED and weights set, every solve starts from x0 = s y0 and
+ * scales the diagonal part of the preconditioner by omega = 2 sigmoid(y1); after 5 iterations a
+ * residual larger than the plain solver's after 5 iterations (plain start, plain preconditioner,
+ * measured every 16 solves) restarts the solve the plain way (timings.ppe_learned_fallback).
+ * Parameters: mpsr_mpnn_param_count() FP32 values, flat, order U1 W1 V1 c1 U2 W2 V2 c2 U3 W3 V3
+ * c3 (row-major; U: F x 32, W: F x out, V: 32 x out, c: out).  The handle must be in PPE mode. */
+int mpsr_mpnn_param_count(void);
+mpsr_status mpsr_mpnn_set_weights(mpsr_handle *h, const float *w, int n);
+mpsr_status mpsr_mpnn_get_weights(mpsr_handle *h, float *w, int n);
+/* One training sample from the last solve (its system, its solution as the target): loss parts
+ * {total, supervised, residual, preconditioner} into parts[4], gradients into grads (may be
+ * NULL), then one Adam step of rate lr when update != 0. */
+mpsr_status mpsr_mpnn_train_step(mpsr_handle *h, double lr, double beta, int update, float *grads,
+                                 double *parts);
+/* Features [n][16] and outputs [n][2] of the network on the last system, slot order, and the
+ * pressure scale s. */
+mpsr_status mpsr_mpnn_rows(mpsr_handle *h, float *features, float *outputs, double *scale);
 
 /* ---- input decks and output frames (host only, no device use) --------------------------------
  * A deck is the reference's configuration file and the files it names, parsed as

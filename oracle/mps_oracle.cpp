@@ -790,7 +790,8 @@ template <typename R> struct Sim {
         for (int s = 0; s < n; ++s) {
             if (active[s] && diag[s] > 0.0) {
                 ppe_dir[s] = 0;
-                ppe_rhs[s] = alpha * ((double)out.dens[s] - n0) / n0;
+                // compression-only source: with the M-matrix A the solution is then >= 0
+                ppe_rhs[s] = alpha * std::max((double)out.dens[s] - n0, 0.0) / n0;
                 csr_diag[s] = diag[s];
             }
         }

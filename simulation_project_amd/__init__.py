@@ -76,6 +76,11 @@ def load_library(path=None):
         "mpsr_get_particle_count": (C.c_int, [vp, P(C.c_int)]),
         "mpsr_set_buckets": (C.c_int, [vp, P(abi.ParticlesC), P(C.c_int), P(C.c_int), C.c_int]),
         "mpsr_bucket_addition": (C.c_int, [vp, C.c_int, P(C.c_int)]),
+        "mpsr_mpnn_param_count": (C.c_int, []),
+        "mpsr_mpnn_set_weights": (C.c_int, [vp, P(C.c_float), C.c_int]),
+        "mpsr_mpnn_get_weights": (C.c_int, [vp, P(C.c_float), C.c_int]),
+        "mpsr_mpnn_train_step": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, P(C.c_float), P(C.c_double)]),
+        "mpsr_mpnn_rows": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_double)]),
         "mpsr_deck_load": (C.c_int, [C.c_char_p, P(vp)]),
         "mpsr_deck_free": (None, [vp]),
         "mpsr_deck_last_error": (C.c_char_p, []),
@@ -167,6 +172,38 @@ class Solver:
         tot = (C.c_int * max(1, len(buckets)))(*[b[2] for b in buckets])
         self._check(self.lib.mpsr_set_buckets(self.h, arr, itv, tot, len(buckets)), "mpsr_set_buckets")
         self._buckets = bool(buckets)
+
+    # ---- learned warm start / preconditioner ----
+    def mpnn_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(self.lib.mpsr_mpnn_set_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_float)), len(w)),
+                    "mpsr_mpnn_set_weights")
+
+    def mpnn_get_weights(self):
+        n = self.lib.mpsr_mpnn_param_count()
+        w = np.zeros(n, np.float32)
+        self._check(self.lib.mpsr_mpnn_get_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_float)), n),
+                    "mpsr_mpnn_get_weights")
+        return w
+
+    def mpnn_train_step(self, lr=1e-3, beta=1.0, update=True, want_grads=False):
+        """One training sample from the last solve; returns (parts[4], grads or None)."""
+        parts = np.zeros(4)
+        g = np.zeros(self.lib.mpsr_mpnn_param_count(), np.float32) if want_grads else None
+        gp = g.ctypes.data_as(C.POINTER(C.c_float)) if want_grads else None
+        self._check(self.lib.mpsr_mpnn_train_step(self.h, lr, beta, 1 if update else 0, gp,
+                                                  parts.ctypes.data_as(C.POINTER(C.c_double))),
+                    "mpsr_mpnn_train_step")
+        return parts, g
+
+    def mpnn_rows(self):
+        """(features [n,16], outputs [n,2], s) of the network on the last system, slot order."""
+        X = np.zeros((self.n, 16), np.float32)
+        Y = np.zeros((self.n, 2), np.float32)
+        sc = C.c_double()
+        self._check(self.lib.mpsr_mpnn_rows(self.h, X.ctypes.data_as(C.POINTER(C.c_float)),
+                                            Y.ctypes.data_as(C.POINTER(C.c_float)), C.byref(sc)), "mpsr_mpnn_rows")
+        return X, Y, sc.value
 
     def bucket_addition(self, step):
         added = C.c_int()

@@ -262,8 +262,9 @@ def drum(lib, l0=2.5e-3, fill=0.4, rpm=30, radius=0.15, length=0.3, seed=DEFAULT
          n_r=16, n_x=36, steps=200, explicit_pressure=False):
     """Rotating drum: cylinder of radius 0.15 m and length 0.3 m along x (9984 triangles by
     default) spinning at rpm about its axis, filled to the given area fraction.  Lattice sites
-    are l0 apart, at l0, 2 l0, ... from the end caps and at least l0 inside the mantle.  l0 = 2.5e-3 m and l0 = 2.039e-3 m are the two sizes
-    quoted for this case (about 0.54M and 1M particles)."""
+    are l0 apart, at l0, 2 l0, ... from the end caps (the last plane at least l0 from the far
+    cap) and at least l0 inside the mantle.  l0 = 2.5e-3 m gives 516,936 particles and
+    l0 = 2.012e-3 m gives 1,008,472 (the 1M size)."""
     yc = zc = radius
     # fill level h (relative to the axis) whose circular segment below holds `fill` of the disc
     lo_h, hi_h = -radius, radius
@@ -276,7 +277,8 @@ def drum(lib, l0=2.5e-3, fill=0.4, rpm=30, radius=0.15, length=0.3, seed=DEFAULT
         else:
             hi_h = h
     h = 0.5 * (lo_h + hi_h)
-    nx = int(round(length / l0)) - 1
+    # planes at l0, 2 l0, ... and at least l0 from the far cap as well
+    nx = int(math.floor(length / l0 + 1e-9)) - 1
     m = int(radius / l0)
     off = np.arange(-m, m + 1) * l0
     yy, zz = np.meshgrid(yc + off, zc + off, indexing="ij")

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include "mpsr_internal.h"
 #include "ppe.h"
+#include "mpnn.h"
 
 namespace mpsr {
 
@@ -58,6 +59,10 @@ struct EngineBase {
     virtual mpsr_status set_buckets(const mpsr_particles* b, const int* interval, const int* total, int count) = 0;
     virtual mpsr_status bucket_phase(int step, int* added) = 0;
     virtual int count() const = 0;
+    virtual mpsr_status mpnn_set(const float* w) = 0;
+    virtual mpsr_status mpnn_get(float* w) = 0;
+    virtual mpsr_status mpnn_train(double lr, double beta, int update, float* grads, double* parts) = 0;
+    virtual mpsr_status mpnn_rows(float* X, float* Y, double* scale) = 0;
     mpsr_gridbox gb{};
     mpsr_timings tm{};
     int profiling = 0;
@@ -313,6 +318,31 @@ template <typename R> struct Engine : EngineBase {
         return MPSR_NOERR;
     }
     int count() const override { return d.N; }
+    mpsr_status mpnn_set(const float* w) override {
+        HIPC(hipSetDevice(device));
+        if (!ppe) return fail(MPSR_PARAMERR, "the network needs a handle in PPE mode");
+        HIPC(hipStreamSynchronize(st));
+        mpsr_status s = ppe_mpnn_set(ppe, w, st);
+        return s == MPSR_NOERR ? s : fail(s, "network weights could not be set");
+    }
+    mpsr_status mpnn_get(float* w) override {
+        HIPC(hipSetDevice(device));
+        if (!ppe) return fail(MPSR_PARAMERR, "the network needs a handle in PPE mode");
+        mpsr_status s = ppe_mpnn_get(ppe, w, st);
+        return s == MPSR_NOERR ? s : fail(s, "no network weights set");
+    }
+    mpsr_status mpnn_train(double lr, double beta, int update, float* grads, double* parts) override {
+        HIPC(hipSetDevice(device));
+        if (!ppe) return fail(MPSR_PARAMERR, "the network needs a handle in PPE mode");
+        mpsr_status s = ppe_mpnn_train(ppe, d, lr, beta, update, grads, parts, st);
+        return s == MPSR_NOERR ? s : fail(s, "network training step needs weights and a solved system");
+    }
+    mpsr_status mpnn_rows(float* X, float* Y, double* scale) override {
+        HIPC(hipSetDevice(device));
+        if (!ppe) return fail(MPSR_PARAMERR, "the network needs a handle in PPE mode");
+        mpsr_status s = ppe_mpnn_rows(ppe, d, X, Y, scale, st);
+        return s == MPSR_NOERR ? s : fail(s, "network rows need weights and a solved system");
+    }
     mpsr_status bucket_phase(int step, int* added) override {
         HIPC(hipSetDevice(device));
         bool a = false;
@@ -624,6 +654,10 @@ template <typename R> struct Engine : EngineBase {
         t.ppe_rows = tm.ppe_rows;
         t.ppe_fallbacks = tm.ppe_fallbacks;
         t.ppe_coarse_rows = tm.ppe_coarse_rows;
+        t.ppe_coarse_rebuilt = tm.ppe_coarse_rebuilt;
+        t.ppe_setup_ms = tm.ppe_setup_ms;
+        t.ppe_learned = tm.ppe_learned;
+        t.ppe_learned_fallback = tm.ppe_learned_fallback;
         tm = t;
     }
 
@@ -910,6 +944,25 @@ mpsr_status mpsr_set_buckets(mpsr_handle* h, const mpsr_particles* b, const int*
     return h->e->set_buckets(b, interval, total, count);
 }
 mpsr_status mpsr_bucket_addition(mpsr_handle* h, int step, int* added) { GUARD(h); return h->e->bucket_phase(step, added); }
+int mpsr_mpnn_param_count(void) { return mpsr::mpnn_param_count(); }
+mpsr_status mpsr_mpnn_set_weights(mpsr_handle* h, const float* w, int n) {
+    GUARD(h);
+    if (!w || n != mpsr::mpnn_param_count()) return MPSR_PARAMERR;
+    return h->e->mpnn_set(w);
+}
+mpsr_status mpsr_mpnn_get_weights(mpsr_handle* h, float* w, int n) {
+    GUARD(h);
+    if (!w || n != mpsr::mpnn_param_count()) return MPSR_PARAMERR;
+    return h->e->mpnn_get(w);
+}
+mpsr_status mpsr_mpnn_train_step(mpsr_handle* h, double lr, double beta, int update, float* grads, double* parts) {
+    GUARD(h);
+    return h->e->mpnn_train(lr, beta, update, grads, parts);
+}
+mpsr_status mpsr_mpnn_rows(mpsr_handle* h, float* X, float* Y, double* scale) {
+    GUARD(h);
+    return h->e->mpnn_rows(X, Y, scale);
+}
 mpsr_status mpsr_get_triangles(mpsr_handle* h, mpsr_triangle* o, int n) { GUARD(h); return h->e->get_triangles(o, n); }
 mpsr_status mpsr_get_gridbox(mpsr_handle* h, mpsr_gridbox* o) {
     GUARD(h);

@@ -48,13 +48,21 @@ __global__ __launch_bounds__(256) void k_bucket_reuse(int N, int B, const int* _
     if (j < B) put_bucket_row(out, s, b, j);
 }
 
+// An appended slot holds no particle in the input arrays until the next relocation copies it
+// there: its input ID is -1, the other input fields 0.
 template <typename R>
-__global__ __launch_bounds__(256) void k_bucket_append(int N, int first, int count, PState<R> out, PState<R> b) {
+__global__ __launch_bounds__(256) void k_bucket_append(int N, int first, int count, PState<R> in, PState<R> out,
+                                                        PState<R> b) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     const int s = N + k;
     put_bucket_row(out, s, b, first + k);
     out.p[s] = R(0);
+    in.id[s] = -1;
+    in.type[s] = 0;
+    in.px[s] = R(0); in.py[s] = R(0); in.pz[s] = R(0);
+    in.vx[s] = R(0); in.vy[s] = R(0); in.vz[s] = R(0);
+    in.p[s] = R(0); in.n[s] = R(0); in.temp[s] = R(0); in.nu[s] = R(0); in.solid[s] = R(0);
 }
 
 template <typename R>
@@ -81,7 +89,8 @@ void launch_bucket_reuse(const Dev<R>& d, const PState<R>& b, int B, int* flag, 
 
 template <typename R> void launch_bucket_append(const Dev<R>& d, const PState<R>& b, int first, int count, hipStream_t st) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_bucket_append<R>, dim3(grid_for(count, 256)), dim3(256), 0, st, d.N, first, count, d.out, b);
+    hipLaunchKernelGGL(k_bucket_append<R>, dim3(grid_for(count, 256)), dim3(256), 0, st, d.N, first, count, d.in, d.out,
+                       b);
 }
 
 template <typename R> void launch_reset_walls(const Dev<R>& d, const TriD<R>* tris0, int step, hipStream_t st) {

@@ -24,5 +24,13 @@ template <typename R>
 mpsr_status ppe_get_matrix(PpeSolver<R>* s, const Dev<R>& d, int* ptr, int* col, double* val, double* diag, int cap,
                            int* nnz);
 template <typename R> void ppe_update_params(PpeSolver<R>* s, const mpsr_params& P);
+// the learned warm start / preconditioner network (parameters flat FP32)
+template <typename R> mpsr_status ppe_mpnn_set(PpeSolver<R>* s, const float* w, hipStream_t st);
+template <typename R> mpsr_status ppe_mpnn_get(PpeSolver<R>* s, float* w, hipStream_t st);
+template <typename R>
+mpsr_status ppe_mpnn_train(PpeSolver<R>* s, const Dev<R>& d, double lr, double beta, int update, float* grads,
+                           double* parts, hipStream_t st);
+template <typename R>
+mpsr_status ppe_mpnn_rows(PpeSolver<R>* s, const Dev<R>& d, float* X, float* Y, double* scale, hipStream_t st);
 
 }  // namespace mpsr

@@ -8,14 +8,19 @@
 // step uses an equation of state instead (DeviceProcess_Kernel.h:783-787).
 //
 // System, one row per sorted particle t:
-//   sum_j w_ij (p_t - p_j) = alpha (n*_t - n0s) / n0s,   alpha = lambda_lap n0_lap rho / (2 d dt^2)
+//   sum_j w_ij (p_t - p_j) = alpha max(n*_t - n0s, 0) / n0s,   alpha = lambda_lap n0_lap rho / (2 d dt^2)
 // with w_ij = GetWeight(re_lap, max(r_ij, re_lap 1e-3)) from the build-precision distance.  n0s
 // is the PND of the ideal lattice under this build's distance and weight (zero source for a
 // perfect lattice); lambda_lap and n0_lap are the exact-sqrt lattice values at re_lap.  Rows of
 // particles that are out of the zone, not binned, or at the free surface (n* <= beta n0) are
-// Dirichlet p = 0; the walls contribute no Laplacian term (Neumann), only their PND to n*.  The
-// pressure handed to the corrector is max(p, 0): the corrector is the reference's symmetric
-// (P_i + P_j) gradient, which would turn a suction pressure into an attraction.
+// Dirichlet p = 0; the walls contribute no Laplacian term (Neumann), only their PND to n*.  Only
+// compression is a source: rows just below the free surface (beta n0 < n* < n0s) would otherwise
+// be strong negative sources whose suction cancels the hydrostatic pressure until the fluid has
+// compressed for many steps, and the pressure then switches on abruptly (particles collapse at
+// l0 = 2e-3 m).  A is an M-matrix, so a non-negative source gives p >= 0.  The pressure handed to
+// the corrector is still max(p, 0) (an inexact solve can dip below 0): the corrector is the
+// reference's symmetric (P_i + P_j) gradient, which would turn a suction pressure into an
+// attraction.
 //
 // Storage: SELL-64 -- slices of 64 consecutive rows (one wave), columns stored slice-major so
 // lane i of a wave reads entry k of row i at slice_off + 64 k + i: every matrix load is a
@@ -40,6 +45,7 @@
 #include "mpsr_internal.h"
 #include "neighbors.h"
 #include "ppe.h"
+#include "mpnn.h"
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -102,7 +108,24 @@ template <typename R> struct PpeSolver {
     int* info_dev = nullptr;
     rocblas_handle blas = nullptr;
     SortScratch agg_scan{};
-    int two_level_ok = 0;        // the coarse factorisation of this solve succeeded
+    int two_level_ok = 0;        // this solve applies a coarse correction
+    int coarse_valid = 0;        // Ac holds the inverse built for the frozen aggregate map
+    int coarse_age = 0;          // solves since the last rebuild
+    int coarse_rebuilt = 0;      // this solve rebuilt it
+    int iters_at_build = 0;      // iterations of the first solve after the last rebuild
+    int iters_last = 0;
+    hipEvent_t setup_ev[2] = {nullptr, nullptr};
+    double setup_ms = 0.0;
+    // learned warm start and preconditioner weights
+    const double* pdiag = nullptr; // diagonal used by the preconditioner (diag, or diag / omega)
+    double* pdiag_buf = nullptr;   // [rows]
+    double* omega = nullptr;       // [rows]
+    double* xprev_s = nullptr;     // [rows] previous pressure in sorted order (network input)
+    double* xl = nullptr;          // [rows] learned warm start kept across the calibration probe
+    Mpnn* net = nullptr;
+    int learned = 0, learned_fallback = 0;
+    int calib_age = 0;             // solves until the next plain 5-iteration calibration
+    double plain5 = -1.0;          // ||r||/||b|| of the plain solver after 5 iterations
     int spmv_launches = 0;
     double spmv_ms = 0.0;
     double spmv_bytes = 0.0;
@@ -217,7 +240,7 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
         cols[base + k * 64 + lane] = self;
         vals[base + k * 64 + lane] = (VA)0;
     }
-    if (t < N) rhs[t] = dir[t] ? 0.0 : alpha * ((double)out.n[perm[t]] - n0) / n0;
+    if (t < N) rhs[t] = dir[t] ? 0.0 : alpha * fmax((double)out.n[perm[t]] - n0, 0.0) / n0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -409,6 +432,22 @@ __global__ __launch_bounds__(1024) void k_pcg_reduce(const double* __restrict__ 
     }
 }
 
+// previous pressure in sorted order (0 on Dirichlet rows or before the first solve)
+__global__ __launch_bounds__(PB) void k_xprev(int N, const int* __restrict__ perm, const int* __restrict__ dir,
+                                              const double* __restrict__ prev, int have, double* xs) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) xs[t] = (have && !dir[t]) ? prev[perm[t]] : 0.0;
+}
+__global__ __launch_bounds__(PB) void k_div_diag(int N, const double* __restrict__ diag, const double* __restrict__ om,
+                                                 double* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) out[t] = diag[t] / om[t];
+}
+__global__ __launch_bounds__(PB) void k_copy_d(int N, const double* __restrict__ a, double* b) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < N) b[t] = a[t];
+}
+
 __global__ __launch_bounds__(PB) void k_add_f2d(int N, double* x, const float* e) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < N) x[t] = x[t] + (double)e[t];
@@ -468,12 +507,18 @@ __global__ __launch_bounds__(PB) void k_agg_flag(int N, AggGeom g, const uint32_
     if (t >= N || dir[t]) return;
     flag[agg_full_of_cell(g, (int)skeys[t])] = 1;
 }
+// rows of aggregates outside the (possibly frozen) map get no coarse correction
 __global__ __launch_bounds__(PB) void k_agg_rows(int N, AggGeom g, const uint32_t* __restrict__ skeys,
-                                                  const int* __restrict__ dir, const int* __restrict__ map,
-                                                  int* agg_of_row) {
+                                                  const int* __restrict__ dir, const int* __restrict__ flag,
+                                                  const int* __restrict__ map, int* agg_of_row) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
-    agg_of_row[t] = dir[t] ? -1 : map[agg_full_of_cell(g, (int)skeys[t])];
+    if (dir[t]) {
+        agg_of_row[t] = -1;
+        return;
+    }
+    const int f = agg_full_of_cell(g, (int)skeys[t]);
+    agg_of_row[t] = flag[f] ? map[f] : -1;
 }
 __global__ __launch_bounds__(PB) void k_agg_list(int nfull, const int* __restrict__ flag, const int* __restrict__ map,
                                                   int* full_of) {
@@ -671,6 +716,12 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     ok &= dev_alloc(&s->scan.scan_tmp, s->scan.scan_tmp_cap);
     ok &= hipHostMalloc((void**)&s->scal_h, sizeof(double) * S_COUNT) == hipSuccess;
     ok &= dev_alloc(&s->nnz_dev, 2);
+    ok &= dev_alloc(&s->pdiag_buf, rows) && dev_alloc(&s->omega, rows) && dev_alloc(&s->xprev_s, rows) &&
+          dev_alloc(&s->xl, rows);
+    if (ok && opt.warm_start == MPSR_WARM_LEARNED) {
+        s->net = mpnn_create(rows);
+        ok = s->net != nullptr;
+    }
     if (!ok) {
         ppe_destroy(s);
         return nullptr;
@@ -732,8 +783,8 @@ static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hi
         if (rocblas_create_handle(&s->blas) != rocblas_status_success) return false;
     }
     if (rocblas_set_stream(s->blas, st) != rocblas_status_success) return false;
-    hipLaunchKernelGGL(k_agg_rows, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, g, d.skeys, s->dir, s->agg_map,
-                       s->agg_of_row);
+    hipLaunchKernelGGL(k_agg_rows, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, g, d.skeys, s->dir, s->agg_flag,
+                       s->agg_map, s->agg_of_row);
     const long long n2 = (long long)nc * nc;
     hipLaunchKernelGGL(k_fill_d, dim3(grid_for(n2, PB)), dim3(PB), 0, st, s->Ac, n2, 0.0);
     hipLaunchKernelGGL((k_coarse_assemble<VA>), dim3(nc), dim3(64), 0, st, nc, g, s->agg_full, s->agg_flag, s->agg_map,
@@ -769,7 +820,7 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
     hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, g, s->agg_full, d.cell_start, d.cell_end, s->dir,
                        r, s->rc, s->scal, 0);
     hipLaunchKernelGGL(k_coarse_mv, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 0);
-    hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->diag, s->agg_of_row, s->cc, z, p, 1, s->part,
+    hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, s->agg_of_row, s->cc, z, p, 1, s->part,
                        s->scal, 0);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_START);
     int done_iters = 0;
@@ -789,7 +840,7 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
             hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, g, s->agg_full, d.cell_start, d.cell_end,
                                s->dir, r, s->rc, s->scal, 1);
             hipLaunchKernelGGL(k_coarse_mv, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 1);
-            hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->diag, s->agg_of_row, s->cc, z, p, 0,
+            hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, s->agg_of_row, s->cc, z, p, 0,
                                s->part, s->scal, 1);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_BETA);
             hipLaunchKernelGGL(k_pcg_direction<V>, dim3(nb), dim3(PB), 0, st, N, z, p, s->scal);
@@ -818,11 +869,14 @@ template <typename R> void ppe_destroy(PpeSolver<R>* s) {
                   s->diag, s->rhs, s->x, s->prev, s->r, s->z, s->p0, s->p1, s->q, s->rf, s->zf, s->p0f,
                   s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp, s->nnz_dev, s->agg_flag,
                   s->agg_map, s->agg_full, s->agg_of_row, s->nc_dev, s->info_dev, s->Ac, s->rc, s->cc,
-                  s->agg_scan.scan_tmp};
+                  s->agg_scan.scan_tmp, s->pdiag_buf, s->omega, s->xprev_s, s->xl};
+    if (s->net) mpnn_destroy(s->net);
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (s->scal_h) (void)hipHostFree(s->scal_h);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->setup_ev)
+        if (e) (void)hipEventDestroy(e);
     if (s->blas) (void)rocblas_destroy_handle(s->blas);
     delete s;
 }
@@ -853,7 +907,7 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
         return -1;
     // the PCG start zeroes x only on the refinement path (x is the correction e there)
     const int zero_x = std::is_same<V, float>::value ? 1 : 0;
-    hipLaunchKernelGGL(k_pcg_start<V>, dim3(nb), dim3(PB), 0, st, N, r0, s->diag, r, z, pa, x, zero_x, s->part);
+    hipLaunchKernelGGL(k_pcg_start<V>, dim3(nb), dim3(PB), 0, st, N, r0, s->pdiag, r, z, pa, x, zero_x, s->part);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_START);
     int done_iters = 0;
     for (int it = 0; it < maxit; it += CHUNK) {
@@ -871,7 +925,7 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
                 s->ev_used += 2;
             }
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 1, s->scal, (int)RED_ALPHA);
-            hipLaunchKernelGGL(k_pcg_update<V>, dim3(nb), dim3(PB), 0, st, N, x, r, z, pa, q, s->diag, s->scal,
+            hipLaunchKernelGGL(k_pcg_update<V>, dim3(nb), dim3(PB), 0, st, N, x, r, z, pa, q, s->pdiag, s->scal,
                                s->part);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_BETA);
             hipLaunchKernelGGL(k_pcg_direction<V>, dim3(nb), dim3(PB), 0, st, N, z, pa, s->scal);
@@ -909,6 +963,37 @@ template <typename R> static double resid_pass(PpeSolver<R>* s, int N, hipStream
     return s->scal_h[S_RR];
 }
 
+// the assembled system and the particle geometry as the network sees them
+template <typename R> static MpnnSys mpnn_sys(const PpeSolver<R>* s, const Dev<R>& d) {
+    MpnnSys y;
+    y.N = d.N;
+    y.nslices = s->nslices;
+    y.rows_active = s->rows_real;
+    y.slice_off = s->slice_off;
+    y.slice_w = s->slice_w;
+    y.cols = s->cols;
+    y.cnt = s->cnt;
+    y.dir = s->dir;
+    y.v32 = s->mixed ? s->vals32 : nullptr;
+    y.v64 = s->mixed ? nullptr : s->vals64;
+    y.diag = s->diag;
+    y.rhs = s->rhs;
+    y.xprev = s->xprev_s;
+    y.xsol = s->x;
+    return y;
+}
+template <typename R> static MpnnGeo<R> mpnn_geo(const Dev<R>& d) {
+    MpnnGeo<R> g;
+    g.spos = d.spos;
+    g.svel = d.svel;
+    g.perm = d.perm;
+    g.stl_mag = d.stl_mag;
+    g.l0 = (double)d.c.l0;
+    g.dt = (double)d.c.dt;
+    g.re = (double)d.c.re_lap;
+    return g;
+}
+
 template <typename R>
 mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, int profile, hipStream_t st) {
     const int N = d.N;
@@ -917,6 +1002,8 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     if (profile && s->ev.empty()) {
         s->ev.resize(2 * 4096);
         for (auto& e : s->ev)
+            if (hipEventCreate(&e) != hipSuccess) return MPSR_GPUERR;
+        for (auto& e : s->setup_ev)
             if (hipEventCreate(&e) != hipSuccess) return MPSR_GPUERR;
     }
     const int nsl = (N + 63) / 64;
@@ -964,8 +1051,23 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     // ---- solve ----
     const double tol = opt.rel_tol > 0 ? opt.rel_tol : 1e-8;
     const int maxit = opt.max_iter > 0 ? opt.max_iter : 20000;
-    const int use_prev = (opt.warm_start == MPSR_WARM_PREVIOUS && s->have_prev) ? 1 : 0;
-    hipLaunchKernelGGL(k_warm, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.perm, s->dir, s->prev, use_prev, s->x);
+    hipLaunchKernelGGL(k_xprev, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.perm, s->dir, s->prev, s->have_prev,
+                       s->xprev_s);
+    s->pdiag = s->diag;
+    s->learned = opt.warm_start == MPSR_WARM_LEARNED && s->net && mpnn_has_weights(s->net) && N > 0;
+    s->learned_fallback = 0;
+    // the plain warm start: the previous pressure (learned and previous modes) or zero
+    const int use_prev = (opt.warm_start != MPSR_WARM_ZERO && s->have_prev) ? 1 : 0;
+    if (s->learned) {
+        const MpnnSys sys = mpnn_sys(s, d);
+        mpnn_features<R>(s->net, sys, mpnn_geo(d), st);
+        mpnn_forward(s->net, sys, st);
+        mpnn_apply(s->net, sys, s->x, s->omega, st);
+        hipLaunchKernelGGL(k_div_diag, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, s->diag, s->omega, s->pdiag_buf);
+        s->pdiag = s->pdiag_buf;
+    } else {
+        hipLaunchKernelGGL(k_warm, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.perm, s->dir, s->prev, use_prev, s->x);
+    }
     double bb = 0.0;
     double rr = resid_pass(s, N, st, &bb);
     if (rr < 0) return MPSR_GPUERR;
@@ -976,10 +1078,78 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->fallbacks = 0;
     s->why[0] = 0;
     s->two_level_ok = 0;
-    if (opt.precond == MPSR_PRECOND_TWO_LEVEL && bb > 0 && rr > target)
-        s->two_level_ok = s->mixed ? setup_two_level<R, float>(s, d, s->vals32, st)
-                                   : setup_two_level<R, double>(s, d, s->vals64, st);
+    s->coarse_rebuilt = 0;
+    s->setup_ms = 0.0;
+    if (opt.precond == MPSR_PRECOND_TWO_LEVEL && bb > 0 && rr > target) {
+        const int every = opt.coarse_rebuild > 0 ? opt.coarse_rebuild : 10;
+        const bool rebuild = !s->coarse_valid || s->coarse_age + 1 >= every ||
+                             (s->iters_at_build > 0 && 4 * s->iters_last > 5 * s->iters_at_build);
+        if (rebuild) {
+            if (profile) (void)hipEventRecord(s->setup_ev[0], st);
+            s->two_level_ok = s->mixed ? setup_two_level<R, float>(s, d, s->vals32, st)
+                                       : setup_two_level<R, double>(s, d, s->vals64, st);
+            if (profile) (void)hipEventRecord(s->setup_ev[1], st);
+            s->coarse_valid = s->two_level_ok;
+            s->coarse_age = 0;
+            s->coarse_rebuilt = 1;
+        } else {
+            // the frozen aggregate map and coarse inverse, applied to this solve's rows
+            const AggGeom g{d.c.gx, d.c.gy, d.c.gz, s->agg_edge, s->cgx, s->cgy, s->cgz};
+            hipLaunchKernelGGL(k_agg_rows, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, g, d.skeys, s->dir,
+                               s->agg_flag, s->agg_map, s->agg_of_row);
+            s->two_level_ok = 1;
+            s->coarse_age += 1;
+        }
+    }
     const bool two = s->two_level_ok != 0;
+    if (s->learned && bb > 0 && rr > target) {
+        // Five PCG iterations from the network's start with its preconditioner weights.  When the
+        // residual is then larger than the plain solver's after five iterations (measured from
+        // the plain start every 16 solves), the solve falls back to the plain start and
+        // preconditioner.
+        auto five = [&]() -> int {
+            if (s->mixed)
+                return two ? run_pcg2<R, float, double>(s, d, N, s->vals32, s->x, s->r, s->z, s->p0, s->q, s->r, 0.0, 5,
+                                                        0, st)
+                           : run_pcg<R, float, double>(s, N, s->vals32, s->x, s->r, s->z, s->p0, s->p1, s->q, s->r,
+                                                       0.0, 5, st);
+            return two ? run_pcg2<R, double, double>(s, d, N, s->vals64, s->x, s->r, s->z, s->p0, s->q, s->r, 0.0, 5, 0,
+                                                     st)
+                       : run_pcg<R, double, double>(s, N, s->vals64, s->x, s->r, s->z, s->p0, s->p1, s->q, s->r, 0.0, 5,
+                                                    st);
+        };
+        if (s->calib_age <= 0 || s->plain5 < 0) {
+            hipLaunchKernelGGL(k_copy_d, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, s->x, s->xl);
+            hipLaunchKernelGGL(k_warm, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.perm, s->dir, s->prev, use_prev,
+                               s->x);
+            s->pdiag = s->diag;
+            double bbp = 0.0;
+            if (resid_pass(s, N, st, &bbp) < 0) return MPSR_GPUERR;
+            const int itp = five();
+            const double rrp = resid_pass(s, N, st, &bbp);
+            if (rrp < 0) return MPSR_GPUERR;
+            s->plain5 = (itp >= 0 && bbp > 0) ? std::sqrt(rrp / bbp) : 1e300;
+            s->calib_age = 16;
+            hipLaunchKernelGGL(k_copy_d, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, s->xl, s->x);
+            s->pdiag = s->pdiag_buf;
+            rr = resid_pass(s, N, st, &bb);
+            if (rr < 0) return MPSR_GPUERR;
+        }
+        s->calib_age -= 1;
+        const int itl = five();
+        rr = resid_pass(s, N, st, &bb);
+        if (rr < 0) return MPSR_GPUERR;
+        if (itl < 0 || !(std::sqrt(rr / bb) <= s->plain5)) {
+            s->learned_fallback = 1;
+            hipLaunchKernelGGL(k_warm, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.perm, s->dir, s->prev, use_prev,
+                               s->x);
+            s->pdiag = s->diag;
+            rr = resid_pass(s, N, st, &bb);
+            if (rr < 0) return MPSR_GPUERR;
+        } else {
+            s->iters += itl;
+        }
+    }
     if (!s->mixed) {
         if (bb > 0 && rr > target) {
             // FP64 PCG from the warm start; r already holds b - A x
@@ -988,7 +1158,7 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
                                : run_pcg<R, double, double>(s, N, s->vals64, s->x, s->r, s->z, s->p0, s->p1, s->q,
                                                             s->r, target, maxit, st);
             if (it < 0) s->status = 1;
-            else s->iters = it;
+            else s->iters += it;
             s->passes = 1;
             rr = resid_pass(s, N, st, &bb);
             if (rr < 0) return MPSR_GPUERR;
@@ -1035,6 +1205,12 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
         }
     }
     s->relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    if (s->coarse_rebuilt && s->two_level_ok) s->iters_at_build = s->iters;
+    s->iters_last = s->iters;
+    if (profile && s->coarse_rebuilt) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, s->setup_ev[0], s->setup_ev[1]) == hipSuccess) s->setup_ms = ms;
+    }
     if (!std::isfinite(s->relres)) {
         s->status = 1;
         std::snprintf(s->why, sizeof(s->why), "non-finite residual (r.r = %.3e, b.b = %.3e)", rr, bb);
@@ -1079,10 +1255,72 @@ template <typename R> mpsr_status ppe_check(PpeSolver<R>* s, mpsr_timings* tm) {
     tm->ppe_initial_rel_residual = s->relres0;
     tm->ppe_fallbacks = s->fallbacks;
     tm->ppe_coarse_rows = s->two_level_ok ? s->nc : 0;
+    tm->ppe_coarse_rebuilt = s->coarse_rebuilt;
+    tm->ppe_setup_ms = s->setup_ms;
+    tm->ppe_learned = s->learned;
+    tm->ppe_learned_fallback = s->learned_fallback;
     return s->status == 0 ? MPSR_NOERR : MPSR_CALCERR;
 }
 
 template <typename R> const char* ppe_why(PpeSolver<R>* s) { return s->why; }
+
+template <typename R> mpsr_status ppe_mpnn_set(PpeSolver<R>* s, const float* w, hipStream_t st) {
+    if (!s->net) {
+        s->net = mpnn_create(((s->cap + 63) / 64) * 64);
+        if (!s->net) return MPSR_MEMERR;
+    }
+    return mpnn_set_weights(s->net, w, st);
+}
+template <typename R> mpsr_status ppe_mpnn_get(PpeSolver<R>* s, float* w, hipStream_t st) {
+    if (!s->net || !mpnn_has_weights(s->net)) return MPSR_PARAMERR;
+    return mpnn_get_weights(s->net, w, st);
+}
+// One training sample: the system of the last solve with its solution as the target.  Computes
+// the loss and all gradients; with update set, one Adam step of rate lr follows.
+template <typename R>
+mpsr_status ppe_mpnn_train(PpeSolver<R>* s, const Dev<R>& d, double lr, double beta, int update, float* grads,
+                           double* parts, hipStream_t st) {
+    if (!s->net || !mpnn_has_weights(s->net)) return MPSR_PARAMERR;
+    if (s->nslices <= 0 || d.N <= 0) return MPSR_PARAMERR;
+    const MpnnSys sys = mpnn_sys(s, d);
+    mpnn_features<R>(s->net, sys, mpnn_geo(d), st);
+    mpnn_forward(s->net, sys, st);
+    mpsr_status r = mpnn_loss_grad(s->net, sys, beta, parts, st);
+    if (r != MPSR_NOERR) return r;
+    if (grads) {
+        r = mpnn_get_grads(s->net, grads, st);
+        if (r != MPSR_NOERR) return r;
+    }
+    if (update) mpnn_adam(s->net, (float)lr, st);
+    return hipStreamSynchronize(st) == hipSuccess ? MPSR_NOERR : MPSR_GPUERR;
+}
+// Features and outputs of the network on the last system, slot order (rows of slots that are not
+// binned are 0), and the pressure scale s.
+template <typename R>
+mpsr_status ppe_mpnn_rows(PpeSolver<R>* s, const Dev<R>& d, float* X, float* Y, double* scale, hipStream_t st) {
+    if (!s->net || !mpnn_has_weights(s->net)) return MPSR_PARAMERR;
+    const MpnnSys sys = mpnn_sys(s, d);
+    mpnn_features<R>(s->net, sys, mpnn_geo(d), st);
+    mpnn_forward(s->net, sys, st);
+    const int N = d.N;
+    std::vector<float> xs((size_t)std::max(N, 1) * NN_F), ys((size_t)std::max(N, 1) * 2);
+    std::vector<int> perm(std::max(N, 1));
+    mpsr_status r = mpnn_get_rows(s->net, N, xs.data(), ys.data(), scale, st);
+    if (r != MPSR_NOERR) return r;
+    if (N > 0 && hipMemcpy(perm.data(), d.perm, sizeof(int) * N, hipMemcpyDeviceToHost) != hipSuccess)
+        return MPSR_GPUERR;
+    if (X) std::fill(X, X + (size_t)N * NN_F, 0.f);
+    if (Y) std::fill(Y, Y + (size_t)N * 2, 0.f);
+    for (int t = 0; t < N; ++t) {
+        const int slot = perm[t];
+        if (X) std::copy(&xs[(size_t)t * NN_F], &xs[(size_t)t * NN_F] + NN_F, X + (size_t)slot * NN_F);
+        if (Y) {
+            Y[2 * (size_t)slot] = ys[2 * (size_t)t];
+            Y[2 * (size_t)slot + 1] = ys[2 * (size_t)t + 1];
+        }
+    }
+    return MPSR_NOERR;
+}
 
 template <typename R> mpsr_status ppe_get(PpeSolver<R>* s, const Dev<R>& d, double* rhs, double* x, int* dir) {
     const int N = d.N;
@@ -1161,7 +1399,12 @@ namespace mpsr {
     template mpsr_status ppe_check<R>(PpeSolver<R>*, mpsr_timings*);                                                 \
     template const char* ppe_why<R>(PpeSolver<R>*);                                                                 \
     template mpsr_status ppe_get<R>(PpeSolver<R>*, const Dev<R>&, double*, double*, int*);                           \
-    template void ppe_update_params<R>(PpeSolver<R>*, const mpsr_params&);
+    template void ppe_update_params<R>(PpeSolver<R>*, const mpsr_params&);                                         \
+    template mpsr_status ppe_mpnn_set<R>(PpeSolver<R>*, const float*, hipStream_t);                                   \
+    template mpsr_status ppe_mpnn_get<R>(PpeSolver<R>*, float*, hipStream_t);                                         \
+    template mpsr_status ppe_mpnn_train<R>(PpeSolver<R>*, const Dev<R>&, double, double, int, float*, double*,       \
+                                           hipStream_t);                                                              \
+    template mpsr_status ppe_mpnn_rows<R>(PpeSolver<R>*, const Dev<R>&, float*, float*, double*, hipStream_t);
 MPSR_INSTANTIATE_P(float)
 MPSR_INSTANTIATE_P(double)
 }  // namespace mpsr
