@@ -229,6 +229,8 @@ typedef struct mpsr_ppe_options {
                           count grows by a quarter over the one after the last rebuild); 0 = 10,
                           1 = every solve.  In between, the last coarse inverse is applied to the
                           current aggregates: the preconditioner stays symmetric positive definite. */
+    int learned_omega; /* with MPSR_WARM_LEARNED: 1 = also scale the preconditioner's diagonal part
+                          by the network's omega (clamped to [0.5, 1.5]), 0 = warm start only */
 } mpsr_ppe_options;
 
 typedef struct mpsr_create_info {
@@ -279,6 +281,7 @@ typedef struct mpsr_timings {
     double ppe_setup_ms;     /* coarse operator set-up of this solve (0 when reused; profiling on) */
     int ppe_learned;         /* 1 when the network gave the warm start and preconditioner weights */
     int ppe_learned_fallback;/* 1 when it lost to the plain solver after 5 iterations (plain restart) */
+    double sphere_ms;        /* sphere phase (profiling on) */
 } mpsr_timings;
 
 typedef struct mpsr_handle mpsr_handle;
@@ -337,6 +340,27 @@ mpsr_status mpsr_get_particles(mpsr_handle *h, int which, mpsr_particles *out);
 /* = ParticleHostToDevice + ParticleNumberToConst (CGPUCellSet.cpp:893-952, DeviceProcess.cu:265-271). */
 mpsr_status mpsr_set_particles(mpsr_handle *h, const mpsr_particles *in);
 mpsr_status mpsr_get_triangles(mpsr_handle *h, mpsr_triangle *out, int n);
+/* Rigid spheres carried by the fluid (no counterpart in the reference): diameter d, density
+ * rho_s, Schiller-Naumann drag F = 3 pi mu d (u_f - u_s)(1 + 0.15 Re^0.687) (Re < 1000, else
+ * Cd = 0.44) with u_f the MPS-weighted fluid velocity within re_lap of the centre and
+ * mu = fluid density (params.Density) x fluid_viscosity.  The reaction -F is spread over the fluid
+ * particles in reach with the same weights (momentum-conserving), the spheres move under drag,
+ * gravity and buoyancy and are kept d/2 off the non-drag walls.  Runs every step between the
+ * drag and the predictor.  position/velocity/force: [3n], xyz interleaved. */
+typedef struct mpsr_spheres {
+    int n;
+    double *position;
+    double *velocity;
+    double *force;          /* output only: the drag of the last step */
+    double diameter;
+    double density;
+    double fluid_viscosity;
+} mpsr_spheres;
+mpsr_status mpsr_set_spheres(mpsr_handle *h, const mpsr_spheres *s);
+/* Fills the non-NULL arrays of out for min(out->n, sphere count) spheres; out->n = sphere count. */
+mpsr_status mpsr_get_spheres(mpsr_handle *h, mpsr_spheres *out);
+/* The sphere phase alone, for drivers that call the phases one by one. */
+mpsr_status mpsr_calc_spheres(mpsr_handle *h);
 /* Current slot count (m_ParticleNum): grows when bucket particles are appended. */
 mpsr_status mpsr_get_particle_count(mpsr_handle *h, int *n);
 /* Bucket inflow = the reference's multi-bucket set-up (LoadMultiBucket,

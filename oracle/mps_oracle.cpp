@@ -341,6 +341,12 @@ template <typename R> struct Sim {
     std::vector<Tri<R>> tris, tris0;
     std::vector<mpsr_tri_params> models;
     int cap = 0;
+    // rigid spheres with Schiller-Naumann drag (no reference counterpart)
+    std::vector<V3<R>> sph_pos, sph_vel, sph_force;
+    std::vector<R> sph_w;
+    std::vector<V3<R>> sph_dv;      // per slot: velocity change of the fluid from the spheres
+    R sph_d = 0, sph_rho = 0, sph_nu = 0;
+    std::vector<int> sph_cs, sph_ce, sph_perm;
     struct Bucket {
         Fields<R> f;
         int count = 0, interval = 1, total = 0, added = 0;
@@ -588,6 +594,7 @@ template <typename R> struct Sim {
             lap = mul(lap, in.nu[s]);
             acc = add(acc, lap);
             v0 = sub(add(v0, mul(acc, dt)), dragv);
+            if (!sph_pos.empty()) v0 = add(v0, sph_dv[s]);
             if (expl) {
                 R coeff = -(R)P.Dimension / ((R)P.ConstantParticleDensity * (R)P.Density);
                 acc = add(mul(art, coeff), mul(col, (R)0.01));
@@ -867,6 +874,126 @@ template <typename R> struct Sim {
         }
     }
 
+    // Schiller-Naumann drag force on a sphere of diameter d moving at relative velocity du
+    // through fluid (rho_f, nu_f): 3 pi mu d du (1 + 0.15 Re^0.687) for Re < 1000, else
+    // Cd = 0.44 on the frontal area
+    static V3<R> sn_force(V3<R> du, R d, R rho_f, R nu_f) {
+        const R mag = std::sqrt(du.x * du.x + du.y * du.y + du.z * du.z);
+        const R re = mag * d / nu_f;
+        R k;
+        if (re < (R)1000) k = (R)3 * (R)M_PI * rho_f * nu_f * d * ((R)1 + (R)0.15 * powd(re, (R)0.687));
+        else k = (R)0.5 * rho_f * (R)0.44 * ((R)M_PI * d * d / (R)4) * mag;
+        return mk(k * du.x, k * du.y, k * du.z);
+    }
+
+    // one sphere phase: bin the spheres, sample the fluid velocity at each centre with the MPS
+    // weight at re_lap, drag force, its reaction spread over the fluid particles in reach
+    // (momentum-conserving), then the spheres' own motion with gravity/buoyancy and wall contact
+    void spheres_phase() {
+        const int S = (int)sph_pos.size();
+        sph_dv.assign(n, mk<R>(0, 0, 0));
+        if (S == 0) return;
+        const R re = re_lap();
+        const R mind = (R)((double)re * 1e-3);
+        const R dt = (R)P.Dt;
+        const long long C = g.cells();
+        // 1. spheres by cell, stable by index
+        std::vector<long long> key(S);
+        for (int k = 0; k < S; ++k) {
+            int ci, cj, ck;
+            key[k] = C;
+            if (cell_coords(g, sph_pos[k], ci, cj, ck)) {
+                const long long m = cell_id(g, ci, cj, ck);
+                if (m >= 0 && m < C) key[k] = m;
+            }
+        }
+        sph_perm.resize(S);
+        for (int k = 0; k < S; ++k) sph_perm[k] = k;
+        std::stable_sort(sph_perm.begin(), sph_perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+        sph_cs.assign(C, -1); sph_ce.assign(C, -1);
+        for (int t = 0; t < S; ++t) {
+            const long long m = key[sph_perm[t]];
+            if (m == C) continue;
+            if (t == 0 || key[sph_perm[t - 1]] != m) sph_cs[m] = t;
+            if (t == S - 1 || key[sph_perm[t + 1]] != m) sph_ce[m] = t + 1;
+        }
+        // 2. sampling and force
+        sph_force.assign(S, mk<R>(0, 0, 0));
+        sph_w.assign(S, 0);
+        for (int k = 0; k < S; ++k) {
+            const V3<R> x = sph_pos[k];
+            R w = 0;
+            V3<R> u = mk<R>(0, 0, 0);
+            for_candidates(-1, x, [&](int q) {
+                R dd = dist(x, in.pos[q]);
+                if (dd < re) {
+                    if (dd < mind) dd = mind;
+                    const R wq = weight(re, dd);
+                    w = w + wq;
+                    u = add(u, mul(in.vel[q], wq));
+                }
+            });
+            sph_w[k] = w;
+            if (w > 0) {
+                const V3<R> uf = mk(u.x / w, u.y / w, u.z / w);
+                sph_force[k] = sn_force(sub(uf, sph_vel[k]), sph_d, (R)P.Density, sph_nu);
+            }
+        }
+        // 3. reaction on the fluid: dv_j = -F_s (w_js / W_s) dt / m_f, spheres in canonical order
+        const R mf = (R)P.Density * l0() * l0() * l0();
+#pragma omp parallel for schedule(static)
+        for (int s = 0; s < n; ++s) {
+            if (out.id[s] < 0 || key[s] == (int)C) continue;
+            const V3<R> x = in.pos[s];
+            int ci, cj, ck;
+            if (!cell_coords(g, x, ci, cj, ck)) continue;
+            V3<R> dv = mk<R>(0, 0, 0);
+            for (int kk = -1; kk <= 1; ++kk)
+                for (int jj = -1; jj <= 1; ++jj)
+                    for (int ii = -1; ii <= 1; ++ii) {
+                        const long long m = cell_id(g, ci + ii, cj + jj, ck + kk);
+                        if (m < 0 || m >= C || sph_cs[m] < 0) continue;
+                        for (int t = sph_cs[m]; t < sph_ce[m]; ++t) {
+                            const int k = sph_perm[t];
+                            R dd = dist(x, sph_pos[k]);
+                            if (dd < re && sph_w[k] > 0) {
+                                if (dd < mind) dd = mind;
+                                const R c = -(weight(re, dd) / sph_w[k]) * dt / mf;
+                                dv = add(dv, mul(sph_force[k], c));
+                            }
+                        }
+                    }
+            sph_dv[s] = dv;
+        }
+        // 4. sphere motion: drag, gravity and buoyancy; contact with the nearest non-drag wall
+        const R ms = sph_rho * (R)M_PI * sph_d * sph_d * sph_d / (R)6;
+        const R buoy = (R)1 - (R)P.Density / sph_rho;
+        const V3<R> grav = mk((R)P.GravityConstant.x, (R)P.GravityConstant.y, (R)P.GravityConstant.z);
+        const R half = sph_d / (R)2;
+        for (int k = 0; k < S; ++k) {
+            const V3<R> a = add(mul(sph_force[k], (R)1 / ms), mul(grav, buoy));
+            V3<R> v = add(sph_vel[k], mul(a, dt));
+            V3<R> x = add(sph_pos[k], mul(v, dt));
+            R best = (R)1e8;
+            V3<R> bdir = mk<R>(0, 0, 0);
+            if (!tcount.empty())   // triangle lists of this step's registration
+                for_triangles(x, [&](int t) {
+                    if (models[tris[t].model].isDrag) return;
+                    V3<R> dv2;
+                    const R dd = tri_distance(tris[t], x, &dv2);
+                    if (dd < best) { best = dd; bdir = dv2; }
+                });
+            if (best < half && best > 0) {
+                const V3<R> nrm = mk(-bdir.x / best, -bdir.y / best, -bdir.z / best);
+                x = add(x, mul(nrm, half - best));
+                const R vn = v.x * nrm.x + v.y * nrm.y + v.z * nrm.z;
+                if (vn < 0) v = sub(v, mul(nrm, vn));
+            }
+            sph_vel[k] = v;
+            sph_pos[k] = x;
+        }
+    }
+
     // Kern :913-925 ResetWallPosition: linearly moving models back to their initial triangles
     void reset_walls(int step) {
         for (size_t t = 0; t < tris.size(); ++t) {
@@ -922,6 +1049,7 @@ template <typename R> struct Sim {
         register_triangles();
         stl_distance();
         if (!drag.empty()) drag_effect();
+        spheres_phase();
         explicit_step();
         relocate();
         register_particles();
@@ -1106,6 +1234,30 @@ template <typename R> static void set_buckets(Sim<R>* S, const mpsr_particles* b
         S->buckets.push_back(B);
     }
 }
+template <typename R> static void set_spheres(Sim<R>* S, int n, const double* pos, const double* vel, double d,
+                                               double rho, double nu) {
+    S->sph_pos.resize(n); S->sph_vel.resize(n);
+    for (int k = 0; k < n; ++k) {
+        S->sph_pos[k] = mk((R)pos[3 * k], (R)pos[3 * k + 1], (R)pos[3 * k + 2]);
+        S->sph_vel[k] = vel ? mk((R)vel[3 * k], (R)vel[3 * k + 1], (R)vel[3 * k + 2]) : mk<R>(0, 0, 0);
+    }
+    S->sph_d = (R)d; S->sph_rho = (R)rho; S->sph_nu = (R)nu;
+    S->sph_force.assign(n, mk<R>(0, 0, 0));
+}
+template <typename R> static void get_spheres(Sim<R>* S, double* pos, double* vel, double* force) {
+    for (size_t k = 0; k < S->sph_pos.size(); ++k) {
+        const V3<R> a = S->sph_pos[k], b = S->sph_vel[k], f = S->sph_force[k];
+        if (pos) { pos[3 * k] = a.x; pos[3 * k + 1] = a.y; pos[3 * k + 2] = a.z; }
+        if (vel) { vel[3 * k] = b.x; vel[3 * k + 1] = b.y; vel[3 * k + 2] = b.z; }
+        if (force) { force[3 * k] = f.x; force[3 * k + 1] = f.y; force[3 * k + 2] = f.z; }
+    }
+}
+template <typename R> static void get_sphere_dv(Sim<R>* S, double* dv) {
+    for (int s = 0; s < S->n; ++s) {
+        const V3<R> v = s < (int)S->sph_dv.size() ? S->sph_dv[s] : V3<R>{0, 0, 0};
+        dv[3 * s] = v.x; dv[3 * s + 1] = v.y; dv[3 * s + 2] = v.z;
+    }
+}
 extern "C" {
 
 float orc_sqrt14(float x) { return sqrt14(x); }
@@ -1165,6 +1317,26 @@ void orc_pressure_ppe(void* p) { Handle* h = (Handle*)p; DISPATCH(h, pressure_pp
 void orc_gradient(void* p) { Handle* h = (Handle*)p; DISPATCH(h, gradient()); }
 void orc_translate(void* p) { Handle* h = (Handle*)p; DISPATCH(h, translate()); }
 void orc_rotate(void* p, int i) { Handle* h = (Handle*)p; DISPATCH(h, rotate(i)); }
+void orc_set_spheres(void* p, int n, const double* pos, const double* vel, double d, double rho, double nu) {
+    Handle* h = (Handle*)p;
+    if (h->d) set_spheres(h->d, n, pos, vel, d, rho, nu);
+    else set_spheres(h->f, n, pos, vel, d, rho, nu);
+}
+void orc_get_spheres(void* p, double* pos, double* vel, double* force) {
+    Handle* h = (Handle*)p;
+    if (h->d) get_spheres(h->d, pos, vel, force);
+    else get_spheres(h->f, pos, vel, force);
+}
+void orc_get_sphere_dv(void* p, double* dv) {
+    Handle* h = (Handle*)p;
+    if (h->d) get_sphere_dv(h->d, dv);
+    else get_sphere_dv(h->f, dv);
+}
+void orc_spheres_phase(void* p) { Handle* h = (Handle*)p; DISPATCH(h, spheres_phase()); }
+void orc_sn_force(double dux, double duy, double duz, double d, double rho, double nu, double* f) {
+    const V3<double> F = Sim<double>::sn_force(mk(dux, duy, duz), d, rho, nu);
+    f[0] = F.x; f[1] = F.y; f[2] = F.z;
+}
 void orc_set_buckets(void* p, const mpsr_particles* b, const int* itv, const int* tot, int count) {
     Handle* h = (Handle*)p;
     if (h->d) set_buckets(h->d, b, itv, tot, count);

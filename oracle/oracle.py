@@ -78,6 +78,11 @@ def load():
         "orc_bucket_addition": (C.c_int, [vp, C.c_int]),
         "orc_count": (C.c_int, [vp]),
         "orc_reset_walls": (None, [vp, C.c_int]),
+        "orc_set_spheres": (None, [vp, C.c_int, dp, dp, C.c_double, C.c_double, C.c_double]),
+        "orc_get_spheres": (None, [vp, dp, dp, dp]),
+        "orc_get_sphere_dv": (None, [vp, dp]),
+        "orc_spheres_phase": (None, [vp]),
+        "orc_sn_force": (None, [C.c_double] * 6 + [dp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -138,6 +143,28 @@ class Oracle:
         itv = (C.c_int * max(1, len(buckets)))(*[b[1] for b in buckets])
         tot = (C.c_int * max(1, len(buckets)))(*[b[2] for b in buckets])
         self.lib.orc_set_buckets(self.h, arr, itv, tot, len(buckets))
+
+    def set_spheres(self, sph):
+        """sph: cases.Spheres (positions [n,3], velocities [n,3], diameter, density, fluid viscosity)."""
+        self._sph_n = len(sph.position)
+        pos = np.ascontiguousarray(sph.position, np.float64)
+        vel = np.ascontiguousarray(sph.velocity, np.float64)
+        self.lib.orc_set_spheres(self.h, self._sph_n, _dp(pos), _dp(vel), sph.diameter, sph.density,
+                                 sph.fluid_viscosity)
+
+    def get_spheres(self):
+        n = getattr(self, "_sph_n", 0)
+        pos, vel, force = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        self.lib.orc_get_spheres(self.h, _dp(pos), _dp(vel), _dp(force))
+        return pos, vel, force
+
+    def sphere_dv(self):
+        dv = np.zeros((self.n, 3))
+        self.lib.orc_get_sphere_dv(self.h, _dp(dv))
+        return dv
+
+    def SpheresPhase(self):
+        self.lib.orc_spheres_phase(self.h)
 
     def bucket_addition(self, step):
         added = self.lib.orc_bucket_addition(self.h, step)

@@ -76,6 +76,9 @@ def load_library(path=None):
         "mpsr_get_particle_count": (C.c_int, [vp, P(C.c_int)]),
         "mpsr_set_buckets": (C.c_int, [vp, P(abi.ParticlesC), P(C.c_int), P(C.c_int), C.c_int]),
         "mpsr_bucket_addition": (C.c_int, [vp, C.c_int, P(C.c_int)]),
+        "mpsr_set_spheres": (C.c_int, [vp, P(abi.SpheresC)]),
+        "mpsr_get_spheres": (C.c_int, [vp, P(abi.SpheresC)]),
+        "mpsr_calc_spheres": (C.c_int, [vp]),
         "mpsr_mpnn_param_count": (C.c_int, []),
         "mpsr_mpnn_set_weights": (C.c_int, [vp, P(C.c_float), C.c_int]),
         "mpsr_mpnn_get_weights": (C.c_int, [vp, P(C.c_float), C.c_int]),
@@ -172,6 +175,26 @@ class Solver:
         tot = (C.c_int * max(1, len(buckets)))(*[b[2] for b in buckets])
         self._check(self.lib.mpsr_set_buckets(self.h, arr, itv, tot, len(buckets)), "mpsr_set_buckets")
         self._buckets = bool(buckets)
+
+    # ---- rigid spheres ----
+    def set_spheres(self, sph):
+        """sph: cases.Spheres."""
+        pos = np.ascontiguousarray(sph.position, np.float64)
+        vel = np.ascontiguousarray(sph.velocity, np.float64)
+        st = abi.SpheresC(len(pos), _dp(pos), _dp(vel), None, sph.diameter, sph.density, sph.fluid_viscosity)
+        self._check(self.lib.mpsr_set_spheres(self.h, C.byref(st)), "mpsr_set_spheres")
+        self._sph_n = len(pos)
+
+    def get_spheres(self):
+        """(positions, velocities, last drag forces), each [n, 3]."""
+        n = getattr(self, "_sph_n", 0)
+        pos, vel, force = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        st = abi.SpheresC(n, _dp(pos), _dp(vel), _dp(force), 0.0, 0.0, 0.0)
+        self._check(self.lib.mpsr_get_spheres(self.h, C.byref(st)), "mpsr_get_spheres")
+        return pos, vel, force
+
+    def SpheresPhase(self):
+        self._check(self.lib.mpsr_calc_spheres(self.h), "mpsr_calc_spheres")
 
     # ---- learned warm start / preconditioner ----
     def mpnn_set_weights(self, w):

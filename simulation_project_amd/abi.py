@@ -115,7 +115,13 @@ MPSR_PRECOND_JACOBI, MPSR_PRECOND_TWO_LEVEL = 0, 1
 class PpeOptions(C.Structure):
     _fields_ = [("solver", C.c_int), ("rel_tol", C.c_double), ("max_iter", C.c_int),
                 ("warm_start", C.c_int), ("n0_lap", C.c_double), ("lambda_lap", C.c_double),
-                ("precond", C.c_int), ("coarse_cells", C.c_int), ("coarse_rebuild", C.c_int)]
+                ("precond", C.c_int), ("coarse_cells", C.c_int), ("coarse_rebuild", C.c_int),
+                ("learned_omega", C.c_int)]
+
+
+class SpheresC(C.Structure):
+    _fields_ = [("n", C.c_int), ("position", _dp), ("velocity", _dp), ("force", _dp), ("diameter", C.c_double),
+                ("density", C.c_double), ("fluid_viscosity", C.c_double)]
 
 
 class CreateInfo(C.Structure):
@@ -136,7 +142,7 @@ class Timings(C.Structure):
                 ("spmv_launches", C.c_int), ("spmv_bytes", C.c_double), ("ppe_nnz", C.c_longlong),
                 ("ppe_rows", C.c_int), ("ppe_fallbacks", C.c_int),
                 ("ppe_coarse_rows", C.c_int), ("ppe_coarse_rebuilt", C.c_int), ("ppe_setup_ms", C.c_double),
-                ("ppe_learned", C.c_int), ("ppe_learned_fallback", C.c_int)]
+                ("ppe_learned", C.c_int), ("ppe_learned_fallback", C.c_int), ("sphere_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -219,6 +219,68 @@ class Case:
         return arr
 
 
+@dataclass
+class Spheres:
+    """Rigid spheres carried by the fluid through Schiller-Naumann drag."""
+    position: np.ndarray        # [n, 3]
+    velocity: np.ndarray        # [n, 3]
+    diameter: float
+    density: float
+    fluid_viscosity: float = 1e-6
+
+
+def drum_spheres(case, count=2000, diameter_l0=5.0, density=2500.0, seed=DEFAULT_SEED):
+    """`count` non-overlapping spheres of diameter diameter_l0 * l0 placed at random (SplitMix64
+    stream of `seed`) inside the fluid of a drum case: centres at least d/2 + l0 from the mantle
+    and the caps and d/2 below the free surface, centre distances at least d.  Fewer are returned
+    when the fluid cannot hold `count` of them after 200 draws per sphere."""
+    meta = case.meta
+    l0 = meta["l0"]
+    d = diameter_l0 * l0
+    R, L = meta["radius"], meta["length"]
+    yc = zc = R
+    h = meta["level"]
+    u = (_splitmix_uniform(seed ^ 0x5EED5, count * 200 * 3)).reshape(count * 200, 3)
+    cell = d
+    grid = {}
+    out = []
+    k = 0
+    for draw in u:
+        if len(out) >= count:
+            break
+        x = d / 2 + l0 + draw[0] * (L - d - 2 * l0)
+        ang = 2 * math.pi * draw[1]
+        rad = (R - d / 2 - l0) * math.sqrt(draw[2])
+        y, z = yc + rad * math.cos(ang), zc + rad * math.sin(ang)
+        if z - zc > h - d / 2:
+            continue
+        key = (int(x // cell), int(y // cell), int(z // cell))
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for q in grid.get((key[0] + dx, key[1] + dy, key[2] + dz), ()):
+                        if (q[0] - x) ** 2 + (q[1] - y) ** 2 + (q[2] - z) ** 2 < d * d:
+                            ok = False
+        if ok:
+            grid.setdefault(key, []).append((x, y, z))
+            out.append((x, y, z))
+        k += 1
+    pos = np.array(out, np.float64).reshape(-1, 3).astype(np.float32).astype(np.float64)
+    return Spheres(pos, np.zeros_like(pos), float(np.float32(d)), density, 1e-6)
+
+
+def _splitmix_uniform(seed, n):
+    """n uniform doubles in [0, 1) from a SplitMix64 stream."""
+    idx = np.arange(1, n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+
+
 def _finish(case, lib):
     """Lambda fit and automatic compute zone through the library's host helpers."""
     case.params.LambdaValue = lib.mpsr_calculate_lambda(C.c_float(case.params.InitialParticleDistance))
@@ -298,7 +360,8 @@ def drum(lib, l0=2.5e-3, fill=0.4, rpm=30, radius=0.15, length=0.3, seed=DEFAULT
     m = tri_params(rotating=True, rpm=rpm, dt=dt, center=(0.0, yc, zc), second=(length, yc, zc),
                    friction=1.0)
     case = Case("drum", p, _particles(pos, 1e-6), verts, np.zeros(len(verts), np.int32), [m],
-                steps=steps, meta={"l0": l0, "fill": fill, "rpm": rpm, "seed": seed})
+                steps=steps, meta={"l0": l0, "fill": fill, "rpm": rpm, "seed": seed, "radius": radius,
+                                   "length": length, "level": h})
     return _finish(case, lib)
 
 

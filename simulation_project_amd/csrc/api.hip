@@ -38,7 +38,7 @@ static mpsr_status fail(mpsr_status s, const std::string& m) {
         if (e_ != hipSuccess) return fail(MPSR_GPUERR, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum Phase { PH_BIN, PH_TRI, PH_STL, PH_DRAG, PH_EXPL, PH_RELOC, PH_PND, PH_PPE, PH_GRAD, PH_WALL, PH_COUNT };
+enum Phase { PH_BIN, PH_TRI, PH_STL, PH_DRAG, PH_EXPL, PH_RELOC, PH_PND, PH_PPE, PH_GRAD, PH_WALL, PH_SPH, PH_COUNT };
 
 struct EngineBase {
     virtual ~EngineBase() {}
@@ -63,6 +63,8 @@ struct EngineBase {
     virtual mpsr_status mpnn_get(float* w) = 0;
     virtual mpsr_status mpnn_train(double lr, double beta, int update, float* grads, double* parts) = 0;
     virtual mpsr_status mpnn_rows(float* X, float* Y, double* scale) = 0;
+    virtual mpsr_status set_spheres(const mpsr_spheres* s) = 0;
+    virtual mpsr_status get_spheres(mpsr_spheres* s) = 0;
     mpsr_gridbox gb{};
     mpsr_timings tm{};
     int profiling = 0;
@@ -213,6 +215,8 @@ template <typename R> struct Engine : EngineBase {
     int* bk_dead = nullptr;            // [1] dead-slot count
     int* bk_dead_h = nullptr;          // pinned copy of it
     TriD<R>* tris0 = nullptr;          // initial triangles (ResetWallPosition)
+    SphDev<R> sph{};                   // rigid spheres (Schiller-Naumann drag)
+    int sph_cap = 0;
     std::vector<hipEvent_t> pev;      // profiling events, two per phase launch
     std::vector<int> pev_phase;
     size_t pev_used = 0;
@@ -351,6 +355,69 @@ template <typename R> struct Engine : EngineBase {
         HIPC(hipGetLastError());
         HIPC(hipStreamSynchronize(st));
         if (added) *added = a ? 1 : 0;
+        return MPSR_NOERR;
+    }
+    mpsr_status set_spheres(const mpsr_spheres* in) override {
+        HIPC(hipSetDevice(device));
+        if (!in || in->n < 0 || (in->n > 0 && !in->position)) return fail(MPSR_PARAMERR, "spheres missing");
+        if (in->n > 0 && (!(in->diameter > 0) || !(in->density > 0) || !(in->fluid_viscosity > 0)))
+            return fail(MPSR_PARAMERR, "sphere diameter, density and fluid viscosity must be positive");
+        HIPC(hipStreamSynchronize(st));
+        const int S = in->n;
+        if (S > sph_cap) {
+            ALLOC(sph.pos, S); ALLOC(sph.vel, S); ALLOC(sph.force, S); ALLOC(sph.wsum, S);
+            ALLOC(sph.keys, S); ALLOC(sph.vals, S);
+            if (alloc_sort(sph.sort, S) != MPSR_NOERR) return MPSR_GPUERR;
+            if (!sph.cs) { ALLOC(sph.cs, d.c.ncell); ALLOC(sph.ce, d.c.ncell); }
+            sph_cap = S;
+        }
+        if (!d.sph_dv) ALLOC(d.sph_dv, d.cap);
+        std::vector<R4<R>> p(std::max(1, S)), v(std::max(1, S));
+        for (int k = 0; k < S; ++k) {
+            p[k] = R4<R>{(R)in->position[3 * k], (R)in->position[3 * k + 1], (R)in->position[3 * k + 2], R(0)};
+            v[k] = in->velocity ? R4<R>{(R)in->velocity[3 * k], (R)in->velocity[3 * k + 1], (R)in->velocity[3 * k + 2], R(0)}
+                                : R4<R>{R(0), R(0), R(0), R(0)};
+        }
+        if (S > 0) {
+            HIPC(hipMemcpy(sph.pos, p.data(), sizeof(R4<R>) * S, hipMemcpyHostToDevice));
+            HIPC(hipMemcpy(sph.vel, v.data(), sizeof(R4<R>) * S, hipMemcpyHostToDevice));
+            HIPC(hipMemset(sph.force, 0, sizeof(R4<R>) * S));
+        }
+        sph.S = S;
+        sph.d = (R)in->diameter;
+        sph.rho_s = (R)in->density;
+        sph.rho_f = (R)P.Density;
+        sph.nu_f = (R)in->fluid_viscosity;
+        const R l0 = (R)P.InitialParticleDistance;
+        sph.mf = (R)P.Density * l0 * l0 * l0;
+        d.c.has_sph = S > 0 ? 1 : 0;
+        return MPSR_NOERR;
+    }
+    mpsr_status get_spheres(mpsr_spheres* o) override {
+        HIPC(hipSetDevice(device));
+        if (!o) return MPSR_PARAMERR;
+        HIPC(hipStreamSynchronize(st));
+        const int n = std::min(o->n, sph.S);
+        std::vector<R4<R>> p(std::max(1, n)), v(std::max(1, n)), f(std::max(1, n));
+        if (n > 0) {
+            HIPC(hipMemcpy(p.data(), sph.pos, sizeof(R4<R>) * n, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(v.data(), sph.vel, sizeof(R4<R>) * n, hipMemcpyDeviceToHost));
+            HIPC(hipMemcpy(f.data(), sph.force, sizeof(R4<R>) * n, hipMemcpyDeviceToHost));
+        }
+        for (int k = 0; k < n; ++k) {
+            const R4<R>* src[3] = {&p[k], &v[k], &f[k]};
+            double* dst[3] = {o->position, o->velocity, o->force};
+            for (int a = 0; a < 3; ++a)
+                if (dst[a]) {
+                    dst[a][3 * k] = (double)src[a]->x;
+                    dst[a][3 * k + 1] = (double)src[a]->y;
+                    dst[a][3 * k + 2] = (double)src[a]->z;
+                }
+        }
+        o->n = sph.S;
+        o->diameter = (double)sph.d;
+        o->density = (double)sph.rho_s;
+        o->fluid_viscosity = (double)sph.nu_f;
         return MPSR_NOERR;
     }
     // Returns whether a bucket was added at this step (the wall translation is then skipped).
@@ -521,6 +588,12 @@ template <typename R> struct Engine : EngineBase {
         launch_drag(d, st);
         mark_end();
     }
+    void do_spheres() {
+        if (sph.S <= 0) return;
+        mark_begin(PH_SPH);
+        launch_spheres(d, sph, st);
+        mark_end();
+    }
     void do_explicit() {
         mark_begin(PH_EXPL);
         launch_explicit(d, st);
@@ -577,6 +650,7 @@ template <typename R> struct Engine : EngineBase {
         do_tri();
         do_stl();
         if (d.ndrag > 0) do_drag();
+        do_spheres();
         do_explicit();
         do_relocate();
         do_bin();
@@ -636,6 +710,7 @@ template <typename R> struct Engine : EngineBase {
             t.tri_ms = acc[PH_TRI] / count;
             t.stl_ms = acc[PH_STL] / count;
             t.drag_ms = acc[PH_DRAG] / count;
+            t.sphere_ms = acc[PH_SPH] / count;
             t.explicit_ms = acc[PH_EXPL] / count;
             t.relocate_ms = acc[PH_RELOC] / count;
             t.pnd_ms = acc[PH_PND] / count;
@@ -689,6 +764,7 @@ template <typename R> struct Engine : EngineBase {
                 s = upload_rot(arg, 1);
                 if (s == MPSR_NOERR) { d.rot = rot_table; launch_rotate(d, st); }
                 break;
+            case 11: do_spheres(); break;
             default: s = fail(MPSR_PARAMERR, "unknown phase");
         }
         profiling = saved;
@@ -944,6 +1020,9 @@ mpsr_status mpsr_set_buckets(mpsr_handle* h, const mpsr_particles* b, const int*
     return h->e->set_buckets(b, interval, total, count);
 }
 mpsr_status mpsr_bucket_addition(mpsr_handle* h, int step, int* added) { GUARD(h); return h->e->bucket_phase(step, added); }
+mpsr_status mpsr_set_spheres(mpsr_handle* h, const mpsr_spheres* s) { GUARD(h); return h->e->set_spheres(s); }
+mpsr_status mpsr_get_spheres(mpsr_handle* h, mpsr_spheres* s) { GUARD(h); return h->e->get_spheres(s); }
+mpsr_status mpsr_calc_spheres(mpsr_handle* h) { GUARD(h); return h->e->phase(11, 0); }
 int mpsr_mpnn_param_count(void) { return mpsr::mpnn_param_count(); }
 mpsr_status mpsr_mpnn_set_weights(mpsr_handle* h, const float* w, int n) {
     GUARD(h);

@@ -51,7 +51,7 @@ bool mpnn_has_weights(const Mpnn* n);
 // features of the active rows (X, bf16) and the scale s
 template <typename R> void mpnn_features(Mpnn* n, const MpnnSys& sys, const MpnnGeo<R>& geo, hipStream_t st);
 void mpnn_forward(Mpnn* n, const MpnnSys& sys, hipStream_t st);
-// x0 = s y0 on the active rows (0 elsewhere), omega = 2 sigmoid(y1)
+// x0 = s y0 on the active rows (0 elsewhere), omega = 2 sigmoid(y1) clamped to [0.5, 1.5]
 void mpnn_apply(Mpnn* n, const MpnnSys& sys, double* x0, double* omega, hipStream_t st);
 // loss of the current system against sys.xsol and the gradients of all parameters; parts =
 // {total, supervised, residual, preconditioner} (synchronises)

@@ -489,7 +489,7 @@ __global__ __launch_bounds__(NB) void k_nn_apply(MpnnSys sys, const float* __res
         return;
     }
     if (x0) x0[t] = dscal[0] * (double)Y[(size_t)t * 16];
-    if (omega) omega[t] = 2.0 / (1.0 + std::exp(-(double)Y[(size_t)t * 16 + 1]));
+    if (omega) omega[t] = fmin(1.5, fmax(0.5, 2.0 / (1.0 + std::exp(-(double)Y[(size_t)t * 16 + 1]))));
 }
 
 // ---------------------------------------------------------------------------------------------

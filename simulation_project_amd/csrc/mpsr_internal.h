@@ -67,6 +67,7 @@ template <typename R> struct Consts {
     R re_lap, mind_lap, lapc, rc, art_p, art_coeff, mu_wall;
     R gx_acc, gy_acc, gz_acc;
     int expl, has_drag;
+    int has_sph;       // spheres present: the predictor adds their reaction
     // PND / EOS, Kern :702-801
     R re_grad, mind_grad, eos_coef, n0, beta_n0;
     // gradient corrector, Kern :804-912
@@ -244,6 +245,7 @@ template <typename R> struct Dev {
     R4<R>* stl_dir = nullptr;
     int* stl_id = nullptr;
     R4<R>* drag_acc = nullptr;
+    R4<R>* sph_dv = nullptr;          // [cap] sorted: fluid velocity change from the spheres
     R* drag_temp = nullptr;
     // radix-sort ping-pong buffers: one set for the particle keys (the sorted keys and the
     // permutation may end up in them and must live until the next binning), one for the
@@ -268,6 +270,22 @@ template <typename R> void launch_stl_distance(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_drag(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_translate(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_rotate(const Dev<R>& d, hipStream_t st);
+// spheres.hip: rigid spheres with Schiller-Naumann drag
+template <typename R> struct SphDev {
+    int S = 0;
+    R4<R>* pos = nullptr;              // [S]
+    R4<R>* vel = nullptr;              // [S]
+    R4<R>* force = nullptr;            // [S] drag of the last phase
+    R* wsum = nullptr;                 // [S] sum of the fluid weights at the centre
+    uint32_t* keys = nullptr;          // [S]
+    int* vals = nullptr;               // [S]
+    SortScratch sort{};
+    const int* perm = nullptr;         // sorted -> sphere index
+    int* cs = nullptr;                 // [ncell]
+    int* ce = nullptr;                 // [ncell]
+    R d = 0, rho_s = 0, rho_f = 0, nu_f = 0, mf = 0;
+};
+template <typename R> void launch_spheres(Dev<R>& d, SphDev<R>& s, hipStream_t st);
 // inflow.hip
 template <typename R>
 void launch_bucket_reuse(const Dev<R>& d, const PState<R>& b, int B, int* flag, int* rank, int* dead, hipStream_t st);

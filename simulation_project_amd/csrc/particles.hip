@@ -101,7 +101,8 @@ __global__ __launch_bounds__(256) void k_explicit(int N, Consts<R> c, const int*
                                                    const R4<R>* __restrict__ svel, const int* __restrict__ cs,
                                                    const int* __restrict__ ce, const R* __restrict__ stl_mag,
                                                    const R4<R>* __restrict__ stl_dir, const int* __restrict__ stl_id,
-                                                   const R4<R>* __restrict__ drag_acc, const TriD<R>* __restrict__ tris,
+                                                   const R4<R>* __restrict__ drag_acc, const R4<R>* __restrict__ sph_dv,
+                                                   const TriD<R>* __restrict__ tris,
                                                    const mpsr_tri_params* __restrict__ models) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
@@ -175,6 +176,12 @@ __global__ __launch_bounds__(256) void k_explicit(int N, Consts<R> c, const int*
     R vx = (V0.x + accx * c.dt) - dvx;
     R vy = (V0.y + accy * c.dt) - dvy;
     R vz = (V0.z + accz * c.dt) - dvz;
+    if (c.has_sph) {
+        const R4<R> Sd = sph_dv[t];
+        vx = vx + Sd.x;
+        vy = vy + Sd.y;
+        vz = vz + Sd.z;
+    }
     if (c.expl) {
         accx = ax * c.art_coeff + qx * (R)0.01;
         accy = ay * c.art_coeff + qy * (R)0.01;
@@ -376,8 +383,8 @@ template <typename R> void launch_bin_ranges(Dev<R>& d, hipStream_t st) {
 template <typename R> void launch_explicit(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL(k_explicit<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.perm, d.in, d.out,
-                       d.spos, d.svel, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id, d.drag_acc, d.tris,
-                       d.models);
+                       d.spos, d.svel, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id, d.drag_acc, d.sph_dv,
+                       d.tris, d.models);
 }
 template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;

@@ -126,6 +126,8 @@ template <typename R> struct PpeSolver {
     int learned = 0, learned_fallback = 0;
     int calib_age = 0;             // solves until the next plain 5-iteration calibration
     double plain5 = -1.0;          // ||r||/||b|| of the plain solver after 5 iterations
+    int probe_at = 0;              // next PCG: check r.r after this many iterations (0 = no check)
+    double probe_rr = 0.0;         // ... and return -2 when it exceeds this
     int spmv_launches = 0;
     double spmv_ms = 0.0;
     double spmv_bytes = 0.0;
@@ -824,10 +826,13 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
                        s->scal, 0);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_START);
     int done_iters = 0;
-    for (int it = 0; it < maxit; it += CHUNK) {
+    const int probe = s->probe_at;
+    s->probe_at = 0;
+    for (int it = 0; it < maxit;) {
+        const int len = (probe > 0 && it == 0) ? std::min(probe, CHUNK) : CHUNK;
         const size_t chunk_ev0 = s->ev_used;
         const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
-        for (int k = 0; k < CHUNK; ++k) {
+        for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
             hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, N, s->slice_off, s->slice_w, s->cols,
                                vals, s->diag, p, q, s->scal, s->part);
@@ -849,10 +854,12 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
             std::snprintf(s->why, sizeof(s->why), "device error while reading the PCG scalars");
             return -1;
         }
-        const int live = std::max(0, std::min(CHUNK, (int)s->scal_h[S_ITERS] - done_iters));
+        const int live = std::max(0, std::min(len, (int)s->scal_h[S_ITERS] - done_iters));
         if (timed) s->ev_used = chunk_ev0 + 2 * (size_t)live;
         done_iters = (int)s->scal_h[S_ITERS];
+        it += len;
         if (s->scal_h[S_DONE] != 0.0) break;
+        if (probe > 0 && it == probe && s->scal_h[S_RR] > s->probe_rr) return -2;
     }
     if (s->scal_h[S_DONE] == 3.0) {
         std::snprintf(s->why, sizeof(s->why),
@@ -910,13 +917,17 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
     hipLaunchKernelGGL(k_pcg_start<V>, dim3(nb), dim3(PB), 0, st, N, r0, s->pdiag, r, z, pa, x, zero_x, s->part);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_START);
     int done_iters = 0;
-    for (int it = 0; it < maxit; it += CHUNK) {
+    const int probe = s->probe_at;
+    s->probe_at = 0;
+    for (int it = 0; it < maxit;) {
         // SpMV launches are bracketed by events while the pool lasts; after the chunk, the
         // events of launches that found the solve already converged (and returned at once) are
-        // dropped, so the mean covers working launches only
+        // dropped, so the mean covers working launches only.  With a probe, the first chunk is
+        // `probe` iterations long and r.r is checked after it.
+        const int len = (probe > 0 && it == 0) ? std::min(probe, CHUNK) : CHUNK;
         const size_t chunk_ev0 = s->ev_used;
         const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
-        for (int k = 0; k < CHUNK; ++k) {
+        for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
             hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, N, s->slice_off, s->slice_w, s->cols,
                                vals, s->diag, pa, q, s->scal, s->part);
@@ -934,10 +945,12 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
             std::snprintf(s->why, sizeof(s->why), "device error while reading the PCG scalars");
             return -1;
         }
-        const int live = std::max(0, std::min(CHUNK, (int)s->scal_h[S_ITERS] - done_iters));
+        const int live = std::max(0, std::min(len, (int)s->scal_h[S_ITERS] - done_iters));
         if (timed) s->ev_used = chunk_ev0 + 2 * (size_t)live;
         done_iters = (int)s->scal_h[S_ITERS];
+        it += len;
         if (s->scal_h[S_DONE] != 0.0) break;
+        if (probe > 0 && it == probe && s->scal_h[S_RR] > s->probe_rr) return -2;
     }
     if (s->scal_h[S_DONE] == 3.0) {
         std::snprintf(s->why, sizeof(s->why),
@@ -1062,9 +1075,12 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
         const MpnnSys sys = mpnn_sys(s, d);
         mpnn_features<R>(s->net, sys, mpnn_geo(d), st);
         mpnn_forward(s->net, sys, st);
-        mpnn_apply(s->net, sys, s->x, s->omega, st);
-        hipLaunchKernelGGL(k_div_diag, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, s->diag, s->omega, s->pdiag_buf);
-        s->pdiag = s->pdiag_buf;
+        mpnn_apply(s->net, sys, s->x, opt.learned_omega ? s->omega : nullptr, st);
+        if (opt.learned_omega) {
+            hipLaunchKernelGGL(k_div_diag, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, s->diag, s->omega,
+                               s->pdiag_buf);
+            s->pdiag = s->pdiag_buf;
+        }
     } else {
         hipLaunchKernelGGL(k_warm, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.perm, s->dir, s->prev, use_prev, s->x);
     }
@@ -1103,10 +1119,8 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     }
     const bool two = s->two_level_ok != 0;
     if (s->learned && bb > 0 && rr > target) {
-        // Five PCG iterations from the network's start with its preconditioner weights.  When the
-        // residual is then larger than the plain solver's after five iterations (measured from
-        // the plain start every 16 solves), the solve falls back to the plain start and
-        // preconditioner.
+        // Calibration, every 16 solves: five PCG iterations from the plain start with the plain
+        // preconditioner give the residual the network's start has to beat after five iterations.
         auto five = [&]() -> int {
             if (s->mixed)
                 return two ? run_pcg2<R, float, double>(s, d, N, s->vals32, s->x, s->r, s->z, s->p0, s->q, s->r, 0.0, 5,
@@ -1131,32 +1145,43 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
             s->plain5 = (itp >= 0 && bbp > 0) ? std::sqrt(rrp / bbp) : 1e300;
             s->calib_age = 16;
             hipLaunchKernelGGL(k_copy_d, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, s->xl, s->x);
-            s->pdiag = s->pdiag_buf;
+            s->pdiag = opt.learned_omega ? s->pdiag_buf : s->diag;
             rr = resid_pass(s, N, st, &bb);
             if (rr < 0) return MPSR_GPUERR;
         }
         s->calib_age -= 1;
-        const int itl = five();
-        rr = resid_pass(s, N, st, &bb);
-        if (rr < 0) return MPSR_GPUERR;
-        if (itl < 0 || !(std::sqrt(rr / bb) <= s->plain5)) {
-            s->learned_fallback = 1;
-            hipLaunchKernelGGL(k_warm, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.perm, s->dir, s->prev, use_prev,
-                               s->x);
-            s->pdiag = s->diag;
-            rr = resid_pass(s, N, st, &bb);
-            if (rr < 0) return MPSR_GPUERR;
-        } else {
-            s->iters += itl;
-        }
     }
+    // with the network, the first PCG checks its residual after 5 iterations against the plain
+    // solver's (same preconditioner family, plain start) and returns -2 when it is larger
+    auto arm_probe = [&]() {
+        if (s->learned && s->plain5 >= 0) {
+            s->probe_at = 5;
+            s->probe_rr = s->plain5 * s->plain5 * bb;
+        }
+    };
+    auto fall_back = [&]() -> bool {
+        s->learned_fallback = 1;
+        s->iters += 5;
+        hipLaunchKernelGGL(k_warm, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.perm, s->dir, s->prev, use_prev, s->x);
+        s->pdiag = s->diag;
+        rr = resid_pass(s, N, st, &bb);
+        return rr >= 0;
+    };
     if (!s->mixed) {
         if (bb > 0 && rr > target) {
             // FP64 PCG from the warm start; r already holds b - A x
-            const int it = two ? run_pcg2<R, double, double>(s, d, N, s->vals64, s->x, s->r, s->z, s->p0, s->q, s->r,
-                                                             target, maxit, 0, st)
-                               : run_pcg<R, double, double>(s, N, s->vals64, s->x, s->r, s->z, s->p0, s->p1, s->q,
-                                                            s->r, target, maxit, st);
+            auto pcg = [&]() {
+                return two ? run_pcg2<R, double, double>(s, d, N, s->vals64, s->x, s->r, s->z, s->p0, s->q, s->r,
+                                                         target, maxit, 0, st)
+                           : run_pcg<R, double, double>(s, N, s->vals64, s->x, s->r, s->z, s->p0, s->p1, s->q, s->r,
+                                                        target, maxit, st);
+            };
+            arm_probe();
+            int it = pcg();
+            if (it == -2) {
+                if (!fall_back()) return MPSR_GPUERR;
+                it = pcg();
+            }
             if (it < 0) s->status = 1;
             else s->iters += it;
             s->passes = 1;
@@ -1166,6 +1191,7 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     } else {
         // iterative refinement: FP64 residual, FP32 inner PCG with FP64 dots, FP64 update
         bool fallback = false;
+        arm_probe();
         while (bb > 0 && rr > target && s->passes < 16 && s->iters < maxit) {
             const double inner_rel = std::max(1e-6, 0.5 * tol * std::sqrt(bb / rr));
             const int it =
@@ -1173,6 +1199,10 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
                                                 inner_rel * inner_rel * rr, maxit - s->iters, 1, st)
                     : run_pcg<R, float, float>(s, N, s->vals32, s->ef, s->rf, s->zf, s->p0f, s->p1f, s->qf, s->r,
                                                inner_rel * inner_rel * rr, maxit - s->iters, st);
+            if (it == -2) {
+                if (!fall_back()) return MPSR_GPUERR;
+                continue;
+            }
             if (it < 0) {
                 fallback = true;
                 break;

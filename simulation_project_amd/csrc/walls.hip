@@ -13,6 +13,7 @@
 //   k_rotate           RotateTrianglePosition_Kernel (Kern :930-998); cos/sin of the step's
 //                      angle come from the host so the rotation is bit-reproducible
 #include "mpsr_internal.h"
+#include "walls_geom.h"
 
 namespace mpsr {
 
@@ -20,12 +21,6 @@ static constexpr int kCellMaxTriangles = 100;
 static constexpr int kMaxShrink = 4096;
 static constexpr int kMaxLineSteps = 1 << 20;
 
-template <typename R> struct P3 { R x, y, z; };
-template <typename R> __device__ inline P3<R> p3(R x, R y, R z) { return P3<R>{x, y, z}; }
-template <typename R> __device__ inline P3<R> psub(P3<R> a, P3<R> b) { return p3(a.x - b.x, a.y - b.y, a.z - b.z); }
-template <typename R> __device__ inline P3<R> padd(P3<R> a, P3<R> b) { return p3(a.x + b.x, a.y + b.y, a.z + b.z); }
-template <typename R> __device__ inline R pdot(P3<R> a, P3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-template <typename R> __device__ inline R pmag(P3<R> a) { return build_sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
 
 // ---------------------------------------------------------------------------------------------
 // triangle binning
@@ -168,102 +163,6 @@ template <typename R> void launch_tri_bin(Dev<R>& d, hipStream_t st) {
                        d.cell_tri_count);
 }
 
-// ---------------------------------------------------------------------------------------------
-// distance to a triangle, Utility_Inline.h:46-170
-// ---------------------------------------------------------------------------------------------
-template <typename R> __device__ inline R segment_ratio(P3<R> p1, P3<R> p2, P3<R> p3_) {
-    const P3<R> dv = psub(p2, p1);
-    R den = dv.x * dv.x + dv.y * dv.y + dv.z * dv.z;
-    const R num = (p3_.x - p1.x) * (p2.x - p1.x) + (p3_.y - p1.y) * (p2.y - p1.y) + (p3_.z - p1.z) * (p2.z - p1.z);
-    if (den == 0.0) den = (R)1e-5;
-    return num / den;
-}
-// CalcDistanceVectorLine: beyond the ends it returns L1 - P (u < 0) and L0 - P (u > 1)
-template <typename R> __device__ inline P3<R> line_vec(P3<R> l0, P3<R> l1, P3<R> p) {
-    const R u = segment_ratio(l0, l1, p);
-    P3<R> d = p3<R>(0, 0, 0);
-    if (u >= 0 && u <= 1) {
-        const P3<R> m = psub(l1, l0);
-        const P3<R> q = p3(l0.x + u * m.x, l0.y + u * m.y, l0.z + u * m.z);
-        d = psub(p, q);
-    }
-    if (u < 0) d = psub(l1, p);
-    if (u > 1) d = psub(l0, p);
-    return d;
-}
-template <typename R> __device__ inline R tri_distance(const TriD<R>& t, P3<R> p, P3<R>* out) {
-    const P3<R> v0 = p3(t.v[0], t.v[1], t.v[2]), v1 = p3(t.v[3], t.v[4], t.v[5]), v2 = p3(t.v[6], t.v[7], t.v[8]);
-    // CalcNormalDistance with the stored normal and the centroid divided by the double 3.0
-    const P3<R> n = p3(t.nrm[0], t.nrm[1], t.nrm[2]);
-    P3<R> cc = p3<R>(0, 0, 0);
-    cc.x += v0.x; cc.y += v0.y; cc.z += v0.z;
-    cc.x += v1.x; cc.y += v1.y; cc.z += v1.z;
-    cc.x += v2.x; cc.y += v2.y; cc.z += v2.z;
-    cc.x = (R)((double)cc.x / 3.0);
-    cc.y = (R)((double)cc.y / 3.0);
-    cc.z = (R)((double)cc.z / 3.0);
-    const R ln = n.x * cc.x + n.y * cc.y + n.z * cc.z;
-    const R rn = n.x * p.x + n.y * p.y + n.z * p.z;
-    const R den = n.x * n.x + n.y * n.y + n.z * n.z;
-    const R sden = build_sqrt(den);
-    P3<R> dv = p3<R>(0, 0, 0);
-    if (!(sden == 0.0)) {
-        const R coef = (rn - ln) / sden;
-        dv.x = coef / sden * n.x;
-        dv.y = coef / sden * n.y;
-        dv.z = coef / sden * n.z;
-    }
-    const P3<R> P = p3(p.x - dv.x, p.y - dv.y, p.z - dv.z);
-    const P3<R> e0 = psub(v2, v0), e1 = psub(v1, v0), e2 = psub(P, v0);
-    const R d00 = pdot(e0, e0), d01 = pdot(e0, e1), d02 = pdot(e0, e2), d11 = pdot(e1, e1), d12 = pdot(e1, e2);
-    const R inv = 1 / (d00 * d11 - d01 * d01);
-    const R u = (d11 * d02 - d01 * d12) * inv;
-    const R v = (d00 * d12 - d01 * d02) * inv;
-    if (u >= 0 && v >= 0 && u + v > 1) dv = line_vec(v1, v2, p);
-    if (u < 0) dv = line_vec(v0, v1, p);
-    if (v < 0) dv = line_vec(v0, v2, p);
-    *out = dv;
-    return pmag(dv);
-}
-
-// nearest triangle of the wanted kind (drag or not) over the 27-cell stencil
-template <typename R>
-__device__ inline int nearest_triangle(const Consts<R>& c, R x, R y, R z, const int* __restrict__ tstart,
-                                       const int* __restrict__ tcount, const int* __restrict__ tids,
-                                       const TriD<R>* __restrict__ tris, int want_drag, R* best, P3<R>* bdir) {
-    *best = (R)1e8;
-    *bdir = p3<R>(0, 0, 0);
-    int bid = -1;
-    int ci, cj, ck;
-    if (!cell_coords(c, x, y, z, ci, cj, ck)) return bid;
-    const long long cid = cell_of(c, ci, cj, ck);
-    if (cid < 0 || cid >= c.ncell) return bid;
-    const P3<R> p = p3(x, y, z);
-    for (int k = -1; k <= 1; ++k)
-        for (int j = -1; j <= 1; ++j)
-            for (int i = -1; i <= 1; ++i) {
-                const long long m = cell_of(c, ci + i, cj + j, ck + k);
-                if (m < 0 || m >= c.ncell) continue;
-                const int n = tcount[m];
-                const int a = tstart[m];
-                for (int e = 0; e < n; ++e) {
-                    const int tid = tids[a + e];
-                    const TriD<R>& tr = tris[tid];
-                    if (tr.is_drag != want_drag) continue;
-                    if (tid != bid) {
-                        P3<R> dv;
-                        const R dm = tri_distance(tr, p, &dv);
-                        if (dm < *best) {
-                            *best = dm;
-                            *bdir = dv;
-                            bid = tid;
-                        }
-                    }
-                }
-            }
-    return bid;
-}
-
 template <typename R> __device__ inline R4<R> mk4w(R x, R y, R z);
 template <> __device__ inline float4 mk4w<float>(float x, float y, float z) { return make_float4(x, y, z, 0.f); }
 template <> __device__ inline double4 mk4w<double>(double x, double y, double z) { return make_double4(x, y, z, 0.0); }
@@ -284,12 +183,6 @@ __global__ __launch_bounds__(256) void k_stl_distance(int N, Consts<R> c, const 
     stl_id[t] = bid;
 }
 
-// deterministic pow: exp(y log x) in double, rounded once
-template <typename R> __device__ inline R powd(R x, R y) {
-    if (y == R(0)) return R(1);
-    if (x == R(0)) return y > R(0) ? R(0) : (R)INFINITY;
-    return (R)exp((double)y * log((double)x));
-}
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_drag(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,

@@ -356,3 +356,21 @@ def test_driver_frames_match_oracle(lib, tmp_path):
             o.RotateTrianglePosition(i)
         else:
             o.step(i)
+
+
+@pytest.mark.parametrize("precision", [abi.MPSR_FP32, abi.MPSR_FP64])
+def test_spheres_identical(lib, precision):
+    """150 rigid spheres (d = 2 l0) moving through the drum's fluid with Schiller-Naumann drag,
+    20 steps: fluid state, sphere positions, velocities and forces identical to the oracle."""
+    case = cases.drum(lib, l0=8e-3)
+    sph = cases.drum_spheres(case, 150, diameter_l0=2.0)
+    sph.velocity[:] = 0.05
+    g, o = pair(case, precision=precision)
+    for s in (g, o):
+        s.set_spheres(sph)
+    g.step_many(0, 20)
+    o.step_many(0, 20)
+    same_state(g, o, label="spheres")
+    for a, b, name in zip(g.get_spheres(), o.get_spheres(), ("position", "velocity", "force")):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    assert np.abs(o.get_spheres()[2]).sum() > 0
