@@ -35,6 +35,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per SpMV launch of the mixed-precision solver on the default drum (l0 = 2.012e-3 m),
+# from two rocprofv3 counter passes (FETCH_SIZE scaled by the factor a plain copy kernel of known
+# size reads, WRITE_SIZE unscaled): profiles/k_pcg_spmv_drum1M_traffic.md
+SPMV_TRAFFIC_BYTES = {(2.012e-3, "mixed"): 1005.5e6}
 WEIGHTS = os.path.join(ROOT, "simulation_project_amd", "weights", "mpnn_drum.npy")
 
 
@@ -191,7 +195,8 @@ def main():
         "step_roofline": {"achieved": round(step_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(step_achieved / HBM_PEAK_GBS, 4), "bytes_per_step": step_bytes},
         "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": SPMV_TRAFFIC_BYTES.get((args.l0, kind)),
                      "bytes_per_launch": spmv_b, "avg_launch_ms": spmv_avg_ms, "launches": launches},
         "cpu_baseline": cpu,
     }

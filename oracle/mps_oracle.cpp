@@ -159,11 +159,39 @@ static float lambda_fit(float l0) {
     return a * l0 + b;
 }
 
-// deterministic pow: exp(y log x) in double, rounded once to the build precision
+// Deterministic pow, exp(y log x) in double and rounded once to the build precision, built only
+// from IEEE-exact operations (+ - * /, frexp, ldexp, floor) so that the device, whose libm
+// differs from the host's, computes the same bits:
+//   log x = e ln2 + 2 atanh(s), s = (m - 1)/(m + 1), m in [sqrt(1/2), sqrt(2)), series to s^23
+//   exp z = 2^k exp(r), z = k ln2 + r (ln2 split in a high part with 32 low mantissa bits zero
+//           and the rest, Cody-Waite), |r| <= ln2/2, Taylor series to r^15
+static double det_log(double x) {
+    int e = 0;
+    double m = std::frexp(x, &e);
+    if (m < 0.70710678118654752440) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+    double p = 1.0 / 23.0;
+    for (int k = 10; k >= 0; --k) p = p * s2 + 1.0 / (double)(2 * k + 1);
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    return (double)e * ln2_hi + ((double)e * ln2_lo + 2.0 * s * p);
+}
+static double det_exp(double z) {
+    if (z > 709.0) return INFINITY;
+    if (z < -745.0) return 0.0;
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    const double k = std::floor(z / 0.69314718055994530942 + 0.5);
+    const double r = (z - k * ln2_hi) - k * ln2_lo;
+    double p = 1.0;
+    for (int j = 15; j >= 1; --j) p = 1.0 + p * r / (double)j;
+    return std::ldexp(p, (int)k);
+}
 template <typename R> static inline R powd(R x, R y) {
     if (y == R(0)) return R(1);
     if (x == R(0)) return y > R(0) ? R(0) : (R)INFINITY;
-    return (R)std::exp((double)y * std::log((double)x));
+    return (R)det_exp((double)y * det_log((double)x));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -876,14 +904,15 @@ template <typename R> struct Sim {
 
     // Schiller-Naumann drag force on a sphere of diameter d moving at relative velocity du
     // through fluid (rho_f, nu_f): 3 pi mu d du (1 + 0.15 Re^0.687) for Re < 1000, else
-    // Cd = 0.44 on the frontal area
+    // Cd = 0.44 on the frontal area; evaluated in double and rounded once to R
     static V3<R> sn_force(V3<R> du, R d, R rho_f, R nu_f) {
-        const R mag = std::sqrt(du.x * du.x + du.y * du.y + du.z * du.z);
-        const R re = mag * d / nu_f;
-        R k;
-        if (re < (R)1000) k = (R)3 * (R)M_PI * rho_f * nu_f * d * ((R)1 + (R)0.15 * powd(re, (R)0.687));
-        else k = (R)0.5 * rho_f * (R)0.44 * ((R)M_PI * d * d / (R)4) * mag;
-        return mk(k * du.x, k * du.y, k * du.z);
+        const double ux = du.x, uy = du.y, uz = du.z, dd = d, rr = rho_f, nn = nu_f;
+        const double mag = std::sqrt(ux * ux + uy * uy + uz * uz);
+        const double re = mag * dd / nn;
+        double k;
+        if (re < 1000.0) k = 3.0 * M_PI * rr * nn * dd * (1.0 + 0.15 * det_exp(0.687 * det_log(re > 0 ? re : 1.0)) * (re > 0 ? 1.0 : 0.0));
+        else k = 0.5 * rr * 0.44 * (M_PI * dd * dd / 4.0) * mag;
+        return mk((R)(k * ux), (R)(k * uy), (R)(k * uz));
     }
 
     // one sphere phase: bin the spheres, sample the fluid velocity at each centre with the MPS
@@ -898,24 +927,24 @@ template <typename R> struct Sim {
         const R dt = (R)P.Dt;
         const long long C = g.cells();
         // 1. spheres by cell, stable by index
-        std::vector<long long> key(S);
+        std::vector<long long> skey(S);
         for (int k = 0; k < S; ++k) {
             int ci, cj, ck;
-            key[k] = C;
+            skey[k] = C;
             if (cell_coords(g, sph_pos[k], ci, cj, ck)) {
                 const long long m = cell_id(g, ci, cj, ck);
-                if (m >= 0 && m < C) key[k] = m;
+                if (m >= 0 && m < C) skey[k] = m;
             }
         }
         sph_perm.resize(S);
         for (int k = 0; k < S; ++k) sph_perm[k] = k;
-        std::stable_sort(sph_perm.begin(), sph_perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+        std::stable_sort(sph_perm.begin(), sph_perm.end(), [&](int a, int b) { return skey[a] < skey[b]; });
         sph_cs.assign(C, -1); sph_ce.assign(C, -1);
         for (int t = 0; t < S; ++t) {
-            const long long m = key[sph_perm[t]];
+            const long long m = skey[sph_perm[t]];
             if (m == C) continue;
-            if (t == 0 || key[sph_perm[t - 1]] != m) sph_cs[m] = t;
-            if (t == S - 1 || key[sph_perm[t + 1]] != m) sph_ce[m] = t + 1;
+            if (t == 0 || skey[sph_perm[t - 1]] != m) sph_cs[m] = t;
+            if (t == S - 1 || skey[sph_perm[t + 1]] != m) sph_ce[m] = t + 1;
         }
         // 2. sampling and force
         sph_force.assign(S, mk<R>(0, 0, 0));
@@ -1290,6 +1319,7 @@ float orc_tri_distance(const mpsr_triangle* T, float x, float y, float z, float*
     return m;
 }
 float orc_powd(float x, float y) { return powd<float>(x, y); }
+double orc_powd64(double x, double y) { return powd<double>(x, y); }
 
 void* orc_create(const mpsr_create_info* ci) {
     Handle* h = new Handle();

@@ -6,8 +6,8 @@
 // Weight gradients (sums over rows) come from k_nn_wgrad: each block reduces a chunk of rows with
 // the same MFMA and writes a partial; k_nn_sum_partials adds the partials in block order, so the
 // gradients are deterministic.
-// Message aggregation (SpMM over the SELL matrix, width 32): half a wave per row, lane k owns
-// feature k, the row's (column, value) pairs are broadcast loads.
+// Message aggregation (SpMM over the SELL matrix, width 32): half a wave per row, four
+// neighbours in flight, 8-byte BF16 loads; features: half a wave per row over its entries.
 // Loss and gradient of the outputs use FP64 SELL matrix-vector products.
 #include <cmath>
 #include <cstring>
@@ -149,20 +149,32 @@ __global__ void k_nn_scale(double* dscal) {
     dscal[5] = n;
 }
 
+__device__ inline double hw_sum(double v) {
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 32);
+    return v;
+}
+
+// half a wave per row: lane h takes the row's stored entries h, h + 32, ...; the eight neighbour
+// moments are summed over the half-wave, lane 0 writes the row's 16 features
 template <typename R, typename VA>
 __global__ __launch_bounds__(NB) void k_nn_feat(MpnnSys sys, const VA* __restrict__ vals, MpnnGeo<R> geo,
                                                 const double* __restrict__ dscal, uint16_t* X) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 5);
+    const int h = threadIdx.x & 31;
     if (t >= sys.N) return;
     float f[NN_F];
     for (int k = 0; k < NN_F; ++k) f[k] = 0.f;
-    if (!sys.dir[t]) {
-        const double s = dscal[0];
+    const bool live = !sys.dir[t];
+    double sa = 0, sj = 0, sx = 0, rx = 0, ry = 0, rz = 0, rd = 0, dv = 0;
+    int n = 0;
+    R4<R> P{}, V{};
+    if (live) {
         const double dg = sys.diag[t];
-        const int base = sys.slice_off[t >> 6] + (t & 63), n = sys.cnt[t];
-        const R4<R> P = geo.spos[t], V = geo.svel[t];
-        double sa = 0, sj = 0, sx = 0, rx = 0, ry = 0, rz = 0, rd = 0, dv = 0;
-        for (int k = 0; k < n; ++k) {
+        const int base = sys.slice_off[t >> 6] + (t & 63);
+        n = sys.cnt[t];
+        P = geo.spos[t];
+        V = geo.svel[t];
+        for (int k = h; k < n; k += 32) {
             const int e = base + k * 64;
             const int j = sys.cols[e];
             const double a = -(double)vals[e] / dg;
@@ -178,6 +190,13 @@ __global__ __launch_bounds__(NB) void k_nn_feat(MpnnSys sys, const VA* __restric
                 dv += a * (((double)W.x - (double)V.x) * dx + ((double)W.y - (double)V.y) * dy +
                            ((double)W.z - (double)V.z) * dz) / r2;
         }
+    }
+    sa = hw_sum(sa); sj = hw_sum(sj); sx = hw_sum(sx); rx = hw_sum(rx);
+    ry = hw_sum(ry); rz = hw_sum(rz); rd = hw_sum(rd); dv = hw_sum(dv);
+    if (h != 0) return;
+    if (live) {
+        const double s = dscal[0];
+        const double dg = sys.diag[t];
         const double il0 = 1.0 / geo.l0, vs = geo.dt / geo.l0;
         f[0] = (float)(sys.rhs[t] / dg / s);
         f[1] = (float)(sys.xprev[t] / s);
@@ -339,24 +358,45 @@ __global__ __launch_bounds__(256) void k_nn_colsum(int rows, const uint16_t* __r
 template <typename VA, typename TI>
 __global__ __launch_bounds__(256) void k_nn_agg(MpnnSys sys, const VA* __restrict__ vals, const TI* __restrict__ in,
                                                 int mode, uint16_t* out) {
+    // half a wave per row: 4 groups of 8 lanes take every 4th stored entry, lane f of a group
+    // owns features 4f..4f+3 (one 8-byte load of 4 BF16 per neighbour); the groups' partial sums
+    // meet through two shuffles at the end
     const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 5);
-    const int k = threadIdx.x & 31;
+    const int h = threadIdx.x & 31, g = h >> 3, f = h & 7;
     if (t >= sys.N) return;
-    float acc = 0.f;
-    if (!sys.dir[t]) {
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    const bool live = !sys.dir[t];
+    if (live) {
         const int base = sys.slice_off[t >> 6] + (t & 63), n = sys.cnt[t];
         const double dg = sys.diag[t];
-        for (int q = 0; q < n; ++q) {
+        for (int q = g; q < n; q += 4) {
             const int e = base + q * 64;
             const int j = sys.cols[e];
-            const double a = -(double)vals[e] / (mode == 0 ? dg : sys.diag[j]);
-            float x;
-            if constexpr (sizeof(TI) == 2) x = bf2f((uint16_t)in[(size_t)j * NN_M + k]);
-            else x = (float)in[(size_t)j * NN_M + k];
-            acc += (float)a * x;
+            const float a = (float)(-(double)vals[e] / (mode == 0 ? dg : sys.diag[j]));
+            float x0, x1, x2, x3;
+            if constexpr (sizeof(TI) == 2) {
+                const uint2 v = *reinterpret_cast<const uint2*>(in + (size_t)j * NN_M + 4 * f);
+                x0 = bf2f((uint16_t)(v.x & 0xffff)); x1 = bf2f((uint16_t)(v.x >> 16));
+                x2 = bf2f((uint16_t)(v.y & 0xffff)); x3 = bf2f((uint16_t)(v.y >> 16));
+            } else {
+                const float4 v = *reinterpret_cast<const float4*>(in + (size_t)j * NN_M + 4 * f);
+                x0 = v.x; x1 = v.y; x2 = v.z; x3 = v.w;
+            }
+            a0 += a * x0; a1 += a * x1; a2 += a * x2; a3 += a * x3;
         }
     }
-    out[(size_t)t * NN_M + k] = f2bf(acc);
+    for (int o = 8; o <= 16; o <<= 1) {
+        a0 += __shfl_xor(a0, o, 32);
+        a1 += __shfl_xor(a1, o, 32);
+        a2 += __shfl_xor(a2, o, 32);
+        a3 += __shfl_xor(a3, o, 32);
+    }
+    if (g == 0) {
+        uint2 v;
+        v.x = (uint32_t)f2bf(a0) | ((uint32_t)f2bf(a1) << 16);
+        v.y = (uint32_t)f2bf(a2) | ((uint32_t)f2bf(a3) << 16);
+        *reinterpret_cast<uint2*>(out + (size_t)t * NN_M + 4 * f) = v;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -636,10 +676,11 @@ void mpnn_features(Mpnn* n, const MpnnSys& sys, const MpnnGeo<R>& geo, hipStream
     hipLaunchKernelGGL(k_nn_finish, dim3(1), dim3(1024), 0, st, n->dpart, std::max(nb, 1), 2, n->dscal, 6);
     hipLaunchKernelGGL(k_nn_scale, dim3(1), dim3(1), 0, st, n->dscal);
     if (N <= 0) return;
+    const dim3 gf(grid_for((long long)N * 32, NB));
     if (sys.v32)
-        hipLaunchKernelGGL((k_nn_feat<R, float>), dim3(nb), dim3(NB), 0, st, sys, sys.v32, geo, n->dscal, n->X);
+        hipLaunchKernelGGL((k_nn_feat<R, float>), gf, dim3(NB), 0, st, sys, sys.v32, geo, n->dscal, n->X);
     else
-        hipLaunchKernelGGL((k_nn_feat<R, double>), dim3(nb), dim3(NB), 0, st, sys, sys.v64, geo, n->dscal, n->X);
+        hipLaunchKernelGGL((k_nn_feat<R, double>), gf, dim3(NB), 0, st, sys, sys.v64, geo, n->dscal, n->X);
 }
 
 template <int NT>

@@ -45,16 +45,18 @@ __global__ __launch_bounds__(256) void k_sph_ranges(int S, int ncell, const uint
     if (t == S - 1 || sk[t + 1] != k) ce[k] = t + 1;
 }
 
-// Schiller-Naumann drag on a sphere of diameter d at relative velocity du (exact square root)
+// Schiller-Naumann drag on a sphere of diameter d at relative velocity du, evaluated in double
+// (correctly rounded square root on host and device alike) and rounded once to R
 template <typename R> __device__ inline void sn_force(R dux, R duy, R duz, R d, R rho, R nu, R* f) {
-    const R mag = sqrt(dux * dux + duy * duy + duz * duz);
-    const R re = mag * d / nu;
-    R k;
-    if (re < (R)1000) k = (R)3 * (R)M_PI * rho * nu * d * ((R)1 + (R)0.15 * powd(re, (R)0.687));
-    else k = (R)0.5 * rho * (R)0.44 * ((R)M_PI * d * d / (R)4) * mag;
-    f[0] = k * dux;
-    f[1] = k * duy;
-    f[2] = k * duz;
+    const double ux = dux, uy = duy, uz = duz, dd = d, rr = rho, nn = nu;
+    const double mag = sqrt(ux * ux + uy * uy + uz * uz);
+    const double re = mag * dd / nn;
+    double k;
+    if (re < 1000.0) k = 3.0 * M_PI * rr * nn * dd * (1.0 + 0.15 * det_exp(0.687 * det_log(re > 0 ? re : 1.0)) * (re > 0 ? 1.0 : 0.0));
+    else k = 0.5 * rr * 0.44 * (M_PI * dd * dd / 4.0) * mag;
+    f[0] = (R)(k * ux);
+    f[1] = (R)(k * uy);
+    f[2] = (R)(k * uz);
 }
 
 template <typename R>

@@ -108,11 +108,37 @@ __device__ inline int nearest_triangle(const Consts<R>& c, R x, R y, R z, const 
     return bid;
 }
 
-// deterministic pow: exp(y log x) in double, rounded once
+// Deterministic pow, exp(y log x) in double rounded once to R, from IEEE-exact operations only
+// (+ - * /, frexp, ldexp, floor) so that host and device give the same bits whatever their libm:
+//   log x = e ln2 + 2 atanh(s), s = (m - 1)/(m + 1), m in [sqrt(1/2), sqrt(2)), series to s^23
+//   exp z = 2^k exp(r), z = k ln2 + r (Cody-Waite split of ln2), |r| <= ln2/2, Taylor to r^15
+__device__ inline double det_log(double x) {
+    int e = 0;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752440) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+    double p = 1.0 / 23.0;
+    for (int k = 10; k >= 0; --k) p = p * s2 + 1.0 / (double)(2 * k + 1);
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    return (double)e * ln2_hi + ((double)e * ln2_lo + 2.0 * s * p);
+}
+__device__ inline double det_exp(double z) {
+    if (z > 709.0) return INFINITY;
+    if (z < -745.0) return 0.0;
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    const double k = floor(z / 0.69314718055994530942 + 0.5);
+    const double r = (z - k * ln2_hi) - k * ln2_lo;
+    double p = 1.0;
+    for (int j = 15; j >= 1; --j) p = 1.0 + p * r / (double)j;
+    return ldexp(p, (int)k);
+}
 template <typename R> __device__ inline R powd(R x, R y) {
     if (y == R(0)) return R(1);
     if (x == R(0)) return y > R(0) ? R(0) : (R)INFINITY;
-    return (R)exp((double)y * log((double)x));
+    return (R)det_exp((double)y * det_log((double)x));
 }
 
 }  // namespace mpsr

@@ -387,3 +387,18 @@ def test_pow_rounded_once(orc):
     assert orc.orc_powd(0.0, 0.0) == 1.0
     assert orc.orc_powd(0.0, 1.5) == 0.0
     assert math.isinf(orc.orc_powd(0.0, -1.0))
+
+
+def test_pow_double_within_a_few_ulps(orc):
+    """The deterministic pow (exact operations only, so device and host agree bit for bit) against
+    the host's correctly rounded pow: within 1e-14 relative over the range the drag laws use (the
+    double rounding of y log x is amplified by |y log x| in exp)."""
+    orc.orc_powd64.restype = C.c_double
+    orc.orc_powd64.argtypes = [C.c_double, C.c_double]
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.uniform(1e-6, 10.0, 3000), rng.uniform(10.0, 1e5, 1000), [0.5, 1.0, 2.0, 1000.0]])
+    ys = np.concatenate([rng.uniform(-3, 3, 4000), [0.687, 1.0, 2.0, 0.687]])
+    for x, y in zip(xs, ys):
+        want = math.pow(x, y)
+        got = orc.orc_powd64(float(x), float(y))
+        assert abs(got - want) <= 1e-14 * abs(want), (x, y, got, want)
