@@ -412,6 +412,41 @@ mpsr_status mpsr_get_ppe_matrix(mpsr_handle *h, int *row_ptr, int *col, double *
 /* Per-phase HIP-event timing of the following steps (mpsr_get_timings); off by default. */
 mpsr_status mpsr_set_profiling(mpsr_handle *h, int on);
 
+/* Launch shape of the per-particle kernels (block: a multiple of 64 in [64, 512]) and of the
+ * particle sort (tile = 256 * sort_items keys, sort_items 4, 8 or 16).  Results do not depend on
+ * it.  No counterpart in the reference (its launchers fix 256 threads, DeviceProcess.cu:29-304). */
+mpsr_status mpsr_set_launch(mpsr_handle *h, int block, int sort_items);
+
+/* ---- batched buckets (no counterpart in the reference) ------------------------------------------
+ * Many independent domains ("buckets": each its own particles, walls, grid and precision) advanced
+ * together.  The reference steps one domain per CalculateNew call (Src/MultiGPUHandler.cpp:
+ * 362-492); here one step of every bucket is recorded once into a HIP graph, each bucket's
+ * kernels on its own stream forked from the batch stream, and every step is one graph launch.
+ * Buckets use the explicit (EOS) pressure and carry no spheres or bucket inflow (rejected with
+ * MPSR_PARAMERR); every bucket must name the same device.  A bucket's results are those of
+ * mpsr_step_many on the same handle. */
+typedef struct mpsr_batch mpsr_batch;
+#define MPSR_TUNE_BLOCKS 4   /* blocks 64, 128, 256, 512 */
+#define MPSR_TUNE_ITEMS 3    /* sort items 4, 8, 16 */
+mpsr_status mpsr_create_batch(const mpsr_create_info *infos, int nbuckets, mpsr_batch **out);
+void mpsr_destroy_batch(mpsr_batch *b);
+/* The handle of bucket i for the per-domain calls (particles, triangles, timings...); owned by
+ * the batch: mpsr_destroy on it does nothing. */
+mpsr_status mpsr_batch_handle(mpsr_batch *b, int bucket, mpsr_handle **h);
+/* Steps first .. first+count-1 of every bucket: one graph launch per step (the graph is recorded
+ * on the first call and again after a launch-shape change). */
+mpsr_status mpsr_step_batch(mpsr_batch *b, int first, int count);
+/* The same steps with the same kernels launched directly, bucket by bucket (no graph). */
+mpsr_status mpsr_step_batch_eager(mpsr_batch *b, int first, int count);
+mpsr_status mpsr_batch_set_launch(mpsr_batch *b, int block, int sort_items);
+/* Times `steps` graph-launched steps from `first` for every launch shape (MPSR_TUNE_BLOCKS x
+ * MPSR_TUNE_ITEMS, ms per step into ms_table row-major when non-NULL), each from the same saved
+ * state, restores that state and keeps the fastest shape. */
+mpsr_status mpsr_batch_autotune(mpsr_batch *b, int first, int steps, int *best_block, int *best_items,
+                                float *ms_table);
+/* ms per step of the last mpsr_step_batch(_eager) call, nodes of the recorded graph, launch shape. */
+mpsr_status mpsr_batch_info(mpsr_batch *b, float *ms_per_step, int *graph_nodes, int *block, int *sort_items);
+
 /* ---- learned PPE warm start and preconditioner (no counterpart in the reference) ---------------
  * A three-layer message-passing network over the PPE matrix graph (hidden width 128, message
  * width 32, 16 input features), BF16 operands on MFMA with FP32 accumulation.  With

@@ -84,6 +84,15 @@ def load_library(path=None):
         "mpsr_mpnn_get_weights": (C.c_int, [vp, P(C.c_float), C.c_int]),
         "mpsr_mpnn_train_step": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, P(C.c_float), P(C.c_double)]),
         "mpsr_mpnn_rows": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_double)]),
+        "mpsr_set_launch": (C.c_int, [vp, C.c_int, C.c_int]),
+        "mpsr_create_batch": (C.c_int, [P(abi.CreateInfo), C.c_int, P(vp)]),
+        "mpsr_destroy_batch": (None, [vp]),
+        "mpsr_batch_handle": (C.c_int, [vp, C.c_int, P(vp)]),
+        "mpsr_step_batch": (C.c_int, [vp, C.c_int, C.c_int]),
+        "mpsr_step_batch_eager": (C.c_int, [vp, C.c_int, C.c_int]),
+        "mpsr_batch_set_launch": (C.c_int, [vp, C.c_int, C.c_int]),
+        "mpsr_batch_autotune": (C.c_int, [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_int), P(C.c_float)]),
+        "mpsr_batch_info": (C.c_int, [vp, P(C.c_float), P(C.c_int), P(C.c_int), P(C.c_int)]),
         "mpsr_deck_load": (C.c_int, [C.c_char_p, P(vp)]),
         "mpsr_deck_free": (None, [vp]),
         "mpsr_deck_last_error": (C.c_char_p, []),
@@ -136,9 +145,30 @@ class Solver:
         self.gridbox = gb
         self.cells = gb.cells()
 
+    @classmethod
+    def _borrowed(cls, case, handle):
+        """A Solver over a handle that a Batch owns (close() leaves it to the batch)."""
+        self = cls.__new__(cls)
+        self.lib = load_library()
+        self.case = case
+        self.n = case.n
+        self.ntri = len(case.vertices)
+        self._buckets = False
+        self._keep = None
+        self.h = handle
+        gb = abi.GridBox()
+        self._check(self.lib.mpsr_get_gridbox(self.h, C.byref(gb)), "mpsr_get_gridbox")
+        self.gridbox = gb
+        self.cells = gb.cells()
+        return self
+
     def _check(self, st, where):
         if st != 0:
             raise MpsrError(st, where, self.lib.mpsr_last_error().decode(errors="replace"))
+
+    def set_launch(self, block, sort_items):
+        """Launch shape of the per-particle kernels and the particle sort (results do not change)."""
+        self._check(self.lib.mpsr_set_launch(self.h, block, sort_items), "mpsr_set_launch")
 
     def close(self):
         if getattr(self, "h", None):
@@ -362,3 +392,75 @@ class Solver:
         if nnz.value > cap:
             return self.ppe_matrix(nnz.value)
         return ptr, col[:nnz.value], val[:nnz.value], diag
+
+
+class Batch:
+    """Independent buckets (each a case with its own walls, grid and precision) advanced together:
+    one graph launch per step (mpsr_step_batch), or the same kernels launched directly
+    (mpsr_step_batch_eager).  buckets[i] is a Solver over bucket i for the per-domain calls."""
+
+    TUNE_BLOCKS = (64, 128, 256, 512)
+    TUNE_ITEMS = (4, 8, 16)
+
+    def __init__(self, case_list, precisions=None, device=0):
+        from . import cases
+        self.lib = load_library()
+        self.cases = list(case_list)
+        precisions = precisions or [abi.MPSR_FP32] * len(self.cases)
+        infos, self._keep = [], []
+        for c, prec in zip(self.cases, precisions):
+            info, keep = cases.create_info(c, prec, abi.MPSR_PRESSURE_EOS, None, None, device)
+            infos.append(info)
+            self._keep.append(keep)
+        arr = (abi.CreateInfo * len(infos))(*infos)
+        b = C.c_void_p()
+        self._check(self.lib.mpsr_create_batch(arr, len(infos), C.byref(b)), "mpsr_create_batch")
+        self.b = b
+        self.buckets = []
+        for i, c in enumerate(self.cases):
+            h = C.c_void_p()
+            self._check(self.lib.mpsr_batch_handle(self.b, i, C.byref(h)), "mpsr_batch_handle")
+            self.buckets.append(Solver._borrowed(c, h))
+
+    def _check(self, st, where):
+        if st != 0:
+            raise MpsrError(st, where, self.lib.mpsr_last_error().decode(errors="replace"))
+
+    def step_many(self, first, count):
+        self._check(self.lib.mpsr_step_batch(self.b, first, count), "mpsr_step_batch")
+
+    def step_many_eager(self, first, count):
+        self._check(self.lib.mpsr_step_batch_eager(self.b, first, count), "mpsr_step_batch_eager")
+
+    def set_launch(self, block, sort_items):
+        self._check(self.lib.mpsr_batch_set_launch(self.b, block, sort_items), "mpsr_batch_set_launch")
+
+    def autotune(self, first, steps):
+        """(best block, best sort items, {(block, items): ms per step})"""
+        bb, bi = C.c_int(), C.c_int()
+        tab = (C.c_float * (len(self.TUNE_BLOCKS) * len(self.TUNE_ITEMS)))()
+        self._check(self.lib.mpsr_batch_autotune(self.b, first, steps, C.byref(bb), C.byref(bi), tab),
+                    "mpsr_batch_autotune")
+        res = {(blk, it): tab[i * len(self.TUNE_ITEMS) + j] for i, blk in enumerate(self.TUNE_BLOCKS)
+               for j, it in enumerate(self.TUNE_ITEMS)}
+        return bb.value, bi.value, res
+
+    def info(self):
+        """{ms_per_step, graph_nodes, block, sort_items} of the last batched call."""
+        ms, nodes, blk, it = C.c_float(), C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.mpsr_batch_info(self.b, C.byref(ms), C.byref(nodes), C.byref(blk), C.byref(it)),
+                    "mpsr_batch_info")
+        return {"ms_per_step": ms.value, "graph_nodes": nodes.value, "block": blk.value, "sort_items": it.value}
+
+    def close(self):
+        if getattr(self, "b", None):
+            for s in self.buckets:
+                s.h = None
+            self.lib.mpsr_destroy_batch(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

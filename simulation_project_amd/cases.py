@@ -320,6 +320,45 @@ def dam_break(lib, l0=0.01, seed=DEFAULT_SEED, explicit_pressure=True, steps=100
     return _finish(case, lib)
 
 
+def _fill_level(fill):
+    """Height of the fill line above the axis, over the radius, whose circular segment below holds
+    `fill` of the disc (bisection)."""
+    lo_h, hi_h = -1.0, 1.0
+    for _ in range(200):
+        h = 0.5 * (lo_h + hi_h)
+        d = -h
+        seg = math.acos(max(-1.0, min(1.0, d))) - d * math.sqrt(max(0.0, 1.0 - d * d))
+        if seg / math.pi < fill:
+            lo_h = h
+        else:
+            hi_h = h
+    return 0.5 * (lo_h + hi_h)
+
+
+def _drum_lattice(l0, fill, radius, length, count_only=False):
+    """Lattice sites of a drum of the given radius and length (axis along x at y = z = radius)
+    filled to `fill` of its cross-section: planes at l0, 2 l0, ... and at least l0 from the far
+    cap, sites at least l0 inside the mantle.  Returns (positions, fill level h above the axis),
+    or only the site count when count_only."""
+    yc = zc = radius
+    h = _fill_level(fill) * radius
+    nx = int(math.floor(length / l0 + 1e-9)) - 1
+    m = int(radius / l0)
+    off = np.arange(-m, m + 1) * l0
+    yy, zz = np.meshgrid(yc + off, zc + off, indexing="ij")
+    rr = (yy - yc) ** 2 + (zz - zc) ** 2
+    inside = (rr <= (radius - l0) ** 2) & ((zz - zc) < h)
+    if count_only:
+        return max(nx, 0) * int(inside.sum())
+    yz = np.stack([yy[inside], zz[inside]], axis=1)
+    xs = (np.arange(nx) + 1.0) * l0
+    pos = np.empty((max(nx, 0) * len(yz), 3))
+    pos[:, 0] = np.repeat(xs, len(yz))
+    pos[:, 1] = np.tile(yz[:, 0], nx)
+    pos[:, 2] = np.tile(yz[:, 1], nx)
+    return pos, h
+
+
 def drum(lib, l0=2.5e-3, fill=0.4, rpm=30, radius=0.15, length=0.3, seed=DEFAULT_SEED, dt=5e-5,
          n_r=16, n_x=36, steps=200, explicit_pressure=False):
     """Rotating drum: cylinder of radius 0.15 m and length 0.3 m along x (9984 triangles by
@@ -328,30 +367,7 @@ def drum(lib, l0=2.5e-3, fill=0.4, rpm=30, radius=0.15, length=0.3, seed=DEFAULT
     cap) and at least l0 inside the mantle.  l0 = 2.5e-3 m gives 516,936 particles and
     l0 = 2.012e-3 m gives 1,008,472 (the 1M size)."""
     yc = zc = radius
-    # fill level h (relative to the axis) whose circular segment below holds `fill` of the disc
-    lo_h, hi_h = -radius, radius
-    for _ in range(200):
-        h = 0.5 * (lo_h + hi_h)
-        d = -h
-        seg = radius * radius * math.acos(max(-1.0, min(1.0, d / radius))) - d * math.sqrt(max(0.0, radius * radius - d * d))
-        if seg / (math.pi * radius * radius) < fill:
-            lo_h = h
-        else:
-            hi_h = h
-    h = 0.5 * (lo_h + hi_h)
-    # planes at l0, 2 l0, ... and at least l0 from the far cap as well
-    nx = int(math.floor(length / l0 + 1e-9)) - 1
-    m = int(radius / l0)
-    off = np.arange(-m, m + 1) * l0
-    yy, zz = np.meshgrid(yc + off, zc + off, indexing="ij")
-    rr = (yy - yc) ** 2 + (zz - zc) ** 2
-    inside = (rr <= (radius - l0) ** 2) & ((zz - zc) < h)
-    yz = np.stack([yy[inside], zz[inside]], axis=1)
-    xs = (np.arange(nx) + 1.0) * l0
-    pos = np.empty((nx * len(yz), 3))
-    pos[:, 0] = np.repeat(xs, len(yz))
-    pos[:, 1] = np.tile(yz[:, 0], nx)
-    pos[:, 2] = np.tile(yz[:, 1], nx)
+    pos, h = _drum_lattice(l0, fill, radius, length)
     pos = pos + jitter(seed, len(pos), 1e-4 * l0)
     p = default_params(l0, dt, steps=steps, explicit_pressure=explicit_pressure,
                        artificial=0.01 if explicit_pressure else 0.0,
@@ -523,3 +539,38 @@ def from_deck(lib, config_path, steps=None):
     name = os.path.splitext(os.path.basename(config_path))[0]
     return Case(name, params, pa, verts, model, models, drag, zone,
                 steps if steps is not None else params.AnalysisStep, meta)
+
+
+def buckets(lib, count=64, seed=DEFAULT_SEED, l0=5e-3, dt=5e-5, n_min=5e3, n_max=2e5, box_share=0.25):
+    """The batched multi-bucket workload: `count` independent rotating buckets, particle counts
+    drawn log-uniformly from [n_min, n_max] (64 buckets: about 3.4M particles in all), rotation
+    rates from 10-60 rpm, a quarter of them closed boxes (12 triangles), the rest drums of length
+    2 R whose radius gives the drawn count at 40 % fill, meshed with up to 9984 triangles.  Every
+    bucket uses the reference's explicit (EOS) pressure; l0 = 5e-3 m and dt = 5e-5 s keep the
+    sound-speed Courant number c dt / l0 at 0.2.  Deterministic in seed."""
+    out = []
+    idx = np.arange(count)
+    u_n, u_rpm, u_kind = uniform01(seed, idx, 0), uniform01(seed, idx, 1), uniform01(seed, idx, 2)
+    for i in range(count):
+        target = float(np.exp(np.log(n_min) + u_n[i] * (np.log(n_max) - np.log(n_min))))
+        rpm = 10.0 + 50.0 * float(u_rpm[i])
+        if u_kind[i] < box_share:
+            nx = max(2, int(round((target / 0.75) ** (1.0 / 3.0))))
+            ny = max(2, int(round(0.75 * nx)))
+            c = small_box(lib, nx=nx, ny=ny, nz=nx, l0=l0, seed=seed + i, rotating=True, rpm=rpm, dt=dt)
+        else:
+            lo, hi = 4.0 * l0, 2.0
+            for _ in range(40):
+                r = 0.5 * (lo + hi)
+                if _drum_lattice(l0, 0.4, r, 2.0 * r, count_only=True) < target:
+                    lo = r
+                else:
+                    hi = r
+            r = 0.5 * (lo + hi)
+            n_r = int(min(16, max(2, round(16 * r / 0.15))))
+            n_x = int(min(36, max(2, round(36 * 2.0 * r / 0.3))))
+            c = drum(lib, l0=l0, fill=0.4, rpm=rpm, radius=r, length=2.0 * r, seed=seed + i, dt=dt, n_r=n_r,
+                     n_x=n_x, explicit_pressure=True)
+        c.meta.update({"bucket": i, "target": target, "rpm": rpm})
+        out.append(c)
+    return out

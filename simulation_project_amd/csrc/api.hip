@@ -65,6 +65,15 @@ struct EngineBase {
     virtual mpsr_status mpnn_rows(float* X, float* Y, double* scale) = 0;
     virtual mpsr_status set_spheres(const mpsr_spheres* s) = 0;
     virtual mpsr_status get_spheres(mpsr_spheres* s) = 0;
+    // launch shape of the per-particle kernels and the particle sort (results do not depend on it)
+    virtual mpsr_status set_launch(int block, int sort_items) = 0;
+    // batched form: one step of this domain recorded on its own stream (into a shared graph)
+    virtual mpsr_status batch_check() = 0;
+    virtual mpsr_status batch_prepare(int first, int count, int* moved) = 0;
+    virtual mpsr_status batch_enqueue() = 0;
+    virtual mpsr_status batch_finish() = 0;
+    virtual mpsr_status snapshot(int restore) = 0;
+    virtual hipStream_t stream() const = 0;
     mpsr_gridbox gb{};
     mpsr_timings tm{};
     int profiling = 0;
@@ -697,6 +706,108 @@ template <typename R> struct Engine : EngineBase {
         return MPSR_NOERR;
     }
 
+    // ---- launch shape and the batched form --------------------------------------------------
+    int* step_ctr_buf = nullptr;       // device step counter of graph replays
+    std::vector<char*> snap;           // device copy of the in/out state and the triangles
+    int snap_n = 0;
+    // what the last recorded step holds: rotation table address and launch shape
+    const RotCoef<R>* rec_rot = nullptr;
+    int rec_block = 0, rec_items = 0;
+
+    mpsr_status set_launch(int block, int items) override {
+        if (block < 64 || block > kMaxBlock || block % 64)
+            return fail(MPSR_PARAMERR, "launch block must be a multiple of 64 in [64, 512]");
+        if (!radix_items_valid(items)) return fail(MPSR_PARAMERR, "sort items per thread must be 4, 8 or 16");
+        d.block = block;
+        d.sort_items = items;
+        return MPSR_NOERR;
+    }
+    mpsr_status batch_check() override {
+        if (pressure_mode == MPSR_PRESSURE_PPE)
+            return fail(MPSR_PARAMERR, "batched step: the PPE solve reads its convergence on the host; batched "
+                                       "buckets use the explicit (EOS) pressure");
+        if (sph.S > 0) return fail(MPSR_PARAMERR, "batched step: rigid spheres are not part of the batched form");
+        if (!buckets.empty()) return fail(MPSR_PARAMERR, "batched step: bucket inflow changes the particle count");
+        return MPSR_NOERR;
+    }
+    // rotation table for steps first .. first+count-1 and the step counter at 0; *moved when the
+    // table address or the launch shape differ from the last recorded step (a graph holds both)
+    mpsr_status batch_prepare(int first, int count, int* moved) override {
+        HIPC(hipSetDevice(device));
+        if (!step_ctr_buf) ALLOC(step_ctr_buf, 1);
+        mpsr_status s = upload_rot(first, count);
+        if (s != MPSR_NOERR) return s;
+        *moved = rec_rot != rot_table || rec_block != d.block || rec_items != d.sort_items;
+        fill_int(step_ctr_buf, 0, 1, st);
+        HIPC(hipStreamSynchronize(st));   // rot_host is refilled by the next call
+        return MPSR_NOERR;
+    }
+    // the EOS step of one_step() with the rotation row taken from the device counter, which the
+    // step then advances; no host synchronisation, no memset, no profiling events
+    mpsr_status batch_enqueue() override {
+        const int prof = profiling;
+        profiling = 0;
+        d.rot = rot_table;
+        d.step_ctr = step_ctr_buf;
+        rec_rot = rot_table;
+        rec_block = d.block;
+        rec_items = d.sort_items;
+        do_bin();
+        do_tri();
+        do_stl();
+        if (d.ndrag > 0) do_drag();
+        do_explicit();
+        do_relocate();
+        do_bin();
+        do_stl();
+        launch_pnd(d, 1, st);
+        do_gradient();
+        do_relocate();
+        launch_translate(d, st);
+        launch_rotate(d, st);
+        launch_step_advance(step_ctr_buf, st);
+        d.step_ctr = nullptr;
+        profiling = prof;
+        HIPC(hipGetLastError());
+        return MPSR_NOERR;
+    }
+    mpsr_status batch_finish() override {
+        int ovf = 0;
+        HIPC(hipMemcpy(&ovf, d.overflow, sizeof(int), hipMemcpyDeviceToHost));
+        if (ovf) return fail(MPSR_MEMERR, "triangle/cell pair buffer overflow: walls grew beyond the set-up capacity");
+        return MPSR_NOERR;
+    }
+    // restore = 0: copy the state (in and out arrays, triangles) aside; 1: copy it back
+    mpsr_status snapshot(int restore) override {
+        HIPC(hipSetDevice(device));
+        std::vector<std::pair<void*, size_t>> f;
+        const size_t n = (size_t)d.cap;
+        for (PState<R>* ps : {&d.in, &d.out}) {
+            f.push_back({ps->id, n * sizeof(int)});
+            f.push_back({ps->type, n * sizeof(int)});
+            for (R* q : {ps->px, ps->py, ps->pz, ps->vx, ps->vy, ps->vz, ps->p, ps->n, ps->temp, ps->nu, ps->solid})
+                f.push_back({q, n * sizeof(R)});
+        }
+        f.push_back({d.tris, sizeof(TriD<R>) * std::max(1, d.ntri)});
+        if (restore && snap.empty()) return fail(MPSR_PARAMERR, "no snapshot to restore");
+        if (snap.empty()) {
+            for (auto& x : f) {
+                char* b = nullptr;
+                ALLOC(b, x.second);
+                snap.push_back(b);
+            }
+        }
+        for (size_t i = 0; i < f.size(); ++i) {
+            if (restore) HIPC(hipMemcpyAsync(f[i].first, snap[i], f[i].second, hipMemcpyDeviceToDevice, st));
+            else HIPC(hipMemcpyAsync(snap[i], f[i].first, f[i].second, hipMemcpyDeviceToDevice, st));
+        }
+        if (restore) d.N = snap_n;
+        else snap_n = d.N;
+        HIPC(hipStreamSynchronize(st));
+        return MPSR_NOERR;
+    }
+    hipStream_t stream() const override { return st; }
+
     void collect_timings(float total_ms, int count) {
         mpsr_timings t{};
         t.step_ms = total_ms / count;
@@ -950,7 +1061,145 @@ using namespace mpsr;
 
 struct mpsr_handle {
     EngineBase* e;
+    bool borrowed = false;   // owned by a batch: mpsr_destroy leaves it alone
 };
+
+// Many independent domains ("buckets") advanced together: each keeps its own engine (grid, walls,
+// precision, stream); one step of all of them is recorded once into a HIP graph by forking every
+// bucket's stream from the batch stream and joining it back, and each step is then one graph
+// launch.  The rotation row of a replay comes from each bucket's device step counter.
+struct mpsr_batch {
+    std::vector<mpsr_handle*> h;
+    int device = 0;
+    hipStream_t main = nullptr;
+    hipEvent_t fork = nullptr, e0 = nullptr, e1 = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool dirty = true;
+    int block = 256, items = 8;
+    size_t nodes = 0;
+    float ms_per_step = 0.f;
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        dirty = true;
+    }
+    ~mpsr_batch() {
+        drop_graph();
+        for (mpsr_handle* x : h) {
+            delete x->e;
+            delete x;
+        }
+        for (hipEvent_t e : join) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (main) (void)hipStreamDestroy(main);
+    }
+};
+
+namespace {
+// records one step of every bucket into b->graph; on any failure the capture is ended and
+// discarded so that no stream is left capturing
+mpsr_status batch_capture(mpsr_batch* b) {
+    b->drop_graph();
+    HIPC(hipSetDevice(b->device));
+    HIPC(hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
+    mpsr_status st = MPSR_NOERR;
+    hipError_t he = hipEventRecord(b->fork, b->main);
+    for (size_t i = 0; i < b->h.size() && he == hipSuccess && st == MPSR_NOERR; ++i) {
+        EngineBase* e = b->h[i]->e;
+        he = hipStreamWaitEvent(e->stream(), b->fork, 0);
+        if (he != hipSuccess) break;
+        st = e->batch_enqueue();
+        if (st != MPSR_NOERR) break;
+        he = hipEventRecord(b->join[i], e->stream());
+        if (he == hipSuccess) he = hipStreamWaitEvent(b->main, b->join[i], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(b->main, &g);
+    if (st != MPSR_NOERR || he != hipSuccess || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        if (st != MPSR_NOERR) return st;
+        return fail(MPSR_GPUERR, std::string("batched step capture: ") +
+                                     hipGetErrorString(he != hipSuccess ? he : ee));
+    }
+    b->graph = g;
+    HIPC(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    HIPC(hipGraphGetNodes(b->graph, nullptr, &b->nodes));
+    b->dirty = false;
+    return MPSR_NOERR;
+}
+
+mpsr_status batch_prepare_all(mpsr_batch* b, int first, int count) {
+    int moved_any = 0;
+    for (mpsr_handle* x : b->h) {
+        int moved = 0;
+        mpsr_status s = x->e->batch_prepare(first, count, &moved);
+        if (s != MPSR_NOERR) return s;
+        moved_any |= moved;
+    }
+    if (moved_any) b->dirty = true;
+    return MPSR_NOERR;
+}
+
+mpsr_status batch_finish_all(mpsr_batch* b, int count) {
+    HIPC(hipEventRecord(b->e1, b->main));
+    HIPC(hipEventSynchronize(b->e1));
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, b->e0, b->e1));
+    b->ms_per_step = ms / count;
+    for (mpsr_handle* x : b->h) {
+        mpsr_status s = x->e->batch_finish();
+        if (s != MPSR_NOERR) return s;
+    }
+    return MPSR_NOERR;
+}
+
+mpsr_status batch_run(mpsr_batch* b, int first, int count, bool graph) {
+    HIPC(hipSetDevice(b->device));
+    if (count <= 0) return MPSR_NOERR;
+    mpsr_status s = batch_prepare_all(b, first, count);
+    if (s != MPSR_NOERR) return s;
+    if (graph && (b->dirty || !b->exec)) {
+        s = batch_capture(b);
+        if (s != MPSR_NOERR) return s;
+    }
+    HIPC(hipEventRecord(b->e0, b->main));
+    if (graph) {
+        for (int k = 0; k < count; ++k) HIPC(hipGraphLaunch(b->exec, b->main));
+    } else {
+        // the same kernels launched directly, bucket by bucket on the buckets' own streams
+        for (int k = 0; k < count; ++k) {
+            HIPC(hipEventRecord(b->fork, b->main));
+            for (size_t i = 0; i < b->h.size(); ++i) {
+                EngineBase* e = b->h[i]->e;
+                HIPC(hipStreamWaitEvent(e->stream(), b->fork, 0));
+                s = e->batch_enqueue();
+                if (s != MPSR_NOERR) return s;
+                HIPC(hipEventRecord(b->join[i], e->stream()));
+                HIPC(hipStreamWaitEvent(b->main, b->join[i], 0));
+            }
+        }
+    }
+    return batch_finish_all(b, count);
+}
+
+mpsr_status batch_set_launch(mpsr_batch* b, int block, int items) {
+    for (mpsr_handle* x : b->h) {
+        mpsr_status s = x->e->set_launch(block, items);
+        if (s != MPSR_NOERR) return s;
+    }
+    b->block = block;
+    b->items = items;
+    b->drop_graph();
+    return MPSR_NOERR;
+}
+}  // namespace
 
 #define GUARD(h)                                                       \
     do {                                                               \
@@ -985,11 +1234,109 @@ mpsr_status mpsr_create(const mpsr_create_info* info, mpsr_handle** out) {
     return MPSR_NOERR;
 }
 void mpsr_destroy(mpsr_handle* h) {
-    if (!h) return;
+    if (!h || h->borrowed) return;
     delete h->e;
     delete h;
 }
 mpsr_status mpsr_step(mpsr_handle* h, int i) { GUARD(h); return h->e->step_many(i, 1); }
+mpsr_status mpsr_set_launch(mpsr_handle* h, int block, int sort_items) { GUARD(h); return h->e->set_launch(block, sort_items); }
+
+mpsr_status mpsr_create_batch(const mpsr_create_info* infos, int nbuckets, mpsr_batch** out) {
+    if (!infos || !out || nbuckets <= 0) return fail(MPSR_PARAMERR, "mpsr_create_batch: no buckets");
+    *out = nullptr;
+    auto* b = new mpsr_batch();
+    b->device = infos[0].device;
+    for (int i = 0; i < nbuckets; ++i) {
+        if (infos[i].device != b->device) {
+            delete b;
+            return fail(MPSR_PARAMERR, "mpsr_create_batch: every bucket must be on the batch's device");
+        }
+        mpsr_handle* h = nullptr;
+        mpsr_status s = mpsr_create(&infos[i], &h);
+        if (s == MPSR_NOERR) {
+            h->borrowed = true;
+            b->h.push_back(h);
+            s = h->e->batch_check();
+        }
+        if (s != MPSR_NOERR) {
+            const std::string why = g_err;
+            delete b;
+            return fail(s, "bucket " + std::to_string(i) + ": " + why);
+        }
+    }
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
+        hipEventCreate(&b->e1) != hipSuccess) {
+        delete b;
+        return fail(MPSR_GPUERR, "mpsr_create_batch: stream or event creation failed");
+    }
+    b->join.resize(b->h.size(), nullptr);
+    for (auto& e : b->join) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            delete b;
+            return fail(MPSR_GPUERR, "mpsr_create_batch: event creation failed");
+        }
+    }
+    *out = b;
+    return MPSR_NOERR;
+}
+void mpsr_destroy_batch(mpsr_batch* b) { delete b; }
+mpsr_status mpsr_batch_handle(mpsr_batch* b, int bucket, mpsr_handle** h) {
+    if (!b || !h || bucket < 0 || bucket >= (int)b->h.size()) return fail(MPSR_PARAMERR, "mpsr_batch_handle: bad bucket");
+    *h = b->h[bucket];
+    return MPSR_NOERR;
+}
+mpsr_status mpsr_step_batch(mpsr_batch* b, int first, int count) {
+    if (!b) return fail(MPSR_PARAMERR, "null batch");
+    return batch_run(b, first, count, true);
+}
+mpsr_status mpsr_step_batch_eager(mpsr_batch* b, int first, int count) {
+    if (!b) return fail(MPSR_PARAMERR, "null batch");
+    return batch_run(b, first, count, false);
+}
+mpsr_status mpsr_batch_set_launch(mpsr_batch* b, int block, int sort_items) {
+    if (!b) return fail(MPSR_PARAMERR, "null batch");
+    return batch_set_launch(b, block, sort_items);
+}
+mpsr_status mpsr_batch_info(mpsr_batch* b, float* ms_per_step, int* graph_nodes, int* block, int* sort_items) {
+    if (!b) return fail(MPSR_PARAMERR, "null batch");
+    if (ms_per_step) *ms_per_step = b->ms_per_step;
+    if (graph_nodes) *graph_nodes = (int)b->nodes;
+    if (block) *block = b->block;
+    if (sort_items) *sort_items = b->items;
+    return MPSR_NOERR;
+}
+mpsr_status mpsr_batch_autotune(mpsr_batch* b, int first, int steps, int* best_block, int* best_items, float* ms_table) {
+    if (!b || steps <= 0) return fail(MPSR_PARAMERR, "mpsr_batch_autotune: null batch or no steps");
+    static const int blocks[MPSR_TUNE_BLOCKS] = {64, 128, 256, 512};
+    static const int items[MPSR_TUNE_ITEMS] = {4, 8, 16};
+    for (mpsr_handle* x : b->h) {
+        mpsr_status s = x->e->snapshot(0);
+        if (s != MPSR_NOERR) return s;
+    }
+    float best = -1.f;
+    int bb = b->block, bi = b->items;
+    for (int i = 0; i < MPSR_TUNE_BLOCKS; ++i)
+        for (int j = 0; j < MPSR_TUNE_ITEMS; ++j) {
+            mpsr_status s = batch_set_launch(b, blocks[i], items[j]);
+            if (s == MPSR_NOERR) s = batch_run(b, first, steps, true);   // records the graph, warms up
+            for (mpsr_handle* x : b->h)
+                if (s == MPSR_NOERR) s = x->e->snapshot(1);
+            if (s == MPSR_NOERR) s = batch_run(b, first, steps, true);
+            for (mpsr_handle* x : b->h)
+                if (s == MPSR_NOERR) s = x->e->snapshot(1);
+            if (s != MPSR_NOERR) return s;
+            if (ms_table) ms_table[i * MPSR_TUNE_ITEMS + j] = b->ms_per_step;
+            if (best < 0.f || b->ms_per_step < best) {
+                best = b->ms_per_step;
+                bb = blocks[i];
+                bi = items[j];
+            }
+        }
+    if (best_block) *best_block = bb;
+    if (best_items) *best_items = bi;
+    return batch_set_launch(b, bb, bi);
+}
 mpsr_status mpsr_step_many(mpsr_handle* h, int first, int count) { GUARD(h); return h->e->step_many(first, count); }
 mpsr_status mpsr_synchronize(mpsr_handle* h) { GUARD(h); return h->e->sync(); }
 mpsr_status mpsr_register_particle_topology(mpsr_handle* h) { GUARD(h); return h->e->phase(0, 0); }

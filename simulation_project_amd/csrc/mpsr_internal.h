@@ -190,8 +190,10 @@ size_t scan_tmp_ints(int n_cap);
 // Stable LSD radix sort of (key, value) pairs on `bits` low key bits.  n is *d_n when d_n is
 // non-null (device-resident count, n_cap bounds it), else n_host.  The result ends in (k0, v0)
 // or in (s.k1, s.v1); the return value says which (0 or 1).
+// `items` (4, 8 or 16) sets the tile of 256 * items keys; every tile gives the same result.
 int radix_sort_pairs(uint32_t* k0, int* v0, const int* d_n, int n_host, int n_cap, int bits,
-                     const SortScratch& s, hipStream_t st);
+                     const SortScratch& s, hipStream_t st, int items = 8);
+bool radix_items_valid(int items);
 // Exclusive scan of n ints (n = *d_n or n_host); *d_total (if non-null) receives the sum.
 void exclusive_scan(const int* in, int* out, const int* d_n, int n_host, int n_cap, int* d_total,
                     const SortScratch& s, hipStream_t st);
@@ -203,8 +205,13 @@ namespace mpsr {
 
 // All device pointers of one handle.  Slot-order state (in/out) is the reference's layout;
 // everything prefixed s* is in cell-sorted order (index t), valid between two binnings.
+// Largest block of the per-particle kernels; the launch block (64-512) is a runtime choice.
+static constexpr int kMaxBlock = 512;
+
 template <typename R> struct Dev {
     int N = 0, cap = 0;
+    int block = 256;                  // threads per block of the per-particle kernels (64..512)
+    int sort_items = 8;               // radix-sort tile = 256 * sort_items keys (4, 8, 16)
     int ntri = 0, nmodel = 0, ndrag = 0;
     Consts<R> c{};
     PState<R> in{}, out{};
@@ -223,7 +230,8 @@ template <typename R> struct Dev {
     TriD<R>* tris = nullptr;
     mpsr_tri_params* models = nullptr;
     mpsr_drag_params* drag = nullptr;
-    RotCoef<R>* rot = nullptr;        // [nmodel] for the current step
+    RotCoef<R>* rot = nullptr;        // [nmodel] for the current step, or the table base when
+    const int* step_ctr = nullptr;    // ... this device step counter picks the row (graph replays)
     int* tri_emit = nullptr;          // [ntri + 1] emitted cell count per triangle
     int* tri_off = nullptr;           // [ntri + 1]
     int* pair_n = nullptr;            // device count of emitted (cell, triangle) pairs
@@ -270,6 +278,7 @@ template <typename R> void launch_stl_distance(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_drag(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_translate(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_rotate(const Dev<R>& d, hipStream_t st);
+void launch_step_advance(int* ctr, hipStream_t st);
 // spheres.hip: rigid spheres with Schiller-Naumann drag
 template <typename R> struct SphDev {
     int S = 0;

@@ -47,7 +47,7 @@ __constant__ ZTable kZlap = {
 
 // ---------------------------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_bin_hash(int N, Consts<R> c, PState<R> in, PState<R> out, uint32_t* keys,
+__global__ __launch_bounds__(kMaxBlock) void k_bin_hash(int N, Consts<R> c, PState<R> in, PState<R> out, uint32_t* keys,
                                                    int* vals) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= N) return;
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void k_bin_hash(int N, Consts<R> c, PState<R> 
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_bin_ranges(int N, int ncell, const uint32_t* __restrict__ sk,
+__global__ __launch_bounds__(kMaxBlock) void k_bin_ranges(int N, int ncell, const uint32_t* __restrict__ sk,
                                                      const int* __restrict__ perm, int* cs, int* ce, PState<R> in,
                                                      R4<R>* __restrict__ spos, R4<R>* __restrict__ svel) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void k_bin_ranges(int N, int ncell, const uint
 
 // ---------------------------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_explicit(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+__global__ __launch_bounds__(kMaxBlock) void k_explicit(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
                                                    PState<R> out, const R4<R>* __restrict__ spos,
                                                    const R4<R>* __restrict__ svel, const int* __restrict__ cs,
                                                    const int* __restrict__ ce, const R* __restrict__ stl_mag,
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void k_explicit(int N, Consts<R> c, const int*
 
 // ---------------------------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_relocate(int N, Consts<R> c, PState<R> in, PState<R> out) {
+__global__ __launch_bounds__(kMaxBlock) void k_relocate(int N, Consts<R> c, PState<R> in, PState<R> out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= N) return;
     if (out.id[s] < 0) return;
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_relocate(int N, Consts<R> c, PState<R> 
 // PND (and the EOS pressure when eos != 0).  spres[t] receives the slot's out-pressure after the
 // kernel, which is what the gradient kernel reads for neighbours (Kern :860).
 template <typename R>
-__global__ __launch_bounds__(256) void k_pnd(int N, Consts<R> c, int eos, const int* __restrict__ perm, PState<R> in,
+__global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, const int* __restrict__ perm, PState<R> in,
                                               PState<R> out, const R4<R>* __restrict__ spos, R* __restrict__ spres,
                                               const int* __restrict__ cs, const int* __restrict__ ce,
                                               const R* __restrict__ stl_mag, const int* __restrict__ stl_id) {
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(256) void k_pnd(int N, Consts<R> c, int eos, const 
 
 // ---------------------------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void k_gradient(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+__global__ __launch_bounds__(kMaxBlock) void k_gradient(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
                                                    PState<R> out, const R4<R>* __restrict__ spos,
                                                    const R4<R>* __restrict__ svel, const R* __restrict__ spres,
                                                    const int* __restrict__ cs, const int* __restrict__ ce,
@@ -367,37 +367,37 @@ __global__ __launch_bounds__(256) void k_nbr_fill(int N, Consts<R> c, R re, cons
 // ---------------------------------------------------------------------------------------------
 template <typename R> void launch_bin_hash(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
-    hipLaunchKernelGGL(k_bin_hash<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.in, d.out, d.keys,
+    hipLaunchKernelGGL(k_bin_hash<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.in, d.out, d.keys,
                        d.vals);
 }
 template <typename R> void launch_bin_ranges(Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
-    const int which = radix_sort_pairs(d.keys, d.vals, nullptr, d.N, d.N, d.key_bits, d.sort, st);
+    const int which = radix_sort_pairs(d.keys, d.vals, nullptr, d.N, d.N, d.key_bits, d.sort, st, d.sort_items);
     d.skeys = which ? d.sort.k1 : d.keys;
     d.perm = which ? d.sort.v1 : d.vals;
     fill_int(d.cell_start, -1, d.c.ncell, st);
     fill_int(d.cell_end, -1, d.c.ncell, st);
-    hipLaunchKernelGGL(k_bin_ranges<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c.ncell, d.skeys,
+    hipLaunchKernelGGL(k_bin_ranges<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c.ncell, d.skeys,
                        d.perm, d.cell_start, d.cell_end, d.in, d.spos, d.svel);
 }
 template <typename R> void launch_explicit(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
-    hipLaunchKernelGGL(k_explicit<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.perm, d.in, d.out,
+    hipLaunchKernelGGL(k_explicit<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
                        d.spos, d.svel, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id, d.drag_acc, d.sph_dv,
                        d.tris, d.models);
 }
 template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
-    hipLaunchKernelGGL(k_relocate<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.in, d.out);
+    hipLaunchKernelGGL(k_relocate<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.in, d.out);
 }
 template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st) {
     if (d.N <= 0) return;
-    hipLaunchKernelGGL(k_pnd<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, eos, d.perm, d.in, d.out,
+    hipLaunchKernelGGL(k_pnd<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, eos, d.perm, d.in, d.out,
                        d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id);
 }
 template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
-    hipLaunchKernelGGL(k_gradient<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.perm, d.in, d.out,
+    hipLaunchKernelGGL(k_gradient<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
                        d.spos, d.svel, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id);
 }
 template <typename R> void launch_neighbor_count(const Dev<R>& d, R re, int* counts, hipStream_t st) {

@@ -19,13 +19,16 @@ namespace mpsr {
 
 static constexpr int RS_BLOCK = 256;  // 4 waves
 static constexpr int RS_WAVES = RS_BLOCK / 64;
-static constexpr int RS_ITEMS = 8;    // rounds of 64 keys per wave
-static constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
+// rounds of 64 keys per wave (tile = 256 * items keys); any of 4, 8, 16 sorts the same way,
+// the choice only trades histogram size against LDS and registers per tile
+static constexpr int RS_ITEMS_MIN = 4;
+static constexpr int RS_TILE_MIN = RS_BLOCK * RS_ITEMS_MIN;
 static constexpr int SCAN_BLOCK = 1024;
 static constexpr int SCAN_ITEMS = 4;
 static constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
 
-size_t radix_hist_ints(int n_cap) { return (size_t)256 * ((n_cap + RS_TILE - 1) / RS_TILE + 1); }
+size_t radix_hist_ints(int n_cap) { return (size_t)256 * ((n_cap + RS_TILE_MIN - 1) / RS_TILE_MIN + 1); }
+bool radix_items_valid(int items) { return items == 4 || items == 8 || items == 16; }
 size_t scan_tmp_ints(int n_cap) { return (size_t)((n_cap + SCAN_TILE - 1) / SCAN_TILE + 2) + 64; }
 
 // element count: the device-resident count clamped to the capacity, or the host count
@@ -138,8 +141,10 @@ void exclusive_scan(const int* in, int* out, const int* d_n, int n_host, int n_c
 // ---------------------------------------------------------------------------------------------
 // radix sort
 // ---------------------------------------------------------------------------------------------
+template <int RS_ITEMS>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_upsweep(const uint32_t* keys, const int* d_n, int n_host, int n_cap, int shift,
                                                           int nbits, int* hist) {
+    constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
     const int n = dev_n(d_n, n_host, n_cap);
     const int ntiles = (n + RS_TILE - 1) / RS_TILE;
     const int b = blockIdx.x;
@@ -159,10 +164,12 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_upsweep(const uint32_t* keys, c
     for (int d = threadIdx.x; d < radix; d += RS_BLOCK) hist[d * ntiles + b] = h[d];
 }
 
+template <int RS_ITEMS>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_downsweep(const uint32_t* __restrict__ kin, const int* __restrict__ vin,
                                                             uint32_t* __restrict__ kout, int* __restrict__ vout,
                                                             const int* d_n, int n_host, int n_cap, int shift, int nbits,
                                                             const int* __restrict__ hist) {
+    constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
     const int n = dev_n(d_n, n_host, n_cap);
     const int ntiles = (n + RS_TILE - 1) / RS_TILE;
     const int b = blockIdx.x;
@@ -246,11 +253,11 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_downsweep(const uint32_t* __res
     }
 }
 
-int radix_sort_pairs(uint32_t* k0, int* v0, const int* d_n, int n_host, int n_cap, int bits,
-                     const SortScratch& s, hipStream_t st) {
-    const int cap = d_n ? n_cap : n_host;
-    if (cap <= 0 || bits <= 0) return 0;
-    const int tiles = (cap + RS_TILE - 1) / RS_TILE;
+template <int I>
+static int radix_sort_tiles(uint32_t* k0, int* v0, const int* d_n, int n_host, int cap, int bits, const SortScratch& s,
+                            hipStream_t st) {
+    constexpr int TILE = RS_BLOCK * I;
+    const int tiles = (cap + TILE - 1) / TILE;
     const int passes = (bits + 7) / 8;
     const int width = (bits + passes - 1) / passes;
     uint32_t* ka = k0;
@@ -261,17 +268,28 @@ int radix_sort_pairs(uint32_t* k0, int* v0, const int* d_n, int n_host, int n_ca
     for (int p = 0; p < passes; ++p) {
         const int shift = p * width;
         const int nb = min(width, bits - shift);
-        hipLaunchKernelGGL(k_rs_upsweep, dim3(tiles), dim3(RS_BLOCK), 0, st, ka, d_n, n_host, cap, shift, nb, s.hist);
+        hipLaunchKernelGGL(k_rs_upsweep<I>, dim3(tiles), dim3(RS_BLOCK), 0, st, ka, d_n, n_host, cap, shift, nb, s.hist);
         // exclusive scan of the digit-major histogram: (ntiles * radix) entries
-        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(SCAN_BLOCK), 0, st, s.hist, d_n, n_host, cap, RS_TILE, 1 << nb,
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(SCAN_BLOCK), 0, st, s.hist, d_n, n_host, cap, TILE, 1 << nb,
                            (int*)nullptr);
-        hipLaunchKernelGGL(k_rs_downsweep, dim3(tiles), dim3(RS_BLOCK), 0, st, ka, va, kb, vb, d_n, n_host, cap, shift, nb,
-                           s.hist);
+        hipLaunchKernelGGL(k_rs_downsweep<I>, dim3(tiles), dim3(RS_BLOCK), 0, st, ka, va, kb, vb, d_n, n_host, cap, shift,
+                           nb, s.hist);
         uint32_t* tk = ka; ka = kb; kb = tk;
         int* tv = va; va = vb; vb = tv;
         which ^= 1;
     }
     return which;
+}
+
+int radix_sort_pairs(uint32_t* k0, int* v0, const int* d_n, int n_host, int n_cap, int bits, const SortScratch& s,
+                     hipStream_t st, int items) {
+    const int cap = d_n ? n_cap : n_host;
+    if (cap <= 0 || bits <= 0) return 0;
+    switch (items) {
+        case 4: return radix_sort_tiles<4>(k0, v0, d_n, n_host, cap, bits, s, st);
+        case 16: return radix_sort_tiles<16>(k0, v0, d_n, n_host, cap, bits, s, st);
+        default: return radix_sort_tiles<8>(k0, v0, d_n, n_host, cap, bits, s, st);
+    }
 }
 
 }  // namespace mpsr

@@ -168,7 +168,7 @@ template <> __device__ inline float4 mk4w<float>(float x, float y, float z) { re
 template <> __device__ inline double4 mk4w<double>(double x, double y, double z) { return make_double4(x, y, z, 0.0); }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_stl_distance(int N, Consts<R> c, const R4<R>* __restrict__ spos,
+__global__ __launch_bounds__(kMaxBlock) void k_stl_distance(int N, Consts<R> c, const R4<R>* __restrict__ spos,
                                                        const int* __restrict__ tstart, const int* __restrict__ tcount,
                                                        const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
                                                        R* stl_mag, R4<R>* stl_dir, int* stl_id) {
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_stl_distance(int N, Consts<R> c, const 
 
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_drag(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+__global__ __launch_bounds__(kMaxBlock) void k_drag(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
                                                PState<R> out, R4<R>* __restrict__ spos, R4<R>* __restrict__ svel,
                                                const int* __restrict__ tstart, const int* __restrict__ tcount,
                                                const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
@@ -261,11 +261,13 @@ __global__ __launch_bounds__(256) void k_translate(int T, R dt, TriD<R>* tris, c
     tris[t] = tr;
 }
 
-template <typename R> __global__ __launch_bounds__(256) void k_rotate(int T, TriD<R>* tris, const RotCoef<R>* __restrict__ rot) {
+template <typename R>
+__global__ __launch_bounds__(256) void k_rotate(int T, TriD<R>* tris, const RotCoef<R>* __restrict__ rot,
+                                                const int* __restrict__ step_ctr, int nmodel) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
     TriD<R> tr = tris[t];
-    const RotCoef<R> r = rot[tr.model];
+    const RotCoef<R> r = rot[(step_ctr ? (size_t)(*step_ctr) * nmodel : 0) + tr.model];
     if (!r.active) return;
     const R a = r.a, b = r.b, cc = r.c, u = r.u, v = r.v, w = r.w, cs = r.cs, sn = r.sn, omc = r.omc;
     for (int k = 0; k < 3; ++k) {
@@ -282,12 +284,12 @@ template <typename R> __global__ __launch_bounds__(256) void k_rotate(int T, Tri
 
 template <typename R> void launch_stl_distance(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
-    hipLaunchKernelGGL(k_stl_distance<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.spos,
+    hipLaunchKernelGGL(k_stl_distance<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.spos,
                        d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.stl_mag, d.stl_dir, d.stl_id);
 }
 template <typename R> void launch_drag(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
-    hipLaunchKernelGGL(k_drag<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, d.perm, d.in, d.out, d.spos,
+    hipLaunchKernelGGL(k_drag<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out, d.spos,
                        d.svel, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.models, d.drag, d.drag_acc,
                        d.drag_temp);
 }
@@ -298,8 +300,15 @@ template <typename R> void launch_translate(const Dev<R>& d, hipStream_t st) {
 }
 template <typename R> void launch_rotate(const Dev<R>& d, hipStream_t st) {
     if (d.ntri <= 0) return;
-    hipLaunchKernelGGL(k_rotate<R>, dim3(grid_for(d.ntri, 256)), dim3(256), 0, st, d.ntri, d.tris, d.rot);
+    hipLaunchKernelGGL(k_rotate<R>, dim3(grid_for(d.ntri, 256)), dim3(256), 0, st, d.ntri, d.tris, d.rot, d.step_ctr,
+                       d.nmodel > 0 ? d.nmodel : 1);
 }
+
+// advances the step counter of graph replays (one thread, a vector store)
+__global__ void k_step_advance(int* ctr) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[0] = ctr[0] + 1;
+}
+void launch_step_advance(int* ctr, hipStream_t st) { hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, st, ctr); }
 
 #define MPSR_INSTANTIATE_W(R)                                               \
     template void launch_tri_bin<R>(Dev<R>&, hipStream_t);                  \
