@@ -478,6 +478,8 @@ template <typename R> struct Engine : EngineBase {
         d.ndrag = ci->num_drag;
         d.c = make_consts<R>(P, gb, d.ntri, d.ndrag);
         pressure_mode = ci->pressure_mode;
+        if (ci->ppe.coarse_cells < 0 || ci->ppe.coarse_cells > 8)
+            return fail(MPSR_PARAMERR, "ppe.coarse_cells must be 0 (default 4) or 1..8");
         ppe_opt = ci->ppe;
         int bits = 1;
         while ((1ll << bits) <= (long long)d.c.ncell) ++bits;  // keys 0..ncell (sentinel = ncell)
@@ -1050,6 +1052,8 @@ template <typename R> struct Engine : EngineBase {
         return MPSR_NOERR;
     }
     mpsr_status set_ppe(const mpsr_ppe_options* o) override {
+        if (o->coarse_cells < 0 || o->coarse_cells > 8)
+            return fail(MPSR_PARAMERR, "ppe.coarse_cells must be 0 (default 4) or 1..8");
         ppe_opt = *o;
         return MPSR_NOERR;
     }
@@ -1070,6 +1074,7 @@ struct mpsr_handle {
 // launch.  The rotation row of a replay comes from each bucket's device step counter.
 struct mpsr_batch {
     std::vector<mpsr_handle*> h;
+    std::vector<int> order;            // recording order: largest bucket first
     int device = 0;
     hipStream_t main = nullptr;
     hipEvent_t fork = nullptr, e0 = nullptr, e1 = nullptr;
@@ -1110,7 +1115,8 @@ mpsr_status batch_capture(mpsr_batch* b) {
     HIPC(hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
     mpsr_status st = MPSR_NOERR;
     hipError_t he = hipEventRecord(b->fork, b->main);
-    for (size_t i = 0; i < b->h.size() && he == hipSuccess && st == MPSR_NOERR; ++i) {
+    for (size_t k = 0; k < b->order.size() && he == hipSuccess && st == MPSR_NOERR; ++k) {
+        const int i = b->order[k];
         EngineBase* e = b->h[i]->e;
         he = hipStreamWaitEvent(e->stream(), b->fork, 0);
         if (he != hipSuccess) break;
@@ -1176,7 +1182,7 @@ mpsr_status batch_run(mpsr_batch* b, int first, int count, bool graph) {
         // the same kernels launched directly, bucket by bucket on the buckets' own streams
         for (int k = 0; k < count; ++k) {
             HIPC(hipEventRecord(b->fork, b->main));
-            for (size_t i = 0; i < b->h.size(); ++i) {
+            for (const int i : b->order) {
                 EngineBase* e = b->h[i]->e;
                 HIPC(hipStreamWaitEvent(e->stream(), b->fork, 0));
                 s = e->batch_enqueue();
@@ -1270,6 +1276,10 @@ mpsr_status mpsr_create_batch(const mpsr_create_info* infos, int nbuckets, mpsr_
         delete b;
         return fail(MPSR_GPUERR, "mpsr_create_batch: stream or event creation failed");
     }
+    b->order.resize(b->h.size());
+    for (size_t i = 0; i < b->h.size(); ++i) b->order[i] = (int)i;
+    std::stable_sort(b->order.begin(), b->order.end(),
+                     [&](int x, int y) { return b->h[x]->e->count() > b->h[y]->e->count(); });
     b->join.resize(b->h.size(), nullptr);
     for (auto& e : b->join) {
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {

@@ -176,6 +176,45 @@ template <int NW> __device__ inline int block_exclusive_scan(int v, int* lds, in
 // ---- launch helpers shared by the translation units ------------------------------------------
 inline unsigned grid_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// PPE matrix layout (SELL-64/4): rows in slices of 64, every slice as wide as its longest row
+// rounded up to a multiple of 4, entries kept in groups of 4 per row: entry k of the slice's row
+// L sits at slice_base + 256 (k / 4) + 4 L + k % 4.  A lane's 4 consecutive entries are 16
+// contiguous bytes (int32 / FP32) and a wave's group of 4 entries is one contiguous 1 KB run.
+__host__ __device__ inline int sell_at(int slice_base, int lane, int k) {
+    return slice_base + ((k >> 2) << 8) + (lane << 2) + (k & 3);
+}
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+// four consecutive matrix entries (16-byte aligned), loaded non-temporally or stored at once
+__device__ inline void quad_load(const int* p, int* o) {
+    const v4i v = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(p));
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ inline void quad_load(const float* p, float* o) {
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ inline void quad_load(const double* p, double* o) {
+    const v2d a = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p));
+    const v2d b = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p + 2));
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ inline void quad_store(int* p, int a, int b, int c, int d) {
+    v4i v; v.x = a; v.y = b; v.z = c; v.w = d;
+    *reinterpret_cast<v4i*>(p) = v;
+}
+__device__ inline void quad_store(float* p, float a, float b, float c, float d) {
+    v4f v; v.x = a; v.y = b; v.z = c; v.w = d;
+    *reinterpret_cast<v4f*>(p) = v;
+}
+__device__ inline void quad_store(double* p, double a, double b, double c, double d) {
+    v2d u; u.x = a; u.y = b;
+    v2d v; v.x = c; v.y = d;
+    reinterpret_cast<v2d*>(p)[0] = u;
+    reinterpret_cast<v2d*>(p)[1] = v;
+}
+
 // sort.hip
 struct SortScratch {
     uint32_t* k1;

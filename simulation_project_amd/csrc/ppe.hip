@@ -202,12 +202,15 @@ __global__ __launch_bounds__(PB) void k_ppe_count(int N, int nslices, Consts<R> 
     const int rows = wave_sum((t < N && !dr) ? 1 : 0);
     const int slice = t >> 6;
     if (lane_id() == 0 && slice < nslices) {
-        slice_w[slice] = m * 64;
+        slice_w[slice] = ((m + 3) & ~3) * 64;
         atomicAdd(&nnz[0], tot);
         atomicAdd(&nnz[1], rows);
     }
 }
 
+// Fill: each row's entries in candidate order, written straight into SELL-64/4: a lane collects
+// 4 entries in registers and stores them as one 16-byte group (the groups of a lane are 1 KB
+// apart); padding entries up to the slice width repeat the row itself with value 0.
 template <typename R, typename VA>
 __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c, const int* __restrict__ active,
                                                   const int* __restrict__ dir, const R4<R>* __restrict__ spos,
@@ -220,6 +223,18 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
     if (slice >= nslices) return;
     const int base = slice_off[slice], width = slice_w[slice] >> 6;
     int k = 0;
+    int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    VA v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    auto push = [&](int col, VA val) {
+        c0 = c1; c1 = c2; c2 = c3; c3 = col;
+        v0 = v1; v1 = v2; v2 = v3; v3 = val;
+        ++k;
+        if ((k & 3) == 0) {
+            const int e = sell_at(base, lane, k - 4);
+            quad_store(cols + e, c0, c1, c2, c3);
+            quad_store(vals + e, v0, v1, v2, v3);
+        }
+    };
     if (t < N && !dir[t]) {
         const R4<R> P = spos[t];
         const R re = c.re_lap, mind = c.mind_lap;
@@ -229,19 +244,12 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
             if (d < re) {
                 if (d < mind) d = mind;
                 const R w = weight(re, d);
-                if (active[q] && k < width) {
-                    cols[base + k * 64 + lane] = q;
-                    vals[base + k * 64 + lane] = (VA)(-(double)w);
-                    ++k;
-                }
+                if (active[q] && k < width) push(q, (VA)(-(double)w));
             }
         });
     }
     const int self = t < N ? t : 0;
-    for (; k < width; ++k) {
-        cols[base + k * 64 + lane] = self;
-        vals[base + k * 64 + lane] = (VA)0;
-    }
+    while (k < width) push(self, (VA)0);
     if (t < N) rhs[t] = dir[t] ? 0.0 : alpha * fmax((double)out.n[perm[t]] - n0, 0.0) / n0;
 }
 
@@ -262,9 +270,33 @@ __global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __r
     if (t < N) {
         const int base = slice_off[slice], width = slice_w[slice] >> 6;
         double a = diag[t] * x[t];
-        for (int k = 0; k < width; ++k) {
-            const int e = base + k * 64 + lane;
-            a += (double)vals[e] * x[cols[e]];
+        // entries in ascending order, 8 column/value loads and then 8 gathers in flight
+        int k = 0;
+        for (; k + 8 <= width; k += 8) {
+            int c[8];
+            VA v[8];
+            const int e = sell_at(base, lane, k);
+            quad_load(cols + e, c);
+            quad_load(cols + e + 256, c + 4);
+            quad_load(vals + e, v);
+            quad_load(vals + e + 256, v + 4);
+            double g[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) g[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a += (double)v[u] * g[u];
+        }
+        if (k < width) {
+            int c[4];
+            VA v[4];
+            const int e = sell_at(base, lane, k);
+            quad_load(cols + e, c);
+            quad_load(vals + e, v);
+            double g[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) g[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a += (double)v[u] * g[u];
         }
         const double rv = b[t] - a;
         r[t] = rv;
@@ -323,25 +355,33 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int N, const int* __restrict__ 
     if (t < N) {
         const V pt = p[t];
         V a = (V)diag[t] * pt;
-        const int base = slice_off[slice] + lane, width = slice_w[slice] >> 6;
+        const int base = slice_off[slice], width = slice_w[slice] >> 6;
         int k = 0;
         for (; k + 8 <= width; k += 8) {
             int c[8];
             VA v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                c[u] = __builtin_nontemporal_load(&cols[base + (k + u) * 64]);
-                v[u] = __builtin_nontemporal_load(&vals[base + (k + u) * 64]);
-            }
+            const int e = sell_at(base, lane, k);
+            quad_load(cols + e, c);
+            quad_load(cols + e + 256, c + 4);
+            quad_load(vals + e, v);
+            quad_load(vals + e + 256, v + 4);
             V g[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) g[u] = p[c[u]];
 #pragma unroll
             for (int u = 0; u < 8; ++u) a += (V)v[u] * g[u];
         }
-        for (; k < width; ++k) {
-            const int e = base + k * 64;
-            a += (V)__builtin_nontemporal_load(&vals[e]) * p[__builtin_nontemporal_load(&cols[e])];
+        if (k < width) {   // one group of 4 left
+            int c[4];
+            VA v[4];
+            const int e = sell_at(base, lane, k);
+            quad_load(cols + e, c);
+            quad_load(vals + e, v);
+            V g[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) g[u] = p[c[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a += (V)v[u] * g[u];
         }
         q[t] = a;
         pq = (double)pt * (double)a;
@@ -563,9 +603,9 @@ __global__ __launch_bounds__(64) void k_coarse_assemble(int nc, AggGeom g, const
         for (int t = a0; t < a1; ++t) {
             if (dir[t]) continue;
             acc[13][lane] += diag[t];
-            const int base = slice_off[t >> 6] + (t & 63), n = cnt[t];
+            const int sb = slice_off[t >> 6], n = cnt[t];
             for (int k = 0; k < n; ++k) {
-                const int e = base + k * 64;
+                const int e = sell_at(sb, t & 63, k);
                 const int fj = full_of[agg_of_row[cols[e]]];
                 const int dx = fj / (g.cgy * g.cgz) - ax, dy = (fj / g.cgz) % g.cgy - ay, dz = fj % g.cgz - az;
                 // entries reach at most one aggregate away; anything else came from a cell-id
@@ -602,7 +642,10 @@ __global__ __launch_bounds__(PB) void k_fill_d(double* a, long long n, double v)
     if (i < n) a[i] = v;
 }
 
-// rc[I] = sum of r over the non-Dirichlet rows of aggregate I (one wave each)
+// rc[I] = sum of r over the non-Dirichlet rows of aggregate I (one wave each).  Cells are ordered
+// z fastest, so the a cells (i, j, k0 .. k0+a-1) of an aggregate hold one contiguous range of
+// sorted rows: lane L < a^2 finds the range of stack L, then the whole wave sweeps the a^2 ranges
+// with coalesced loads (the stacks in order, rows lane-strided inside a stack).
 template <typename V>
 __global__ __launch_bounds__(64) void k_restrict(int nc, AggGeom g, const int* __restrict__ full_of,
                                                   const int* __restrict__ cs, const int* __restrict__ ce,
@@ -611,16 +654,32 @@ __global__ __launch_bounds__(64) void k_restrict(int nc, AggGeom g, const int* _
     if (check_done && scal[S_DONE] != 0.0) return;
     const int I = blockIdx.x;
     if (I >= nc) return;
+    const int f = full_of[I], lane = lane_id();
+    const int ax = f / (g.cgy * g.cgz), ay = (f / g.cgz) % g.cgy, az = f % g.cgz;
+    const int stacks = g.a * g.a;
+    int lo = 0, hi = 0;
+    if (lane < stacks) {
+        const int i = ax * g.a + lane / g.a, j = ay * g.a + lane % g.a;
+        if (i < g.gx && j < g.gy) {
+            lo = -1;
+            for (int k = az * g.a; k < az * g.a + g.a && k < g.gz; ++k) {
+                const int cell = (i * g.gy + j) * g.gz + k;
+                const int a0 = cs[cell];
+                if (a0 < 0) continue;
+                if (lo < 0) lo = a0;
+                hi = ce[cell];
+            }
+            if (lo < 0) lo = 0;
+        }
+    }
     double acc = 0.0;
-    for_agg_cells(g, full_of[I], [&](int cell) {
-        const int a0 = cs[cell];
-        if (a0 < 0) return;
-        const int a1 = ce[cell];
-        for (int t = a0; t < a1; ++t)
+    for (int st = 0; st < stacks; ++st) {
+        const int b = __shfl(lo, st, 64), e = __shfl(hi, st, 64);
+        for (int t = b + lane; t < e; t += 64)
             if (!dir[t]) acc += (double)r[t];
-    });
+    }
     acc = wave_sum(acc);
-    if (lane_id() == 0) rc[I] = acc;
+    if (lane == 0) rc[I] = acc;
 }
 
 // cc = Ainv rc (one wave per row; Ainv symmetric so row I = column I, contiguous)
@@ -1405,7 +1464,7 @@ mpsr_status ppe_get_matrix(PpeSolver<R>* s, const Dev<R>& d, int* ptr, int* col,
         if (diag) diag[sl] = dg[t];
         const int n = dd[t] ? 0 : cn[t];
         for (int k = 0; k < n; ++k, ++e) {
-            const long long idx = (long long)so[t >> 6] + (long long)k * 64 + (t & 63);
+            const long long idx = sell_at(so[t >> 6], t & 63, k);
             if (e < cap) {
                 col[e] = pp[cc[idx]];
                 val[e] = vv[idx];

@@ -4,7 +4,7 @@
 // One phase per step, after the first binning (sorted fluid positions and velocities) and before
 // the predictor:
 //   k_sph_keys / sort / k_sph_ranges   spheres by cell (stable by sphere index)
-//   k_sph_sample   per sphere: fluid velocity at the centre, u_f = sum w v / sum w over the fluid
+//   k_sph_sample   per sphere (one wave): fluid velocity at the centre, u_f = sum w v / sum w over the fluid
 //                  particles within re_lap (MPS weight, canonical neighbour order), and the drag
 //                  F = 3 pi mu d (u_f - u_s)(1 + 0.15 Re^0.687) for Re < 1000, else Cd = 0.44
 //   k_sph_react    per fluid particle: dv = -sum_s F_s (w_js / W_s) dt / m_f over the spheres in
@@ -59,30 +59,64 @@ template <typename R> __device__ inline void sn_force(R dux, R duy, R duz, R d, 
     f[2] = (R)(k * uz);
 }
 
+// One wave per sphere: the lanes test the candidates of a stencil cell in parallel (lane L takes
+// the cell's L-th, (L+64)-th, ... particle), then every lane adds the hits in ascending candidate
+// order, read out of the hit lanes by shuffle, so the sums are the sequential canonical-order sums.
 template <typename R>
 __global__ __launch_bounds__(256) void k_sph_sample(int S, Consts<R> c, const R4<R>* __restrict__ pos,
                                                      const R4<R>* __restrict__ vel, const R4<R>* __restrict__ spos,
                                                      const R4<R>* __restrict__ svel, const int* __restrict__ cs,
                                                      const int* __restrict__ ce, R d, R rho, R nu, R4<R>* force,
                                                      R* wsum) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = lane_id();
     if (k >= S) return;
     const R4<R> X = pos[k];
     const R re = c.re_lap, mind = c.mind_lap;
     R w = 0, ux = 0, uy = 0, uz = 0;
-    for_candidates(c, cs, ce, X.x, X.y, X.z, -1, [&](int q) {
-        const R4<R> Q = spos[q];
-        R dd = dist3(X.x, X.y, X.z, Q.x, Q.y, Q.z);
-        if (dd < re) {
-            if (dd < mind) dd = mind;
-            const R wq = weight(re, dd);
-            const R4<R> V = svel[q];
-            w = w + wq;
-            ux = ux + V.x * wq;
-            uy = uy + V.y * wq;
-            uz = uz + V.z * wq;
+    int ci, cj, ck;
+    if (cell_coords(c, X.x, X.y, X.z, ci, cj, ck)) {
+        const long long cid = cell_of(c, ci, cj, ck);
+        if (cid >= 0 && cid < c.ncell) {
+            for (int kk = -1; kk <= 1; ++kk)
+                for (int jj = -1; jj <= 1; ++jj)
+                    for (int ii = -1; ii <= 1; ++ii) {
+                        const long long m = cell_of(c, ci + ii, cj + jj, ck + kk);
+                        if (m < 0 || m >= c.ncell) continue;
+                        const int a = cs[m];
+                        if (a < 0) continue;
+                        const int b = ce[m];
+                        for (int base = a; base < b; base += 64) {
+                            const int q = base + lane;
+                            R wq = 0, vx = 0, vy = 0, vz = 0;
+                            bool hit = false;
+                            if (q < b) {
+                                const R4<R> Q = spos[q];
+                                R dd = dist3(X.x, X.y, X.z, Q.x, Q.y, Q.z);
+                                if (dd < re) {
+                                    if (dd < mind) dd = mind;
+                                    wq = weight(re, dd);
+                                    const R4<R> V = svel[q];
+                                    vx = V.x * wq;
+                                    vy = V.y * wq;
+                                    vz = V.z * wq;
+                                    hit = true;
+                                }
+                            }
+                            unsigned long long m64 = __ballot(hit);
+                            while (m64) {
+                                const int h = __ffsll((long long)m64) - 1;
+                                m64 &= m64 - 1ull;
+                                w = w + __shfl(wq, h, 64);
+                                ux = ux + __shfl(vx, h, 64);
+                                uy = uy + __shfl(vy, h, 64);
+                                uz = uz + __shfl(vz, h, 64);
+                            }
+                        }
+                    }
         }
-    });
+    }
+    if (lane != 0) return;
     wsum[k] = w;
     R f[3] = {0, 0, 0};
     if (w > 0) {
@@ -169,7 +203,8 @@ template <typename R> void launch_spheres(Dev<R>& d, SphDev<R>& s, hipStream_t s
     fill_int(s.cs, -1, d.c.ncell, st);
     fill_int(s.ce, -1, d.c.ncell, st);
     hipLaunchKernelGGL(k_sph_ranges, gs, b, 0, st, s.S, (int)d.c.ncell, sk, s.cs, s.ce);
-    hipLaunchKernelGGL(k_sph_sample<R>, gs, b, 0, st, s.S, d.c, s.pos, s.vel, d.spos, d.svel, d.cell_start,
+    hipLaunchKernelGGL(k_sph_sample<R>, dim3(grid_for((long long)s.S * 64, 256)), b, 0, st, s.S, d.c, s.pos, s.vel,
+                       d.spos, d.svel, d.cell_start,
                        d.cell_end, s.d, s.rho_f, s.nu_f, s.force, s.wsum);
     if (d.N > 0)
         hipLaunchKernelGGL(k_sph_react<R>, dim3(grid_for(d.N, 256)), b, 0, st, d.N, d.c, d.skeys, d.spos, s.pos, s.perm,
