@@ -197,6 +197,7 @@ def main():
         "ppe_iterations": iters,
         "ppe_rel_residual_max": max(relres),
         "ppe_learned_fallbacks": learned_fb,
+        "ppe_entries": {"real": int(nnz), "stored": int(np.mean([t["ppe_stored"] for t in per_step]))},
         "plain_fp64_pcg": plain,
         "phase_ms": phases,
         "step_roofline": {"achieved": round(step_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

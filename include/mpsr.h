@@ -282,6 +282,7 @@ typedef struct mpsr_timings {
     int ppe_learned;         /* 1 when the network gave the warm start and preconditioner weights */
     int ppe_learned_fallback;/* 1 when it lost to the plain solver after 5 iterations (plain restart) */
     double sphere_ms;        /* sphere phase (profiling on) */
+    long long ppe_stored;    /* stored entries of the last PPE matrix (SELL padding included) */
 } mpsr_timings;
 
 typedef struct mpsr_handle mpsr_handle;

@@ -142,7 +142,8 @@ class Timings(C.Structure):
                 ("spmv_launches", C.c_int), ("spmv_bytes", C.c_double), ("ppe_nnz", C.c_longlong),
                 ("ppe_rows", C.c_int), ("ppe_fallbacks", C.c_int),
                 ("ppe_coarse_rows", C.c_int), ("ppe_coarse_rebuilt", C.c_int), ("ppe_setup_ms", C.c_double),
-                ("ppe_learned", C.c_int), ("ppe_learned_fallback", C.c_int), ("sphere_ms", C.c_double)]
+                ("ppe_learned", C.c_int), ("ppe_learned_fallback", C.c_int), ("sphere_ms", C.c_double),
+                ("ppe_stored", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

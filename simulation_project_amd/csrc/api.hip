@@ -839,6 +839,7 @@ template <typename R> struct Engine : EngineBase {
         t.spmv_launches = tm.spmv_launches;
         t.spmv_bytes = tm.spmv_bytes;
         t.ppe_nnz = tm.ppe_nnz;
+        t.ppe_stored = tm.ppe_stored;
         t.ppe_rows = tm.ppe_rows;
         t.ppe_fallbacks = tm.ppe_fallbacks;
         t.ppe_coarse_rows = tm.ppe_coarse_rows;

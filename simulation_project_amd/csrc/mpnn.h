@@ -20,6 +20,7 @@ constexpr int NN_F = 16, NN_H = 128, NN_M = 32, NN_OUT = 2;
 struct MpnnSys {
     int N = 0, nslices = 0, rows_active = 0;
     const int* slice_off = nullptr;
+    const int* pos = nullptr;      // SELL position of each row
     const int* slice_w = nullptr;
     const int* cols = nullptr;
     const int* cnt = nullptr;      // stored entries per row

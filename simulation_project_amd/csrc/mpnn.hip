@@ -170,12 +170,12 @@ __global__ __launch_bounds__(NB) void k_nn_feat(MpnnSys sys, const VA* __restric
     R4<R> P{}, V{};
     if (live) {
         const double dg = sys.diag[t];
-        const int sb = sys.slice_off[t >> 6];
+        const int sp = sys.pos[t], sb = sys.slice_off[sp >> 6];
         n = sys.cnt[t];
         P = geo.spos[t];
         V = geo.svel[t];
         for (int k = h; k < n; k += 32) {
-            const int e = sell_at(sb, t & 63, k);
+            const int e = sell_at(sb, sp & 63, k);
             const int j = sys.cols[e];
             const double a = -(double)vals[e] / dg;
             sa += a;
@@ -367,10 +367,10 @@ __global__ __launch_bounds__(256) void k_nn_agg(MpnnSys sys, const VA* __restric
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
     const bool live = !sys.dir[t];
     if (live) {
-        const int sb = sys.slice_off[t >> 6], n = sys.cnt[t];
+        const int P = sys.pos[t], sb = sys.slice_off[P >> 6], n = sys.cnt[t];
         const double dg = sys.diag[t];
         for (int q = g; q < n; q += 4) {
-            const int e = sell_at(sb, t & 63, q);
+            const int e = sell_at(sb, P & 63, q);
             const int j = sys.cols[e];
             const float a = (float)(-(double)vals[e] / (mode == 0 ? dg : sys.diag[j]));
             float x0, x1, x2, x3;
@@ -411,10 +411,10 @@ __global__ __launch_bounds__(NB) void k_nn_mv(MpnnSys sys, const VA* __restrict_
         y[t] = 0.0;
         return;
     }
-    const int sb = sys.slice_off[t >> 6], n = sys.cnt[t];
+    const int P = sys.pos[t], sb = sys.slice_off[P >> 6], n = sys.cnt[t];
     double a = sys.diag[t] * (scale ? scale[t] * x[t] : x[t]);
     for (int q = 0; q < n; ++q) {
-        const int e = sell_at(sb, t & 63, q);
+        const int e = sell_at(sb, P & 63, q);
         const int j = sys.cols[e];
         a += (double)vals[e] * (scale ? scale[j] * x[j] : x[j]);
     }

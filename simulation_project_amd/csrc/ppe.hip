@@ -53,6 +53,7 @@ namespace mpsr {
 
 static constexpr int PB = 256;          // block of the vector kernels (4 waves = 4 slices)
 static constexpr int CHUNK = 32;        // PCG iterations queued between two flag reads
+static constexpr int SELL_SIGMA = 256;  // rows ordered by stored length inside windows of this many
 enum Scal { S_RZ, S_PQ, S_ALPHA, S_BETA, S_RR, S_TARGET, S_DONE, S_ITERS, S_MAXIT, S_BB, S_RR0, S_COUNT };
 
 template <typename R> struct PpeSolver {
@@ -63,6 +64,8 @@ template <typename R> struct PpeSolver {
     int* dir = nullptr;
     int* cnt = nullptr;
     int* slice_w = nullptr;
+    int* pos = nullptr;          // [positions] SELL position of each row (rows ordered by length per window)
+    int* rowp = nullptr;         // [positions] row at each SELL position, -1 when empty
     int* slice_off = nullptr;
     int* total = nullptr;
     int* cols = nullptr;
@@ -87,6 +90,7 @@ template <typename R> struct PpeSolver {
     char why[256] = {0};         // what went wrong in the last solve
     int* nnz_dev = nullptr;      // [2]: off-diagonal entries, non-Dirichlet rows
     long long nnz_real = 0;
+    long long nnz_stored = 0;     // entries kept, SELL padding included
     int rows_real = 0;
     // SpMV timing (profiling)
     int profile = 0;
@@ -103,6 +107,11 @@ template <typename R> struct PpeSolver {
     int* nc_dev = nullptr;
     int nc = 0, nc_cap = 0;
     double* Ac = nullptr;        // [nc_cap^2] coarse matrix, then its inverse (column-major)
+    float* Af = nullptr;         // [nc_cap^2] the inverse in FP32 (the FP32 inner solve reads it)
+    int2* agg_rng = nullptr;     // [nc_cap * a^2] row range of every cell stack of an aggregate
+    double* cpart = nullptr;     // [nc_cap * a^3 * 27] per-cell slot sums of the coarse assembly
+    long long cpart_cap = 0;
+    int* rxyz = nullptr;         // [rows] packed aggregate coordinates of each row's aggregate
     double* rc = nullptr;        // [nc_cap] restricted residual
     double* cc = nullptr;        // [nc_cap] coarse correction
     int* info_dev = nullptr;
@@ -169,7 +178,7 @@ template <typename R>
 __global__ __launch_bounds__(PB) void k_ppe_count(int N, int nslices, Consts<R> c, const int* __restrict__ active,
                                                    const R4<R>* __restrict__ spos, const int* __restrict__ cs,
                                                    const int* __restrict__ ce, int* cnt, double* diag, int* dir,
-                                                   int* slice_w, int* nnz) {
+                                                   int* nnz) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int n = 0;
     double sum = 0.0;
@@ -195,32 +204,76 @@ __global__ __launch_bounds__(PB) void k_ppe_count(int N, int nslices, Consts<R> 
         diag[t] = dr ? 1.0 : sum;
         dir[t] = dr ? 1 : 0;
     }
-    int m = n;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
     const int tot = wave_sum(n);
     const int rows = wave_sum((t < N && !dr) ? 1 : 0);
-    const int slice = t >> 6;
-    if (lane_id() == 0 && slice < nslices) {
-        slice_w[slice] = ((m + 3) & ~3) * 64;
+    if (lane_id() == 0 && (t >> 6) < nslices) {
         atomicAdd(&nnz[0], tot);
         atomicAdd(&nnz[1], rows);
     }
 }
 
-// Fill: each row's entries in candidate order, written straight into SELL-64/4: a lane collects
-// 4 entries in registers and stores them as one 16-byte group (the groups of a lane are 1 KB
-// apart); padding entries up to the slice width repeat the row itself with value 0.
+// SELL-C-sigma order: inside every window of SELL_SIGMA rows the rows take positions by stored
+// length (descending, ties by row), so the 64 rows of a slice have similar lengths and little
+// padding.  One workgroup per window sorts the words ((2^20 - 2 - length) << 12 | row in window)
+// with a bitonic network in LDS (rows past N carry the largest word and sort last); the slice
+// width is the length of the slice's first (longest) row rounded up to 4.  Positions past the
+// window's last row are empty (row -1) and their slices have width 0.
+static constexpr int SELL_BLOCK = SELL_SIGMA < 1024 ? SELL_SIGMA : 1024;
+__global__ __launch_bounds__(SELL_BLOCK) void k_sell_order(int N, int nslices, const int* __restrict__ cnt,
+                                                           const int* __restrict__ dir, int* pos, int* rowp,
+                                                           int* slice_w) {
+    __shared__ uint32_t v[SELL_SIGMA];
+    const int w0 = blockIdx.x * SELL_SIGMA;
+    const int pc = min(SELL_SIGMA, nslices * 64 - w0);
+    constexpr uint32_t KTOP = (1u << 20) - 2u;
+    for (int i = threadIdx.x; i < SELL_SIGMA; i += SELL_BLOCK) {
+        const int t = w0 + i;
+        const int k = t < N ? (dir[t] ? 0 : min(cnt[t], (int)KTOP)) : -1;
+        v[i] = ((KTOP - (uint32_t)k) << 12) | (uint32_t)i;   // k = -1 gives the largest key
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= SELL_SIGMA; kk <<= 1)
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < SELL_SIGMA; i += SELL_BLOCK) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const uint32_t a = v[i], b = v[ixj];
+                    const bool up = (i & kk) == 0;
+                    if ((a > b) == up) {
+                        v[i] = b;
+                        v[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int p = threadIdx.x; p < pc; p += SELL_BLOCK) {
+        const uint32_t w = v[p];
+        const int t = w0 + (int)(w & 0xfffu);
+        const int P = w0 + p;
+        if (t < N) {
+            const int k = (int)(KTOP - (w >> 12));
+            pos[t] = P;
+            rowp[P] = t;
+            if ((p & 63) == 0) slice_w[P >> 6] = ((k + 3) & ~3) * 64;
+        } else {
+            rowp[P] = -1;
+            if ((p & 63) == 0) slice_w[P >> 6] = 0;
+        }
+    }
+}
+
 template <typename R, typename VA>
 __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c, const int* __restrict__ active,
                                                   const int* __restrict__ dir, const R4<R>* __restrict__ spos,
                                                   const int* __restrict__ cs, const int* __restrict__ ce,
                                                   const int* __restrict__ slice_off, const int* __restrict__ slice_w,
                                                   const int* __restrict__ perm, PState<R> out, double alpha, double n0,
-                                                  int* cols, VA* vals, double* rhs) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int slice = t >> 6, lane = lane_id();
+                                                  const int* __restrict__ rowp, int* cols, VA* vals, double* rhs) {
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = P >> 6, lane = lane_id();
     if (slice >= nslices) return;
+    const int t = rowp[P];   // -1: empty position
     const int base = slice_off[slice], width = slice_w[slice] >> 6;
     int k = 0;
     int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
@@ -235,12 +288,12 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
             quad_store(vals + e, v0, v1, v2, v3);
         }
     };
-    if (t < N && !dir[t]) {
-        const R4<R> P = spos[t];
+    if (t >= 0 && !dir[t]) {
+        const R4<R> X = spos[t];
         const R re = c.re_lap, mind = c.mind_lap;
-        for_candidates(c, cs, ce, P.x, P.y, P.z, t, [&](int q) {
+        for_candidates(c, cs, ce, X.x, X.y, X.z, t, [&](int q) {
             const R4<R> Q = spos[q];
-            R d = dist3(P.x, P.y, P.z, Q.x, Q.y, Q.z);
+            R d = dist3(X.x, X.y, X.z, Q.x, Q.y, Q.z);
             if (d < re) {
                 if (d < mind) d = mind;
                 const R w = weight(re, d);
@@ -248,9 +301,10 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
             }
         });
     }
-    const int self = t < N ? t : 0;
+    const int self = t >= 0 ? t : 0;
     while (k < width) push(self, (VA)0);
-    if (t < N) rhs[t] = dir[t] ? 0.0 : alpha * fmax((double)out.n[perm[t]] - n0, 0.0) / n0;
+    if (t >= 0) rhs[t] = dir[t] ? 0.0 : alpha * fmax((double)out.n[perm[t]] - n0, 0.0) / n0;
+    (void)N;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -262,12 +316,13 @@ __global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __r
                                                const int* __restrict__ slice_w, const int* __restrict__ cols,
                                                const VA* __restrict__ vals, const double* __restrict__ diag,
                                                const double* __restrict__ x, const double* __restrict__ b,
-                                               double* r, double* part) {
+                                               const int* __restrict__ rowp, double* r, double* part) {
     __shared__ double lds[PB / 64];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int slice = t >> 6, lane = lane_id();
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;   // SELL position
+    const int slice = P >> 6, lane = lane_id();
+    const int t = slice < nslices ? rowp[P] : -1;
     double rr = 0.0, bb = 0.0;
-    if (t < N) {
+    if (t >= 0) {
         const int base = slice_off[slice], width = slice_w[slice] >> 6;
         double a = diag[t] * x[t];
         // entries in ascending order, 8 column/value loads and then 8 gathers in flight
@@ -309,7 +364,7 @@ __global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __r
         part[2 * blockIdx.x] = rr;
         part[2 * blockIdx.x + 1] = bb;
     }
-    (void)nslices;
+    (void)N;
 }
 
 // start of a PCG on A e = r0 (vectors V): r = r0, z = r / diag, p = z, partial r.z and r.r
@@ -342,17 +397,19 @@ __global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restric
 // flight (the loop is latency-bound otherwise).  Matrix streams are loaded non-temporally: they
 // are read once per SpMV and would only evict p from L2.
 template <typename VA, typename V>
-__global__ __launch_bounds__(PB) void k_pcg_spmv(int N, const int* __restrict__ slice_off,
+__global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restrict__ rowp,
+                                                  const int* __restrict__ slice_off,
                                                   const int* __restrict__ slice_w, const int* __restrict__ cols,
                                                   const VA* __restrict__ vals, const double* __restrict__ diag,
                                                   const V* __restrict__ p, V* q, const double* __restrict__ scal,
                                                   double* part) {
     if (scal[S_DONE] != 0.0) return;
     __shared__ double lds[PB / 64];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int slice = t >> 6, lane = lane_id();
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;   // SELL position
+    const int slice = P >> 6, lane = lane_id();
+    const int t = slice < nslices ? rowp[P] : -1;
     double pq = 0.0;
-    if (t < N) {
+    if (t >= 0) {
         const V pt = p[t];
         V a = (V)diag[t] * pt;
         const int base = slice_off[slice], width = slice_w[slice] >> 6;
@@ -568,68 +625,102 @@ __global__ __launch_bounds__(PB) void k_agg_list(int nfull, const int* __restric
     if (f < nfull && flag[f]) full_of[map[f]] = f;
 }
 
-// visit the cells of aggregate (full id f), lane-strided; F(cell)
-template <typename F> __device__ inline void for_agg_cells(const AggGeom& g, int f, F&& fn) {
-    const int ax = f / (g.cgy * g.cgz), ay = (f / g.cgz) % g.cgy, az = f % g.cgz;
-    const int n = g.a * g.a * g.a;
-    for (int c = lane_id(); c < n; c += 64) {
-        const int i = ax * g.a + c / (g.a * g.a), j = ay * g.a + (c / g.a) % g.a, k = az * g.a + c % g.a;
-        if (i < g.gx && j < g.gy && k < g.gz) fn((i * g.gy + j) * g.gz + k);
+// Aggregate coordinates of every row's aggregate, packed (ax << 20 | ay << 10 | az), or -1 for
+// rows outside the aggregates
+__global__ __launch_bounds__(PB) void k_row_aggxyz(int N, AggGeom g, const int* __restrict__ agg_of_row,
+                                                    const int* __restrict__ full_of, int* xyz) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int I = agg_of_row[t];
+    int v = -1;
+    if (I >= 0) {
+        const int f = full_of[I];
+        const int ax = f / (g.cgy * g.cgz), ay = (f / g.cgz) % g.cgy, az = f % g.cgz;
+        v = (ax << 20) | (ay << 10) | az;
     }
+    xyz[t] = v;
 }
 
-// Ac (column-major, nc x nc, zeroed before): one wave per aggregate I sums, per neighbour slot
-// (dx, dy, dz) in {-1,0,1}^3, the diagonal and off-diagonal entries of its rows
+// Ac = P^T A P, part 1: one wave per (aggregate I, cell c of its a^3 cells), lanes over the cell's
+// rows.  A row's entries come in stencil order, so consecutive entries mostly fall in the same
+// neighbour slot (dx, dy, dz) in {-1,0,1}^3: each lane sums a run of one slot in a register and
+// adds it to its LDS column when the slot changes.  The 27 slot sums of the cell go to
+// part[(I * a^3 + c) * 27 + slot].
 template <typename VA>
-__global__ __launch_bounds__(64) void k_coarse_assemble(int nc, AggGeom g, const int* __restrict__ full_of,
-                                                         const int* __restrict__ flag, const int* __restrict__ map,
-                                                         const int* __restrict__ cs, const int* __restrict__ ce,
-                                                         const int* __restrict__ dir, const int* __restrict__ cnt,
-                                                         const double* __restrict__ diag,
-                                                         const int* __restrict__ slice_off,
-                                                         const int* __restrict__ cols, const VA* __restrict__ vals,
-                                                         const int* __restrict__ agg_of_row, double* Ac) {
-    const int I = blockIdx.x;
+__global__ __launch_bounds__(64) void k_coarse_cells(int nc, AggGeom g, const int* __restrict__ full_of,
+                                                      const int* __restrict__ cs, const int* __restrict__ ce,
+                                                      const int* __restrict__ dir, const int* __restrict__ cnt,
+                                                      const double* __restrict__ diag,
+                                                      const int* __restrict__ slice_off, const int* __restrict__ pos,
+                                                      const int* __restrict__ cols, const VA* __restrict__ vals,
+                                                      const int* __restrict__ xyz, double* part) {
+    const int cells = g.a * g.a * g.a;
+    const int I = blockIdx.x / cells, c = blockIdx.x % cells;
     if (I >= nc) return;
     const int f = full_of[I];
     const int ax = f / (g.cgy * g.cgz), ay = (f / g.cgz) % g.cgy, az = f % g.cgz;
-    __shared__ double acc[27][64];   // per-lane running sums, one column per lane
+    __shared__ double acc[27][64];
     const int lane = lane_id();
     for (int q = 0; q < 27; ++q) acc[q][lane] = 0.0;
-    for_agg_cells(g, f, [&](int cell) {
+    const int i = ax * g.a + c / (g.a * g.a), j = ay * g.a + (c / g.a) % g.a, k = az * g.a + c % g.a;
+    if (i < g.gx && j < g.gy && k < g.gz) {
+        const int cell = (i * g.gy + j) * g.gz + k;
         const int a0 = cs[cell];
-        if (a0 < 0) return;
-        const int a1 = ce[cell];
-        for (int t = a0; t < a1; ++t) {
+        const int a1 = a0 < 0 ? a0 : ce[cell];
+        for (int t = a0 + lane; t < a1; t += 64) {
             if (dir[t]) continue;
             acc[13][lane] += diag[t];
-            const int sb = slice_off[t >> 6], n = cnt[t];
-            for (int k = 0; k < n; ++k) {
-                const int e = sell_at(sb, t & 63, k);
-                const int fj = full_of[agg_of_row[cols[e]]];
-                const int dx = fj / (g.cgy * g.cgz) - ax, dy = (fj / g.cgz) % g.cgy - ay, dz = fj % g.cgz - az;
-                // entries reach at most one aggregate away; anything else came from a cell-id
-                // wrap at the grid edge and is left out of the preconditioner
-                if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;
-                acc[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)][lane] += (double)vals[e];
+            const int P = pos[t], sb = slice_off[P >> 6], n = cnt[t];
+            int cur = -1;
+            double run = 0.0;
+            for (int e4 = 0; e4 < n; ++e4) {
+                const int e = sell_at(sb, P & 63, e4);
+                const int v = xyz[cols[e]];
+                int slot = -1;
+                if (v >= 0) {
+                    const int dx = (v >> 20) - ax, dy = ((v >> 10) & 1023) - ay, dz = (v & 1023) - az;
+                    // entries reach at most one aggregate away; anything else came from a cell-id
+                    // wrap at the grid edge and is left out of the preconditioner
+                    if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1)
+                        slot = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+                }
+                if (slot != cur) {
+                    if (cur >= 0) acc[cur][lane] += run;
+                    cur = slot;
+                    run = 0.0;
+                }
+                if (slot >= 0) run += (double)vals[e];
             }
+            if (cur >= 0) acc[cur][lane] += run;
         }
-    });
+    }
     __syncthreads();
     if (lane < 27) {
         double sum = 0.0;
         for (int l = 0; l < 64; ++l) sum += acc[lane][l];
-        const int q = lane;
-        const int dx = q / 9 - 1, dy = (q / 3) % 3 - 1, dz = q % 3 - 1;
-        const int bx = ax + dx, by = ay + dy, bz = az + dz;
-        if (bx >= 0 && by >= 0 && bz >= 0 && bx < g.cgx && by < g.cgy && bz < g.cgz) {
-            const int fb = (bx * g.cgy + by) * g.cgz + bz;
-            if (flag[fb]) Ac[(size_t)map[fb] * nc + I] = sum;
-        }
+        part[((size_t)I * cells + c) * 27 + lane] = sum;
     }
 }
 
-// rocsolver_dpotri leaves the inverse in the lower triangle (column-major): mirror it
+// Ac, part 2: entry (neighbour of I in slot q, I) = sum over the a^3 cells of I in cell order
+__global__ __launch_bounds__(PB) void k_coarse_gather(int nc, AggGeom g, const int* __restrict__ full_of,
+                                                       const int* __restrict__ flag, const int* __restrict__ map,
+                                                       const double* __restrict__ part, double* Ac) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nc * 27) return;
+    const int I = id / 27, q = id % 27;
+    const int cells = g.a * g.a * g.a;
+    double sum = 0.0;
+    for (int c = 0; c < cells; ++c) sum += part[((size_t)I * cells + c) * 27 + q];
+    const int f = full_of[I];
+    const int ax = f / (g.cgy * g.cgz), ay = (f / g.cgz) % g.cgy, az = f % g.cgz;
+    const int bx = ax + q / 9 - 1, by = ay + (q / 3) % 3 - 1, bz = az + q % 3 - 1;
+    if (bx >= 0 && by >= 0 && bz >= 0 && bx < g.cgx && by < g.cgy && bz < g.cgz) {
+        const int fb = (bx * g.cgy + by) * g.cgz + bz;
+        if (flag[fb]) Ac[(size_t)map[fb] * nc + I] = sum;
+    }
+}
+
 __global__ __launch_bounds__(PB) void k_symmetrize(int nc, double* A) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)nc * nc) return;
@@ -637,45 +728,70 @@ __global__ __launch_bounds__(PB) void k_symmetrize(int nc, double* A) {
     if (i < j) A[(size_t)j * nc + i] = A[(size_t)i * nc + j];
 }
 
+// FP32 copy of the coarse inverse: entries (I, J) and (J, I) are equal, so they round alike and
+// the copy stays symmetric
+__global__ __launch_bounds__(PB) void k_d2f(long long n, const double* __restrict__ a, float* b) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
 __global__ __launch_bounds__(PB) void k_fill_d(double* a, long long n, double v) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
 }
 
-// rc[I] = sum of r over the non-Dirichlet rows of aggregate I (one wave each).  Cells are ordered
-// z fastest, so the a cells (i, j, k0 .. k0+a-1) of an aggregate hold one contiguous range of
-// sorted rows: lane L < a^2 finds the range of stack L, then the whole wave sweeps the a^2 ranges
-// with coalesced loads (the stacks in order, rows lane-strided inside a stack).
+// Row ranges of the aggregates' cell stacks for this solve's binning.  Cells are ordered z
+// fastest, so the a cells (i, j, k0 .. k0+a-1) of aggregate I hold one contiguous range of sorted
+// rows: rng[I * a^2 + s] = [first row, end) of stack s (empty when its cells are).
+__global__ __launch_bounds__(PB) void k_agg_ranges(int nc, AggGeom g, const int* __restrict__ full_of,
+                                                    const int* __restrict__ cs, const int* __restrict__ ce, int2* rng) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stacks = g.a * g.a;
+    if (id >= nc * stacks) return;
+    const int I = id / stacks, st = id % stacks;
+    const int f = full_of[I];
+    const int ax = f / (g.cgy * g.cgz), ay = (f / g.cgz) % g.cgy, az = f % g.cgz;
+    const int i = ax * g.a + st / g.a, j = ay * g.a + st % g.a;
+    int lo = -1, hi = 0;
+    if (i < g.gx && j < g.gy) {
+        for (int k = az * g.a; k < az * g.a + g.a && k < g.gz; ++k) {
+            const int cell = (i * g.gy + j) * g.gz + k;
+            const int a0 = cs[cell];
+            if (a0 < 0) continue;
+            if (lo < 0) lo = a0;
+            hi = ce[cell];
+        }
+    }
+    if (lo < 0) lo = hi = 0;
+    rng[id] = make_int2(lo, hi);
+}
+
+// rc[I] = sum of r over the non-Dirichlet rows of aggregate I: one wave per aggregate, 64 / a^2
+// lanes per cell stack, each lane with 4 rows in flight
 template <typename V>
-__global__ __launch_bounds__(64) void k_restrict(int nc, AggGeom g, const int* __restrict__ full_of,
-                                                  const int* __restrict__ cs, const int* __restrict__ ce,
+__global__ __launch_bounds__(64) void k_restrict(int nc, int stacks, const int2* __restrict__ rng,
                                                   const int* __restrict__ dir, const V* __restrict__ r, double* rc,
                                                   const double* __restrict__ scal, int check_done) {
     if (check_done && scal[S_DONE] != 0.0) return;
     const int I = blockIdx.x;
     if (I >= nc) return;
-    const int f = full_of[I], lane = lane_id();
-    const int ax = f / (g.cgy * g.cgz), ay = (f / g.cgz) % g.cgy, az = f % g.cgz;
-    const int stacks = g.a * g.a;
-    int lo = 0, hi = 0;
-    if (lane < stacks) {
-        const int i = ax * g.a + lane / g.a, j = ay * g.a + lane % g.a;
-        if (i < g.gx && j < g.gy) {
-            lo = -1;
-            for (int k = az * g.a; k < az * g.a + g.a && k < g.gz; ++k) {
-                const int cell = (i * g.gy + j) * g.gz + k;
-                const int a0 = cs[cell];
-                if (a0 < 0) continue;
-                if (lo < 0) lo = a0;
-                hi = ce[cell];
-            }
-            if (lo < 0) lo = 0;
-        }
-    }
+    const int lane = lane_id();
+    const int lps = 64 / stacks;
+    const int st = lane / lps, sub = lane % lps;
     double acc = 0.0;
-    for (int st = 0; st < stacks; ++st) {
-        const int b = __shfl(lo, st, 64), e = __shfl(hi, st, 64);
-        for (int t = b + lane; t < e; t += 64)
+    if (st < stacks) {
+        const int2 b = rng[(size_t)I * stacks + st];
+        int t = b.x + sub;
+        for (; t + 3 * lps < b.y; t += 4 * lps) {
+            const int t1 = t + lps, t2 = t + 2 * lps, t3 = t + 3 * lps;
+            const int d0 = dir[t], d1 = dir[t1], d2 = dir[t2], d3 = dir[t3];
+            const V r0 = r[t], r1 = r[t1], r2 = r[t2], r3 = r[t3];
+            if (!d0) acc += (double)r0;
+            if (!d1) acc += (double)r1;
+            if (!d2) acc += (double)r2;
+            if (!d3) acc += (double)r3;
+        }
+        for (; t < b.y; t += lps)
             if (!dir[t]) acc += (double)r[t];
     }
     acc = wave_sum(acc);
@@ -683,14 +799,15 @@ __global__ __launch_bounds__(64) void k_restrict(int nc, AggGeom g, const int* _
 }
 
 // cc = Ainv rc (one wave per row; Ainv symmetric so row I = column I, contiguous)
-__global__ __launch_bounds__(64) void k_coarse_mv(int nc, const double* __restrict__ Ainv, const double* __restrict__ rc,
+template <typename M>
+__global__ __launch_bounds__(64) void k_coarse_mv(int nc, const M* __restrict__ Ainv, const double* __restrict__ rc,
                                                    double* cc, const double* __restrict__ scal, int check_done) {
     if (check_done && scal[S_DONE] != 0.0) return;
     const int I = blockIdx.x;
     if (I >= nc) return;
-    const double* row = Ainv + (size_t)I * nc;
+    const M* row = Ainv + (size_t)I * nc;
     double acc = 0.0;
-    for (int J = lane_id(); J < nc; J += 64) acc += row[J] * rc[J];
+    for (int J = lane_id(); J < nc; J += 64) acc += (double)row[J] * rc[J];
     acc = wave_sum(acc);
     if (lane_id() == 0) cc[I] = acc;
 }
@@ -767,6 +884,8 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     bool ok = true;
     ok &= dev_alloc(&s->active, rows) && dev_alloc(&s->dir, rows) && dev_alloc(&s->cnt, rows);
     ok &= dev_alloc(&s->slice_w, s->nslices + 1) && dev_alloc(&s->slice_off, s->nslices + 1) && dev_alloc(&s->total, 1);
+    const int positions = ((d.cap + SELL_SIGMA - 1) / SELL_SIGMA) * SELL_SIGMA;
+    ok &= dev_alloc(&s->pos, positions) && dev_alloc(&s->rowp, positions);
     ok &= dev_alloc(&s->diag, rows) && dev_alloc(&s->rhs, rows) && dev_alloc(&s->x, rows) && dev_alloc(&s->prev, rows);
     ok &= dev_alloc(&s->r, rows) && dev_alloc(&s->z, rows) && dev_alloc(&s->p0, rows) && dev_alloc(&s->p1, rows) &&
           dev_alloc(&s->q, rows);
@@ -798,6 +917,7 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     s->nfull = s->cgx * s->cgy * s->cgz;
     ok &= dev_alloc(&s->agg_flag, s->nfull) && dev_alloc(&s->agg_map, s->nfull) && dev_alloc(&s->agg_full, s->nfull);
     ok &= dev_alloc(&s->agg_of_row, rows) && dev_alloc(&s->nc_dev, 1) && dev_alloc(&s->info_dev, 2);
+    ok &= dev_alloc(&s->rxyz, rows);
     s->agg_scan.scan_tmp_cap = (int)scan_tmp_ints(s->nfull + 1);
     ok &= dev_alloc(&s->agg_scan.scan_tmp, s->agg_scan.scan_tmp_cap);
     if (!ok) {
@@ -828,13 +948,17 @@ static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hi
     std::memcpy(&nc, &s->scal_h[S_COUNT - 1], sizeof(int));
     s->nc = nc;
     if (nc <= 0 || nc > NC_MAX) return false;
+    if (g.cgx >= 1024 || g.cgy >= 1024 || g.cgz >= 1024) return false;   // packed coordinates hold 10 bits
     if (nc > s->nc_cap) {
         if (s->Ac) (void)hipFree(s->Ac);
+        if (s->Af) (void)hipFree(s->Af);
+        if (s->agg_rng) (void)hipFree(s->agg_rng);
         if (s->rc) (void)hipFree(s->rc);
         if (s->cc) (void)hipFree(s->cc);
-        s->Ac = nullptr; s->rc = nullptr; s->cc = nullptr;
+        s->Ac = nullptr; s->Af = nullptr; s->agg_rng = nullptr; s->rc = nullptr; s->cc = nullptr;
         s->nc_cap = std::min(NC_MAX, nc + nc / 4 + 64);
-        if (!dev_alloc(&s->Ac, (size_t)s->nc_cap * s->nc_cap) || !dev_alloc(&s->rc, s->nc_cap) ||
+        if (!dev_alloc(&s->Ac, (size_t)s->nc_cap * s->nc_cap) || !dev_alloc(&s->Af, (size_t)s->nc_cap * s->nc_cap) ||
+            !dev_alloc(&s->agg_rng, (size_t)s->nc_cap * 64) || !dev_alloc(&s->rc, s->nc_cap) ||
             !dev_alloc(&s->cc, s->nc_cap)) {
             s->nc_cap = 0;
             return false;
@@ -848,9 +972,23 @@ static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hi
                        s->agg_map, s->agg_of_row);
     const long long n2 = (long long)nc * nc;
     hipLaunchKernelGGL(k_fill_d, dim3(grid_for(n2, PB)), dim3(PB), 0, st, s->Ac, n2, 0.0);
-    hipLaunchKernelGGL((k_coarse_assemble<VA>), dim3(nc), dim3(64), 0, st, nc, g, s->agg_full, s->agg_flag, s->agg_map,
-                       d.cell_start, d.cell_end, s->dir, s->cnt, s->diag, s->slice_off, s->cols, vals, s->agg_of_row,
-                       s->Ac);
+    const int cells = g.a * g.a * g.a;
+    const long long need = (long long)nc * cells * 27;
+    if (need > s->cpart_cap) {
+        if (s->cpart) (void)hipFree(s->cpart);
+        s->cpart = nullptr;
+        s->cpart_cap = need + need / 4;
+        if (!dev_alloc(&s->cpart, (size_t)s->cpart_cap)) {
+            s->cpart_cap = 0;
+            return false;
+        }
+    }
+    hipLaunchKernelGGL(k_row_aggxyz, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, g, s->agg_of_row, s->agg_full, s->rxyz);
+    hipLaunchKernelGGL((k_coarse_cells<VA>), dim3((unsigned)(nc * cells)), dim3(64), 0, st, nc, g, s->agg_full,
+                       d.cell_start, d.cell_end, s->dir, s->cnt, s->diag, s->slice_off, s->pos, s->cols, vals, s->rxyz,
+                       s->cpart);
+    hipLaunchKernelGGL(k_coarse_gather, dim3(grid_for((long long)nc * 27, PB)), dim3(PB), 0, st, nc, g, s->agg_full,
+                       s->agg_flag, s->agg_map, s->cpart, s->Ac);
     if (rocsolver_dpotrf(s->blas, rocblas_fill_lower, nc, s->Ac, nc, s->info_dev) != rocblas_status_success) return false;
     if (rocsolver_dpotri(s->blas, rocblas_fill_lower, nc, s->Ac, nc, s->info_dev + 1) != rocblas_status_success)
         return false;
@@ -859,6 +997,7 @@ static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hi
     if (hipStreamSynchronize(st) != hipSuccess) return false;
     if (info[0] != 0 || info[1] != 0) return false;
     hipLaunchKernelGGL(k_symmetrize, dim3(grid_for(n2, PB)), dim3(PB), 0, st, nc, s->Ac);
+    hipLaunchKernelGGL(k_d2f, dim3(grid_for(n2, PB)), dim3(PB), 0, st, n2, s->Ac, s->Af);
     return true;
 }
 
@@ -877,10 +1016,15 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
         return -1;
     if (hipMemcpyAsync(s->scal + S_MAXIT, s->scal_h + S_MAXIT, sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
         return -1;
+    // the FP32 inner solve applies the FP32 copy of the coarse inverse
+    const bool f32 = std::is_same<V, float>::value;
+    const int stacks = g.a * g.a;
+    hipLaunchKernelGGL(k_agg_ranges, dim3(grid_for((long long)nc * stacks, PB)), dim3(PB), 0, st, nc, g, s->agg_full,
+                       d.cell_start, d.cell_end, s->agg_rng);
     hipLaunchKernelGGL(k_pcg_start2<V>, dim3(nb), dim3(PB), 0, st, N, r0, r, x, zero_x);
-    hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, g, s->agg_full, d.cell_start, d.cell_end, s->dir,
-                       r, s->rc, s->scal, 0);
-    hipLaunchKernelGGL(k_coarse_mv, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 0);
+    hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, stacks, s->agg_rng, s->dir, r, s->rc, s->scal, 0);
+    if (f32) hipLaunchKernelGGL(k_coarse_mv<float>, dim3(nc), dim3(64), 0, st, nc, s->Af, s->rc, s->cc, s->scal, 0);
+    else hipLaunchKernelGGL(k_coarse_mv<double>, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 0);
     hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, s->agg_of_row, s->cc, z, p, 1, s->part,
                        s->scal, 0);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_START);
@@ -893,7 +1037,7 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
         const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
-            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, N, s->slice_off, s->slice_w, s->cols,
+            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->slice_off, s->slice_w, s->cols,
                                vals, s->diag, p, q, s->scal, s->part);
             if (timed) {
                 (void)hipEventRecord(s->ev[s->ev_used + 1], st);
@@ -901,9 +1045,10 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
             }
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 1, s->scal, (int)RED_ALPHA);
             hipLaunchKernelGGL(k_pcg_update2<V>, dim3(nb), dim3(PB), 0, st, N, x, r, p, q, s->scal);
-            hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, g, s->agg_full, d.cell_start, d.cell_end,
-                               s->dir, r, s->rc, s->scal, 1);
-            hipLaunchKernelGGL(k_coarse_mv, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 1);
+            hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, stacks, s->agg_rng, s->dir, r, s->rc,
+                               s->scal, 1);
+            if (f32) hipLaunchKernelGGL(k_coarse_mv<float>, dim3(nc), dim3(64), 0, st, nc, s->Af, s->rc, s->cc, s->scal, 1);
+            else hipLaunchKernelGGL(k_coarse_mv<double>, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 1);
             hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, s->agg_of_row, s->cc, z, p, 0,
                                s->part, s->scal, 1);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_BETA);
@@ -931,10 +1076,10 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
 
 template <typename R> void ppe_destroy(PpeSolver<R>* s) {
     if (!s) return;
-    void* ps[] = {s->active, s->dir, s->cnt, s->slice_w, s->slice_off, s->total, s->cols, s->vals64, s->vals32,
+    void* ps[] = {s->active, s->dir, s->cnt, s->slice_w, s->slice_off, s->pos, s->rowp, s->total, s->cols, s->vals64, s->vals32,
                   s->diag, s->rhs, s->x, s->prev, s->r, s->z, s->p0, s->p1, s->q, s->rf, s->zf, s->p0f,
                   s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp, s->nnz_dev, s->agg_flag,
-                  s->agg_map, s->agg_full, s->agg_of_row, s->nc_dev, s->info_dev, s->Ac, s->rc, s->cc,
+                  s->agg_map, s->agg_full, s->agg_of_row, s->nc_dev, s->info_dev, s->Ac, s->Af, s->agg_rng, s->cpart, s->rxyz, s->rc, s->cc,
                   s->agg_scan.scan_tmp, s->pdiag_buf, s->omega, s->xprev_s, s->xl};
     if (s->net) mpnn_destroy(s->net);
     for (void* p : ps)
@@ -988,7 +1133,7 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
         const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
-            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, N, s->slice_off, s->slice_w, s->cols,
+            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->slice_off, s->slice_w, s->cols,
                                vals, s->diag, pa, q, s->scal, s->part);
             if (timed) {
                 (void)hipEventRecord(s->ev[s->ev_used + 1], st);
@@ -1025,10 +1170,10 @@ template <typename R> static double resid_pass(PpeSolver<R>* s, int N, hipStream
     const int nb = (rows + PB - 1) / PB;
     if (s->mixed)
         hipLaunchKernelGGL(k_resid<float>, dim3(nb), dim3(PB), 0, st, N, s->nslices, s->slice_off, s->slice_w, s->cols,
-                           s->vals32, s->diag, s->x, s->rhs, s->r, s->part);
+                           s->vals32, s->diag, s->x, s->rhs, s->rowp, s->r, s->part);
     else
         hipLaunchKernelGGL(k_resid<double>, dim3(nb), dim3(PB), 0, st, N, s->nslices, s->slice_off, s->slice_w,
-                           s->cols, s->vals64, s->diag, s->x, s->rhs, s->r, s->part);
+                           s->cols, s->vals64, s->diag, s->x, s->rhs, s->rowp, s->r, s->part);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_RESID);
     if (read_scal(s, st) != hipSuccess) return -1.0;
     *bb_out = s->scal_h[S_BB];
@@ -1042,6 +1187,7 @@ template <typename R> static MpnnSys mpnn_sys(const PpeSolver<R>* s, const Dev<R
     y.nslices = s->nslices;
     y.rows_active = s->rows_real;
     y.slice_off = s->slice_off;
+    y.pos = s->pos;
     y.slice_w = s->slice_w;
     y.cols = s->cols;
     y.cnt = s->cnt;
@@ -1088,7 +1234,10 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
                        d.spos, s->active);
     fill_int(s->nnz_dev, 0, 2, st);
     hipLaunchKernelGGL(k_ppe_count<R>, dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, d.spos, d.cell_start,
-                       d.cell_end, s->cnt, s->diag, s->dir, s->slice_w, s->nnz_dev);
+                       d.cell_end, s->cnt, s->diag, s->dir, s->nnz_dev);
+    if (N > 0)
+        hipLaunchKernelGGL(k_sell_order, dim3(grid_for(N, SELL_SIGMA)), dim3(SELL_BLOCK), 0, st, N, nsl, s->cnt, s->dir,
+                           s->pos, s->rowp, s->slice_w);
     exclusive_scan(s->slice_w, s->slice_off, nullptr, nsl, nsl, s->total, s->scan, st);
     int total = 0;
     int counts[2] = {0, 0};
@@ -1100,6 +1249,7 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     std::memcpy(&total, &s->scal_h[0], sizeof(int));
     std::memcpy(counts, &s->scal_h[1], 2 * sizeof(int));
     s->nnz_real = counts[0];
+    s->nnz_stored = total;
     s->rows_real = counts[1];
     if ((long long)total > s->nnz_cap) {
         if (s->cols) (void)hipFree(s->cols);
@@ -1114,11 +1264,11 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     if (s->mixed)
         hipLaunchKernelGGL((k_ppe_fill<R, float>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, s->alpha, s->n0,
-                           s->cols, s->vals32, s->rhs);
+                           s->rowp, s->cols, s->vals32, s->rhs);
     else
         hipLaunchKernelGGL((k_ppe_fill<R, double>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, s->alpha, s->n0,
-                           s->cols, s->vals64, s->rhs);
+                           s->rowp, s->cols, s->vals64, s->rhs);
     s->nslices = nsl;
     // ---- solve ----
     const double tol = opt.rel_tol > 0 ? opt.rel_tol : 1e-8;
@@ -1337,6 +1487,7 @@ template <typename R> mpsr_status ppe_check(PpeSolver<R>* s, mpsr_timings* tm) {
     tm->spmv_launches = s->spmv_launches;
     tm->spmv_bytes = s->spmv_bytes;
     tm->ppe_nnz = s->nnz_real;
+    tm->ppe_stored = s->nnz_stored;
     tm->ppe_rows = s->rows_real;
     tm->ppe_iterations = s->iters;
     tm->ppe_refinements = s->passes;
@@ -1433,11 +1584,12 @@ mpsr_status ppe_get_matrix(PpeSolver<R>* s, const Dev<R>& d, int* ptr, int* col,
                            int* nnz) {
     const int N = d.N;
     const int nsl = (N + 63) / 64;
-    std::vector<int> sw(nsl), so(nsl), cn(N), pp(N), dd(N);
+    std::vector<int> sw(nsl), so(nsl), cn(N), pp(N), dd(N), po(N);
     std::vector<double> dg(N);
     if (hipMemcpy(sw.data(), s->slice_w, sizeof(int) * nsl, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
     if (hipMemcpy(so.data(), s->slice_off, sizeof(int) * nsl, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
     if (hipMemcpy(cn.data(), s->cnt, sizeof(int) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    if (hipMemcpy(po.data(), s->pos, sizeof(int) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
     if (hipMemcpy(pp.data(), d.perm, sizeof(int) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
     if (hipMemcpy(dd.data(), s->dir, sizeof(int) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
     if (hipMemcpy(dg.data(), s->diag, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
@@ -1464,7 +1616,7 @@ mpsr_status ppe_get_matrix(PpeSolver<R>* s, const Dev<R>& d, int* ptr, int* col,
         if (diag) diag[sl] = dg[t];
         const int n = dd[t] ? 0 : cn[t];
         for (int k = 0; k < n; ++k, ++e) {
-            const long long idx = sell_at(so[t >> 6], t & 63, k);
+            const long long idx = sell_at(so[po[t] >> 6], po[t] & 63, k);
             if (e < cap) {
                 col[e] = pp[cc[idx]];
                 val[e] = vv[idx];
