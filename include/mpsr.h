@@ -283,6 +283,7 @@ typedef struct mpsr_timings {
     int ppe_learned_fallback;/* 1 when it lost to the plain solver after 5 iterations (plain restart) */
     double sphere_ms;        /* sphere phase (profiling on) */
     long long ppe_stored;    /* stored entries of the last PPE matrix (SELL padding included) */
+    double thermal_ms;       /* CalcTemperatureFactor (profiling on) */
 } mpsr_timings;
 
 typedef struct mpsr_handle mpsr_handle;
@@ -331,6 +332,7 @@ mpsr_status mpsr_calc_pressure_explicit(mpsr_handle *h);      /* CalcPressureExp
 mpsr_status mpsr_calc_pressure_poisson(mpsr_handle *h);       /* CalcPoisson :50 (declared only there) */
 mpsr_status mpsr_calc_implicit_explicitly(mpsr_handle *h);    /* CalcImplicitExplicitly :49 */
 mpsr_status mpsr_update_triangle_position(mpsr_handle *h);    /* UpdateTrianglePosition :58 */
+mpsr_status mpsr_calc_temperature_factor(mpsr_handle *h);     /* CalcTemperatureFactor :60 */
 mpsr_status mpsr_rotate_triangle_position(mpsr_handle *h, int analysis_step); /* RotateTrianglePosition :65 */
 
 /* ---- state in and out ---------------------------------------------------------------------- */

@@ -557,6 +557,66 @@ template <typename R> struct Sim {
         }
     }
 
+    // Kern :485-632 CalcTemperatureFactor: heat from the neighbours within
+    // re_T = TemperatureInfluenceRadiusCoefficient * l0, (T_j - T_i) w / (r^2 (1/k_f + R_f/r)) in
+    // canonical order (Kern :540-550), and from the nearest non-drag wall within re_T,
+    // (T_model - T_i) Z_grad(r) / (r^2 (1/k + R_w/r)) with k = 2 k_f k_w / (k_f + k_w)
+    // (Kern :569-594); the drag wall's increment (Kern :597-603); each term counted only above
+    // SCALAR_EPSILON = FLT_EPSILON (Kern :607-615, DataType.h:19); T += EnthalpyCoefficient
+    // (Hf + Hw) with EnthalpyCoefficient = Dt 2 d / (n0 rho Cp) (Kern :497-498); the wall
+    // temperature limits (Kern :620-627); out: A T + B, T, the enthalpy term (Kern :628-631)
+    void thermal() {
+        const R re = (R)P.TemperatureInfluenceRadiusCoefficient * (R)P.InitialParticleDistance;
+        const R mind = (R)((double)re * 1e-3);
+        const R enth = (R)P.Dt * (R)2 * (R)P.Dimension /
+                       ((R)P.ConstantParticleDensity * (R)P.Density * (R)P.FruidSpecificHeat);
+        const R kf = (R)P.FruidThermalConductivity, inv_kf = (R)1 / (R)P.FruidThermalConductivity;
+        const R rf = (R)P.FruidThermalResistance;
+        const int T = (int)tris.size();
+#pragma omp parallel for schedule(static)
+        for (int s = 0; s < n; ++s) {
+            if (out.id[s] < 0) continue;
+            const V3<R> x = in.pos[s];
+            if (!include(g, x, tol())) continue;
+            int ci, cj, ck;
+            if (!cell_coords(g, x, ci, cj, ck)) continue;
+            const long long cid = cell_id(g, ci, cj, ck);
+            if (cid < 0 || cid >= g.cells()) continue;
+            const R t0 = in.temp[s];
+            R hf = 0, hw = 0;
+            for_candidates(s, x, [&](int q) {
+                R d = dist(x, in.pos[q]);
+                if (d < re) {
+                    if (d < mind) d = mind;
+                    const R num = (in.temp[q] - t0) * weight(re, d);
+                    const R den = d * d * (inv_kf + rf / d);
+                    hf = hf + num / den;
+                }
+            });
+            const R dw = stl_mag[s];
+            const int tid = stl_id[s];
+            if (tid >= 0 && tid < T && dw > 0 && dw < re) {
+                const mpsr_tri_params& m = models[tris[tid].model];
+                const R rw = (R)m.WallThermalResistivity, kw = (R)m.WallThermalConductivity;
+                const R k = (R)2 * kf * kw / (kf + kw);
+                const R num = ((R)m.Temperature - t0) * zgrad(dw, l0());
+                const R den = dw * dw * ((R)1 / k + rw / dw);
+                hw = num / den;
+            }
+            R temp = t0;
+            if (!drag.empty()) temp = temp + drag_temp[s];
+            R comb = 0;
+            if (std::fabs(hf) > (R)FLT_EPSILON) comb = comb + hf;
+            if (std::fabs(hw) > (R)FLT_EPSILON) comb = comb + hw;
+            temp = temp + enth * comb;
+            if (temp > (R)P.MaxWallTemperature) temp = (R)P.MaxWallTemperature;
+            if (temp < (R)P.MinWallTemperature) temp = (R)P.MinWallTemperature;
+            out.nu[s] = (R)P.ViscosityUpdateParameterA * temp + (R)P.ViscosityUpdateParameterB;
+            out.temp[s] = temp;
+            out.solid[s] = enth * comb;
+        }
+    }
+
     // Kern :283-441 CalcExplicitly (viscosity, gravity, drag, optional artificial pressure)
     void explicit_step() {
         const R re = re_lap();
@@ -1078,6 +1138,8 @@ template <typename R> struct Sim {
         register_triangles();
         stl_distance();
         if (!drag.empty()) drag_effect();
+        // MultiGPUHandler.cpp:385-390: every ThermalStep steps when the temperature is updated
+        if (P.bUpdateTemperature && P.ThermalStep > 0 && i % P.ThermalStep == 0) thermal();
         spheres_phase();
         explicit_step();
         relocate();
@@ -1293,6 +1355,7 @@ float orc_sqrt14(float x) { return sqrt14(x); }
 float orc_weight(float re, float r) { return weight<float>(re, r); }
 double orc_weight_f64(double re, double r) { return weight<double>(re, r); }
 float orc_zgrad(float r, float l0) { return zgrad<float>(r, l0); }
+double orc_zgrad64(double r, double l0) { return zgrad<double>(r, l0); }
 float orc_zlap(float r, float l0) { return zlap<float>(r, l0); }
 float orc_lambda(float l0) { return lambda_fit(l0); }
 float orc_dist(float ax, float ay, float az, float bx, float by, float bz) { return dist<float>(mk(ax, ay, az), mk(bx, by, bz)); }
@@ -1346,6 +1409,7 @@ void orc_pressure_eos(void* p) { Handle* h = (Handle*)p; DISPATCH(h, pnd_pressur
 void orc_pressure_ppe(void* p) { Handle* h = (Handle*)p; DISPATCH(h, pressure_ppe()); }
 void orc_gradient(void* p) { Handle* h = (Handle*)p; DISPATCH(h, gradient()); }
 void orc_translate(void* p) { Handle* h = (Handle*)p; DISPATCH(h, translate()); }
+void orc_thermal(void* p) { Handle* h = (Handle*)p; DISPATCH(h, thermal()); }
 void orc_rotate(void* p, int i) { Handle* h = (Handle*)p; DISPATCH(h, rotate(i)); }
 void orc_set_spheres(void* p, int n, const double* pos, const double* vel, double d, double rho, double nu) {
     Handle* h = (Handle*)p;

@@ -37,6 +37,7 @@ def load():
         "orc_weight": (C.c_float, [C.c_float, C.c_float]),
         "orc_weight_f64": (C.c_double, [C.c_double, C.c_double]),
         "orc_zgrad": (C.c_float, [C.c_float, C.c_float]),
+        "orc_zgrad64": (C.c_double, [C.c_double, C.c_double]),
         "orc_zlap": (C.c_float, [C.c_float, C.c_float]),
         "orc_lambda": (C.c_float, [C.c_float]),
         "orc_dist": (C.c_float, [C.c_float] * 6),
@@ -61,6 +62,7 @@ def load():
         "orc_pressure_ppe": (None, [vp]),
         "orc_gradient": (None, [vp]),
         "orc_translate": (None, [vp]),
+        "orc_thermal": (None, [vp]),
         "orc_rotate": (None, [vp, C.c_int]),
         "orc_set_ppe_options": (None, [vp, P(abi.PpeOptions)]),
         "orc_get_particles": (None, [vp, C.c_int, P(abi.ParticlesC)]),
@@ -203,6 +205,9 @@ class Oracle:
 
     def UpdateTrianglePosition(self):
         self.lib.orc_translate(self.h)
+
+    def CalcTemperatureFactor(self):
+        self.lib.orc_thermal(self.h)
 
     def RotateTrianglePosition(self, step):
         self.lib.orc_rotate(self.h, step)

@@ -57,6 +57,7 @@ def load_library(path=None):
         "mpsr_calc_pressure_poisson": (C.c_int, [vp]),
         "mpsr_calc_implicit_explicitly": (C.c_int, [vp]),
         "mpsr_update_triangle_position": (C.c_int, [vp]),
+        "mpsr_calc_temperature_factor": (C.c_int, [vp]),
         "mpsr_rotate_triangle_position": (C.c_int, [vp, C.c_int]),
         "mpsr_get_particles": (C.c_int, [vp, C.c_int, P(abi.ParticlesC)]),
         "mpsr_set_particles": (C.c_int, [vp, P(abi.ParticlesC)]),
@@ -297,6 +298,9 @@ class Solver:
 
     def UpdateTrianglePosition(self):
         self._check(self.lib.mpsr_update_triangle_position(self.h), "UpdateTrianglePosition")
+
+    def CalcTemperatureFactor(self):
+        self._check(self.lib.mpsr_calc_temperature_factor(self.h), "CalcTemperatureFactor")
 
     def RotateTrianglePosition(self, step):
         self._check(self.lib.mpsr_rotate_triangle_position(self.h, step), "RotateTrianglePosition")

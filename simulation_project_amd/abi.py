@@ -143,7 +143,7 @@ class Timings(C.Structure):
                 ("ppe_rows", C.c_int), ("ppe_fallbacks", C.c_int),
                 ("ppe_coarse_rows", C.c_int), ("ppe_coarse_rebuilt", C.c_int), ("ppe_setup_ms", C.c_double),
                 ("ppe_learned", C.c_int), ("ppe_learned_fallback", C.c_int), ("sphere_ms", C.c_double),
-                ("ppe_stored", C.c_longlong)]
+                ("ppe_stored", C.c_longlong), ("thermal_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

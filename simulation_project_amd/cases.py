@@ -574,3 +574,30 @@ def buckets(lib, count=64, seed=DEFAULT_SEED, l0=5e-3, dt=5e-5, n_min=5e3, n_max
         c.meta.update({"bucket": i, "target": target, "rpm": rpm})
         out.append(c)
     return out
+
+
+def with_thermal(case, *, temperature=None, wall_temperature=None, conductivity=0.6, resistance=1e-6,
+                 wall_conductivity=50.0, wall_resistivity=1e-6, specific_heat=1.0, viscosity=(1e-9, 1e-6),
+                 limits=(-1e4, 1e4), step=1, radius=2.1):
+    """Switch the heat conduction (CalcTemperatureFactor) on for a case: the parameter-file fields
+    it reads (conductivity, resistance, specific heat, viscosity A T + B, temperature limits,
+    ThermalStep, radius coefficient), the initial particle temperatures and, for every non-drag
+    wall model, its temperature, conductivity and resistivity.  Returns the case."""
+    p = case.params
+    p.bUpdateTemperature = 1
+    p.ThermalStep = step
+    p.TemperatureInfluenceRadiusCoefficient = radius
+    p.FruidThermalConductivity = conductivity
+    p.FruidThermalResistance = resistance
+    p.FruidSpecificHeat = specific_heat
+    p.ViscosityUpdateParameterA, p.ViscosityUpdateParameterB = viscosity
+    p.MinWallTemperature, p.MaxWallTemperature = limits
+    if temperature is not None:
+        case.particles.temperature[:] = temperature
+    for m in case.models:
+        if not m.isDrag:
+            if wall_temperature is not None:
+                m.Temperature = wall_temperature
+            m.WallThermalConductivity = wall_conductivity
+            m.WallThermalResistivity = wall_resistivity
+    return case

@@ -38,7 +38,7 @@ static mpsr_status fail(mpsr_status s, const std::string& m) {
         if (e_ != hipSuccess) return fail(MPSR_GPUERR, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum Phase { PH_BIN, PH_TRI, PH_STL, PH_DRAG, PH_EXPL, PH_RELOC, PH_PND, PH_PPE, PH_GRAD, PH_WALL, PH_SPH, PH_COUNT };
+enum Phase { PH_BIN, PH_TRI, PH_STL, PH_DRAG, PH_EXPL, PH_RELOC, PH_PND, PH_PPE, PH_GRAD, PH_WALL, PH_SPH, PH_THERM, PH_COUNT };
 
 struct EngineBase {
     virtual ~EngineBase() {}
@@ -124,6 +124,19 @@ template <typename R> static Consts<R> make_consts(const mpsr_params& P, const m
     c.cp = (R)P.FruidSpecificHeat;
     c.thermal_step = (R)P.ThermalStep;
     c.ntri = ntri;
+    // CalcTemperatureFactor_Kernel (Kern :497-505): EnthalpyCoefficient = Dt 2 d / (n0 rho Cp),
+    // ReTemperature = coefficient * l0, MinDist = ReTemperature * 1e-3 (a double product)
+    c.re_temp = (R)P.TemperatureInfluenceRadiusCoefficient * (R)P.InitialParticleDistance;
+    c.mind_temp = (R)((double)c.re_temp * 1e-3);
+    c.enth = (R)P.Dt * (R)2 * (R)P.Dimension /
+             ((R)P.ConstantParticleDensity * (R)P.Density * (R)P.FruidSpecificHeat);
+    c.kf = (R)P.FruidThermalConductivity;
+    c.inv_kf = (R)1 / (R)P.FruidThermalConductivity;
+    c.rf = (R)P.FruidThermalResistance;
+    c.tmax = (R)P.MaxWallTemperature;
+    c.tmin = (R)P.MinWallTemperature;
+    c.visc_a = (R)P.ViscosityUpdateParameterA;
+    c.visc_b = (R)P.ViscosityUpdateParameterB;
     return c;
 }
 
@@ -452,7 +465,8 @@ template <typename R> struct Engine : EngineBase {
     mpsr_status init(const mpsr_create_info* ci) {
         P = ci->params;
         if (P.Dimension != 3) return fail(MPSR_PARAMERR, "only the 3D operators are built (Dimension must be 3)");
-        if (P.bUpdateTemperature) return fail(MPSR_PARAMERR, "the thermal model (CalcTemperatureFactor) is not built");
+        if (P.bUpdateTemperature && P.ThermalStep < 1)
+            return fail(MPSR_PARAMERR, "ThermalStep must be at least 1 when the temperature is updated");
         if (!ci->particles || ci->particles->n < 0) return fail(MPSR_PARAMERR, "particles missing");
         if (ci->num_triangles > 0 && (!ci->triangles || ci->num_models <= 0 || !ci->models))
             return fail(MPSR_PARAMERR, "triangles need their model parameters");
@@ -599,6 +613,13 @@ template <typename R> struct Engine : EngineBase {
         launch_drag(d, st);
         mark_end();
     }
+    // CalcTemperatureFactor every ThermalStep steps (MultiGPUHandler.cpp:385-390)
+    void do_thermal(int step) {
+        if (!P.bUpdateTemperature || step % P.ThermalStep != 0) return;
+        mark_begin(PH_THERM);
+        launch_thermal(d, nullptr, nullptr, P.ThermalStep, st);
+        mark_end();
+    }
     void do_spheres() {
         if (sph.S <= 0) return;
         mark_begin(PH_SPH);
@@ -661,6 +682,7 @@ template <typename R> struct Engine : EngineBase {
         do_tri();
         do_stl();
         if (d.ndrag > 0) do_drag();
+        do_thermal(step);
         do_spheres();
         do_explicit();
         do_relocate();
@@ -710,6 +732,7 @@ template <typename R> struct Engine : EngineBase {
 
     // ---- launch shape and the batched form --------------------------------------------------
     int* step_ctr_buf = nullptr;       // device step counter of graph replays
+    int* step_base_buf = nullptr;      // device index of the first replayed step
     std::vector<char*> snap;           // device copy of the in/out state and the triangles
     int snap_n = 0;
     // what the last recorded step holds: rotation table address and launch shape
@@ -737,10 +760,12 @@ template <typename R> struct Engine : EngineBase {
     mpsr_status batch_prepare(int first, int count, int* moved) override {
         HIPC(hipSetDevice(device));
         if (!step_ctr_buf) ALLOC(step_ctr_buf, 1);
+        if (!step_base_buf) ALLOC(step_base_buf, 1);
         mpsr_status s = upload_rot(first, count);
         if (s != MPSR_NOERR) return s;
         *moved = rec_rot != rot_table || rec_block != d.block || rec_items != d.sort_items;
         fill_int(step_ctr_buf, 0, 1, st);
+        fill_int(step_base_buf, first, 1, st);
         HIPC(hipStreamSynchronize(st));   // rot_host is refilled by the next call
         return MPSR_NOERR;
     }
@@ -758,6 +783,7 @@ template <typename R> struct Engine : EngineBase {
         do_tri();
         do_stl();
         if (d.ndrag > 0) do_drag();
+        if (P.bUpdateTemperature) launch_thermal(d, step_base_buf, step_ctr_buf, P.ThermalStep, st);
         do_explicit();
         do_relocate();
         do_bin();
@@ -824,6 +850,7 @@ template <typename R> struct Engine : EngineBase {
             t.stl_ms = acc[PH_STL] / count;
             t.drag_ms = acc[PH_DRAG] / count;
             t.sphere_ms = acc[PH_SPH] / count;
+            t.thermal_ms = acc[PH_THERM] / count;
             t.explicit_ms = acc[PH_EXPL] / count;
             t.relocate_ms = acc[PH_RELOC] / count;
             t.pnd_ms = acc[PH_PND] / count;
@@ -879,6 +906,7 @@ template <typename R> struct Engine : EngineBase {
                 if (s == MPSR_NOERR) { d.rot = rot_table; launch_rotate(d, st); }
                 break;
             case 11: do_spheres(); break;
+            case 12: launch_thermal(d, nullptr, nullptr, P.ThermalStep, st); break;
             default: s = fail(MPSR_PARAMERR, "unknown phase");
         }
         profiling = saved;
@@ -1360,6 +1388,7 @@ mpsr_status mpsr_calc_pressure_explicit(mpsr_handle* h) { GUARD(h); return h->e-
 mpsr_status mpsr_calc_pressure_poisson(mpsr_handle* h) { GUARD(h); return h->e->phase(7, 0); }
 mpsr_status mpsr_calc_implicit_explicitly(mpsr_handle* h) { GUARD(h); return h->e->phase(8, 0); }
 mpsr_status mpsr_update_triangle_position(mpsr_handle* h) { GUARD(h); return h->e->phase(9, 0); }
+mpsr_status mpsr_calc_temperature_factor(mpsr_handle* h) { GUARD(h); return h->e->phase(12, 0); }
 mpsr_status mpsr_rotate_triangle_position(mpsr_handle* h, int step) { GUARD(h); return h->e->phase(10, step); }
 mpsr_status mpsr_get_particles(mpsr_handle* h, int which, mpsr_particles* o) {
     GUARD(h);

@@ -75,6 +75,8 @@ template <typename R> struct Consts {
     // drag, Kern :106-281
     R density, cp, thermal_step;
     int ntri;
+    // heat conduction, Kern :485-632
+    R re_temp, mind_temp, enth, kf, inv_kf, rf, tmax, tmin, visc_a, visc_b;
 };
 
 // position -> cell coordinates, Utility_Inline.h:1636-1674 (C truncation toward zero)
@@ -308,6 +310,10 @@ template <typename R> void launch_explicit(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st);
 template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st);
+// CalcTemperatureFactor; with step_ctr the launch returns at once unless (*step_base + *step_ctr)
+// is a multiple of thermal_step (graph replays), else the caller decides
+template <typename R> void launch_thermal(const Dev<R>& d, const int* step_base, const int* step_ctr, int thermal_step,
+                                          hipStream_t st);
 template <typename R> void launch_neighbor_count(const Dev<R>& d, R re, int* counts, hipStream_t st);
 template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int* offsets, int* nbr, hipStream_t st);
 // walls.hip

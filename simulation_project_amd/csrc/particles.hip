@@ -15,6 +15,7 @@
 // cell.  Consecutive threads share cells, so a wave walks the same 27 ranges and its neighbour
 // records come from L2 in whole 16 B (float4) or 32 B (double4) loads.  Results go to the
 // slot-order arrays through the sorted->slot permutation.
+#include <cfloat>
 #include "mpsr_internal.h"
 #include "neighbors.h"
 
@@ -365,6 +366,69 @@ __global__ __launch_bounds__(256) void k_nbr_fill(int N, Consts<R> c, R re, cons
 }
 
 // ---------------------------------------------------------------------------------------------
+// CalcTemperatureFactor_Kernel (Kern :485-632), one thread per sorted particle, slot-order state:
+// heat from the neighbours within re_T, (T_j - T_i) w / (r^2 (1/k_f + R_f/r)) summed in the
+// canonical order, and from the nearest non-drag wall within re_T, (T_wall - T_i) Z(r) /
+// (r^2 (1/k + R_w/r)) with k the harmonic mean of the fluid and wall conductivities; the drag
+// wall's temperature increment is added, each heat term counts only above FLT_EPSILON (the
+// reference's SCALAR_EPSILON in both precisions), T += EnthalpyCoefficient (Hf + Hw), limited to
+// [MinWallTemperature, MaxWallTemperature]; the out arrays get T, A T + B and the enthalpy term.
+template <typename R>
+__global__ __launch_bounds__(kMaxBlock) void k_thermal(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                                    PState<R> out, const R4<R>* __restrict__ spos,
+                                                    const int* __restrict__ cs, const int* __restrict__ ce,
+                                                    const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
+                                                    const TriD<R>* __restrict__ tris,
+                                                    const mpsr_tri_params* __restrict__ models,
+                                                    const R* __restrict__ drag_temp, const int* __restrict__ step_base,
+                                                    const int* __restrict__ step_ctr, int thermal_step) {
+    if (step_ctr && ((*step_base + *step_ctr) % thermal_step) != 0) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int s = perm[t];
+    if (out.id[s] < 0) return;
+    const R4<R> P = spos[t];
+    if (!include(c, P.x, P.y, P.z)) return;
+    int ci, cj, ck;
+    if (!cell_coords(c, P.x, P.y, P.z, ci, cj, ck)) return;
+    const long long cid = cell_of(c, ci, cj, ck);
+    if (cid < 0 || cid >= c.ncell) return;
+    const R t0 = in.temp[s];
+    const R re = c.re_temp, mind = c.mind_temp;
+    R hf = 0, hw = 0;
+    for_candidates(c, cs, ce, P.x, P.y, P.z, t, [&](int q) {
+        const R4<R> Q = spos[q];
+        R d = dist3(P.x, P.y, P.z, Q.x, Q.y, Q.z);
+        if (d < re) {
+            if (d < mind) d = mind;
+            const R num = (in.temp[perm[q]] - t0) * weight(re, d);
+            const R den = d * d * (c.inv_kf + c.rf / d);
+            hf = hf + num / den;
+        }
+    });
+    const R dist = stl_mag[t];
+    const int tid = stl_id[t];
+    if (tid >= 0 && tid < c.ntri && dist > 0 && dist < re) {
+        const mpsr_tri_params& m = models[tris[tid].model];
+        const R rw = (R)m.WallThermalResistivity, kw = (R)m.WallThermalConductivity;
+        const R k = (R)2 * c.kf * kw / (c.kf + kw);
+        const R num = ((R)m.Temperature - t0) * ztable_eval(kZgrad, dist, c.l0);
+        const R den = dist * dist * ((R)1 / k + rw / dist);
+        hw = num / den;
+    }
+    R temp = t0;
+    if (c.has_drag) temp = temp + drag_temp[t];
+    R comb = 0;
+    if (fabs(hf) > (R)FLT_EPSILON) comb = comb + hf;
+    if (fabs(hw) > (R)FLT_EPSILON) comb = comb + hw;
+    temp = temp + c.enth * comb;
+    if (temp > c.tmax) temp = c.tmax;
+    if (temp < c.tmin) temp = c.tmin;
+    out.nu[s] = c.visc_a * temp + c.visc_b;
+    out.temp[s] = temp;
+    out.solid[s] = c.enth * comb;
+}
+
 template <typename R> void launch_bin_hash(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL(k_bin_hash<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.in, d.out, d.keys,
@@ -400,6 +464,13 @@ template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st) {
     hipLaunchKernelGGL(k_gradient<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
                        d.spos, d.svel, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id);
 }
+template <typename R>
+void launch_thermal(const Dev<R>& d, const int* step_base, const int* step_ctr, int thermal_step, hipStream_t st) {
+    if (d.N <= 0) return;
+    hipLaunchKernelGGL(k_thermal<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
+                       d.spos, d.cell_start, d.cell_end, d.stl_mag, d.stl_id, d.tris, d.models, d.drag_temp, step_base,
+                       step_ctr, thermal_step > 0 ? thermal_step : 1);
+}
 template <typename R> void launch_neighbor_count(const Dev<R>& d, R re, int* counts, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL(k_nbr_count<R>, dim3(grid_for(d.N, 256)), dim3(256), 0, st, d.N, d.c, re, d.spos,
@@ -418,6 +489,7 @@ template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int
     template void launch_relocate<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_pnd<R>(const Dev<R>&, int, hipStream_t);                             \
     template void launch_gradient<R>(const Dev<R>&, hipStream_t);                             \
+    template void launch_thermal<R>(const Dev<R>&, const int*, const int*, int, hipStream_t);   \
     template void launch_neighbor_count<R>(const Dev<R>&, R, int*, hipStream_t);              \
     template void launch_neighbor_fill<R>(const Dev<R>&, R, const int*, int*, hipStream_t);
 MPSR_INSTANTIATE(float)

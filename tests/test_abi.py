@@ -58,7 +58,7 @@ def test_lambda_sweep_matches_oracle(lib, orc):
         assert lib.mpsr_calculate_lambda(C.c_float(l0)) == orc.orc_lambda(C.c_float(l0)), l0
 
 
-def test_rejects_2d_and_thermal_before_touching_the_device(lib):
+def test_rejects_2d_and_thermal_without_step_before_touching_the_device(lib):
     case = cases.small_box(lib)
     case.params.Dimension = 2
     with pytest.raises(MpsrError) as e:
@@ -66,6 +66,7 @@ def test_rejects_2d_and_thermal_before_touching_the_device(lib):
     assert e.value.status == abi.MPSR_PARAMERR
     case.params.Dimension = 3
     case.params.bUpdateTemperature = 1
+    case.params.ThermalStep = 0     # the reference divides by it (MultiGPUHandler.cpp:385)
     with pytest.raises(MpsrError) as e:
         Solver(case)
     assert e.value.status == abi.MPSR_PARAMERR

@@ -143,3 +143,18 @@ def test_batch_rejects_ppe_buckets(lib):
     assert st == abi.MPSR_PARAMERR
     assert b"bucket 1" in lib.mpsr_last_error() and b"EOS" in lib.mpsr_last_error()
     assert not h.value
+
+
+def test_graph_steps_with_heat_conduction_every_second_step(lib):
+    """Buckets with heat conduction on every second step: the graph reads the step parity from
+    the device step counter; every bucket equals its single-domain run."""
+    bl = [cases.with_thermal(c, temperature=300.0, wall_temperature=330.0, step=2)
+          for c in cases.buckets(lib, count=3, n_min=2e3, n_max=4e3, seed=11)]
+    b = Batch(bl, precisions(len(bl)))
+    b.step_many(0, 3)
+    b.step_many(3, 4)
+    ref = singles(bl, 0, 7)
+    for i, (g, s) in enumerate(zip(b.buckets, ref)):
+        assert_same(g, s, f"bucket {i}")
+    assert not np.all(b.buckets[0].get_particles(0).temperature == 300.0)
+    b.close()

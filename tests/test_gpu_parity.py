@@ -374,3 +374,34 @@ def test_spheres_identical(lib, precision):
     for a, b, name in zip(g.get_spheres(), o.get_spheres(), ("position", "velocity", "force")):
         np.testing.assert_array_equal(a, b, err_msg=name)
     assert np.abs(o.get_spheres()[2]).sum() > 0
+
+
+@pytest.mark.parametrize("precision", [abi.MPSR_FP32, abi.MPSR_FP64])
+def test_thermal_phase_identical(lib, precision):
+    """CalcTemperatureFactor after the binning, triangle binning and STL distance of a box with
+    random temperatures and hot conducting walls: temperature, viscosity and solid rate identical
+    to the oracle."""
+    case = cases.small_box(lib, nx=12, ny=10, nz=12)
+    rng = np.random.default_rng(3)
+    case = cases.with_thermal(case, temperature=280.0 + 40.0 * rng.random(case.n), wall_temperature=400.0)
+    g, o = pair(case, precision=precision)
+    for name in ("RegisterParticleTopology", "RegisterTriangleTopology", "CalcSTLDistance", "CalcTemperatureFactor"):
+        run_phase(name, g, o)
+    same_state(g, o, label="thermal phase")
+    assert np.abs(o.get_particles(1).temperature - o.get_particles(0).temperature).max() > 0
+
+
+@pytest.mark.parametrize("precision", [abi.MPSR_FP32, abi.MPSR_FP64])
+def test_thermal_steps_with_drag_heat_identical(lib, precision):
+    """30 full steps with heat conduction every second step, hot walls, a drag plate that also
+    heats (drag mode E with both tokens B) and a temperature-dependent viscosity: identical."""
+    case = cases.drag_box(lib, "B")
+    rng = np.random.default_rng(4)
+    case = cases.with_thermal(case, temperature=290.0 + 20.0 * rng.random(case.n), wall_temperature=350.0,
+                              viscosity=(1e-9, 1e-6), step=2)
+    g, o = pair(case, precision=precision)
+    g.step_many(0, 30)
+    o.step_many(0, 30)
+    same_state(g, o, label="thermal steps")
+    t = o.get_particles(0).temperature
+    assert np.ptp(t) > 0 and not np.allclose(t, case.particles.temperature)
