@@ -316,14 +316,15 @@ __global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __r
                                                const int* __restrict__ slice_w, const int* __restrict__ cols,
                                                const VA* __restrict__ vals, const double* __restrict__ diag,
                                                const double* __restrict__ x, const double* __restrict__ b,
-                                               const int* __restrict__ rowp, double* r, double* part) {
+                                               const int* __restrict__ rowp, const int* __restrict__ cnt, double* r,
+                                               double* part) {
     __shared__ double lds[PB / 64];
     const int P = blockIdx.x * blockDim.x + threadIdx.x;   // SELL position
     const int slice = P >> 6, lane = lane_id();
     const int t = slice < nslices ? rowp[P] : -1;
     double rr = 0.0, bb = 0.0;
     if (t >= 0) {
-        const int base = slice_off[slice], width = slice_w[slice] >> 6;
+        const int base = slice_off[slice], width = min(slice_w[slice] >> 6, (cnt[t] + 3) & ~3);
         double a = diag[t] * x[t];
         // entries in ascending order, 8 column/value loads and then 8 gathers in flight
         int k = 0;
@@ -398,7 +399,7 @@ __global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restric
 // are read once per SpMV and would only evict p from L2.
 template <typename VA, typename V>
 __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restrict__ rowp,
-                                                  const int* __restrict__ slice_off,
+                                                  const int* __restrict__ cnt, const int* __restrict__ slice_off,
                                                   const int* __restrict__ slice_w, const int* __restrict__ cols,
                                                   const VA* __restrict__ vals, const double* __restrict__ diag,
                                                   const V* __restrict__ p, V* q, const double* __restrict__ scal,
@@ -412,8 +413,45 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
     if (t >= 0) {
         const V pt = p[t];
         V a = (V)diag[t] * pt;
-        const int base = slice_off[slice], width = slice_w[slice] >> 6;
+        // the row's own entries (rounded up to a group of 4): a lane stops fetching at its row's end,
+        // so the slice padding beyond it is not read
+        const int base = slice_off[slice], width = min(slice_w[slice] >> 6, (cnt[t] + 3) & ~3);
         int k = 0;
+        // FP32 matrix and vectors: 32 and then 16 entries per round (16 or 8 vector loads, then 32
+        // or 16 gathers of p in flight); with FP64 the registers of such rounds cut the occupancy,
+        // so FP64 keeps rounds of 8
+        if constexpr (sizeof(VA) == 4 && sizeof(V) == 4) {
+            for (; k + 32 <= width; k += 32) {
+                int c[32];
+                VA v[32];
+                const int e = sell_at(base, lane, k);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    quad_load(cols + e + 256 * u, c + 4 * u);
+                    quad_load(vals + e + 256 * u, v + 4 * u);
+                }
+                V g[32];
+#pragma unroll
+                for (int u = 0; u < 32; ++u) g[u] = p[c[u]];
+#pragma unroll
+                for (int u = 0; u < 32; ++u) a += (V)v[u] * g[u];
+            }
+            for (; k + 16 <= width; k += 16) {
+                int c[16];
+                VA v[16];
+                const int e = sell_at(base, lane, k);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    quad_load(cols + e + 256 * u, c + 4 * u);
+                    quad_load(vals + e + 256 * u, v + 4 * u);
+                }
+                V g[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) g[u] = p[c[u]];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) a += (V)v[u] * g[u];
+            }
+        }
         for (; k + 8 <= width; k += 8) {
             int c[8];
             VA v[8];
@@ -1037,7 +1075,7 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
         const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
-            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->slice_off, s->slice_w, s->cols,
+            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt, s->slice_off, s->slice_w, s->cols,
                                vals, s->diag, p, q, s->scal, s->part);
             if (timed) {
                 (void)hipEventRecord(s->ev[s->ev_used + 1], st);
@@ -1133,7 +1171,7 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
         const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
-            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->slice_off, s->slice_w, s->cols,
+            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt, s->slice_off, s->slice_w, s->cols,
                                vals, s->diag, pa, q, s->scal, s->part);
             if (timed) {
                 (void)hipEventRecord(s->ev[s->ev_used + 1], st);
@@ -1170,10 +1208,10 @@ template <typename R> static double resid_pass(PpeSolver<R>* s, int N, hipStream
     const int nb = (rows + PB - 1) / PB;
     if (s->mixed)
         hipLaunchKernelGGL(k_resid<float>, dim3(nb), dim3(PB), 0, st, N, s->nslices, s->slice_off, s->slice_w, s->cols,
-                           s->vals32, s->diag, s->x, s->rhs, s->rowp, s->r, s->part);
+                           s->vals32, s->diag, s->x, s->rhs, s->rowp, s->cnt, s->r, s->part);
     else
         hipLaunchKernelGGL(k_resid<double>, dim3(nb), dim3(PB), 0, st, N, s->nslices, s->slice_off, s->slice_w,
-                           s->cols, s->vals64, s->diag, s->x, s->rhs, s->rowp, s->r, s->part);
+                           s->cols, s->vals64, s->diag, s->x, s->rhs, s->rowp, s->cnt, s->r, s->part);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_RESID);
     if (read_scal(s, st) != hipSuccess) return -1.0;
     *bb_out = s->scal_h[S_BB];
