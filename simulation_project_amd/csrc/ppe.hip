@@ -806,14 +806,19 @@ __global__ __launch_bounds__(PB) void k_agg_ranges(int nc, AggGeom g, const int*
 
 // rc[I] = sum of r over the non-Dirichlet rows of aggregate I: one wave per aggregate, 64 / a^2
 // lanes per cell stack, each lane with 4 rows in flight
+// With upd the residual is the updated one, r - alpha q (the PCG update that k_zcomp then
+// writes; the same expression, so both kernels see the same values).
 template <typename V>
 __global__ __launch_bounds__(64) void k_restrict(int nc, int stacks, const int2* __restrict__ rng,
-                                                  const int* __restrict__ dir, const V* __restrict__ r, double* rc,
+                                                  const int* __restrict__ dir, const V* __restrict__ r,
+                                                  const V* __restrict__ q, int upd, double* rc,
                                                   const double* __restrict__ scal, int check_done) {
     if (check_done && scal[S_DONE] != 0.0) return;
     const int I = blockIdx.x;
     if (I >= nc) return;
     const int lane = lane_id();
+    const V alpha = upd ? (V)scal[S_ALPHA] : (V)0;
+    auto rv = [&](int t) -> V { return upd ? (V)(r[t] - alpha * q[t]) : r[t]; };
     const int lps = 64 / stacks;
     const int st = lane / lps, sub = lane % lps;
     double acc = 0.0;
@@ -823,14 +828,14 @@ __global__ __launch_bounds__(64) void k_restrict(int nc, int stacks, const int2*
         for (; t + 3 * lps < b.y; t += 4 * lps) {
             const int t1 = t + lps, t2 = t + 2 * lps, t3 = t + 3 * lps;
             const int d0 = dir[t], d1 = dir[t1], d2 = dir[t2], d3 = dir[t3];
-            const V r0 = r[t], r1 = r[t1], r2 = r[t2], r3 = r[t3];
+            const V r0 = rv(t), r1 = rv(t1), r2 = rv(t2), r3 = rv(t3);
             if (!d0) acc += (double)r0;
             if (!d1) acc += (double)r1;
             if (!d2) acc += (double)r2;
             if (!d3) acc += (double)r3;
         }
         for (; t < b.y; t += lps)
-            if (!dir[t]) acc += (double)r[t];
+            if (!dir[t]) acc += (double)rv(t);
     }
     acc = wave_sum(acc);
     if (lane == 0) rc[I] = acc;
@@ -851,17 +856,24 @@ __global__ __launch_bounds__(64) void k_coarse_mv(int nc, const M* __restrict__ 
 }
 
 // z = r / diag + cc[agg]; optionally p = z; partial r.z and r.r
+// With upd first the PCG update x += alpha p, r -= alpha q (written back).
 template <typename V>
-__global__ __launch_bounds__(PB) void k_zcomp(int N, const V* __restrict__ r, const double* __restrict__ diag,
+__global__ __launch_bounds__(PB) void k_zcomp(int N, V* r, const double* __restrict__ diag,
                                                const int* __restrict__ agg_of_row, const double* __restrict__ cc,
-                                               V* z, V* p, int set_p, double* part, const double* __restrict__ scal,
-                                               int check_done) {
+                                               V* z, V* p, int set_p, V* x, const V* __restrict__ q, int upd,
+                                               double* part, const double* __restrict__ scal, int check_done) {
     if (check_done && scal[S_DONE] != 0.0) return;
     __shared__ double lds[PB / 64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0, rr = 0.0;
     if (t < N) {
-        const V rv = r[t];
+        V rv = r[t];
+        if (upd) {
+            const V alpha = (V)scal[S_ALPHA];
+            x[t] = x[t] + alpha * p[t];
+            rv = rv - alpha * q[t];
+            r[t] = rv;
+        }
         const int I = agg_of_row[t];
         const V zv = (V)((double)(rv / (V)diag[t]) + (I >= 0 ? cc[I] : 0.0));
         z[t] = zv;
@@ -884,18 +896,6 @@ __global__ __launch_bounds__(PB) void k_pcg_start2(int N, const double* __restri
     if (t >= N) return;
     r[t] = (V)r0[t];
     if (zero_e) e[t] = (V)0;
-}
-
-// x += alpha p; r -= alpha q (the preconditioner and the dots follow in k_zcomp)
-template <typename V>
-__global__ __launch_bounds__(PB) void k_pcg_update2(int N, V* x, V* r, const V* __restrict__ p, const V* __restrict__ q,
-                                                     const double* __restrict__ scal) {
-    if (scal[S_DONE] != 0.0) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N) return;
-    const V alpha = (V)scal[S_ALPHA];
-    x[t] = x[t] + alpha * p[t];
-    r[t] = r[t] - alpha * q[t];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1060,11 +1060,12 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
     hipLaunchKernelGGL(k_agg_ranges, dim3(grid_for((long long)nc * stacks, PB)), dim3(PB), 0, st, nc, g, s->agg_full,
                        d.cell_start, d.cell_end, s->agg_rng);
     hipLaunchKernelGGL(k_pcg_start2<V>, dim3(nb), dim3(PB), 0, st, N, r0, r, x, zero_x);
-    hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, stacks, s->agg_rng, s->dir, r, s->rc, s->scal, 0);
+    hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, stacks, s->agg_rng, s->dir, r, q, 0, s->rc,
+                       s->scal, 0);
     if (f32) hipLaunchKernelGGL(k_coarse_mv<float>, dim3(nc), dim3(64), 0, st, nc, s->Af, s->rc, s->cc, s->scal, 0);
     else hipLaunchKernelGGL(k_coarse_mv<double>, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 0);
-    hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, s->agg_of_row, s->cc, z, p, 1, s->part,
-                       s->scal, 0);
+    hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, s->agg_of_row, s->cc, z, p, 1, x, q, 0,
+                       s->part, s->scal, 0);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_START);
     int done_iters = 0;
     const int probe = s->probe_at;
@@ -1082,13 +1083,12 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
                 s->ev_used += 2;
             }
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 1, s->scal, (int)RED_ALPHA);
-            hipLaunchKernelGGL(k_pcg_update2<V>, dim3(nb), dim3(PB), 0, st, N, x, r, p, q, s->scal);
-            hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, stacks, s->agg_rng, s->dir, r, s->rc,
+            hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, stacks, s->agg_rng, s->dir, r, q, 1, s->rc,
                                s->scal, 1);
             if (f32) hipLaunchKernelGGL(k_coarse_mv<float>, dim3(nc), dim3(64), 0, st, nc, s->Af, s->rc, s->cc, s->scal, 1);
             else hipLaunchKernelGGL(k_coarse_mv<double>, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 1);
-            hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, s->agg_of_row, s->cc, z, p, 0,
-                               s->part, s->scal, 1);
+            hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, s->agg_of_row, s->cc, z, p, 0, x,
+                               q, 1, s->part, s->scal, 1);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_BETA);
             hipLaunchKernelGGL(k_pcg_direction<V>, dim3(nb), dim3(PB), 0, st, N, z, p, s->scal);
         }
