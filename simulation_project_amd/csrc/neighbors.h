@@ -29,6 +29,43 @@ __device__ inline void for_candidates(const Consts<R>& c, const int* __restrict_
             }
 }
 
+// for_candidates restricted to the stencil cells whose box lies within `reach` of (x, y, z), with
+// a 1 % margin over the build's distance (sqrt14 errs by at most 0.18 %, the cell boxes by a few
+// ulp): a skipped cell holds no particle closer than `reach`, and the cells that remain are
+// visited in the same order, so every sum over the neighbours within `reach` is unchanged
+template <typename R, typename F>
+__device__ inline void for_candidates_within(const Consts<R>& c, const int* __restrict__ cs,
+                                             const int* __restrict__ ce, R x, R y, R z, int self, R reach,
+                                             F&& f) {
+    int ci, cj, ck;
+    if (!cell_coords(c, x, y, z, ci, cj, ck)) return;
+    const long long cid = cell_of(c, ci, cj, ck);
+    if (cid < 0 || cid >= c.ncell) return;
+    const R m = reach * (R)1.01;
+    const R m2 = m * m;
+    // squared distance to the cell slabs at offsets -1 and +1 per axis (0 for the own slab)
+    const R lx = c.bminx + (R)ci * c.cell, ly = c.bminy + (R)cj * c.cell, lz = c.bminz + (R)ck * c.cell;
+    const R xm = x - lx, xp = lx + c.cell - x, ym = y - ly, yp = ly + c.cell - y, zm = z - lz, zp = lz + c.cell - z;
+    const R xm2 = xm * xm, xp2 = xp * xp, ym2 = ym * ym, yp2 = yp * yp, zm2 = zm * zm, zp2 = zp * zp;
+    for (int k = -1; k <= 1; ++k) {
+        const R dz2 = k < 0 ? zm2 : (k > 0 ? zp2 : (R)0);
+        for (int j = -1; j <= 1; ++j) {
+            const R dyz2 = dz2 + (j < 0 ? ym2 : (j > 0 ? yp2 : (R)0));
+            if (dyz2 > m2) continue;
+            for (int i = -1; i <= 1; ++i) {
+                if (dyz2 + (i < 0 ? xm2 : (i > 0 ? xp2 : (R)0)) > m2) continue;
+                const long long mm = cell_of(c, ci + i, cj + j, ck + k);
+                if (mm < 0 || mm >= c.ncell) continue;
+                const int a = cs[mm];
+                if (a < 0) continue;
+                const int b = ce[mm];
+                for (int t = a; t < b; ++t)
+                    if (t != self) f(t);
+            }
+        }
+    }
+}
+
 template <typename R>
 __device__ inline R dist3(R x, R y, R z, R qx, R qy, R qz) {
     return build_sqrt((x - qx) * (x - qx) + (y - qy) * (y - qy) + (z - qz) * (z - qz));

@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, 
     }
     const R re = c.re_grad, mind = c.mind_grad;
     R dens = 0;
-    for_candidates(c, cs, ce, x, y, z, t, [&](int q) {
+    for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
         const R4<R> Q = spos[q];
         R d = dist3(x, y, z, Q.x, Q.y, Q.z);
         if (d < re) {
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_gradient(int N, Consts<R> c, cons
     R gx = 0, gy = 0, gz = 0, wx = 0, wy = 0, wz = 0;
     const R pi = in.p[s];
     const R re = c.re_grad, mind = c.mind_grad;
-    for_candidates(c, cs, ce, x, y, z, t, [&](int q) {
+    for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
         const R4<R> Q = spos[q];
         R d = dist3(x, y, z, Q.x, Q.y, Q.z);
         if (d < re) {
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_nbr_count(int N, Consts<R> c, R re, con
     if (t >= N) return;
     const R4<R> P = spos[t];
     int n = 0;
-    for_candidates(c, cs, ce, P.x, P.y, P.z, t, [&](int q) {
+    for_candidates_within(c, cs, ce, P.x, P.y, P.z, t, re, [&](int q) {
         const R4<R> Q = spos[q];
         if (dist3(P.x, P.y, P.z, Q.x, Q.y, Q.z) < re) ++n;
     });
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256) void k_nbr_fill(int N, Consts<R> c, R re, cons
     if (t >= N) return;
     const R4<R> P = spos[t];
     int e = off[t];
-    for_candidates(c, cs, ce, P.x, P.y, P.z, t, [&](int q) {
+    for_candidates_within(c, cs, ce, P.x, P.y, P.z, t, re, [&](int q) {
         const R4<R> Q = spos[q];
         if (dist3(P.x, P.y, P.z, Q.x, Q.y, Q.z) < re) nbr[e++] = perm[q];
     });
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_thermal(int N, Consts<R> c, const
     const R t0 = in.temp[s];
     const R re = c.re_temp, mind = c.mind_temp;
     R hf = 0, hw = 0;
-    for_candidates(c, cs, ce, P.x, P.y, P.z, t, [&](int q) {
+    for_candidates_within(c, cs, ce, P.x, P.y, P.z, t, re, [&](int q) {
         const R4<R> Q = spos[q];
         R d = dist3(P.x, P.y, P.z, Q.x, Q.y, Q.z);
         if (d < re) {
