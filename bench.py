@@ -3,6 +3,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--l0 2.012e-3] [--solver learned|mixed|fp64]
                     [--precond two|jacobi] [--no-cpu-baseline] [--no-plain]
     python bench.py --workload batched [--buckets 64] [--steps K] [--warmup W] [--no-tune]
+    python bench.py --workload precision --l0 1.6e-3 --steps 200
 
 Workload: the rotating drum (cylinder R = 0.15 m, L = 0.3 m, 9984-triangle STL, 30 rpm, 40 % fill)
 at l0 = 2.012e-3 m: 1,008,472 fluid particles and 2,000 rigid spheres (d = 5 l0, 2500 kg/m3,
@@ -76,8 +77,9 @@ def main():
     ap.add_argument("--spheres", type=int, default=2000, help="rigid spheres of diameter 5 l0, density 2500")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 PCG comparison run")
-    ap.add_argument("--workload", choices=["drum", "batched"], default="drum",
-                    help="drum: the 1M-particle drum (default); batched: 64 independent rotating buckets")
+    ap.add_argument("--workload", choices=["drum", "batched", "precision"], default="drum",
+                    help="drum: the 1M-particle drum (default); batched: 64 independent rotating buckets; "
+                         "precision: the mixed-precision drum against the FP64 build after --steps steps")
     ap.add_argument("--buckets", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="batched: keep the default launch shape (256/8)")
     args = ap.parse_args()
@@ -87,6 +89,8 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         return batched(args)
+    if args.workload == "precision":
+        return precision(args)
 
     from simulation_project_amd import Solver, abi, cases, load_library
     lib = load_library()
@@ -260,6 +264,71 @@ def batched(args):
         "graph_nodes": info["graph_nodes"],
         "launch": {"block": blk, "sort_items": items,
                    "autotune_ms": {f"{k[0]}/{k[1]}": round(v, 4) for k, v in sorted(table.items())}},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def precision(args):
+    """The mixed-precision drum (FP32 particle operators and SpMV, FP64 dots, residuals and
+    refinement, two-level preconditioner) against the FP64 build (FP64 operators, FP64 PCG with
+    the same preconditioner, 1e-8) from the same start, both on the GPU, after --steps steps:
+    relative L2 and max errors of pressure, velocity and position over the particles alive in
+    both, and the bulk kinetic energy and mean pressure.  The FP64 build is the stand-in for the
+    FP64 oracle at this size (it equals the oracle bit for bit in every particle operator)."""
+    from simulation_project_amd import Solver, abi, cases, load_library
+    lib = load_library()
+    t0 = time.time()
+    case = cases.drum(lib, l0=args.l0, steps=args.steps)
+    n = case.n
+    sph = cases.drum_spheres(case, args.spheres) if args.spheres > 0 else None
+    log(f"[bench] precision: drum l0={args.l0} n={n}, {args.steps} steps, built in {time.time() - t0:.1f}s")
+    res = {}
+    for name, prec, kind in (("mixed", abi.MPSR_FP32, "mixed"), ("fp64", abi.MPSR_FP64, "fp64")):
+        s = Solver(case, precision=prec, pressure_mode=abi.MPSR_PRESSURE_PPE,
+                   ppe=ppe_options(abi, kind, "two", abi.MPSR_WARM_PREVIOUS))
+        if sph is not None:
+            s.set_spheres(sph)
+        s.synchronize()
+        tt = time.perf_counter()
+        for k0 in range(0, args.steps, 50):
+            s.step_many(k0, min(50, args.steps - k0))
+            log(f"[bench] {name}: {min(k0 + 50, args.steps)} steps")
+        s.synchronize()
+        el = time.perf_counter() - tt
+        pa = s.get_particles(0)
+        res[name] = {"ms_per_step": 1e3 * el / args.steps, "p": pa.pressure.copy(), "v": pa.velocity.copy(),
+                     "x": pa.position.copy(), "id": pa.id.copy()}
+        s.close()
+    a, b = res["mixed"], res["fp64"]
+    alive = (a["id"] >= 0) & (b["id"] >= 0)
+    def rel(x, y):
+        return float(np.linalg.norm((x - y)[alive]) / max(np.linalg.norm(y[alive]), 1e-300))
+    dv = np.linalg.norm((a["v"] - b["v"])[alive], axis=1)
+    dx = np.linalg.norm((a["x"] - b["x"])[alive], axis=1)
+    ke_a = 0.5 * float(np.sum(a["v"][alive] ** 2))
+    ke_b = 0.5 * float(np.sum(b["v"][alive] ** 2))
+    out = {
+        "metric": f"mixed-precision drum vs FP64 build after {args.steps} steps",
+        "value": rel(a["p"], b["p"]),
+        "unit": "relative L2 error of pressure",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": round(a["ms_per_step"], 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"rotating drum, {n} particles (l0={args.l0} m), "
+                               f"{0 if sph is None else len(sph.position)} spheres, PPE to 1e-8, "
+                               f"mixed (FP32 operators and SpMV, FP64 dots and refinement) vs FP64",
+                   "particles": n},
+        "errors": {"pressure_rel_l2": rel(a["p"], b["p"]),
+                   "pressure_max_abs_over_max": float(np.abs(a["p"] - b["p"])[alive].max() / np.abs(b["p"][alive]).max()),
+                   "velocity_rel_l2": rel(a["v"], b["v"]),
+                   "velocity_max_abs": float(dv.max()),
+                   "position_rms_over_l0": float(np.sqrt(np.mean(dx ** 2)) / args.l0),
+                   "position_max_over_l0": float(dx.max() / args.l0),
+                   "kinetic_energy_rel": abs(ke_a - ke_b) / ke_b,
+                   "mean_pressure_rel": abs(float(a["p"][alive].mean()) - float(b["p"][alive].mean())) /
+                                        abs(float(b["p"][alive].mean())),
+                   "alive_mixed": int((a["id"] >= 0).sum()), "alive_fp64": int((b["id"] >= 0).sum())},
+        "fp64_ms_per_step": round(b["ms_per_step"], 3),
     }
     print(json.dumps(out), flush=True)
 

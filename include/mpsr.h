@@ -231,6 +231,10 @@ typedef struct mpsr_ppe_options {
                           current aggregates: the preconditioner stays symmetric positive definite. */
     int learned_omega; /* with MPSR_WARM_LEARNED: 1 = also scale the preconditioner's diagonal part
                           by the network's omega (clamped to [0.5, 1.5]), 0 = warm start only */
+    double source_relaxation; /* gamma in the source gamma * alpha * max(n* - n0s, 0) / n0s: the share
+                          of the density error corrected per step; 0 = 0.1.  With 1 (all of it per
+                          step) the source is stiff enough to amplify PND noise into unphysical
+                          motion in the drum at 1M particles and beyond */
 } mpsr_ppe_options;
 
 typedef struct mpsr_create_info {

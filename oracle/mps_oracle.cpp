@@ -853,7 +853,8 @@ template <typename R> struct Sim {
         const R mind = (R)((double)re * 1e-3);
         const double n0 = n0_src;
         const double alpha = lambda_lap * n0_lap * (double)(R)P.Density /
-                             (2.0 * P.Dimension * (double)(R)P.Dt * (double)(R)P.Dt);
+                             (2.0 * P.Dimension * (double)(R)P.Dt * (double)(R)P.Dt) *
+                             (ppe.source_relaxation > 0 ? ppe.source_relaxation : 0.1);
         const double beta_n0 = (double)((R)P.ConstantParticleDensity * (R)P.FreeSurfaceThreshold);
         std::vector<int> active(n, 0);
         const int C = (int)g.cells();

@@ -116,7 +116,7 @@ class PpeOptions(C.Structure):
     _fields_ = [("solver", C.c_int), ("rel_tol", C.c_double), ("max_iter", C.c_int),
                 ("warm_start", C.c_int), ("n0_lap", C.c_double), ("lambda_lap", C.c_double),
                 ("precond", C.c_int), ("coarse_cells", C.c_int), ("coarse_rebuild", C.c_int),
-                ("learned_omega", C.c_int)]
+                ("learned_omega", C.c_int), ("source_relaxation", C.c_double)]
 
 
 class SpheresC(C.Structure):

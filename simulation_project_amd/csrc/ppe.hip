@@ -1299,13 +1299,16 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
         if (!dev_alloc(&s->vals64, (size_t)s->nnz_cap)) return MPSR_MEMERR;
         if (!dev_alloc(&s->vals32, (size_t)s->nnz_cap)) return MPSR_MEMERR;
     }
+    // the source's share of the density error corrected per step, 0.1 unless set (a double product,
+    // evaluated alike by the oracle)
+    const double alpha_src = s->alpha * (opt.source_relaxation > 0 ? opt.source_relaxation : 0.1);
     if (s->mixed)
         hipLaunchKernelGGL((k_ppe_fill<R, float>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
-                           d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, s->alpha, s->n0,
+                           d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
                            s->rowp, s->cols, s->vals32, s->rhs);
     else
         hipLaunchKernelGGL((k_ppe_fill<R, double>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
-                           d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, s->alpha, s->n0,
+                           d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
                            s->rowp, s->cols, s->vals64, s->rhs);
     s->nslices = nsl;
     // ---- solve ----
