@@ -296,8 +296,11 @@ def precision(args):
         s.synchronize()
         el = time.perf_counter() - tt
         pa = s.get_particles(0)
+        tl = s.timings()
         res[name] = {"ms_per_step": 1e3 * el / args.steps, "p": pa.pressure.copy(), "v": pa.velocity.copy(),
-                     "x": pa.position.copy(), "id": pa.id.copy()}
+                     "x": pa.position.copy(), "id": pa.id.copy(),
+                     "last_solve": {"iterations": tl["ppe_iterations"], "coarse_rows": tl["ppe_coarse_rows"],
+                                    "rel_residual": tl["ppe_rel_residual"]}}
         s.close()
     a, b = res["mixed"], res["fp64"]
     alive = (a["id"] >= 0) & (b["id"] >= 0)
@@ -329,6 +332,7 @@ def precision(args):
                                         abs(float(b["p"][alive].mean())),
                    "alive_mixed": int((a["id"] >= 0).sum()), "alive_fp64": int((b["id"] >= 0).sum())},
         "fp64_ms_per_step": round(b["ms_per_step"], 3),
+        "last_solve": {"mixed": a["last_solve"], "fp64": b["last_solve"]},
     }
     print(json.dumps(out), flush=True)
 

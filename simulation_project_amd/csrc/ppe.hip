@@ -967,23 +967,34 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
 
 static constexpr int NC_MAX = 3072;   // largest coarse system factorised densely
 
-// Builds the aggregates of this solve's non-Dirichlet rows, Ac = P^T A P and its inverse.
-// Returns false (and the solve uses the Jacobi preconditioner) when there are no aggregates,
-// too many, or the factorisation fails.
+// Builds the aggregates of this solve's non-Dirichlet rows, Ac = P^T A P and its inverse.  When
+// the aggregates outnumber what is factorised densely, the aggregate edge grows by one cell at a
+// time (up to 8; the flag and map arrays were sized for the configured, smaller edge) and the
+// larger edge is kept for later solves.  Returns false (and the solve uses the Jacobi
+// preconditioner) when there are no aggregates, still too many, or the factorisation fails.
 template <typename R, typename VA>
 static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hipStream_t st) {
     const int N = d.N;
+    int nc = 0;
+    for (;;) {
+        const AggGeom g0{d.c.gx, d.c.gy, d.c.gz, s->agg_edge, s->cgx, s->cgy, s->cgz};
+        fill_int(s->agg_flag, 0, s->nfull, st);
+        hipLaunchKernelGGL(k_agg_flag, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, g0, d.skeys, s->dir, s->agg_flag);
+        exclusive_scan(s->agg_flag, s->agg_map, nullptr, s->nfull, s->nfull, s->nc_dev, s->agg_scan, st);
+        if (hipMemcpyAsync(&s->scal_h[S_COUNT - 1], s->nc_dev, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+            return false;
+        if (hipStreamSynchronize(st) != hipSuccess) return false;
+        std::memcpy(&nc, &s->scal_h[S_COUNT - 1], sizeof(int));
+        if (nc <= NC_MAX || s->agg_edge >= 8) break;
+        s->agg_edge += 1;
+        s->cgx = (d.c.gx + s->agg_edge - 1) / s->agg_edge;
+        s->cgy = (d.c.gy + s->agg_edge - 1) / s->agg_edge;
+        s->cgz = (d.c.gz + s->agg_edge - 1) / s->agg_edge;
+        s->nfull = s->cgx * s->cgy * s->cgz;
+    }
     const AggGeom g{d.c.gx, d.c.gy, d.c.gz, s->agg_edge, s->cgx, s->cgy, s->cgz};
-    fill_int(s->agg_flag, 0, s->nfull, st);
-    hipLaunchKernelGGL(k_agg_flag, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, g, d.skeys, s->dir, s->agg_flag);
-    exclusive_scan(s->agg_flag, s->agg_map, nullptr, s->nfull, s->nfull, s->nc_dev, s->agg_scan, st);
     hipLaunchKernelGGL(k_agg_list, dim3(grid_for(s->nfull, PB)), dim3(PB), 0, st, s->nfull, s->agg_flag, s->agg_map,
                        s->agg_full);
-    int nc = 0;
-    if (hipMemcpyAsync(&s->scal_h[S_COUNT - 1], s->nc_dev, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
-        return false;
-    if (hipStreamSynchronize(st) != hipSuccess) return false;
-    std::memcpy(&nc, &s->scal_h[S_COUNT - 1], sizeof(int));
     s->nc = nc;
     if (nc <= 0 || nc > NC_MAX) return false;
     if (g.cgx >= 1024 || g.cgy >= 1024 || g.cgz >= 1024) return false;   // packed coordinates hold 10 bits
