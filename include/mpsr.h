@@ -288,6 +288,7 @@ typedef struct mpsr_timings {
     double sphere_ms;        /* sphere phase (profiling on) */
     long long ppe_stored;    /* stored entries of the last PPE matrix (SELL padding included) */
     double thermal_ms;       /* CalcTemperatureFactor (profiling on) */
+    double turbulence_ms;    /* CalcTurbulenceViscosity (profiling on) */
 } mpsr_timings;
 
 typedef struct mpsr_handle mpsr_handle;
@@ -418,6 +419,47 @@ mpsr_status mpsr_get_ppe_matrix(mpsr_handle *h, int *row_ptr, int *col, double *
                                 int cap, int *nnz);
 /* Per-phase HIP-event timing of the following steps (mpsr_get_timings); off by default. */
 mpsr_status mpsr_set_profiling(mpsr_handle *h, int on);
+
+/* ---- turbulence and negative pressure (the radix solver variant's terms) ------------------------
+ * Fields of the radix variant's CParameter (radix/SPVSOLVER/MultiGPUSolver/DataType.h:510-520,
+ * read by IO.cpp:393-431), in their order.  This path's CParameter carries bTurbulance,
+ * SmagorinskyConstant and FilterWidth too (mpsr_params), read from the parameter file but used by
+ * none of its kernels; the terms run only when set here.  Zeroed = both terms off. */
+typedef struct mpsr_radix_terms {
+    int bTurbulance;              /* Smagorinsky viscosity added to the particle viscosity */
+    float SmagorinskyConstant;
+    float FilterWidth;
+    float TurbulanViscosityRatio; /* upper clamp of the turbulent viscosity */
+    float Cnue;
+    float Cepsilon;
+    float TurbulanceIntensity;    /* upper clamp of the turbulent kinetic energy */
+    int bNegativePressure;        /* gas-pressure term near rotating or flagged walls */
+} mpsr_radix_terms;
+
+/* Per-model fields of the radix CTriangleParameters (DataType.h:160-168, Configuration.cpp:
+ * 346-360, MultiGPUHandler.cpp:482-527) that this path's mpsr_tri_params lacks. */
+typedef struct mpsr_negative_pressure_model {
+    float RotationInRPM;          /* the term applies when != 0 or NegativePressureCheck */
+    int NegativePressureCheck;
+    float InfluenceRegion;        /* in units of InitialParticleDistance */
+    mpsr_float3 DirectionVector;
+    float PressureGas;
+} mpsr_negative_pressure_model;
+
+/* Sets both terms for the following steps; models may be NULL when bNegativePressure is 0, else
+ * it holds num_models entries (the handle's model count).  With bTurbulance the step runs
+ * mpsr_calc_turbulence_viscosity after the temperature phase and the predictor multiplies its
+ * Laplacian by (nu + nu_t) (radix DeviceProcess_Kernel.h:545-552, MultiGPUHandler.cpp:1421-1427);
+ * with bNegativePressure the gradient corrector adds the gas-pressure term of the nearest wall
+ * (radix DeviceProcess_Kernel.h:1110-1134, 1214-1245). */
+mpsr_status mpsr_set_radix_terms(mpsr_handle *h, const mpsr_radix_terms *terms,
+                                 const mpsr_negative_pressure_model *models, int num_models);
+/* CalcTurbulenceViscosity (radix DeviceProcess.h:80, DeviceProcess_Kernel.h:1273-1476): the
+ * turbulent viscosity and kinetic energy of every particle from the current binning. */
+mpsr_status mpsr_calc_turbulence_viscosity(mpsr_handle *h);
+/* The last mpsr_calc_turbulence_viscosity results in slot order, n rows each (either may be NULL):
+ * c_daParticleTurbulaceViscosity and c_daParticleStrainTensorProduct. */
+mpsr_status mpsr_get_turbulence(mpsr_handle *h, double *viscosity, double *kinetic_energy, int n);
 
 /* Launch shape of the per-particle kernels (block: a multiple of 64 in [64, 512]) and of the
  * particle sort (tile = 256 * sort_items keys, sort_items 4, 8 or 16).  Results do not depend on

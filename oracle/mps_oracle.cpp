@@ -396,6 +396,10 @@ template <typename R> struct Sim {
     int last_iters = 0; double last_relres = 0, last_relres0 = 0;
     double n0_lap = 0, lambda_lap = 0;
     double n0_src = 0;   // PND of the ideal lattice with this build's distance and weight at re_grad
+    // the radix variant's turbulence and negative-pressure terms
+    mpsr_radix_terms rt{};
+    std::vector<mpsr_negative_pressure_model> negm;
+    std::vector<R> turb_nu, turb_k;   // per slot
 
     R tol() const { return (R)P.Tolerance; }
     R l0() const { return (R)P.InitialParticleDistance; }
@@ -617,6 +621,64 @@ template <typename R> struct Sim {
         }
     }
 
+    // radix DeviceProcess_Kernel.h:1273-1476 CalcTurbulenceViscosity_Kernel: velocity gradient
+    // tensor over the gradient radius (:1324-1397), its symmetric part's contraction S (:1405-1429),
+    // wall damping 1 - exp(-r_wall / re) (:1437-1440), nu_t = (Cs f)^2 sqrt(2 S) clamped by
+    // TurbulanViscosityRatio (:1443-1458), k = S 2 Cnue l0^2 / Cepsilon clamped by
+    // TurbulanceIntensity (:1466-1474).  The vorticity and density sums there feed nothing.  A
+    // particle whose cell is outside the grid gets 0 (the reference leaves the array as it was).
+    void turbulence() {
+        const R re = re_grad();
+        const R mind = (R)((double)re * 1e-3);
+        const R gsc = (R)P.Dimension / (R)P.ConstantParticleDensity;
+        const R cs = (R)rt.SmagorinskyConstant, fw = (R)rt.FilterWidth;
+        const R numax = (R)rt.TurbulanViscosityRatio, kmax = (R)rt.TurbulanceIntensity;
+        const R tk = (R)2 * (R)rt.Cnue * l0() * l0() / (R)rt.Cepsilon;
+        if ((int)turb_nu.size() < n) { turb_nu.resize(n, 0); turb_k.resize(n, 0); }
+#pragma omp parallel for schedule(static)
+        for (int s = 0; s < n; ++s) {
+            if (out.id[s] < 0) continue;
+            const V3<R> x = in.pos[s];
+            if (!include(g, x, tol())) { turb_nu[s] = 0; continue; }
+            int ci, cj, ck;
+            long long cid = -1;
+            if (cell_coords(g, x, ci, cj, ck)) cid = cell_id(g, ci, cj, ck);
+            if (cid < 0 || cid >= g.cells()) { turb_nu[s] = 0; turb_k[s] = 0; continue; }
+            const V3<R> v0 = in.vel[s];
+            R txx = 0, txy = 0, txz = 0, tyx = 0, tyy = 0, tyz = 0, tzx = 0, tzy = 0, tzz = 0;
+            for_candidates(s, x, [&](int q) {
+                R d = dist(x, in.pos[q]);
+                if (d < re) {
+                    if (d < mind) d = mind;
+                    const R w = weight(re, d);
+                    const V3<R> v1 = in.vel[q], xq = in.pos[q];
+                    const R ux = v1.x - v0.x, uy = v1.y - v0.y, uz = v1.z - v0.z;
+                    const R rx = xq.x - x.x, ry = xq.y - x.y, rz = xq.z - x.z;
+                    const R dd = d * d;
+                    txx = txx + ux * rx * w / dd; txy = txy + ux * ry * w / dd; txz = txz + ux * rz * w / dd;
+                    tyx = tyx + uy * rx * w / dd; tyy = tyy + uy * ry * w / dd; tyz = tyz + uy * rz * w / dd;
+                    tzx = tzx + uz * rx * w / dd; tzy = tzy + uz * ry * w / dd; tzz = tzz + uz * rz * w / dd;
+                }
+            });
+            txx = gsc * txx; txy = gsc * txy; txz = gsc * txz;
+            tyx = gsc * tyx; tyy = gsc * tyy; tyz = gsc * tyz;
+            tzx = gsc * tzx; tzy = gsc * tzy; tzz = gsc * tzz;
+            const R D = (R)0.5 * (txy + tyx), E = (R)0.5 * (txz + tzx), F = (R)0.5 * (tyz + tzy);
+            const R S = txx * txx + tyy * tyy + tzz * tzz + (R)2 * D * D + (R)2 * E * E + (R)2 * F * F;
+            const R ratio = -(stl_mag[s] / re);
+            const R damp = (R)1 - (R)(float)det_exp((double)(float)ratio);
+            const R cd = cs * damp;
+            const R coef = (cd * fw) * (cd * fw);
+            R nut = 0;
+            if (S > 0) nut = coef * std::sqrt((R)2 * S);
+            if (nut > numax) nut = numax;
+            R k = S * tk;
+            if (k > kmax) k = kmax;
+            turb_nu[s] = nut;
+            turb_k[s] = k;
+        }
+    }
+
     // Kern :283-441 CalcExplicitly (viscosity, gravity, drag, optional artificial pressure)
     void explicit_step() {
         const R re = re_lap();
@@ -679,7 +741,10 @@ template <typename R> struct Sim {
             V3<R> acc = mk<R>(0, 0, 0);
             acc = add(acc, grav);
             if (!drag.empty()) dragv = drag_acc[s];
-            lap = mul(lap, in.nu[s]);
+            // radix DeviceProcess_Kernel.h:545-552: the turbulent viscosity joins the laminar one.
+            // Its k-gradient term (:477-485, :558-562) is scaled by the integer quotient 2/3 = 0
+            // there, so it changes no finite velocity and is not formed.
+            lap = mul(lap, rt.bTurbulance ? in.nu[s] + (s < (int)turb_nu.size() ? turb_nu[s] : (R)0) : in.nu[s]);
             acc = add(acc, lap);
             v0 = sub(add(v0, mul(acc, dt)), dragv);
             if (!sph_pos.empty()) v0 = add(v0, sph_dv[s]);
@@ -781,6 +846,7 @@ template <typename R> struct Sim {
                 }
             }
             V3<R> gr = add(gf, gw);
+            if (rt.bNegativePressure) gr = add(gr, negative_pressure(x, pi, c1));
             V3<R> iv = mk(c3 * gr.x, c3 * gr.y, c3 * gr.z);
             if (std::isfinite(iv.x) && std::isfinite(iv.y) && std::isfinite(iv.z)) {
                 out.pos[s] = mk(x.x + iv.x * dt, x.y + iv.y * dt, x.z + iv.z * dt);
@@ -792,6 +858,49 @@ template <typename R> struct Sim {
                 out.vel[s] = v;
             }
         }
+    }
+
+    // radix DeviceProcess_Kernel.h:1110-1134 and :1214-1241: the wall nearest to x over the
+    // 27-cell stencil's triangle lists, where the first listed triangle of every cell replaces
+    // the running nearest (l == 0) unless it is that triangle already; then, for a model that
+    // rotates or is flagged and lies within InfluenceRegion l0, the gas-pressure gradient
+    // 0.5 (PressureGas + p) Z(l0) / l0^2 along the model's DirectionVector, times 2d/n0.
+    V3<R> negative_pressure(V3<R> x, R pi, R c1) const {
+        V3<R> gn = mk<R>(0, 0, 0);
+        const int T = (int)tris.size();
+        int ci, cj, ck;
+        if (!cell_coords(g, x, ci, cj, ck)) return gn;
+        const long long cid = cell_id(g, ci, cj, ck);
+        if (cid < 0 || cid >= g.cells()) return gn;
+        R best = (R)1e8;
+        int nid = -1;
+        for (int k = -1; k <= 1; ++k)
+            for (int j = -1; j <= 1; ++j)
+                for (int i = -1; i <= 1; ++i) {
+                    const long long m = cell_id(g, ci + i, cj + j, ck + k);
+                    if (m < 0 || m >= g.cells()) continue;
+                    for (int l = 0; l < tcount[m]; ++l) {
+                        const int tid = tids[tstart[m] + l];
+                        if (tid >= 0 && tid < T && tid != nid) {
+                            V3<R> dv;
+                            const R dm = tri_distance(tris[tid], x, &dv);
+                            if (l == 0) { best = dm; nid = tid; }
+                            if (dm < best) { best = dm; nid = tid; }
+                        }
+                    }
+                }
+        if (nid != -1 && best != (R)1e8) {
+            const mpsr_negative_pressure_model& m = negm[tris[nid].model];
+            if ((R)m.RotationInRPM != 0 || m.NegativePressureCheck) {
+                if (best <= (R)m.InfluenceRegion * l0()) {
+                    const R iw = zgrad(l0(), l0());
+                    const R pc = (R)(0.5 * (double)((R)m.PressureGas + pi) * (double)iw / (double)(l0() * l0()));
+                    gn = mk(pc * (R)m.DirectionVector.x, pc * (R)m.DirectionVector.y, pc * (R)m.DirectionVector.z);
+                }
+            }
+        }
+        gn.x *= c1; gn.y *= c1; gn.z *= c1;
+        return gn;
     }
 
     // Kern :463-482 UpdateTrianglePosition
@@ -1141,6 +1250,8 @@ template <typename R> struct Sim {
         if (!drag.empty()) drag_effect();
         // MultiGPUHandler.cpp:385-390: every ThermalStep steps when the temperature is updated
         if (P.bUpdateTemperature && P.ThermalStep > 0 && i % P.ThermalStep == 0) thermal();
+        // radix MultiGPUHandler.cpp:1421-1427: after the temperature, before the predictor
+        if (rt.bTurbulance) turbulence();
         spheres_phase();
         explicit_step();
         relocate();
@@ -1411,6 +1522,25 @@ void orc_pressure_ppe(void* p) { Handle* h = (Handle*)p; DISPATCH(h, pressure_pp
 void orc_gradient(void* p) { Handle* h = (Handle*)p; DISPATCH(h, gradient()); }
 void orc_translate(void* p) { Handle* h = (Handle*)p; DISPATCH(h, translate()); }
 void orc_thermal(void* p) { Handle* h = (Handle*)p; DISPATCH(h, thermal()); }
+void orc_turbulence(void* p) { Handle* h = (Handle*)p; DISPATCH(h, turbulence()); }
+void orc_set_radix_terms(void* p, const mpsr_radix_terms* t, const mpsr_negative_pressure_model* m, int nm) {
+    Handle* h = (Handle*)p;
+    std::vector<mpsr_negative_pressure_model> v(m ? m : nullptr, m ? m + nm : nullptr);
+    if (h->d) { h->d->rt = *t; h->d->negm = v; }
+    if (h->f) { h->f->rt = *t; h->f->negm = v; }
+}
+void orc_get_turbulence(void* p, double* nu, double* k, int n) {
+    Handle* h = (Handle*)p;
+    for (int s = 0; s < n; ++s) {
+        if (h->d) {
+            if (nu) nu[s] = s < (int)h->d->turb_nu.size() ? h->d->turb_nu[s] : 0.0;
+            if (k) k[s] = s < (int)h->d->turb_k.size() ? h->d->turb_k[s] : 0.0;
+        } else {
+            if (nu) nu[s] = s < (int)h->f->turb_nu.size() ? (double)h->f->turb_nu[s] : 0.0;
+            if (k) k[s] = s < (int)h->f->turb_k.size() ? (double)h->f->turb_k[s] : 0.0;
+        }
+    }
+}
 void orc_rotate(void* p, int i) { Handle* h = (Handle*)p; DISPATCH(h, rotate(i)); }
 void orc_set_spheres(void* p, int n, const double* pos, const double* vel, double d, double rho, double nu) {
     Handle* h = (Handle*)p;

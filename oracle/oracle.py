@@ -63,6 +63,9 @@ def load():
         "orc_gradient": (None, [vp]),
         "orc_translate": (None, [vp]),
         "orc_thermal": (None, [vp]),
+        "orc_turbulence": (None, [vp]),
+        "orc_set_radix_terms": (None, [vp, P(abi.RadixTerms), P(abi.NegativePressureModel), C.c_int]),
+        "orc_get_turbulence": (None, [vp, P(C.c_double), P(C.c_double), C.c_int]),
         "orc_rotate": (None, [vp, C.c_int]),
         "orc_set_ppe_options": (None, [vp, P(abi.PpeOptions)]),
         "orc_get_particles": (None, [vp, C.c_int, P(abi.ParticlesC)]),
@@ -208,6 +211,19 @@ class Oracle:
 
     def CalcTemperatureFactor(self):
         self.lib.orc_thermal(self.h)
+
+    def set_radix_terms(self, terms, models=None):
+        t, arr, nm = abi.radix_terms_args(terms, models)
+        self.lib.orc_set_radix_terms(self.h, C.byref(t), arr, nm)
+
+    def CalcTurbulenceViscosity(self):
+        self.lib.orc_turbulence(self.h)
+
+    def get_turbulence(self):
+        nu, k = np.zeros(self.n), np.zeros(self.n)
+        self.lib.orc_get_turbulence(self.h, nu.ctypes.data_as(C.POINTER(C.c_double)),
+                                    k.ctypes.data_as(C.POINTER(C.c_double)), self.n)
+        return nu, k
 
     def RotateTrianglePosition(self, step):
         self.lib.orc_rotate(self.h, step)

@@ -86,6 +86,9 @@ def load_library(path=None):
         "mpsr_mpnn_train_step": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, P(C.c_float), P(C.c_double)]),
         "mpsr_mpnn_rows": (C.c_int, [vp, P(C.c_float), P(C.c_float), P(C.c_double)]),
         "mpsr_set_launch": (C.c_int, [vp, C.c_int, C.c_int]),
+        "mpsr_set_radix_terms": (C.c_int, [vp, P(abi.RadixTerms), P(abi.NegativePressureModel), C.c_int]),
+        "mpsr_calc_turbulence_viscosity": (C.c_int, [vp]),
+        "mpsr_get_turbulence": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_int]),
         "mpsr_create_batch": (C.c_int, [P(abi.CreateInfo), C.c_int, P(vp)]),
         "mpsr_destroy_batch": (None, [vp]),
         "mpsr_batch_handle": (C.c_int, [vp, C.c_int, P(vp)]),
@@ -170,6 +173,22 @@ class Solver:
     def set_launch(self, block, sort_items):
         """Launch shape of the per-particle kernels and the particle sort (results do not change)."""
         self._check(self.lib.mpsr_set_launch(self.h, block, sort_items), "mpsr_set_launch")
+
+    def set_radix_terms(self, terms, models=None):
+        """The radix variant's turbulence and negative-pressure terms for the following steps.
+        terms: abi.RadixTerms or a dict of its fields; models: NegativePressureModel per model."""
+        t, arr, nm = abi.radix_terms_args(terms, models)
+        self._check(self.lib.mpsr_set_radix_terms(self.h, C.byref(t), arr, nm), "mpsr_set_radix_terms")
+
+    def CalcTurbulenceViscosity(self):
+        self._check(self.lib.mpsr_calc_turbulence_viscosity(self.h), "CalcTurbulenceViscosity")
+
+    def get_turbulence(self):
+        """(turbulent viscosity, turbulent kinetic energy) per slot."""
+        n = self.n
+        nu, k = np.zeros(n), np.zeros(n)
+        self._check(self.lib.mpsr_get_turbulence(self.h, _dp(nu), _dp(k), n), "mpsr_get_turbulence")
+        return nu, k
 
     def close(self):
         if getattr(self, "h", None):

@@ -119,6 +119,37 @@ class PpeOptions(C.Structure):
                 ("learned_omega", C.c_int), ("source_relaxation", C.c_double)]
 
 
+class RadixTerms(C.Structure):
+    """mpsr_radix_terms: the radix variant's turbulence and negative-pressure switches
+    (radix/SPVSOLVER/MultiGPUSolver/DataType.h:510-520)."""
+    _fields_ = [("bTurbulance", C.c_int), ("SmagorinskyConstant", C.c_float), ("FilterWidth", C.c_float),
+                ("TurbulanViscosityRatio", C.c_float), ("Cnue", C.c_float), ("Cepsilon", C.c_float),
+                ("TurbulanceIntensity", C.c_float), ("bNegativePressure", C.c_int)]
+
+
+class NegativePressureModel(C.Structure):
+    """mpsr_negative_pressure_model: per-model fields of the radix CTriangleParameters
+    (DataType.h:160-168)."""
+    _fields_ = [("RotationInRPM", C.c_float), ("NegativePressureCheck", C.c_int), ("InfluenceRegion", C.c_float),
+                ("DirectionVector", Float3), ("PressureGas", C.c_float)]
+
+
+def radix_terms_args(terms, models):
+    """(RadixTerms, NegativePressureModel array or None, count) from structs or dicts."""
+    t = terms if isinstance(terms, RadixTerms) else RadixTerms(**dict(terms))
+    if models is None:
+        return t, None, 0
+    arr = (NegativePressureModel * max(1, len(models)))()
+    for i, m in enumerate(models):
+        if isinstance(m, NegativePressureModel):
+            arr[i] = m
+        else:
+            m = dict(m)
+            dv = m.pop("DirectionVector", (0.0, 0.0, 0.0))
+            arr[i] = NegativePressureModel(DirectionVector=Float3(*dv), **m)
+    return t, arr, len(models)
+
+
 class SpheresC(C.Structure):
     _fields_ = [("n", C.c_int), ("position", _dp), ("velocity", _dp), ("force", _dp), ("diameter", C.c_double),
                 ("density", C.c_double), ("fluid_viscosity", C.c_double)]
@@ -143,7 +174,7 @@ class Timings(C.Structure):
                 ("ppe_rows", C.c_int), ("ppe_fallbacks", C.c_int),
                 ("ppe_coarse_rows", C.c_int), ("ppe_coarse_rebuilt", C.c_int), ("ppe_setup_ms", C.c_double),
                 ("ppe_learned", C.c_int), ("ppe_learned_fallback", C.c_int), ("sphere_ms", C.c_double),
-                ("ppe_stored", C.c_longlong), ("thermal_ms", C.c_double)]
+                ("ppe_stored", C.c_longlong), ("thermal_ms", C.c_double), ("turbulence_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -38,7 +38,8 @@ static mpsr_status fail(mpsr_status s, const std::string& m) {
         if (e_ != hipSuccess) return fail(MPSR_GPUERR, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-enum Phase { PH_BIN, PH_TRI, PH_STL, PH_DRAG, PH_EXPL, PH_RELOC, PH_PND, PH_PPE, PH_GRAD, PH_WALL, PH_SPH, PH_THERM, PH_COUNT };
+enum Phase { PH_BIN, PH_TRI, PH_STL, PH_DRAG, PH_EXPL, PH_RELOC, PH_PND, PH_PPE, PH_GRAD, PH_WALL, PH_SPH, PH_THERM, PH_TURB,
+             PH_COUNT };
 
 struct EngineBase {
     virtual ~EngineBase() {}
@@ -67,6 +68,9 @@ struct EngineBase {
     virtual mpsr_status get_spheres(mpsr_spheres* s) = 0;
     // launch shape of the per-particle kernels and the particle sort (results do not depend on it)
     virtual mpsr_status set_launch(int block, int sort_items) = 0;
+    // the radix variant's turbulence and negative-pressure terms
+    virtual mpsr_status set_radix(const mpsr_radix_terms* t, const mpsr_negative_pressure_model* m, int nm) = 0;
+    virtual mpsr_status get_turbulence(double* nu, double* k, int n) = 0;
     // batched form: one step of this domain recorded on its own stream (into a shared graph)
     virtual mpsr_status batch_check() = 0;
     virtual mpsr_status batch_prepare(int first, int count, int* moved) = 0;
@@ -220,6 +224,7 @@ template <typename R> struct Engine : EngineBase {
     RotCoef<R>* rot_table = nullptr;   // device [rot_steps * nmodel]
     int rot_steps = 0;
     std::vector<mpsr_tri_params> models_h;
+    int consts_ver = 0;                // bumped when the per-launch constants change
     std::vector<RotCoef<R>> rot_host;
     int pressure_mode = 0;
     mpsr_ppe_options ppe_opt{};
@@ -620,6 +625,13 @@ template <typename R> struct Engine : EngineBase {
         launch_thermal(d, nullptr, nullptr, P.ThermalStep, st);
         mark_end();
     }
+    // CalcTurbulenceViscosity (radix MultiGPUHandler.cpp:1421-1427): after the temperature
+    void do_turbulence() {
+        if (!d.c.turb) return;
+        mark_begin(PH_TURB);
+        launch_turbulence(d, st);
+        mark_end();
+    }
     void do_spheres() {
         if (sph.S <= 0) return;
         mark_begin(PH_SPH);
@@ -683,6 +695,7 @@ template <typename R> struct Engine : EngineBase {
         do_stl();
         if (d.ndrag > 0) do_drag();
         do_thermal(step);
+        do_turbulence();
         do_spheres();
         do_explicit();
         do_relocate();
@@ -737,8 +750,64 @@ template <typename R> struct Engine : EngineBase {
     int snap_n = 0;
     // what the last recorded step holds: rotation table address and launch shape
     const RotCoef<R>* rec_rot = nullptr;
-    int rec_block = 0, rec_items = 0;
+    int rec_block = 0, rec_items = 0, rec_consts = 0;
 
+    mpsr_status set_radix(const mpsr_radix_terms* t, const mpsr_negative_pressure_model* m, int nm) override {
+        HIPC(hipSetDevice(device));
+        if (!t) return fail(MPSR_PARAMERR, "radix terms: NULL");
+        if (t->bTurbulance && !(t->Cepsilon != 0.0f))
+            return fail(MPSR_PARAMERR, "radix terms: Cepsilon must be nonzero with bTurbulance");
+        if (t->bNegativePressure && (!m || nm != d.nmodel))
+            return fail(MPSR_PARAMERR, "radix terms: bNegativePressure needs one negative-pressure entry per model");
+        HIPC(hipStreamSynchronize(st));
+        if (t->bTurbulance && !d.turb_nu) {
+            ALLOC(d.turb_nu, d.cap);
+            ALLOC(d.turb_k, d.cap);
+            HIPC(hipMemset(d.turb_nu, 0, sizeof(R) * d.cap));
+            HIPC(hipMemset(d.turb_k, 0, sizeof(R) * d.cap));
+        }
+        if (t->bNegativePressure) {
+            std::vector<NegModel<R>> h(std::max(1, d.nmodel));
+            for (int i = 0; i < d.nmodel; ++i) {
+                NegModel<R> q{};
+                q.active = ((R)m[i].RotationInRPM != 0 || m[i].NegativePressureCheck) ? 1 : 0;
+                q.infl = (R)m[i].InfluenceRegion;
+                q.pgas = (R)m[i].PressureGas;
+                q.dx = (R)m[i].DirectionVector.x;
+                q.dy = (R)m[i].DirectionVector.y;
+                q.dz = (R)m[i].DirectionVector.z;
+                h[i] = q;
+            }
+            if (!d.negm) ALLOC(d.negm, std::max(1, d.nmodel));
+            HIPC(hipMemcpy(d.negm, h.data(), sizeof(NegModel<R>) * h.size(), hipMemcpyHostToDevice));
+        }
+        // radix DeviceProcess_Kernel.h:1304, 1405-1466: the expressions of the kernel's constants
+        Consts<R>& c = d.c;
+        c.turb = t->bTurbulance ? 1 : 0;
+        c.negp = t->bNegativePressure ? 1 : 0;
+        c.turb_g = (R)P.Dimension / (R)P.ConstantParticleDensity;
+        c.turb_cs = (R)t->SmagorinskyConstant;
+        c.turb_fw = (R)t->FilterWidth;
+        c.turb_numax = (R)t->TurbulanViscosityRatio;
+        c.turb_kmax = (R)t->TurbulanceIntensity;
+        c.turb_tk = (R)2 * (R)t->Cnue * (R)P.InitialParticleDistance * (R)P.InitialParticleDistance / (R)t->Cepsilon;
+        ++consts_ver;
+        return MPSR_NOERR;
+    }
+    mpsr_status get_turbulence(double* nu, double* k, int n) override {
+        HIPC(hipStreamSynchronize(st));
+        if (n < 0 || n > d.cap) return fail(MPSR_PARAMERR, "row count exceeds the allocated slots");
+        if (!d.turb_nu) return fail(MPSR_PARAMERR, "turbulence is off (mpsr_set_radix_terms)");
+        std::vector<R> a, b;
+        mpsr_status ds = d2h(a, d.turb_nu, (size_t)n);
+        if (ds == MPSR_NOERR) ds = d2h(b, d.turb_k, (size_t)n);
+        if (ds != MPSR_NOERR) return ds;
+        for (int s = 0; s < n; ++s) {
+            if (nu) nu[s] = (double)a[s];
+            if (k) k[s] = (double)b[s];
+        }
+        return MPSR_NOERR;
+    }
     mpsr_status set_launch(int block, int items) override {
         if (block < 64 || block > kMaxBlock || block % 64)
             return fail(MPSR_PARAMERR, "launch block must be a multiple of 64 in [64, 512]");
@@ -763,7 +832,7 @@ template <typename R> struct Engine : EngineBase {
         if (!step_base_buf) ALLOC(step_base_buf, 1);
         mpsr_status s = upload_rot(first, count);
         if (s != MPSR_NOERR) return s;
-        *moved = rec_rot != rot_table || rec_block != d.block || rec_items != d.sort_items;
+        *moved = rec_rot != rot_table || rec_block != d.block || rec_items != d.sort_items || rec_consts != consts_ver;
         fill_int(step_ctr_buf, 0, 1, st);
         fill_int(step_base_buf, first, 1, st);
         HIPC(hipStreamSynchronize(st));   // rot_host is refilled by the next call
@@ -779,11 +848,13 @@ template <typename R> struct Engine : EngineBase {
         rec_rot = rot_table;
         rec_block = d.block;
         rec_items = d.sort_items;
+        rec_consts = consts_ver;
         do_bin();
         do_tri();
         do_stl();
         if (d.ndrag > 0) do_drag();
         if (P.bUpdateTemperature) launch_thermal(d, step_base_buf, step_ctr_buf, P.ThermalStep, st);
+        do_turbulence();
         do_explicit();
         do_relocate();
         do_bin();
@@ -851,6 +922,7 @@ template <typename R> struct Engine : EngineBase {
             t.drag_ms = acc[PH_DRAG] / count;
             t.sphere_ms = acc[PH_SPH] / count;
             t.thermal_ms = acc[PH_THERM] / count;
+            t.turbulence_ms = acc[PH_TURB] / count;
             t.explicit_ms = acc[PH_EXPL] / count;
             t.relocate_ms = acc[PH_RELOC] / count;
             t.pnd_ms = acc[PH_PND] / count;
@@ -907,6 +979,10 @@ template <typename R> struct Engine : EngineBase {
                 break;
             case 11: do_spheres(); break;
             case 12: launch_thermal(d, nullptr, nullptr, P.ThermalStep, st); break;
+            case 13:
+                if (!d.turb_nu) { s = fail(MPSR_PARAMERR, "turbulence is off (mpsr_set_radix_terms)"); break; }
+                launch_turbulence(d, st);
+                break;
             default: s = fail(MPSR_PARAMERR, "unknown phase");
         }
         profiling = saved;
@@ -1275,6 +1351,15 @@ void mpsr_destroy(mpsr_handle* h) {
 }
 mpsr_status mpsr_step(mpsr_handle* h, int i) { GUARD(h); return h->e->step_many(i, 1); }
 mpsr_status mpsr_set_launch(mpsr_handle* h, int block, int sort_items) { GUARD(h); return h->e->set_launch(block, sort_items); }
+mpsr_status mpsr_set_radix_terms(mpsr_handle* h, const mpsr_radix_terms* t, const mpsr_negative_pressure_model* m, int nm) {
+    GUARD(h);
+    return h->e->set_radix(t, m, nm);
+}
+mpsr_status mpsr_calc_turbulence_viscosity(mpsr_handle* h) { GUARD(h); return h->e->phase(13, 0); }
+mpsr_status mpsr_get_turbulence(mpsr_handle* h, double* nu, double* k, int n) {
+    GUARD(h);
+    return h->e->get_turbulence(nu, k, n);
+}
 
 mpsr_status mpsr_create_batch(const mpsr_create_info* infos, int nbuckets, mpsr_batch** out) {
     if (!infos || !out || nbuckets <= 0) return fail(MPSR_PARAMERR, "mpsr_create_batch: no buckets");

@@ -77,6 +77,19 @@ template <typename R> struct Consts {
     int ntri;
     // heat conduction, Kern :485-632
     R re_temp, mind_temp, enth, kf, inv_kf, rf, tmax, tmin, visc_a, visc_b;
+    // the radix variant's terms: Smagorinsky viscosity (radix DeviceProcess_Kernel.h:1273-1476)
+    // and the negative-pressure gradient (:1214-1241); both off unless set by the caller
+    int turb, negp;
+    R turb_g;                 // Dimension / ConstantParticleDensity
+    R turb_cs, turb_fw;       // SmagorinskyConstant, FilterWidth
+    R turb_numax, turb_kmax;  // TurbulanViscosityRatio, TurbulanceIntensity
+    R turb_tk;                // 2 Cnue l0 l0 / Cepsilon
+};
+
+// per-model negative-pressure fields (radix CTriangleParameters, DataType.h:160-168)
+template <typename R> struct NegModel {
+    int active;   // RotationInRPM != 0 || NegativePressureCheck
+    R infl, pgas, dx, dy, dz;
 };
 
 // position -> cell coordinates, Utility_Inline.h:1636-1674 (C truncation toward zero)
@@ -295,6 +308,9 @@ template <typename R> struct Dev {
     int* stl_id = nullptr;
     R4<R>* drag_acc = nullptr;
     R4<R>* sph_dv = nullptr;          // [cap] sorted: fluid velocity change from the spheres
+    R* turb_nu = nullptr;             // [cap] slot order: turbulent viscosity (zeroed at allocation)
+    R* turb_k = nullptr;              // [cap] slot order: turbulent kinetic energy
+    NegModel<R>* negm = nullptr;      // [nmodel] negative-pressure fields
     R* drag_temp = nullptr;
     // radix-sort ping-pong buffers: one set for the particle keys (the sorted keys and the
     // permutation may end up in them and must live until the next binning), one for the
@@ -314,6 +330,7 @@ template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st);
 // is a multiple of thermal_step (graph replays), else the caller decides
 template <typename R> void launch_thermal(const Dev<R>& d, const int* step_base, const int* step_ctr, int thermal_step,
                                           hipStream_t st);
+template <typename R> void launch_turbulence(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_neighbor_count(const Dev<R>& d, R re, int* counts, hipStream_t st);
 template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int* offsets, int* nbr, hipStream_t st);
 // walls.hip

@@ -8,7 +8,9 @@
 //   k_explicit      CalcExplicitly_Kernel (Kern :283-441)
 //   k_relocate      RelocateParticleData_Kernel (Kern :61-92)
 //   k_pnd           CalcPressureExplicit_Kernel (Kern :702-801)
-//   k_gradient      CalcImplicitExplicitly_Kernel (Kern :804-912)
+//   k_gradient      CalcImplicitExplicitly_Kernel (Kern :804-912), optionally with the radix
+//                   variant's negative-pressure term (radix/.../DeviceProcess_Kernel.h:1214-1245)
+//   k_turbulence    CalcTurbulenceViscosity_Kernel (radix/.../DeviceProcess_Kernel.h:1273-1476)
 //
 // The operator kernels run one thread per sorted index t.  Neighbours are visited in the
 // canonical order: stencil z outer, x inner (Kern :317-330), ascending sorted index inside a
@@ -18,6 +20,7 @@
 #include <cfloat>
 #include "mpsr_internal.h"
 #include "neighbors.h"
+#include "walls_geom.h"
 
 namespace mpsr {
 
@@ -104,7 +107,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_explicit(int N, Consts<R> c, cons
                                                    const R4<R>* __restrict__ stl_dir, const int* __restrict__ stl_id,
                                                    const R4<R>* __restrict__ drag_acc, const R4<R>* __restrict__ sph_dv,
                                                    const TriD<R>* __restrict__ tris,
-                                                   const mpsr_tri_params* __restrict__ models) {
+                                                   const mpsr_tri_params* __restrict__ models,
+                                                   const R* __restrict__ turb_nu) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
@@ -171,7 +175,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_explicit(int N, Consts<R> c, cons
         const R4<R> Dg = drag_acc[t];
         dvx = Dg.x; dvy = Dg.y; dvz = Dg.z;
     }
-    const R nu = in.nu[s];
+    // with the radix variant's turbulence the viscosity is nu + nu_t (radix
+    // DeviceProcess_Kernel.h:545-552); its k-gradient term (:558-562) is scaled by the integer
+    // quotient 2/3 = 0 there, so it changes no finite velocity and is not formed
+    const R nu = c.turb ? in.nu[s] + turb_nu[s] : in.nu[s];
     lx = lx * nu; ly = ly * nu; lz = lz * nu;
     accx = accx + lx; accy = accy + ly; accz = accz + lz;
     R vx = (V0.x + accx * c.dt) - dvx;
@@ -283,7 +290,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_gradient(int N, Consts<R> c, cons
                                                    const R4<R>* __restrict__ svel, const R* __restrict__ spres,
                                                    const int* __restrict__ cs, const int* __restrict__ ce,
                                                    const R* __restrict__ stl_mag, const R4<R>* __restrict__ stl_dir,
-                                                   const int* __restrict__ stl_id) {
+                                                   const int* __restrict__ stl_id, const int* __restrict__ tstart,
+                                                   const int* __restrict__ tcount, const int* __restrict__ tids,
+                                                   const TriD<R>* __restrict__ tris,
+                                                   const NegModel<R>* __restrict__ negm) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
@@ -319,7 +329,25 @@ __global__ __launch_bounds__(kMaxBlock) void k_gradient(int N, Consts<R> c, cons
             wz = mc * (-D.z);
         }
     }
-    const R grx = gx + wx, gry = gy + wy, grz = gz + wz;
+    R grx = gx + wx, gry = gy + wy, grz = gz + wz;
+    if (c.negp) {
+        // radix DeviceProcess_Kernel.h:1214-1245: gas pressure PressureGas acting from the nearest
+        // wall of a rotating or flagged model within InfluenceRegion l0, along its DirectionVector
+        R nx = 0, ny = 0, nz = 0, best;
+        const int nid = negp_triangle(c, x, y, z, tstart, tcount, tids, tris, &best);
+        if (nid != -1 && best != (R)1e8) {
+            const NegModel<R> m = negm[tris[nid].model];
+            if (m.active && best <= m.infl * c.l0) {
+                const R iw = ztable_eval(kZgrad, c.l0, c.l0);
+                const R pc = (R)(0.5 * (double)(m.pgas + pi) * (double)iw / (double)(c.l0 * c.l0));
+                nx = pc * m.dx;
+                ny = pc * m.dy;
+                nz = pc * m.dz;
+            }
+        }
+        nx *= c.c1; ny *= c.c1; nz *= c.c1;
+        grx = grx + nx; gry = gry + ny; grz = grz + nz;
+    }
     const R ix = c.c3 * grx, iy = c.c3 * gry, iz = c.c3 * grz;
     if (isfinite(ix) && isfinite(iy) && isfinite(iz)) {
         out.px[s] = x + ix * c.dt;
@@ -444,11 +472,86 @@ template <typename R> void launch_bin_ranges(Dev<R>& d, hipStream_t st) {
     hipLaunchKernelGGL(k_bin_ranges<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c.ncell, d.skeys,
                        d.perm, d.cell_start, d.cell_end, d.in, d.spos, d.svel);
 }
+// ---------------------------------------------------------------------------------------------
+// CalcTurbulenceViscosity_Kernel (radix DeviceProcess_Kernel.h:1273-1476): the velocity gradient
+// tensor over the gradient radius (:1324-1397), S = A^2 + B^2 + C^2 + 2 D^2 + 2 E^2 + 2 F^2 of its
+// symmetric part (:1405-1429), wall damping 1 - exp(-r_wall / re) with exp at single precision
+// as the reference's expf (:1437-1440), nu_t = (Cs damp f)^2 sqrt(2 S) clamped by
+// TurbulanViscosityRatio, k = S 2 Cnue l0^2 / Cepsilon clamped by TurbulanceIntensity
+// (:1443-1474).  The vorticity and density sums there feed nothing and are not formed.  Results go
+// to slot order; a particle whose cell lies outside the grid gets 0 (the reference leaves the
+// array as it was).
+template <typename R>
+__global__ __launch_bounds__(kMaxBlock) void k_turbulence(int N, Consts<R> c, const int* __restrict__ perm,
+                                                     PState<R> out, const R4<R>* __restrict__ spos,
+                                                     const R4<R>* __restrict__ svel, const int* __restrict__ cs,
+                                                     const int* __restrict__ ce, const R* __restrict__ stl_mag,
+                                                     R* __restrict__ turb_nu, R* __restrict__ turb_k) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int s = perm[t];
+    if (out.id[s] < 0) return;
+    const R4<R> P = spos[t];
+    const R x = P.x, y = P.y, z = P.z;
+    if (!include(c, x, y, z)) {
+        turb_nu[s] = 0;
+        return;
+    }
+    int ci, cj, ck;
+    long long cid = -1;
+    if (cell_coords(c, x, y, z, ci, cj, ck)) cid = cell_of(c, ci, cj, ck);
+    if (cid < 0 || cid >= c.ncell) {
+        turb_nu[s] = 0;
+        turb_k[s] = 0;
+        return;
+    }
+    const R4<R> V0 = svel[t];
+    R txx = 0, txy = 0, txz = 0, tyx = 0, tyy = 0, tyz = 0, tzx = 0, tzy = 0, tzz = 0;
+    const R re = c.re_grad, mind = c.mind_grad;
+    for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
+        const R4<R> Q = spos[q];
+        R d = dist3(x, y, z, Q.x, Q.y, Q.z);
+        if (d < re) {
+            if (d < mind) d = mind;
+            const R w = weight(re, d);
+            const R4<R> V1 = svel[q];
+            const R ux = V1.x - V0.x, uy = V1.y - V0.y, uz = V1.z - V0.z;
+            const R rx = Q.x - x, ry = Q.y - y, rz = Q.z - z;
+            const R dd = d * d;
+            txx = txx + ux * rx * w / dd; txy = txy + ux * ry * w / dd; txz = txz + ux * rz * w / dd;
+            tyx = tyx + uy * rx * w / dd; tyy = tyy + uy * ry * w / dd; tyz = tyz + uy * rz * w / dd;
+            tzx = tzx + uz * rx * w / dd; tzy = tzy + uz * ry * w / dd; tzz = tzz + uz * rz * w / dd;
+        }
+    });
+    const R g = c.turb_g;
+    txx = g * txx; txy = g * txy; txz = g * txz;
+    tyx = g * tyx; tyy = g * tyy; tyz = g * tyz;
+    tzx = g * tzx; tzy = g * tzy; tzz = g * tzz;
+    const R D = (R)0.5 * (txy + tyx), E = (R)0.5 * (txz + tzx), F = (R)0.5 * (tyz + tzy);
+    const R S = txx * txx + tyy * tyy + tzz * tzz + (R)2 * D * D + (R)2 * E * E + (R)2 * F * F;
+    const R ratio = -(stl_mag[t] / re);
+    const R damp = (R)1 - (R)(float)det_exp((double)(float)ratio);
+    const R cd = c.turb_cs * damp;
+    const R coef = (cd * c.turb_fw) * (cd * c.turb_fw);
+    R nut = 0;
+    if (S > 0) nut = coef * sqrt((R)2 * S);
+    if (nut > c.turb_numax) nut = c.turb_numax;
+    R k = S * c.turb_tk;
+    if (k > c.turb_kmax) k = c.turb_kmax;
+    turb_nu[s] = nut;
+    turb_k[s] = k;
+}
+
 template <typename R> void launch_explicit(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL(k_explicit<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
                        d.spos, d.svel, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id, d.drag_acc, d.sph_dv,
-                       d.tris, d.models);
+                       d.tris, d.models, d.turb_nu);
+}
+template <typename R> void launch_turbulence(const Dev<R>& d, hipStream_t st) {
+    if (d.N <= 0 || !d.turb_nu) return;
+    hipLaunchKernelGGL(k_turbulence<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.out,
+                       d.spos, d.svel, d.cell_start, d.cell_end, d.stl_mag, d.turb_nu, d.turb_k);
 }
 template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
@@ -462,7 +565,8 @@ template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st) 
 template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL(k_gradient<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
-                       d.spos, d.svel, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id);
+                       d.spos, d.svel, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id,
+                       d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.negm);
 }
 template <typename R>
 void launch_thermal(const Dev<R>& d, const int* step_base, const int* step_ctr, int thermal_step, hipStream_t st) {
@@ -490,6 +594,7 @@ template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int
     template void launch_pnd<R>(const Dev<R>&, int, hipStream_t);                             \
     template void launch_gradient<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_thermal<R>(const Dev<R>&, const int*, const int*, int, hipStream_t);   \
+    template void launch_turbulence<R>(const Dev<R>&, hipStream_t);                           \
     template void launch_neighbor_count<R>(const Dev<R>&, R, int*, hipStream_t);              \
     template void launch_neighbor_fill<R>(const Dev<R>&, R, const int*, int*, hipStream_t);
 MPSR_INSTANTIATE(float)

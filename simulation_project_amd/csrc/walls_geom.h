@@ -108,6 +108,41 @@ __device__ inline int nearest_triangle(const Consts<R>& c, R x, R y, R z, const 
     return bid;
 }
 
+// The wall of the negative-pressure term (radix DeviceProcess_Kernel.h:1110-1134): every triangle
+// of the 27-cell stencil's lists, drag or not; the first listed triangle of each cell replaces the
+// running nearest (the reference's l == 0 branch) unless it is that triangle already, the others
+// replace it when strictly nearer.  Returns the id (-1 for none) and the distance in *best.
+template <typename R>
+__device__ inline int negp_triangle(const Consts<R>& c, R x, R y, R z, const int* __restrict__ tstart,
+                                    const int* __restrict__ tcount, const int* __restrict__ tids,
+                                    const TriD<R>* __restrict__ tris, R* best) {
+    *best = (R)1e8;
+    int nid = -1;
+    int ci, cj, ck;
+    if (!cell_coords(c, x, y, z, ci, cj, ck)) return nid;
+    const long long cid = cell_of(c, ci, cj, ck);
+    if (cid < 0 || cid >= c.ncell) return nid;
+    const P3<R> p = p3(x, y, z);
+    for (int k = -1; k <= 1; ++k)
+        for (int j = -1; j <= 1; ++j)
+            for (int i = -1; i <= 1; ++i) {
+                const long long m = cell_of(c, ci + i, cj + j, ck + k);
+                if (m < 0 || m >= c.ncell) continue;
+                const int n = tcount[m];
+                const int a = tstart[m];
+                for (int l = 0; l < n; ++l) {
+                    const int tid = tids[a + l];
+                    if (tid >= 0 && tid < c.ntri && tid != nid) {
+                        P3<R> dv;
+                        const R dm = tri_distance(tris[tid], p, &dv);
+                        if (l == 0) { *best = dm; nid = tid; }
+                        if (dm < *best) { *best = dm; nid = tid; }
+                    }
+                }
+            }
+    return nid;
+}
+
 // Deterministic pow, exp(y log x) in double rounded once to R, from IEEE-exact operations only
 // (+ - * /, frexp, ldexp, floor) so that host and device give the same bits whatever their libm:
 //   log x = e ln2 + 2 atanh(s), s = (m - 1)/(m + 1), m in [sqrt(1/2), sqrt(2)), series to s^23
