@@ -16,8 +16,9 @@ resident in HBM when the timed region starts; inside it only the PCG's convergen
 32 iterations) and the per-step timings cross PCIe.
 
 Production solver: mixed precision (FP32 matrix and SpMV, FP64 dots and residuals, FP64
-refinement) with the two-level preconditioner (3^3-cell aggregates, coarse operator rebuilt every
-10 solves), warm-started by the previous pressure.  --solver learned adds the trained network's
+refinement) with the two-level preconditioner (2^3-cell aggregates, coarse operator rebuilt every
+50 solves; the JSON line gives the set-up time of one rebuild and its share per step),
+warm-started by the previous pressure.  --solver learned adds the trained network's
 warm start (simulation_project_amd/weights/mpnn_drum.npy).  The plain comparison solves the same
 steps with FP64 Jacobi-PCG from the previous pressure.
 
@@ -52,7 +53,7 @@ def log(msg):
 def ppe_options(abi, kind, precond, warm):
     solver = abi.MPSR_PCG_FP64 if kind == "fp64" else abi.MPSR_PCG_MIXED
     pre = abi.MPSR_PRECOND_TWO_LEVEL if precond == "two" else abi.MPSR_PRECOND_JACOBI
-    return abi.PpeOptions(solver, 1e-8, 20000, warm, 0.0, 0.0, pre, 3, 10)
+    return abi.PpeOptions(solver, 1e-8, 20000, warm, 0.0, 0.0, pre, 2, 50)
 
 
 def run_steps(s, first, count):
@@ -142,6 +143,18 @@ def main():
     log(f"[bench] {ms_per_step:.2f} ms/step, PPE {np.mean(ppe_ms):.2f} ms, iterations {iters}, "
         f"SpMV {spmv_avg_ms * 1e3:.1f} us x {launches}")
     learned_fb = sum(t.get("ppe_learned_fallback", 0) for t in per_step)
+    # the coarse operator is rebuilt every 50 solves (and when the iterations grow by a quarter):
+    # one more step, after the timed region, with a forced rebuild gives its set-up time
+    opt = ppe_options(abi, kind, args.precond, warm)
+    every = opt.coarse_rebuild
+    opt.coarse_rebuild = 1
+    s.set_ppe_options(opt)
+    run_steps(s, args.warmup + args.steps, 1)
+    tr = s.timings()
+    coarse = {"rows": tr["ppe_coarse_rows"], "rebuild_every": every, "setup_ms": round(tr["ppe_setup_ms"], 3),
+              "amortized_ms_per_step": round(tr["ppe_setup_ms"] / every, 3),
+              "rebuilds_in_timed_steps": int(sum(t["ppe_coarse_rebuilt"] for t in per_step))}
+    log(f"[bench] coarse operator: {coarse}")
     s.close()
 
     plain = None
@@ -205,6 +218,7 @@ def main():
         "ppe_rel_residual_max": max(relres),
         "ppe_learned_fallbacks": learned_fb,
         "ppe_entries": {"real": int(nnz), "stored": int(np.mean([t["ppe_stored"] for t in per_step]))},
+        "ppe_coarse": coarse,
         "plain_fp64_pcg": plain,
         "phase_ms": phases,
         "step_roofline": {"achieved": round(step_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

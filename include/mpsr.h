@@ -224,9 +224,10 @@ typedef struct mpsr_ppe_options {
     double n0_lap;     /* PND of the ideal lattice at the Laplacian radius; 0 = computed */
     double lambda_lap; /* Laplacian model constant at the Laplacian radius; 0 = computed */
     int precond;       /* mpsr_precond */
-    int coarse_cells;  /* aggregate edge in cells for the two-level preconditioner, 1..8; 0 = 4 */
+    int coarse_cells;  /* aggregate edge in cells for the two-level preconditioner, 1..8; 0 = 2 (the
+                          edge grows while the aggregates outnumber 8192) */
     int coarse_rebuild;/* rebuild the coarse operator every this many solves (or when the iteration
-                          count grows by a quarter over the one after the last rebuild); 0 = 10,
+                          count grows by a quarter over the one after the last rebuild); 0 = 50,
                           1 = every solve.  In between, the last coarse inverse is applied to the
                           current aggregates: the preconditioner stays symmetric positive definite. */
     int learned_omega; /* with MPSR_WARM_LEARNED: 1 = also scale the preconditioner's diagonal part

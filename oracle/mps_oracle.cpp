@@ -1523,6 +1523,12 @@ void orc_gradient(void* p) { Handle* h = (Handle*)p; DISPATCH(h, gradient()); }
 void orc_translate(void* p) { Handle* h = (Handle*)p; DISPATCH(h, translate()); }
 void orc_thermal(void* p) { Handle* h = (Handle*)p; DISPATCH(h, thermal()); }
 void orc_turbulence(void* p) { Handle* h = (Handle*)p; DISPATCH(h, turbulence()); }
+// CheckAndUpdateDt (Src MultiGPUHandler.cpp:1048): the new Dt and Courant number in the parameters
+void orc_set_dt(void* p, float dt, float courant) {
+    Handle* h = (Handle*)p;
+    if (h->d) { h->d->P.Dt = dt; h->d->P.CourantNumber = courant; }
+    if (h->f) { h->f->P.Dt = dt; h->f->P.CourantNumber = courant; }
+}
 void orc_set_radix_terms(void* p, const mpsr_radix_terms* t, const mpsr_negative_pressure_model* m, int nm) {
     Handle* h = (Handle*)p;
     std::vector<mpsr_negative_pressure_model> v(m ? m : nullptr, m ? m + nm : nullptr);

@@ -64,6 +64,7 @@ def load():
         "orc_translate": (None, [vp]),
         "orc_thermal": (None, [vp]),
         "orc_turbulence": (None, [vp]),
+        "orc_set_dt": (None, [vp, C.c_float, C.c_float]),
         "orc_set_radix_terms": (None, [vp, P(abi.RadixTerms), P(abi.NegativePressureModel), C.c_int]),
         "orc_get_turbulence": (None, [vp, P(C.c_double), P(C.c_double), C.c_int]),
         "orc_rotate": (None, [vp, C.c_int]),
@@ -218,6 +219,9 @@ class Oracle:
 
     def CalcTurbulenceViscosity(self):
         self.lib.orc_turbulence(self.h)
+
+    def set_dt(self, dt, courant):
+        self.lib.orc_set_dt(self.h, dt, courant)
 
     def get_turbulence(self):
         nu, k = np.zeros(self.n), np.zeros(self.n)

@@ -498,7 +498,7 @@ template <typename R> struct Engine : EngineBase {
         d.c = make_consts<R>(P, gb, d.ntri, d.ndrag);
         pressure_mode = ci->pressure_mode;
         if (ci->ppe.coarse_cells < 0 || ci->ppe.coarse_cells > 8)
-            return fail(MPSR_PARAMERR, "ppe.coarse_cells must be 0 (default 4) or 1..8");
+            return fail(MPSR_PARAMERR, "ppe.coarse_cells must be 0 (default 2) or 1..8");
         ppe_opt = ci->ppe;
         int bits = 1;
         while ((1ll << bits) <= (long long)d.c.ncell) ++bits;  // keys 0..ncell (sentinel = ncell)
@@ -651,7 +651,7 @@ template <typename R> struct Engine : EngineBase {
     mpsr_status do_pressure() {
         if (pressure_mode == MPSR_PRESSURE_PPE) {
             mark_begin(PH_PND);
-            launch_pnd(d, 0, st);
+            ppe_start(ppe, d, st);
             mark_end();
             mark_begin(PH_PPE);
             mpsr_status s = ppe_solve(ppe, d, ppe_opt, profiling, st);
@@ -968,7 +968,7 @@ template <typename R> struct Engine : EngineBase {
                     ppe = ppe_create<R>(d, P, ppe_opt, n0_lap, lambda_lap, n0_src);
                     if (!ppe) { s = fail(MPSR_MEMERR, "PPE solver allocation failed"); break; }
                 }
-                launch_pnd(d, 0, st);
+                ppe_start(ppe, d, st);
                 s = ppe_solve(ppe, d, ppe_opt, 0, st);
                 break;
             case 8: do_gradient(); break;
@@ -1152,13 +1152,25 @@ template <typename R> struct Engine : EngineBase {
         HIPC(hipStreamSynchronize(st));
         P.Dt = dt;
         P.CourantNumber = courant;
+        // the constants set by mpsr_set_spheres and mpsr_set_radix_terms do not depend on dt
+        const Consts<R> old = d.c;
         d.c = make_consts<R>(P, gb, d.ntri, d.ndrag);
+        d.c.has_sph = old.has_sph;
+        d.c.turb = old.turb;
+        d.c.negp = old.negp;
+        d.c.turb_g = old.turb_g;
+        d.c.turb_cs = old.turb_cs;
+        d.c.turb_fw = old.turb_fw;
+        d.c.turb_numax = old.turb_numax;
+        d.c.turb_kmax = old.turb_kmax;
+        d.c.turb_tk = old.turb_tk;
+        ++consts_ver;
         if (ppe) ppe_update_params(ppe, P);
         return MPSR_NOERR;
     }
     mpsr_status set_ppe(const mpsr_ppe_options* o) override {
         if (o->coarse_cells < 0 || o->coarse_cells > 8)
-            return fail(MPSR_PARAMERR, "ppe.coarse_cells must be 0 (default 4) or 1..8");
+            return fail(MPSR_PARAMERR, "ppe.coarse_cells must be 0 (default 2) or 1..8");
         ppe_opt = *o;
         return MPSR_NOERR;
     }

@@ -230,19 +230,38 @@ __global__ __launch_bounds__(kMaxBlock) void k_relocate(int N, Consts<R> c, PSta
 // ---------------------------------------------------------------------------------------------
 // PND (and the EOS pressure when eos != 0).  spres[t] receives the slot's out-pressure after the
 // kernel, which is what the gradient kernel reads for neighbours (Kern :860).
-template <typename R>
+//
+// With PPE the same pass also gives the pressure Poisson rows their start: the row is active
+// (unknown pressure) when the particle is alive, binned, inside the zone and its PND exceeds
+// beta n0; over the Laplacian radius it counts the neighbours (an upper bound of the row's
+// entries: the Dirichlet ones are dropped when the row is filled) and sums their weights in FP64
+// for the diagonal.  A row that is not active or has no neighbour is a Dirichlet row (p = 0,
+// diagonal 1).  The PND sum sees the gradient-radius neighbours in the same order as without PPE,
+// so both give the same bits.
+template <typename R, bool PPE>
 __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, const int* __restrict__ perm, PState<R> in,
                                               PState<R> out, const R4<R>* __restrict__ spos, R* __restrict__ spres,
                                               const int* __restrict__ cs, const int* __restrict__ ce,
-                                              const R* __restrict__ stl_mag, const int* __restrict__ stl_id) {
+                                              const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
+                                              const uint32_t* __restrict__ skeys, int* __restrict__ active,
+                                              int* __restrict__ rcnt, double* __restrict__ rdiag, int* __restrict__ rdir) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
     const R4<R> P = spos[t];
     const R x = P.x, y = P.y, z = P.z;
+    auto dirichlet = [&]() {
+        if constexpr (PPE) {
+            active[t] = 0;
+            rcnt[t] = 0;
+            rdiag[t] = 1.0;
+            rdir[t] = 1;
+        }
+    };
     if (!include(c, x, y, z)) {
         in.p[s] = 0;
         spres[t] = out.p[s];
+        dirichlet();
         return;
     }
     int ci, cj, ck;
@@ -251,18 +270,40 @@ __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, 
     if (cid < 0 || cid >= c.ncell) {
         in.p[s] = 0;
         spres[t] = out.p[s];
+        dirichlet();
         return;
     }
     const R re = c.re_grad, mind = c.mind_grad;
     R dens = 0;
-    for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
-        const R4<R> Q = spos[q];
-        R d = dist3(x, y, z, Q.x, Q.y, Q.z);
-        if (d < re) {
-            if (d < mind) d = mind;
-            dens = dens + weight(re, d);
-        }
-    });
+    int nl = 0;
+    double wl = 0.0;
+    if constexpr (PPE) {
+        const R rl = c.re_lap, mindl = c.mind_lap;
+        for_candidates(c, cs, ce, x, y, z, t, [&](int q) {
+            const R4<R> Q = spos[q];
+            const R d0 = dist3(x, y, z, Q.x, Q.y, Q.z);
+            if (d0 < re) {
+                R d = d0;
+                if (d < mind) d = mind;
+                dens = dens + weight(re, d);
+            }
+            if (d0 < rl) {
+                R d = d0;
+                if (d < mindl) d = mindl;
+                wl += (double)weight(rl, d);
+                ++nl;
+            }
+        });
+    } else {
+        for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
+            const R4<R> Q = spos[q];
+            R d = dist3(x, y, z, Q.x, Q.y, Q.z);
+            if (d < re) {
+                if (d < mind) d = mind;
+                dens = dens + weight(re, d);
+            }
+        });
+    }
     const R riw = stl_mag[t];
     R ww = 0;
     const int tid = stl_id[t];
@@ -270,6 +311,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, 
         if (riw >= 0 && riw <= re) ww = ztable_eval(kZgrad, riw, c.l0);
     dens += ww;
     out.n[s] = dens;
+    if constexpr (PPE) {
+        const bool act = out.id[s] >= 0 && skeys[t] != (uint32_t)c.ncell && (double)dens > (double)c.beta_n0;
+        const bool dr = !(act && wl > 0.0);
+        active[t] = act ? 1 : 0;
+        rcnt[t] = dr ? 0 : nl;
+        rdiag[t] = dr ? 1.0 : wl;
+        rdir[t] = dr ? 1 : 0;
+    }
     R pout = out.p[s];
     if (eos) {
         R pr = 0;
@@ -557,10 +606,18 @@ template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL(k_relocate<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.in, d.out);
 }
+template <typename R>
+void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, hipStream_t st) {
+    if (d.N <= 0) return;
+    hipLaunchKernelGGL((k_pnd<R, true>), dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, 0, d.perm, d.in,
+                       d.out, d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id, d.skeys, active, cnt, diag,
+                       dir);
+}
 template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st) {
     if (d.N <= 0) return;
-    hipLaunchKernelGGL(k_pnd<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, eos, d.perm, d.in, d.out,
-                       d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id);
+    hipLaunchKernelGGL((k_pnd<R, false>), dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, eos, d.perm, d.in, d.out,
+                       d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id, nullptr, nullptr, nullptr,
+                       nullptr, nullptr);
 }
 template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
@@ -592,6 +649,7 @@ template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int
     template void launch_explicit<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_relocate<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_pnd<R>(const Dev<R>&, int, hipStream_t);                             \
+    template void launch_pnd_ppe<R>(const Dev<R>&, int*, int*, double*, int*, hipStream_t);  \
     template void launch_gradient<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_thermal<R>(const Dev<R>&, const int*, const int*, int, hipStream_t);   \
     template void launch_turbulence<R>(const Dev<R>&, hipStream_t);                           \

@@ -10,7 +10,10 @@ template <typename R>
 PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_options& opt, double n0_lap,
                          double lambda_lap, double n0_src);
 template <typename R> void ppe_destroy(PpeSolver<R>* s);
-// Assembles A p = b from the current binning and PND (out.n) and solves it; writes the
+// The PND pass of the PPE path (PND into out.n, and each Poisson row's active flag, neighbour
+// bound, diagonal and Dirichlet flag); ppe_solve assembles from its results.
+template <typename R> void ppe_start(PpeSolver<R>* s, const Dev<R>& d, hipStream_t st);
+// Assembles A p = b from the current binning and the ppe_start results and solves it; writes the
 // pressure to in.p / out.p and to the sorted pressure array used by the gradient kernel.
 template <typename R>
 mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, int profile, hipStream_t st);

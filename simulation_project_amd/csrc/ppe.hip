@@ -22,20 +22,29 @@
 // reference's symmetric (P_i + P_j) gradient, which would turn a suction pressure into an
 // attraction.
 //
-// Storage: SELL-64 -- slices of 64 consecutive rows (one wave), columns stored slice-major so
-// lane i of a wave reads entry k of row i at slice_off + 64 k + i: every matrix load is a
-// coalesced 256 B (int32 columns, FP32 values) or 512 B (FP64 values) wave access.  The diagonal
-// is a separate FP64 vector.  Rows keep the canonical neighbour order, so the assembled matrix
-// equals the CPU restatement's bit for bit.
+// Storage: SELL-64/4 -- slices of 64 row positions (one wave).  Inside every window of 256 rows
+// the rows take positions by decreasing entry bound (SELL-C-sigma, sigma = 256); entry k of
+// position P sits at slice_off[P/64] + 256 (k/4) + 4 (P%64) + k%4, so a lane loads four
+// consecutive entries in one 16 B access and a wave's group of four is 1 KB contiguous.  Every
+// reader stops at its row's entry count rounded up to four; only that last group is padded.  The
+// diagonal is a separate FP64 vector.  Rows keep the canonical neighbour order, so the assembled
+// matrix equals the CPU restatement's bit for bit.
+//
+// Assembly: the PND pass of the PPE path (k_pnd with PPE, particles.hip) gives every row its
+// active flag, a bound on its entries (all Laplacian-radius neighbours), its FP64 diagonal and its
+// Dirichlet flag; k_sell_order places the rows and sizes the slices from the bounds; k_ppe_fill
+// writes the entries (Dirichlet neighbours dropped), the real counts and the source.
 //
 // Solvers:
 //   FP64   Jacobi-preconditioned CG, FP64 values, vectors and dots.
 //   MIXED  iterative refinement: FP64 residual r = b - A x (FP32 values, FP64 accumulation),
-//          inner Jacobi-PCG with FP32 values, FP32 vectors and SpMV, FP64 dot products, then
-//          x += e in FP64.  Repeats until the FP64 residual meets the tolerance.
-// Each PCG iteration is four launches (SpMV+dot, alpha, update+dots, beta); every kernel reads
-// a device-side "done" flag and returns at once after convergence, so the host queues fixed
-// chunks of iterations and looks at the flag once per chunk.
+//          inner PCG with FP32 values, FP32 vectors and SpMV, FP64 dot products, then x += e in
+//          FP64.  Repeats until the FP64 residual meets the tolerance.
+//   Either with the Jacobi or the two-level preconditioner (aggregates of cells, dense coarse
+//   inverse).  A two-level iteration is seven launches (SpMV + p.q, alpha, restriction, coarse
+//   product, update + z + dots, beta, direction); every kernel reads a device-side "done" flag and
+//   returns at once after convergence, so the host queues fixed chunks of iterations and looks at
+//   the flag once per chunk.
 #include <cstdio>
 #include <cmath>
 #include <cstring>
@@ -82,6 +91,7 @@ template <typename R> struct PpeSolver {
     double* part = nullptr;      // [2 * nblocks]
     double* scal = nullptr;      // device scalars
     double* scal_h = nullptr;    // pinned host mirror
+    int* cnt_h = nullptr;        // pinned host mirror of nnz_dev
     int mixed = 0;
     SortScratch scan{};
     // last solve
@@ -159,59 +169,6 @@ template <int NW> __device__ inline double block_sum(double v, double* lds) {
 // ---------------------------------------------------------------------------------------------
 // assembly
 // ---------------------------------------------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(PB) void k_ppe_active(int N, Consts<R> c, const uint32_t* __restrict__ skeys,
-                                                    const int* __restrict__ perm, PState<R> out,
-                                                    const R4<R>* __restrict__ spos, int* active) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= N) return;
-    const int s = perm[t];
-    int a = 0;
-    if (out.id[s] >= 0 && skeys[t] != (uint32_t)c.ncell) {
-        const R4<R> P = spos[t];
-        if (include(c, P.x, P.y, P.z)) a = ((double)out.n[s] > (double)c.beta_n0) ? 1 : 0;
-    }
-    active[t] = a;
-}
-
-template <typename R>
-__global__ __launch_bounds__(PB) void k_ppe_count(int N, int nslices, Consts<R> c, const int* __restrict__ active,
-                                                   const R4<R>* __restrict__ spos, const int* __restrict__ cs,
-                                                   const int* __restrict__ ce, int* cnt, double* diag, int* dir,
-                                                   int* nnz) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int n = 0;
-    double sum = 0.0;
-    const bool act = t < N && active[t];
-    if (act) {
-        const R4<R> P = spos[t];
-        const R re = c.re_lap, mind = c.mind_lap;
-        for_candidates(c, cs, ce, P.x, P.y, P.z, t, [&](int q) {
-            const R4<R> Q = spos[q];
-            R d = dist3(P.x, P.y, P.z, Q.x, Q.y, Q.z);
-            if (d < re) {
-                if (d < mind) d = mind;
-                const R w = weight(re, d);
-                sum += (double)w;
-                if (active[q]) ++n;
-            }
-        });
-    }
-    const bool dr = !(act && sum > 0.0);
-    if (dr) n = 0;
-    if (t < N) {
-        cnt[t] = n;
-        diag[t] = dr ? 1.0 : sum;
-        dir[t] = dr ? 1 : 0;
-    }
-    const int tot = wave_sum(n);
-    const int rows = wave_sum((t < N && !dr) ? 1 : 0);
-    if (lane_id() == 0 && (t >> 6) < nslices) {
-        atomicAdd(&nnz[0], tot);
-        atomicAdd(&nnz[1], rows);
-    }
-}
-
 // SELL-C-sigma order: inside every window of SELL_SIGMA rows the rows take positions by stored
 // length (descending, ties by row), so the 64 rows of a slice have similar lengths and little
 // padding.  One workgroup per window sorts the words ((2^20 - 2 - length) << 12 | row in window)
@@ -269,7 +226,8 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
                                                   const int* __restrict__ cs, const int* __restrict__ ce,
                                                   const int* __restrict__ slice_off, const int* __restrict__ slice_w,
                                                   const int* __restrict__ perm, PState<R> out, double alpha, double n0,
-                                                  const int* __restrict__ rowp, int* cols, VA* vals, double* rhs) {
+                                                  const int* __restrict__ rowp, int* cols, VA* vals, double* rhs,
+                                                  int* cnt, int* nnz) {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     const int slice = P >> 6, lane = lane_id();
     if (slice >= nslices) return;
@@ -301,9 +259,21 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
             }
         });
     }
+    // the row's entries end at k (its Dirichlet neighbours dropped); readers stop at k rounded up
+    // to a group of 4, so only the last group is padded
+    const int real = k;
     const int self = t >= 0 ? t : 0;
-    while (k < width) push(self, (VA)0);
-    if (t >= 0) rhs[t] = dir[t] ? 0.0 : alpha * fmax((double)out.n[perm[t]] - n0, 0.0) / n0;
+    while (k & 3) push(self, (VA)0);
+    if (t >= 0) {
+        cnt[t] = real;
+        rhs[t] = dir[t] ? 0.0 : alpha * fmax((double)out.n[perm[t]] - n0, 0.0) / n0;
+    }
+    const int tot = wave_sum(real);
+    const int rows = wave_sum((t >= 0 && !dir[t]) ? 1 : 0);
+    if (lane == 0) {
+        atomicAdd(&nnz[0], tot);
+        atomicAdd(&nnz[1], rows);
+    }
     (void)N;
 }
 
@@ -759,18 +729,22 @@ __global__ __launch_bounds__(PB) void k_coarse_gather(int nc, AggGeom g, const i
     }
 }
 
-__global__ __launch_bounds__(PB) void k_symmetrize(int nc, double* A) {
+template <typename T> __global__ __launch_bounds__(PB) void k_symmetrize(int nc, T* A) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)nc * nc) return;
     const int j = (int)(idx / nc), i = (int)(idx % nc);   // element (i, j) at A[j * nc + i]
     if (i < j) A[(size_t)j * nc + i] = A[(size_t)i * nc + j];
 }
 
-// FP32 copy of the coarse inverse: entries (I, J) and (J, I) are equal, so they round alike and
-// the copy stays symmetric
+// FP32 copy of the assembled coarse operator, factorised and inverted in single precision
 __global__ __launch_bounds__(PB) void k_d2f(long long n, const double* __restrict__ a, float* b) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
+}
+// FP64 copy of the symmetric FP32 inverse, for the FP64 solver paths
+__global__ __launch_bounds__(PB) void k_f2d(long long n, const float* __restrict__ a, double* b) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
 }
 
 __global__ __launch_bounds__(PB) void k_fill_d(double* a, long long n, double v) {
@@ -933,6 +907,7 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     s->scan.scan_tmp_cap = (int)scan_tmp_ints(s->nslices + 1);
     ok &= dev_alloc(&s->scan.scan_tmp, s->scan.scan_tmp_cap);
     ok &= hipHostMalloc((void**)&s->scal_h, sizeof(double) * S_COUNT) == hipSuccess;
+    ok &= hipHostMalloc((void**)&s->cnt_h, sizeof(int) * 2) == hipSuccess;
     ok &= dev_alloc(&s->nnz_dev, 2);
     ok &= dev_alloc(&s->pdiag_buf, rows) && dev_alloc(&s->omega, rows) && dev_alloc(&s->xprev_s, rows) &&
           dev_alloc(&s->xl, rows);
@@ -948,7 +923,7 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     (void)hipMemset(s->scal, 0, sizeof(double) * S_COUNT);
     s->mixed = opt.solver == MPSR_PCG_MIXED;
     // two-level preconditioner geometry and buffers
-    s->agg_edge = opt.coarse_cells > 0 ? opt.coarse_cells : 4;
+    s->agg_edge = opt.coarse_cells > 0 ? opt.coarse_cells : 2;
     s->cgx = (d.c.gx + s->agg_edge - 1) / s->agg_edge;
     s->cgy = (d.c.gy + s->agg_edge - 1) / s->agg_edge;
     s->cgz = (d.c.gz + s->agg_edge - 1) / s->agg_edge;
@@ -965,7 +940,7 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     return s;
 }
 
-static constexpr int NC_MAX = 3072;   // largest coarse system factorised densely
+static constexpr int NC_MAX = 8192;   // largest coarse system factorised densely
 
 // Builds the aggregates of this solve's non-Dirichlet rows, Ac = P^T A P and its inverse.  When
 // the aggregates outnumber what is factorised densely, the aggregate edge grows by one cell at a
@@ -1038,15 +1013,18 @@ static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hi
                        s->cpart);
     hipLaunchKernelGGL(k_coarse_gather, dim3(grid_for((long long)nc * 27, PB)), dim3(PB), 0, st, nc, g, s->agg_full,
                        s->agg_flag, s->agg_map, s->cpart, s->Ac);
-    if (rocsolver_dpotrf(s->blas, rocblas_fill_lower, nc, s->Ac, nc, s->info_dev) != rocblas_status_success) return false;
-    if (rocsolver_dpotri(s->blas, rocblas_fill_lower, nc, s->Ac, nc, s->info_dev + 1) != rocblas_status_success)
+    // Cholesky factorisation and inverse in single precision: the inverse only preconditions, the
+    // coarse operator is a well-conditioned aggregated Laplacian, and FP32 halves the set-up
+    hipLaunchKernelGGL(k_d2f, dim3(grid_for(n2, PB)), dim3(PB), 0, st, n2, s->Ac, s->Af);
+    if (rocsolver_spotrf(s->blas, rocblas_fill_lower, nc, s->Af, nc, s->info_dev) != rocblas_status_success) return false;
+    if (rocsolver_spotri(s->blas, rocblas_fill_lower, nc, s->Af, nc, s->info_dev + 1) != rocblas_status_success)
         return false;
     int info[2] = {0, 0};
     if (hipMemcpyAsync(info, s->info_dev, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return false;
     if (hipStreamSynchronize(st) != hipSuccess) return false;
     if (info[0] != 0 || info[1] != 0) return false;
-    hipLaunchKernelGGL(k_symmetrize, dim3(grid_for(n2, PB)), dim3(PB), 0, st, nc, s->Ac);
-    hipLaunchKernelGGL(k_d2f, dim3(grid_for(n2, PB)), dim3(PB), 0, st, n2, s->Ac, s->Af);
+    hipLaunchKernelGGL(k_symmetrize<float>, dim3(grid_for(n2, PB)), dim3(PB), 0, st, nc, s->Af);
+    hipLaunchKernelGGL(k_f2d, dim3(grid_for(n2, PB)), dim3(PB), 0, st, n2, s->Af, s->Ac);
     return true;
 }
 
@@ -1134,6 +1112,7 @@ template <typename R> void ppe_destroy(PpeSolver<R>* s) {
     for (void* p : ps)
         if (p) (void)hipFree(p);
     if (s->scal_h) (void)hipHostFree(s->scal_h);
+    if (s->cnt_h) (void)hipHostFree(s->cnt_h);
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : s->setup_ev)
         if (e) (void)hipEventDestroy(e);
@@ -1261,6 +1240,10 @@ template <typename R> static MpnnGeo<R> mpnn_geo(const Dev<R>& d) {
     return g;
 }
 
+template <typename R> void ppe_start(PpeSolver<R>* s, const Dev<R>& d, hipStream_t st) {
+    launch_pnd_ppe(d, s->active, s->cnt, s->diag, s->dir, st);
+}
+
 template <typename R>
 mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, int profile, hipStream_t st) {
     const int N = d.N;
@@ -1279,27 +1262,18 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->mixed = opt.solver == MPSR_PCG_MIXED;
     s->status = 0;
     // ---- assembly ----
-    hipLaunchKernelGGL(k_ppe_active<R>, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.c, d.skeys, d.perm, d.out,
-                       d.spos, s->active);
+    // active flags, row bounds, diagonal and Dirichlet flags come from the PND pass (ppe_start)
     fill_int(s->nnz_dev, 0, 2, st);
-    hipLaunchKernelGGL(k_ppe_count<R>, dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, d.spos, d.cell_start,
-                       d.cell_end, s->cnt, s->diag, s->dir, s->nnz_dev);
     if (N > 0)
         hipLaunchKernelGGL(k_sell_order, dim3(grid_for(N, SELL_SIGMA)), dim3(SELL_BLOCK), 0, st, N, nsl, s->cnt, s->dir,
                            s->pos, s->rowp, s->slice_w);
     exclusive_scan(s->slice_w, s->slice_off, nullptr, nsl, nsl, s->total, s->scan, st);
     int total = 0;
-    int counts[2] = {0, 0};
     if (hipMemcpyAsync(&s->scal_h[0], s->total, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
-        return MPSR_GPUERR;
-    if (hipMemcpyAsync(&s->scal_h[1], s->nnz_dev, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
         return MPSR_GPUERR;
     if (hipStreamSynchronize(st) != hipSuccess) return MPSR_GPUERR;
     std::memcpy(&total, &s->scal_h[0], sizeof(int));
-    std::memcpy(counts, &s->scal_h[1], 2 * sizeof(int));
-    s->nnz_real = counts[0];
     s->nnz_stored = total;
-    s->rows_real = counts[1];
     if ((long long)total > s->nnz_cap) {
         if (s->cols) (void)hipFree(s->cols);
         if (s->vals64) (void)hipFree(s->vals64);
@@ -1316,11 +1290,14 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     if (s->mixed)
         hipLaunchKernelGGL((k_ppe_fill<R, float>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
-                           s->rowp, s->cols, s->vals32, s->rhs);
+                           s->rowp, s->cols, s->vals32, s->rhs, s->cnt, s->nnz_dev);
     else
         hipLaunchKernelGGL((k_ppe_fill<R, double>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
-                           s->rowp, s->cols, s->vals64, s->rhs);
+                           s->rowp, s->cols, s->vals64, s->rhs, s->cnt, s->nnz_dev);
+    // entry and row counts of the filled matrix, read after the first residual pass synchronises
+    if (hipMemcpyAsync(s->cnt_h, s->nnz_dev, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return MPSR_GPUERR;
     s->nslices = nsl;
     // ---- solve ----
     const double tol = opt.rel_tol > 0 ? opt.rel_tol : 1e-8;
@@ -1348,6 +1325,8 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     double bb = 0.0;
     double rr = resid_pass(s, N, st, &bb);
     if (rr < 0) return MPSR_GPUERR;
+    s->nnz_real = s->cnt_h[0];
+    s->rows_real = s->cnt_h[1];
     s->relres0 = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     const double target = tol * tol * bb;
     s->iters = 0;
@@ -1358,7 +1337,7 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->coarse_rebuilt = 0;
     s->setup_ms = 0.0;
     if (opt.precond == MPSR_PRECOND_TWO_LEVEL && bb > 0 && rr > target) {
-        const int every = opt.coarse_rebuild > 0 ? opt.coarse_rebuild : 10;
+        const int every = opt.coarse_rebuild > 0 ? opt.coarse_rebuild : 50;
         const bool rebuild = !s->coarse_valid || s->coarse_age + 1 >= every ||
                              (s->iters_at_build > 0 && 4 * s->iters_last > 5 * s->iters_at_build);
         if (rebuild) {
@@ -1688,6 +1667,7 @@ namespace mpsr {
     template PpeSolver<R>* ppe_create<R>(const Dev<R>&, const mpsr_params&, const mpsr_ppe_options&, double, double, \
                                          double);                                                                    \
     template void ppe_destroy<R>(PpeSolver<R>*);                                                                     \
+    template void ppe_start<R>(PpeSolver<R>*, const Dev<R>&, hipStream_t);                                          \
     template mpsr_status ppe_solve<R>(PpeSolver<R>*, Dev<R>&, const mpsr_ppe_options&, int, hipStream_t);            \
     template mpsr_status ppe_check<R>(PpeSolver<R>*, mpsr_timings*);                                                 \
     template const char* ppe_why<R>(PpeSolver<R>*);                                                                 \

@@ -139,3 +139,23 @@ def test_batched_graph_with_terms_switched_on_between_replays(lib):
         np.testing.assert_array_equal(h.get_turbulence()[0], s.get_turbulence()[0])
     assert max(h.get_turbulence()[0].max() for h in b.buckets) > 0
     b.close()
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_dt_change_keeps_spheres_and_terms(lib, precision):
+    """mpsr_set_dt (CheckAndUpdateDt, Src MultiGPUHandler.cpp:1048) between steps recomputes the
+    per-launch constants; the spheres' reaction and both radix terms stay switched on."""
+    case = small_drum(lib)
+    sph = cases.drum_spheres(case, 8, diameter_l0=2.0)
+    assert len(sph.position) == 8
+    l0 = case.params.InitialParticleDistance
+    print(f"[radix] dt change n={case.n} precision={precision}", flush=True)
+    g, o = Solver(case, precision=precision), Oracle(case, precision=precision)
+    for s in (g, o):
+        s.set_spheres(sph)
+        s.set_radix_terms(turb_terms(l0, bNegativePressure=1), [NEG_MODEL])
+        s.step_many(0, 3)
+        s.set_dt(float(np.float32(case.params.Dt * 0.5)), float(np.float32(0.15)))
+        s.step_many(3, 4)
+    same_state(g, o, "after the dt change")
+    np.testing.assert_array_equal(g.get_spheres()[0], o.get_spheres()[0])
