@@ -326,9 +326,11 @@ template <typename R> void launch_explicit(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st);
 // PND on the PPE path, with each Poisson row's active flag, neighbour bound, diagonal and
-// Dirichlet flag from the same neighbour pass
+// Dirichlet flag from the same neighbour pass, and its first nb_width Laplacian-radius neighbours
+// and weights (SELL-64/4 by sorted index, row width nb_width)
 template <typename R>
-void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, hipStream_t st);
+void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, int* nbq, R* nbw, int nb_width,
+                    hipStream_t st);
 template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st);
 // CalcTemperatureFactor; with step_ctr the launch returns at once unless (*step_base + *step_ctr)
 // is a multiple of thermal_step (graph replays), else the caller decides

@@ -237,14 +237,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_relocate(int N, Consts<R> c, PSta
 // entries: the Dirichlet ones are dropped when the row is filled) and sums their weights in FP64
 // for the diagonal.  A row that is not active or has no neighbour is a Dirichlet row (p = 0,
 // diagonal 1).  The PND sum sees the gradient-radius neighbours in the same order as without PPE,
-// so both give the same bits.
+// so both give the same bits.  The first nbw_width Laplacian-radius neighbours and their weights go
+// to a per-row list (SELL-64/4 by sorted index) that the fill reads instead of scanning again.
 template <typename R, bool PPE>
 __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, const int* __restrict__ perm, PState<R> in,
                                               PState<R> out, const R4<R>* __restrict__ spos, R* __restrict__ spres,
                                               const int* __restrict__ cs, const int* __restrict__ ce,
                                               const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
                                               const uint32_t* __restrict__ skeys, int* __restrict__ active,
-                                              int* __restrict__ rcnt, double* __restrict__ rdiag, int* __restrict__ rdir) {
+                                              int* __restrict__ rcnt, double* __restrict__ rdiag, int* __restrict__ rdir,
+                                              int* __restrict__ nbq, R* __restrict__ nbw, int nb_width) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
@@ -279,6 +281,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, 
     double wl = 0.0;
     if constexpr (PPE) {
         const R rl = c.re_lap, mindl = c.mind_lap;
+        const int lbase = (t >> 6) * 64 * nb_width, lane = t & 63;
+        int q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+        R w0 = 0, w1 = 0, w2 = 0, w3 = 0;
         for_candidates(c, cs, ce, x, y, z, t, [&](int q) {
             const R4<R> Q = spos[q];
             const R d0 = dist3(x, y, z, Q.x, Q.y, Q.z);
@@ -290,10 +295,32 @@ __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, 
             if (d0 < rl) {
                 R d = d0;
                 if (d < mindl) d = mindl;
-                wl += (double)weight(rl, d);
+                const R w = weight(rl, d);
+                wl += (double)w;
+                if (nl < nb_width) {
+                    q0 = q1; q1 = q2; q2 = q3; q3 = q;
+                    w0 = w1; w1 = w2; w2 = w3; w3 = w;
+                    if ((nl & 3) == 3) {
+                        const int e = sell_at(lbase, lane, nl - 3);
+                        quad_store(nbq + e, q0, q1, q2, q3);
+                        quad_store(nbw + e, w0, w1, w2, w3);
+                    }
+                }
                 ++nl;
             }
         });
+        const int kept = nl < nb_width ? nl : nb_width;
+        if (kept & 3) {   // the last, partial group of the list
+            const int used = kept & 3;
+            int qq[4] = {q3, q3, q3, q3};
+            R ww[4] = {w3, w3, w3, w3};
+            if (used == 1) { qq[0] = q3; ww[0] = w3; }
+            if (used == 2) { qq[0] = q2; ww[0] = w2; qq[1] = q3; ww[1] = w3; }
+            if (used == 3) { qq[0] = q1; ww[0] = w1; qq[1] = q2; ww[1] = w2; qq[2] = q3; ww[2] = w3; }
+            const int e = sell_at(lbase, lane, kept - used);
+            quad_store(nbq + e, qq[0], qq[1], qq[2], qq[3]);
+            quad_store(nbw + e, ww[0], ww[1], ww[2], ww[3]);
+        }
     } else {
         for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
             const R4<R> Q = spos[q];
@@ -607,17 +634,18 @@ template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st) {
     hipLaunchKernelGGL(k_relocate<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.in, d.out);
 }
 template <typename R>
-void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, hipStream_t st) {
+void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, int* nbq, R* nbw, int nb_width,
+                    hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL((k_pnd<R, true>), dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, 0, d.perm, d.in,
                        d.out, d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id, d.skeys, active, cnt, diag,
-                       dir);
+                       dir, nbq, nbw, nb_width);
 }
 template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL((k_pnd<R, false>), dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, eos, d.perm, d.in, d.out,
                        d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id, nullptr, nullptr, nullptr,
-                       nullptr, nullptr);
+                       nullptr, nullptr, nullptr, nullptr, 0);
 }
 template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
@@ -649,7 +677,7 @@ template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int
     template void launch_explicit<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_relocate<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_pnd<R>(const Dev<R>&, int, hipStream_t);                             \
-    template void launch_pnd_ppe<R>(const Dev<R>&, int*, int*, double*, int*, hipStream_t);  \
+    template void launch_pnd_ppe<R>(const Dev<R>&, int*, int*, double*, int*, int*, R*, int, hipStream_t); \
     template void launch_gradient<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_thermal<R>(const Dev<R>&, const int*, const int*, int, hipStream_t);   \
     template void launch_turbulence<R>(const Dev<R>&, hipStream_t);                           \

@@ -63,6 +63,7 @@ namespace mpsr {
 static constexpr int PB = 256;          // block of the vector kernels (4 waves = 4 slices)
 static constexpr int CHUNK = 32;        // PCG iterations queued between two flag reads
 static constexpr int SELL_SIGMA = 256;  // rows ordered by stored length inside windows of this many
+static constexpr int NB_WIDTH = 192;    // Laplacian-radius neighbours listed per row by the PND pass
 enum Scal { S_RZ, S_PQ, S_ALPHA, S_BETA, S_RR, S_TARGET, S_DONE, S_ITERS, S_MAXIT, S_BB, S_RR0, S_COUNT };
 
 template <typename R> struct PpeSolver {
@@ -72,6 +73,8 @@ template <typename R> struct PpeSolver {
     int* active = nullptr;
     int* dir = nullptr;
     int* cnt = nullptr;
+    int* nbq = nullptr;          // [rows * NB_WIDTH] neighbour list of the PND pass (sorted index)
+    R* nbw = nullptr;            // ... and the weights
     int* slice_w = nullptr;
     int* pos = nullptr;          // [positions] SELL position of each row (rows ordered by length per window)
     int* rowp = nullptr;         // [positions] row at each SELL position, -1 when empty
@@ -227,7 +230,8 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
                                                   const int* __restrict__ slice_off, const int* __restrict__ slice_w,
                                                   const int* __restrict__ perm, PState<R> out, double alpha, double n0,
                                                   const int* __restrict__ rowp, int* cols, VA* vals, double* rhs,
-                                                  int* cnt, int* nnz) {
+                                                  int* cnt, int* nnz, const int* __restrict__ nbq,
+                                                  const R* __restrict__ nbw, int nb_width) {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     const int slice = P >> 6, lane = lane_id();
     if (slice >= nslices) return;
@@ -247,17 +251,33 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
         }
     };
     if (t >= 0 && !dir[t]) {
-        const R4<R> X = spos[t];
-        const R re = c.re_lap, mind = c.mind_lap;
-        for_candidates(c, cs, ce, X.x, X.y, X.z, t, [&](int q) {
-            const R4<R> Q = spos[q];
-            R d = dist3(X.x, X.y, X.z, Q.x, Q.y, Q.z);
-            if (d < re) {
-                if (d < mind) d = mind;
-                const R w = weight(re, d);
-                if (active[q] && k < width) push(q, (VA)(-(double)w));
+        const int bound = cnt[t];   // Laplacian-radius neighbours, from the PND pass
+        if (bound <= nb_width) {
+            // the PND pass listed them with their weights, in the canonical order
+            const int lbase = (t >> 6) * 64 * nb_width, ll = t & 63;
+            for (int j = 0; j < bound; j += 4) {
+                int q[4];
+                R w[4];
+                const int e = sell_at(lbase, ll, j);
+                quad_load(nbq + e, q);
+                quad_load(nbw + e, w);
+                const int m = bound - j < 4 ? bound - j : 4;
+                for (int u = 0; u < m; ++u)
+                    if (active[q[u]] && k < width) push(q[u], (VA)(-(double)w[u]));
             }
-        });
+        } else {
+            const R4<R> X = spos[t];
+            const R re = c.re_lap, mind = c.mind_lap;
+            for_candidates(c, cs, ce, X.x, X.y, X.z, t, [&](int q) {
+                const R4<R> Q = spos[q];
+                R d = dist3(X.x, X.y, X.z, Q.x, Q.y, Q.z);
+                if (d < re) {
+                    if (d < mind) d = mind;
+                    const R w = weight(re, d);
+                    if (active[q] && k < width) push(q, (VA)(-(double)w));
+                }
+            });
+        }
     }
     // the row's entries end at k (its Dirichlet neighbours dropped); readers stop at k rounded up
     // to a group of 4, so only the last group is padded
@@ -895,6 +915,7 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     const int nb = (rows + PB - 1) / PB;
     bool ok = true;
     ok &= dev_alloc(&s->active, rows) && dev_alloc(&s->dir, rows) && dev_alloc(&s->cnt, rows);
+    ok &= dev_alloc(&s->nbq, (size_t)rows * NB_WIDTH) && dev_alloc(&s->nbw, (size_t)rows * NB_WIDTH);
     ok &= dev_alloc(&s->slice_w, s->nslices + 1) && dev_alloc(&s->slice_off, s->nslices + 1) && dev_alloc(&s->total, 1);
     const int positions = ((d.cap + SELL_SIGMA - 1) / SELL_SIGMA) * SELL_SIGMA;
     ok &= dev_alloc(&s->pos, positions) && dev_alloc(&s->rowp, positions);
@@ -1065,8 +1086,8 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
         const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
-            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt, s->slice_off, s->slice_w, s->cols,
-                               vals, s->diag, p, q, s->scal, s->part);
+            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
+                               s->slice_off, s->slice_w, s->cols, vals, s->diag, p, q, s->scal, s->part);
             if (timed) {
                 (void)hipEventRecord(s->ev[s->ev_used + 1], st);
                 s->ev_used += 2;
@@ -1107,7 +1128,7 @@ template <typename R> void ppe_destroy(PpeSolver<R>* s) {
                   s->diag, s->rhs, s->x, s->prev, s->r, s->z, s->p0, s->p1, s->q, s->rf, s->zf, s->p0f,
                   s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp, s->nnz_dev, s->agg_flag,
                   s->agg_map, s->agg_full, s->agg_of_row, s->nc_dev, s->info_dev, s->Ac, s->Af, s->agg_rng, s->cpart, s->rxyz, s->rc, s->cc,
-                  s->agg_scan.scan_tmp, s->pdiag_buf, s->omega, s->xprev_s, s->xl};
+                  s->agg_scan.scan_tmp, s->pdiag_buf, s->omega, s->xprev_s, s->xl, s->nbq, s->nbw};
     if (s->net) mpnn_destroy(s->net);
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -1161,8 +1182,8 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
         const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
-            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt, s->slice_off, s->slice_w, s->cols,
-                               vals, s->diag, pa, q, s->scal, s->part);
+            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
+                               s->slice_off, s->slice_w, s->cols, vals, s->diag, pa, q, s->scal, s->part);
             if (timed) {
                 (void)hipEventRecord(s->ev[s->ev_used + 1], st);
                 s->ev_used += 2;
@@ -1241,7 +1262,7 @@ template <typename R> static MpnnGeo<R> mpnn_geo(const Dev<R>& d) {
 }
 
 template <typename R> void ppe_start(PpeSolver<R>* s, const Dev<R>& d, hipStream_t st) {
-    launch_pnd_ppe(d, s->active, s->cnt, s->diag, s->dir, st);
+    launch_pnd_ppe(d, s->active, s->cnt, s->diag, s->dir, s->nbq, s->nbw, NB_WIDTH, st);
 }
 
 template <typename R>
@@ -1290,11 +1311,11 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     if (s->mixed)
         hipLaunchKernelGGL((k_ppe_fill<R, float>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
-                           s->rowp, s->cols, s->vals32, s->rhs, s->cnt, s->nnz_dev);
+                           s->rowp, s->cols, s->vals32, s->rhs, s->cnt, s->nnz_dev, s->nbq, s->nbw, NB_WIDTH);
     else
         hipLaunchKernelGGL((k_ppe_fill<R, double>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
-                           s->rowp, s->cols, s->vals64, s->rhs, s->cnt, s->nnz_dev);
+                           s->rowp, s->cols, s->vals64, s->rhs, s->cnt, s->nnz_dev, s->nbq, s->nbw, NB_WIDTH);
     // entry and row counts of the filled matrix, read after the first residual pass synchronises
     if (hipMemcpyAsync(s->cnt_h, s->nnz_dev, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
         return MPSR_GPUERR;
