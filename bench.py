@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per SpMV launch of the mixed-precision solver on the default drum (l0 = 2.012e-3 m),
 # from two rocprofv3 counter passes (FETCH_SIZE scaled by the factor a plain copy kernel of known
 # size reads, WRITE_SIZE unscaled): profiles/k_pcg_spmv_drum1M_traffic.md
-SPMV_TRAFFIC_BYTES = {(2.012e-3, "mixed"): 951.1e6}
+SPMV_TRAFFIC_BYTES = {(2.012e-3, "mixed"): 958.4e6}
 WEIGHTS = os.path.join(ROOT, "simulation_project_amd", "weights", "mpnn_drum.npy")
 
 
