@@ -62,6 +62,8 @@ namespace mpsr {
 
 static constexpr int PB = 256;          // block of the vector kernels (4 waves = 4 slices)
 static constexpr int CHUNK = 32;        // PCG iterations queued between two flag reads
+static constexpr int CHUNK_TAIL = 8;    // ... after a first chunk sized from the previous solve
+static constexpr int HINT_PASSES = 4;   // refinement passes whose iteration counts are remembered
 static constexpr int SELL_SIGMA = 256;  // rows ordered by stored length inside windows of this many
 static constexpr int NB_WIDTH = 192;    // Laplacian-radius neighbours listed per row by the PND pass
 enum Scal { S_RZ, S_PQ, S_ALPHA, S_BETA, S_RR, S_TARGET, S_DONE, S_ITERS, S_MAXIT, S_BB, S_RR0, S_COUNT };
@@ -73,6 +75,7 @@ template <typename R> struct PpeSolver {
     int* active = nullptr;
     int* dir = nullptr;
     int* cnt = nullptr;
+    int hint[HINT_PASSES] = {};  // iterations of each refinement pass of the last solve (0: none)
     int* nbq = nullptr;          // [rows * NB_WIDTH] neighbour list of the PND pass (sorted index)
     R* nbw = nullptr;            // ... and the weights
     int* slice_w = nullptr;
@@ -1051,9 +1054,19 @@ static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hi
 
 // PCG on A e = r0 (V = FP32, e from 0) or A x = b (V = FP64, x = warm start, r0 = b - A x) with
 // the two-level preconditioner.  Returns iterations or -1 on breakdown.
+// Iterations queued before the next flag read.  Every launch queued after convergence returns at
+// once but still costs its dispatch, so with a hint (the iterations the same pass took in the last
+// solve) the first chunk stops two short of it and the rest come in chunks of CHUNK_TAIL.
+static int chunk_len(int it, int hint, int probe) {
+    if (probe > 0 && it == 0) return std::min(probe, CHUNK);
+    if (hint <= 0) return CHUNK;
+    if (it == 0) return std::max(CHUNK_TAIL, hint - 2);
+    return CHUNK_TAIL;
+}
+
 template <typename R, typename VA, typename V>
 static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* x, V* r, V* z, V* p, V* q,
-                    const double* r0, double target, int maxit, int zero_x, hipStream_t st) {
+                    const double* r0, double target, int maxit, int zero_x, hipStream_t st, int hint = 0) {
     const int rows = s->nslices * 64;
     const int nb = (rows + PB - 1) / PB;
     const int nc = s->nc;
@@ -1081,9 +1094,9 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
     const int probe = s->probe_at;
     s->probe_at = 0;
     for (int it = 0; it < maxit;) {
-        const int len = (probe > 0 && it == 0) ? std::min(probe, CHUNK) : CHUNK;
+        const int len = chunk_len(it, hint, probe);
         const size_t chunk_ev0 = s->ev_used;
-        const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
+        const bool timed = s->profile && s->ev_used + 2 * (size_t)len <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
             hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
@@ -1156,7 +1169,7 @@ template <typename R> static hipError_t read_scal(PpeSolver<R>* s, hipStream_t s
 // until r.r <= target.  Returns iterations, or -1 on breakdown.
 template <typename R, typename VA, typename V>
 static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* pa, V* pb, V* q,
-                   const double* r0, double target, int maxit, hipStream_t st) {
+                   const double* r0, double target, int maxit, hipStream_t st, int hint = 0) {
     const int rows = s->nslices * 64;
     const int nb = (rows + PB - 1) / PB;
     s->scal_h[S_TARGET] = target;
@@ -1177,9 +1190,9 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
         // events of launches that found the solve already converged (and returned at once) are
         // dropped, so the mean covers working launches only.  With a probe, the first chunk is
         // `probe` iterations long and r.r is checked after it.
-        const int len = (probe > 0 && it == 0) ? std::min(probe, CHUNK) : CHUNK;
+        const int len = chunk_len(it, hint, probe);
         const size_t chunk_ev0 = s->ev_used;
-        const bool timed = s->profile && s->ev_used + 2 * CHUNK <= s->ev.size();
+        const bool timed = s->profile && s->ev_used + 2 * (size_t)len <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
             hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
@@ -1433,9 +1446,9 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
             // FP64 PCG from the warm start; r already holds b - A x
             auto pcg = [&]() {
                 return two ? run_pcg2<R, double, double>(s, d, N, s->vals64, s->x, s->r, s->z, s->p0, s->q, s->r,
-                                                         target, maxit, 0, st)
+                                                         target, maxit, 0, st, s->hint[0])
                            : run_pcg<R, double, double>(s, N, s->vals64, s->x, s->r, s->z, s->p0, s->p1, s->q, s->r,
-                                                        target, maxit, st);
+                                                        target, maxit, st, s->hint[0]);
             };
             arm_probe();
             int it = pcg();
@@ -1445,6 +1458,7 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
             }
             if (it < 0) s->status = 1;
             else s->iters += it;
+            s->hint[0] = it > 0 ? it : 0;
             s->passes = 1;
             rr = resid_pass(s, N, st, &bb);
             if (rr < 0) return MPSR_GPUERR;
@@ -1455,11 +1469,13 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
         arm_probe();
         while (bb > 0 && rr > target && s->passes < 16 && s->iters < maxit) {
             const double inner_rel = std::max(1e-6, 0.5 * tol * std::sqrt(bb / rr));
+            const int hp = std::min(s->passes, HINT_PASSES - 1);
             const int it =
                 two ? run_pcg2<R, float, float>(s, d, N, s->vals32, s->ef, s->rf, s->zf, s->p0f, s->qf, s->r,
-                                                inner_rel * inner_rel * rr, maxit - s->iters, 1, st)
+                                                inner_rel * inner_rel * rr, maxit - s->iters, 1, st, s->hint[hp])
                     : run_pcg<R, float, float>(s, N, s->vals32, s->ef, s->rf, s->zf, s->p0f, s->p1f, s->qf, s->r,
-                                               inner_rel * inner_rel * rr, maxit - s->iters, st);
+                                               inner_rel * inner_rel * rr, maxit - s->iters, st, s->hint[hp]);
+            if (it >= 0) s->hint[hp] = it;
             if (it == -2) {
                 if (!fall_back()) return MPSR_GPUERR;
                 continue;
