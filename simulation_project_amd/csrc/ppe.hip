@@ -62,7 +62,7 @@ namespace mpsr {
 
 static constexpr int PB = 256;          // block of the vector kernels (4 waves = 4 slices)
 static constexpr int CHUNK = 32;        // PCG iterations queued between two flag reads
-static constexpr int CHUNK_TAIL = 8;    // ... after a first chunk sized from the previous solve
+static constexpr int CHUNK_TAIL = 4;    // ... after a first chunk sized from the previous solve
 static constexpr int HINT_PASSES = 4;   // refinement passes whose iteration counts are remembered
 static constexpr int SELL_SIGMA = 256;  // rows ordered by stored length inside windows of this many
 static constexpr int NB_WIDTH = 192;    // Laplacian-radius neighbours listed per row by the PND pass
@@ -1056,11 +1056,11 @@ static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hi
 // the two-level preconditioner.  Returns iterations or -1 on breakdown.
 // Iterations queued before the next flag read.  Every launch queued after convergence returns at
 // once but still costs its dispatch, so with a hint (the iterations the same pass took in the last
-// solve) the first chunk stops two short of it and the rest come in chunks of CHUNK_TAIL.
+// solve) the first chunk stops one short of it and the rest come in chunks of CHUNK_TAIL.
 static int chunk_len(int it, int hint, int probe) {
     if (probe > 0 && it == 0) return std::min(probe, CHUNK);
     if (hint <= 0) return CHUNK;
-    if (it == 0) return std::max(CHUNK_TAIL, hint - 2);
+    if (it == 0) return std::max(CHUNK_TAIL, hint - 1);
     return CHUNK_TAIL;
 }
 
