@@ -594,9 +594,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_turbulence(int N, Consts<R> c, co
             const R ux = V1.x - V0.x, uy = V1.y - V0.y, uz = V1.z - V0.z;
             const R rx = Q.x - x, ry = Q.y - y, rz = Q.z - z;
             const R dd = d * d;
-            txx = txx + ux * rx * w / dd; txy = txy + ux * ry * w / dd; txz = txz + ux * rz * w / dd;
-            tyx = tyx + uy * rx * w / dd; tyy = tyy + uy * ry * w / dd; tyz = tyz + uy * rz * w / dd;
-            tzx = tzx + uz * rx * w / dd; tzy = tzy + uz * ry * w / dd; tzz = tzz + uz * rz * w / dd;
+            if constexpr (sizeof(R) == 4) {
+                // nine quotients by one divisor: n / dd == (float)((double)n * RN(1 / (double)dd)) bit
+                // for bit.  The double product is within 2^-51 of n / dd, and a quotient of two
+                // 24-bit floats that is not a float midpoint (none is) lies more than 2^-48 of its
+                // size from every midpoint, so both round to the same float.
+                const double yd = 1.0 / (double)dd;
+                auto qt = [&](R n) { return (R)((double)n * yd); };
+                txx = txx + qt(ux * rx * w); txy = txy + qt(ux * ry * w); txz = txz + qt(ux * rz * w);
+                tyx = tyx + qt(uy * rx * w); tyy = tyy + qt(uy * ry * w); tyz = tyz + qt(uy * rz * w);
+                tzx = tzx + qt(uz * rx * w); tzy = tzy + qt(uz * ry * w); tzz = tzz + qt(uz * rz * w);
+            } else {
+                txx = txx + ux * rx * w / dd; txy = txy + ux * ry * w / dd; txz = txz + ux * rz * w / dd;
+                tyx = tyx + uy * rx * w / dd; tyy = tyy + uy * ry * w / dd; tyz = tyz + uy * rz * w / dd;
+                tzx = tzx + uz * rx * w / dd; tzy = tzy + uz * ry * w / dd; tzz = tzz + uz * rz * w / dd;
+            }
         }
     });
     const R g = c.turb_g;

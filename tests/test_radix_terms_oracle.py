@@ -289,3 +289,21 @@ def test_full_step_runs_turbulence_after_the_temperature(lib):
     pa, pb, pd = (o.get_particles(0) for o in (a, b, d))
     np.testing.assert_array_equal(pa.velocity, pb.velocity)
     assert np.abs(pd.velocity - pa.velocity).max() > 0
+
+
+def test_reciprocal_product_equals_float_division():
+    """The FP32 turbulence kernel divides its nine tensor terms by one r^2 as
+    (float)((double)n * (1 / (double)r^2)), relying on that product rounding to the IEEE float
+    quotient n / r^2 (the double product is within 2^-51 of the quotient, and no quotient of two
+    floats lies within 2^-48 of its size of a float midpoint).  Checked on random and on
+    all-ones-significand divisors."""
+    rng = np.random.default_rng(7)
+    n = 2_000_000
+    a = (rng.standard_normal(n) * np.exp(rng.uniform(-30, 30, n))).astype(np.float32)
+    b = np.exp(rng.uniform(-30, 30, n)).astype(np.float32)
+    ones = (np.float32(2.0) - np.float32(2.0 ** -23) * rng.integers(1, 4, n).astype(np.float32))
+    b2 = ones * (np.float32(2.0) ** rng.integers(-20, 20, n).astype(np.float32))
+    for d in (b, b2):
+        q = a / d
+        q2 = (a.astype(np.float64) * (1.0 / d.astype(np.float64))).astype(np.float32)
+        np.testing.assert_array_equal(q, q2)
