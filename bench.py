@@ -52,7 +52,7 @@ def log(msg):
 
 def ppe_options(abi, kind, precond, warm):
     solver = abi.MPSR_PCG_FP64 if kind == "fp64" else abi.MPSR_PCG_MIXED
-    pre = abi.MPSR_PRECOND_TWO_LEVEL if precond == "two" else abi.MPSR_PRECOND_JACOBI
+    pre = {"vcycle": abi.MPSR_PRECOND_CELL_VCYCLE, "two": abi.MPSR_PRECOND_TWO_LEVEL}.get(precond, abi.MPSR_PRECOND_JACOBI)
     return abi.PpeOptions(solver, 1e-8, 20000, warm, 0.0, 0.0, pre, 2, 50)
 
 
@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--l0", type=float, default=2.012e-3)
     ap.add_argument("--solver", choices=["learned", "mixed", "fp64"], default=None)
-    ap.add_argument("--precond", choices=["two", "jacobi"], default="two")
+    ap.add_argument("--precond", choices=["vcycle", "two", "jacobi"], default="vcycle")
     ap.add_argument("--spheres", type=int, default=2000, help="rigid spheres of diameter 5 l0, density 2500")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 PCG comparison run")

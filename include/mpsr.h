@@ -212,8 +212,11 @@ typedef enum mpsr_warm_start {
 } mpsr_warm_start;
 
 typedef enum mpsr_precond {
-    MPSR_PRECOND_JACOBI = 0,   /* z = D^-1 r */
-    MPSR_PRECOND_TWO_LEVEL = 1 /* z = D^-1 r + P Ac^-1 P^T r, P = aggregates of cells */
+    MPSR_PRECOND_JACOBI = 0,     /* z = D^-1 r */
+    MPSR_PRECOND_TWO_LEVEL = 1,  /* z = D^-1 r + P Ac^-1 P^T r, P = aggregates of cells */
+    MPSR_PRECOND_CELL_VCYCLE = 2 /* z = D^-1 r + P1 V(P1^T r): P1 = cells, V = one symmetric V-cycle on
+                                    P1^T A P1 (weighted Jacobi, the aggregate correction of
+                                    TWO_LEVEL, weighted Jacobi) */
 } mpsr_precond;
 
 typedef struct mpsr_ppe_options {

@@ -109,7 +109,7 @@ class ParticlesC(C.Structure):
                 ("solid_phase_rate", _dp), ("type", _ip)]
 
 
-MPSR_PRECOND_JACOBI, MPSR_PRECOND_TWO_LEVEL = 0, 1
+MPSR_PRECOND_JACOBI, MPSR_PRECOND_TWO_LEVEL, MPSR_PRECOND_CELL_VCYCLE = 0, 1, 2
 
 
 class PpeOptions(C.Structure):

@@ -120,6 +120,18 @@ template <typename R> struct PpeSolver {
     int* agg_map = nullptr;      // [nfull] compact index (valid where flagged)
     int* agg_full = nullptr;     // [nfull] compact -> full
     int* agg_of_row = nullptr;   // [rows] compact aggregate of a non-Dirichlet row, else -1
+    // cell level of the V-cycle preconditioner (MPSR_PRECOND_CELL_VCYCLE), over the grid cells
+    int vcycle = 0;              // this solve uses it
+    int ncell = 0;
+    int cell_age = -1;           // solves since Ac1 was built (-1: never)
+    int* cxyz = nullptr;         // [rows] packed cell coordinates of every row
+    int* cell_of_row = nullptr;  // [rows] cell of a non-Dirichlet row, else -1
+    int* c_agg = nullptr;        // [ncell] compact aggregate of the cell, else -1
+    float* c_op = nullptr;       // [ncell * 27] Ac1 = P1^T A P1 as a 27-point stencil on the cell grid
+    float* c_dinv = nullptr;     // [ncell] omega / Ac1 diagonal (0 for a cell without rows)
+    double* c_rc = nullptr;      // [ncell] restricted residual
+    double* c_e = nullptr;       // [ncell] pre-smoothed correction
+    double* c_ef = nullptr;      // [ncell] correction after the coarse correction and post-smoothing
     int* nc_dev = nullptr;
     int nc = 0, nc_cap = 0;
     double* Ac = nullptr;        // [nc_cap^2] coarse matrix, then its inverse (column-major)
@@ -838,6 +850,164 @@ __global__ __launch_bounds__(64) void k_restrict(int nc, int stacks, const int2*
     if (lane == 0) rc[I] = acc;
 }
 
+// ---- cell level of the V-cycle preconditioner -------------------------------------------------
+// z = D^-1 r + P1 V(P1^T r): P1 sums the non-Dirichlet rows of each grid cell (the rows of a cell
+// are one sorted range), and V is one symmetric V-cycle on Ac1 = P1^T A P1: weighted Jacobi
+// (omega / diag Ac1), the dense coarse correction on the aggregates of cells, weighted Jacobi
+// again.  Ac1 couples a cell to its 26 neighbours only (the Laplacian radius is one cell), so it
+// is kept as a 27-point stencil.  Every step is linear and symmetric, and the pre- and
+// post-smoothers are the same, so the preconditioner is SPD.
+static constexpr double VC_OMEGA = 0.7;
+static constexpr int CELL_REBUILD = 10;   // Ac1 is rebuilt every this many solves and with the aggregates
+
+__device__ inline int stencil_cell(const AggGeom& g, int ci, int cj, int ck, int b) {
+    const int x = ci + b / 9 - 1, y = cj + (b / 3) % 3 - 1, z = ck + b % 3 - 1;
+    if (x < 0 || y < 0 || z < 0 || x >= g.gx || y >= g.gy || z >= g.gz) return -1;
+    return (x * g.gy + y) * g.gz + z;
+}
+
+// per row: packed cell coordinates; the cell of a non-Dirichlet row
+__global__ __launch_bounds__(PB) void k_row_cell(int N, AggGeom g, const uint32_t* __restrict__ skeys,
+                                                  const int* __restrict__ dir, int* cxyz, int* cell_of_row) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int key = (int)skeys[t];
+    const int ci = key / (g.gy * g.gz), cj = (key / g.gz) % g.gy, ck = key % g.gz;
+    cxyz[t] = (ci << 20) | (cj << 10) | ck;
+    cell_of_row[t] = dir[t] ? -1 : key;
+}
+// per cell: its compact aggregate in the (possibly frozen) map
+__global__ __launch_bounds__(PB) void k_cell_agg_map(int ncell, AggGeom g, const int* __restrict__ flag,
+                                                      const int* __restrict__ map, int* c_agg) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= ncell) return;
+    const int f = agg_full_of_cell(g, I);
+    c_agg[I] = flag[f] ? map[f] : -1;
+}
+// Ac1 row of cell I: one wave, lanes over the cell's rows, each lane summing its rows' entries per
+// neighbour slot in its own LDS column; the slots are then summed over the lanes in lane order
+template <typename VA>
+__global__ __launch_bounds__(64) void k_cell_op(int ncell, AggGeom g, const int* __restrict__ cs,
+                                                 const int* __restrict__ ce, const int* __restrict__ dir,
+                                                 const int* __restrict__ cnt, const int* __restrict__ pos,
+                                                 const int* __restrict__ slice_off, const int* __restrict__ cols,
+                                                 const VA* __restrict__ vals, const double* __restrict__ diag,
+                                                 const int* __restrict__ cxyz, float* op, float* dinv) {
+    __shared__ double lds[27 * 64];
+    const int I = blockIdx.x;
+    if (I >= ncell) return;
+    const int lane = lane_id();
+    for (int b = 0; b < 27; ++b) lds[b * 64 + lane] = 0.0;
+    const int a0 = cs[I];
+    if (a0 >= 0) {
+        const int b0 = ce[I];
+        const int ci = I / (g.gy * g.gz), cj = (I / g.gz) % g.gy, ck = I % g.gz;
+        for (int t = a0 + lane; t < b0; t += 64) {
+            if (dir[t]) continue;
+            lds[13 * 64 + lane] += diag[t];
+            const int P = pos[t];
+            const int base = slice_off[P >> 6], l = P & 63, n = cnt[t];
+            for (int k = 0; k < n; ++k) {
+                const int e = sell_at(base, l, k);
+                const int x = cxyz[cols[e]];
+                const int dx = (x >> 20) - ci, dy = ((x >> 10) & 1023) - cj, dz = (x & 1023) - ck;
+                // a neighbour within the Laplacian radius lies in an adjacent cell
+                if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;
+                lds[((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)) * 64 + lane] += (double)vals[e];
+            }
+        }
+    }
+    __syncthreads();
+    if (lane < 27) {
+        double sum = 0.0;
+        for (int l = 0; l < 64; ++l) sum += lds[lane * 64 + l];
+        op[(size_t)I * 27 + lane] = (float)sum;
+        if (lane == 13) dinv[I] = sum > 0.0 ? (float)(VC_OMEGA / sum) : 0.0f;
+    }
+}
+// rc1 = P1^T r (r - alpha q with upd, as k_zcomp then writes it) and e = omega rc1 / diag Ac1:
+// eight lanes per cell striding over its contiguous rows, summed in a fixed lane order
+template <typename V>
+__global__ __launch_bounds__(PB) void k_cell_pre(int ncell, const int* __restrict__ cs, const int* __restrict__ ce,
+                                                  const int* __restrict__ dir, const V* __restrict__ r,
+                                                  const V* __restrict__ q, int upd, const float* __restrict__ dinv,
+                                                  double* rc, double* e, const double* __restrict__ scal,
+                                                  int check_done) {
+    if (check_done && scal[S_DONE] != 0.0) return;
+    const int g8 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int I = g8 >> 3, sub = g8 & 7;
+    const V alpha = upd ? (V)scal[S_ALPHA] : (V)0;
+    double acc = 0.0;
+    const int a0 = I < ncell ? cs[I] : -1;
+    if (a0 >= 0) {
+        const int b0 = ce[I];
+        for (int t = a0 + sub; t < b0; t += 8)
+            if (!dir[t]) acc += (double)(upd ? (V)(r[t] - alpha * q[t]) : r[t]);
+    }
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    acc += __shfl_xor(acc, 4, 64);
+    if (I < ncell && sub == 0) {
+        rc[I] = acc;
+        e[I] = (double)dinv[I] * acc;
+    }
+}
+// rc2[A] = sum over the cells I of aggregate A of (rc1 - Ac1 e)[I]: one wave per aggregate, lanes
+// over its (cell, stencil slot) pairs
+__global__ __launch_bounds__(64) void k_cell_agg(int nc, AggGeom g, const int* __restrict__ full_of,
+                                                  const float* __restrict__ op, const double* __restrict__ rc1,
+                                                  const double* __restrict__ e, double* rc2,
+                                                  const double* __restrict__ scal, int check_done) {
+    if (check_done && scal[S_DONE] != 0.0) return;
+    const int A = blockIdx.x;
+    if (A >= nc) return;
+    const int lane = lane_id();
+    const int f = full_of[A];
+    const int ax = f / (g.cgy * g.cgz), ay = (f / g.cgz) % g.cgy, az = f % g.cgz;
+    const int ncells = g.a * g.a * g.a;
+    double acc = 0.0;
+    for (int it = lane; it < ncells * 28; it += 64) {
+        const int c = it / 28, b = it % 28;
+        const int ci = ax * g.a + c / (g.a * g.a), cj = ay * g.a + (c / g.a) % g.a, ck = az * g.a + c % g.a;
+        if (ci >= g.gx || cj >= g.gy || ck >= g.gz) continue;
+        const int I = (ci * g.gy + cj) * g.gz + ck;
+        if (b == 27) {
+            acc += rc1[I];
+        } else {
+            const int J = stencil_cell(g, ci, cj, ck, b);
+            if (J >= 0) acc -= (double)op[(size_t)I * 27 + b] * e[J];
+        }
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) rc2[A] = acc;
+}
+// ef = e1 + omega (rc1 - Ac1 e1) / diag Ac1 with e1 = e + the coarse correction of the cell's aggregate
+__global__ __launch_bounds__(PB) void k_cell_post(int ncell, AggGeom g, const int* __restrict__ c_agg,
+                                                   const float* __restrict__ op, const float* __restrict__ dinv,
+                                                   const double* __restrict__ rc1, const double* __restrict__ e,
+                                                   const double* __restrict__ cc, double* ef,
+                                                   const double* __restrict__ scal, int check_done) {
+    if (check_done && scal[S_DONE] != 0.0) return;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= ncell) return;
+    if (dinv[I] == 0.0f) {
+        ef[I] = 0.0;
+        return;
+    }
+    auto e1 = [&](int J) {
+        const int A = c_agg[J];
+        return e[J] + (A >= 0 ? cc[A] : 0.0);
+    };
+    const int ci = I / (g.gy * g.gz), cj = (I / g.gz) % g.gy, ck = I % g.gz;
+    double sres = rc1[I];
+    const float* o = op + (size_t)I * 27;
+    for (int b = 0; b < 27; ++b) {
+        const int J = stencil_cell(g, ci, cj, ck, b);
+        if (J >= 0) sres -= (double)o[b] * e1(J);
+    }
+    ef[I] = e1(I) + (double)dinv[I] * sres;
+}
+
 // cc = Ainv rc (one wave per row; Ainv symmetric so row I = column I, contiguous)
 template <typename M>
 __global__ __launch_bounds__(64) void k_coarse_mv(int nc, const M* __restrict__ Ainv, const double* __restrict__ rc,
@@ -1054,6 +1224,39 @@ static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hi
 
 // PCG on A e = r0 (V = FP32, e from 0) or A x = b (V = FP64, x = warm start, r0 = b - A x) with
 // the two-level preconditioner.  Returns iterations or -1 on breakdown.
+// The cell level of the V-cycle for this solve: every row's cell and every cell's aggregate in the
+// (possibly frozen) aggregate map; Ac1 = P1^T A P1 from this solve's matrix every CELL_REBUILD
+// solves and whenever the aggregates were rebuilt (in between the last Ac1 is applied: a fixed
+// SPD operator, so the preconditioner stays SPD).  Returns 1; 0 when
+// the cell grid is too large for the packed coordinates (the solve keeps the two-level
+// preconditioner); -1 when the cell arrays cannot be allocated.
+template <typename R, typename VA>
+static int setup_cells(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hipStream_t st) {
+    const int N = d.N;
+    const int ncell = d.c.ncell;
+    if (d.c.gx >= 1024 || d.c.gy >= 1024 || d.c.gz >= 1024) return 0;   // packed coordinates hold 10 bits
+    if (!s->c_op) {
+        bool ok = dev_alloc(&s->cxyz, (size_t)s->nslices * 64) && dev_alloc(&s->cell_of_row, (size_t)s->nslices * 64) &&
+                  dev_alloc(&s->c_agg, ncell) && dev_alloc(&s->c_op, (size_t)ncell * 27) && dev_alloc(&s->c_dinv, ncell) &&
+                  dev_alloc(&s->c_rc, ncell) && dev_alloc(&s->c_e, ncell) && dev_alloc(&s->c_ef, ncell);
+        if (!ok) return -1;
+    }
+    s->ncell = ncell;
+    const AggGeom g{d.c.gx, d.c.gy, d.c.gz, s->agg_edge, s->cgx, s->cgy, s->cgz};
+    hipLaunchKernelGGL(k_row_cell, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, g, d.skeys, s->dir, s->cxyz,
+                       s->cell_of_row);
+    hipLaunchKernelGGL(k_cell_agg_map, dim3(grid_for(ncell, PB)), dim3(PB), 0, st, ncell, g, s->agg_flag, s->agg_map,
+                       s->c_agg);
+    if (s->cell_age < 0 || s->cell_age + 1 >= CELL_REBUILD || s->coarse_rebuilt) {
+        hipLaunchKernelGGL((k_cell_op<VA>), dim3(ncell), dim3(64), 0, st, ncell, g, d.cell_start, d.cell_end, s->dir,
+                           s->cnt, s->pos, s->slice_off, s->cols, vals, s->diag, s->cxyz, s->c_op, s->c_dinv);
+        s->cell_age = 0;
+    } else {
+        s->cell_age += 1;
+    }
+    return 1;
+}
+
 // Iterations queued before the next flag read.  Every launch queued after convergence returns at
 // once but still costs its dispatch, so with a hint (the iterations the same pass took in the last
 // solve) the first chunk stops one short of it and the rest come in chunks of CHUNK_TAIL.
@@ -1083,12 +1286,31 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
     hipLaunchKernelGGL(k_agg_ranges, dim3(grid_for((long long)nc * stacks, PB)), dim3(PB), 0, st, nc, g, s->agg_full,
                        d.cell_start, d.cell_end, s->agg_rng);
     hipLaunchKernelGGL(k_pcg_start2<V>, dim3(nb), dim3(PB), 0, st, N, r0, r, x, zero_x);
-    hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, stacks, s->agg_rng, s->dir, r, q, 0, s->rc,
+    // the coarse part of the preconditioner on r (upd = 0) or on r - alpha q (upd = 1): the aggregate
+    // correction cc, or with the V-cycle the cell correction c_ef
+    const bool vc = s->vcycle != 0;
+    const int ncell = s->ncell;
+    auto coarse = [&](int upd, int check) {
+        if (vc) {
+            hipLaunchKernelGGL(k_cell_pre<V>, dim3(grid_for((long long)ncell * 8, PB)), dim3(PB), 0, st, ncell,
+                               d.cell_start, d.cell_end, s->dir, r, q, upd, s->c_dinv, s->c_rc, s->c_e, s->scal, check);
+            hipLaunchKernelGGL(k_cell_agg, dim3(nc), dim3(64), 0, st, nc, g, s->agg_full, s->c_op, s->c_rc, s->c_e,
+                               s->rc, s->scal, check);
+        } else {
+            hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, stacks, s->agg_rng, s->dir, r, q, upd,
+                               s->rc, s->scal, check);
+        }
+        if (f32) hipLaunchKernelGGL(k_coarse_mv<float>, dim3(nc), dim3(64), 0, st, nc, s->Af, s->rc, s->cc, s->scal, check);
+        else hipLaunchKernelGGL(k_coarse_mv<double>, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, check);
+        if (vc)
+            hipLaunchKernelGGL(k_cell_post, dim3(grid_for(ncell, PB)), dim3(PB), 0, st, ncell, g, s->c_agg, s->c_op,
+                               s->c_dinv, s->c_rc, s->c_e, s->cc, s->c_ef, s->scal, check);
+    };
+    const int* zrow = vc ? s->cell_of_row : s->agg_of_row;
+    const double* zcor = vc ? s->c_ef : s->cc;
+    coarse(0, 0);
+    hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, zrow, zcor, z, p, 1, x, q, 0, s->part,
                        s->scal, 0);
-    if (f32) hipLaunchKernelGGL(k_coarse_mv<float>, dim3(nc), dim3(64), 0, st, nc, s->Af, s->rc, s->cc, s->scal, 0);
-    else hipLaunchKernelGGL(k_coarse_mv<double>, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 0);
-    hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, s->agg_of_row, s->cc, z, p, 1, x, q, 0,
-                       s->part, s->scal, 0);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_START);
     int done_iters = 0;
     const int probe = s->probe_at;
@@ -1106,12 +1328,9 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
                 s->ev_used += 2;
             }
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 1, s->scal, (int)RED_ALPHA);
-            hipLaunchKernelGGL(k_restrict<V>, dim3(nc), dim3(64), 0, st, nc, stacks, s->agg_rng, s->dir, r, q, 1, s->rc,
-                               s->scal, 1);
-            if (f32) hipLaunchKernelGGL(k_coarse_mv<float>, dim3(nc), dim3(64), 0, st, nc, s->Af, s->rc, s->cc, s->scal, 1);
-            else hipLaunchKernelGGL(k_coarse_mv<double>, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, 1);
-            hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, s->agg_of_row, s->cc, z, p, 0, x,
-                               q, 1, s->part, s->scal, 1);
+            coarse(1, 1);
+            hipLaunchKernelGGL(k_zcomp<V>, dim3(nb), dim3(PB), 0, st, N, r, s->pdiag, zrow, zcor, z, p, 0, x, q, 1,
+                               s->part, s->scal, 1);
             hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_BETA);
             hipLaunchKernelGGL(k_pcg_direction<V>, dim3(nb), dim3(PB), 0, st, N, z, p, s->scal);
         }
@@ -1141,7 +1360,8 @@ template <typename R> void ppe_destroy(PpeSolver<R>* s) {
                   s->diag, s->rhs, s->x, s->prev, s->r, s->z, s->p0, s->p1, s->q, s->rf, s->zf, s->p0f,
                   s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp, s->nnz_dev, s->agg_flag,
                   s->agg_map, s->agg_full, s->agg_of_row, s->nc_dev, s->info_dev, s->Ac, s->Af, s->agg_rng, s->cpart, s->rxyz, s->rc, s->cc,
-                  s->agg_scan.scan_tmp, s->pdiag_buf, s->omega, s->xprev_s, s->xl, s->nbq, s->nbw};
+                  s->agg_scan.scan_tmp, s->pdiag_buf, s->omega, s->xprev_s, s->xl, s->nbq, s->nbw, s->cxyz,
+                  s->cell_of_row, s->c_agg, s->c_op, s->c_dinv, s->c_rc, s->c_e, s->c_ef};
     if (s->net) mpnn_destroy(s->net);
     for (void* p : ps)
         if (p) (void)hipFree(p);
@@ -1370,7 +1590,8 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->two_level_ok = 0;
     s->coarse_rebuilt = 0;
     s->setup_ms = 0.0;
-    if (opt.precond == MPSR_PRECOND_TWO_LEVEL && bb > 0 && rr > target) {
+    const bool coarse_pre = opt.precond == MPSR_PRECOND_TWO_LEVEL || opt.precond == MPSR_PRECOND_CELL_VCYCLE;
+    if (coarse_pre && bb > 0 && rr > target) {
         const int every = opt.coarse_rebuild > 0 ? opt.coarse_rebuild : 50;
         const bool rebuild = !s->coarse_valid || s->coarse_age + 1 >= every ||
                              (s->iters_at_build > 0 && 4 * s->iters_last > 5 * s->iters_at_build);
@@ -1390,6 +1611,12 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
             s->two_level_ok = 1;
             s->coarse_age += 1;
         }
+    }
+    s->vcycle = 0;
+    if (s->two_level_ok && opt.precond == MPSR_PRECOND_CELL_VCYCLE) {
+        const int ok = s->mixed ? setup_cells<R, float>(s, d, s->vals32, st) : setup_cells<R, double>(s, d, s->vals64, st);
+        if (ok < 0) return MPSR_MEMERR;
+        s->vcycle = ok;
     }
     const bool two = s->two_level_ok != 0;
     if (s->learned && bb > 0 && rr > target) {

@@ -248,6 +248,53 @@ def test_two_level_reused_coarse_operator_steps_within_tolerance(lib, solver):
         assert err < 1e-3, (f, err)
 
 
+@pytest.mark.parametrize("solver", [abi.MPSR_PCG_FP64, abi.MPSR_PCG_MIXED])
+def test_cell_vcycle_same_solution_fewer_iterations(lib, solver):
+    """The cell V-cycle preconditioner (one symmetric V-cycle on the cell-level Galerkin operator
+    with the aggregate correction as its coarse level) solves the same system to the same
+    tolerance as the two-level one and the oracle, in fewer iterations."""
+    case = cases.drum(lib, l0=6e-3)
+    base = dict(pressure_mode=abi.MPSR_PRESSURE_PPE)
+    opts = [abi.PpeOptions(solver, 1e-8, 20000, abi.MPSR_WARM_ZERO, 0.0, 0.0, pre, 2, 0)
+            for pre in (abi.MPSR_PRECOND_TWO_LEVEL, abi.MPSR_PRECOND_CELL_VCYCLE)]
+    print(f"[vcycle] solver={solver} n={case.n}", flush=True)
+    g_t, g_v = (Solver(case, ppe=o, **base) for o in opts)
+    o = Oracle(case, ppe=abi.PpeOptions(abi.MPSR_PCG_FP64, 1e-8, 20000, abi.MPSR_WARM_ZERO, 0.0, 0.0), **base)
+    for s in (g_t, g_v, o):
+        for name in ("RegisterParticleTopology", "RegisterTriangleTopology", "CalcSTLDistance", "CalcExplicitly",
+                     "RelocateAllParticleTopology", "RegisterParticleTopology", "CalcSTLDistance",
+                     "CalcPressurePoisson"):
+            getattr(s, name)()
+    tt, tv = g_t.timings(), g_v.timings()
+    assert tt["ppe_rel_residual"] <= 1e-8 and tv["ppe_rel_residual"] <= 1e-8
+    assert tv["ppe_iterations"] < tt["ppe_iterations"] and tv["ppe_coarse_rows"] > 0, (tv, tt)
+    _, xo, _, _ = o.ppe_system()
+    _, xv, _ = g_v.ppe_system()
+    assert np.linalg.norm(xv - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("solver", [abi.MPSR_PCG_FP64, abi.MPSR_PCG_MIXED])
+def test_cell_vcycle_steps_within_tolerance(lib, solver):
+    """Eight steps with the cell V-cycle, the aggregate operator rebuilt every fourth solve and
+    the cell operator every solve: every solve reaches 1e-8, and the state stays within the FP32
+    bound of the oracle's plain FP64 Jacobi-PCG run."""
+    case = cases.drum(lib, l0=6e-3)
+    base = dict(pressure_mode=abi.MPSR_PRESSURE_PPE)
+    vc = abi.PpeOptions(solver, 1e-8, 20000, abi.MPSR_WARM_PREVIOUS, 0.0, 0.0, abi.MPSR_PRECOND_CELL_VCYCLE, 2, 4)
+    print(f"[vcycle] steps solver={solver}", flush=True)
+    g = Solver(case, ppe=vc, **base)
+    o = Oracle(case, ppe=abi.PpeOptions(abi.MPSR_PCG_FP64, 1e-8, 20000, abi.MPSR_WARM_PREVIOUS, 0.0, 0.0), **base)
+    for i in range(8):
+        g.step(i)
+        o.step(i)
+        t = g.timings()
+        assert t["ppe_rel_residual"] <= 1e-8 and t["ppe_coarse_rows"] > 0 and t["ppe_fallbacks"] == 0, (i, t)
+    a, b = g.get_particles(0), o.get_particles(0)
+    for f in ("position", "velocity"):
+        err = np.linalg.norm(getattr(a, f) - getattr(b, f)) / np.linalg.norm(getattr(b, f))
+        assert err < 1e-3, (f, err)
+
+
 DECK_CONFIG = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "velocity_deck",
                                          "configurationDragTemp.txt")
 
