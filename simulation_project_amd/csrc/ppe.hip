@@ -858,7 +858,7 @@ __global__ __launch_bounds__(64) void k_restrict(int nc, int stacks, const int2*
 // is kept as a 27-point stencil.  Every step is linear and symmetric, and the pre- and
 // post-smoothers are the same, so the preconditioner is SPD.
 static constexpr double VC_OMEGA = 0.7;
-static constexpr int CELL_REBUILD = 10;   // Ac1 is rebuilt every this many solves and with the aggregates
+static constexpr int CELL_REBUILD = 50;   // Ac1 is rebuilt every this many solves and with the aggregates
 
 __device__ inline int stencil_cell(const AggGeom& g, int ci, int cj, int ck, int b) {
     const int x = ci + b / 9 - 1, y = cj + (b / 3) % 3 - 1, z = ck + b % 3 - 1;
