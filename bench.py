@@ -16,9 +16,10 @@ resident in HBM when the timed region starts; inside it only the PCG's convergen
 32 iterations) and the per-step timings cross PCIe.
 
 Production solver: mixed precision (FP32 matrix and SpMV, FP64 dots and residuals, FP64
-refinement) with the two-level preconditioner (2^3-cell aggregates, coarse operator rebuilt every
-50 solves; the JSON line gives the set-up time of one rebuild and its share per step),
-warm-started by the previous pressure.  --solver learned adds the trained network's
+refinement) with the cell V-cycle preconditioner: Jacobi plus one symmetric V-cycle on the
+cell-level Galerkin operator whose coarse level is the dense inverse on 2^3-cell aggregates
+(rebuilt every 50 solves; the JSON line gives the set-up time of one rebuild and its share per
+step), warm-started by the previous pressure.  --solver learned adds the trained network's
 warm start (simulation_project_amd/weights/mpnn_drum.npy).  The plain comparison solves the same
 steps with FP64 Jacobi-PCG from the previous pressure.
 
@@ -42,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per SpMV launch of the mixed-precision solver on the default drum (l0 = 2.012e-3 m),
 # from two rocprofv3 counter passes (FETCH_SIZE scaled by the factor a plain copy kernel of known
 # size reads, WRITE_SIZE unscaled): profiles/k_pcg_spmv_drum1M_traffic.md
-SPMV_TRAFFIC_BYTES = {(2.012e-3, "mixed"): 958.4e6}
+SPMV_TRAFFIC_BYTES = {(2.012e-3, "mixed"): 958.6e6}
 WEIGHTS = os.path.join(ROOT, "simulation_project_amd", "weights", "mpnn_drum.npy")
 
 
@@ -313,7 +314,7 @@ def radix(args):
     res = {}
     for name, on in (("on", True), ("off", False)):
         s = Solver(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE,
-                   ppe=ppe_options(abi, "mixed", "two", abi.MPSR_WARM_PREVIOUS))
+                   ppe=ppe_options(abi, "mixed", "vcycle", abi.MPSR_WARM_PREVIOUS))
         if sph is not None:
             s.set_spheres(sph)
         if on:
@@ -360,7 +361,7 @@ def radix(args):
         "ms_per_step": round(on["ms_per_step"], 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"rotating drum, {n} particles (l0={args.l0} m), "
-                               f"{0 if sph is None else len(sph.position)} spheres, PPE to 1e-8 (mixed, two-level), "
+                               f"{0 if sph is None else len(sph.position)} spheres, PPE to 1e-8 (mixed, cell V-cycle), "
                                f"Smagorinsky viscosity and negative pressure on", "particles": n},
         "terms_off_ms_per_step": round(off["ms_per_step"], 3),
         "phase_ms": {"on": on["phase_ms"], "off": off["phase_ms"]},
@@ -376,7 +377,7 @@ def radix(args):
 
 def precision(args):
     """The mixed-precision drum (FP32 particle operators and SpMV, FP64 dots, residuals and
-    refinement, two-level preconditioner) against the FP64 build (FP64 operators, FP64 PCG with
+    refinement, cell V-cycle preconditioner) against the FP64 build (FP64 operators, FP64 PCG with
     the same preconditioner, 1e-8) from the same start, both on the GPU, after --steps steps:
     relative L2 and max errors of pressure, velocity and position over the particles alive in
     both, and the bulk kinetic energy and mean pressure.  The FP64 build is the stand-in for the
@@ -391,7 +392,7 @@ def precision(args):
     res = {}
     for name, prec, kind in (("mixed", abi.MPSR_FP32, "mixed"), ("fp64", abi.MPSR_FP64, "fp64")):
         s = Solver(case, precision=prec, pressure_mode=abi.MPSR_PRESSURE_PPE,
-                   ppe=ppe_options(abi, kind, "two", abi.MPSR_WARM_PREVIOUS))
+                   ppe=ppe_options(abi, kind, "vcycle", abi.MPSR_WARM_PREVIOUS))
         if sph is not None:
             s.set_spheres(sph)
         s.synchronize()
