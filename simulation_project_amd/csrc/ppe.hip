@@ -124,6 +124,8 @@ template <typename R> struct PpeSolver {
     int vcycle = 0;              // this solve uses it
     int ncell = 0;
     int cell_age = -1;           // solves since Ac1 was built (-1: never)
+    int cell_rebuilt = 0;        // this solve rebuilt Ac1
+    int iters_at_cell = 0;       // fewest iterations of a solve since Ac1 was built
     int* cxyz = nullptr;         // [rows] packed cell coordinates of every row
     int* cell_of_row = nullptr;  // [rows] cell of a non-Dirichlet row, else -1
     int* c_agg = nullptr;        // [ncell] compact aggregate of the cell, else -1
@@ -858,7 +860,12 @@ __global__ __launch_bounds__(64) void k_restrict(int nc, int stacks, const int2*
 // is kept as a 27-point stencil.  Every step is linear and symmetric, and the pre- and
 // post-smoothers are the same, so the preconditioner is SPD.
 static constexpr double VC_OMEGA = 0.7;
-static constexpr int CELL_REBUILD = 50;   // Ac1 is rebuilt every this many solves and with the aggregates
+static constexpr int GROWTH_MIN_AGE = 10;  // the iteration-growth rebuild waits this many solves after a rebuild
+static constexpr int PASS_CAP_MIN = 100;    // an inner refinement pass stops after max(this, 4 x its hint) iterations
+static constexpr int PASS_CAP_COLD = 500;   // ... or after this many when the pass has no hint yet
+static constexpr int CELL_REBUILD = 50;  // Ac1 is rebuilt every this many solves and with the aggregates,
+static constexpr int CELL_MIN_AGE = 5;   // ... and from this many solves on when the last solve took more
+                                         // than 1.2 x the fewest iterations of any solve since it was built
 
 __device__ inline int stencil_cell(const AggGeom& g, int ci, int cj, int ck, int b) {
     const int x = ci + b / 9 - 1, y = cj + (b / 3) % 3 - 1, z = ck + b % 3 - 1;
@@ -1247,7 +1254,10 @@ static int setup_cells(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hipStre
                        s->cell_of_row);
     hipLaunchKernelGGL(k_cell_agg_map, dim3(grid_for(ncell, PB)), dim3(PB), 0, st, ncell, g, s->agg_flag, s->agg_map,
                        s->c_agg);
-    if (s->cell_age < 0 || s->cell_age + 1 >= CELL_REBUILD || s->coarse_rebuilt) {
+    s->cell_rebuilt = s->cell_age < 0 || s->cell_age + 1 >= CELL_REBUILD || s->coarse_rebuilt ||
+                      (s->iters_at_cell > 0 && s->cell_age + 1 >= CELL_MIN_AGE &&
+                       5 * s->iters_last > 6 * s->iters_at_cell);
+    if (s->cell_rebuilt) {
         hipLaunchKernelGGL((k_cell_op<VA>), dim3(ncell), dim3(64), 0, st, ncell, g, d.cell_start, d.cell_end, s->dir,
                            s->cnt, s->pos, s->slice_off, s->cols, vals, s->diag, s->cxyz, s->c_op, s->c_dinv);
         s->cell_age = 0;
@@ -1594,7 +1604,8 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     if (coarse_pre && bb > 0 && rr > target) {
         const int every = opt.coarse_rebuild > 0 ? opt.coarse_rebuild : 50;
         const bool rebuild = !s->coarse_valid || s->coarse_age + 1 >= every ||
-                             (s->iters_at_build > 0 && 4 * s->iters_last > 5 * s->iters_at_build);
+                             (s->iters_at_build > 0 && s->coarse_age + 1 >= GROWTH_MIN_AGE &&
+                              4 * s->iters_last > 5 * s->iters_at_build);
         if (rebuild) {
             if (profile) (void)hipEventRecord(s->setup_ev[0], st);
             s->two_level_ok = s->mixed ? setup_two_level<R, float>(s, d, s->vals32, st)
@@ -1697,12 +1708,15 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
         while (bb > 0 && rr > target && s->passes < 16 && s->iters < maxit) {
             const double inner_rel = std::max(1e-6, 0.5 * tol * std::sqrt(bb / rr));
             const int hp = std::min(s->passes, HINT_PASSES - 1);
+            // a pass that stalls in FP32 ends at its cap; the next pass restarts from the FP64 residual
+            const int cap = std::min(maxit - s->iters, s->hint[hp] > 0 ? std::max(PASS_CAP_MIN, 4 * s->hint[hp])
+                                                                       : PASS_CAP_COLD);
             const int it =
                 two ? run_pcg2<R, float, float>(s, d, N, s->vals32, s->ef, s->rf, s->zf, s->p0f, s->qf, s->r,
-                                                inner_rel * inner_rel * rr, maxit - s->iters, 1, st, s->hint[hp])
+                                                inner_rel * inner_rel * rr, cap, 1, st, s->hint[hp])
                     : run_pcg<R, float, float>(s, N, s->vals32, s->ef, s->rf, s->zf, s->p0f, s->p1f, s->qf, s->r,
-                                               inner_rel * inner_rel * rr, maxit - s->iters, st, s->hint[hp]);
-            if (it >= 0) s->hint[hp] = it;
+                                               inner_rel * inner_rel * rr, cap, st, s->hint[hp]);
+            if (it >= 0 && it < cap) s->hint[hp] = it;
             if (it == -2) {
                 if (!fall_back()) return MPSR_GPUERR;
                 continue;
@@ -1740,6 +1754,7 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     }
     s->relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     if (s->coarse_rebuilt && s->two_level_ok) s->iters_at_build = s->iters;
+    if (s->vcycle) s->iters_at_cell = s->cell_rebuilt ? s->iters : std::min(s->iters_at_cell, s->iters);
     s->iters_last = s->iters;
     if (profile && s->coarse_rebuilt) {
         float ms = 0.f;
