@@ -1023,10 +1023,33 @@ __global__ __launch_bounds__(64) void k_coarse_mv(int nc, const M* __restrict__ 
     const int I = blockIdx.x;
     if (I >= nc) return;
     const M* row = Ainv + (size_t)I * nc;
+    const int l = lane_id();
     double acc = 0.0;
-    for (int J = lane_id(); J < nc; J += 64) acc += (double)row[J] * rc[J];
+    if constexpr (std::is_same<M, float>::value) {
+        // the FP32 row in 16-byte loads: a scalar head up to the first 16-byte boundary (the
+        // buffer itself is 256-byte aligned), float4 loads, a scalar tail; four independent sums
+        const int h = min((int)((4 - (((size_t)I * nc) & 3)) & 3), nc);
+        const int nv = (nc - h) >> 2;
+        const float4* row4 = reinterpret_cast<const float4*>(row + h);
+        const double* rc4 = rc + h;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        if (l < h) a0 = (double)row[l] * rc[l];
+#pragma unroll 4
+        for (int v = l; v < nv; v += 64) {
+            const float4 a = row4[v];
+            const double* rv = rc4 + 4 * v;
+            a0 += (double)a.x * rv[0];
+            a1 += (double)a.y * rv[1];
+            a2 += (double)a.z * rv[2];
+            a3 += (double)a.w * rv[3];
+        }
+        for (int J = h + 4 * nv + l; J < nc; J += 64) a0 += (double)row[J] * rc[J];
+        acc = (a0 + a1) + (a2 + a3);
+    } else {
+        for (int J = l; J < nc; J += 64) acc += (double)row[J] * rc[J];
+    }
     acc = wave_sum(acc);
-    if (lane_id() == 0) cc[I] = acc;
+    if (l == 0) cc[I] = acc;
 }
 
 // z = r / diag + cc[agg]; optionally p = z; partial r.z and r.r
