@@ -988,31 +988,34 @@ __global__ __launch_bounds__(64) void k_cell_agg(int nc, AggGeom g, const int* _
     acc = wave_sum(acc);
     if (lane == 0) rc2[A] = acc;
 }
-// ef = e1 + omega (rc1 - Ac1 e1) / diag Ac1 with e1 = e + the coarse correction of the cell's aggregate
+// ef = e1 + omega (rc1 - Ac1 e1) / diag Ac1 with e1 = e + the coarse correction of the cell's aggregate:
+// eight lanes per cell, lane `sub` over the stencil slots sub, sub + 8, ..., summed in a fixed lane order
 __global__ __launch_bounds__(PB) void k_cell_post(int ncell, AggGeom g, const int* __restrict__ c_agg,
                                                    const float* __restrict__ op, const float* __restrict__ dinv,
                                                    const double* __restrict__ rc1, const double* __restrict__ e,
                                                    const double* __restrict__ cc, double* ef,
                                                    const double* __restrict__ scal, int check_done) {
     if (check_done && scal[S_DONE] != 0.0) return;
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= ncell) return;
-    if (dinv[I] == 0.0f) {
-        ef[I] = 0.0;
-        return;
-    }
+    const int g8 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int I = g8 >> 3, sub = g8 & 7;
     auto e1 = [&](int J) {
         const int A = c_agg[J];
         return e[J] + (A >= 0 ? cc[A] : 0.0);
     };
-    const int ci = I / (g.gy * g.gz), cj = (I / g.gz) % g.gy, ck = I % g.gz;
-    double sres = rc1[I];
-    const float* o = op + (size_t)I * 27;
-    for (int b = 0; b < 27; ++b) {
-        const int J = stencil_cell(g, ci, cj, ck, b);
-        if (J >= 0) sres -= (double)o[b] * e1(J);
+    const bool live = I < ncell && dinv[I] != 0.0f;
+    double acc = 0.0;
+    if (live) {
+        const int ci = I / (g.gy * g.gz), cj = (I / g.gz) % g.gy, ck = I % g.gz;
+        const float* o = op + (size_t)I * 27;
+        for (int b = sub; b < 27; b += 8) {
+            const int J = stencil_cell(g, ci, cj, ck, b);
+            if (J >= 0) acc += (double)o[b] * e1(J);
+        }
     }
-    ef[I] = e1(I) + (double)dinv[I] * sres;
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    acc += __shfl_xor(acc, 4, 64);
+    if (I < ncell && sub == 0) ef[I] = live ? e1(I) + (double)dinv[I] * (rc1[I] - acc) : 0.0;
 }
 
 // cc = Ainv rc (one wave per row; Ainv symmetric so row I = column I, contiguous)
@@ -1336,7 +1339,7 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
         if (f32) hipLaunchKernelGGL(k_coarse_mv<float>, dim3(nc), dim3(64), 0, st, nc, s->Af, s->rc, s->cc, s->scal, check);
         else hipLaunchKernelGGL(k_coarse_mv<double>, dim3(nc), dim3(64), 0, st, nc, s->Ac, s->rc, s->cc, s->scal, check);
         if (vc)
-            hipLaunchKernelGGL(k_cell_post, dim3(grid_for(ncell, PB)), dim3(PB), 0, st, ncell, g, s->c_agg, s->c_op,
+            hipLaunchKernelGGL(k_cell_post, dim3(grid_for((long long)ncell * 8, PB)), dim3(PB), 0, st, ncell, g, s->c_agg, s->c_op,
                                s->c_dinv, s->c_rc, s->c_e, s->cc, s->c_ef, s->scal, check);
     };
     const int* zrow = vc ? s->cell_of_row : s->agg_of_row;
