@@ -859,7 +859,7 @@ __global__ __launch_bounds__(64) void k_restrict(int nc, int stacks, const int2*
 // again.  Ac1 couples a cell to its 26 neighbours only (the Laplacian radius is one cell), so it
 // is kept as a 27-point stencil.  Every step is linear and symmetric, and the pre- and
 // post-smoothers are the same, so the preconditioner is SPD.
-static constexpr double VC_OMEGA = 0.7;
+static constexpr double VC_OMEGA = 0.9;
 static constexpr int GROWTH_MIN_AGE = 10;  // the iteration-growth rebuild waits this many solves after a rebuild
 static constexpr int PASS_CAP_MIN = 100;    // an inner refinement pass stops after max(this, 4 x its hint) iterations
 static constexpr int PASS_CAP_COLD = 500;   // ... or after this many when the pass has no hint yet
