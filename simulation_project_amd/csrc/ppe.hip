@@ -333,8 +333,26 @@ __global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __r
     if (t >= 0) {
         const int base = slice_off[slice], width = min(slice_w[slice] >> 6, (cnt[t] + 3) & ~3);
         double a = diag[t] * x[t];
-        // entries in ascending order, 8 column/value loads and then 8 gathers in flight
+        // entries in ascending order, 16 (FP32 matrix) or 8 column/value loads and then as many
+        // gathers in flight; the sum runs entry by entry either way
         int k = 0;
+        if constexpr (sizeof(VA) == 4) {
+            for (; k + 16 <= width; k += 16) {
+                int c[16];
+                VA v[16];
+                const int e = sell_at(base, lane, k);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    quad_load(cols + e + 256 * u, c + 4 * u);
+                    quad_load(vals + e + 256 * u, v + 4 * u);
+                }
+                double g[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) g[u] = x[c[u]];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) a += (double)v[u] * g[u];
+            }
+        }
         for (; k + 8 <= width; k += 8) {
             int c[8];
             VA v[8];
