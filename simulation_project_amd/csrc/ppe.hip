@@ -977,8 +977,7 @@ __global__ __launch_bounds__(PB) void k_cell_pre(int ncell, const int* __restric
         e[I] = (double)dinv[I] * acc;
     }
 }
-// rc2[A] = sum over the cells I of aggregate A of (rc1 - Ac1 e)[I]: one wave per aggregate, lanes
-// over its (cell, stencil slot) pairs
+// rc2[A] = sum over the cells I of aggregate A of (rc1 - Ac1 e)[I]: one wave per aggregate
 __global__ __launch_bounds__(64) void k_cell_agg(int nc, AggGeom g, const int* __restrict__ full_of,
                                                   const float* __restrict__ op, const double* __restrict__ rc1,
                                                   const double* __restrict__ e, double* rc2,
@@ -991,16 +990,18 @@ __global__ __launch_bounds__(64) void k_cell_agg(int nc, AggGeom g, const int* _
     const int ax = f / (g.cgy * g.cgz), ay = (f / g.cgz) % g.cgy, az = f % g.cgz;
     const int ncells = g.a * g.a * g.a;
     double acc = 0.0;
-    for (int it = lane; it < ncells * 28; it += 64) {
-        const int c = it / 28, b = it % 28;
+    // eight lanes per cell: lane `sub` takes the cell's own residual term (sub 0) and the stencil
+    // slots sub, sub + 8, ...; a cell's coordinates are decoded once
+    for (int it = lane; it < ncells * 8; it += 64) {
+        const int c = it >> 3, sub = it & 7;
         const int ci = ax * g.a + c / (g.a * g.a), cj = ay * g.a + (c / g.a) % g.a, ck = az * g.a + c % g.a;
         if (ci >= g.gx || cj >= g.gy || ck >= g.gz) continue;
         const int I = (ci * g.gy + cj) * g.gz + ck;
-        if (b == 27) {
-            acc += rc1[I];
-        } else {
+        if (sub == 0) acc += rc1[I];
+        const float* o = op + (size_t)I * 27;
+        for (int b = sub; b < 27; b += 8) {
             const int J = stencil_cell(g, ci, cj, ck, b);
-            if (J >= 0) acc -= (double)op[(size_t)I * 27 + b] * e[J];
+            if (J >= 0) acc -= (double)o[b] * e[J];
         }
     }
     acc = wave_sum(acc);
