@@ -222,7 +222,9 @@ typedef enum mpsr_precond {
 typedef struct mpsr_ppe_options {
     int solver;        /* mpsr_pcg_kind */
     double rel_tol;    /* stop when ||b - A x||_2 / ||b||_2 <= rel_tol (FP64 residual) */
-    int max_iter;      /* per solve, over all refinement passes */
+    int max_iter;      /* per solve, over all refinement passes.  A solve that ends above rel_tol
+                          (this cap, the 16-pass refinement cap), breaks down or turns non-finite
+                          fails the step with MPSR_CALCERR; no pressure is written back */
     int warm_start;    /* mpsr_warm_start */
     double n0_lap;     /* PND of the ideal lattice at the Laplacian radius; 0 = computed */
     double lambda_lap; /* Laplacian model constant at the Laplacian radius; 0 = computed */

@@ -1493,6 +1493,40 @@ float orc_tri_distance(const mpsr_triangle* T, float x, float y, float z, float*
     if (dir) { dir[0] = d.x; dir[1] = d.y; dir[2] = d.z; }
     return m;
 }
+// Brute force for the nearest-wall pin: every triangle in ascending id (the first of equal
+// distances wins), no cells, no stencil.  Returns per point the nearest distance, the second
+// nearest distance (over the other triangles) and the nearest id; triangles with skip[t] != 0 are
+// left out.  FP64 arithmetic, as the FP64 build's CalcSTLDistance.
+void orc_nearest_brute_f64(const mpsr_triangle* T, int nt, const int* skip, const double* xyz, int n, double* mag,
+                           double* second, int* id) {
+    std::vector<Tri<double>> tr(std::max(nt, 0));
+    for (int t = 0; t < nt; ++t) {
+        tr[t].model = T[t].ModelIDNumber;
+        tr[t].n = mk<double>(T[t].Normal.x, T[t].Normal.y, T[t].Normal.z);
+        for (int k = 0; k < 3; ++k) tr[t].v[k] = mk<double>(T[t].Vertex[k].x, T[t].Vertex[k].y, T[t].Vertex[k].z);
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        const V3<double> x = mk(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        double b1 = 1e300, b2 = 1e300;
+        int bid = -1;
+        for (int t = 0; t < nt; ++t) {
+            if (skip && skip[t]) continue;
+            V3<double> d;
+            const double m = tri_distance(tr[t], x, &d);
+            if (m < b1) {
+                b2 = b1;
+                b1 = m;
+                bid = t;
+            } else if (m < b2) {
+                b2 = m;
+            }
+        }
+        mag[i] = b1;
+        second[i] = b2;
+        id[i] = bid;
+    }
+}
 float orc_powd(float x, float y) { return powd<float>(x, y); }
 double orc_powd64(double x, double y) { return powd<double>(x, y); }
 

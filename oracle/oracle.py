@@ -89,6 +89,7 @@ def load():
         "orc_get_sphere_dv": (None, [vp, dp]),
         "orc_spheres_phase": (None, [vp]),
         "orc_sn_force": (None, [C.c_double] * 6 + [dp]),
+        "orc_nearest_brute_f64": (None, [P(abi.Triangle), C.c_int, ip, dp, C.c_int, dp, dp, ip]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -405,6 +405,30 @@ def small_box(lib, nx=8, ny=6, nz=8, l0=0.01, seed=DEFAULT_SEED, rotating=False,
     return _finish(case, lib)
 
 
+def moving_plate_box(lib, plate_velocity=(-0.3, 0.0, 0.1), tank_velocity=(0.0, 0.0, 0.05), nx=8, ny=6, nz=8,
+                     l0=0.01, seed=DEFAULT_SEED, dt=1e-4, explicit_pressure=True):
+    """A small box whose walls translate (UpdateTrianglePosition_Kernel, DeviceProcess_Kernel.h:
+    463-482): a vertical paddle (a second, non-drag STL model of two triangles at x = (nx + 2) l0,
+    two spacings from the water block's last plane, spanning the tank's y and z) moves with
+    plate_velocity toward the water, and the tank itself with tank_velocity.  The paddle's velocity is also the wall velocity of the Laplacian's
+    wall term (DeviceProcess_Kernel.h:384) for the particles whose nearest wall it is."""
+    case = small_box(lib, nx=nx, ny=ny, nz=nz, l0=l0, seed=seed, dt=dt, explicit_pressure=explicit_pressure)
+    xp = (nx + 2) * l0
+    y1, z1 = (ny + 4) * l0, (nz + 6) * l0
+    plate = np.asarray([[[xp, 0.0, 0.0], [xp, y1, 0.0], [xp, y1, z1]],
+                        [[xp, 0.0, 0.0], [xp, y1, z1], [xp, 0.0, z1]]], np.float32)
+    case.vertices = np.concatenate([case.vertices, plate])
+    case.model_of_triangle = np.concatenate([case.model_of_triangle, np.array([1, 1], np.int32)])
+    tank = case.models[0]
+    tank.Velocity = abi.Float3(*tank_velocity)
+    tank.LinearMovableOrNot = 1 if any(tank_velocity) else 0
+    paddle = tri_params(velocity=plate_velocity, friction=1.0)
+    paddle.LinearMovableOrNot = 1
+    case.models = [tank, paddle]
+    case.name = "moving_plate_box"
+    return _finish(case, lib)
+
+
 def create_info(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_EOS, ppe=None,
                 max_particles=None, device=0):
     """Build an mpsr_create_info; returns (info, keepalive)."""

@@ -597,7 +597,10 @@ template <typename R> struct Engine : EngineBase {
     }
 
     // ---- phases -------------------------------------------------------------------------------
+    int bin_epoch = 0;                 // binnings so far
+    int nb_epoch = -1;                 // the binning the PPE neighbour list was written for
     void do_bin() {
+        ++bin_epoch;
         mark_begin(PH_BIN);
         launch_bin_hash(d, st);
         launch_bin_ranges(d, st);
@@ -652,10 +655,12 @@ template <typename R> struct Engine : EngineBase {
         if (pressure_mode == MPSR_PRESSURE_PPE) {
             mark_begin(PH_PND);
             ppe_start(ppe, d, st);
+            nb_epoch = bin_epoch;
             mark_end();
             mark_begin(PH_PPE);
             mpsr_status s = ppe_solve(ppe, d, ppe_opt, profiling, st);
             mark_end();
+            if (s != MPSR_NOERR) return fail(s, std::string("PPE solve failed: ") + (ppe_why(ppe)[0] ? ppe_why(ppe) : "device error"));
             return s;
         }
         mark_begin(PH_PND);
@@ -665,7 +670,9 @@ template <typename R> struct Engine : EngineBase {
     }
     void do_gradient() {
         mark_begin(PH_GRAD);
-        launch_gradient(d, st);
+        // the PPE path's PND pass listed every row's neighbours; the list serves while the binning
+        // it was written for is the current one
+        launch_gradient(d, st, ppe && nb_epoch == bin_epoch ? ppe_neighbors(ppe) : NbList{});
         mark_end();
     }
     mpsr_status upload_rot(int first, int count) {
@@ -969,7 +976,10 @@ template <typename R> struct Engine : EngineBase {
                     if (!ppe) { s = fail(MPSR_MEMERR, "PPE solver allocation failed"); break; }
                 }
                 ppe_start(ppe, d, st);
+                nb_epoch = bin_epoch;
                 s = ppe_solve(ppe, d, ppe_opt, 0, st);
+                if (s != MPSR_NOERR)
+                    s = fail(s, std::string("PPE solve failed: ") + (ppe_why(ppe)[0] ? ppe_why(ppe) : "device error"));
                 break;
             case 8: do_gradient(); break;
             case 9: launch_translate(d, st); break;

@@ -104,6 +104,18 @@ __device__ inline bool cell_coords(const Consts<R>& c, R x, R y, R z, int& i, in
 template <typename R> __device__ inline long long cell_of(const Consts<R>& c, int i, int j, int k) {
     return (long long)c.gy * c.gz * i + (long long)c.gz * j + k;
 }
+// RegisterParticleTopology's binning key: the cell of a live particle (id >= 0) whose cell lies
+// inside the grid, else ncell ("not binned", sorted after every cell)
+template <typename R> __device__ inline uint32_t bin_key(const Consts<R>& c, int id, R x, R y, R z) {
+    uint32_t key = (uint32_t)c.ncell;
+    if (id < 0) return key;
+    int i, j, k;
+    if (cell_coords(c, x, y, z, i, j, k)) {
+        const long long m = cell_of(c, i, j, k);
+        if (m >= 0 && m < c.ncell) key = (uint32_t)m;
+    }
+    return key;
+}
 // Utility_Inline.h:1936-1975 IsInclude
 template <typename R> __device__ inline bool include(const Consts<R>& c, R x, R y, R z) {
     if (!isfinite(x) || !isfinite(y) || !isfinite(z)) return false;
@@ -329,9 +341,19 @@ template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st);
 // Dirichlet flag from the same neighbour pass, and its first nb_width Laplacian-radius neighbours
 // and weights (SELL-64/4 by sorted index, row width nb_width)
 template <typename R>
-void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, int* nbq, R* nbw, int nb_width,
-                    hipStream_t st);
-template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st);
+void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, int* nbq, R* nbw, int* nbn,
+                    int nb_width, hipStream_t st);
+// The Laplacian-radius neighbour list the PPE path's PND pass writes after the second binning:
+// per sorted row its first `width` neighbours in the canonical order (SELL-64/4 by sorted index)
+// and n[t] = the number of neighbours (all of them, listed or not), -1 for a row the pass skipped
+struct NbList {
+    const int* q = nullptr;
+    const int* n = nullptr;
+    int width = 0;
+};
+// The corrector; with a neighbour list of the same binning, rows whose neighbours are all listed
+// walk the list instead of the 27-cell stencil (the same neighbours in the same order)
+template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st, NbList nb = NbList{});
 // CalcTemperatureFactor; with step_ctr the launch returns at once unless (*step_base + *step_ctr)
 // is a multiple of thermal_step (graph replays), else the caller decides
 template <typename R> void launch_thermal(const Dev<R>& d, const int* step_base, const int* step_ctr, int thermal_step,

@@ -55,27 +55,18 @@ __global__ __launch_bounds__(kMaxBlock) void k_bin_hash(int N, Consts<R> c, PSta
                                                    int* vals) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= N) return;
-    uint32_t key = (uint32_t)c.ncell;
-    if (in.id[s] < 0) {
+    const int id = in.id[s];
+    const R x = in.px[s], y = in.py[s], z = in.pz[s];
+    if (id < 0 || !include(c, x, y, z)) {
         out.id[s] = -1;
     } else {
-        const R x = in.px[s], y = in.py[s], z = in.pz[s];
-        if (!include(c, x, y, z)) {
-            out.id[s] = -1;
-        } else {
-            out.id[s] = in.id[s];
-            out.px[s] = x; out.py[s] = y; out.pz[s] = z;
-            out.vx[s] = in.vx[s]; out.vy[s] = in.vy[s]; out.vz[s] = in.vz[s];
-            out.p[s] = in.p[s]; out.n[s] = in.n[s]; out.temp[s] = in.temp[s];
-            out.nu[s] = in.nu[s]; out.solid[s] = in.solid[s]; out.type[s] = in.type[s];
-        }
-        int i, j, k;
-        if (cell_coords(c, x, y, z, i, j, k)) {
-            const long long m = cell_of(c, i, j, k);
-            if (m >= 0 && m < c.ncell) key = (uint32_t)m;
-        }
+        out.id[s] = id;
+        out.px[s] = x; out.py[s] = y; out.pz[s] = z;
+        out.vx[s] = in.vx[s]; out.vy[s] = in.vy[s]; out.vz[s] = in.vz[s];
+        out.p[s] = in.p[s]; out.n[s] = in.n[s]; out.temp[s] = in.temp[s];
+        out.nu[s] = in.nu[s]; out.solid[s] = in.solid[s]; out.type[s] = in.type[s];
     }
-    keys[s] = key;
+    keys[s] = bin_key(c, id, x, y, z);
     vals[s] = s;
 }
 
@@ -246,7 +237,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, 
                                               const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
                                               const uint32_t* __restrict__ skeys, int* __restrict__ active,
                                               int* __restrict__ rcnt, double* __restrict__ rdiag, int* __restrict__ rdir,
-                                              int* __restrict__ nbq, R* __restrict__ nbw, int nb_width) {
+                                              int* __restrict__ nbq, R* __restrict__ nbw, int* __restrict__ nbn,
+                                              int nb_width) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
@@ -258,6 +250,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, 
             rcnt[t] = 0;
             rdiag[t] = 1.0;
             rdir[t] = 1;
+            nbn[t] = -1;
         }
     };
     if (!include(c, x, y, z)) {
@@ -342,6 +335,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, 
         const bool act = out.id[s] >= 0 && skeys[t] != (uint32_t)c.ncell && (double)dens > (double)c.beta_n0;
         const bool dr = !(act && wl > 0.0);
         active[t] = act ? 1 : 0;
+        nbn[t] = nl;
         rcnt[t] = dr ? 0 : nl;
         rdiag[t] = dr ? 1.0 : wl;
         rdir[t] = dr ? 1 : 0;
@@ -369,7 +363,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_gradient(int N, Consts<R> c, cons
                                                    const int* __restrict__ stl_id, const int* __restrict__ tstart,
                                                    const int* __restrict__ tcount, const int* __restrict__ tids,
                                                    const TriD<R>* __restrict__ tris,
-                                                   const NegModel<R>* __restrict__ negm) {
+                                                   const NegModel<R>* __restrict__ negm, NbList nb) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
@@ -380,7 +374,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_gradient(int N, Consts<R> c, cons
     R gx = 0, gy = 0, gz = 0, wx = 0, wy = 0, wz = 0;
     const R pi = in.p[s];
     const R re = c.re_grad, mind = c.mind_grad;
-    for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
+    auto body = [&](int q) {
         const R4<R> Q = spos[q];
         R d = dist3(x, y, z, Q.x, Q.y, Q.z);
         if (d < re) {
@@ -391,7 +385,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_gradient(int N, Consts<R> c, cons
             gy = gy + (Q.y - y) * pc;
             gz = gz + (Q.z - z) * pc;
         }
-    });
+    };
+    const int nn = nb.n ? nb.n[t] : -1;
+    if (nn >= 0 && nn <= nb.width) {
+        // every Laplacian-radius neighbour is listed, in the stencil walk's order; the gradient
+        // radius is the smaller one, so the same distances pick the same neighbours in the same order
+        const int lbase = (t >> 6) * 64 * nb.width, lane = t & 63;
+        for (int j = 0; j < nn; j += 4) {
+            int q4[4];
+            quad_load(nb.q + sell_at(lbase, lane, j), q4);
+            const int m = nn - j < 4 ? nn - j : 4;
+            for (int u = 0; u < m; ++u) body(q4[u]);
+        }
+    } else {
+        for_candidates_within(c, cs, ce, x, y, z, t, re, body);
+    }
     gx *= c.c1; gy *= c.c1; gz *= c.c1;
     const int tid = stl_id[t];
     if (tid >= 0 && tid < c.ntri) {
@@ -646,24 +654,24 @@ template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st) {
     hipLaunchKernelGGL(k_relocate<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.in, d.out);
 }
 template <typename R>
-void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, int* nbq, R* nbw, int nb_width,
-                    hipStream_t st) {
+void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, int* nbq, R* nbw, int* nbn,
+                    int nb_width, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL((k_pnd<R, true>), dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, 0, d.perm, d.in,
                        d.out, d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id, d.skeys, active, cnt, diag,
-                       dir, nbq, nbw, nb_width);
+                       dir, nbq, nbw, nbn, nb_width);
 }
 template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL((k_pnd<R, false>), dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, eos, d.perm, d.in, d.out,
                        d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, nullptr, 0);
+                       nullptr, nullptr, nullptr, nullptr, nullptr, 0);
 }
-template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st) {
+template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st, NbList nb) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL(k_gradient<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
                        d.spos, d.svel, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id,
-                       d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.negm);
+                       d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.negm, nb);
 }
 template <typename R>
 void launch_thermal(const Dev<R>& d, const int* step_base, const int* step_ctr, int thermal_step, hipStream_t st) {
@@ -689,8 +697,8 @@ template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int
     template void launch_explicit<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_relocate<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_pnd<R>(const Dev<R>&, int, hipStream_t);                             \
-    template void launch_pnd_ppe<R>(const Dev<R>&, int*, int*, double*, int*, int*, R*, int, hipStream_t); \
-    template void launch_gradient<R>(const Dev<R>&, hipStream_t);                             \
+    template void launch_pnd_ppe<R>(const Dev<R>&, int*, int*, double*, int*, int*, R*, int*, int, hipStream_t); \
+    template void launch_gradient<R>(const Dev<R>&, hipStream_t, NbList);                     \
     template void launch_thermal<R>(const Dev<R>&, const int*, const int*, int, hipStream_t);   \
     template void launch_turbulence<R>(const Dev<R>&, hipStream_t);                           \
     template void launch_neighbor_count<R>(const Dev<R>&, R, int*, hipStream_t);              \

@@ -27,6 +27,8 @@ template <typename R>
 mpsr_status ppe_get_matrix(PpeSolver<R>* s, const Dev<R>& d, int* ptr, int* col, double* val, double* diag, int cap,
                            int* nnz);
 template <typename R> void ppe_update_params(PpeSolver<R>* s, const mpsr_params& P);
+// the neighbour list of the last ppe_start (valid until the next binning)
+template <typename R> NbList ppe_neighbors(const PpeSolver<R>* s);
 // the learned warm start / preconditioner network (parameters flat FP32)
 template <typename R> mpsr_status ppe_mpnn_set(PpeSolver<R>* s, const float* w, hipStream_t st);
 template <typename R> mpsr_status ppe_mpnn_get(PpeSolver<R>* s, float* w, hipStream_t st);

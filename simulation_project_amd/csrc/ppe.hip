@@ -77,6 +77,7 @@ template <typename R> struct PpeSolver {
     int* cnt = nullptr;
     int hint[HINT_PASSES] = {};  // iterations of each refinement pass of the last solve (0: none)
     int* nbq = nullptr;          // [rows * NB_WIDTH] neighbour list of the PND pass (sorted index)
+    int* nbn = nullptr;          // [rows] its neighbour count per row (-1: row skipped)
     R* nbw = nullptr;            // ... and the weights
     int* slice_w = nullptr;
     int* pos = nullptr;          // [positions] SELL position of each row (rows ordered by length per window)
@@ -950,6 +951,24 @@ __global__ __launch_bounds__(64) void k_cell_op(int ncell, AggGeom g, const int*
         if (lane == 13) dinv[I] = sum > 0.0 ? (float)(VC_OMEGA / sum) : 0.0f;
     }
 }
+// k_cell_op sums each row of Ac1 in its own order, so the mirrored entries op[I][b] and
+// op[J][26 - b] (J the neighbour of I in slot b) can differ in the last bit.  Both take the mean of
+// the two, rounded once, so Ac1 and with it the preconditioner are exactly symmetric.  Every
+// mirrored pair is handled by one thread (the one holding its slot below the centre).
+__global__ __launch_bounds__(PB) void k_cell_sym(int ncell, AggGeom g, float* op) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)ncell * 27) return;
+    const int I = (int)(id / 27), b = (int)(id % 27);
+    if (b >= 13) return;
+    const int ci = I / (g.gy * g.gz), cj = (I / g.gz) % g.gy, ck = I % g.gz;
+    const int J = stencil_cell(g, ci, cj, ck, b);
+    if (J < 0) return;
+    float* a = op + (size_t)I * 27 + b;
+    float* c = op + (size_t)J * 27 + (26 - b);
+    const float m = (float)(0.5 * ((double)*a + (double)*c));
+    *a = m;
+    *c = m;
+}
 // rc1 = P1^T r (r - alpha q with upd, as k_zcomp then writes it) and e = omega rc1 / diag Ac1:
 // eight lanes per cell striding over its contiguous rows, summed in a fixed lane order
 template <typename V>
@@ -1140,7 +1159,8 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     const int nb = (rows + PB - 1) / PB;
     bool ok = true;
     ok &= dev_alloc(&s->active, rows) && dev_alloc(&s->dir, rows) && dev_alloc(&s->cnt, rows);
-    ok &= dev_alloc(&s->nbq, (size_t)rows * NB_WIDTH) && dev_alloc(&s->nbw, (size_t)rows * NB_WIDTH);
+    ok &= dev_alloc(&s->nbq, (size_t)rows * NB_WIDTH) && dev_alloc(&s->nbw, (size_t)rows * NB_WIDTH) &&
+          dev_alloc(&s->nbn, rows);
     ok &= dev_alloc(&s->slice_w, s->nslices + 1) && dev_alloc(&s->slice_off, s->nslices + 1) && dev_alloc(&s->total, 1);
     const int positions = ((d.cap + SELL_SIGMA - 1) / SELL_SIGMA) * SELL_SIGMA;
     ok &= dev_alloc(&s->pos, positions) && dev_alloc(&s->rowp, positions);
@@ -1288,7 +1308,10 @@ static int setup_cells(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hipStre
     const int ncell = d.c.ncell;
     if (d.c.gx >= 1024 || d.c.gy >= 1024 || d.c.gz >= 1024) return 0;   // packed coordinates hold 10 bits
     if (!s->c_op) {
-        bool ok = dev_alloc(&s->cxyz, (size_t)s->nslices * 64) && dev_alloc(&s->cell_of_row, (size_t)s->nslices * 64) &&
+        // per-row arrays cover every slot the handle can hold (N grows with bucket inflow and
+        // mpsr_set_particles), as the row buffers of ppe_create do
+        const size_t rows_cap = (size_t)((s->cap + 63) / 64) * 64;
+        bool ok = dev_alloc(&s->cxyz, rows_cap) && dev_alloc(&s->cell_of_row, rows_cap) &&
                   dev_alloc(&s->c_agg, ncell) && dev_alloc(&s->c_op, (size_t)ncell * 27) && dev_alloc(&s->c_dinv, ncell) &&
                   dev_alloc(&s->c_rc, ncell) && dev_alloc(&s->c_e, ncell) && dev_alloc(&s->c_ef, ncell);
         if (!ok) return -1;
@@ -1305,6 +1328,7 @@ static int setup_cells(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hipStre
     if (s->cell_rebuilt) {
         hipLaunchKernelGGL((k_cell_op<VA>), dim3(ncell), dim3(64), 0, st, ncell, g, d.cell_start, d.cell_end, s->dir,
                            s->cnt, s->pos, s->slice_off, s->cols, vals, s->diag, s->cxyz, s->c_op, s->c_dinv);
+        hipLaunchKernelGGL(k_cell_sym, dim3(grid_for((long long)ncell * 27, PB)), dim3(PB), 0, st, ncell, g, s->c_op);
         s->cell_age = 0;
     } else {
         s->cell_age += 1;
@@ -1415,7 +1439,7 @@ template <typename R> void ppe_destroy(PpeSolver<R>* s) {
                   s->diag, s->rhs, s->x, s->prev, s->r, s->z, s->p0, s->p1, s->q, s->rf, s->zf, s->p0f,
                   s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp, s->nnz_dev, s->agg_flag,
                   s->agg_map, s->agg_full, s->agg_of_row, s->nc_dev, s->info_dev, s->Ac, s->Af, s->agg_rng, s->cpart, s->rxyz, s->rc, s->cc,
-                  s->agg_scan.scan_tmp, s->pdiag_buf, s->omega, s->xprev_s, s->xl, s->nbq, s->nbw, s->cxyz,
+                  s->agg_scan.scan_tmp, s->pdiag_buf, s->omega, s->xprev_s, s->xl, s->nbq, s->nbw, s->nbn, s->cxyz,
                   s->cell_of_row, s->c_agg, s->c_op, s->c_dinv, s->c_rc, s->c_e, s->c_ef};
     if (s->net) mpnn_destroy(s->net);
     for (void* p : ps)
@@ -1550,7 +1574,14 @@ template <typename R> static MpnnGeo<R> mpnn_geo(const Dev<R>& d) {
 }
 
 template <typename R> void ppe_start(PpeSolver<R>* s, const Dev<R>& d, hipStream_t st) {
-    launch_pnd_ppe(d, s->active, s->cnt, s->diag, s->dir, s->nbq, s->nbw, NB_WIDTH, st);
+    launch_pnd_ppe(d, s->active, s->cnt, s->diag, s->dir, s->nbq, s->nbw, s->nbn, NB_WIDTH, st);
+}
+template <typename R> NbList ppe_neighbors(const PpeSolver<R>* s) {
+    NbList nb;
+    nb.q = s->nbq;
+    nb.n = s->nbn;
+    nb.width = NB_WIDTH;
+    return nb;
 }
 
 template <typename R>
@@ -1570,6 +1601,7 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     const int nb = (rows + PB - 1) / PB;
     s->mixed = opt.solver == MPSR_PCG_MIXED;
     s->status = 0;
+    s->why[0] = 0;
     // ---- assembly ----
     // active flags, row bounds, diagonal and Dirichlet flags come from the PND pass (ppe_start)
     fill_int(s->nnz_dev, 0, 2, st);
@@ -1641,7 +1673,6 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->iters = 0;
     s->passes = 0;
     s->fallbacks = 0;
-    s->why[0] = 0;
     s->two_level_ok = 0;
     s->coarse_rebuilt = 0;
     s->setup_ms = 0.0;
@@ -1808,6 +1839,12 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     if (!std::isfinite(s->relres)) {
         s->status = 1;
         std::snprintf(s->why, sizeof(s->why), "non-finite residual (r.r = %.3e, b.b = %.3e)", rr, bb);
+    } else if (s->status == 0 && bb > 0 && rr > target) {
+        // the iteration or refinement cap ended the solve above the tolerance
+        s->status = 2;
+        std::snprintf(s->why, sizeof(s->why),
+                      "not converged: relative residual %.3e above %.3e after %d iterations (max_iter %d)",
+                      s->relres, tol, s->iters, maxit);
     }
     // SpMV launch statistics (events already complete: resid_pass synchronised the stream)
     s->spmv_launches = 0;
@@ -1830,6 +1867,12 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
         // per row p and diag read and q written; gathers of p at the columns not counted
         const double sv = s->mixed ? 4.0 : 8.0;
         s->spmv_bytes = (double)s->nnz_real * (sv + 4.0) + (double)N * (2.0 * sv + 8.0);
+    }
+    if (s->status != 0) {
+        // a failed solve leaves the state as it was: no pressure is written back, and the caller's
+        // step stops here with the reason in ppe_why()
+        if (!s->why[0]) std::snprintf(s->why, sizeof(s->why), "PCG breakdown");
+        return MPSR_CALCERR;
     }
     hipLaunchKernelGGL(k_ppe_writeback<R>, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.c, d.perm, s->dir, s->x, d.in,
                        d.out, d.spos, d.spres, s->prev);
@@ -1997,6 +2040,7 @@ namespace mpsr {
     template const char* ppe_why<R>(PpeSolver<R>*);                                                                 \
     template mpsr_status ppe_get<R>(PpeSolver<R>*, const Dev<R>&, double*, double*, int*);                           \
     template void ppe_update_params<R>(PpeSolver<R>*, const mpsr_params&);                                         \
+    template NbList ppe_neighbors<R>(const PpeSolver<R>*);                                                          \
     template mpsr_status ppe_mpnn_set<R>(PpeSolver<R>*, const float*, hipStream_t);                                   \
     template mpsr_status ppe_mpnn_get<R>(PpeSolver<R>*, float*, hipStream_t);                                         \
     template mpsr_status ppe_mpnn_train<R>(PpeSolver<R>*, const Dev<R>&, double, double, int, float*, double*,       \
