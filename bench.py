@@ -1,32 +1,32 @@
 """Benchmark: MPS rotating-drum timestep at 1M particles on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--l0 2.012e-3] [--solver learned|mixed|fp64]
-                    [--precond two|jacobi] [--no-cpu-baseline] [--no-plain]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--arm-steps A] [--l0 2.012e-3]
+                    [--solver learned|mixed|fp64] [--precond vcycle|two|jacobi] [--no-cpu-baseline] [--no-arms]
     python bench.py --workload batched [--buckets 64] [--steps K] [--warmup W] [--no-tune]
     python bench.py --workload precision --l0 1.6e-3 --steps 200
     python bench.py --workload radix [--steps K] [--warmup W]
 
 Workload: the rotating drum (cylinder R = 0.15 m, L = 0.3 m, 9984-triangle STL, 30 rpm, 40 % fill)
 at l0 = 2.012e-3 m: 1,008,472 fluid particles and 2,000 rigid spheres (d = 5 l0, 2500 kg/m3,
-Schiller-Naumann drag), FP32 particle operators, pressure from the pressure
-Poisson equation solved every step to an FP64 relative residual of 1e-8.  A step is one body of
-the reference's CalculateNew loop: two binnings, wall binning, two STL distance passes,
-predictor, PND, PPE assembly + PCG, corrector, two relocations, wall rotation.  Inputs are
-resident in HBM when the timed region starts; inside it only the PCG's convergence flag (once per
-32 iterations) and the per-step timings cross PCIe.
+Schiller-Naumann drag), FP32 particle operators, pressure from the pressure Poisson equation
+solved every step to an FP64 relative residual of 1e-8.  A step is one body of the reference's
+CalculateNew loop: two binnings, wall binning, two STL distance passes, spheres, predictor, PND,
+PPE assembly + PCG, corrector, two relocations, wall rotation.  Inputs are resident in HBM when
+the timed region starts; the K timed steps are one mpsr_step_many call (inside it only the PCG's
+convergence flag crosses PCIe), so the coarse-operator rebuild every 50 solves falls inside it.
 
 Production solver: mixed precision (FP32 matrix and SpMV, FP64 dots and residuals, FP64
-refinement) with the cell V-cycle preconditioner: Jacobi plus one symmetric V-cycle on the
-cell-level Galerkin operator whose coarse level is the dense inverse on 2^3-cell aggregates
-(rebuilt every 50 solves; the JSON line gives the set-up time of one rebuild and its share per
-step), warm-started by the previous pressure.  --solver learned adds the trained network's
-warm start (simulation_project_amd/weights/mpnn_drum.npy).  The plain comparison solves the same
-steps with FP64 Jacobi-PCG from the previous pressure.
+refinement) with the cell V-cycle preconditioner (Jacobi plus one symmetric V-cycle on the
+cell-level Galerkin operator over the aggregates' dense coarse inverse), warm-started by the
+previous pressure.  Three more arms solve the same drum from the same start for A steps each:
+plain FP64 Jacobi-PCG, FP64 PCG with the cell V-cycle, and the mixed solver with the trained
+network's warm start (simulation_project_amd/weights/mpnn_drum.npy).
 
-One JSON line: value = particle-updates/s over the K timed steps; PPE ms/step of both solvers;
-roofline of the dominant kernel (the PCG SpMV) from HIP events around every SpMV launch of the
-timed region, on the solver's stream; a whole-step roofline from the algorithmic bytes of a step;
-the CPU oracle timed on the host cores on one step of the same drum.
+One JSON line: value = particle-updates/s over the K timed steps; PPE ms/step and iterations of
+every arm; roofline of the dominant kernel (the PCG SpMV) from HIP events around every SpMV launch
+of the timed region, on the solver's stream; a whole-step roofline from the algorithmic bytes of a
+step; the CPU oracle timed on the host cores (median of three steady-state steps on all cores, and
+one step on one core).
 """
 import argparse
 import json
@@ -69,17 +69,81 @@ def run_steps(s, first, count):
     return time.perf_counter() - t0, out
 
 
+def run_arm(Solver, abi, case, sph, kind, precond, warm, warmup, steps, weights=None):
+    """One solver arm on a fresh handle: `warmup` untimed steps, then `steps` steps in one
+    mpsr_step_many call timed on the host around a synchronised stream.  Returns the wall-clock
+    figures and the call's statistics (phase times and SpMV launch times from HIP events)."""
+    s = Solver(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE,
+               ppe=ppe_options(abi, kind, precond, warm))
+    if weights is not None:
+        s.mpnn_set_weights(weights)
+    if sph is not None:
+        s.set_spheres(sph)
+    s.set_profiling(True)
+    if warmup:
+        s.step_many(0, warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step_many(warmup, steps)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    t = s.timings()
+    s.close()
+    return elapsed, t
+
+
+def arm_summary(elapsed, t, steps):
+    return {"ms_per_step": round(1e3 * elapsed / steps, 3), "ppe_ms_per_step": round(t["ppe_ms"], 3),
+            "iterations_per_solve": round(t["ppe_iterations_total"] / steps, 2),
+            "coarse_rebuilds": t["ppe_rebuilds"], "rel_residual_max": t["ppe_rel_residual_max"],
+            "steps": steps}
+
+
+def cpu_baseline(abi, case, sph, n):
+    """The CPU oracle (FP32 operators as the reference's Scalar, FP64 Jacobi-PCG to 1e-8) on the same
+    drum: step 0 (PCG from zero) untimed, then three steady-state steps on all host cores (median),
+    then one more step on one core."""
+    from oracle.oracle import Oracle, load as load_oracle
+    lib = load_oracle()
+    o = Oracle(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE,
+               ppe=abi.PpeOptions(abi.MPSR_PCG_FP64, 1e-8, 20000, abi.MPSR_WARM_PREVIOUS, 0.0, 0.0))
+    if sph is not None:
+        o.set_spheres(sph)
+    cores = lib.orc_get_threads()
+    log(f"[bench] CPU oracle: step 0, then steps 1-3 on {cores} threads, step 4 on 1 thread")
+    o.step(0)
+    times = []
+    for k in (1, 2, 3):
+        tc = time.perf_counter()
+        o.step(k)
+        times.append(time.perf_counter() - tc)
+    lib.orc_set_threads(1)
+    tc = time.perf_counter()
+    o.step(4)
+    t1 = time.perf_counter() - tc
+    lib.orc_set_threads(cores)
+    o.close()
+    med = float(np.median(times))
+    return {"value": round(n / med, 1), "unit": "particle-updates/s", "cores": cores, "kind": "port",
+            "sample": f"steps 1-3 of the same {n}-particle drum after an untimed step 0 (FP32 operators, FP64 "
+                      f"Jacobi-PCG to 1e-8), median of {', '.join(f'{x:.2f}' for x in times)} s per step, "
+                      f"OpenMP over particles on {cores} threads",
+            "single_thread": {"value": round(n / t1, 1), "cores": 1, "sample": f"step 4 on one thread, {t1:.2f} s"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--arm-steps", type=int, default=10, help="timed steps of each comparison arm")
     ap.add_argument("--l0", type=float, default=2.012e-3)
     ap.add_argument("--solver", choices=["learned", "mixed", "fp64"], default=None)
     ap.add_argument("--precond", choices=["vcycle", "two", "jacobi"], default="vcycle")
     ap.add_argument("--spheres", type=int, default=2000, help="rigid spheres of diameter 5 l0, density 2500")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 PCG comparison run")
+    ap.add_argument("--no-arms", action="store_true", help="skip the comparison arms (plain FP64, FP64 + V-cycle, "
+                                                           "learned)")
     ap.add_argument("--workload", choices=["drum", "batched", "precision", "radix"], default="drum",
                     help="drum: the 1M-particle drum (default); batched: 64 independent rotating buckets; "
                          "precision: the mixed-precision drum against the FP64 build after --steps steps")
@@ -104,95 +168,59 @@ def main():
     case = cases.drum(lib, l0=args.l0, steps=args.warmup + args.steps)
     n = case.n
     sph = cases.drum_spheres(case, args.spheres) if args.spheres > 0 else None
+    weights = np.load(WEIGHTS).astype(np.float32)
     log(f"[bench] drum l0={args.l0} n={n} triangles={len(case.vertices)} built in {time.time() - t0:.1f}s; "
         f"solver {solver}, preconditioner {args.precond}")
 
     warm = abi.MPSR_WARM_LEARNED if solver == "learned" else abi.MPSR_WARM_PREVIOUS
     kind = "fp64" if solver == "fp64" else "mixed"
-    s = Solver(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE,
-               ppe=ppe_options(abi, kind, args.precond, warm))
-    if solver == "learned":
-        s.mpnn_set_weights(np.load(WEIGHTS).astype(np.float32))
-    if sph is not None:
-        s.set_spheres(sph)
-    s.set_profiling(True)
-    log("[bench] warm-up")
-    if args.warmup:
-        run_steps(s, 0, args.warmup)
-    log(f"[bench] timed region: {args.steps} steps")
-    elapsed, per_step = run_steps(s, args.warmup, args.steps)
+    log(f"[bench] timed region: {args.steps} steps after {args.warmup} warm-up steps")
+    elapsed, t = run_arm(Solver, abi, case, sph, kind, args.precond, warm, args.warmup, args.steps,
+                         weights if solver == "learned" else None)
     value = n * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    ppe_ms = [t["ppe_ms"] for t in per_step]
-    iters = [t["ppe_iterations"] for t in per_step]
-    relres = [t["ppe_rel_residual"] for t in per_step]
-    launches = sum(t["spmv_launches"] for t in per_step)
-    spmv_total_ms = sum(t["spmv_ms"] * t["spmv_launches"] for t in per_step)
-    spmv_avg_ms = spmv_total_ms / max(1, launches)
-    spmv_b = float(np.mean([t["spmv_bytes"] for t in per_step]))
+    iters = t["ppe_iterations_total"] / args.steps
+    launches = t["spmv_launches"]
+    spmv_avg_ms = t["spmv_ms"]
+    spmv_b = t["spmv_bytes"]
     achieved = spmv_b / (spmv_avg_ms * 1e-3) / 1e9 if spmv_avg_ms > 0 else 0.0
-    phases = {k: round(float(np.mean([t[k] for t in per_step])), 4) for k in
+    phases = {k: round(float(t[k]), 4) for k in
               ("bin_ms", "tri_ms", "stl_ms", "explicit_ms", "relocate_ms", "pnd_ms", "ppe_ms", "gradient_ms",
-               "wall_ms", "sphere_ms", "ppe_setup_ms")}
+               "wall_ms", "sphere_ms")}
     # whole-step algorithmic bytes: the particle phases (FP32: 442 B per particle) plus the PPE
     # assembly and every PCG iteration (SpMV values and columns, 19 vector words per row)
-    nnz = float(np.mean([t["ppe_nnz"] for t in per_step]))
-    rows = float(np.mean([t["ppe_rows"] for t in per_step]))
+    nnz = float(t["ppe_nnz"])
+    rows = float(t["ppe_rows"])
     sv, sx = (4.0, 4.0) if kind == "mixed" else (8.0, 8.0)
-    step_bytes = n * 442.0 + rows * 5 * 4 + nnz * (sv + 4) + float(np.mean(iters)) * (nnz * (sv + 4) + rows * 19 * sx)
+    step_bytes = n * 442.0 + rows * 5 * 4 + nnz * (sv + 4) + iters * (nnz * (sv + 4) + rows * 19 * sx)
     step_achieved = step_bytes / (ms_per_step * 1e-3) / 1e9
-    log(f"[bench] {ms_per_step:.2f} ms/step, PPE {np.mean(ppe_ms):.2f} ms, iterations {iters}, "
-        f"SpMV {spmv_avg_ms * 1e3:.1f} us x {launches}")
-    learned_fb = sum(t.get("ppe_learned_fallback", 0) for t in per_step)
-    # the coarse operator is rebuilt every 50 solves (and when the iterations grow by a quarter):
-    # one more step, after the timed region, with a forced rebuild gives its set-up time
-    opt = ppe_options(abi, kind, args.precond, warm)
-    every = opt.coarse_rebuild
-    opt.coarse_rebuild = 1
-    s.set_ppe_options(opt)
-    run_steps(s, args.warmup + args.steps, 1)
-    tr = s.timings()
-    coarse = {"rows": tr["ppe_coarse_rows"], "rebuild_every": every, "setup_ms": round(tr["ppe_setup_ms"], 3),
-              "amortized_ms_per_step": round(tr["ppe_setup_ms"] / every, 3),
-              "rebuilds_in_timed_steps": int(sum(t["ppe_coarse_rebuilt"] for t in per_step))}
-    log(f"[bench] coarse operator: {coarse}")
-    s.close()
+    main_arm = arm_summary(elapsed, t, args.steps)
+    log(f"[bench] {ms_per_step:.2f} ms/step, PPE {t['ppe_ms']:.2f} ms, {iters:.1f} iterations per solve, "
+        f"{t['ppe_rebuilds']} coarse rebuilds, SpMV {spmv_avg_ms * 1e3:.1f} us x {launches}")
 
-    plain = None
-    if not args.no_plain:
-        log("[bench] plain FP64 Jacobi-PCG, same steps")
-        p = Solver(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE,
-                   ppe=ppe_options(abi, "fp64", "jacobi", abi.MPSR_WARM_PREVIOUS))
-        if sph is not None:
-            p.set_spheres(sph)
-        p.set_profiling(True)
-        if args.warmup:
-            run_steps(p, 0, args.warmup)
-        pe, pt = run_steps(p, args.warmup, args.steps)
-        p.close()
-        pm = float(np.mean([t["ppe_ms"] for t in pt]))
-        plain = {"ppe_ms_per_step": round(pm, 3), "iterations": [t["ppe_iterations"] for t in pt],
-                 "ms_per_step": round(1e3 * pe / args.steps, 3),
-                 "ppe_speedup": round(pm / float(np.mean(ppe_ms)), 3)}
-        log(f"[bench] plain: {plain}")
+    arms = {f"{kind}_{args.precond}" + ("_learned" if solver == "learned" else ""): main_arm}
+    if not args.no_arms:
+        for name, k2, pre2, warm2, w2 in (("plain_fp64_jacobi", "fp64", "jacobi", abi.MPSR_WARM_PREVIOUS, None),
+                                          ("fp64_vcycle", "fp64", "vcycle", abi.MPSR_WARM_PREVIOUS, None),
+                                          ("mixed_vcycle", "mixed", "vcycle", abi.MPSR_WARM_PREVIOUS, None),
+                                          ("learned_mixed_vcycle", "mixed", "vcycle", abi.MPSR_WARM_LEARNED, weights)):
+            if name in arms:
+                continue
+            log(f"[bench] arm {name}: {args.warmup} + {args.arm_steps} steps")
+            e2, t2 = run_arm(Solver, abi, case, sph, k2, pre2, warm2, args.warmup, args.arm_steps, w2)
+            arms[name] = arm_summary(e2, t2, args.arm_steps)
+            if warm2 == abi.MPSR_WARM_LEARNED:
+                arms[name]["learned_fallbacks_last_step"] = t2["ppe_learned_fallback"]
+            log(f"[bench] arm {name}: {arms[name]}")
+    learned_vs_plain = None
+    if "learned_mixed_vcycle" in arms and "plain_fp64_jacobi" in arms:
+        learned_vs_plain = round(arms["plain_fp64_jacobi"]["ppe_ms_per_step"] /
+                                 arms["learned_mixed_vcycle"]["ppe_ms_per_step"], 3)
 
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            from oracle.oracle import Oracle
-            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-            o = Oracle(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE,
-                       ppe=abi.PpeOptions(abi.MPSR_PCG_FP64, 1e-8, 20000, abi.MPSR_WARM_PREVIOUS, 0.0, 0.0))
-            if sph is not None:
-                o.set_spheres(sph)
-            log(f"[bench] CPU oracle: 1 step of the same drum on {threads} threads")
-            tc = time.perf_counter()
-            o.step_many(0, 1)
-            tcpu = time.perf_counter() - tc
-            cpu = {"value": round(n / tcpu, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
-                   "sample": f"step 0 of the same {n}-particle drum (FP32 operators, FP64 Jacobi-PCG to 1e-8), "
-                             f"{tcpu:.1f} s, OpenMP over particles"}
-            o.close()
+            cpu = cpu_baseline(abi, case, sph, n)
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] CPU baseline skipped: {e}")
 
@@ -213,14 +241,15 @@ def main():
                                f"40% fill, {0 if sph is None else len(sph.position)} rigid spheres (d=5 l0, 2500 kg/m3, "
                                f"Schiller-Naumann drag), PPE to 1e-8 ({solver}, {args.precond} preconditioner)",
                    "particles": n, "triangles": len(case.vertices), "spheres": 0 if sph is None else len(sph.position),
-                   "dt": case.params.Dt},
-        "ppe_ms_per_step": round(float(np.mean(ppe_ms)), 3),
-        "ppe_iterations": iters,
-        "ppe_rel_residual_max": max(relres),
-        "ppe_learned_fallbacks": learned_fb,
-        "ppe_entries": {"real": int(nnz), "stored": int(np.mean([t["ppe_stored"] for t in per_step]))},
-        "ppe_coarse": coarse,
-        "plain_fp64_pcg": plain,
+                   "dt": case.params.Dt,
+                   "ppe_system": "compression-only source, gamma = 0.1 of the density error per step, Neumann walls"},
+        "ppe_ms_per_step": round(float(t["ppe_ms"]), 3),
+        "ppe_iterations_per_solve": round(iters, 2),
+        "ppe_rel_residual_max": t["ppe_rel_residual_max"],
+        "ppe_coarse_rebuilds_in_timed_steps": t["ppe_rebuilds"],
+        "ppe_entries": {"real": int(nnz), "stored": int(t["ppe_stored"])},
+        "ppe_arms": arms,
+        "ppe_learned_vs_plain_speedup": learned_vs_plain,
         "phase_ms": phases,
         "step_roofline": {"achieved": round(step_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(step_achieved / HBM_PEAK_GBS, 4), "bytes_per_step": step_bytes},
@@ -378,10 +407,13 @@ def radix(args):
 def precision(args):
     """The mixed-precision drum (FP32 particle operators and SpMV, FP64 dots, residuals and
     refinement, cell V-cycle preconditioner) against the FP64 build (FP64 operators, FP64 PCG with
-    the same preconditioner, 1e-8) from the same start, both on the GPU, after --steps steps:
-    relative L2 and max errors of pressure, velocity and position over the particles alive in
-    both, and the bulk kinetic energy and mean pressure.  The FP64 build is the stand-in for the
-    FP64 oracle at this size (it equals the oracle bit for bit in every particle operator)."""
+    the same preconditioner, 1e-8) from the same start, both on the GPU: relative L2 and max errors
+    of pressure, velocity and position over the particles alive in both, the bulk kinetic energy and
+    mean pressure, after every 100 steps up to --steps.  A control run of the FP64 build whose
+    initial positions are moved by one float ulp (toward +infinity) measures how far trajectories
+    that start one rounding apart drift in the same number of steps.  The FP64 build is the
+    stand-in for the FP64 oracle at this size (it equals the oracle bit for bit in every particle
+    operator)."""
     from simulation_project_amd import Solver, abi, cases, load_library
     lib = load_library()
     t0 = time.time()
@@ -389,57 +421,77 @@ def precision(args):
     n = case.n
     sph = cases.drum_spheres(case, args.spheres) if args.spheres > 0 else None
     log(f"[bench] precision: drum l0={args.l0} n={n}, {args.steps} steps, built in {time.time() - t0:.1f}s")
+    marks = [k for k in range(100, args.steps + 1, 100)] or [args.steps]
+    if marks[-1] != args.steps:
+        marks.append(args.steps)
+    perturbed = cases.drum(lib, l0=args.l0, steps=args.steps)
+    pos32 = perturbed.particles.position.astype(np.float32)
+    perturbed.particles.position[:] = np.nextafter(pos32, np.float32(np.inf)).astype(np.float64)
     res = {}
-    for name, prec, kind in (("mixed", abi.MPSR_FP32, "mixed"), ("fp64", abi.MPSR_FP64, "fp64")):
-        s = Solver(case, precision=prec, pressure_mode=abi.MPSR_PRESSURE_PPE,
+    for name, c, prec, kind in (("mixed", case, abi.MPSR_FP32, "mixed"), ("fp64", case, abi.MPSR_FP64, "fp64"),
+                                ("fp64_ulp", perturbed, abi.MPSR_FP64, "fp64")):
+        s = Solver(c, precision=prec, pressure_mode=abi.MPSR_PRESSURE_PPE,
                    ppe=ppe_options(abi, kind, "vcycle", abi.MPSR_WARM_PREVIOUS))
         if sph is not None:
             s.set_spheres(sph)
         s.synchronize()
         tt = time.perf_counter()
-        for k0 in range(0, args.steps, 50):
-            s.step_many(k0, min(50, args.steps - k0))
-            log(f"[bench] {name}: {min(k0 + 50, args.steps)} steps")
+        snaps = {}
+        done = 0
+        for m in marks:
+            while done < m:
+                k = min(50, m - done)
+                s.step_many(done, k)
+                done += k
+                log(f"[bench] {name}: {done} steps")
+            pa = s.get_particles(0)
+            snaps[m] = {"p": pa.pressure.copy(), "v": pa.velocity.copy(), "x": pa.position.copy(), "id": pa.id.copy()}
         s.synchronize()
         el = time.perf_counter() - tt
-        pa = s.get_particles(0)
         tl = s.timings()
-        res[name] = {"ms_per_step": 1e3 * el / args.steps, "p": pa.pressure.copy(), "v": pa.velocity.copy(),
-                     "x": pa.position.copy(), "id": pa.id.copy(),
+        res[name] = {"ms_per_step": 1e3 * el / args.steps, "snaps": snaps,
                      "last_solve": {"iterations": tl["ppe_iterations"], "coarse_rows": tl["ppe_coarse_rows"],
                                     "rel_residual": tl["ppe_rel_residual"]}}
         s.close()
-    a, b = res["mixed"], res["fp64"]
-    alive = (a["id"] >= 0) & (b["id"] >= 0)
-    def rel(x, y):
-        return float(np.linalg.norm((x - y)[alive]) / max(np.linalg.norm(y[alive]), 1e-300))
-    dv = np.linalg.norm((a["v"] - b["v"])[alive], axis=1)
-    dx = np.linalg.norm((a["x"] - b["x"])[alive], axis=1)
-    ke_a = 0.5 * float(np.sum(a["v"][alive] ** 2))
-    ke_b = 0.5 * float(np.sum(b["v"][alive] ** 2))
+
+    def compare(a, b):
+        alive = (a["id"] >= 0) & (b["id"] >= 0)
+
+        def rel(x, y):
+            return float(np.linalg.norm((x - y)[alive]) / max(np.linalg.norm(y[alive]), 1e-300))
+        dv = np.linalg.norm((a["v"] - b["v"])[alive], axis=1)
+        dx = np.linalg.norm((a["x"] - b["x"])[alive], axis=1)
+        ke_a = 0.5 * float(np.sum(a["v"][alive] ** 2))
+        ke_b = 0.5 * float(np.sum(b["v"][alive] ** 2))
+        return {"pressure_rel_l2": rel(a["p"], b["p"]),
+                "pressure_max_abs_over_max": float(np.abs(a["p"] - b["p"])[alive].max() / np.abs(b["p"][alive]).max()),
+                "velocity_rel_l2": rel(a["v"], b["v"]),
+                "velocity_max_abs": float(dv.max()),
+                "position_rms_over_l0": float(np.sqrt(np.mean(dx ** 2)) / args.l0),
+                "position_max_over_l0": float(dx.max() / args.l0),
+                "kinetic_energy_rel": abs(ke_a - ke_b) / ke_b,
+                "mean_pressure_rel": abs(float(a["p"][alive].mean()) - float(b["p"][alive].mean())) /
+                                     abs(float(b["p"][alive].mean())),
+                "alive_a": int((a["id"] >= 0).sum()), "alive_b": int((b["id"] >= 0).sum())}
+
+    errors = {str(m): {"mixed_vs_fp64": compare(res["mixed"]["snaps"][m], res["fp64"]["snaps"][m]),
+                       "fp64_ulp_vs_fp64": compare(res["fp64_ulp"]["snaps"][m], res["fp64"]["snaps"][m])}
+              for m in marks}
+    last = errors[str(marks[-1])]["mixed_vs_fp64"]
     out = {
         "metric": f"mixed-precision drum vs FP64 build after {args.steps} steps",
-        "value": rel(a["p"], b["p"]),
+        "value": last["pressure_rel_l2"],
         "unit": "relative L2 error of pressure",
         "n_gpus": 1, "steps": args.steps, "warmup": 0,
-        "ms_per_step": round(a["ms_per_step"], 3), "higher_is_better": False, "scaling": "weak",
+        "ms_per_step": round(res["mixed"]["ms_per_step"], 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"rotating drum, {n} particles (l0={args.l0} m), "
                                f"{0 if sph is None else len(sph.position)} spheres, PPE to 1e-8, "
                                f"mixed (FP32 operators and SpMV, FP64 dots and refinement) vs FP64",
                    "particles": n},
-        "errors": {"pressure_rel_l2": rel(a["p"], b["p"]),
-                   "pressure_max_abs_over_max": float(np.abs(a["p"] - b["p"])[alive].max() / np.abs(b["p"][alive]).max()),
-                   "velocity_rel_l2": rel(a["v"], b["v"]),
-                   "velocity_max_abs": float(dv.max()),
-                   "position_rms_over_l0": float(np.sqrt(np.mean(dx ** 2)) / args.l0),
-                   "position_max_over_l0": float(dx.max() / args.l0),
-                   "kinetic_energy_rel": abs(ke_a - ke_b) / ke_b,
-                   "mean_pressure_rel": abs(float(a["p"][alive].mean()) - float(b["p"][alive].mean())) /
-                                        abs(float(b["p"][alive].mean())),
-                   "alive_mixed": int((a["id"] >= 0).sum()), "alive_fp64": int((b["id"] >= 0).sum())},
-        "fp64_ms_per_step": round(b["ms_per_step"], 3),
-        "last_solve": {"mixed": a["last_solve"], "fp64": b["last_solve"]},
+        "errors": errors,
+        "fp64_ms_per_step": round(res["fp64"]["ms_per_step"], 3),
+        "last_solve": {k: res[k]["last_solve"] for k in res},
     }
     print(json.dumps(out), flush=True)
 

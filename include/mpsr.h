@@ -281,9 +281,9 @@ typedef struct mpsr_timings {
     int ppe_refinements;
     double ppe_rel_residual; /* FP64 ||b - A x|| / ||b|| after the last solve */
     double ppe_initial_rel_residual;
-    /* PCG SpMV launches of the last step (profiling on): mean duration from HIP events around
-     * each launch, launch count, and algorithmic bytes per launch (matrix entries and the
-     * row-wise vectors once; neighbour gathers not counted). */
+    /* PCG SpMV launches of the last call (profiling on): mean duration from HIP events around
+     * each launch, launch count, and algorithmic bytes per launch of its last solve (matrix
+     * entries and the row-wise vectors once; neighbour gathers not counted). */
     double spmv_ms;
     int spmv_launches;
     double spmv_bytes;
@@ -299,6 +299,12 @@ typedef struct mpsr_timings {
     long long ppe_stored;    /* stored entries of the last PPE matrix (SELL padding included) */
     double thermal_ms;       /* CalcTemperatureFactor (profiling on) */
     double turbulence_ms;    /* CalcTurbulenceViscosity (profiling on) */
+    /* over all steps of the last mpsr_step / mpsr_step_many call (the fields above that say "the
+     * last" describe its last step; the SpMV launch statistics above cover the whole call) */
+    int steps;                    /* steps of the call */
+    long long ppe_iterations_total; /* PCG iterations summed over its solves */
+    int ppe_rebuilds;             /* coarse-operator rebuilds among its solves */
+    double ppe_rel_residual_max;  /* the largest final FP64 relative residual of its solves */
 } mpsr_timings;
 
 typedef struct mpsr_handle mpsr_handle;
@@ -409,6 +415,7 @@ mpsr_status mpsr_set_ppe_options(mpsr_handle *h, const mpsr_ppe_options *opt);
  * cellEnd ([cells] each, -1 = empty), as radix DeviceProcess_Kernel.h:39-124. */
 mpsr_status mpsr_get_binning(mpsr_handle *h, int *sorted_keys, int *perm, int *cell_start,
                              int *cell_end);
+
 /* Triangle lists per cell after mpsr_register_triangle_topology: start/count per cell and the
  * ascending triangle ids (capacity cap; *total receives the entry count). */
 mpsr_status mpsr_get_triangle_cells(mpsr_handle *h, int *cell_start, int *cell_count,
@@ -567,6 +574,23 @@ int mpsr_deck_model_offsets(const mpsr_deck *d, mpsr_float3 *out, int cap);
  * rate, type), time, model count, per model position and rotation angle. */
 mpsr_status mpsr_write_frame(const char *path, int ascii, const mpsr_particles *p, float time,
                              int nmodels, const mpsr_float3 *model_pos, const float *angles);
+
+/* One record of the time-step schedule file (DeltaTime, Source/DtFrame.cpp:11-35): from step
+ * `step` on, Dt, CourantNumber, ParticleGenerationStep and OutputStep take these values. */
+typedef struct mpsr_dt_entry {
+    int step;
+    float dt;
+    float courant;
+    int inflow_step;
+    int output_step;
+} mpsr_dt_entry;
+/* DtFrame::Load: reads the records of `path` in file order into out[0 .. cap-1] and their number
+ * into *count (which may exceed cap).  As the reference's stream reading does, a record counts only
+ * when the stream is still good after its last value (a last record ending the file without a line
+ * break is dropped) and the first unreadable value ends the list.  The driver applies one due record
+ * per step, as CMultiGPUHandler::CheckAndUpdateDt (Source/MultiGPUHandler.cpp:1048-1066), through
+ * mpsr_set_dt. */
+mpsr_status mpsr_dt_schedule_load(const char *path, mpsr_dt_entry *out, int cap, int *count);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,7 @@
 #include <algorithm>
 #include <numeric>
 #include <vector>
+#include <omp.h>
 #include <set>
 #include "../include/mpsr.h"
 
@@ -1527,6 +1528,9 @@ void orc_nearest_brute_f64(const mpsr_triangle* T, int nt, const int* skip, cons
         id[i] = bid;
     }
 }
+// OpenMP threads of the oracle's parallel loops (the CPU baseline times 1 and all cores)
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 float orc_powd(float x, float y) { return powd<float>(x, y); }
 double orc_powd64(double x, double y) { return powd<double>(x, y); }
 

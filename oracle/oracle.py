@@ -90,6 +90,8 @@ def load():
         "orc_spheres_phase": (None, [vp]),
         "orc_sn_force": (None, [C.c_double] * 6 + [dp]),
         "orc_nearest_brute_f64": (None, [P(abi.Triangle), C.c_int, ip, dp, C.c_int, dp, dp, ip]),
+        "orc_set_threads": (None, [C.c_int]),
+        "orc_get_threads": (C.c_int, []),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

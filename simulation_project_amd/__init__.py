@@ -64,6 +64,7 @@ def load_library(path=None):
         "mpsr_get_triangles": (C.c_int, [vp, P(abi.Triangle), C.c_int]),
         "mpsr_get_gridbox": (C.c_int, [vp, P(abi.GridBox)]),
         "mpsr_set_dt": (C.c_int, [vp, C.c_float, C.c_float]),
+        "mpsr_dt_schedule_load": (C.c_int, [C.c_char_p, P(abi.DtEntry), C.c_int, P(C.c_int)]),
         "mpsr_get_timings": (C.c_int, [vp, P(abi.Timings)]),
         "mpsr_set_ppe_options": (C.c_int, [vp, P(abi.PpeOptions)]),
         "mpsr_set_profiling": (C.c_int, [vp, C.c_int]),
@@ -119,6 +120,21 @@ def load_library(path=None):
     if path is None:
         _lib = lib
     return lib
+
+
+def load_dt_schedule(path):
+    """The records of a time-step schedule file (DtFrame::Load) as a list of
+    (step, dt, courant, inflow_step, output_step)."""
+    lib = load_library()
+    n = C.c_int()
+    st = lib.mpsr_dt_schedule_load(path.encode(), None, 0, C.byref(n))
+    if st != 0:
+        raise MpsrError(st, "mpsr_dt_schedule_load", lib.mpsr_deck_last_error().decode(errors="replace"))
+    arr = (abi.DtEntry * max(1, n.value))()
+    st = lib.mpsr_dt_schedule_load(path.encode(), arr, n.value, C.byref(n))
+    if st != 0:
+        raise MpsrError(st, "mpsr_dt_schedule_load", lib.mpsr_deck_last_error().decode(errors="replace"))
+    return [(e.step, e.dt, e.courant, e.inflow_step, e.output_step) for e in arr[:n.value]]
 
 
 def _ip(a):

@@ -174,7 +174,9 @@ class Timings(C.Structure):
                 ("ppe_rows", C.c_int), ("ppe_fallbacks", C.c_int),
                 ("ppe_coarse_rows", C.c_int), ("ppe_coarse_rebuilt", C.c_int), ("ppe_setup_ms", C.c_double),
                 ("ppe_learned", C.c_int), ("ppe_learned_fallback", C.c_int), ("sphere_ms", C.c_double),
-                ("ppe_stored", C.c_longlong), ("thermal_ms", C.c_double), ("turbulence_ms", C.c_double)]
+                ("ppe_stored", C.c_longlong), ("thermal_ms", C.c_double), ("turbulence_ms", C.c_double),
+                ("steps", C.c_int), ("ppe_iterations_total", C.c_longlong), ("ppe_rebuilds", C.c_int),
+                ("ppe_rel_residual_max", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -221,3 +223,9 @@ def triangles_array(tris):
     for i, t in enumerate(tris):
         arr[i] = t
     return arr
+
+
+class DtEntry(C.Structure):
+    """mpsr_dt_entry: one record of the time-step schedule (DeltaTime, Source/DtFrame.cpp:11-35)."""
+    _fields_ = [("step", C.c_int), ("dt", C.c_float), ("courant", C.c_float), ("inflow_step", C.c_int),
+                ("output_step", C.c_int)]

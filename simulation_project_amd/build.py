@@ -72,15 +72,18 @@ def build(force=False, verbose=False):
     if failed:
         raise RuntimeError("libmpsr build failed: " + " ".join(failed[0][0]))
     if jobs or force or not os.path.exists(LIB):
-        cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", LIB] + objs + \
+        # link next to the library and rename over it, so a reader never sees a half-written file
+        tmp = LIB + ".tmp"
+        cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + \
             [f"-L{ROCM}/lib", "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{ROCM}/lib"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("libmpsr link failed:\n" + r.stdout + r.stderr)
+        os.replace(tmp, LIB)
     # the command-line driver: host code over the C ABI, linked against the library next to it
     src = os.path.join(CSRC, "mpsr_run.cpp")
     if force or not os.path.exists(RUNNER) or os.path.getmtime(RUNNER) < max(os.path.getmtime(src), os.path.getmtime(LIB)):
-        cmd = ["g++"] + CXX_FLAGS + [src, "-o", RUNNER, f"-L{HERE}", "-lmpsr", "-Wl,-rpath,$ORIGIN",
+        cmd = ["g++"] + CXX_FLAGS + ["-pthread", src, "-o", RUNNER, f"-L{HERE}", "-lmpsr", "-Wl,-rpath,$ORIGIN",
                                      f"-Wl,-rpath-link,{ROCM}/lib", f"-Wl,-rpath,{ROCM}/lib"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

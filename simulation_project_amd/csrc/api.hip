@@ -597,6 +597,9 @@ template <typename R> struct Engine : EngineBase {
     }
 
     // ---- phases -------------------------------------------------------------------------------
+    long long call_iters = 0;          // PCG iterations of the current step / step_many call
+    int call_rebuilds = 0;             // ... its coarse-operator rebuilds
+    double call_relres_max = 0.0;      // ... and its worst final relative residual
     int bin_epoch = 0;                 // binnings so far
     int nb_epoch = -1;                 // the binning the PPE neighbour list was written for
     void do_bin() {
@@ -661,6 +664,12 @@ template <typename R> struct Engine : EngineBase {
             mpsr_status s = ppe_solve(ppe, d, ppe_opt, profiling, st);
             mark_end();
             if (s != MPSR_NOERR) return fail(s, std::string("PPE solve failed: ") + (ppe_why(ppe)[0] ? ppe_why(ppe) : "device error"));
+            int it = 0, rb = 0;
+            double rr = 0.0;
+            ppe_last(ppe, &it, &rr, &rb);
+            call_iters += it;
+            call_rebuilds += rb;
+            call_relres_max = std::max(call_relres_max, rr);
             return s;
         }
         mark_begin(PH_PND);
@@ -726,6 +735,10 @@ template <typename R> struct Engine : EngineBase {
         HIPC(hipSetDevice(device));
         if (count <= 0) return MPSR_NOERR;
         pev_used = 0;
+        call_iters = 0;
+        call_rebuilds = 0;
+        call_relres_max = 0.0;
+        if (ppe) ppe_begin_call(ppe);
         mpsr_status s = upload_rot(first, count);
         if (s != MPSR_NOERR) return s;
         // the host copy above must finish before rot_host is refilled by a later call
@@ -953,6 +966,10 @@ template <typename R> struct Engine : EngineBase {
         t.ppe_setup_ms = tm.ppe_setup_ms;
         t.ppe_learned = tm.ppe_learned;
         t.ppe_learned_fallback = tm.ppe_learned_fallback;
+        t.steps = count;
+        t.ppe_iterations_total = call_iters;
+        t.ppe_rebuilds = call_rebuilds;
+        t.ppe_rel_residual_max = call_relres_max;
         tm = t;
     }
 

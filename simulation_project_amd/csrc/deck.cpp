@@ -6,6 +6,7 @@
 //                      solver/Common/inc/STLSerializer.cpp:226-351 and CalcTriangleNormal, IO.cpp:435-453),
 //                      and the model set-up of CMultiGPUHandler::Run (Src MultiGPUHandler.cpp:640-786)
 //   mpsr_write_frame   IO::SaveAscii / IO::SaveBinary (IO.cpp:635-742)
+//   mpsr_dt_schedule_load  DtFrame::Load (Src DtFrame.cpp:11-35), the time-step schedule file
 //
 // Paths: the reference joins the configuration's input directory and file names with '\'.  Here
 // the directory is tried first; when it does not exist (decks written on another machine), the
@@ -545,6 +546,25 @@ int mpsr_deck_model_offsets(const mpsr_deck* d, mpsr_float3* out, int cap) {
     if (!d) return 0;
     for (int i = 0; i < (int)d->model_offset.size() && i < cap; ++i) out[i] = d->model_offset[i];
     return (int)d->model_offset.size();
+}
+
+mpsr_status mpsr_dt_schedule_load(const char* path, mpsr_dt_entry* out, int cap, int* count) {
+    if (!path || !count || cap < 0 || (cap > 0 && !out)) return deck_fail(MPSR_PARAMERR, "null argument");
+    std::ifstream in(path);
+    if (!in.is_open()) return deck_fail(MPSR_FILEERR, std::string("cannot open dt schedule ") + path);
+    // records of five whitespace-separated values; a record counts only when the stream is still
+    // good after its last value (a last record that ends the file without a line break is dropped,
+    // and the first unreadable value ends the list)
+    int n = 0;
+    while (in.good()) {
+        mpsr_dt_entry e{};
+        in >> e.step >> e.dt >> e.courant >> e.inflow_step >> e.output_step;
+        if (!in.good()) break;
+        if (n < cap) out[n] = e;
+        ++n;
+    }
+    *count = n;
+    return MPSR_NOERR;
 }
 
 mpsr_status mpsr_write_frame(const char* path, int ascii, const mpsr_particles* p, float time, int nmodels,

@@ -9,6 +9,7 @@
 // Message aggregation (SpMM over the SELL matrix, width 32): half a wave per row, four
 // neighbours in flight, 8-byte BF16 loads; features: half a wave per row over its entries.
 // Loss and gradient of the outputs use FP64 SELL matrix-vector products.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -268,6 +269,115 @@ __global__ __launch_bounds__(256) void k_nn_gemm(int rows, const uint16_t* __res
             if (dir && dir[row]) v = 0.f;
             if (Cb) Cb[(size_t)row * ldc + n] = f2bf(v);
             if (Cf) Cf[(size_t)row * ldc + n] = v;
+        }
+    }
+}
+
+// The same product on gfx950's v_mfma_f32_16x16x32_bf16 (32 K values per instruction).  A block
+// of four waves stages B^T once in LDS and then walks 64-row tiles (grid-stride): each tile's
+// [A1 | A2] rows come in with 16-byte coalesced loads, zero-padded to a multiple of 32 in K, every
+// LDS row 8 BF16 longer than that so the 16 rows a wave reads at once fall in distinct banks.
+// Lane l holds row l % 16 and K values 8 (l / 16) .. + 7 of its A and B operands; the accumulator
+// holds rows 4 (l / 16) .. + 3 of column l % 16.  BF16 results go back through the wave's own LDS
+// rows (whose stride also fits a row of 16 NT results) and leave in 16-byte stores.  LDS: 16 NT
+// (Kp + 8) + 64 max(Kp + 8, 16 NT + 8) BF16, 63 KB at NT = 8, K = 160.
+static constexpr int MM_ROWS = 64;    // rows per tile (4 waves of 16)
+static constexpr int MM_BLOCKS = 1024;
+typedef __bf16 nn_bf8 __attribute__((ext_vector_type(8)));
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_nn_mm(int rows, const uint16_t* __restrict__ A1, int lda1, int K1,
+                                                const uint16_t* __restrict__ A2, int lda2, int K2,
+                                                const uint16_t* __restrict__ Bt, const float* __restrict__ bias,
+                                                int relu, const int* __restrict__ dir,
+                                                const uint16_t* __restrict__ mask, int ldm, int fo_store,
+                                                uint16_t* Cb, float* Cf, int ldc) {
+    extern __shared__ __align__(16) uint16_t lds[];
+    const int K = K1 + K2;
+    const int Kp = (K + 31) & ~31;     // K padded to the instruction's 32
+    const int ST = Kp + 8;             // LDS row stride of B^T (BF16)
+    const int FO = 16 * NT;
+    const int SA = ST > FO + 8 ? ST : FO + 8;   // row stride of the A tile, which also holds a row of results
+    uint16_t* sB = lds;                // [FO][ST]
+    uint16_t* sA = lds + FO * ST;      // [MM_ROWS][SA]
+    const int kc = Kp / 8;
+    // B^T: rows of K BF16, zero padding to Kp (8-BF16 chunks; K is a multiple of 8)
+    for (int i = threadIdx.x; i < FO * kc; i += blockDim.x) {
+        const int n = i / kc, c = i % kc;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (8 * c < K) v = *reinterpret_cast<const uint4*>(Bt + (size_t)n * K + 8 * c);
+        *reinterpret_cast<uint4*>(sB + n * ST + 8 * c) = v;
+    }
+    const int lane = lane_id(), w = (int)(threadIdx.x >> 6);
+    const int lr = lane & 15, kg = 8 * (lane >> 4), rq = 4 * (lane >> 4);
+    const int tiles = (rows + MM_ROWS - 1) / MM_ROWS;
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int row0 = tile * MM_ROWS;
+        __syncthreads();   // the last tile's results have left sA
+        for (int i = threadIdx.x; i < MM_ROWS * kc; i += blockDim.x) {
+            const int r = i / kc, k = 8 * (i % kc);
+            const int row = row0 + r;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (row < rows && k < K)
+                v = k < K1 ? *reinterpret_cast<const uint4*>(A1 + (size_t)row * lda1 + k)
+                           : *reinterpret_cast<const uint4*>(A2 + (size_t)row * lda2 + (k - K1));
+            *reinterpret_cast<uint4*>(sA + r * SA + k) = v;
+        }
+        __syncthreads();
+        nn_f4 acc[NT];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[j] = nn_f4{0.f, 0.f, 0.f, 0.f};
+        const uint16_t* arow = sA + (16 * w + lr) * SA + kg;
+        for (int kk = 0; kk < Kp; kk += 32) {
+            const nn_bf8 a = *reinterpret_cast<const nn_bf8*>(arow + kk);
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const nn_bf8 b = *reinterpret_cast<const nn_bf8*>(sB + (16 * j + lr) * ST + kg + kk);
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+            }
+        }
+        auto value = [&](int j, int i, int& row, int& n) -> float {
+            n = 16 * j + lr;
+            row = row0 + 16 * w + rq + i;
+            float v = acc[j][i] + (bias ? bias[n] : 0.f);
+            if (relu && v < 0.f) v = 0.f;
+            if (row < rows) {
+                if (mask && bf2f(mask[(size_t)row * ldm + n]) <= 0.f) v = 0.f;
+                if (dir && dir[row]) v = 0.f;
+            }
+            return v;
+        };
+        if (Cf) {
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                for (int i = 0; i < 4; ++i) {
+                    int row, n;
+                    const float v = value(j, i, row, n);
+                    if (row < rows && n < fo_store) Cf[(size_t)row * ldc + n] = v;
+                }
+        }
+        if (Cb) {
+            // the wave's 16 rows of sA are its own: they take its BF16 results
+            uint16_t* out = sA + (16 * w) * SA;
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                for (int i = 0; i < 4; ++i) {
+                    int row, n;
+                    const float v = value(j, i, row, n);
+                    out[(rq + i) * SA + n] = f2bf(v);
+                }
+            // the 16-bit writes above reach LDS before the 16-byte reads below (a barrier also for
+            // the compiler, which could otherwise move loads of another type ahead of the stores)
+            __syncthreads();
+            // 16 rows of fo_store BF16 (a multiple of 8) in 16-byte chunks
+            const int cpr = fo_store / 8;
+            for (int i = lane; i < 16 * cpr; i += 64) {
+                const int r = i / cpr, c = i % cpr;
+                const int row = row0 + 16 * w + r;
+                if (row < rows)
+                    *reinterpret_cast<uint4*>(Cb + (size_t)row * ldc + 8 * c) =
+                        *reinterpret_cast<const uint4*>(out + r * SA + 8 * c);
+            }
         }
     }
 }
@@ -688,9 +798,13 @@ static void gemm(int rows, const uint16_t* A1, int lda1, int K1, const uint16_t*
                  const uint16_t* Bt, const float* bias, int relu, const int* dir, const uint16_t* mask, int ldm,
                  int fo_store, uint16_t* Cb, float* Cf, int ldc, hipStream_t st) {
     if (rows <= 0) return;
-    const size_t lds = (size_t)16 * NT * (K1 + K2) * sizeof(uint16_t);
-    hipLaunchKernelGGL(k_nn_gemm<NT>, dim3(grid_for(rows, 64)), dim3(256), lds, st, rows, A1, lda1, K1, A2, lda2, K2,
-                       Bt, bias, relu, dir, mask, ldm, fo_store, Cb, Cf, ldc);
+    // every operand width here is a multiple of 8 BF16 (16 bytes), as the 16-byte staging needs
+    const int Kp = ((K1 + K2) + 31) & ~31;
+    const int ST = Kp + 8, SA = std::max(ST, 16 * NT + 8);
+    const size_t lds = ((size_t)16 * NT * ST + (size_t)MM_ROWS * SA) * sizeof(uint16_t);
+    const int tiles = (int)grid_for(rows, MM_ROWS);
+    hipLaunchKernelGGL(k_nn_mm<NT>, dim3(std::min(tiles, MM_BLOCKS)), dim3(256), lds, st, rows, A1, lda1, K1, A2,
+                       lda2, K2, Bt, bias, relu, dir, mask, ldm, fo_store, Cb, Cf, ldc);
 }
 
 static void agg(Mpnn* n, const MpnnSys& sys, const void* in, bool in_bf16, int mode, uint16_t* out, hipStream_t st) {

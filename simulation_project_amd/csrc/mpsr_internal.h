@@ -227,6 +227,19 @@ __device__ inline void quad_load(const double* p, double* o) {
     const v2d b = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p + 2));
     o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
 }
+// four consecutive 16-bit matrix column codes (8-byte aligned), loaded non-temporally
+__device__ inline void quad_load(const uint16_t* p, int* o) {
+    const unsigned long long v = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(p));
+    o[0] = (int)(v & 0xffffu);
+    o[1] = (int)((v >> 16) & 0xffffu);
+    o[2] = (int)((v >> 32) & 0xffffu);
+    o[3] = (int)(v >> 48);
+}
+__device__ inline void quad_store(uint16_t* p, int a, int b, int c, int d) {
+    const unsigned long long v = (unsigned long long)(uint16_t)a | ((unsigned long long)(uint16_t)b << 16) |
+                                 ((unsigned long long)(uint16_t)c << 32) | ((unsigned long long)(uint16_t)d << 48);
+    *reinterpret_cast<unsigned long long*>(p) = v;
+}
 __device__ inline void quad_store(int* p, int a, int b, int c, int d) {
     v4i v; v.x = a; v.y = b; v.z = c; v.w = d;
     *reinterpret_cast<v4i*>(p) = v;

@@ -1,7 +1,7 @@
 // mpsr_run -- command-line driver of one deck on one GPU, through the C ABI only.
 //
 //   mpsr_run <configuration file> [--steps K] [--out DIR] [--fp64] [--ppe] [--mixed] [--two-level]
-//            [--device D] [--no-output] [--quiet]
+//            [--dt FILE] [--device D] [--no-output] [--quiet]
 //
 // Mirrors CMultiGPUHandler::Run + CalculateNew (Src MultiGPUHandler.cpp:334-492, 574-840) for a
 // single section: load the deck, create the domain, register the buckets, then for
@@ -18,9 +18,26 @@
 // (AddToOutputBuffer with Output = true, CGPUCellSet.cpp:1388-1428).  Model positions follow
 // COutputFrame::SetModelPosition (OutputHandler.h:150-166) and the rotation angle is the sum of the
 // per-step angles of SetRotationAngle (OutputHandler.h:168-193) over steps 1..i.
+//
+// --dt FILE is the reference's third argument (Src MultiGPUHandler.cpp:812-826): the time-step
+// schedule (DtFrame::Load, Src DtFrame.cpp:11-35).  Before every step i the next record whose step
+// is <= i is applied, one record per step (CheckAndUpdateDt, MultiGPUHandler.cpp:1048-1066): the
+// new Dt and Courant number go to the device through mpsr_set_dt, as CCellSet::CheckAndUpdateDt
+// uploads them (CGPUCellSet.cpp:1260-1282), and the host copy used for frame times and the model
+// angles and positions changes with them.  (The reference's live loop calls the handler's version,
+// which changes only that host copy and leaves the device at the first Dt.)  Batches of
+// mpsr_step_many end before a step that applies a record.
+//
+// Frames are written by a writer thread (the reference's output thread, OutputHandler.cpp:23-36):
+// the step loop copies the particles to host memory and queues the frame; at most two frames wait,
+// and the loop goes on stepping while the files are written.
 #include <sys/stat.h>
 
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -51,9 +68,74 @@ struct Host {
 int usage() {
     std::fprintf(stderr,
                  "usage: mpsr_run <configuration file> [--steps K] [--out DIR] [--fp64] [--ppe] [--mixed]\n"
-                 "                [--two-level] [--device D] [--no-output] [--quiet]\n");
+                 "                [--two-level] [--dt FILE] [--device D] [--no-output] [--quiet]\n");
     return 2;
 }
+
+// one frame waiting for the writer thread: its own copy of the particles and the model data
+struct FrameJob {
+    std::string path;
+    int ascii = 1;
+    Host data;
+    float time = 0.0f;
+    int nmodels = 0;
+    std::vector<mpsr_float3> model_pos;
+    std::vector<float> angles;
+};
+
+class FrameWriter {
+public:
+    FrameWriter() : th_([this] { run(); }) {}
+    ~FrameWriter() { finish(); }
+    void push(FrameJob&& j) {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return q_.size() < kMaxQueued; });
+        q_.push_back(std::move(j));
+        cv_.notify_all();
+    }
+    // waits for the queued frames; false when one could not be written
+    bool finish() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            done_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+        if (failed_) std::fprintf(stderr, "mpsr_run: frame writer: %s\n", err_.c_str());
+        return !failed_;
+    }
+
+private:
+    static constexpr size_t kMaxQueued = 2;
+    void run() {
+        for (;;) {
+            FrameJob j;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return done_ || !q_.empty(); });
+                if (q_.empty()) return;
+                j = std::move(q_.front());
+                q_.pop_front();
+                cv_.notify_all();
+            }
+            mpsr_particles v = j.data.view();
+            const mpsr_status st = mpsr_write_frame(j.path.c_str(), j.ascii, &v, j.time, j.nmodels, j.model_pos.data(),
+                                                    j.angles.data());
+            if (st != MPSR_NOERR) {
+                std::lock_guard<std::mutex> lk(mu_);
+                failed_ = true;
+                err_ = mpsr_deck_last_error();
+            }
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<FrameJob> q_;
+    bool done_ = false;
+    bool failed_ = false;
+    std::string err_;
+    std::thread th_;
+};
 
 bool check(mpsr_status s, const char* what) {
     if (s == MPSR_NOERR) return true;
@@ -69,9 +151,11 @@ int main(int argc, char** argv) {
     long long steps = -1;
     std::string out_dir;
     int fp64 = 0, ppe = 0, mixed = 0, two_level = 0, device = 0, no_output = 0, quiet = 0;
+    std::string dt_file;
     for (int a = 2; a < argc; ++a) {
         const std::string k = argv[a];
         if (k == "--steps" && a + 1 < argc) steps = std::atoll(argv[++a]);
+        else if (k == "--dt" && a + 1 < argc) dt_file = argv[++a];
         else if (k == "--out" && a + 1 < argc) out_dir = argv[++a];
         else if (k == "--device" && a + 1 < argc) device = std::atoi(argv[++a]);
         else if (k == "--fp64") fp64 = 1;
@@ -100,7 +184,15 @@ int main(int argc, char** argv) {
         info.ppe.precond = two_level ? MPSR_PRECOND_TWO_LEVEL : MPSR_PRECOND_JACOBI;
         info.ppe.coarse_cells = 4;
     }
-    const mpsr_params P = info.params;
+    mpsr_params P = info.params;
+    std::vector<mpsr_dt_entry> dts;
+    if (!dt_file.empty()) {
+        int cnt = 0;
+        if (!check(mpsr_dt_schedule_load(dt_file.c_str(), nullptr, 0, &cnt), "dt schedule")) return 1;
+        dts.resize(cnt);
+        if (cnt > 0 && !check(mpsr_dt_schedule_load(dt_file.c_str(), dts.data(), cnt, &cnt), "dt schedule")) return 1;
+    }
+    size_t next_dt = 0;
     const int interval = std::max(1, mpsr_deck_output_interval(deck));
     const int ascii = mpsr_deck_output_ascii(deck);
     const int start = mpsr_deck_start_step(deck);
@@ -152,11 +244,13 @@ int main(int argc, char** argv) {
         }
     };
     int file_no = start / interval;
-    Host fr;
+    FrameWriter writer;
     auto frame = [&](int i, const char* type, int number) -> bool {
         if (no_output) return true;
         int n = 0;
         if (!check(mpsr_get_particle_count(h, &n), "mpsr_get_particle_count")) return false;
+        FrameJob job;
+        Host& fr = job.data;
         fr.resize(n);
         mpsr_particles v = fr.view();
         if (!check(mpsr_get_particles(h, 1, &v), "mpsr_get_particles")) return false;
@@ -180,9 +274,24 @@ int main(int argc, char** argv) {
         }
         char num[32];
         std::snprintf(num, sizeof(num), number < 10 ? "0%d" : "%d", number);
-        const std::string path = out_dir + "/" + name + "_" + type + "_" + std::to_string(interval) + "_" + num + ".dat";
-        const float time = P.Dt * interval * number;
-        return check(mpsr_write_frame(path.c_str(), ascii, &v, time, nmodels, mp.data(), angle.data()), "mpsr_write_frame");
+        job.path = out_dir + "/" + name + "_" + type + "_" + std::to_string(interval) + "_" + num + ".dat";
+        job.ascii = ascii;
+        job.time = P.Dt * interval * number;
+        job.nmodels = nmodels;
+        job.model_pos = std::move(mp);
+        job.angles.assign(angle.begin(), angle.end());
+        writer.push(std::move(job));
+        return true;
+    };
+    // CheckAndUpdateDt: at most one due record per step
+    auto update_dt = [&](long long i) -> bool {
+        if (next_dt >= dts.size() || dts[next_dt].step < 0 || i < dts[next_dt].step) return true;
+        const mpsr_dt_entry& e = dts[next_dt++];
+        P.Dt = e.dt;
+        P.CourantNumber = e.courant;
+        P.ParticleGenerationStep = e.inflow_step;
+        P.OutputStep = e.output_step;
+        return check(mpsr_set_dt(h, e.dt, e.courant), "mpsr_set_dt");
     };
     auto phase_step = [&](int i) -> bool {
         bool ok = check(mpsr_register_particle_topology(h), "register") &&
@@ -209,13 +318,17 @@ int main(int argc, char** argv) {
     const auto t0 = std::chrono::steady_clock::now();
     long long i = start;
     while (i <= last) {
+        if (!update_dt(i)) return 1;
         if (i % interval == 0) {
             if (!phase_step((int)i)) return 1;
             ++i;
             continue;
         }
         const long long next_out = ((i + interval - 1) / interval) * interval;  // next output step
-        const long long end = std::min(last + 1, next_out);
+        long long end = std::min(last + 1, next_out);
+        // the batch ends before the next step that applies a schedule record
+        if (next_dt < dts.size() && dts[next_dt].step > i) end = std::min(end, (long long)dts[next_dt].step);
+        if (next_dt < dts.size() && dts[next_dt].step >= 0 && dts[next_dt].step <= i) end = i + 1;
         if (!check(mpsr_step_many(h, (int)i, (int)(end - i)), "mpsr_step_many")) return 1;
         for (long long k = i; k < end; ++k)
             if (k != 0) add_rotation((int)k);
@@ -229,6 +342,7 @@ int main(int argc, char** argv) {
         }
     }
     mpsr_synchronize(h);
+    if (!writer.finish()) return 1;
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     int n = 0;
     mpsr_get_particle_count(h, &n);

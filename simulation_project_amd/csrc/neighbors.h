@@ -71,4 +71,35 @@ __device__ inline R dist3(R x, R y, R z, R qx, R qy, R qz) {
     return build_sqrt((x - qx) * (x - qx) + (y - qy) * (y - qy) + (z - qz) * (z - qz));
 }
 
+// for_candidates with the records loaded ahead: every candidate t != self of the 27-cell stencil in
+// the canonical order, as f(t, spos[t]).  Inside a cell the records come in groups of four loads
+// issued together, so four gathers are in flight per lane instead of one; the calls keep the order.
+template <typename R, typename F>
+__device__ inline void for_candidates_rec(const Consts<R>& c, const int* __restrict__ cs, const int* __restrict__ ce,
+                                          const R4<R>* __restrict__ spos, R x, R y, R z, int self, F&& f) {
+    int ci, cj, ck;
+    if (!cell_coords(c, x, y, z, ci, cj, ck)) return;
+    const long long cid = cell_of(c, ci, cj, ck);
+    if (cid < 0 || cid >= c.ncell) return;
+    for (int k = -1; k <= 1; ++k)
+        for (int j = -1; j <= 1; ++j)
+            for (int i = -1; i <= 1; ++i) {
+                const long long m = cell_of(c, ci + i, cj + j, ck + k);
+                if (m < 0 || m >= c.ncell) continue;
+                const int a = cs[m];
+                if (a < 0) continue;
+                const int b = ce[m];
+                int t = a;
+                for (; t + 4 <= b; t += 4) {
+                    const R4<R> Q0 = spos[t], Q1 = spos[t + 1], Q2 = spos[t + 2], Q3 = spos[t + 3];
+                    if (t != self) f(t, Q0);
+                    if (t + 1 != self) f(t + 1, Q1);
+                    if (t + 2 != self) f(t + 2, Q2);
+                    if (t + 3 != self) f(t + 3, Q3);
+                }
+                for (; t < b; ++t)
+                    if (t != self) f(t, spos[t]);
+            }
+}
+
 }  // namespace mpsr

@@ -110,8 +110,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_explicit(int N, Consts<R> c, cons
     const R4<R> V0 = svel[t];
     R lx = 0, ly = 0, lz = 0, ax = 0, ay = 0, az = 0, qx = 0, qy = 0, qz = 0;
     const R re = c.re_lap, mind = c.mind_lap;
-    for_candidates(c, cs, ce, x, y, z, t, [&](int q) {
-        const R4<R> Q = spos[q];
+    for_candidates_rec(c, cs, ce, spos, x, y, z, t, [&](int q, const R4<R>& Q) {
         R d = dist3(x, y, z, Q.x, Q.y, Q.z);
         if (d < re) {
             if (d < mind) d = mind;
@@ -277,8 +276,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, 
         const int lbase = (t >> 6) * 64 * nb_width, lane = t & 63;
         int q0 = 0, q1 = 0, q2 = 0, q3 = 0;
         R w0 = 0, w1 = 0, w2 = 0, w3 = 0;
-        for_candidates(c, cs, ce, x, y, z, t, [&](int q) {
-            const R4<R> Q = spos[q];
+        for_candidates_rec(c, cs, ce, spos, x, y, z, t, [&](int q, const R4<R>& Q) {
             const R d0 = dist3(x, y, z, Q.x, Q.y, Q.z);
             if (d0 < re) {
                 R d = d0;

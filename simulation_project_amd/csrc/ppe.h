@@ -29,6 +29,10 @@ mpsr_status ppe_get_matrix(PpeSolver<R>* s, const Dev<R>& d, int* ptr, int* col,
 template <typename R> void ppe_update_params(PpeSolver<R>* s, const mpsr_params& P);
 // the neighbour list of the last ppe_start (valid until the next binning)
 template <typename R> NbList ppe_neighbors(const PpeSolver<R>* s);
+// SpMV launch timing of a step / step_many call starts afresh
+template <typename R> void ppe_begin_call(PpeSolver<R>* s);
+// the last solve's iterations, final FP64 relative residual and whether it rebuilt the coarse operator
+template <typename R> void ppe_last(const PpeSolver<R>* s, int* iters, double* relres, int* rebuilt);
 // the learned warm start / preconditioner network (parameters flat FP32)
 template <typename R> mpsr_status ppe_mpnn_set(PpeSolver<R>* s, const float* w, hipStream_t st);
 template <typename R> mpsr_status ppe_mpnn_get(PpeSolver<R>* s, float* w, hipStream_t st);

@@ -46,6 +46,7 @@
 //   returns at once after convergence, so the host queues fixed chunks of iterations and looks at
 //   the flag once per chunk.
 #include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -66,6 +67,14 @@ static constexpr int CHUNK_TAIL = 4;    // ... after a first chunk sized from th
 static constexpr int HINT_PASSES = 4;   // refinement passes whose iteration counts are remembered
 static constexpr int SELL_SIGMA = 256;  // rows ordered by stored length inside windows of this many
 static constexpr int NB_WIDTH = 192;    // Laplacian-radius neighbours listed per row by the PND pass
+// MPSR_PPE_COLUMNS=32 keeps the SpMV on the 32-bit columns (for comparing the two forms)
+static bool c16_off() {
+    static const int off = [] {
+        const char* e = std::getenv("MPSR_PPE_COLUMNS");
+        return (e && std::strcmp(e, "32") == 0) ? 1 : 0;
+    }();
+    return off != 0;
+}
 enum Scal { S_RZ, S_PQ, S_ALPHA, S_BETA, S_RR, S_TARGET, S_DONE, S_ITERS, S_MAXIT, S_BB, S_RR0, S_COUNT };
 
 template <typename R> struct PpeSolver {
@@ -85,6 +94,11 @@ template <typename R> struct PpeSolver {
     int* slice_off = nullptr;
     int* total = nullptr;
     int* cols = nullptr;
+    uint16_t* cols16 = nullptr;  // the columns as 16-bit codes for the PCG SpMV (k_ppe_fill explains them)
+    int* first_ge = nullptr;     // [ncell + 1] first sorted row whose cell key is >= the index
+    int c16 = 0;                 // this solve's SpMV reads the 16-bit codes (every column fitted one)
+    const uint32_t* skeys = nullptr;  // this solve's sorted cell keys
+    int gz = 0, gygz = 0;        // cell-key strides of one grid row and one x-slab
     double* vals64 = nullptr;
     float* vals32 = nullptr;
     long long nnz_cap = 0;
@@ -241,6 +255,51 @@ __global__ __launch_bounds__(SELL_BLOCK) void k_sell_order(int N, int nslices, c
     }
 }
 
+// first_ge[K] = the first sorted row whose cell key is >= K (a lower bound per cell, empty cells
+// included), K = 0 .. ncell
+__global__ __launch_bounds__(PB) void k_first_ge(int N, int ncell, const uint32_t* __restrict__ skeys, int* fg) {
+    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K > ncell) return;
+    int lo = 0, hi = N;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)skeys[mid] < K) lo = mid + 1;
+        else hi = mid;
+    }
+    fg[K] = lo;
+}
+
+// 16-bit column codes.  The cells are numbered x-slowest, z-fastest, so the neighbours of a row in
+// cell (i, j, k) that lie in the x-slab i + dx (cells (i + dx, j - 1 .. j + 1, k - 1 .. k + 1))
+// fill the sorted range [first_ge[K + dx gy gz - gz - 1], first_ge[K + dx gy gz + gz + 2]), K the
+// row's key.  A column is coded as (dx + 1) << 14 | (its offset in that range); the SpMV rebuilds
+// it from the row's three range starts.  A column outside its ranges or 16384 or more into one
+// (a cell-id wrap at the grid edge, or very dense cells) marks the matrix wide, and the SpMV then
+// reads the 32-bit columns.
+struct C16Row {
+    int b0, b1, b2, e0, e1, e2;
+};
+__device__ inline bool c16_row(const uint32_t* skeys, const int* fg, int t, int gz, int gygz, int ncell, C16Row& r) {
+    const long long key = (long long)skeys[t];
+    const long long lo = key - gygz - gz - 1, hi = key + gygz + gz + 2;
+    if (lo < 0 || hi > ncell) return false;
+    r.b0 = fg[lo];
+    r.e0 = fg[key - gygz + gz + 2];
+    r.b1 = fg[key - gz - 1];
+    r.e1 = fg[key + gz + 2];
+    r.b2 = fg[key + gygz - gz - 1];
+    r.e2 = fg[hi];
+    return true;
+}
+__device__ inline int c16_code(const C16Row& r, int q, bool& wide) {
+    int g, off;
+    if (q < r.e0) { g = 0; off = q - r.b0; }
+    else if (q < r.e1) { g = 1; off = q - r.b1; }
+    else { g = 2; off = q - r.b2; if (q >= r.e2) wide = true; }
+    if (off < 0 || off >= (1 << 14)) wide = true;
+    return (g << 14) | (off & 0x3fff);
+}
+
 template <typename R, typename VA>
 __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c, const int* __restrict__ active,
                                                   const int* __restrict__ dir, const R4<R>* __restrict__ spos,
@@ -249,7 +308,8 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
                                                   const int* __restrict__ perm, PState<R> out, double alpha, double n0,
                                                   const int* __restrict__ rowp, int* cols, VA* vals, double* rhs,
                                                   int* cnt, int* nnz, const int* __restrict__ nbq,
-                                                  const R* __restrict__ nbw, int nb_width) {
+                                                  const R* __restrict__ nbw, int nb_width, uint16_t* cols16,
+                                                  const uint32_t* __restrict__ skeys, const int* __restrict__ fg) {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     const int slice = P >> 6, lane = lane_id();
     if (slice >= nslices) return;
@@ -257,17 +317,23 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
     const int base = slice_off[slice], width = slice_w[slice] >> 6;
     int k = 0;
     int c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    int h0 = 0, h1 = 0, h2 = 0, h3 = 0;
     VA v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    C16Row cr{};
+    bool wide = false;
     auto push = [&](int col, VA val) {
         c0 = c1; c1 = c2; c2 = c3; c3 = col;
+        h0 = h1; h1 = h2; h2 = h3; h3 = c16_code(cr, col, wide);
         v0 = v1; v1 = v2; v2 = v3; v3 = val;
         ++k;
         if ((k & 3) == 0) {
             const int e = sell_at(base, lane, k - 4);
             quad_store(cols + e, c0, c1, c2, c3);
+            quad_store(cols16 + e, h0, h1, h2, h3);
             quad_store(vals + e, v0, v1, v2, v3);
         }
     };
+    if (t >= 0 && !dir[t] && !c16_row(skeys, fg, t, c.gz, c.gy * c.gz, c.ncell, cr)) wide = true;
     if (t >= 0 && !dir[t]) {
         const int bound = cnt[t];   // Laplacian-radius neighbours, from the PND pass
         if (bound <= nb_width) {
@@ -308,9 +374,11 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
     }
     const int tot = wave_sum(real);
     const int rows = wave_sum((t >= 0 && !dir[t]) ? 1 : 0);
+    const int wides = wave_sum(wide ? 1 : 0);
     if (lane == 0) {
         atomicAdd(&nnz[0], tot);
         atomicAdd(&nnz[1], rows);
+        if (wides) atomicAdd(&nnz[2], wides);
     }
     (void)N;
 }
@@ -422,14 +490,35 @@ __global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restric
 // q = A p (values VA, arithmetic V), partial p.q.  One lane per row of a 64-row slice; the entry
 // loop is unrolled by 8 so every lane keeps 8 column/value loads and then 8 gathers of p in
 // flight (the loop is latency-bound otherwise).  Matrix streams are loaded non-temporally: they
-// are read once per SpMV and would only evict p from L2.
-template <typename VA, typename V>
+// are read once per SpMV and would only evict p from L2.  With C16 the columns come from the
+// 16-bit codes (2 bytes an entry instead of 4) and the row's three x-slab range starts.
+template <bool C16> struct ColSource {
+    const int* cols;
+    const uint16_t* codes;
+    int b0, b1, b2;
+    __device__ inline void load4(int e, int* c) const {
+        if constexpr (C16) {
+            quad_load(codes + e, c);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int g = c[u] >> 14;
+                c[u] = (g == 0 ? b0 : (g == 1 ? b1 : b2)) + (c[u] & 0x3fff);
+            }
+        } else {
+            quad_load(cols + e, c);
+        }
+    }
+};
+
+template <typename VA, typename V, bool C16>
 __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restrict__ rowp,
                                                   const int* __restrict__ cnt, const int* __restrict__ slice_off,
                                                   const int* __restrict__ slice_w, const int* __restrict__ cols,
                                                   const VA* __restrict__ vals, const double* __restrict__ diag,
                                                   const V* __restrict__ p, V* q, const double* __restrict__ scal,
-                                                  double* part) {
+                                                  double* part, const uint16_t* __restrict__ codes,
+                                                  const uint32_t* __restrict__ skeys, const int* __restrict__ fg,
+                                                  int gz, int gygz) {
     if (scal[S_DONE] != 0.0) return;
     __shared__ double lds[PB / 64];
     const int P = blockIdx.x * blockDim.x + threadIdx.x;   // SELL position
@@ -442,6 +531,15 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
         // the row's own entries (rounded up to a group of 4): a lane stops fetching at its row's end,
         // so the slice padding beyond it is not read
         const int base = slice_off[slice], width = min(slice_w[slice] >> 6, (cnt[t] + 3) & ~3);
+        ColSource<C16> src{cols, codes, 0, 0, 0};
+        if constexpr (C16) {
+            if (width > 0) {   // a row with entries lies inside the grid (k_ppe_fill checked its ranges)
+                const int key = (int)skeys[t];
+                src.b0 = fg[key - gygz - gz - 1];
+                src.b1 = fg[key - gz - 1];
+                src.b2 = fg[key + gygz - gz - 1];
+            }
+        }
         int k = 0;
         // FP32 matrix and vectors: 32 and then 16 entries per round (16 or 8 vector loads, then 32
         // or 16 gathers of p in flight); with FP64 the registers of such rounds cut the occupancy,
@@ -453,7 +551,7 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
                 const int e = sell_at(base, lane, k);
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    quad_load(cols + e + 256 * u, c + 4 * u);
+                    src.load4(e + 256 * u, c + 4 * u);
                     quad_load(vals + e + 256 * u, v + 4 * u);
                 }
                 V g[32];
@@ -468,7 +566,7 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
                 const int e = sell_at(base, lane, k);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    quad_load(cols + e + 256 * u, c + 4 * u);
+                    src.load4(e + 256 * u, c + 4 * u);
                     quad_load(vals + e + 256 * u, v + 4 * u);
                 }
                 V g[16];
@@ -482,8 +580,8 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
             int c[8];
             VA v[8];
             const int e = sell_at(base, lane, k);
-            quad_load(cols + e, c);
-            quad_load(cols + e + 256, c + 4);
+            src.load4(e, c);
+            src.load4(e + 256, c + 4);
             quad_load(vals + e, v);
             quad_load(vals + e + 256, v + 4);
             V g[8];
@@ -496,7 +594,7 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
             int c[4];
             VA v[4];
             const int e = sell_at(base, lane, k);
-            quad_load(cols + e, c);
+            src.load4(e, c);
             quad_load(vals + e, v);
             V g[4];
 #pragma unroll
@@ -1160,7 +1258,7 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     bool ok = true;
     ok &= dev_alloc(&s->active, rows) && dev_alloc(&s->dir, rows) && dev_alloc(&s->cnt, rows);
     ok &= dev_alloc(&s->nbq, (size_t)rows * NB_WIDTH) && dev_alloc(&s->nbw, (size_t)rows * NB_WIDTH) &&
-          dev_alloc(&s->nbn, rows);
+          dev_alloc(&s->nbn, rows) && dev_alloc(&s->first_ge, (size_t)d.c.ncell + 1);
     ok &= dev_alloc(&s->slice_w, s->nslices + 1) && dev_alloc(&s->slice_off, s->nslices + 1) && dev_alloc(&s->total, 1);
     const int positions = ((d.cap + SELL_SIGMA - 1) / SELL_SIGMA) * SELL_SIGMA;
     ok &= dev_alloc(&s->pos, positions) && dev_alloc(&s->rowp, positions);
@@ -1173,8 +1271,8 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     s->scan.scan_tmp_cap = (int)scan_tmp_ints(s->nslices + 1);
     ok &= dev_alloc(&s->scan.scan_tmp, s->scan.scan_tmp_cap);
     ok &= hipHostMalloc((void**)&s->scal_h, sizeof(double) * S_COUNT) == hipSuccess;
-    ok &= hipHostMalloc((void**)&s->cnt_h, sizeof(int) * 2) == hipSuccess;
-    ok &= dev_alloc(&s->nnz_dev, 2);
+    ok &= hipHostMalloc((void**)&s->cnt_h, sizeof(int) * 3) == hipSuccess;
+    ok &= dev_alloc(&s->nnz_dev, 3);
     ok &= dev_alloc(&s->pdiag_buf, rows) && dev_alloc(&s->omega, rows) && dev_alloc(&s->xprev_s, rows) &&
           dev_alloc(&s->xl, rows);
     if (ok && opt.warm_start == MPSR_WARM_LEARNED) {
@@ -1336,6 +1434,21 @@ static int setup_cells(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hipStre
     return 1;
 }
 
+// q = A p and the partial p.q, with the 16-bit column codes when every column of the matrix fitted
+// one, else with the 32-bit columns
+template <typename R, typename VA, typename V>
+static void launch_spmv(PpeSolver<R>* s, const VA* vals, const V* p, V* q, hipStream_t st) {
+    const int nb = (s->nslices * 64 + PB - 1) / PB;
+    if (s->c16)
+        hipLaunchKernelGGL((k_pcg_spmv<VA, V, true>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
+                           s->slice_off, s->slice_w, s->cols, vals, s->diag, p, q, s->scal, s->part, s->cols16,
+                           s->skeys, s->first_ge, s->gz, s->gygz);
+    else
+        hipLaunchKernelGGL((k_pcg_spmv<VA, V, false>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
+                           s->slice_off, s->slice_w, s->cols, vals, s->diag, p, q, s->scal, s->part,
+                           (const uint16_t*)nullptr, (const uint32_t*)nullptr, (const int*)nullptr, 0, 0);
+}
+
 // Iterations queued before the next flag read.  Every launch queued after convergence returns at
 // once but still costs its dispatch, so with a hint (the iterations the same pass took in the last
 // solve) the first chunk stops one short of it and the rest come in chunks of CHUNK_TAIL.
@@ -1400,8 +1513,7 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
         const bool timed = s->profile && s->ev_used + 2 * (size_t)len <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
-            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
-                               s->slice_off, s->slice_w, s->cols, vals, s->diag, p, q, s->scal, s->part);
+            launch_spmv<R, VA, V>(s, vals, p, q, st);
             if (timed) {
                 (void)hipEventRecord(s->ev[s->ev_used + 1], st);
                 s->ev_used += 2;
@@ -1436,6 +1548,7 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
 template <typename R> void ppe_destroy(PpeSolver<R>* s) {
     if (!s) return;
     void* ps[] = {s->active, s->dir, s->cnt, s->slice_w, s->slice_off, s->pos, s->rowp, s->total, s->cols, s->vals64, s->vals32,
+                  s->cols16, s->first_ge,
                   s->diag, s->rhs, s->x, s->prev, s->r, s->z, s->p0, s->p1, s->q, s->rf, s->zf, s->p0f,
                   s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp, s->nnz_dev, s->agg_flag,
                   s->agg_map, s->agg_full, s->agg_of_row, s->nc_dev, s->info_dev, s->Ac, s->Af, s->agg_rng, s->cpart, s->rxyz, s->rc, s->cc,
@@ -1494,8 +1607,7 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
         const bool timed = s->profile && s->ev_used + 2 * (size_t)len <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
-            hipLaunchKernelGGL((k_pcg_spmv<VA, V>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
-                               s->slice_off, s->slice_w, s->cols, vals, s->diag, pa, q, s->scal, s->part);
+            launch_spmv<R, VA, V>(s, vals, pa, q, st);
             if (timed) {
                 (void)hipEventRecord(s->ev[s->ev_used + 1], st);
                 s->ev_used += 2;
@@ -1588,7 +1700,6 @@ template <typename R>
 mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, int profile, hipStream_t st) {
     const int N = d.N;
     s->profile = profile;
-    s->ev_used = 0;
     if (profile && s->ev.empty()) {
         s->ev.resize(2 * 4096);
         for (auto& e : s->ev)
@@ -1604,7 +1715,13 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->why[0] = 0;
     // ---- assembly ----
     // active flags, row bounds, diagonal and Dirichlet flags come from the PND pass (ppe_start)
-    fill_int(s->nnz_dev, 0, 2, st);
+    fill_int(s->nnz_dev, 0, 3, st);
+    s->skeys = d.skeys;
+    s->gz = d.c.gz;
+    s->gygz = d.c.gy * d.c.gz;
+    if (N > 0)
+        hipLaunchKernelGGL(k_first_ge, dim3(grid_for((long long)d.c.ncell + 1, PB)), dim3(PB), 0, st, N, d.c.ncell,
+                           d.skeys, s->first_ge);
     if (N > 0)
         hipLaunchKernelGGL(k_sell_order, dim3(grid_for(N, SELL_SIGMA)), dim3(SELL_BLOCK), 0, st, N, nsl, s->cnt, s->dir,
                            s->pos, s->rowp, s->slice_w);
@@ -1617,11 +1734,13 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->nnz_stored = total;
     if ((long long)total > s->nnz_cap) {
         if (s->cols) (void)hipFree(s->cols);
+        if (s->cols16) (void)hipFree(s->cols16);
         if (s->vals64) (void)hipFree(s->vals64);
         if (s->vals32) (void)hipFree(s->vals32);
-        s->cols = nullptr; s->vals64 = nullptr; s->vals32 = nullptr;
+        s->cols = nullptr; s->cols16 = nullptr; s->vals64 = nullptr; s->vals32 = nullptr;
         s->nnz_cap = (long long)(total * 1.25) + 4096;
         if (!dev_alloc(&s->cols, (size_t)s->nnz_cap)) return MPSR_MEMERR;
+        if (!dev_alloc(&s->cols16, (size_t)s->nnz_cap)) return MPSR_MEMERR;
         if (!dev_alloc(&s->vals64, (size_t)s->nnz_cap)) return MPSR_MEMERR;
         if (!dev_alloc(&s->vals32, (size_t)s->nnz_cap)) return MPSR_MEMERR;
     }
@@ -1631,13 +1750,15 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     if (s->mixed)
         hipLaunchKernelGGL((k_ppe_fill<R, float>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
-                           s->rowp, s->cols, s->vals32, s->rhs, s->cnt, s->nnz_dev, s->nbq, s->nbw, NB_WIDTH);
+                           s->rowp, s->cols, s->vals32, s->rhs, s->cnt, s->nnz_dev, s->nbq, s->nbw, NB_WIDTH,
+                           s->cols16, d.skeys, s->first_ge);
     else
         hipLaunchKernelGGL((k_ppe_fill<R, double>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
-                           s->rowp, s->cols, s->vals64, s->rhs, s->cnt, s->nnz_dev, s->nbq, s->nbw, NB_WIDTH);
+                           s->rowp, s->cols, s->vals64, s->rhs, s->cnt, s->nnz_dev, s->nbq, s->nbw, NB_WIDTH,
+                           s->cols16, d.skeys, s->first_ge);
     // entry and row counts of the filled matrix, read after the first residual pass synchronises
-    if (hipMemcpyAsync(s->cnt_h, s->nnz_dev, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+    if (hipMemcpyAsync(s->cnt_h, s->nnz_dev, 3 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
         return MPSR_GPUERR;
     s->nslices = nsl;
     // ---- solve ----
@@ -1668,6 +1789,7 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     if (rr < 0) return MPSR_GPUERR;
     s->nnz_real = s->cnt_h[0];
     s->rows_real = s->cnt_h[1];
+    s->c16 = (s->cnt_h[2] == 0 && !c16_off()) ? 1 : 0;
     s->relres0 = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     const double target = tol * tol * bb;
     s->iters = 0;
@@ -1846,27 +1968,13 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
                       "not converged: relative residual %.3e above %.3e after %d iterations (max_iter %d)",
                       s->relres, tol, s->iters, maxit);
     }
-    // SpMV launch statistics (events already complete: resid_pass synchronised the stream)
-    s->spmv_launches = 0;
-    s->spmv_ms = 0.0;
-    if (profile) {
-        double tot_ms = 0.0;
-        int cnt = 0;
-        for (size_t e = 0; e + 1 < s->ev_used; e += 2) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, s->ev[e], s->ev[e + 1]) == hipSuccess) {
-                tot_ms += ms;
-                ++cnt;
-            }
-        }
-        s->spmv_launches = cnt;
-        s->spmv_ms = cnt ? tot_ms / cnt : 0.0;
-    }
     {
         // algorithmic bytes of one SpMV launch: every stored entry's value and column once, and
         // per row p and diag read and q written; gathers of p at the columns not counted
         const double sv = s->mixed ? 4.0 : 8.0;
-        s->spmv_bytes = (double)s->nnz_real * (sv + 4.0) + (double)N * (2.0 * sv + 8.0);
+        // 16-bit codes: 2 bytes a column, and per row its cell key
+        s->spmv_bytes = s->c16 ? (double)s->nnz_real * (sv + 2.0) + (double)N * (2.0 * sv + 8.0 + 4.0)
+                               : (double)s->nnz_real * (sv + 4.0) + (double)N * (2.0 * sv + 8.0);
     }
     if (s->status != 0) {
         // a failed solve leaves the state as it was: no pressure is written back, and the caller's
@@ -1881,6 +1989,22 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
 }
 
 template <typename R> mpsr_status ppe_check(PpeSolver<R>* s, mpsr_timings* tm) {
+    // SpMV launch statistics over the events of the call (complete: the caller synchronised)
+    s->spmv_launches = 0;
+    s->spmv_ms = 0.0;
+    if (s->profile) {
+        double tot_ms = 0.0;
+        int cnt = 0;
+        for (size_t e = 0; e + 1 < s->ev_used; e += 2) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, s->ev[e], s->ev[e + 1]) == hipSuccess) {
+                tot_ms += ms;
+                ++cnt;
+            }
+        }
+        s->spmv_launches = cnt;
+        s->spmv_ms = cnt ? tot_ms / cnt : 0.0;
+    }
     tm->spmv_ms = s->spmv_ms;
     tm->spmv_launches = s->spmv_launches;
     tm->spmv_bytes = s->spmv_bytes;
@@ -1901,6 +2025,13 @@ template <typename R> mpsr_status ppe_check(PpeSolver<R>* s, mpsr_timings* tm) {
 }
 
 template <typename R> const char* ppe_why(PpeSolver<R>* s) { return s->why; }
+// the SpMV launch events of a step or step_many call start from an empty pool
+template <typename R> void ppe_begin_call(PpeSolver<R>* s) { s->ev_used = 0; }
+template <typename R> void ppe_last(const PpeSolver<R>* s, int* iters, double* relres, int* rebuilt) {
+    *iters = s->iters;
+    *relres = s->relres;
+    *rebuilt = s->coarse_rebuilt && s->two_level_ok;
+}
 
 template <typename R> mpsr_status ppe_mpnn_set(PpeSolver<R>* s, const float* w, hipStream_t st) {
     if (!s->net) {
@@ -2041,6 +2172,8 @@ namespace mpsr {
     template mpsr_status ppe_get<R>(PpeSolver<R>*, const Dev<R>&, double*, double*, int*);                           \
     template void ppe_update_params<R>(PpeSolver<R>*, const mpsr_params&);                                         \
     template NbList ppe_neighbors<R>(const PpeSolver<R>*);                                                          \
+    template void ppe_begin_call<R>(PpeSolver<R>*);                                                                 \
+    template void ppe_last<R>(const PpeSolver<R>*, int*, double*, int*);                                            \
     template mpsr_status ppe_mpnn_set<R>(PpeSolver<R>*, const float*, hipStream_t);                                   \
     template mpsr_status ppe_mpnn_get<R>(PpeSolver<R>*, float*, hipStream_t);                                         \
     template mpsr_status ppe_mpnn_train<R>(PpeSolver<R>*, const Dev<R>&, double, double, int, float*, double*,       \

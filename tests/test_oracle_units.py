@@ -22,9 +22,15 @@ What pins what:
                     the triangle's bounding box grown by one cell
   PPE / PCG         symmetric, weakly diagonally dominant, solution = dense LU solve
   pow (drag)        double-precision pow rounded once, special cases
-Parity unpinned (no closed form or reference data exists): the full floating-point operation
-order inside CalcExplicitly / CalcPressureExplicit / CalcImplicitExplicitly beyond the cases
-above, and the RegisterTriangle march's exact cell coverage; both are restatements of
+tests/test_oracle_pins.py pins the rest on isolated particles in closed form: CalcDragEffect (modes
+E/V, E/T, E/B, C, F, the region test and the nearest drag wall), the EOS line on both sides of
+beta n0, CorrectVelocity, the artificial-pressure and collision sums (particle pair and wall), the
+wall translation, and the nearest-wall search against a brute force over every triangle (which
+also covers the triangle march: every triangle within one cell of a particle is found).
+Parity unpinned (no closed form or reference data exists): the floating-point operation order
+inside CalcExplicitly / CalcPressureExplicit / CalcImplicitExplicitly for general particle
+configurations (the closed forms fix every term but not the rounding of their sums), and the
+RegisterTriangle march's cells beyond one cell from the triangle; both are restatements of
 Source/DeviceProcess_Kernel.h and solver/Common/inc/Utility_Inline.h.
 """
 import ctypes as C
