@@ -416,6 +416,27 @@ mpsr_status mpsr_set_ppe_options(mpsr_handle *h, const mpsr_ppe_options *opt);
 mpsr_status mpsr_get_binning(mpsr_handle *h, int *sorted_keys, int *perm, int *cell_start,
                              int *cell_end);
 
+/* ---- binning that allocates nothing ---------------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the binning of
+ * mpsr_register_particle_topology (cell key per slot, stable radix sort, cellStart/cellEnd), with
+ * the same results as mpsr_get_binning returns after it, on caller-owned device arrays.
+ *   gb          the handle's grid (mpsr_make_gridbox / mpsr_get_gridbox)
+ *   id, px..pz  device arrays [n]: particle id (< 0 = dead) and FP32 position, slot order
+ *   sorted_keys device [n]: cell key per sorted index (the cell count marks a slot not binned)
+ *   perm        device [n]: sorted index -> slot
+ *   cell_start, cell_end  device [cell_capacity >= cells of gb]: first / end sorted index, -1 empty
+ *   stream      a hipStream_t (NULL: the null stream); the call only queues work on it
+ * The sort's scratch is a fixed device area inside the library (MPSR_NOALLOC_MAX_PARTICLES
+ * particles, about 9 MB per GPU, held until the process ends): n above it is MPSR_PARAMERR.  No
+ * hipMalloc, hipFree or other allocation happens in this call.  Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU race. */
+#define MPSR_NOALLOC_MAX_PARTICLES (1 << 20)
+mpsr_status mpsr_register_particle_topology_noalloc(const mpsr_gridbox *gb, int n, const int *id,
+                                                    const float *px, const float *py, const float *pz,
+                                                    unsigned int *sorted_keys, int *perm, int *cell_start,
+                                                    int *cell_end, int cell_capacity, int device,
+                                                    void *stream);
 /* Triangle lists per cell after mpsr_register_triangle_topology: start/count per cell and the
  * ascending triangle ids (capacity cap; *total receives the entry count). */
 mpsr_status mpsr_get_triangle_cells(mpsr_handle *h, int *cell_start, int *cell_count,

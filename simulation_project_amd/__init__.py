@@ -69,6 +69,8 @@ def load_library(path=None):
         "mpsr_set_ppe_options": (C.c_int, [vp, P(abi.PpeOptions)]),
         "mpsr_set_profiling": (C.c_int, [vp, C.c_int]),
         "mpsr_get_binning": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "mpsr_register_particle_topology_noalloc": (C.c_int, [P(abi.GridBox), C.c_int] + [vp] * 8 +
+                                                    [C.c_int, C.c_int, vp]),
         "mpsr_get_triangle_cells": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_int), C.c_int, P(C.c_int)]),
         "mpsr_get_stl_distance": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_int)]),
         "mpsr_get_neighbors": (C.c_int, [vp, C.c_double, P(C.c_int), P(C.c_int), C.c_int, P(C.c_int)]),

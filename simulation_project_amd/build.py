@@ -21,7 +21,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("MPSR_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["sort.hip", "particles.hip", "walls.hip", "inflow.hip", "spheres.hip", "mpnn.hip", "ppe.hip", "api.hip"]
+HIP_SOURCES = ["sort.hip", "particles.hip", "walls.hip", "inflow.hip", "spheres.hip", "mpnn.hip", "ppe.hip", "api.hip",
+               "noalloc.hip"]
 CXX_SOURCES = ["host_setup.cpp", "deck.cpp"]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
