@@ -67,11 +67,21 @@ static constexpr int CHUNK_TAIL = 4;    // ... after a first chunk sized from th
 static constexpr int HINT_PASSES = 4;   // refinement passes whose iteration counts are remembered
 static constexpr int SELL_SIGMA = 256;  // rows ordered by stored length inside windows of this many
 static constexpr int NB_WIDTH = 192;    // Laplacian-radius neighbours listed per row by the PND pass
+static constexpr int WIN_BYTES = 32768; // LDS a workgroup of the FP32 SpMV may stage p in
+static_assert(SELL_SIGMA == PB, "one SpMV workgroup covers one SELL window");
 // MPSR_PPE_COLUMNS=32 keeps the SpMV on the 32-bit columns (for comparing the two forms)
 static bool c16_off() {
     static const int off = [] {
         const char* e = std::getenv("MPSR_PPE_COLUMNS");
         return (e && std::strcmp(e, "32") == 0) ? 1 : 0;
+    }();
+    return off != 0;
+}
+// MPSR_PPE_WINDOW=0 keeps the FP32 SpMV's gathers of p in global memory (for comparing the two forms)
+static bool win_off() {
+    static const int off = [] {
+        const char* e = std::getenv("MPSR_PPE_WINDOW");
+        return (e && std::strcmp(e, "0") == 0) ? 1 : 0;
     }();
     return off != 0;
 }
@@ -96,6 +106,8 @@ template <typename R> struct PpeSolver {
     int* cols = nullptr;
     uint16_t* cols16 = nullptr;  // the columns as 16-bit codes for the PCG SpMV (k_ppe_fill explains them)
     int* first_ge = nullptr;     // [ncell + 1] first sorted row whose cell key is >= the index
+    int* wbox = nullptr;         // [windows * 8] per SELL window: the p ranges its rows read (k_win_box)
+    int wcap = 0;                // p values a window stages in LDS in this solve's SpMV (0: none)
     int c16 = 0;                 // this solve's SpMV reads the 16-bit codes (every column fitted one)
     const uint32_t* skeys = nullptr;  // this solve's sorted cell keys
     int gz = 0, gygz = 0;        // cell-key strides of one grid row and one x-slab
@@ -119,7 +131,7 @@ template <typename R> struct PpeSolver {
     int iters = 0, passes = 0, status = 0, fallbacks = 0;
     double relres = 0, relres0 = 0;
     char why[256] = {0};         // what went wrong in the last solve
-    int* nnz_dev = nullptr;      // [2]: off-diagonal entries, non-Dirichlet rows
+    int* nnz_dev = nullptr;      // [4]: off-diagonal entries, non-Dirichlet rows, wide columns, largest window
     long long nnz_real = 0;
     long long nnz_stored = 0;     // entries kept, SELL padding included
     int rows_real = 0;
@@ -298,6 +310,37 @@ __device__ inline int c16_code(const C16Row& r, int q, bool& wide) {
     else { g = 2; off = q - r.b2; if (q >= r.e2) wide = true; }
     if (off < 0 || off >= (1 << 14)) wide = true;
     return (g << 14) | (off & 0x3fff);
+}
+
+// The p values a SELL window's rows read.  Window w holds the sorted rows t0 .. t1 (keys K0 <= K1),
+// so every column of its rows in the x-slab i + dx lies in [first_ge[K0 + dx gy gz - gz - 1],
+// first_ge[K1 + dx gy gz + gz + 2]) (bounds clamped to the grid), whatever the rows' cells.
+// wbox[8 w ..] = the three range starts, the three ends and their total length; *maxw = the
+// largest total.
+__global__ __launch_bounds__(PB) void k_win_box(int N, int nwin, int ncell, int gz, int gygz,
+                                                const uint32_t* __restrict__ skeys, const int* __restrict__ fg,
+                                                int* wbox, int* maxw) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwin) return;
+    const int t0 = w * SELL_SIGMA, t1 = min(N, t0 + SELL_SIGMA) - 1;
+    int* o = wbox + 8 * w;
+    int tot = 0;
+    for (int g = 0; g < 3; ++g) {
+        int b = 0, e = 0;
+        if (t0 <= t1) {
+            const long long sh = (long long)(g - 1) * gygz;
+            const long long lo = min(max((long long)skeys[t0] + sh - gz - 1, 0LL), (long long)ncell);
+            const long long hi = min(max((long long)skeys[t1] + sh + gz + 2, 0LL), (long long)ncell);
+            b = fg[lo];
+            e = max(fg[hi], b);
+        }
+        o[g] = b;
+        o[3 + g] = e;
+        tot += e - b;
+    }
+    o[6] = tot;
+    o[7] = 0;
+    atomicMax(maxw, tot);
 }
 
 template <typename R, typename VA>
@@ -491,7 +534,12 @@ __global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restric
 // loop is unrolled by 8 so every lane keeps 8 column/value loads and then 8 gathers of p in
 // flight (the loop is latency-bound otherwise).  Matrix streams are loaded non-temporally: they
 // are read once per SpMV and would only evict p from L2.  With C16 the columns come from the
-// 16-bit codes (2 bytes an entry instead of 4) and the row's three x-slab range starts.
+// 16-bit codes (2 bytes an entry instead of 4) and the row's three x-slab range starts.  With WIN
+// (FP32 vectors, 16-bit codes) a workgroup first copies the p values its window's rows can read
+// (the three slab ranges of k_win_box, a few thousand values) into LDS with coalesced loads, and
+// the gathers then read LDS: the range starts become LDS offsets, the codes are unchanged.  A
+// window larger than wcap gathers from global memory.  Both forms add the same products in the
+// same order.
 template <bool C16> struct ColSource {
     const int* cols;
     const uint16_t* codes;
@@ -499,10 +547,13 @@ template <bool C16> struct ColSource {
     __device__ inline void load4(int e, int* c) const {
         if constexpr (C16) {
             quad_load(codes + e, c);
+            // the range start by arithmetic on the slab number (a select between the three members
+            // becomes an indexed load, and the object then lives in LDS or scratch)
+            const int d1 = b1 - b0, d2 = b2 - b0;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int g = c[u] >> 14;
-                c[u] = (g == 0 ? b0 : (g == 1 ? b1 : b2)) + (c[u] & 0x3fff);
+                c[u] = b0 + (d1 & -(int)(g == 1)) + (d2 & -(int)(g == 2)) + (c[u] & 0x3fff);
             }
         } else {
             quad_load(cols + e, c);
@@ -510,7 +561,85 @@ template <bool C16> struct ColSource {
     }
 };
 
-template <typename VA, typename V, bool C16>
+template <typename V> struct GatherGlobal {
+    const V* __restrict__ p;
+    __device__ inline V operator()(int c) const { return p[c]; }
+};
+template <typename V> struct GatherLds {
+    const V* w;
+    __device__ inline V operator()(int c) const { return w[c]; }
+};
+
+// a += the products of a row's entries [0, width) (width a multiple of 4) in entry order
+template <typename VA, typename V, bool C16, typename G>
+__device__ inline V spmv_row(V a, int base, int lane, int width, const ColSource<C16>& src,
+                             const VA* __restrict__ vals, G gather) {
+    int k = 0;
+    // FP32 matrix and vectors: 32 and then 16 entries per round (16 or 8 vector loads, then 32
+    // or 16 gathers of p in flight); with FP64 the registers of such rounds cut the occupancy,
+    // so FP64 keeps rounds of 8
+    if constexpr (sizeof(VA) == 4 && sizeof(V) == 4) {
+        for (; k + 32 <= width; k += 32) {
+            int c[32];
+            VA v[32];
+            const int e = sell_at(base, lane, k);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                src.load4(e + 256 * u, c + 4 * u);
+                quad_load(vals + e + 256 * u, v + 4 * u);
+            }
+            V g[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) g[u] = gather(c[u]);
+#pragma unroll
+            for (int u = 0; u < 32; ++u) a += (V)v[u] * g[u];
+        }
+        for (; k + 16 <= width; k += 16) {
+            int c[16];
+            VA v[16];
+            const int e = sell_at(base, lane, k);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                src.load4(e + 256 * u, c + 4 * u);
+                quad_load(vals + e + 256 * u, v + 4 * u);
+            }
+            V g[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) g[u] = gather(c[u]);
+#pragma unroll
+            for (int u = 0; u < 16; ++u) a += (V)v[u] * g[u];
+        }
+    }
+    for (; k + 8 <= width; k += 8) {
+        int c[8];
+        VA v[8];
+        const int e = sell_at(base, lane, k);
+        src.load4(e, c);
+        src.load4(e + 256, c + 4);
+        quad_load(vals + e, v);
+        quad_load(vals + e + 256, v + 4);
+        V g[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) g[u] = gather(c[u]);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a += (V)v[u] * g[u];
+    }
+    if (k < width) {   // one group of 4 left
+        int c[4];
+        VA v[4];
+        const int e = sell_at(base, lane, k);
+        src.load4(e, c);
+        quad_load(vals + e, v);
+        V g[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) g[u] = gather(c[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a += (V)v[u] * g[u];
+    }
+    return a;
+}
+
+template <typename VA, typename V, bool C16, bool WIN>
 __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restrict__ rowp,
                                                   const int* __restrict__ cnt, const int* __restrict__ slice_off,
                                                   const int* __restrict__ slice_w, const int* __restrict__ cols,
@@ -518,9 +647,30 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
                                                   const V* __restrict__ p, V* q, const double* __restrict__ scal,
                                                   double* part, const uint16_t* __restrict__ codes,
                                                   const uint32_t* __restrict__ skeys, const int* __restrict__ fg,
-                                                  int gz, int gygz) {
+                                                  int gz, int gygz, const int* __restrict__ wbox, int wcap) {
+    static_assert(!WIN || C16, "the staged window needs the 16-bit codes");
     if (scal[S_DONE] != 0.0) return;
     __shared__ double lds[PB / 64];
+    extern __shared__ __align__(16) unsigned char spmv_dyn[];
+    V* win = reinterpret_cast<V*>(spmv_dyn);
+    // LDS offset minus first p index of each slab range (0 when gathering from global memory)
+    int lb0 = 0, lb1 = 0, lb2 = 0;
+    bool staged = false;
+    if constexpr (WIN) {
+        const int* o = wbox + 8 * blockIdx.x;
+        if (o[6] <= wcap) {   // the same for the whole workgroup
+            staged = true;
+            const int B0 = o[0], B1 = o[1], B2 = o[2], E0 = o[3], E1 = o[4], E2 = o[5];
+            const int S1 = E0 - B0, S2 = S1 + (E1 - B1);
+            for (int i = threadIdx.x; i < E0 - B0; i += PB) win[i] = p[B0 + i];
+            for (int i = threadIdx.x; i < E1 - B1; i += PB) win[S1 + i] = p[B1 + i];
+            for (int i = threadIdx.x; i < E2 - B2; i += PB) win[S2 + i] = p[B2 + i];
+            lb0 = -B0;
+            lb1 = S1 - B1;
+            lb2 = S2 - B2;
+            __syncthreads();
+        }
+    }
     const int P = blockIdx.x * blockDim.x + threadIdx.x;   // SELL position
     const int slice = P >> 6, lane = lane_id();
     const int t = slice < nslices ? rowp[P] : -1;
@@ -535,73 +685,15 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
         if constexpr (C16) {
             if (width > 0) {   // a row with entries lies inside the grid (k_ppe_fill checked its ranges)
                 const int key = (int)skeys[t];
-                src.b0 = fg[key - gygz - gz - 1];
-                src.b1 = fg[key - gz - 1];
-                src.b2 = fg[key + gygz - gz - 1];
+                src.b0 = fg[key - gygz - gz - 1] + lb0;
+                src.b1 = fg[key - gz - 1] + lb1;
+                src.b2 = fg[key + gygz - gz - 1] + lb2;
             }
         }
-        int k = 0;
-        // FP32 matrix and vectors: 32 and then 16 entries per round (16 or 8 vector loads, then 32
-        // or 16 gathers of p in flight); with FP64 the registers of such rounds cut the occupancy,
-        // so FP64 keeps rounds of 8
-        if constexpr (sizeof(VA) == 4 && sizeof(V) == 4) {
-            for (; k + 32 <= width; k += 32) {
-                int c[32];
-                VA v[32];
-                const int e = sell_at(base, lane, k);
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    src.load4(e + 256 * u, c + 4 * u);
-                    quad_load(vals + e + 256 * u, v + 4 * u);
-                }
-                V g[32];
-#pragma unroll
-                for (int u = 0; u < 32; ++u) g[u] = p[c[u]];
-#pragma unroll
-                for (int u = 0; u < 32; ++u) a += (V)v[u] * g[u];
-            }
-            for (; k + 16 <= width; k += 16) {
-                int c[16];
-                VA v[16];
-                const int e = sell_at(base, lane, k);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    src.load4(e + 256 * u, c + 4 * u);
-                    quad_load(vals + e + 256 * u, v + 4 * u);
-                }
-                V g[16];
-#pragma unroll
-                for (int u = 0; u < 16; ++u) g[u] = p[c[u]];
-#pragma unroll
-                for (int u = 0; u < 16; ++u) a += (V)v[u] * g[u];
-            }
-        }
-        for (; k + 8 <= width; k += 8) {
-            int c[8];
-            VA v[8];
-            const int e = sell_at(base, lane, k);
-            src.load4(e, c);
-            src.load4(e + 256, c + 4);
-            quad_load(vals + e, v);
-            quad_load(vals + e + 256, v + 4);
-            V g[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) g[u] = p[c[u]];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) a += (V)v[u] * g[u];
-        }
-        if (k < width) {   // one group of 4 left
-            int c[4];
-            VA v[4];
-            const int e = sell_at(base, lane, k);
-            src.load4(e, c);
-            quad_load(vals + e, v);
-            V g[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) g[u] = p[c[u]];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) a += (V)v[u] * g[u];
-        }
+        if (staged)
+            a = spmv_row<VA, V, C16>(a, base, lane, width, src, vals, GatherLds<V>{win});
+        else
+            a = spmv_row<VA, V, C16>(a, base, lane, width, src, vals, GatherGlobal<V>{p});
         q[t] = a;
         pq = (double)pt * (double)a;
     }
@@ -1271,8 +1363,8 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     s->scan.scan_tmp_cap = (int)scan_tmp_ints(s->nslices + 1);
     ok &= dev_alloc(&s->scan.scan_tmp, s->scan.scan_tmp_cap);
     ok &= hipHostMalloc((void**)&s->scal_h, sizeof(double) * S_COUNT) == hipSuccess;
-    ok &= hipHostMalloc((void**)&s->cnt_h, sizeof(int) * 3) == hipSuccess;
-    ok &= dev_alloc(&s->nnz_dev, 3);
+    ok &= hipHostMalloc((void**)&s->cnt_h, sizeof(int) * 4) == hipSuccess;
+    ok &= dev_alloc(&s->nnz_dev, 4) && dev_alloc(&s->wbox, (size_t)(positions / SELL_SIGMA) * 8);
     ok &= dev_alloc(&s->pdiag_buf, rows) && dev_alloc(&s->omega, rows) && dev_alloc(&s->xprev_s, rows) &&
           dev_alloc(&s->xl, rows);
     if (ok && opt.warm_start == MPSR_WARM_LEARNED) {
@@ -1435,18 +1527,27 @@ static int setup_cells(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hipStre
 }
 
 // q = A p and the partial p.q, with the 16-bit column codes when every column of the matrix fitted
-// one, else with the 32-bit columns
+// one, else with the 32-bit columns; FP32 vectors with the codes stage each window's p in LDS
 template <typename R, typename VA, typename V>
 static void launch_spmv(PpeSolver<R>* s, const VA* vals, const V* p, V* q, hipStream_t st) {
     const int nb = (s->nslices * 64 + PB - 1) / PB;
+    if constexpr (sizeof(V) == 4) {
+        if (s->c16 && s->wcap > 0) {
+            hipLaunchKernelGGL((k_pcg_spmv<VA, V, true, true>), dim3(nb), dim3(PB), sizeof(V) * (size_t)s->wcap, st,
+                               s->nslices, s->rowp, s->cnt, s->slice_off, s->slice_w, s->cols, vals, s->diag, p, q,
+                               s->scal, s->part, s->cols16, s->skeys, s->first_ge, s->gz, s->gygz, s->wbox, s->wcap);
+            return;
+        }
+    }
     if (s->c16)
-        hipLaunchKernelGGL((k_pcg_spmv<VA, V, true>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
+        hipLaunchKernelGGL((k_pcg_spmv<VA, V, true, false>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
                            s->slice_off, s->slice_w, s->cols, vals, s->diag, p, q, s->scal, s->part, s->cols16,
-                           s->skeys, s->first_ge, s->gz, s->gygz);
+                           s->skeys, s->first_ge, s->gz, s->gygz, (const int*)nullptr, 0);
     else
-        hipLaunchKernelGGL((k_pcg_spmv<VA, V, false>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
+        hipLaunchKernelGGL((k_pcg_spmv<VA, V, false, false>), dim3(nb), dim3(PB), 0, st, s->nslices, s->rowp, s->cnt,
                            s->slice_off, s->slice_w, s->cols, vals, s->diag, p, q, s->scal, s->part,
-                           (const uint16_t*)nullptr, (const uint32_t*)nullptr, (const int*)nullptr, 0, 0);
+                           (const uint16_t*)nullptr, (const uint32_t*)nullptr, (const int*)nullptr, 0, 0,
+                           (const int*)nullptr, 0);
 }
 
 // Iterations queued before the next flag read.  Every launch queued after convergence returns at
@@ -1548,7 +1649,7 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
 template <typename R> void ppe_destroy(PpeSolver<R>* s) {
     if (!s) return;
     void* ps[] = {s->active, s->dir, s->cnt, s->slice_w, s->slice_off, s->pos, s->rowp, s->total, s->cols, s->vals64, s->vals32,
-                  s->cols16, s->first_ge,
+                  s->cols16, s->first_ge, s->wbox,
                   s->diag, s->rhs, s->x, s->prev, s->r, s->z, s->p0, s->p1, s->q, s->rf, s->zf, s->p0f,
                   s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp, s->nnz_dev, s->agg_flag,
                   s->agg_map, s->agg_full, s->agg_of_row, s->nc_dev, s->info_dev, s->Ac, s->Af, s->agg_rng, s->cpart, s->rxyz, s->rc, s->cc,
@@ -1715,13 +1816,16 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->why[0] = 0;
     // ---- assembly ----
     // active flags, row bounds, diagonal and Dirichlet flags come from the PND pass (ppe_start)
-    fill_int(s->nnz_dev, 0, 3, st);
+    fill_int(s->nnz_dev, 0, 4, st);
     s->skeys = d.skeys;
     s->gz = d.c.gz;
     s->gygz = d.c.gy * d.c.gz;
     if (N > 0)
         hipLaunchKernelGGL(k_first_ge, dim3(grid_for((long long)d.c.ncell + 1, PB)), dim3(PB), 0, st, N, d.c.ncell,
                            d.skeys, s->first_ge);
+    if (N > 0 && s->mixed)
+        hipLaunchKernelGGL(k_win_box, dim3(grid_for(nb, PB)), dim3(PB), 0, st, N, nb, d.c.ncell, s->gz, s->gygz,
+                           d.skeys, s->first_ge, s->wbox, s->nnz_dev + 3);
     if (N > 0)
         hipLaunchKernelGGL(k_sell_order, dim3(grid_for(N, SELL_SIGMA)), dim3(SELL_BLOCK), 0, st, N, nsl, s->cnt, s->dir,
                            s->pos, s->rowp, s->slice_w);
@@ -1758,7 +1862,7 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
                            s->rowp, s->cols, s->vals64, s->rhs, s->cnt, s->nnz_dev, s->nbq, s->nbw, NB_WIDTH,
                            s->cols16, d.skeys, s->first_ge);
     // entry and row counts of the filled matrix, read after the first residual pass synchronises
-    if (hipMemcpyAsync(s->cnt_h, s->nnz_dev, 3 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
+    if (hipMemcpyAsync(s->cnt_h, s->nnz_dev, 4 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
         return MPSR_GPUERR;
     s->nslices = nsl;
     // ---- solve ----
@@ -1790,6 +1894,10 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->nnz_real = s->cnt_h[0];
     s->rows_real = s->cnt_h[1];
     s->c16 = (s->cnt_h[2] == 0 && !c16_off()) ? 1 : 0;
+    // LDS per SpMV workgroup: the largest window, at most WIN_BYTES (larger windows gather from
+    // global memory)
+    s->wcap = (s->mixed && s->c16 && s->cnt_h[3] > 0 && !win_off())
+                  ? std::min((s->cnt_h[3] + 63) / 64 * 64, WIN_BYTES / (int)sizeof(float)) : 0;
     s->relres0 = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     const double target = tol * tol * bb;
     s->iters = 0;
