@@ -203,6 +203,24 @@ template <int NW> __device__ inline int block_exclusive_scan(int v, int* lds, in
 // ---- launch helpers shared by the translation units ------------------------------------------
 inline unsigned grid_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// XCD-contiguous workgroup order for the neighbour kernels.  The GPU deals the workgroups of a
+// launch to its 8 XCDs in turn (workgroup b runs on XCD b % 8) and every XCD has its own L2, so in
+// launch order each L2 sees every eighth block of sorted particles and fetches nearly the whole
+// candidate set.  Launched with xcd_grid blocks, the workgroups of one XCD take one contiguous run of
+// the work blocks instead, and the cells a block's candidates come from are mostly the ones its
+// neighbouring blocks on the same L2 read.
+static constexpr int kXcds = 8;
+inline unsigned xcd_grid(long long n, int block) {
+    return (grid_for(n, block) + kXcds - 1) / kXcds * kXcds;
+}
+// the work block of this workgroup in a launch of xcd_grid(n, blockDim.x) workgroups, -1 for padding
+__device__ inline int xcd_block(int n) {
+    const unsigned nb = ((unsigned)n + blockDim.x - 1) / blockDim.x;
+    const unsigned per = (nb + kXcds - 1) / kXcds;
+    const unsigned b = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    return b < nb ? (int)b : -1;
+}
+
 // PPE matrix layout (SELL-64/4): rows in slices of 64, every slice as wide as its longest row
 // rounded up to a multiple of 4, entries kept in groups of 4 per row: entry k of the slice's row
 // L sits at slice_base + 256 (k / 4) + 4 L + k % 4.  A lane's 4 consecutive entries are 16

@@ -100,7 +100,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_explicit(int N, Consts<R> c, cons
                                                    const TriD<R>* __restrict__ tris,
                                                    const mpsr_tri_params* __restrict__ models,
                                                    const R* __restrict__ turb_nu) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int blk = xcd_block(N);
+    if (blk < 0) return;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
     if (out.id[s] < 0) return;
@@ -362,7 +364,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_gradient(int N, Consts<R> c, cons
                                                    const int* __restrict__ tcount, const int* __restrict__ tids,
                                                    const TriD<R>* __restrict__ tris,
                                                    const NegModel<R>* __restrict__ negm, NbList nb) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int blk = xcd_block(N);
+    if (blk < 0) return;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
     if (out.id[s] < 0) return;
@@ -493,7 +497,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_thermal(int N, Consts<R> c, const
                                                     const R* __restrict__ drag_temp, const int* __restrict__ step_base,
                                                     const int* __restrict__ step_ctr, int thermal_step) {
     if (step_ctr && ((*step_base + *step_ctr) % thermal_step) != 0) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int blk = xcd_block(N);
+    if (blk < 0) return;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
     if (out.id[s] < 0) return;
@@ -569,7 +575,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_turbulence(int N, Consts<R> c, co
                                                      const R4<R>* __restrict__ svel, const int* __restrict__ cs,
                                                      const int* __restrict__ ce, const R* __restrict__ stl_mag,
                                                      R* __restrict__ turb_nu, R* __restrict__ turb_k) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int blk = xcd_block(N);
+    if (blk < 0) return;
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
     if (out.id[s] < 0) return;
@@ -638,13 +646,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_turbulence(int N, Consts<R> c, co
 
 template <typename R> void launch_explicit(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
-    hipLaunchKernelGGL(k_explicit<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
+    hipLaunchKernelGGL(k_explicit<R>, dim3(xcd_grid(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
                        d.spos, d.svel, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id, d.drag_acc, d.sph_dv,
                        d.tris, d.models, d.turb_nu);
 }
 template <typename R> void launch_turbulence(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0 || !d.turb_nu) return;
-    hipLaunchKernelGGL(k_turbulence<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.out,
+    hipLaunchKernelGGL(k_turbulence<R>, dim3(xcd_grid(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.out,
                        d.spos, d.svel, d.cell_start, d.cell_end, d.stl_mag, d.turb_nu, d.turb_k);
 }
 template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st) {
@@ -667,14 +675,14 @@ template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st) 
 }
 template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st, NbList nb) {
     if (d.N <= 0) return;
-    hipLaunchKernelGGL(k_gradient<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
+    hipLaunchKernelGGL(k_gradient<R>, dim3(xcd_grid(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
                        d.spos, d.svel, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_dir, d.stl_id,
                        d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.negm, nb);
 }
 template <typename R>
 void launch_thermal(const Dev<R>& d, const int* step_base, const int* step_ctr, int thermal_step, hipStream_t st) {
     if (d.N <= 0) return;
-    hipLaunchKernelGGL(k_thermal<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
+    hipLaunchKernelGGL(k_thermal<R>, dim3(xcd_grid(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out,
                        d.spos, d.cell_start, d.cell_end, d.stl_mag, d.stl_id, d.tris, d.models, d.drag_temp, step_base,
                        step_ctr, thermal_step > 0 ? thermal_step : 1);
 }

@@ -68,6 +68,7 @@ static constexpr int HINT_PASSES = 4;   // refinement passes whose iteration cou
 static constexpr int SELL_SIGMA = 256;  // rows ordered by stored length inside windows of this many
 static constexpr int NB_WIDTH = 192;    // Laplacian-radius neighbours listed per row by the PND pass
 static constexpr int WIN_BYTES = 32768; // LDS a workgroup of the FP32 SpMV may stage p in
+static constexpr int RESID_WIN_BYTES = 49152;  // ... and of the residual, x in FP64
 static_assert(SELL_SIGMA == PB, "one SpMV workgroup covers one SELL window");
 // MPSR_PPE_COLUMNS=32 keeps the SpMV on the 32-bit columns (for comparing the two forms)
 static bool c16_off() {
@@ -429,82 +430,6 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
 // ---------------------------------------------------------------------------------------------
 // solver kernels
 // ---------------------------------------------------------------------------------------------
-// r = b - A x with FP64 accumulation (x, b, r FP64; values VA); partial sums of r.r and b.b
-template <typename VA>
-__global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __restrict__ slice_off,
-                                               const int* __restrict__ slice_w, const int* __restrict__ cols,
-                                               const VA* __restrict__ vals, const double* __restrict__ diag,
-                                               const double* __restrict__ x, const double* __restrict__ b,
-                                               const int* __restrict__ rowp, const int* __restrict__ cnt, double* r,
-                                               double* part) {
-    __shared__ double lds[PB / 64];
-    const int P = blockIdx.x * blockDim.x + threadIdx.x;   // SELL position
-    const int slice = P >> 6, lane = lane_id();
-    const int t = slice < nslices ? rowp[P] : -1;
-    double rr = 0.0, bb = 0.0;
-    if (t >= 0) {
-        const int base = slice_off[slice], width = min(slice_w[slice] >> 6, (cnt[t] + 3) & ~3);
-        double a = diag[t] * x[t];
-        // entries in ascending order, 16 (FP32 matrix) or 8 column/value loads and then as many
-        // gathers in flight; the sum runs entry by entry either way
-        int k = 0;
-        if constexpr (sizeof(VA) == 4) {
-            for (; k + 16 <= width; k += 16) {
-                int c[16];
-                VA v[16];
-                const int e = sell_at(base, lane, k);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    quad_load(cols + e + 256 * u, c + 4 * u);
-                    quad_load(vals + e + 256 * u, v + 4 * u);
-                }
-                double g[16];
-#pragma unroll
-                for (int u = 0; u < 16; ++u) g[u] = x[c[u]];
-#pragma unroll
-                for (int u = 0; u < 16; ++u) a += (double)v[u] * g[u];
-            }
-        }
-        for (; k + 8 <= width; k += 8) {
-            int c[8];
-            VA v[8];
-            const int e = sell_at(base, lane, k);
-            quad_load(cols + e, c);
-            quad_load(cols + e + 256, c + 4);
-            quad_load(vals + e, v);
-            quad_load(vals + e + 256, v + 4);
-            double g[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) g[u] = x[c[u]];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) a += (double)v[u] * g[u];
-        }
-        if (k < width) {
-            int c[4];
-            VA v[4];
-            const int e = sell_at(base, lane, k);
-            quad_load(cols + e, c);
-            quad_load(vals + e, v);
-            double g[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) g[u] = x[c[u]];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) a += (double)v[u] * g[u];
-        }
-        const double rv = b[t] - a;
-        r[t] = rv;
-        rr = rv * rv;
-        bb = b[t] * b[t];
-    }
-    rr = block_sum<PB / 64>(rr, lds);
-    bb = block_sum<PB / 64>(bb, lds);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = rr;
-        part[2 * blockIdx.x + 1] = bb;
-    }
-    (void)N;
-}
-
 // start of a PCG on A e = r0 (vectors V): r = r0, z = r / diag, p = z, partial r.z and r.r
 template <typename V>
 __global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restrict__ r0, const double* __restrict__ diag,
@@ -576,8 +501,8 @@ __device__ inline V spmv_row(V a, int base, int lane, int width, const ColSource
                              const VA* __restrict__ vals, G gather) {
     int k = 0;
     // FP32 matrix and vectors: 32 and then 16 entries per round (16 or 8 vector loads, then 32
-    // or 16 gathers of p in flight); with FP64 the registers of such rounds cut the occupancy,
-    // so FP64 keeps rounds of 8
+    // or 16 gathers of p in flight); FP32 values with FP64 vectors: rounds of 16; with FP64
+    // values the registers of such rounds cut the occupancy, so they keep rounds of 8
     if constexpr (sizeof(VA) == 4 && sizeof(V) == 4) {
         for (; k + 32 <= width; k += 32) {
             int c[32];
@@ -594,6 +519,9 @@ __device__ inline V spmv_row(V a, int base, int lane, int width, const ColSource
 #pragma unroll
             for (int u = 0; u < 32; ++u) a += (V)v[u] * g[u];
         }
+    }
+    // FP32 values (FP32 or FP64 vectors): then rounds of 16
+    if constexpr (sizeof(VA) == 4) {
         for (; k + 16 <= width; k += 16) {
             int c[16];
             VA v[16];
@@ -639,6 +567,26 @@ __device__ inline V spmv_row(V a, int base, int lane, int width, const ColSource
     return a;
 }
 
+// Copies the three slab ranges of v that the rows of this workgroup's SELL window read (k_win_box)
+// into LDS and sets lb[g] = LDS offset minus first index of range g; false (nothing staged) when
+// the window holds more than wcap values.  The same for every thread of the workgroup.
+template <typename V>
+__device__ inline bool stage_window(const int* __restrict__ wbox, int wcap, const V* __restrict__ v, V* win,
+                                    int& lb0, int& lb1, int& lb2) {
+    const int* o = wbox + 8 * blockIdx.x;
+    if (o[6] > wcap) return false;
+    const int B0 = o[0], B1 = o[1], B2 = o[2], E0 = o[3], E1 = o[4], E2 = o[5];
+    const int S1 = E0 - B0, S2 = S1 + (E1 - B1);
+    for (int i = threadIdx.x; i < E0 - B0; i += PB) win[i] = v[B0 + i];
+    for (int i = threadIdx.x; i < E1 - B1; i += PB) win[S1 + i] = v[B1 + i];
+    for (int i = threadIdx.x; i < E2 - B2; i += PB) win[S2 + i] = v[B2 + i];
+    lb0 = -B0;
+    lb1 = S1 - B1;
+    lb2 = S2 - B2;
+    __syncthreads();
+    return true;
+}
+
 template <typename VA, typename V, bool C16, bool WIN>
 __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restrict__ rowp,
                                                   const int* __restrict__ cnt, const int* __restrict__ slice_off,
@@ -656,21 +604,7 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
     // LDS offset minus first p index of each slab range (0 when gathering from global memory)
     int lb0 = 0, lb1 = 0, lb2 = 0;
     bool staged = false;
-    if constexpr (WIN) {
-        const int* o = wbox + 8 * blockIdx.x;
-        if (o[6] <= wcap) {   // the same for the whole workgroup
-            staged = true;
-            const int B0 = o[0], B1 = o[1], B2 = o[2], E0 = o[3], E1 = o[4], E2 = o[5];
-            const int S1 = E0 - B0, S2 = S1 + (E1 - B1);
-            for (int i = threadIdx.x; i < E0 - B0; i += PB) win[i] = p[B0 + i];
-            for (int i = threadIdx.x; i < E1 - B1; i += PB) win[S1 + i] = p[B1 + i];
-            for (int i = threadIdx.x; i < E2 - B2; i += PB) win[S2 + i] = p[B2 + i];
-            lb0 = -B0;
-            lb1 = S1 - B1;
-            lb2 = S2 - B2;
-            __syncthreads();
-        }
-    }
+    if constexpr (WIN) staged = stage_window(wbox, wcap, p, win, lb0, lb1, lb2);
     const int P = blockIdx.x * blockDim.x + threadIdx.x;   // SELL position
     const int slice = P >> 6, lane = lane_id();
     const int t = slice < nslices ? rowp[P] : -1;
@@ -699,6 +633,59 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
     }
     pq = block_sum<PB / 64>(pq, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = pq;
+}
+
+// r = b - A x with FP64 accumulation (x, b, r FP64; values VA); partial sums of r.r and b.b.
+// Columns and the staged window as in k_pcg_spmv (the window holds FP64 values of x); the sum runs
+// entry by entry in either form.
+template <typename VA, bool C16, bool WIN>
+__global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __restrict__ slice_off,
+                                               const int* __restrict__ slice_w, const int* __restrict__ cols,
+                                               const VA* __restrict__ vals, const double* __restrict__ diag,
+                                               const double* __restrict__ x, const double* __restrict__ b,
+                                               const int* __restrict__ rowp, const int* __restrict__ cnt, double* r,
+                                               double* part, const uint16_t* __restrict__ codes,
+                                               const uint32_t* __restrict__ skeys, const int* __restrict__ fg, int gz,
+                                               int gygz, const int* __restrict__ wbox, int wcap) {
+    static_assert(!WIN || C16, "the staged window needs the 16-bit codes");
+    __shared__ double lds[PB / 64];
+    extern __shared__ __align__(16) unsigned char resid_dyn[];
+    double* win = reinterpret_cast<double*>(resid_dyn);
+    int lb0 = 0, lb1 = 0, lb2 = 0;
+    bool staged = false;
+    if constexpr (WIN) staged = stage_window(wbox, wcap, x, win, lb0, lb1, lb2);
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;   // SELL position
+    const int slice = P >> 6, lane = lane_id();
+    const int t = slice < nslices ? rowp[P] : -1;
+    double rr = 0.0, bb = 0.0;
+    if (t >= 0) {
+        const int base = slice_off[slice], width = min(slice_w[slice] >> 6, (cnt[t] + 3) & ~3);
+        double a = diag[t] * x[t];
+        ColSource<C16> src{cols, codes, 0, 0, 0};
+        if constexpr (C16) {
+            if (width > 0) {
+                const int key = (int)skeys[t];
+                src.b0 = fg[key - gygz - gz - 1] + lb0;
+                src.b1 = fg[key - gz - 1] + lb1;
+                src.b2 = fg[key + gygz - gz - 1] + lb2;
+            }
+        }
+        if (staged)
+            a = spmv_row<VA, double, C16>(a, base, lane, width, src, vals, GatherLds<double>{win});
+        else
+            a = spmv_row<VA, double, C16>(a, base, lane, width, src, vals, GatherGlobal<double>{x});
+        const double rv = b[t] - a;
+        r[t] = rv;
+        rr = rv * rv;
+        bb = b[t] * b[t];
+    }
+    rr = block_sum<PB / 64>(rr, lds);
+    bb = block_sum<PB / 64>(bb, lds);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = rr;
+        part[2 * blockIdx.x + 1] = bb;
+    }
+    (void)N;
 }
 
 // x += alpha p; r -= alpha q; z = r / diag; partial r.z and r.r
@@ -1739,15 +1726,33 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
     return done_iters;
 }
 
+// r = b - A x: with the 32-bit columns until the solve knows that every column has a 16-bit code
+// (the first residual of a solve), then with the codes and, for the FP32 matrix, the staged window
+template <typename R, typename VA>
+static void launch_resid(PpeSolver<R>* s, int N, int nb, const VA* vals, hipStream_t st) {
+    const int wcap = s->wcap > 0 ? std::min(s->wcap, RESID_WIN_BYTES / (int)sizeof(double)) : 0;
+    if (s->c16 && wcap > 0 && sizeof(VA) == 4)
+        hipLaunchKernelGGL((k_resid<VA, true, true>), dim3(nb), dim3(PB), sizeof(double) * (size_t)wcap, st, N,
+                           s->nslices, s->slice_off, s->slice_w, s->cols, vals, s->diag, s->x, s->rhs, s->rowp, s->cnt,
+                           s->r, s->part, s->cols16, s->skeys, s->first_ge, s->gz, s->gygz, s->wbox, wcap);
+    else if (s->c16)
+        hipLaunchKernelGGL((k_resid<VA, true, false>), dim3(nb), dim3(PB), 0, st, N, s->nslices, s->slice_off,
+                           s->slice_w, s->cols, vals, s->diag, s->x, s->rhs, s->rowp, s->cnt, s->r, s->part, s->cols16,
+                           s->skeys, s->first_ge, s->gz, s->gygz, (const int*)nullptr, 0);
+    else
+        hipLaunchKernelGGL((k_resid<VA, false, false>), dim3(nb), dim3(PB), 0, st, N, s->nslices, s->slice_off,
+                           s->slice_w, s->cols, vals, s->diag, s->x, s->rhs, s->rowp, s->cnt, s->r, s->part,
+                           (const uint16_t*)nullptr, (const uint32_t*)nullptr, (const int*)nullptr, 0, 0,
+                           (const int*)nullptr, 0);
+}
+
 template <typename R> static double resid_pass(PpeSolver<R>* s, int N, hipStream_t st, double* bb_out) {
     const int rows = s->nslices * 64;
     const int nb = (rows + PB - 1) / PB;
     if (s->mixed)
-        hipLaunchKernelGGL(k_resid<float>, dim3(nb), dim3(PB), 0, st, N, s->nslices, s->slice_off, s->slice_w, s->cols,
-                           s->vals32, s->diag, s->x, s->rhs, s->rowp, s->cnt, s->r, s->part);
+        launch_resid(s, N, nb, s->vals32, st);
     else
-        hipLaunchKernelGGL(k_resid<double>, dim3(nb), dim3(PB), 0, st, N, s->nslices, s->slice_off, s->slice_w,
-                           s->cols, s->vals64, s->diag, s->x, s->rhs, s->rowp, s->cnt, s->r, s->part);
+        launch_resid(s, N, nb, s->vals64, st);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_RESID);
     if (read_scal(s, st) != hipSuccess) return -1.0;
     *bb_out = s->scal_h[S_BB];
@@ -1814,6 +1819,9 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->mixed = opt.solver == MPSR_PCG_MIXED;
     s->status = 0;
     s->why[0] = 0;
+    // the column form and the window are decided anew for this matrix after its first residual
+    s->c16 = 0;
+    s->wcap = 0;
     // ---- assembly ----
     // active flags, row bounds, diagonal and Dirichlet flags come from the PND pass (ppe_start)
     fill_int(s->nnz_dev, 0, 4, st);
