@@ -71,10 +71,9 @@ struct EngineBase {
     // the radix variant's turbulence and negative-pressure terms
     virtual mpsr_status set_radix(const mpsr_radix_terms* t, const mpsr_negative_pressure_model* m, int nm) = 0;
     virtual mpsr_status get_turbulence(double* nu, double* k, int n) = 0;
-    // batched form: one step of this domain recorded on its own stream (into a shared graph)
+    // batched form: this domain as one bucket of a batched step (BatchGroup)
     virtual mpsr_status batch_check() = 0;
-    virtual mpsr_status batch_prepare(int first, int count, int* moved) = 0;
-    virtual mpsr_status batch_enqueue() = 0;
+    virtual mpsr_status batch_prepare(int first, int count) = 0;
     virtual mpsr_status batch_finish() = 0;
     virtual mpsr_status snapshot(int restore) = 0;
     virtual hipStream_t stream() const = 0;
@@ -768,9 +767,6 @@ template <typename R> struct Engine : EngineBase {
     int* step_base_buf = nullptr;      // device index of the first replayed step
     std::vector<char*> snap;           // device copy of the in/out state and the triangles
     int snap_n = 0;
-    // what the last recorded step holds: rotation table address and launch shape
-    const RotCoef<R>* rec_rot = nullptr;
-    int rec_block = 0, rec_items = 0, rec_consts = 0;
 
     mpsr_status set_radix(const mpsr_radix_terms* t, const mpsr_negative_pressure_model* m, int nm) override {
         HIPC(hipSetDevice(device));
@@ -844,50 +840,16 @@ template <typename R> struct Engine : EngineBase {
         if (!buckets.empty()) return fail(MPSR_PARAMERR, "batched step: bucket inflow changes the particle count");
         return MPSR_NOERR;
     }
-    // rotation table for steps first .. first+count-1 and the step counter at 0; *moved when the
-    // table address or the launch shape differ from the last recorded step (a graph holds both)
-    mpsr_status batch_prepare(int first, int count, int* moved) override {
+    // rotation table for steps first .. first+count-1 and the step counter at 0
+    mpsr_status batch_prepare(int first, int count) override {
         HIPC(hipSetDevice(device));
         if (!step_ctr_buf) ALLOC(step_ctr_buf, 1);
         if (!step_base_buf) ALLOC(step_base_buf, 1);
         mpsr_status s = upload_rot(first, count);
         if (s != MPSR_NOERR) return s;
-        *moved = rec_rot != rot_table || rec_block != d.block || rec_items != d.sort_items || rec_consts != consts_ver;
         fill_int(step_ctr_buf, 0, 1, st);
         fill_int(step_base_buf, first, 1, st);
         HIPC(hipStreamSynchronize(st));   // rot_host is refilled by the next call
-        return MPSR_NOERR;
-    }
-    // the EOS step of one_step() with the rotation row taken from the device counter, which the
-    // step then advances; no host synchronisation, no memset, no profiling events
-    mpsr_status batch_enqueue() override {
-        const int prof = profiling;
-        profiling = 0;
-        d.rot = rot_table;
-        d.step_ctr = step_ctr_buf;
-        rec_rot = rot_table;
-        rec_block = d.block;
-        rec_items = d.sort_items;
-        rec_consts = consts_ver;
-        do_bin();
-        do_tri();
-        do_stl();
-        if (d.ndrag > 0) do_drag();
-        if (P.bUpdateTemperature) launch_thermal(d, step_base_buf, step_ctr_buf, P.ThermalStep, st);
-        do_turbulence();
-        do_explicit();
-        do_relocate();
-        do_bin();
-        do_stl();
-        launch_pnd(d, 1, st);
-        do_gradient();
-        do_relocate();
-        launch_translate(d, st);
-        launch_rotate(d, st);
-        launch_step_advance(step_ctr_buf, st);
-        d.step_ctr = nullptr;
-        profiling = prof;
-        HIPC(hipGetLastError());
         return MPSR_NOERR;
     }
     mpsr_status batch_finish() override {
@@ -1203,6 +1165,261 @@ template <typename R> struct Engine : EngineBase {
     }
 };
 
+// ---------------------------------------------------------------------------------------------
+// The batched step of the buckets of one precision: every kernel of the EOS step is one launch
+// over a work list of (bucket, block) items of all buckets, so a step of all buckets is the same
+// few dozen launches whatever their number, and small buckets share the GPU with large ones
+// inside each launch.  The kernels read each bucket's buffers, grid and constants from a device
+// table of the buckets' Dev structs; the work lists and the per-bucket sort, scan and fill jobs
+// sit in one device block built once per launch shape.
+// ---------------------------------------------------------------------------------------------
+template <typename R> struct BatchGroup {
+    std::vector<Engine<R>*> eng;       // largest bucket first
+    std::vector<Dev<R>> host;
+    Dev<R>* devs = nullptr;
+    char* tab = nullptr;
+    size_t tab_bytes = 0;
+    int block = 0, items = 0;
+    long long sig = -1;                // launch shape and optional phases the tables were built for
+    struct Pass {
+        const SortJob* jobs = nullptr;
+        WorkList tiles, singles;
+    };
+    WorkList wp, wt64, wt256, wpairs, wone, wdrag, wtherm, wturb;
+    std::vector<Pass> psort, tsort;
+    const FillJob* fill_bin = nullptr;
+    const FillJob* fill_tri = nullptr;
+    WorkList fill_bin_w, fill_tri_w;
+    const ScanJob* scan_emit = nullptr;
+    const ScanJob* scan_flag = nullptr;
+    WorkList scan_emit_t, scan_emit_1, scan_flag_t, scan_flag_1;
+
+    ~BatchGroup() {
+        if (devs) (void)hipFree(devs);
+        if (tab) (void)hipFree(tab);
+    }
+    BatchLaunch<R> launch() const { return BatchLaunch<R>{devs, block}; }
+
+    // the shape of the tables: launch shape and which buckets run the optional phases
+    long long signature(int blk, int itm) const {
+        long long h = (long long)blk * 131 + itm;
+        for (Engine<R>* e : eng)
+            h = h * 8 + (e->d.ndrag > 0 ? 1 : 0) + (e->P.bUpdateTemperature ? 2 : 0) +
+                (e->d.c.turb && e->d.turb_nu ? 4 : 0);
+        return h;
+    }
+
+    // every bucket's Dev for the batched kernels: rotation table and step counter of replays, and
+    // the buffers the particle sort leaves the keys and permutation in
+    mpsr_status refresh() {
+        host.resize(eng.size());
+        for (size_t k = 0; k < eng.size(); ++k) {
+            Engine<R>* e = eng[k];
+            Dev<R>& d = e->d;
+            const bool odd = d.N > 0 && (radix_passes(d.key_bits) & 1);
+            d.skeys = odd ? d.sort.k1 : d.keys;
+            d.perm = odd ? d.sort.v1 : d.vals;
+            Dev<R> x = d;
+            x.rot = e->rot_table;
+            x.step_ctr = e->step_ctr_buf;
+            x.step_base = e->step_base_buf;
+            x.thermal_step = e->P.ThermalStep > 0 ? e->P.ThermalStep : 1;
+            host[k] = x;
+        }
+        if (!devs) HIPC(hipMalloc((void**)&devs, sizeof(Dev<R>) * std::max<size_t>(1, eng.size())));
+        HIPC(hipMemcpy(devs, host.data(), sizeof(Dev<R>) * host.size(), hipMemcpyHostToDevice));
+        return MPSR_NOERR;
+    }
+
+    mpsr_status build(int blk, int itm) {
+        const long long sg = signature(blk, itm);
+        if (tab && sg == sig) return MPSR_NOERR;
+        block = blk;
+        items = itm;
+        // host images of the lists and jobs, packed into one device block
+        std::vector<char> img;
+        auto put = [&](const void* p, size_t n) {
+            const size_t off = (img.size() + 15) & ~(size_t)15;
+            img.resize(off + n);
+            if (n) std::memcpy(img.data() + off, p, n);
+            return off;
+        };
+        struct Ref {
+            size_t off;
+            int n;
+        };
+        auto items_of = [&](const std::vector<int2>& v) { return Ref{put(v.data(), v.size() * sizeof(int2)), (int)v.size()}; };
+        std::vector<int2> p, t64, t256, pairs, one, drag, therm, turb;
+        const int n = (int)eng.size();
+        for (int k = 0; k < n; ++k) {
+            const Dev<R>& d = host[k];
+            const int nb = (int)grid_for(d.N, block);
+            for (int j = 0; j < nb; ++j) {
+                p.push_back(make_int2(k, j));
+                if (d.ndrag > 0) drag.push_back(make_int2(k, j));
+                if (eng[k]->P.bUpdateTemperature) therm.push_back(make_int2(k, j));
+                if (d.c.turb && d.turb_nu) turb.push_back(make_int2(k, j));
+            }
+            if (d.ntri > 0) {
+                for (int j = 0; j < (int)grid_for(d.ntri, 64); ++j) t64.push_back(make_int2(k, j));
+                for (int j = 0; j < (int)grid_for(d.ntri, 256); ++j) t256.push_back(make_int2(k, j));
+                for (int j = 0; j < (int)grid_for(d.pair_cap, 256); ++j) pairs.push_back(make_int2(k, j));
+            }
+            one.push_back(make_int2(k, 0));
+        }
+        const Ref rp = items_of(p), r64 = items_of(t64), r256 = items_of(t256), rpairs = items_of(pairs),
+                  rone = items_of(one), rdrag = items_of(drag), rtherm = items_of(therm), rturb = items_of(turb);
+        // fills: cell ranges of the particle binning; triangle counts per cell (and the pair count of
+        // a bucket without triangles)
+        std::vector<FillJob> fb, ft;
+        std::vector<int2> fbw, ftw;
+        for (int k = 0; k < n; ++k) {
+            const Dev<R>& d = host[k];
+            auto add = [&](std::vector<FillJob>& jobs, std::vector<int2>& w, int* ptr, int value, int cnt) {
+                const int j = (int)jobs.size();
+                jobs.push_back(FillJob{ptr, value, cnt});
+                for (int b = 0; b < (int)grid_for(cnt, 256); ++b) w.push_back(make_int2(j, b));
+            };
+            if (d.N > 0) {
+                add(fb, fbw, d.cell_start, -1, (int)d.c.ncell);
+                add(fb, fbw, d.cell_end, -1, (int)d.c.ncell);
+            }
+            add(ft, ftw, d.cell_tri_count, 0, (int)d.c.ncell);
+            if (d.ntri <= 0) add(ft, ftw, d.uniq_n, 0, 1);
+        }
+        const size_t ofb = put(fb.data(), fb.size() * sizeof(FillJob)), oft = put(ft.data(), ft.size() * sizeof(FillJob));
+        const Ref rfbw = items_of(fbw), rftw = items_of(ftw);
+        // scans of the triangle pass: emitted pairs per triangle, then the unique-pair flags
+        std::vector<ScanJob> se, sf;
+        std::vector<int2> se_t, se_1, sf_t, sf_1;
+        for (int k = 0; k < n; ++k) {
+            const Dev<R>& d = host[k];
+            if (d.ntri <= 0) continue;
+            const int je = (int)se.size();
+            se.push_back(ScanJob{d.tri_emit, d.tri_off, nullptr, d.ntri, d.ntri, d.pair_n, d.tsort.scan_tmp});
+            for (int b = 0; b < (int)grid_for(d.ntri, scan_tile()); ++b) se_t.push_back(make_int2(je, b));
+            se_1.push_back(make_int2(je, 0));
+            const int jf = (int)sf.size();
+            sf.push_back(ScanJob{d.pair_flag, d.pair_pos, d.pair_n, 0, d.pair_cap, d.uniq_n, d.tsort.scan_tmp});
+            for (int b = 0; b < (int)grid_for(d.pair_cap, scan_tile()); ++b) sf_t.push_back(make_int2(jf, b));
+            sf_1.push_back(make_int2(jf, 0));
+        }
+        const size_t ose = put(se.data(), se.size() * sizeof(ScanJob)), osf = put(sf.data(), sf.size() * sizeof(ScanJob));
+        const Ref rse_t = items_of(se_t), rse_1 = items_of(se_1), rsf_t = items_of(sf_t), rsf_1 = items_of(sf_1);
+        // radix passes: the particle sort (the buckets' key bits, `items` keys per thread) and the
+        // pair sort (8 per thread, device-resident pair count), pass by pass over the buckets that
+        // have that many passes
+        struct PassImg {
+            size_t jobs;
+            Ref tiles, singles;
+        };
+        auto passes = [&](bool pairs_sort, std::vector<PassImg>& out) {
+            const int tile = radix_tile(pairs_sort ? 8 : items);
+            for (int pass = 0;; ++pass) {
+                std::vector<SortJob> jobs;
+                std::vector<int2> tiles, singles;
+                for (int k = 0; k < n; ++k) {
+                    const Dev<R>& d = host[k];
+                    const int bits = pairs_sort ? tri_pair_bits(d.c.ncell) : d.key_bits;
+                    const int cap = pairs_sort ? d.pair_cap : d.N;
+                    if ((pairs_sort ? d.ntri <= 0 : d.N <= 0) || bits <= 0 || cap <= 0) continue;
+                    if (pass >= radix_passes(bits)) continue;
+                    uint32_t* k0 = pairs_sort ? d.pair_cell : d.keys;
+                    int* v0 = pairs_sort ? d.pair_tri : d.vals;
+                    const SortScratch& sc = pairs_sort ? d.tsort : d.sort;
+                    const bool even = (pass & 1) == 0;
+                    SortJob q{};
+                    q.kin = even ? k0 : sc.k1;
+                    q.vin = even ? v0 : sc.v1;
+                    q.kout = even ? sc.k1 : k0;
+                    q.vout = even ? sc.v1 : v0;
+                    q.d_n = pairs_sort ? d.pair_n : nullptr;
+                    q.n_host = pairs_sort ? 0 : d.N;
+                    q.n_cap = cap;
+                    q.nbits = radix_pass_bits(bits, pass, &q.shift);
+                    q.hist = sc.hist;
+                    const int j = (int)jobs.size();
+                    jobs.push_back(q);
+                    for (int b = 0; b < (int)grid_for(cap, tile); ++b) tiles.push_back(make_int2(j, b));
+                    singles.push_back(make_int2(j, 0));
+                }
+                if (jobs.empty()) break;
+                out.push_back(PassImg{put(jobs.data(), jobs.size() * sizeof(SortJob)), items_of(tiles), items_of(singles)});
+            }
+        };
+        std::vector<PassImg> pp, tp;
+        passes(false, pp);
+        passes(true, tp);
+        // upload and resolve the offsets
+        if (tab && img.size() > tab_bytes) {
+            (void)hipFree(tab);
+            tab = nullptr;
+        }
+        if (!tab) {
+            tab_bytes = std::max<size_t>(16, img.size());
+            HIPC(hipMalloc((void**)&tab, tab_bytes));
+        }
+        HIPC(hipMemcpy(tab, img.data(), img.size(), hipMemcpyHostToDevice));
+        auto wl = [&](Ref r) { return WorkList{reinterpret_cast<const int2*>(tab + r.off), r.n}; };
+        wp = wl(rp); wt64 = wl(r64); wt256 = wl(r256); wpairs = wl(rpairs); wone = wl(rone);
+        wdrag = wl(rdrag); wtherm = wl(rtherm); wturb = wl(rturb);
+        fill_bin = reinterpret_cast<const FillJob*>(tab + ofb);
+        fill_tri = reinterpret_cast<const FillJob*>(tab + oft);
+        fill_bin_w = wl(rfbw);
+        fill_tri_w = wl(rftw);
+        scan_emit = reinterpret_cast<const ScanJob*>(tab + ose);
+        scan_flag = reinterpret_cast<const ScanJob*>(tab + osf);
+        scan_emit_t = wl(rse_t); scan_emit_1 = wl(rse_1); scan_flag_t = wl(rsf_t); scan_flag_1 = wl(rsf_1);
+        psort.clear();
+        tsort.clear();
+        for (const PassImg& q : pp) psort.push_back(Pass{reinterpret_cast<const SortJob*>(tab + q.jobs), wl(q.tiles), wl(q.singles)});
+        for (const PassImg& q : tp) tsort.push_back(Pass{reinterpret_cast<const SortJob*>(tab + q.jobs), wl(q.tiles), wl(q.singles)});
+        sig = sg;
+        return MPSR_NOERR;
+    }
+
+    void bin(hipStream_t st) {
+        const BatchLaunch<R> b = launch();
+        fill_int_batched(fill_bin, fill_bin_w, st);
+        launch_bin_hash_b(b, wp, st);
+        for (const Pass& q : psort) radix_pass_batched(items, q.jobs, q.tiles, q.singles, st);
+        launch_bin_ranges_b(b, wp, st);
+    }
+    void tri(hipStream_t st) {
+        const BatchLaunch<R> b = launch();
+        fill_int_batched(fill_tri, fill_tri_w, st);
+        launch_tri_count_b(b, wt64, st);
+        exclusive_scan_batched(scan_emit, scan_emit_t, scan_emit_1, st);
+        launch_tri_fill_b(b, wt64, st);
+        for (const Pass& q : tsort) radix_pass_batched(8, q.jobs, q.tiles, q.singles, st);
+        launch_pairs_b(b, 0, wpairs, st);
+        exclusive_scan_batched(scan_flag, scan_flag_t, scan_flag_1, st);
+        launch_pairs_b(b, 1, wpairs, st);
+        launch_pairs_b(b, 2, wpairs, st);
+    }
+    // one EOS step of every bucket (the order of Engine::one_step), the rotation row of each bucket
+    // from its device step counter, which the step then advances
+    void enqueue(hipStream_t st) {
+        const BatchLaunch<R> b = launch();
+        bin(st);
+        tri(st);
+        launch_stl_distance_b(b, wp, st);
+        launch_drag_b(b, wdrag, st);
+        launch_thermal_b(b, wtherm, st);
+        launch_turbulence_b(b, wturb, st);
+        launch_explicit_b(b, wp, st);
+        launch_relocate_b(b, wp, st);
+        bin(st);
+        launch_stl_distance_b(b, wp, st);
+        launch_pnd_b(b, wp, st);
+        launch_gradient_b(b, wp, st);
+        launch_relocate_b(b, wp, st);
+        launch_translate_b(b, wt256, st);
+        launch_rotate_b(b, wt256, st);
+        launch_step_advance_b(b, wone, st);
+    }
+};
+
 }  // namespace mpsr
 
 using namespace mpsr;
@@ -1213,22 +1430,24 @@ struct mpsr_handle {
 };
 
 // Many independent domains ("buckets") advanced together: each keeps its own engine (grid, walls,
-// precision, stream); one step of all of them is recorded once into a HIP graph by forking every
-// bucket's stream from the batch stream and joining it back, and each step is then one graph
-// launch.  The rotation row of a replay comes from each bucket's device step counter.
+// precision, buffers); a step of all of them is the batched step of each precision group (every
+// kernel one launch over all buckets of that precision), recorded once into a HIP graph, and each
+// step is then one graph launch.  The rotation row of a replay comes from each bucket's device step
+// counter.
 struct mpsr_batch {
     std::vector<mpsr_handle*> h;
-    std::vector<int> order;            // recording order: largest bucket first
+    std::vector<int> order;            // largest bucket first
     int device = 0;
     hipStream_t main = nullptr;
-    hipEvent_t fork = nullptr, e0 = nullptr, e1 = nullptr;
-    std::vector<hipEvent_t> join;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;
     int block = 256, items = 8;
     size_t nodes = 0;
     float ms_per_step = 0.f;
+    BatchGroup<float> g32;
+    BatchGroup<double> g64;
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1236,14 +1455,16 @@ struct mpsr_batch {
         graph = nullptr;
         dirty = true;
     }
+    void enqueue() {
+        if (!g32.eng.empty()) g32.enqueue(main);
+        if (!g64.eng.empty()) g64.enqueue(main);
+    }
     ~mpsr_batch() {
         drop_graph();
         for (mpsr_handle* x : h) {
             delete x->e;
             delete x;
         }
-        for (hipEvent_t e : join) (void)hipEventDestroy(e);
-        if (fork) (void)hipEventDestroy(fork);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (main) (void)hipStreamDestroy(main);
@@ -1257,26 +1478,14 @@ mpsr_status batch_capture(mpsr_batch* b) {
     b->drop_graph();
     HIPC(hipSetDevice(b->device));
     HIPC(hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
-    mpsr_status st = MPSR_NOERR;
-    hipError_t he = hipEventRecord(b->fork, b->main);
-    for (size_t k = 0; k < b->order.size() && he == hipSuccess && st == MPSR_NOERR; ++k) {
-        const int i = b->order[k];
-        EngineBase* e = b->h[i]->e;
-        he = hipStreamWaitEvent(e->stream(), b->fork, 0);
-        if (he != hipSuccess) break;
-        st = e->batch_enqueue();
-        if (st != MPSR_NOERR) break;
-        he = hipEventRecord(b->join[i], e->stream());
-        if (he == hipSuccess) he = hipStreamWaitEvent(b->main, b->join[i], 0);
-    }
+    b->enqueue();
+    const hipError_t he = hipGetLastError();
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(b->main, &g);
-    if (st != MPSR_NOERR || he != hipSuccess || ee != hipSuccess) {
+    if (he != hipSuccess || ee != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         (void)hipGetLastError();
-        if (st != MPSR_NOERR) return st;
-        return fail(MPSR_GPUERR, std::string("batched step capture: ") +
-                                     hipGetErrorString(he != hipSuccess ? he : ee));
+        return fail(MPSR_GPUERR, std::string("batched step capture: ") + hipGetErrorString(he != hipSuccess ? he : ee));
     }
     b->graph = g;
     HIPC(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -1285,15 +1494,25 @@ mpsr_status batch_capture(mpsr_batch* b) {
     return MPSR_NOERR;
 }
 
+// rotation tables and step counters of every bucket, the device tables of both groups, and the
+// work lists when the launch shape or an optional phase changed (the graph then records anew)
 mpsr_status batch_prepare_all(mpsr_batch* b, int first, int count) {
-    int moved_any = 0;
     for (mpsr_handle* x : b->h) {
-        int moved = 0;
-        mpsr_status s = x->e->batch_prepare(first, count, &moved);
+        mpsr_status s = x->e->batch_prepare(first, count);
         if (s != MPSR_NOERR) return s;
-        moved_any |= moved;
     }
-    if (moved_any) b->dirty = true;
+    const long long s32 = b->g32.sig, s64 = b->g64.sig;
+    mpsr_status s = MPSR_NOERR;
+    if (!b->g32.eng.empty()) {
+        s = b->g32.refresh();
+        if (s == MPSR_NOERR) s = b->g32.build(b->block, b->items);
+    }
+    if (s == MPSR_NOERR && !b->g64.eng.empty()) {
+        s = b->g64.refresh();
+        if (s == MPSR_NOERR) s = b->g64.build(b->block, b->items);
+    }
+    if (s != MPSR_NOERR) return s;
+    if (b->g32.sig != s32 || b->g64.sig != s64) b->dirty = true;
     return MPSR_NOERR;
 }
 
@@ -1323,18 +1542,9 @@ mpsr_status batch_run(mpsr_batch* b, int first, int count, bool graph) {
     if (graph) {
         for (int k = 0; k < count; ++k) HIPC(hipGraphLaunch(b->exec, b->main));
     } else {
-        // the same kernels launched directly, bucket by bucket on the buckets' own streams
-        for (int k = 0; k < count; ++k) {
-            HIPC(hipEventRecord(b->fork, b->main));
-            for (const int i : b->order) {
-                EngineBase* e = b->h[i]->e;
-                HIPC(hipStreamWaitEvent(e->stream(), b->fork, 0));
-                s = e->batch_enqueue();
-                if (s != MPSR_NOERR) return s;
-                HIPC(hipEventRecord(b->join[i], e->stream()));
-                HIPC(hipStreamWaitEvent(b->main, b->join[i], 0));
-            }
-        }
+        // the same launches without the graph
+        for (int k = 0; k < count; ++k) b->enqueue();
+        HIPC(hipGetLastError());
     }
     return batch_finish_all(b, count);
 }
@@ -1424,8 +1634,7 @@ mpsr_status mpsr_create_batch(const mpsr_create_info* infos, int nbuckets, mpsr_
         }
     }
     if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
-        hipEventCreate(&b->e1) != hipSuccess) {
+        hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
         delete b;
         return fail(MPSR_GPUERR, "mpsr_create_batch: stream or event creation failed");
     }
@@ -1433,12 +1642,9 @@ mpsr_status mpsr_create_batch(const mpsr_create_info* infos, int nbuckets, mpsr_
     for (size_t i = 0; i < b->h.size(); ++i) b->order[i] = (int)i;
     std::stable_sort(b->order.begin(), b->order.end(),
                      [&](int x, int y) { return b->h[x]->e->count() > b->h[y]->e->count(); });
-    b->join.resize(b->h.size(), nullptr);
-    for (auto& e : b->join) {
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-            delete b;
-            return fail(MPSR_GPUERR, "mpsr_create_batch: event creation failed");
-        }
+    for (const int i : b->order) {
+        if (auto* e32 = dynamic_cast<Engine<float>*>(b->h[i]->e)) b->g32.eng.push_back(e32);
+        else if (auto* e64 = dynamic_cast<Engine<double>*>(b->h[i]->e)) b->g64.eng.push_back(e64);
     }
     *out = b;
     return MPSR_NOERR;

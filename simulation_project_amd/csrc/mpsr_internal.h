@@ -296,6 +296,59 @@ void exclusive_scan(const int* in, int* out, const int* d_n, int n_host, int n_c
                     const SortScratch& s, hipStream_t st);
 void fill_int(int* p, int value, long long n, hipStream_t st);
 
+// ---- batched launches ----------------------------------------------------------------------
+// One launch runs a kernel body for several domains.  Work item i of a launch is (domain or job,
+// block of it); every item is one workgroup, which runs that block of the domain's body.
+struct WorkList {
+    const int2* items = nullptr;
+    int n = 0;
+};
+template <int MAXT, typename Op>
+__global__ __launch_bounds__(MAXT) void k_batched(Op op, const int2* __restrict__ items) {
+    const int2 w = items[blockIdx.x];
+    op(w.x, w.y);
+}
+template <int MAXT, typename Op> inline void launch_batched(const Op& op, WorkList w, int block, hipStream_t st) {
+    if (w.n > 0) hipLaunchKernelGGL((k_batched<MAXT, Op>), dim3(w.n), dim3(block), 0, st, op, w.items);
+}
+// one pass of one domain's radix sort (the arguments of a pass of radix_sort_pairs)
+struct SortJob {
+    const uint32_t* kin;
+    const int* vin;
+    uint32_t* kout;
+    int* vout;
+    const int* d_n;
+    int n_host, n_cap, shift, nbits;
+    int* hist;
+};
+// one domain's exclusive scan (the arguments of exclusive_scan)
+struct ScanJob {
+    const int* in;
+    int* out;
+    const int* d_n;
+    int n_host, n_cap;
+    int* d_total;
+    int* tmp;
+};
+struct FillJob {
+    int* p;
+    int value;
+    int n;
+};
+int radix_tile(int items);                  // keys per sort tile
+// passes of radix_sort_pairs over `bits` key bits (an odd count leaves the result in the scratch)
+__host__ __device__ inline int radix_passes_dev(int bits) { return bits > 0 ? (bits + 7) / 8 : 0; }
+int radix_passes(int bits);
+int radix_pass_bits(int bits, int pass, int* shift);
+int scan_tile();                            // ints per tile of the exclusive scan
+constexpr int kScanBlock = 1024;            // threads of the scan kernels
+// fills: one work item per (job, block of 256 ints)
+void fill_int_batched(const FillJob* jobs, WorkList blocks, hipStream_t st);
+// exclusive scans: tiles = (job, tile of scan_tile() ints), singles = (job, 0)
+void exclusive_scan_batched(const ScanJob* jobs, WorkList tiles, WorkList singles, hipStream_t st);
+// one radix pass of several sorts: tiles = (job, tile of radix_tile(items) keys), singles = (job, 0)
+void radix_pass_batched(int items, const SortJob* jobs, WorkList tiles, WorkList singles, hipStream_t st);
+
 }  // namespace mpsr
 
 namespace mpsr {
@@ -345,6 +398,8 @@ template <typename R> struct Dev {
     int* overflow = nullptr;          // set when a march emits more than pair_cap pairs
     int* cell_tri_start = nullptr;    // [ncell]
     int* cell_tri_count = nullptr;    // [ncell], at most 100
+    const int* step_base = nullptr;   // batched step: device index of the first replayed step
+    int thermal_step = 1;             // batched step: heat conduction every this many steps
     // nearest wall and drag, sorted order
     R* stl_mag = nullptr;
     R4<R>* stl_dir = nullptr;
@@ -394,6 +449,12 @@ template <typename R> void launch_neighbor_count(const Dev<R>& d, R re, int* cou
 template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int* offsets, int* nbr, hipStream_t st);
 // walls.hip
 template <typename R> void launch_tri_bin(Dev<R>& d, hipStream_t st);
+// key bits of the (cell, triangle) pair sort of a grid of ncell cells
+__host__ __device__ inline int tri_pair_bits(long long ncell) {
+    int bits = 1;
+    while ((1ll << bits) < ncell) ++bits;
+    return bits;
+}
 template <typename R> void launch_tri_count_total(Dev<R>& d, hipStream_t st);
 template <typename R> void launch_stl_distance(const Dev<R>& d, hipStream_t st);
 template <typename R> void launch_drag(const Dev<R>& d, hipStream_t st);
@@ -421,5 +482,32 @@ template <typename R>
 void launch_bucket_reuse(const Dev<R>& d, const PState<R>& b, int B, int* flag, int* rank, int* dead, hipStream_t st);
 template <typename R> void launch_bucket_append(const Dev<R>& d, const PState<R>& b, int first, int count, hipStream_t st);
 template <typename R> void launch_reset_walls(const Dev<R>& d, const TriD<R>* tris0, int step, hipStream_t st);
+
+// ---- batched forms of the step kernels ------------------------------------------------------
+// devs: device table of the domains; a work list of (domain, block).  P: blocks of `block`
+// particles; T64 / T256: blocks of 64 / 256 triangles; pairs: blocks of 256 of the pair capacity;
+// one: one item per domain.  Each domain's Dev holds the buffers its last sort left the keys and
+// permutation in (skeys, perm), and for replays its rotation table and step counter.
+template <typename R> struct BatchLaunch {
+    const Dev<R>* devs = nullptr;
+    int block = 256;
+};
+template <typename R> void launch_bin_hash_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
+template <typename R> void launch_bin_ranges_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
+template <typename R> void launch_explicit_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
+template <typename R> void launch_relocate_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
+template <typename R> void launch_pnd_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);   // EOS pressure
+template <typename R> void launch_gradient_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
+template <typename R> void launch_thermal_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
+template <typename R> void launch_turbulence_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
+template <typename R> void launch_tri_count_b(const BatchLaunch<R>& b, WorkList t64, hipStream_t st);
+template <typename R> void launch_tri_fill_b(const BatchLaunch<R>& b, WorkList t64, hipStream_t st);
+// the pair kernels of launch_tri_bin after the pair sort; `pass` = 0 flags, 1 compacts, 2 ranges
+template <typename R> void launch_pairs_b(const BatchLaunch<R>& b, int pass, WorkList pairs, hipStream_t st);
+template <typename R> void launch_stl_distance_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
+template <typename R> void launch_drag_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
+template <typename R> void launch_translate_b(const BatchLaunch<R>& b, WorkList t256, hipStream_t st);
+template <typename R> void launch_rotate_b(const BatchLaunch<R>& b, WorkList t256, hipStream_t st);
+template <typename R> void launch_step_advance_b(const BatchLaunch<R>& b, WorkList one, hipStream_t st);
 
 }  // namespace mpsr

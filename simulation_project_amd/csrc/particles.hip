@@ -51,9 +51,9 @@ __constant__ ZTable kZlap = {
 
 // ---------------------------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(kMaxBlock) void k_bin_hash(int N, Consts<R> c, PState<R> in, PState<R> out, uint32_t* keys,
-                                                   int* vals) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void bin_hash_body(int blk, int N, Consts<R> c, PState<R> in, PState<R> out, uint32_t* keys,
+                                     int* vals) {
+    const int s = blk * blockDim.x + threadIdx.x;
     if (s >= N) return;
     const int id = in.id[s];
     const R x = in.px[s], y = in.py[s], z = in.pz[s];
@@ -69,12 +69,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_bin_hash(int N, Consts<R> c, PSta
     keys[s] = bin_key(c, id, x, y, z);
     vals[s] = s;
 }
+template <typename R>
+__global__ __launch_bounds__(kMaxBlock) void k_bin_hash(int N, Consts<R> c, PState<R> in, PState<R> out, uint32_t* keys,
+                                                   int* vals) {
+    bin_hash_body<R>(blockIdx.x, N, c, in, out, keys, vals);
+}
 
 template <typename R>
-__global__ __launch_bounds__(kMaxBlock) void k_bin_ranges(int N, int ncell, const uint32_t* __restrict__ sk,
-                                                     const int* __restrict__ perm, int* cs, int* ce, PState<R> in,
-                                                     R4<R>* __restrict__ spos, R4<R>* __restrict__ svel) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void bin_ranges_body(int blk, int N, int ncell, const uint32_t* __restrict__ sk,
+                                       const int* __restrict__ perm, int* cs, int* ce, PState<R> in,
+                                       R4<R>* __restrict__ spos, R4<R>* __restrict__ svel) {
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const uint32_t k = sk[t];
     const uint32_t kp = t > 0 ? sk[t - 1] : 0xffffffffu;
@@ -88,20 +93,24 @@ __global__ __launch_bounds__(kMaxBlock) void k_bin_ranges(int N, int ncell, cons
     spos[t] = mk4<R>(in.px[s], in.py[s], in.pz[s], R(0));
     svel[t] = mk4<R>(in.vx[s], in.vy[s], in.vz[s], R(0));
 }
+template <typename R>
+__global__ __launch_bounds__(kMaxBlock) void k_bin_ranges(int N, int ncell, const uint32_t* __restrict__ sk,
+                                                     const int* __restrict__ perm, int* cs, int* ce, PState<R> in,
+                                                     R4<R>* __restrict__ spos, R4<R>* __restrict__ svel) {
+    bin_ranges_body<R>(blockIdx.x, N, ncell, sk, perm, cs, ce, in, spos, svel);
+}
 
 // ---------------------------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(kMaxBlock) void k_explicit(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
-                                                   PState<R> out, const R4<R>* __restrict__ spos,
-                                                   const R4<R>* __restrict__ svel, const int* __restrict__ cs,
-                                                   const int* __restrict__ ce, const R* __restrict__ stl_mag,
-                                                   const R4<R>* __restrict__ stl_dir, const int* __restrict__ stl_id,
-                                                   const R4<R>* __restrict__ drag_acc, const R4<R>* __restrict__ sph_dv,
-                                                   const TriD<R>* __restrict__ tris,
-                                                   const mpsr_tri_params* __restrict__ models,
-                                                   const R* __restrict__ turb_nu) {
-    const int blk = xcd_block(N);
-    if (blk < 0) return;
+__device__ inline void explicit_body(int blk, int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                     PState<R> out, const R4<R>* __restrict__ spos,
+                                     const R4<R>* __restrict__ svel, const int* __restrict__ cs,
+                                     const int* __restrict__ ce, const R* __restrict__ stl_mag,
+                                     const R4<R>* __restrict__ stl_dir, const int* __restrict__ stl_id,
+                                     const R4<R>* __restrict__ drag_acc, const R4<R>* __restrict__ sph_dv,
+                                     const TriD<R>* __restrict__ tris,
+                                     const mpsr_tri_params* __restrict__ models,
+                                     const R* __restrict__ turb_nu) {
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
@@ -197,11 +206,26 @@ __global__ __launch_bounds__(kMaxBlock) void k_explicit(int N, Consts<R> c, cons
         out.vx[s] = vx; out.vy[s] = vy; out.vz[s] = vz;
     }
 }
+template <typename R>
+__global__ __launch_bounds__(kMaxBlock) void k_explicit(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                                   PState<R> out, const R4<R>* __restrict__ spos,
+                                                   const R4<R>* __restrict__ svel, const int* __restrict__ cs,
+                                                   const int* __restrict__ ce, const R* __restrict__ stl_mag,
+                                                   const R4<R>* __restrict__ stl_dir, const int* __restrict__ stl_id,
+                                                   const R4<R>* __restrict__ drag_acc, const R4<R>* __restrict__ sph_dv,
+                                                   const TriD<R>* __restrict__ tris,
+                                                   const mpsr_tri_params* __restrict__ models,
+                                                   const R* __restrict__ turb_nu) {
+    const int blk = xcd_block(N);
+    if (blk < 0) return;
+    explicit_body<R>(blk, N, c, perm, in, out, spos, svel, cs, ce, stl_mag, stl_dir, stl_id, drag_acc, sph_dv, tris,
+                     models, turb_nu);
+}
 
 // ---------------------------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(kMaxBlock) void k_relocate(int N, Consts<R> c, PState<R> in, PState<R> out) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void relocate_body(int blk, int N, Consts<R> c, PState<R> in, PState<R> out) {
+    const int s = blk * blockDim.x + threadIdx.x;
     if (s >= N) return;
     if (out.id[s] < 0) return;
     const R x = out.px[s], y = out.py[s], z = out.pz[s];
@@ -218,6 +242,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_relocate(int N, Consts<R> c, PSta
     in.p[s] = out.p[s]; in.n[s] = out.n[s]; in.temp[s] = out.temp[s];
     in.nu[s] = out.nu[s]; in.solid[s] = out.solid[s]; in.type[s] = out.type[s];
 }
+template <typename R>
+__global__ __launch_bounds__(kMaxBlock) void k_relocate(int N, Consts<R> c, PState<R> in, PState<R> out) {
+    relocate_body<R>(blockIdx.x, N, c, in, out);
+}
 
 // ---------------------------------------------------------------------------------------------
 // PND (and the EOS pressure when eos != 0).  spres[t] receives the slot's out-pressure after the
@@ -232,15 +260,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_relocate(int N, Consts<R> c, PSta
 // so both give the same bits.  The first nbw_width Laplacian-radius neighbours and their weights go
 // to a per-row list (SELL-64/4 by sorted index) that the fill reads instead of scanning again.
 template <typename R, bool PPE>
-__global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, const int* __restrict__ perm, PState<R> in,
-                                              PState<R> out, const R4<R>* __restrict__ spos, R* __restrict__ spres,
-                                              const int* __restrict__ cs, const int* __restrict__ ce,
-                                              const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
-                                              const uint32_t* __restrict__ skeys, int* __restrict__ active,
-                                              int* __restrict__ rcnt, double* __restrict__ rdiag, int* __restrict__ rdir,
-                                              int* __restrict__ nbq, R* __restrict__ nbw, int* __restrict__ nbn,
-                                              int nb_width) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int* __restrict__ perm, PState<R> in,
+                                PState<R> out, const R4<R>* __restrict__ spos, R* __restrict__ spres,
+                                const int* __restrict__ cs, const int* __restrict__ ce,
+                                const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
+                                const uint32_t* __restrict__ skeys, int* __restrict__ active,
+                                int* __restrict__ rcnt, double* __restrict__ rdiag, int* __restrict__ rdir,
+                                int* __restrict__ nbq, R* __restrict__ nbw, int* __restrict__ nbn,
+                                int nb_width) {
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
     const R4<R> P = spos[t];
@@ -352,20 +380,30 @@ __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, 
     }
     spres[t] = pout;
 }
+template <typename R, bool PPE>
+__global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, const int* __restrict__ perm, PState<R> in,
+                                              PState<R> out, const R4<R>* __restrict__ spos, R* __restrict__ spres,
+                                              const int* __restrict__ cs, const int* __restrict__ ce,
+                                              const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
+                                              const uint32_t* __restrict__ skeys, int* __restrict__ active,
+                                              int* __restrict__ rcnt, double* __restrict__ rdiag, int* __restrict__ rdir,
+                                              int* __restrict__ nbq, R* __restrict__ nbw, int* __restrict__ nbn,
+                                              int nb_width) {
+    pnd_body<R, PPE>(blockIdx.x, N, c, eos, perm, in, out, spos, spres, cs, ce, stl_mag, stl_id, skeys, active, rcnt,
+                     rdiag, rdir, nbq, nbw, nbn, nb_width);
+}
 
 // ---------------------------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(kMaxBlock) void k_gradient(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
-                                                   PState<R> out, const R4<R>* __restrict__ spos,
-                                                   const R4<R>* __restrict__ svel, const R* __restrict__ spres,
-                                                   const int* __restrict__ cs, const int* __restrict__ ce,
-                                                   const R* __restrict__ stl_mag, const R4<R>* __restrict__ stl_dir,
-                                                   const int* __restrict__ stl_id, const int* __restrict__ tstart,
-                                                   const int* __restrict__ tcount, const int* __restrict__ tids,
-                                                   const TriD<R>* __restrict__ tris,
-                                                   const NegModel<R>* __restrict__ negm, NbList nb) {
-    const int blk = xcd_block(N);
-    if (blk < 0) return;
+__device__ inline void gradient_body(int blk, int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                     PState<R> out, const R4<R>* __restrict__ spos,
+                                     const R4<R>* __restrict__ svel, const R* __restrict__ spres,
+                                     const int* __restrict__ cs, const int* __restrict__ ce,
+                                     const R* __restrict__ stl_mag, const R4<R>* __restrict__ stl_dir,
+                                     const int* __restrict__ stl_id, const int* __restrict__ tstart,
+                                     const int* __restrict__ tcount, const int* __restrict__ tids,
+                                     const TriD<R>* __restrict__ tris,
+                                     const NegModel<R>* __restrict__ negm, NbList nb) {
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
@@ -448,6 +486,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_gradient(int N, Consts<R> c, cons
         out.vx[s] = vx; out.vy[s] = vy; out.vz[s] = vz;
     }
 }
+template <typename R>
+__global__ __launch_bounds__(kMaxBlock) void k_gradient(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                                   PState<R> out, const R4<R>* __restrict__ spos,
+                                                   const R4<R>* __restrict__ svel, const R* __restrict__ spres,
+                                                   const int* __restrict__ cs, const int* __restrict__ ce,
+                                                   const R* __restrict__ stl_mag, const R4<R>* __restrict__ stl_dir,
+                                                   const int* __restrict__ stl_id, const int* __restrict__ tstart,
+                                                   const int* __restrict__ tcount, const int* __restrict__ tids,
+                                                   const TriD<R>* __restrict__ tris,
+                                                   const NegModel<R>* __restrict__ negm, NbList nb) {
+    const int blk = xcd_block(N);
+    if (blk < 0) return;
+    gradient_body<R>(blk, N, c, perm, in, out, spos, svel, spres, cs, ce, stl_mag, stl_dir, stl_id, tstart, tcount,
+                     tids, tris, negm, nb);
+}
 
 // ---------------------------------------------------------------------------------------------
 // neighbour lists (inspection and PPE structure): count, then fill with slot ids
@@ -488,17 +541,15 @@ __global__ __launch_bounds__(256) void k_nbr_fill(int N, Consts<R> c, R re, cons
 // reference's SCALAR_EPSILON in both precisions), T += EnthalpyCoefficient (Hf + Hw), limited to
 // [MinWallTemperature, MaxWallTemperature]; the out arrays get T, A T + B and the enthalpy term.
 template <typename R>
-__global__ __launch_bounds__(kMaxBlock) void k_thermal(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
-                                                    PState<R> out, const R4<R>* __restrict__ spos,
-                                                    const int* __restrict__ cs, const int* __restrict__ ce,
-                                                    const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
-                                                    const TriD<R>* __restrict__ tris,
-                                                    const mpsr_tri_params* __restrict__ models,
-                                                    const R* __restrict__ drag_temp, const int* __restrict__ step_base,
-                                                    const int* __restrict__ step_ctr, int thermal_step) {
+__device__ inline void thermal_body(int blk, int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                    PState<R> out, const R4<R>* __restrict__ spos,
+                                    const int* __restrict__ cs, const int* __restrict__ ce,
+                                    const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
+                                    const TriD<R>* __restrict__ tris,
+                                    const mpsr_tri_params* __restrict__ models,
+                                    const R* __restrict__ drag_temp, const int* __restrict__ step_base,
+                                    const int* __restrict__ step_ctr, int thermal_step) {
     if (step_ctr && ((*step_base + *step_ctr) % thermal_step) != 0) return;
-    const int blk = xcd_block(N);
-    if (blk < 0) return;
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
@@ -544,6 +595,20 @@ __global__ __launch_bounds__(kMaxBlock) void k_thermal(int N, Consts<R> c, const
     out.temp[s] = temp;
     out.solid[s] = c.enth * comb;
 }
+template <typename R>
+__global__ __launch_bounds__(kMaxBlock) void k_thermal(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                                    PState<R> out, const R4<R>* __restrict__ spos,
+                                                    const int* __restrict__ cs, const int* __restrict__ ce,
+                                                    const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
+                                                    const TriD<R>* __restrict__ tris,
+                                                    const mpsr_tri_params* __restrict__ models,
+                                                    const R* __restrict__ drag_temp, const int* __restrict__ step_base,
+                                                    const int* __restrict__ step_ctr, int thermal_step) {
+    const int blk = xcd_block(N);
+    if (blk < 0) return;
+    thermal_body<R>(blk, N, c, perm, in, out, spos, cs, ce, stl_mag, stl_id, tris, models, drag_temp, step_base,
+                    step_ctr, thermal_step);
+}
 
 template <typename R> void launch_bin_hash(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
@@ -570,13 +635,11 @@ template <typename R> void launch_bin_ranges(Dev<R>& d, hipStream_t st) {
 // to slot order; a particle whose cell lies outside the grid gets 0 (the reference leaves the
 // array as it was).
 template <typename R>
-__global__ __launch_bounds__(kMaxBlock) void k_turbulence(int N, Consts<R> c, const int* __restrict__ perm,
-                                                     PState<R> out, const R4<R>* __restrict__ spos,
-                                                     const R4<R>* __restrict__ svel, const int* __restrict__ cs,
-                                                     const int* __restrict__ ce, const R* __restrict__ stl_mag,
-                                                     R* __restrict__ turb_nu, R* __restrict__ turb_k) {
-    const int blk = xcd_block(N);
-    if (blk < 0) return;
+__device__ inline void turbulence_body(int blk, int N, Consts<R> c, const int* __restrict__ perm,
+                                       PState<R> out, const R4<R>* __restrict__ spos,
+                                       const R4<R>* __restrict__ svel, const int* __restrict__ cs,
+                                       const int* __restrict__ ce, const R* __restrict__ stl_mag,
+                                       R* __restrict__ turb_nu, R* __restrict__ turb_k) {
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
@@ -643,6 +706,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_turbulence(int N, Consts<R> c, co
     turb_nu[s] = nut;
     turb_k[s] = k;
 }
+template <typename R>
+__global__ __launch_bounds__(kMaxBlock) void k_turbulence(int N, Consts<R> c, const int* __restrict__ perm,
+                                                     PState<R> out, const R4<R>* __restrict__ spos,
+                                                     const R4<R>* __restrict__ svel, const int* __restrict__ cs,
+                                                     const int* __restrict__ ce, const R* __restrict__ stl_mag,
+                                                     R* __restrict__ turb_nu, R* __restrict__ turb_k) {
+    const int blk = xcd_block(N);
+    if (blk < 0) return;
+    turbulence_body<R>(blk, N, c, perm, out, spos, svel, cs, ce, stl_mag, turb_nu, turb_k);
+}
 
 template <typename R> void launch_explicit(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
@@ -697,6 +770,94 @@ template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int
                        d.cell_start, d.cell_end, offsets, nbr);
 }
 
+// ---- batched forms: one work item = one block of one domain's particles ----------------------
+template <typename R> struct OpBinHash {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        bin_hash_body<R>(b, d.N, d.c, d.in, d.out, d.keys, d.vals);
+    }
+};
+template <typename R> struct OpBinRanges {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        bin_ranges_body<R>(b, d.N, d.c.ncell, d.skeys, d.perm, d.cell_start, d.cell_end, d.in, d.spos, d.svel);
+    }
+};
+template <typename R> struct OpExplicit {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        explicit_body<R>(b, d.N, d.c, d.perm, d.in, d.out, d.spos, d.svel, d.cell_start, d.cell_end, d.stl_mag,
+                         d.stl_dir, d.stl_id, d.drag_acc, d.sph_dv, d.tris, d.models, d.turb_nu);
+    }
+};
+template <typename R> struct OpRelocate {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        relocate_body<R>(b, d.N, d.c, d.in, d.out);
+    }
+};
+template <typename R> struct OpPndEos {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        pnd_body<R, false>(b, d.N, d.c, 1, d.perm, d.in, d.out, d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag,
+                           d.stl_id, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+    }
+};
+template <typename R> struct OpGradient {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        gradient_body<R>(b, d.N, d.c, d.perm, d.in, d.out, d.spos, d.svel, d.spres, d.cell_start, d.cell_end,
+                         d.stl_mag, d.stl_dir, d.stl_id, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris,
+                         d.negm, NbList{});
+    }
+};
+template <typename R> struct OpThermal {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        thermal_body<R>(b, d.N, d.c, d.perm, d.in, d.out, d.spos, d.cell_start, d.cell_end, d.stl_mag, d.stl_id,
+                        d.tris, d.models, d.drag_temp, d.step_base, d.step_ctr, d.thermal_step);
+    }
+};
+template <typename R> struct OpTurbulence {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        turbulence_body<R>(b, d.N, d.c, d.perm, d.out, d.spos, d.svel, d.cell_start, d.cell_end, d.stl_mag,
+                           d.turb_nu, d.turb_k);
+    }
+};
+template <typename R> void launch_bin_hash_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
+    launch_batched<kMaxBlock>(OpBinHash<R>{b.devs}, p, b.block, st);
+}
+template <typename R> void launch_bin_ranges_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
+    launch_batched<kMaxBlock>(OpBinRanges<R>{b.devs}, p, b.block, st);
+}
+template <typename R> void launch_explicit_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
+    launch_batched<kMaxBlock>(OpExplicit<R>{b.devs}, p, b.block, st);
+}
+template <typename R> void launch_relocate_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
+    launch_batched<kMaxBlock>(OpRelocate<R>{b.devs}, p, b.block, st);
+}
+template <typename R> void launch_pnd_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
+    launch_batched<kMaxBlock>(OpPndEos<R>{b.devs}, p, b.block, st);
+}
+template <typename R> void launch_gradient_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
+    launch_batched<kMaxBlock>(OpGradient<R>{b.devs}, p, b.block, st);
+}
+template <typename R> void launch_thermal_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
+    launch_batched<kMaxBlock>(OpThermal<R>{b.devs}, p, b.block, st);
+}
+template <typename R> void launch_turbulence_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
+    launch_batched<kMaxBlock>(OpTurbulence<R>{b.devs}, p, b.block, st);
+}
+
 #define MPSR_INSTANTIATE(R)                                                                   \
     template void launch_bin_hash<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_bin_ranges<R>(Dev<R>&, hipStream_t);                                 \
@@ -708,7 +869,15 @@ template <typename R> void launch_neighbor_fill(const Dev<R>& d, R re, const int
     template void launch_thermal<R>(const Dev<R>&, const int*, const int*, int, hipStream_t);   \
     template void launch_turbulence<R>(const Dev<R>&, hipStream_t);                           \
     template void launch_neighbor_count<R>(const Dev<R>&, R, int*, hipStream_t);              \
-    template void launch_neighbor_fill<R>(const Dev<R>&, R, const int*, int*, hipStream_t);
+    template void launch_neighbor_fill<R>(const Dev<R>&, R, const int*, int*, hipStream_t);   \
+    template void launch_bin_hash_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);         \
+    template void launch_bin_ranges_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);       \
+    template void launch_explicit_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);         \
+    template void launch_relocate_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);         \
+    template void launch_pnd_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);              \
+    template void launch_gradient_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);         \
+    template void launch_thermal_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);          \
+    template void launch_turbulence_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);
 MPSR_INSTANTIATE(float)
 MPSR_INSTANTIATE(double)
 

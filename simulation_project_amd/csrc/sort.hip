@@ -34,10 +34,11 @@ size_t scan_tmp_ints(int n_cap) { return (size_t)((n_cap + SCAN_TILE - 1) / SCAN
 // element count: the device-resident count clamped to the capacity, or the host count
 __device__ inline int dev_n(const int* d_n, int n_host, int n_cap) { return d_n ? min(*d_n, n_cap) : n_host; }
 
-__global__ void k_fill_int(int* p, int value, long long n) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void fill_int_body(int blk, int* p, int value, long long n) {
+    long long i = (long long)blk * blockDim.x + threadIdx.x;
     if (i < n) p[i] = value;
 }
+__global__ void k_fill_int(int* p, int value, long long n) { fill_int_body(blockIdx.x, p, value, n); }
 void fill_int(int* p, int value, long long n, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_fill_int, dim3(grid_for(n, 256)), dim3(256), 0, st, p, value, n);
@@ -46,9 +47,9 @@ void fill_int(int* p, int value, long long n, hipStream_t st) {
 // ---------------------------------------------------------------------------------------------
 // generic exclusive scan: per-tile sums -> scan of tile sums in one workgroup -> tile scans
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile_sums(const int* in, const int* d_n, int n_host, int n_cap, int* sums) {
+__device__ inline void scan_tile_sums_body(int blk, const int* in, const int* d_n, int n_host, int n_cap, int* sums) {
     const int n = dev_n(d_n, n_host, n_cap);
-    const int base = blockIdx.x * SCAN_TILE;
+    const int base = blk * SCAN_TILE;
     if (base >= n) return;
     __shared__ int lds[SCAN_BLOCK / 64 + 1];
     int s = 0;
@@ -63,15 +64,18 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile_sums(const int* in, co
     if (threadIdx.x == 0) {
         int t = 0;
         for (int w = 0; w < SCAN_BLOCK / 64; ++w) t += lds[w];
-        sums[blockIdx.x] = t;
+        sums[blk] = t;
     }
+}
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile_sums(const int* in, const int* d_n, int n_host, int n_cap,
+                                                               int* sums) {
+    scan_tile_sums_body(blockIdx.x, in, d_n, n_host, n_cap, sums);
 }
 
 // In-place exclusive scan of m = ceil(n / tile) ints in one workgroup; total to sums[m] and to
 // *d_total.  Used for tile sums (tile = SCAN_TILE) and for radix histograms (tile = 1 with the
 // element count given directly).
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(int* a, const int* d_n, int n_host, int n_cap, int tile, int mult,
-                                                            int* d_total) {
+__device__ inline void scan_single_body(int* a, const int* d_n, int n_host, int n_cap, int tile, int mult, int* d_total) {
     const int n = dev_n(d_n, n_host, n_cap);
     const int m = ((n + tile - 1) / tile) * mult;
     __shared__ int lds[SCAN_BLOCK / 64 + 1];
@@ -100,11 +104,15 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(int* a, const int* d
         if (d_total) *d_total = carry;
     }
 }
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(int* a, const int* d_n, int n_host, int n_cap, int tile, int mult,
+                                                            int* d_total) {
+    scan_single_body(a, d_n, n_host, n_cap, tile, mult, d_total);
+}
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const int* in, int* out, const int* d_n, int n_host,
-                                                           int n_cap, const int* sums) {
+__device__ inline void scan_tiles_body(int blk, const int* in, int* out, const int* d_n, int n_host, int n_cap,
+                                       const int* sums) {
     const int n = dev_n(d_n, n_host, n_cap);
-    const int base = blockIdx.x * SCAN_TILE;
+    const int base = blk * SCAN_TILE;
     if (base >= n) return;
     __shared__ int lds[SCAN_BLOCK / 64 + 1];
     int v[SCAN_ITEMS];
@@ -116,12 +124,16 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const int* in, int* o
         local += v[r];
     }
     int total;
-    int run = block_exclusive_scan<SCAN_BLOCK / 64>(local, lds, &total) + sums[blockIdx.x];
+    int run = block_exclusive_scan<SCAN_BLOCK / 64>(local, lds, &total) + sums[blk];
 #pragma unroll
     for (int r = 0; r < SCAN_ITEMS; ++r) {
         if (i0 + r < n) out[i0 + r] = run;
         run += v[r];
     }
+}
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const int* in, int* out, const int* d_n, int n_host,
+                                                           int n_cap, const int* sums) {
+    scan_tiles_body(blockIdx.x, in, out, d_n, n_host, n_cap, sums);
 }
 
 void exclusive_scan(const int* in, int* out, const int* d_n, int n_host, int n_cap, int* d_total,
@@ -142,12 +154,11 @@ void exclusive_scan(const int* in, int* out, const int* d_n, int n_host, int n_c
 // radix sort
 // ---------------------------------------------------------------------------------------------
 template <int RS_ITEMS>
-__global__ __launch_bounds__(RS_BLOCK) void k_rs_upsweep(const uint32_t* keys, const int* d_n, int n_host, int n_cap, int shift,
-                                                          int nbits, int* hist) {
+__device__ inline void rs_upsweep_body(int b, const uint32_t* keys, const int* d_n, int n_host, int n_cap, int shift,
+                                       int nbits, int* hist) {
     constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
     const int n = dev_n(d_n, n_host, n_cap);
     const int ntiles = (n + RS_TILE - 1) / RS_TILE;
-    const int b = blockIdx.x;
     if (b >= ntiles) return;
     const int radix = 1 << nbits;
     const uint32_t mask = (uint32_t)radix - 1u;
@@ -163,16 +174,19 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_upsweep(const uint32_t* keys, c
     __syncthreads();
     for (int d = threadIdx.x; d < radix; d += RS_BLOCK) hist[d * ntiles + b] = h[d];
 }
+template <int RS_ITEMS>
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_upsweep(const uint32_t* keys, const int* d_n, int n_host, int n_cap, int shift,
+                                                          int nbits, int* hist) {
+    rs_upsweep_body<RS_ITEMS>(blockIdx.x, keys, d_n, n_host, n_cap, shift, nbits, hist);
+}
 
 template <int RS_ITEMS>
-__global__ __launch_bounds__(RS_BLOCK) void k_rs_downsweep(const uint32_t* __restrict__ kin, const int* __restrict__ vin,
-                                                            uint32_t* __restrict__ kout, int* __restrict__ vout,
-                                                            const int* d_n, int n_host, int n_cap, int shift, int nbits,
-                                                            const int* __restrict__ hist) {
+__device__ inline void rs_downsweep_body(int b, const uint32_t* __restrict__ kin, const int* __restrict__ vin,
+                                         uint32_t* __restrict__ kout, int* __restrict__ vout, const int* d_n,
+                                         int n_host, int n_cap, int shift, int nbits, const int* __restrict__ hist) {
     constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
     const int n = dev_n(d_n, n_host, n_cap);
     const int ntiles = (n + RS_TILE - 1) / RS_TILE;
-    const int b = blockIdx.x;
     if (b >= ntiles) return;
     const int radix = 1 << nbits;
     const uint32_t mask = (uint32_t)radix - 1u;
@@ -252,6 +266,13 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_downsweep(const uint32_t* __res
         vout[g] = sv[i];
     }
 }
+template <int RS_ITEMS>
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_downsweep(const uint32_t* __restrict__ kin, const int* __restrict__ vin,
+                                                            uint32_t* __restrict__ kout, int* __restrict__ vout,
+                                                            const int* d_n, int n_host, int n_cap, int shift, int nbits,
+                                                            const int* __restrict__ hist) {
+    rs_downsweep_body<RS_ITEMS>(blockIdx.x, kin, vin, kout, vout, d_n, n_host, n_cap, shift, nbits, hist);
+}
 
 template <int I>
 static int radix_sort_tiles(uint32_t* k0, int* v0, const int* d_n, int n_host, int cap, int bits, const SortScratch& s,
@@ -289,6 +310,87 @@ int radix_sort_pairs(uint32_t* k0, int* v0, const int* d_n, int n_host, int n_ca
         case 4: return radix_sort_tiles<4>(k0, v0, d_n, n_host, cap, bits, s, st);
         case 16: return radix_sort_tiles<16>(k0, v0, d_n, n_host, cap, bits, s, st);
         default: return radix_sort_tiles<8>(k0, v0, d_n, n_host, cap, bits, s, st);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// batched forms: the same bodies, one work item per (job, block)
+// ---------------------------------------------------------------------------------------------
+int radix_tile(int items) { return RS_BLOCK * (radix_items_valid(items) ? items : 8); }
+int radix_passes(int bits) { return bits > 0 ? (bits + 7) / 8 : 0; }
+int radix_pass_bits(int bits, int pass, int* shift) {
+    const int passes = radix_passes(bits);
+    const int width = (bits + passes - 1) / passes;
+    *shift = pass * width;
+    return min(width, bits - *shift);
+}
+int scan_tile() { return SCAN_TILE; }
+
+struct OpFill {
+    const FillJob* jobs;
+    __device__ void operator()(int j, int b) const { fill_int_body(b, jobs[j].p, jobs[j].value, jobs[j].n); }
+};
+struct OpScanSums {
+    const ScanJob* jobs;
+    __device__ void operator()(int j, int b) const {
+        const ScanJob& q = jobs[j];
+        scan_tile_sums_body(b, q.in, q.d_n, q.n_host, q.d_n ? q.n_cap : q.n_host, q.tmp);
+    }
+};
+struct OpScanSingle {
+    const ScanJob* jobs;
+    __device__ void operator()(int j, int) const {
+        const ScanJob& q = jobs[j];
+        scan_single_body(q.tmp, q.d_n, q.n_host, q.d_n ? q.n_cap : q.n_host, SCAN_TILE, 1, q.d_total);
+    }
+};
+struct OpScanTiles {
+    const ScanJob* jobs;
+    __device__ void operator()(int j, int b) const {
+        const ScanJob& q = jobs[j];
+        scan_tiles_body(b, q.in, q.out, q.d_n, q.n_host, q.d_n ? q.n_cap : q.n_host, q.tmp);
+    }
+};
+template <int I> struct OpUpsweep {
+    const SortJob* jobs;
+    __device__ void operator()(int j, int b) const {
+        const SortJob& q = jobs[j];
+        rs_upsweep_body<I>(b, q.kin, q.d_n, q.n_host, q.n_cap, q.shift, q.nbits, q.hist);
+    }
+};
+template <int I> struct OpHistScan {
+    const SortJob* jobs;
+    __device__ void operator()(int j, int) const {
+        const SortJob& q = jobs[j];
+        scan_single_body(q.hist, q.d_n, q.n_host, q.n_cap, RS_BLOCK * I, 1 << q.nbits, nullptr);
+    }
+};
+template <int I> struct OpDownsweep {
+    const SortJob* jobs;
+    __device__ void operator()(int j, int b) const {
+        const SortJob& q = jobs[j];
+        rs_downsweep_body<I>(b, q.kin, q.vin, q.kout, q.vout, q.d_n, q.n_host, q.n_cap, q.shift, q.nbits, q.hist);
+    }
+};
+
+void fill_int_batched(const FillJob* jobs, WorkList blocks, hipStream_t st) {
+    launch_batched<256>(OpFill{jobs}, blocks, 256, st);
+}
+void exclusive_scan_batched(const ScanJob* jobs, WorkList tiles, WorkList singles, hipStream_t st) {
+    launch_batched<SCAN_BLOCK>(OpScanSums{jobs}, tiles, SCAN_BLOCK, st);
+    launch_batched<SCAN_BLOCK>(OpScanSingle{jobs}, singles, SCAN_BLOCK, st);
+    launch_batched<SCAN_BLOCK>(OpScanTiles{jobs}, tiles, SCAN_BLOCK, st);
+}
+template <int I> static void radix_pass_tiles(const SortJob* jobs, WorkList tiles, WorkList singles, hipStream_t st) {
+    launch_batched<RS_BLOCK>(OpUpsweep<I>{jobs}, tiles, RS_BLOCK, st);
+    launch_batched<SCAN_BLOCK>(OpHistScan<I>{jobs}, singles, SCAN_BLOCK, st);
+    launch_batched<RS_BLOCK>(OpDownsweep<I>{jobs}, tiles, RS_BLOCK, st);
+}
+void radix_pass_batched(int items, const SortJob* jobs, WorkList tiles, WorkList singles, hipStream_t st) {
+    switch (items) {
+        case 4: radix_pass_tiles<4>(jobs, tiles, singles, st); break;
+        case 16: radix_pass_tiles<16>(jobs, tiles, singles, st); break;
+        default: radix_pass_tiles<8>(jobs, tiles, singles, st); break;
     }
 }
 

@@ -67,20 +67,24 @@ template <typename R, typename F> __device__ inline void march_triangle(const Co
 }
 
 template <typename R>
-__global__ __launch_bounds__(64) void k_tri_count(int T, Consts<R> c, const TriD<R>* __restrict__ tris, int* emit) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void tri_count_body(int blk, int T, Consts<R> c, const TriD<R>* __restrict__ tris, int* emit) {
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= T) return;
     const TriD<R> tr = tris[t];
     int n = 0;
     march_triangle(c, tr, [&](int) { ++n; });
     emit[t] = n;
 }
+template <typename R>
+__global__ __launch_bounds__(64) void k_tri_count(int T, Consts<R> c, const TriD<R>* __restrict__ tris, int* emit) {
+    tri_count_body<R>(blockIdx.x, T, c, tris, emit);
+}
 
 template <typename R>
-__global__ __launch_bounds__(64) void k_tri_fill(int T, Consts<R> c, const TriD<R>* __restrict__ tris,
-                                                 const int* __restrict__ off, const int* __restrict__ pair_n,
-                                                 int pair_cap, uint32_t* pair_cell, int* pair_tri, int* overflow) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void tri_fill_body(int blk, int T, Consts<R> c, const TriD<R>* __restrict__ tris,
+                                     const int* __restrict__ off, const int* __restrict__ pair_n,
+                                     int pair_cap, uint32_t* pair_cell, int* pair_tri, int* overflow) {
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= T) return;
     if (*pair_n > pair_cap) {
         if (t == 0) *overflow = 1;
@@ -94,19 +98,29 @@ __global__ __launch_bounds__(64) void k_tri_fill(int T, Consts<R> c, const TriD<
         ++e;
     });
 }
+template <typename R>
+__global__ __launch_bounds__(64) void k_tri_fill(int T, Consts<R> c, const TriD<R>* __restrict__ tris,
+                                                 const int* __restrict__ off, const int* __restrict__ pair_n,
+                                                 int pair_cap, uint32_t* pair_cell, int* pair_tri, int* overflow) {
+    tri_fill_body<R>(blockIdx.x, T, c, tris, off, pair_n, pair_cap, pair_cell, pair_tri, overflow);
+}
 
 // 1 where a sorted pair differs from its predecessor
-__global__ __launch_bounds__(256) void k_pair_flags(const int* pair_n, int pair_cap, const uint32_t* __restrict__ cell,
-                                                     const int* __restrict__ tri, int* flag) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void pair_flags_body(int blk, const int* pair_n, int pair_cap, const uint32_t* __restrict__ cell,
+                                       const int* __restrict__ tri, int* flag) {
+    const int e = blk * blockDim.x + threadIdx.x;
     const int n = min(*pair_n, pair_cap);
     if (e >= n) return;
     flag[e] = (e == 0 || cell[e] != cell[e - 1] || tri[e] != tri[e - 1]) ? 1 : 0;
 }
-__global__ __launch_bounds__(256) void k_pair_compact(const int* pair_n, int pair_cap, const uint32_t* __restrict__ cell,
-                                                       const int* __restrict__ tri, const int* __restrict__ flag,
-                                                       const int* __restrict__ pos, uint32_t* ucell, int* utri) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_pair_flags(const int* pair_n, int pair_cap, const uint32_t* __restrict__ cell,
+                                                     const int* __restrict__ tri, int* flag) {
+    pair_flags_body(blockIdx.x, pair_n, pair_cap, cell, tri, flag);
+}
+__device__ inline void pair_compact_body(int blk, const int* pair_n, int pair_cap, const uint32_t* __restrict__ cell,
+                                         const int* __restrict__ tri, const int* __restrict__ flag,
+                                         const int* __restrict__ pos, uint32_t* ucell, int* utri) {
+    const int e = blk * blockDim.x + threadIdx.x;
     const int n = min(*pair_n, pair_cap);
     if (e >= n) return;
     if (flag[e]) {
@@ -114,10 +128,15 @@ __global__ __launch_bounds__(256) void k_pair_compact(const int* pair_n, int pai
         utri[pos[e]] = tri[e];
     }
 }
+__global__ __launch_bounds__(256) void k_pair_compact(const int* pair_n, int pair_cap, const uint32_t* __restrict__ cell,
+                                                       const int* __restrict__ tri, const int* __restrict__ flag,
+                                                       const int* __restrict__ pos, uint32_t* ucell, int* utri) {
+    pair_compact_body(blockIdx.x, pair_n, pair_cap, cell, tri, flag, pos, ucell, utri);
+}
 // first index and (capped) length of every cell's run in the unique list
-__global__ __launch_bounds__(256) void k_pair_ranges(const int* uniq_n, const uint32_t* __restrict__ ucell, int* cstart,
-                                                      int* ccount) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void pair_ranges_body(int blk, const int* uniq_n, const uint32_t* __restrict__ ucell, int* cstart,
+                                        int* ccount) {
+    const int e = blk * blockDim.x + threadIdx.x;
     const int n = *uniq_n;
     if (e >= n) return;
     const uint32_t k = ucell[e];
@@ -127,6 +146,10 @@ __global__ __launch_bounds__(256) void k_pair_ranges(const int* uniq_n, const ui
         cstart[k] = e;
         ccount[k] = f - e;
     }
+}
+__global__ __launch_bounds__(256) void k_pair_ranges(const int* uniq_n, const uint32_t* __restrict__ ucell, int* cstart,
+                                                      int* ccount) {
+    pair_ranges_body(blockIdx.x, uniq_n, ucell, cstart, ccount);
 }
 
 template <typename R> void launch_tri_count_total(Dev<R>& d, hipStream_t st) {
@@ -149,8 +172,7 @@ template <typename R> void launch_tri_bin(Dev<R>& d, hipStream_t st) {
     exclusive_scan(d.tri_emit, d.tri_off, nullptr, T, T, d.pair_n, d.tsort, st);
     hipLaunchKernelGGL(k_tri_fill<R>, dim3(grid_for(T, 64)), dim3(64), 0, st, T, d.c, d.tris, d.tri_off, d.pair_n,
                        d.pair_cap, d.pair_cell, d.pair_tri, d.overflow);
-    int bits = 1;
-    while ((1ll << bits) < (long long)d.c.ncell) ++bits;
+    const int bits = tri_pair_bits(d.c.ncell);
     const int which = radix_sort_pairs(d.pair_cell, d.pair_tri, d.pair_n, 0, d.pair_cap, bits, d.tsort, st);
     const uint32_t* sc = which ? d.tsort.k1 : d.pair_cell;
     const int* stri = which ? d.tsort.v1 : d.pair_tri;
@@ -168,11 +190,11 @@ template <> __device__ inline float4 mk4w<float>(float x, float y, float z) { re
 template <> __device__ inline double4 mk4w<double>(double x, double y, double z) { return make_double4(x, y, z, 0.0); }
 
 template <typename R>
-__global__ __launch_bounds__(kMaxBlock) void k_stl_distance(int N, Consts<R> c, const R4<R>* __restrict__ spos,
-                                                       const int* __restrict__ tstart, const int* __restrict__ tcount,
-                                                       const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
-                                                       R* stl_mag, R4<R>* stl_dir, int* stl_id) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void stl_distance_body(int blk, int N, Consts<R> c, const R4<R>* __restrict__ spos,
+                                         const int* __restrict__ tstart, const int* __restrict__ tcount,
+                                         const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
+                                         R* stl_mag, R4<R>* stl_dir, int* stl_id) {
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const R4<R> P = spos[t];
     R best;
@@ -182,17 +204,24 @@ __global__ __launch_bounds__(kMaxBlock) void k_stl_distance(int N, Consts<R> c, 
     stl_dir[t] = mk4w<R>(dir.x, dir.y, dir.z);
     stl_id[t] = bid;
 }
+template <typename R>
+__global__ __launch_bounds__(kMaxBlock) void k_stl_distance(int N, Consts<R> c, const R4<R>* __restrict__ spos,
+                                                       const int* __restrict__ tstart, const int* __restrict__ tcount,
+                                                       const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
+                                                       R* stl_mag, R4<R>* stl_dir, int* stl_id) {
+    stl_distance_body<R>(blockIdx.x, N, c, spos, tstart, tcount, tids, tris, stl_mag, stl_dir, stl_id);
+}
 
 
 template <typename R>
-__global__ __launch_bounds__(kMaxBlock) void k_drag(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
-                                               PState<R> out, R4<R>* __restrict__ spos, R4<R>* __restrict__ svel,
-                                               const int* __restrict__ tstart, const int* __restrict__ tcount,
-                                               const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
-                                               const mpsr_tri_params* __restrict__ models,
-                                               const mpsr_drag_params* __restrict__ drag, R4<R>* drag_acc,
-                                               R* drag_temp) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void drag_body(int blk, int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                 PState<R> out, R4<R>* __restrict__ spos, R4<R>* __restrict__ svel,
+                                 const int* __restrict__ tstart, const int* __restrict__ tcount,
+                                 const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
+                                 const mpsr_tri_params* __restrict__ models,
+                                 const mpsr_drag_params* __restrict__ drag, R4<R>* drag_acc,
+                                 R* drag_temp) {
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
     if (out.id[s] < 0) return;
@@ -244,10 +273,21 @@ __global__ __launch_bounds__(kMaxBlock) void k_drag(int N, Consts<R> c, const in
         }
     }
 }
+template <typename R>
+__global__ __launch_bounds__(kMaxBlock) void k_drag(int N, Consts<R> c, const int* __restrict__ perm, PState<R> in,
+                                               PState<R> out, R4<R>* __restrict__ spos, R4<R>* __restrict__ svel,
+                                               const int* __restrict__ tstart, const int* __restrict__ tcount,
+                                               const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
+                                               const mpsr_tri_params* __restrict__ models,
+                                               const mpsr_drag_params* __restrict__ drag, R4<R>* drag_acc,
+                                               R* drag_temp) {
+    drag_body<R>(blockIdx.x, N, c, perm, in, out, spos, svel, tstart, tcount, tids, tris, models, drag, drag_acc,
+                 drag_temp);
+}
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_translate(int T, R dt, TriD<R>* tris, const mpsr_tri_params* __restrict__ models) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void translate_body(int blk, int T, R dt, TriD<R>* tris, const mpsr_tri_params* __restrict__ models) {
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= T) return;
     TriD<R> tr = tris[t];
     const mpsr_float3 mv = models[tr.model].Velocity;
@@ -260,11 +300,15 @@ __global__ __launch_bounds__(256) void k_translate(int T, R dt, TriD<R>* tris, c
     }
     tris[t] = tr;
 }
+template <typename R>
+__global__ __launch_bounds__(256) void k_translate(int T, R dt, TriD<R>* tris, const mpsr_tri_params* __restrict__ models) {
+    translate_body<R>(blockIdx.x, T, dt, tris, models);
+}
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_rotate(int T, TriD<R>* tris, const RotCoef<R>* __restrict__ rot,
-                                                const int* __restrict__ step_ctr, int nmodel) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void rotate_body(int blk, int T, TriD<R>* tris, const RotCoef<R>* __restrict__ rot,
+                                   const int* __restrict__ step_ctr, int nmodel) {
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= T) return;
     TriD<R> tr = tris[t];
     const RotCoef<R> r = rot[(step_ctr ? (size_t)(*step_ctr) * nmodel : 0) + tr.model];
@@ -280,6 +324,11 @@ __global__ __launch_bounds__(256) void k_rotate(int T, TriD<R>* tris, const RotC
                           (((-b * u) + a * v - v * x + u * y) * sn);
     }
     tris[t] = tr;
+}
+template <typename R>
+__global__ __launch_bounds__(256) void k_rotate(int T, TriD<R>* tris, const RotCoef<R>* __restrict__ rot,
+                                                const int* __restrict__ step_ctr, int nmodel) {
+    rotate_body<R>(blockIdx.x, T, tris, rot, step_ctr, nmodel);
 }
 
 template <typename R> void launch_stl_distance(const Dev<R>& d, hipStream_t st) {
@@ -310,13 +359,113 @@ __global__ void k_step_advance(int* ctr) {
 }
 void launch_step_advance(int* ctr, hipStream_t st) { hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, st, ctr); }
 
+// ---- batched forms: one work item = one block of one domain's particles or triangles ---------
+template <typename R> struct OpTriCount {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        tri_count_body<R>(b, d.ntri, d.c, d.tris, d.tri_emit);
+    }
+};
+template <typename R> struct OpTriFill {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        tri_fill_body<R>(b, d.ntri, d.c, d.tris, d.tri_off, d.pair_n, d.pair_cap, d.pair_cell, d.pair_tri, d.overflow);
+    }
+};
+// the sorted pairs are where the pair sort's passes left them (radix_sort_pairs' parity)
+template <typename R> struct OpPairs {
+    const Dev<R>* dv;
+    int pass;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        const bool odd = radix_passes_dev(tri_pair_bits(d.c.ncell)) & 1;
+        const uint32_t* sc = odd ? d.tsort.k1 : d.pair_cell;
+        const int* stri = odd ? d.tsort.v1 : d.pair_tri;
+        if (pass == 0) pair_flags_body(b, d.pair_n, d.pair_cap, sc, stri, d.pair_flag);
+        else if (pass == 1) pair_compact_body(b, d.pair_n, d.pair_cap, sc, stri, d.pair_flag, d.pair_pos, d.uniq_cell,
+                                              d.uniq_tri);
+        else pair_ranges_body(b, d.uniq_n, d.uniq_cell, d.cell_tri_start, d.cell_tri_count);
+    }
+};
+template <typename R> struct OpStl {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        stl_distance_body<R>(b, d.N, d.c, d.spos, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.stl_mag,
+                             d.stl_dir, d.stl_id);
+    }
+};
+template <typename R> struct OpDrag {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        drag_body<R>(b, d.N, d.c, d.perm, d.in, d.out, d.spos, d.svel, d.cell_tri_start, d.cell_tri_count, d.uniq_tri,
+                     d.tris, d.models, d.drag, d.drag_acc, d.drag_temp);
+    }
+};
+template <typename R> struct OpTranslate {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        translate_body<R>(b, d.ntri, d.c.dt, d.tris, d.models);
+    }
+};
+template <typename R> struct OpRotate {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        rotate_body<R>(b, d.ntri, d.tris, d.rot, d.step_ctr, d.nmodel > 0 ? d.nmodel : 1);
+    }
+};
+template <typename R> struct OpStepAdvance {
+    const Dev<R>* dv;
+    __device__ void operator()(int k, int) const {
+        int* ctr = const_cast<int*>(dv[k].step_ctr);
+        if (threadIdx.x == 0 && ctr) ctr[0] = ctr[0] + 1;
+    }
+};
+template <typename R> void launch_tri_count_b(const BatchLaunch<R>& b, WorkList t64, hipStream_t st) {
+    launch_batched<64>(OpTriCount<R>{b.devs}, t64, 64, st);
+}
+template <typename R> void launch_tri_fill_b(const BatchLaunch<R>& b, WorkList t64, hipStream_t st) {
+    launch_batched<64>(OpTriFill<R>{b.devs}, t64, 64, st);
+}
+template <typename R> void launch_pairs_b(const BatchLaunch<R>& b, int pass, WorkList pairs, hipStream_t st) {
+    launch_batched<256>(OpPairs<R>{b.devs, pass}, pairs, 256, st);
+}
+template <typename R> void launch_stl_distance_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
+    launch_batched<kMaxBlock>(OpStl<R>{b.devs}, p, b.block, st);
+}
+template <typename R> void launch_drag_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
+    launch_batched<kMaxBlock>(OpDrag<R>{b.devs}, p, b.block, st);
+}
+template <typename R> void launch_translate_b(const BatchLaunch<R>& b, WorkList t256, hipStream_t st) {
+    launch_batched<256>(OpTranslate<R>{b.devs}, t256, 256, st);
+}
+template <typename R> void launch_rotate_b(const BatchLaunch<R>& b, WorkList t256, hipStream_t st) {
+    launch_batched<256>(OpRotate<R>{b.devs}, t256, 256, st);
+}
+template <typename R> void launch_step_advance_b(const BatchLaunch<R>& b, WorkList one, hipStream_t st) {
+    launch_batched<64>(OpStepAdvance<R>{b.devs}, one, 64, st);
+}
+
 #define MPSR_INSTANTIATE_W(R)                                               \
     template void launch_tri_bin<R>(Dev<R>&, hipStream_t);                  \
     template void launch_tri_count_total<R>(Dev<R>&, hipStream_t);          \
     template void launch_stl_distance<R>(const Dev<R>&, hipStream_t);       \
     template void launch_drag<R>(const Dev<R>&, hipStream_t);               \
     template void launch_translate<R>(const Dev<R>&, hipStream_t);          \
-    template void launch_rotate<R>(const Dev<R>&, hipStream_t);
+    template void launch_rotate<R>(const Dev<R>&, hipStream_t);                                  \
+    template void launch_tri_count_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);           \
+    template void launch_tri_fill_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);            \
+    template void launch_pairs_b<R>(const BatchLaunch<R>&, int, WorkList, hipStream_t);          \
+    template void launch_stl_distance_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);        \
+    template void launch_drag_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);                \
+    template void launch_translate_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);           \
+    template void launch_rotate_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);              \
+    template void launch_step_advance_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);
 MPSR_INSTANTIATE_W(float)
 MPSR_INSTANTIATE_W(double)
 

@@ -1,5 +1,5 @@
 """Batched buckets: independent rotating domains advanced by one HIP-graph launch per step
-(mpsr_step_batch).  Every bucket's state after the batched steps equals the state of the same
+(mpsr_step_batch), every kernel of the step one launch over the buckets of a precision.  Every bucket's state after the batched steps equals the state of the same
 bucket stepped on its own (mpsr_step_many), bit for bit, whatever the launch shape; replays
 interleaved with directly launched steps and the autotuner (which restores the state it timed
 from) keep that equality; one bucket is also checked against the oracle."""
@@ -60,7 +60,9 @@ def test_graph_steps_equal_single_domains(small_buckets):
     b = Batch(small_buckets, precisions(len(small_buckets)))
     b.step_many(0, 12)
     info = b.info()
-    assert info["graph_nodes"] > 10 * len(small_buckets)
+    # every kernel of the step is one launch over all buckets of a precision: the graph does not
+    # grow with the bucket count (two precision groups here)
+    assert 0 < info["graph_nodes"] <= 2 * 80
     ref = singles(small_buckets, 0, 12)
     for i, (g, s) in enumerate(zip(b.buckets, ref)):
         assert_same(g, s, f"bucket {i}")
