@@ -438,19 +438,26 @@ class Solver:
 class Batch:
     """Independent buckets (each a case with its own walls, grid and precision) advanced together:
     one graph launch per step (mpsr_step_batch), or the same kernels launched directly
-    (mpsr_step_batch_eager).  buckets[i] is a Solver over bucket i for the per-domain calls."""
+    (mpsr_step_batch_eager).  buckets[i] is a Solver over bucket i for the per-domain calls.
+    pressure_modes[i] = MPSR_PRESSURE_PPE makes bucket i solve the PPE with ppe[i] (FP64
+    Jacobi-PCG options; default 1e-8 from the previous pressure)."""
 
     TUNE_BLOCKS = (64, 128, 256, 512)
     TUNE_ITEMS = (4, 8, 16)
 
-    def __init__(self, case_list, precisions=None, device=0):
+    def __init__(self, case_list, precisions=None, device=0, pressure_modes=None, ppe=None):
         from . import cases
         self.lib = load_library()
         self.cases = list(case_list)
         precisions = precisions or [abi.MPSR_FP32] * len(self.cases)
+        modes = pressure_modes or [abi.MPSR_PRESSURE_EOS] * len(self.cases)
+        ppe = ppe or [None] * len(self.cases)
         infos, self._keep = [], []
-        for c, prec in zip(self.cases, precisions):
-            info, keep = cases.create_info(c, prec, abi.MPSR_PRESSURE_EOS, None, None, device)
+        for c, prec, mode, opt in zip(self.cases, precisions, modes, ppe):
+            if mode == abi.MPSR_PRESSURE_PPE and opt is None:
+                opt = abi.PpeOptions(abi.MPSR_PCG_FP64, 1e-8, 20000, abi.MPSR_WARM_PREVIOUS, 0.0, 0.0,
+                                     abi.MPSR_PRECOND_JACOBI)
+            info, keep = cases.create_info(c, prec, mode, opt, None, device)
             infos.append(info)
             self._keep.append(keep)
         arr = (abi.CreateInfo * len(infos))(*infos)

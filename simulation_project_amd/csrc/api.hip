@@ -60,6 +60,7 @@ struct EngineBase {
     virtual mpsr_status set_buckets(const mpsr_particles* b, const int* interval, const int* total, int count) = 0;
     virtual mpsr_status bucket_phase(int step, int* added) = 0;
     virtual int count() const = 0;
+    virtual int ppe_max_iter() const = 0;   // the PCG iteration cap of a PPE domain, 0 without the PPE
     virtual mpsr_status mpnn_set(const float* w) = 0;
     virtual mpsr_status mpnn_get(float* w) = 0;
     virtual mpsr_status mpnn_train(double lr, double beta, int update, float* grads, double* parts) = 0;
@@ -348,6 +349,9 @@ template <typename R> struct Engine : EngineBase {
         return MPSR_NOERR;
     }
     int count() const override { return d.N; }
+    int ppe_max_iter() const override {
+        return pressure_mode == MPSR_PRESSURE_PPE ? (ppe_opt.max_iter > 0 ? ppe_opt.max_iter : 20000) : 0;
+    }
     mpsr_status mpnn_set(const float* w) override {
         HIPC(hipSetDevice(device));
         if (!ppe) return fail(MPSR_PARAMERR, "the network needs a handle in PPE mode");
@@ -833,9 +837,12 @@ template <typename R> struct Engine : EngineBase {
         return MPSR_NOERR;
     }
     mpsr_status batch_check() override {
-        if (pressure_mode == MPSR_PRESSURE_PPE)
-            return fail(MPSR_PARAMERR, "batched step: the PPE solve reads its convergence on the host; batched "
-                                       "buckets use the explicit (EOS) pressure");
+        if (pressure_mode == MPSR_PRESSURE_PPE &&
+            (ppe_opt.solver != MPSR_PCG_FP64 || ppe_opt.precond != MPSR_PRECOND_JACOBI ||
+             ppe_opt.warm_start == MPSR_WARM_LEARNED))
+            return fail(MPSR_PARAMERR, "batched step: a PPE bucket solves with the FP64 Jacobi-PCG (solver "
+                                       "MPSR_PCG_FP64, preconditioner MPSR_PRECOND_JACOBI, no learned start); "
+                                       "other buckets use the explicit (EOS) pressure");
         if (sph.S > 0) return fail(MPSR_PARAMERR, "batched step: rigid spheres are not part of the batched form");
         if (!buckets.empty()) return fail(MPSR_PARAMERR, "batched step: bucket inflow changes the particle count");
         return MPSR_NOERR;
@@ -843,6 +850,8 @@ template <typename R> struct Engine : EngineBase {
     // rotation table for steps first .. first+count-1 and the step counter at 0
     mpsr_status batch_prepare(int first, int count) override {
         HIPC(hipSetDevice(device));
+        mpsr_status c = batch_check();
+        if (c != MPSR_NOERR) return c;
         if (!step_ctr_buf) ALLOC(step_ctr_buf, 1);
         if (!step_base_buf) ALLOC(step_base_buf, 1);
         mpsr_status s = upload_rot(first, count);
@@ -856,6 +865,7 @@ template <typename R> struct Engine : EngineBase {
         int ovf = 0;
         HIPC(hipMemcpy(&ovf, d.overflow, sizeof(int), hipMemcpyDeviceToHost));
         if (ovf) return fail(MPSR_MEMERR, "triangle/cell pair buffer overflow: walls grew beyond the set-up capacity");
+        if (ppe) (void)ppe_check(ppe, &tm);   // the last batched solve's iterations and residual
         return MPSR_NOERR;
     }
     // restore = 0: copy the state (in and out arrays, triangles) aside; 1: copy it back
@@ -1193,19 +1203,33 @@ template <typename R> struct BatchGroup {
     const ScanJob* scan_emit = nullptr;
     const ScanJob* scan_flag = nullptr;
     WorkList scan_emit_t, scan_emit_1, scan_flag_t, scan_flag_1;
+    WorkList wp_eos;                   // particle blocks of the buckets with the EOS pressure
+    PpeBatch<R>* pb = nullptr;         // the PPE buckets' solves (null: none)
 
     ~BatchGroup() {
         if (devs) (void)hipFree(devs);
         if (tab) (void)hipFree(tab);
+        ppe_batch_destroy(pb);
+    }
+    bool has_ppe() const {
+        for (Engine<R>* e : eng)
+            if (e->pressure_mode == MPSR_PRESSURE_PPE) return true;
+        return false;
     }
     BatchLaunch<R> launch() const { return BatchLaunch<R>{devs, block}; }
 
     // the shape of the tables: launch shape and which buckets run the optional phases
     long long signature(int blk, int itm) const {
         long long h = (long long)blk * 131 + itm;
-        for (Engine<R>* e : eng)
+        for (Engine<R>* e : eng) {
             h = h * 8 + (e->d.ndrag > 0 ? 1 : 0) + (e->P.bUpdateTemperature ? 2 : 0) +
                 (e->d.c.turb && e->d.turb_nu ? 4 : 0);
+            if (e->pressure_mode == MPSR_PRESSURE_PPE) {
+                long long t = 0;
+                std::memcpy(&t, &e->ppe_opt.rel_tol, sizeof(t));
+                h = h * 1000003 + (t ^ ((long long)e->ppe_opt.max_iter << 3) ^ e->ppe_opt.warm_start);
+            }
+        }
         return h;
     }
 
@@ -1249,13 +1273,14 @@ template <typename R> struct BatchGroup {
             int n;
         };
         auto items_of = [&](const std::vector<int2>& v) { return Ref{put(v.data(), v.size() * sizeof(int2)), (int)v.size()}; };
-        std::vector<int2> p, t64, t256, pairs, one, drag, therm, turb;
+        std::vector<int2> p, t64, t256, pairs, one, drag, therm, turb, peos;
         const int n = (int)eng.size();
         for (int k = 0; k < n; ++k) {
             const Dev<R>& d = host[k];
             const int nb = (int)grid_for(d.N, block);
             for (int j = 0; j < nb; ++j) {
                 p.push_back(make_int2(k, j));
+                if (eng[k]->pressure_mode != MPSR_PRESSURE_PPE) peos.push_back(make_int2(k, j));
                 if (d.ndrag > 0) drag.push_back(make_int2(k, j));
                 if (eng[k]->P.bUpdateTemperature) therm.push_back(make_int2(k, j));
                 if (d.c.turb && d.turb_nu) turb.push_back(make_int2(k, j));
@@ -1268,7 +1293,8 @@ template <typename R> struct BatchGroup {
             one.push_back(make_int2(k, 0));
         }
         const Ref rp = items_of(p), r64 = items_of(t64), r256 = items_of(t256), rpairs = items_of(pairs),
-                  rone = items_of(one), rdrag = items_of(drag), rtherm = items_of(therm), rturb = items_of(turb);
+                  rone = items_of(one), rdrag = items_of(drag), rtherm = items_of(therm), rturb = items_of(turb),
+                  rpeos = items_of(peos);
         // fills: cell ranges of the particle binning; triangle counts per cell (and the pair count of
         // a bucket without triangles)
         std::vector<FillJob> fb, ft;
@@ -1362,7 +1388,7 @@ template <typename R> struct BatchGroup {
         HIPC(hipMemcpy(tab, img.data(), img.size(), hipMemcpyHostToDevice));
         auto wl = [&](Ref r) { return WorkList{reinterpret_cast<const int2*>(tab + r.off), r.n}; };
         wp = wl(rp); wt64 = wl(r64); wt256 = wl(r256); wpairs = wl(rpairs); wone = wl(rone);
-        wdrag = wl(rdrag); wtherm = wl(rtherm); wturb = wl(rturb);
+        wdrag = wl(rdrag); wtherm = wl(rtherm); wturb = wl(rturb); wp_eos = wl(rpeos);
         fill_bin = reinterpret_cast<const FillJob*>(tab + ofb);
         fill_tri = reinterpret_cast<const FillJob*>(tab + oft);
         fill_bin_w = wl(rfbw);
@@ -1374,6 +1400,22 @@ template <typename R> struct BatchGroup {
         tsort.clear();
         for (const PassImg& q : pp) psort.push_back(Pass{reinterpret_cast<const SortJob*>(tab + q.jobs), wl(q.tiles), wl(q.singles)});
         for (const PassImg& q : tp) tsort.push_back(Pass{reinterpret_cast<const SortJob*>(tab + q.jobs), wl(q.tiles), wl(q.singles)});
+        // the PPE buckets' solver tables
+        std::vector<PpeSolver<R>*> solvers;
+        std::vector<int> pdev;
+        std::vector<mpsr_ppe_options> popt;
+        for (int k = 0; k < n; ++k)
+            if (eng[k]->pressure_mode == MPSR_PRESSURE_PPE) {
+                solvers.push_back(eng[k]->ppe);
+                pdev.push_back(k);
+                popt.push_back(eng[k]->ppe_opt);
+            }
+        if (!solvers.empty()) {
+            if (!pb) pb = ppe_batch_create<R>();
+            mpsr_status ps = ppe_batch_build(pb, solvers.data(), pdev.data(), host.data(), popt.data(),
+                                             (int)solvers.size(), block);
+            if (ps != MPSR_NOERR) return fail(ps, "batched step: PPE solver tables");
+        }
         sig = sg;
         return MPSR_NOERR;
     }
@@ -1397,9 +1439,10 @@ template <typename R> struct BatchGroup {
         launch_pairs_b(b, 1, wpairs, st);
         launch_pairs_b(b, 2, wpairs, st);
     }
-    // one EOS step of every bucket (the order of Engine::one_step), the rotation row of each bucket
-    // from its device step counter, which the step then advances
-    void enqueue(hipStream_t st) {
+    // one step of every bucket in the order of Engine::one_step, the rotation row of each bucket
+    // from its device step counter, which the step then advances.  head: up to the pressure (with
+    // the PPE buckets' assembly, start and `iters` PCG iterations); tail: from the PPE write-back on.
+    void head(hipStream_t st, int iters) {
         const BatchLaunch<R> b = launch();
         bin(st);
         tri(st);
@@ -1411,7 +1454,18 @@ template <typename R> struct BatchGroup {
         launch_relocate_b(b, wp, st);
         bin(st);
         launch_stl_distance_b(b, wp, st);
-        launch_pnd_b(b, wp, st);
+        launch_pnd_b(b, wp_eos, st);
+        if (pb) {
+            ppe_batch_pnd(pb, b, st);
+            ppe_batch_head(pb, b, iters, st);
+        }
+    }
+    void iterate(hipStream_t st, int iters) {
+        if (pb) ppe_batch_iterate(pb, iters, st);
+    }
+    void tail(hipStream_t st) {
+        const BatchLaunch<R> b = launch();
+        if (pb) ppe_batch_tail(pb, b, st);
         launch_gradient_b(b, wp, st);
         launch_relocate_b(b, wp, st);
         launch_translate_b(b, wt256, st);
@@ -1435,29 +1489,49 @@ struct mpsr_handle {
 // step is then one graph launch.  The rotation row of a replay comes from each bucket's device step
 // counter.
 struct mpsr_batch {
+    static constexpr int kHeadIters = 32;   // PCG iterations of the PPE buckets in the step's first graph
+    static constexpr int kChunkIters = 16;  // ... and in each further chunk until all have stopped
     std::vector<mpsr_handle*> h;
     std::vector<int> order;            // largest bucket first
     int device = 0;
     hipStream_t main = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // without PPE buckets the whole step is graph[0]; with them: head, PCG chunk, tail
+    hipGraph_t graph[3] = {};
+    hipGraphExec_t exec[3] = {};
     bool dirty = true;
     int block = 256, items = 8;
     size_t nodes = 0;
     float ms_per_step = 0.f;
     BatchGroup<float> g32;
     BatchGroup<double> g64;
+    bool ppe() const { return g32.has_ppe() || g64.has_ppe(); }
     void drop_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
+        for (int i = 0; i < 3; ++i) {
+            if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
+            if (graph[i]) (void)hipGraphDestroy(graph[i]);
+            exec[i] = nullptr;
+            graph[i] = nullptr;
+        }
         dirty = true;
     }
-    void enqueue() {
-        if (!g32.eng.empty()) g32.enqueue(main);
-        if (!g64.eng.empty()) g64.enqueue(main);
+    // part 0: head (the whole step without PPE buckets), 1: a PCG chunk, 2: tail
+    void enqueue(int part) {
+        const bool p = ppe();
+        if (part == 0) {
+            if (!g32.eng.empty()) g32.head(main, kHeadIters);
+            if (!g64.eng.empty()) g64.head(main, kHeadIters);
+            if (!p) {
+                if (!g32.eng.empty()) g32.tail(main);
+                if (!g64.eng.empty()) g64.tail(main);
+            }
+        } else if (part == 1) {
+            g32.iterate(main, kChunkIters);
+            g64.iterate(main, kChunkIters);
+        } else {
+            if (!g32.eng.empty()) g32.tail(main);
+            if (!g64.eng.empty()) g64.tail(main);
+        }
     }
     ~mpsr_batch() {
         drop_graph();
@@ -1472,24 +1546,31 @@ struct mpsr_batch {
 };
 
 namespace {
-// records one step of every bucket into b->graph; on any failure the capture is ended and
-// discarded so that no stream is left capturing
+// records the step's graphs (one, or head / PCG chunk / tail with PPE buckets); on any failure the
+// capture is ended and discarded so that no stream is left capturing
 mpsr_status batch_capture(mpsr_batch* b) {
     b->drop_graph();
     HIPC(hipSetDevice(b->device));
-    HIPC(hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
-    b->enqueue();
-    const hipError_t he = hipGetLastError();
-    hipGraph_t g = nullptr;
-    const hipError_t ee = hipStreamEndCapture(b->main, &g);
-    if (he != hipSuccess || ee != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        (void)hipGetLastError();
-        return fail(MPSR_GPUERR, std::string("batched step capture: ") + hipGetErrorString(he != hipSuccess ? he : ee));
+    b->nodes = 0;
+    const int parts = b->ppe() ? 3 : 1;
+    for (int i = 0; i < parts; ++i) {
+        HIPC(hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
+        b->enqueue(i);
+        const hipError_t he = hipGetLastError();
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(b->main, &g);
+        if (he != hipSuccess || ee != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            b->drop_graph();
+            return fail(MPSR_GPUERR, std::string("batched step capture: ") + hipGetErrorString(he != hipSuccess ? he : ee));
+        }
+        b->graph[i] = g;
+        HIPC(hipGraphInstantiate(&b->exec[i], g, nullptr, nullptr, 0));
+        size_t n = 0;
+        HIPC(hipGraphGetNodes(g, nullptr, &n));
+        b->nodes += n;
     }
-    b->graph = g;
-    HIPC(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    HIPC(hipGraphGetNodes(b->graph, nullptr, &b->nodes));
     b->dirty = false;
     return MPSR_NOERR;
 }
@@ -1516,17 +1597,42 @@ mpsr_status batch_prepare_all(mpsr_batch* b, int first, int count) {
     return MPSR_NOERR;
 }
 
+// the PPE buckets' last solves: a failed one (breakdown, not converged, matrix too wide) is the
+// call's error, naming the bucket
+template <typename R> mpsr_status group_ppe_finish(mpsr_batch* b, BatchGroup<R>& g) {
+    if (!g.pb) return MPSR_NOERR;
+    int failed = -1;
+    std::string why;
+    mpsr_status s = ppe_batch_finish(g.pb, &failed, &why);
+    if (s != MPSR_NOERR) return fail(s, "batched step: reading the PPE state");
+    if (failed < 0) return MPSR_NOERR;
+    int k = -1;
+    for (int i = 0, seen = -1; i < (int)g.eng.size(); ++i)
+        if (g.eng[i]->pressure_mode == MPSR_PRESSURE_PPE && ++seen == failed) k = i;
+    int bucket = -1;
+    for (size_t i = 0; i < b->h.size(); ++i)
+        if (k >= 0 && b->h[i]->e == g.eng[k]) bucket = (int)i;
+    return fail(MPSR_CALCERR, "batched step: bucket " + std::to_string(bucket) + ": PPE solve failed: " + why);
+}
+mpsr_status batch_ppe_finish(mpsr_batch* b) {
+    mpsr_status s = group_ppe_finish(b, b->g32);
+    if (s == MPSR_NOERR) s = group_ppe_finish(b, b->g64);
+    return s;
+}
+
 mpsr_status batch_finish_all(mpsr_batch* b, int count) {
     HIPC(hipEventRecord(b->e1, b->main));
     HIPC(hipEventSynchronize(b->e1));
     float ms = 0.f;
     HIPC(hipEventElapsedTime(&ms, b->e0, b->e1));
     b->ms_per_step = ms / count;
+    const mpsr_status ps = batch_ppe_finish(b);
+    const std::string why = g_err;
     for (mpsr_handle* x : b->h) {
         mpsr_status s = x->e->batch_finish();
         if (s != MPSR_NOERR) return s;
     }
-    return MPSR_NOERR;
+    return ps == MPSR_NOERR ? MPSR_NOERR : fail(ps, why);
 }
 
 mpsr_status batch_run(mpsr_batch* b, int first, int count, bool graph) {
@@ -1539,13 +1645,31 @@ mpsr_status batch_run(mpsr_batch* b, int first, int count, bool graph) {
         if (s != MPSR_NOERR) return s;
     }
     HIPC(hipEventRecord(b->e0, b->main));
-    if (graph) {
-        for (int k = 0; k < count; ++k) HIPC(hipGraphLaunch(b->exec, b->main));
-    } else {
-        // the same launches without the graph
-        for (int k = 0; k < count; ++k) b->enqueue();
-        HIPC(hipGetLastError());
+    auto part = [&](int i) -> mpsr_status {
+        if (graph) HIPC(hipGraphLaunch(b->exec[i], b->main));
+        else b->enqueue(i);
+        return MPSR_NOERR;
+    };
+    // the PCG chunks a step may need: every bucket stops at its max_iter at the latest
+    int max_chunks = 2;
+    for (mpsr_handle* x : b->h) max_chunks = std::max(max_chunks, x->e->ppe_max_iter() / mpsr_batch::kChunkIters + 2);
+    for (int k = 0; k < count; ++k) {
+        s = part(0);
+        if (s != MPSR_NOERR) return s;
+        if (!b->ppe()) continue;
+        for (int c = 0; c <= max_chunks; ++c) {
+            bool a32 = true, a64 = true;
+            if (b->g32.pb) s = ppe_batch_converged(b->g32.pb, b->main, &a32);
+            if (s == MPSR_NOERR && b->g64.pb) s = ppe_batch_converged(b->g64.pb, b->main, &a64);
+            if (s != MPSR_NOERR) return fail(s, "batched step: reading the PCG flags");
+            if (a32 && a64) break;
+            s = part(1);
+            if (s != MPSR_NOERR) return s;
+        }
+        s = part(2);
+        if (s != MPSR_NOERR) return s;
     }
+    HIPC(hipGetLastError());
     return batch_finish_all(b, count);
 }
 

@@ -500,6 +500,20 @@ template <typename R> void launch_pnd_b(const BatchLaunch<R>& b, WorkList p, hip
 template <typename R> void launch_gradient_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
 template <typename R> void launch_thermal_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
 template <typename R> void launch_turbulence_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
+// the PND pass of the PPE path for the PPE buckets of a batch: work item (row set j, block), with
+// row set j the Poisson rows of the bucket devs[rows[j].dev]
+template <typename R> struct PndRows {
+    int dev;
+    int* active;
+    int* cnt;
+    double* diag;
+    int* dir;
+    int* nbq;
+    R* nbw;
+    int* nbn;
+    int nb_width;
+};
+template <typename R> void launch_pnd_ppe_b(const BatchLaunch<R>& b, const PndRows<R>* rows, WorkList p, hipStream_t st);
 template <typename R> void launch_tri_count_b(const BatchLaunch<R>& b, WorkList t64, hipStream_t st);
 template <typename R> void launch_tri_fill_b(const BatchLaunch<R>& b, WorkList t64, hipStream_t st);
 // the pair kernels of launch_tri_bin after the pair sort; `pass` = 0 flags, 1 compacts, 2 ranges

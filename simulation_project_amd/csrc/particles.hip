@@ -833,6 +833,19 @@ template <typename R> struct OpTurbulence {
                            d.turb_nu, d.turb_k);
     }
 };
+template <typename R> struct OpPndPpe {
+    const Dev<R>* dv;
+    const PndRows<R>* rows;
+    __device__ void operator()(int j, int b) const {
+        const PndRows<R>& q = rows[j];
+        const Dev<R>& d = dv[q.dev];
+        pnd_body<R, true>(b, d.N, d.c, 0, d.perm, d.in, d.out, d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag,
+                          d.stl_id, d.skeys, q.active, q.cnt, q.diag, q.dir, q.nbq, q.nbw, q.nbn, q.nb_width);
+    }
+};
+template <typename R> void launch_pnd_ppe_b(const BatchLaunch<R>& b, const PndRows<R>* rows, WorkList p, hipStream_t st) {
+    launch_batched<kMaxBlock>(OpPndPpe<R>{b.devs, rows}, p, b.block, st);
+}
 template <typename R> void launch_bin_hash_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
     launch_batched<kMaxBlock>(OpBinHash<R>{b.devs}, p, b.block, st);
 }
@@ -877,7 +890,8 @@ template <typename R> void launch_turbulence_b(const BatchLaunch<R>& b, WorkList
     template void launch_pnd_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);              \
     template void launch_gradient_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);         \
     template void launch_thermal_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);          \
-    template void launch_turbulence_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);
+    template void launch_turbulence_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);       \
+    template void launch_pnd_ppe_b<R>(const BatchLaunch<R>&, const PndRows<R>*, WorkList, hipStream_t);
 MPSR_INSTANTIATE(float)
 MPSR_INSTANTIATE(double)
 

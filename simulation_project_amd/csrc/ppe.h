@@ -1,5 +1,6 @@
 // ppe.h -- pressure Poisson equation: assembly and PCG solve on the device.
 #pragma once
+#include <string>
 #include "mpsr_internal.h"
 
 namespace mpsr {
@@ -41,5 +42,25 @@ mpsr_status ppe_mpnn_train(PpeSolver<R>* s, const Dev<R>& d, double lr, double b
                            double* parts, hipStream_t st);
 template <typename R>
 mpsr_status ppe_mpnn_rows(PpeSolver<R>* s, const Dev<R>& d, float* X, float* Y, double* scale, hipStream_t st);
+
+// The PPE solves of the PPE buckets of a batched step (FP64 Jacobi-PCG, the single-domain solver with
+// MPSR_PCG_FP64 and MPSR_PRECOND_JACOBI): solver j belongs to the bucket dev[j] of the group's Dev
+// table.  A step runs ppe_batch_pnd and ppe_batch_head (assembly, first residual, start and a
+// first chunk of iterations), then chunks of ppe_batch_iterate until ppe_batch_converged, then
+// ppe_batch_tail (final residual, checks, write-back).
+template <typename R> struct PpeBatch;
+template <typename R> PpeBatch<R>* ppe_batch_create();
+template <typename R> void ppe_batch_destroy(PpeBatch<R>* b);
+template <typename R>
+mpsr_status ppe_batch_build(PpeBatch<R>* b, PpeSolver<R>* const* solvers, const int* dev, const Dev<R>* host_devs,
+                            const mpsr_ppe_options* opts, int n, int block);
+template <typename R> void ppe_batch_pnd(PpeBatch<R>* b, const BatchLaunch<R>& bl, hipStream_t st);
+template <typename R> void ppe_batch_head(PpeBatch<R>* b, const BatchLaunch<R>& bl, int iters, hipStream_t st);
+template <typename R> void ppe_batch_iterate(PpeBatch<R>* b, int iters, hipStream_t st);
+template <typename R> mpsr_status ppe_batch_converged(PpeBatch<R>* b, hipStream_t st, bool* all);
+template <typename R> void ppe_batch_tail(PpeBatch<R>* b, const BatchLaunch<R>& bl, hipStream_t st);
+// the solvers' host state after a batched call; *failed = the first solver whose last solve failed
+// (why: the reason), -1 when none
+template <typename R> mpsr_status ppe_batch_finish(PpeBatch<R>* b, int* failed, std::string* why);
 
 }  // namespace mpsr

@@ -86,7 +86,9 @@ static bool win_off() {
     }();
     return off != 0;
 }
-enum Scal { S_RZ, S_PQ, S_ALPHA, S_BETA, S_RR, S_TARGET, S_DONE, S_ITERS, S_MAXIT, S_BB, S_RR0, S_COUNT };
+// S_SKIP and S_OK serve the batched solve: its start found the residual already within the
+// tolerance; its final residual passed the checks that let the pressure be written back
+enum Scal { S_RZ, S_PQ, S_ALPHA, S_BETA, S_RR, S_TARGET, S_DONE, S_ITERS, S_MAXIT, S_BB, S_RR0, S_SKIP, S_OK, S_COUNT };
 
 template <typename R> struct PpeSolver {
     int cap = 0;
@@ -224,11 +226,11 @@ template <int NW> __device__ inline double block_sum(double v, double* lds) {
 // width is the length of the slice's first (longest) row rounded up to 4.  Positions past the
 // window's last row are empty (row -1) and their slices have width 0.
 static constexpr int SELL_BLOCK = SELL_SIGMA < 1024 ? SELL_SIGMA : 1024;
-__global__ __launch_bounds__(SELL_BLOCK) void k_sell_order(int N, int nslices, const int* __restrict__ cnt,
-                                                           const int* __restrict__ dir, int* pos, int* rowp,
-                                                           int* slice_w) {
+__device__ inline void sell_order_body(int blk, int N, int nslices, const int* __restrict__ cnt,
+                                       const int* __restrict__ dir, int* pos, int* rowp,
+                                       int* slice_w) {
     __shared__ uint32_t v[SELL_SIGMA];
-    const int w0 = blockIdx.x * SELL_SIGMA;
+    const int w0 = blk * SELL_SIGMA;
     const int pc = min(SELL_SIGMA, nslices * 64 - w0);
     constexpr uint32_t KTOP = (1u << 20) - 2u;
     for (int i = threadIdx.x; i < SELL_SIGMA; i += SELL_BLOCK) {
@@ -267,11 +269,16 @@ __global__ __launch_bounds__(SELL_BLOCK) void k_sell_order(int N, int nslices, c
         }
     }
 }
+__global__ __launch_bounds__(SELL_BLOCK) void k_sell_order(int N, int nslices, const int* __restrict__ cnt,
+                                                           const int* __restrict__ dir, int* pos, int* rowp,
+                                                           int* slice_w) {
+    sell_order_body(blockIdx.x, N, nslices, cnt, dir, pos, rowp, slice_w);
+}
 
 // first_ge[K] = the first sorted row whose cell key is >= K (a lower bound per cell, empty cells
 // included), K = 0 .. ncell
-__global__ __launch_bounds__(PB) void k_first_ge(int N, int ncell, const uint32_t* __restrict__ skeys, int* fg) {
-    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void first_ge_body(int blk, int N, int ncell, const uint32_t* __restrict__ skeys, int* fg) {
+    const int K = blk * blockDim.x + threadIdx.x;
     if (K > ncell) return;
     int lo = 0, hi = N;
     while (lo < hi) {
@@ -280,6 +287,9 @@ __global__ __launch_bounds__(PB) void k_first_ge(int N, int ncell, const uint32_
         else hi = mid;
     }
     fg[K] = lo;
+}
+__global__ __launch_bounds__(PB) void k_first_ge(int N, int ncell, const uint32_t* __restrict__ skeys, int* fg) {
+    first_ge_body(blockIdx.x, N, ncell, skeys, fg);
 }
 
 // 16-bit column codes.  The cells are numbered x-slowest, z-fastest, so the neighbours of a row in
@@ -345,16 +355,16 @@ __global__ __launch_bounds__(PB) void k_win_box(int N, int nwin, int ncell, int 
 }
 
 template <typename R, typename VA>
-__global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c, const int* __restrict__ active,
-                                                  const int* __restrict__ dir, const R4<R>* __restrict__ spos,
-                                                  const int* __restrict__ cs, const int* __restrict__ ce,
-                                                  const int* __restrict__ slice_off, const int* __restrict__ slice_w,
-                                                  const int* __restrict__ perm, PState<R> out, double alpha, double n0,
-                                                  const int* __restrict__ rowp, int* cols, VA* vals, double* rhs,
-                                                  int* cnt, int* nnz, const int* __restrict__ nbq,
-                                                  const R* __restrict__ nbw, int nb_width, uint16_t* cols16,
-                                                  const uint32_t* __restrict__ skeys, const int* __restrict__ fg) {
-    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void ppe_fill_body(int blk, int N, int nslices, Consts<R> c, const int* __restrict__ active,
+                                     const int* __restrict__ dir, const R4<R>* __restrict__ spos,
+                                     const int* __restrict__ cs, const int* __restrict__ ce,
+                                     const int* __restrict__ slice_off, const int* __restrict__ slice_w,
+                                     const int* __restrict__ perm, PState<R> out, double alpha, double n0,
+                                     const int* __restrict__ rowp, int* cols, VA* vals, double* rhs,
+                                     int* cnt, int* nnz, const int* __restrict__ nbq,
+                                     const R* __restrict__ nbw, int nb_width, uint16_t* cols16,
+                                     const uint32_t* __restrict__ skeys, const int* __restrict__ fg) {
+    const int P = blk * blockDim.x + threadIdx.x;
     const int slice = P >> 6, lane = lane_id();
     if (slice >= nslices) return;
     const int t = rowp[P];   // -1: empty position
@@ -426,16 +436,29 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
     }
     (void)N;
 }
+template <typename R, typename VA>
+__global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c, const int* __restrict__ active,
+                                                  const int* __restrict__ dir, const R4<R>* __restrict__ spos,
+                                                  const int* __restrict__ cs, const int* __restrict__ ce,
+                                                  const int* __restrict__ slice_off, const int* __restrict__ slice_w,
+                                                  const int* __restrict__ perm, PState<R> out, double alpha, double n0,
+                                                  const int* __restrict__ rowp, int* cols, VA* vals, double* rhs,
+                                                  int* cnt, int* nnz, const int* __restrict__ nbq,
+                                                  const R* __restrict__ nbw, int nb_width, uint16_t* cols16,
+                                                  const uint32_t* __restrict__ skeys, const int* __restrict__ fg) {
+    ppe_fill_body<R, VA>(blockIdx.x, N, nslices, c, active, dir, spos, cs, ce, slice_off, slice_w, perm, out, alpha,
+                         n0, rowp, cols, vals, rhs, cnt, nnz, nbq, nbw, nb_width, cols16, skeys, fg);
+}
 
 // ---------------------------------------------------------------------------------------------
 // solver kernels
 // ---------------------------------------------------------------------------------------------
 // start of a PCG on A e = r0 (vectors V): r = r0, z = r / diag, p = z, partial r.z and r.r
 template <typename V>
-__global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restrict__ r0, const double* __restrict__ diag,
-                                                   V* r, V* z, V* p, V* e, int zero_e, double* part) {
+__device__ inline void pcg_start_body(int blk, int N, const double* __restrict__ r0, const double* __restrict__ diag,
+                                      V* r, V* z, V* p, V* e, int zero_e, double* part) {
     __shared__ double lds[PB / 64];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blk * blockDim.x + threadIdx.x;
     double rz = 0.0, rr = 0.0;
     if (t < N) {
         const V rv = (V)r0[t];
@@ -450,9 +473,14 @@ __global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restric
     rz = block_sum<PB / 64>(rz, lds);
     rr = block_sum<PB / 64>(rr, lds);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = rz;
-        part[2 * blockIdx.x + 1] = rr;
+        part[2 * blk] = rz;
+        part[2 * blk + 1] = rr;
     }
+}
+template <typename V>
+__global__ __launch_bounds__(PB) void k_pcg_start(int N, const double* __restrict__ r0, const double* __restrict__ diag,
+                                                   V* r, V* z, V* p, V* e, int zero_e, double* part) {
+    pcg_start_body<V>(blockIdx.x, N, r0, diag, r, z, p, e, zero_e, part);
 }
 
 // q = A p (values VA, arithmetic V), partial p.q.  One lane per row of a 64-row slice; the entry
@@ -588,14 +616,14 @@ __device__ inline bool stage_window(const int* __restrict__ wbox, int wcap, cons
 }
 
 template <typename VA, typename V, bool C16, bool WIN>
-__global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restrict__ rowp,
-                                                  const int* __restrict__ cnt, const int* __restrict__ slice_off,
-                                                  const int* __restrict__ slice_w, const int* __restrict__ cols,
-                                                  const VA* __restrict__ vals, const double* __restrict__ diag,
-                                                  const V* __restrict__ p, V* q, const double* __restrict__ scal,
-                                                  double* part, const uint16_t* __restrict__ codes,
-                                                  const uint32_t* __restrict__ skeys, const int* __restrict__ fg,
-                                                  int gz, int gygz, const int* __restrict__ wbox, int wcap) {
+__device__ inline void pcg_spmv_body(int blk, int nslices, const int* __restrict__ rowp,
+                                     const int* __restrict__ cnt, const int* __restrict__ slice_off,
+                                     const int* __restrict__ slice_w, const int* __restrict__ cols,
+                                     const VA* __restrict__ vals, const double* __restrict__ diag,
+                                     const V* __restrict__ p, V* q, const double* __restrict__ scal,
+                                     double* part, const uint16_t* __restrict__ codes,
+                                     const uint32_t* __restrict__ skeys, const int* __restrict__ fg,
+                                     int gz, int gygz, const int* __restrict__ wbox, int wcap) {
     static_assert(!WIN || C16, "the staged window needs the 16-bit codes");
     if (scal[S_DONE] != 0.0) return;
     __shared__ double lds[PB / 64];
@@ -605,7 +633,7 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
     int lb0 = 0, lb1 = 0, lb2 = 0;
     bool staged = false;
     if constexpr (WIN) staged = stage_window(wbox, wcap, p, win, lb0, lb1, lb2);
-    const int P = blockIdx.x * blockDim.x + threadIdx.x;   // SELL position
+    const int P = blk * blockDim.x + threadIdx.x;   // SELL position
     const int slice = P >> 6, lane = lane_id();
     const int t = slice < nslices ? rowp[P] : -1;
     double pq = 0.0;
@@ -632,21 +660,33 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
         pq = (double)pt * (double)a;
     }
     pq = block_sum<PB / 64>(pq, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = pq;
+    if (threadIdx.x == 0) part[blk] = pq;
+}
+template <typename VA, typename V, bool C16, bool WIN>
+__global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restrict__ rowp,
+                                                  const int* __restrict__ cnt, const int* __restrict__ slice_off,
+                                                  const int* __restrict__ slice_w, const int* __restrict__ cols,
+                                                  const VA* __restrict__ vals, const double* __restrict__ diag,
+                                                  const V* __restrict__ p, V* q, const double* __restrict__ scal,
+                                                  double* part, const uint16_t* __restrict__ codes,
+                                                  const uint32_t* __restrict__ skeys, const int* __restrict__ fg,
+                                                  int gz, int gygz, const int* __restrict__ wbox, int wcap) {
+    pcg_spmv_body<VA, V, C16, WIN>(blockIdx.x, nslices, rowp, cnt, slice_off, slice_w, cols, vals, diag, p, q, scal,
+                                   part, codes, skeys, fg, gz, gygz, wbox, wcap);
 }
 
 // r = b - A x with FP64 accumulation (x, b, r FP64; values VA); partial sums of r.r and b.b.
 // Columns and the staged window as in k_pcg_spmv (the window holds FP64 values of x); the sum runs
 // entry by entry in either form.
 template <typename VA, bool C16, bool WIN>
-__global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __restrict__ slice_off,
-                                               const int* __restrict__ slice_w, const int* __restrict__ cols,
-                                               const VA* __restrict__ vals, const double* __restrict__ diag,
-                                               const double* __restrict__ x, const double* __restrict__ b,
-                                               const int* __restrict__ rowp, const int* __restrict__ cnt, double* r,
-                                               double* part, const uint16_t* __restrict__ codes,
-                                               const uint32_t* __restrict__ skeys, const int* __restrict__ fg, int gz,
-                                               int gygz, const int* __restrict__ wbox, int wcap) {
+__device__ inline void resid_body(int blk, int N, int nslices, const int* __restrict__ slice_off,
+                                  const int* __restrict__ slice_w, const int* __restrict__ cols,
+                                  const VA* __restrict__ vals, const double* __restrict__ diag,
+                                  const double* __restrict__ x, const double* __restrict__ b,
+                                  const int* __restrict__ rowp, const int* __restrict__ cnt, double* r,
+                                  double* part, const uint16_t* __restrict__ codes,
+                                  const uint32_t* __restrict__ skeys, const int* __restrict__ fg, int gz,
+                                  int gygz, const int* __restrict__ wbox, int wcap) {
     static_assert(!WIN || C16, "the staged window needs the 16-bit codes");
     __shared__ double lds[PB / 64];
     extern __shared__ __align__(16) unsigned char resid_dyn[];
@@ -654,7 +694,7 @@ __global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __r
     int lb0 = 0, lb1 = 0, lb2 = 0;
     bool staged = false;
     if constexpr (WIN) staged = stage_window(wbox, wcap, x, win, lb0, lb1, lb2);
-    const int P = blockIdx.x * blockDim.x + threadIdx.x;   // SELL position
+    const int P = blk * blockDim.x + threadIdx.x;   // SELL position
     const int slice = P >> 6, lane = lane_id();
     const int t = slice < nslices ? rowp[P] : -1;
     double rr = 0.0, bb = 0.0;
@@ -682,20 +722,32 @@ __global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __r
     rr = block_sum<PB / 64>(rr, lds);
     bb = block_sum<PB / 64>(bb, lds);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = rr;
-        part[2 * blockIdx.x + 1] = bb;
+        part[2 * blk] = rr;
+        part[2 * blk + 1] = bb;
     }
     (void)N;
+}
+template <typename VA, bool C16, bool WIN>
+__global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __restrict__ slice_off,
+                                               const int* __restrict__ slice_w, const int* __restrict__ cols,
+                                               const VA* __restrict__ vals, const double* __restrict__ diag,
+                                               const double* __restrict__ x, const double* __restrict__ b,
+                                               const int* __restrict__ rowp, const int* __restrict__ cnt, double* r,
+                                               double* part, const uint16_t* __restrict__ codes,
+                                               const uint32_t* __restrict__ skeys, const int* __restrict__ fg, int gz,
+                                               int gygz, const int* __restrict__ wbox, int wcap) {
+    resid_body<VA, C16, WIN>(blockIdx.x, N, nslices, slice_off, slice_w, cols, vals, diag, x, b, rowp, cnt, r, part,
+                             codes, skeys, fg, gz, gygz, wbox, wcap);
 }
 
 // x += alpha p; r -= alpha q; z = r / diag; partial r.z and r.r
 template <typename V>
-__global__ __launch_bounds__(PB) void k_pcg_update(int N, V* x, V* r, V* z, const V* __restrict__ p,
-                                                    const V* __restrict__ q, const double* __restrict__ diag,
-                                                    const double* __restrict__ scal, double* part) {
+__device__ inline void pcg_update_body(int blk, int N, V* x, V* r, V* z, const V* __restrict__ p,
+                                       const V* __restrict__ q, const double* __restrict__ diag,
+                                       const double* __restrict__ scal, double* part) {
     if (scal[S_DONE] != 0.0) return;
     __shared__ double lds[PB / 64];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blk * blockDim.x + threadIdx.x;
     const V alpha = (V)scal[S_ALPHA];
     double rz = 0.0, rr = 0.0;
     if (t < N) {
@@ -710,25 +762,36 @@ __global__ __launch_bounds__(PB) void k_pcg_update(int N, V* x, V* r, V* z, cons
     rz = block_sum<PB / 64>(rz, lds);
     rr = block_sum<PB / 64>(rr, lds);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = rz;
-        part[2 * blockIdx.x + 1] = rr;
+        part[2 * blk] = rz;
+        part[2 * blk + 1] = rr;
     }
+}
+template <typename V>
+__global__ __launch_bounds__(PB) void k_pcg_update(int N, V* x, V* r, V* z, const V* __restrict__ p,
+                                                    const V* __restrict__ q, const double* __restrict__ diag,
+                                                    const double* __restrict__ scal, double* part) {
+    pcg_update_body<V>(blockIdx.x, N, x, r, z, p, q, diag, scal, part);
 }
 
 // p = z + beta p (after the beta reduction; skipped once the solve has converged)
 template <typename V>
+__device__ inline void pcg_direction_body(int blk, int N, const V* __restrict__ z, V* p,
+                                          const double* __restrict__ scal) {
+    if (scal[S_DONE] != 0.0) return;
+    const int t = blk * blockDim.x + threadIdx.x;
+    if (t < N) p[t] = z[t] + (V)scal[S_BETA] * p[t];
+}
+template <typename V>
 __global__ __launch_bounds__(PB) void k_pcg_direction(int N, const V* __restrict__ z, V* p,
                                                        const double* __restrict__ scal) {
-    if (scal[S_DONE] != 0.0) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < N) p[t] = z[t] + (V)scal[S_BETA] * p[t];
+    pcg_direction_body<V>(blockIdx.x, N, z, p, scal);
 }
 
 enum { RED_START = 0, RED_ALPHA = 1, RED_BETA = 2, RED_RESID = 3 };
 
 // one workgroup: sums the block partials (1 or 2 interleaved sums) and advances the scalars
-__global__ __launch_bounds__(1024) void k_pcg_reduce(const double* __restrict__ part, int nb, int nsum, double* scal,
-                                                      int mode) {
+__device__ inline void pcg_reduce_body(int blk, const double* __restrict__ part, int nb, int nsum, double* scal,
+                                       int mode) {
     if (mode == RED_ALPHA || mode == RED_BETA)
         if (scal[S_DONE] != 0.0) return;
     __shared__ double lds[16];
@@ -771,6 +834,10 @@ __global__ __launch_bounds__(1024) void k_pcg_reduce(const double* __restrict__ 
         scal[S_BB] = s1;
     }
 }
+__global__ __launch_bounds__(1024) void k_pcg_reduce(const double* __restrict__ part, int nb, int nsum, double* scal,
+                                                      int mode) {
+    pcg_reduce_body(blockIdx.x, part, nb, nsum, scal, mode);
+}
 
 // previous pressure in sorted order (0 on Dirichlet rows or before the first solve)
 __global__ __launch_bounds__(PB) void k_xprev(int N, const int* __restrict__ perm, const int* __restrict__ dir,
@@ -794,21 +861,25 @@ __global__ __launch_bounds__(PB) void k_add_f2d(int N, double* x, const float* e
 }
 
 // x0 from the previous solution (slot order) or zero; Dirichlet rows are 0
-__global__ __launch_bounds__(PB) void k_warm(int N, const int* __restrict__ perm, const int* __restrict__ dir,
-                                              const double* __restrict__ prev, int use_prev, double* x) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void warm_body(int blk, int N, const int* __restrict__ perm, const int* __restrict__ dir,
+                                 const double* __restrict__ prev, int use_prev, double* x) {
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     x[t] = (use_prev && !dir[t]) ? prev[perm[t]] : 0.0;
+}
+__global__ __launch_bounds__(PB) void k_warm(int N, const int* __restrict__ perm, const int* __restrict__ dir,
+                                              const double* __restrict__ prev, int use_prev, double* x) {
+    warm_body(blockIdx.x, N, perm, dir, prev, use_prev, x);
 }
 
 // pressure to the particle state (the EOS kernel's writes, Kern :796-800) and to the sorted
 // pressure array; the solution is kept in slot order for the next warm start
 template <typename R>
-__global__ __launch_bounds__(PB) void k_ppe_writeback(int N, Consts<R> c, const int* __restrict__ perm,
-                                                       const int* __restrict__ dir, const double* __restrict__ x,
-                                                       PState<R> in, PState<R> out, const R4<R>* __restrict__ spos,
-                                                       R* spres, double* prev) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void ppe_writeback_body(int blk, int N, Consts<R> c, const int* __restrict__ perm,
+                                          const int* __restrict__ dir, const double* __restrict__ x,
+                                          PState<R> in, PState<R> out, const R4<R>* __restrict__ spos,
+                                          R* spres, double* prev) {
+    const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
     const R4<R> P = spos[t];
@@ -823,6 +894,13 @@ __global__ __launch_bounds__(PB) void k_ppe_writeback(int N, Consts<R> c, const 
     out.p[s] = pr;
     in.p[s] = pr;
     spres[t] = pr;
+}
+template <typename R>
+__global__ __launch_bounds__(PB) void k_ppe_writeback(int N, Consts<R> c, const int* __restrict__ perm,
+                                                       const int* __restrict__ dir, const double* __restrict__ x,
+                                                       PState<R> in, PState<R> out, const R4<R>* __restrict__ spos,
+                                                       R* spres, double* prev) {
+    ppe_writeback_body<R>(blockIdx.x, N, c, perm, dir, x, in, out, spos, spres, prev);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2276,6 +2354,382 @@ mpsr_status ppe_get_matrix(PpeSolver<R>* s, const Dev<R>& d, int* ptr, int* col,
 }  // namespace mpsr
 
 namespace mpsr {
+// ---------------------------------------------------------------------------------------------
+// The PPE solves of a batched step: the PPE buckets of one precision, every kernel one launch over
+// (bucket, block) work items.  Each bucket solves with the FP64 Jacobi-PCG of ppe_solve (its matrix,
+// residuals, reductions and stopping rules), so its pressures equal a single-domain run's with
+// that solver.  The host decisions of ppe_solve become device steps: the start is skipped when the
+// first residual is already within the tolerance (S_SKIP), the iterations run in fixed chunks
+// that return at once after convergence, and the final residual decides on the device whether the
+// pressure is written back (S_OK).  The matrix holds NB_WIDTH entries a row; a bucket whose
+// matrix would need more is reported after the step and left unsolved.
+// ---------------------------------------------------------------------------------------------
+template <typename R> struct PpeView {
+    int dev = 0;                 // index of the bucket in the group's device table
+    int N = 0, nslices = 0, nb = 0, use_prev = 0, maxit = 0;
+    double tol = 1e-8, alpha_src = 0.0, n0 = 0.0;
+    long long nnz_cap = 0;
+    int *active, *dir, *cnt, *nbq, *nbn;
+    R* nbw;
+    int *slice_w, *slice_off, *pos, *rowp, *total, *first_ge, *cols, *nnz;
+    uint16_t* cols16;
+    double *vals, *diag, *rhs, *x, *prev, *r, *z, *p, *q, *part, *scal;
+    int* flags;                  // [0] a previous pressure exists, [1] the matrix overflowed
+};
+
+template <typename R> struct OpPFirstGe {
+    const Dev<R>* dv;
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int b) const {
+        const Dev<R>& d = dv[pv[j].dev];
+        first_ge_body(b, d.N, (int)d.c.ncell, d.skeys, pv[j].first_ge);
+    }
+};
+template <typename R> struct OpPSell {
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int b) const {
+        const PpeView<R>& v = pv[j];
+        sell_order_body(b, v.N, v.nslices, v.cnt, v.dir, v.pos, v.rowp, v.slice_w);
+    }
+};
+template <typename R> struct OpPFill {
+    const Dev<R>* dv;
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int b) const {
+        const PpeView<R>& v = pv[j];
+        const Dev<R>& d = dv[v.dev];
+        if ((long long)*v.total > v.nnz_cap) {   // the same for every workgroup of the bucket
+            if (b == 0 && threadIdx.x == 0) v.flags[1] = 1;
+            return;
+        }
+        ppe_fill_body<R, double>(b, d.N, v.nslices, d.c, v.active, v.dir, d.spos, d.cell_start, d.cell_end,
+                                 v.slice_off, v.slice_w, d.perm, d.out, v.alpha_src, v.n0, v.rowp, v.cols, v.vals,
+                                 v.rhs, v.cnt, v.nnz, v.nbq, v.nbw, NB_WIDTH, v.cols16, d.skeys, v.first_ge);
+    }
+};
+template <typename R> struct OpPWarm {
+    const Dev<R>* dv;
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int b) const {
+        const PpeView<R>& v = pv[j];
+        warm_body(b, v.N, dv[v.dev].perm, v.dir, v.prev, v.use_prev && v.flags[0], v.x);
+    }
+};
+template <typename R> struct OpPResid {
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int b) const {
+        const PpeView<R>& v = pv[j];
+        if (v.flags[1]) return;
+        resid_body<double, false, false>(b, v.N, v.nslices, v.slice_off, v.slice_w, v.cols, v.vals, v.diag, v.x, v.rhs,
+                                         v.rowp, v.cnt, v.r, v.part, nullptr, nullptr, nullptr, 0, 0, nullptr, 0);
+    }
+};
+template <typename R> struct OpPReduce {
+    const PpeView<R>* pv;
+    int mode;
+    __device__ void operator()(int j, int b) const {
+        const PpeView<R>& v = pv[j];
+        if (mode == RED_START && v.scal[S_SKIP] != 0.0) {
+            if (threadIdx.x == 0) {
+                v.scal[S_ITERS] = 0.0;
+                v.scal[S_DONE] = 1.0;
+            }
+            return;
+        }
+        pcg_reduce_body(b, v.part, v.nb, mode == RED_ALPHA ? 1 : 2, v.scal, mode);
+    }
+};
+// the decisions ppe_solve takes on the host after the first residual: the target, the iteration
+// cap, and whether the PCG runs at all
+template <typename R> struct OpPGate {
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int) const {
+        const PpeView<R>& v = pv[j];
+        if (threadIdx.x != 0) return;
+        const double rr = v.scal[S_RR], bb = v.scal[S_BB];
+        const double target = v.tol * v.tol * bb;
+        v.scal[S_TARGET] = target;
+        v.scal[S_MAXIT] = (double)v.maxit;
+        v.scal[S_SKIP] = (v.flags[1] || !(bb > 0 && rr > target)) ? 1.0 : 0.0;
+        v.scal[S_DONE] = 0.0;
+        v.scal[S_OK] = 0.0;
+    }
+};
+template <typename R> struct OpPStart {
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int b) const {
+        const PpeView<R>& v = pv[j];
+        if (v.scal[S_SKIP] != 0.0) return;
+        pcg_start_body<double>(b, v.N, v.r, v.diag, v.r, v.z, v.p, v.x, 0, v.part);
+    }
+};
+template <typename R> struct OpPSpmv {
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int b) const {
+        const PpeView<R>& v = pv[j];
+        pcg_spmv_body<double, double, false, false>(b, v.nslices, v.rowp, v.cnt, v.slice_off, v.slice_w, v.cols,
+                                                    v.vals, v.diag, v.p, v.q, v.scal, v.part, nullptr, nullptr,
+                                                    nullptr, 0, 0, nullptr, 0);
+    }
+};
+template <typename R> struct OpPUpdate {
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int b) const {
+        const PpeView<R>& v = pv[j];
+        pcg_update_body<double>(b, v.N, v.x, v.r, v.z, v.p, v.q, v.diag, v.scal, v.part);
+    }
+};
+template <typename R> struct OpPDirection {
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int b) const {
+        const PpeView<R>& v = pv[j];
+        pcg_direction_body<double>(b, v.N, v.z, v.p, v.scal);
+    }
+};
+// ppe_solve's checks of the final residual: no breakdown, a finite relative residual, and the
+// residual within the tolerance
+template <typename R> struct OpPVerdict {
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int) const {
+        const PpeView<R>& v = pv[j];
+        if (threadIdx.x != 0) return;
+        const double rr = v.scal[S_RR], bb = v.scal[S_BB];
+        const double rel = bb > 0 ? sqrt(rr / bb) : 0.0;
+        const bool ok = !v.flags[1] && v.scal[S_DONE] != 3.0 && isfinite(rel) && !(bb > 0 && rr > v.scal[S_TARGET]);
+        v.scal[S_OK] = ok ? 1.0 : 0.0;
+    }
+};
+template <typename R> struct OpPWriteback {
+    const Dev<R>* dv;
+    const PpeView<R>* pv;
+    __device__ void operator()(int j, int b) const {
+        const PpeView<R>& v = pv[j];
+        if (v.scal[S_OK] == 0.0) return;
+        const Dev<R>& d = dv[v.dev];
+        ppe_writeback_body<R>(b, v.N, d.c, d.perm, v.dir, v.x, d.in, d.out, d.spos, d.spres, v.prev);
+        if (b == 0 && threadIdx.x == 0) v.flags[0] = 1;
+    }
+};
+template <typename R> struct OpPFlags {
+    const PpeView<R>* pv;
+    int* done;
+    __device__ void operator()(int j, int) const {
+        if (threadIdx.x == 0) done[j] = pv[j].scal[S_DONE] != 0.0 ? 1 : 0;
+    }
+};
+
+template <typename R> struct PpeBatch {
+    std::vector<PpeSolver<R>*> s;
+    std::vector<PpeView<R>> hv;
+    PpeView<R>* views = nullptr;
+    PndRows<R>* pnd = nullptr;
+    int* flags = nullptr;      // [n * 2]
+    int* done = nullptr;       // [n] device
+    int* done_h = nullptr;     // [n] pinned host
+    char* tab = nullptr;
+    size_t tab_bytes = 0;
+    WorkList wpnd, wcells, wwin, wvec, wrow, wone, wfill, wscan_t, wscan_1;
+    const FillJob* fills = nullptr;
+    const ScanJob* scans = nullptr;
+    int n = 0;
+};
+
+template <typename R> PpeBatch<R>* ppe_batch_create() { return new PpeBatch<R>(); }
+template <typename R> void ppe_batch_destroy(PpeBatch<R>* b) {
+    if (!b) return;
+    for (void* p : {(void*)b->views, (void*)b->pnd, (void*)b->flags, (void*)b->done, (void*)b->tab})
+        if (p) (void)hipFree(p);
+    if (b->done_h) (void)hipHostFree(b->done_h);
+    delete b;
+}
+
+template <typename R>
+mpsr_status ppe_batch_build(PpeBatch<R>* b, PpeSolver<R>* const* solvers, const int* dev, const Dev<R>* host_devs,
+                            const mpsr_ppe_options* opts, int n, int block) {
+    const bool first = b->views == nullptr;
+    if (!first && n != b->n) return MPSR_PARAMERR;
+    b->n = n;
+    b->s.assign(solvers, solvers + n);
+    b->hv.assign(n, PpeView<R>{});
+    if (first) {
+        if (hipMalloc((void**)&b->views, sizeof(PpeView<R>) * std::max(1, n)) != hipSuccess ||
+            hipMalloc((void**)&b->pnd, sizeof(PndRows<R>) * std::max(1, n)) != hipSuccess ||
+            hipMalloc((void**)&b->flags, sizeof(int) * 2 * std::max(1, n)) != hipSuccess ||
+            hipMalloc((void**)&b->done, sizeof(int) * std::max(1, n)) != hipSuccess ||
+            hipHostMalloc((void**)&b->done_h, sizeof(int) * std::max(1, n)) != hipSuccess)
+            return MPSR_MEMERR;
+        std::vector<int> f(2 * std::max(1, n), 0);
+        for (int j = 0; j < n; ++j) f[2 * j] = solvers[j]->have_prev;
+        if (hipMemcpy(b->flags, f.data(), sizeof(int) * f.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return MPSR_GPUERR;
+    }
+    std::vector<PndRows<R>> rows(std::max(1, n));
+    std::vector<int2> pnd, cells, win, vec, row, one, fill, scan_t, scan_1;
+    std::vector<FillJob> fj;
+    std::vector<ScanJob> sj;
+    for (int j = 0; j < n; ++j) {
+        PpeSolver<R>* s = solvers[j];
+        const Dev<R>& d = host_devs[dev[j]];
+        const mpsr_ppe_options& o = opts[j];
+        const int N = d.N;
+        const int nsl = (N + 63) / 64;
+        // capacity for NB_WIDTH entries a row, the matrix arrays of the FP64 solver
+        const long long need = (long long)nsl * 64 * NB_WIDTH;
+        if (need > s->nnz_cap) {
+            for (void* p : {(void*)s->cols, (void*)s->cols16, (void*)s->vals64, (void*)s->vals32})
+                if (p) (void)hipFree(p);
+            s->cols = nullptr; s->cols16 = nullptr; s->vals64 = nullptr; s->vals32 = nullptr;
+            s->nnz_cap = need;
+            if (!dev_alloc(&s->cols, (size_t)need) || !dev_alloc(&s->cols16, (size_t)need) ||
+                !dev_alloc(&s->vals64, (size_t)need) || !dev_alloc(&s->vals32, (size_t)need))
+                return MPSR_MEMERR;
+        }
+        s->nslices = nsl;
+        s->mixed = 0;
+        s->c16 = 0;
+        s->wcap = 0;
+        s->skeys = d.skeys;
+        PpeView<R> v{};
+        v.dev = dev[j];
+        v.N = N;
+        v.nslices = nsl;
+        v.nb = (nsl * 64 + PB - 1) / PB;
+        v.use_prev = o.warm_start != MPSR_WARM_ZERO ? 1 : 0;
+        v.maxit = o.max_iter > 0 ? o.max_iter : 20000;
+        v.tol = o.rel_tol > 0 ? o.rel_tol : 1e-8;
+        v.alpha_src = s->alpha * (o.source_relaxation > 0 ? o.source_relaxation : 0.1);
+        v.n0 = s->n0;
+        v.nnz_cap = s->nnz_cap;
+        v.active = s->active; v.dir = s->dir; v.cnt = s->cnt; v.nbq = s->nbq; v.nbn = s->nbn; v.nbw = s->nbw;
+        v.slice_w = s->slice_w; v.slice_off = s->slice_off; v.pos = s->pos; v.rowp = s->rowp; v.total = s->total;
+        v.first_ge = s->first_ge; v.cols = s->cols; v.nnz = s->nnz_dev; v.cols16 = s->cols16; v.vals = s->vals64;
+        v.diag = s->diag; v.rhs = s->rhs; v.x = s->x; v.prev = s->prev; v.r = s->r; v.z = s->z; v.p = s->p0;
+        v.q = s->q; v.part = s->part; v.scal = s->scal;
+        v.flags = b->flags + 2 * j;
+        b->hv[j] = v;
+        rows[j] = PndRows<R>{dev[j], s->active, s->cnt, s->diag, s->dir, s->nbq, s->nbw, s->nbn, NB_WIDTH};
+        for (int k = 0; k < (int)grid_for(N, block); ++k) pnd.push_back(make_int2(j, k));
+        for (int k = 0; k < (int)grid_for((long long)d.c.ncell + 1, PB); ++k) cells.push_back(make_int2(j, k));
+        for (int k = 0; k < (int)grid_for(N, SELL_SIGMA); ++k) win.push_back(make_int2(j, k));
+        for (int k = 0; k < v.nb; ++k) vec.push_back(make_int2(j, k));
+        for (int k = 0; k < (int)grid_for(N, PB); ++k) row.push_back(make_int2(j, k));
+        one.push_back(make_int2(j, 0));
+        const int jf = (int)fj.size();
+        fj.push_back(FillJob{s->nnz_dev, 0, 4});
+        fj.push_back(FillJob{b->flags + 2 * j + 1, 0, 1});   // the overflow flag of the last step
+        fill.push_back(make_int2(jf, 0));
+        fill.push_back(make_int2(jf + 1, 0));
+        const int js = (int)sj.size();
+        sj.push_back(ScanJob{s->slice_w, s->slice_off, nullptr, nsl, nsl, s->total, s->scan.scan_tmp});
+        for (int k = 0; k < (int)grid_for(nsl, scan_tile()); ++k) scan_t.push_back(make_int2(js, k));
+        scan_1.push_back(make_int2(js, 0));
+    }
+    if (hipMemcpy(b->views, b->hv.data(), sizeof(PpeView<R>) * std::max(1, n), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->pnd, rows.data(), sizeof(PndRows<R>) * rows.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return MPSR_GPUERR;
+    std::vector<char> img;
+    auto put = [&](const void* p, size_t bytes) {
+        const size_t off = (img.size() + 15) & ~(size_t)15;
+        img.resize(off + bytes);
+        if (bytes) std::memcpy(img.data() + off, p, bytes);
+        return off;
+    };
+    std::vector<std::pair<size_t, int>> refs;
+    for (const auto* v : {&pnd, &cells, &win, &vec, &row, &one, &fill, &scan_t, &scan_1})
+        refs.push_back({put(v->data(), v->size() * sizeof(int2)), (int)v->size()});
+    const size_t ofj = put(fj.data(), fj.size() * sizeof(FillJob)), osj = put(sj.data(), sj.size() * sizeof(ScanJob));
+    if (b->tab && img.size() > b->tab_bytes) {
+        (void)hipFree(b->tab);
+        b->tab = nullptr;
+    }
+    if (!b->tab) {
+        b->tab_bytes = std::max<size_t>(16, img.size());
+        if (hipMalloc((void**)&b->tab, b->tab_bytes) != hipSuccess) return MPSR_MEMERR;
+    }
+    if (hipMemcpy(b->tab, img.data(), img.size(), hipMemcpyHostToDevice) != hipSuccess) return MPSR_GPUERR;
+    auto wl = [&](int i) { return WorkList{reinterpret_cast<const int2*>(b->tab + refs[i].first), refs[i].second}; };
+    b->wpnd = wl(0); b->wcells = wl(1); b->wwin = wl(2); b->wvec = wl(3); b->wrow = wl(4); b->wone = wl(5);
+    b->wfill = wl(6); b->wscan_t = wl(7); b->wscan_1 = wl(8);
+    b->fills = reinterpret_cast<const FillJob*>(b->tab + ofj);
+    b->scans = reinterpret_cast<const ScanJob*>(b->tab + osj);
+    return MPSR_NOERR;
+}
+
+// the PND pass of the PPE buckets (active rows, bounds, diagonals, neighbour lists)
+template <typename R> void ppe_batch_pnd(PpeBatch<R>* b, const BatchLaunch<R>& bl, hipStream_t st) {
+    launch_pnd_ppe_b(bl, b->pnd, b->wpnd, st);
+}
+template <typename R> void ppe_batch_iterate(PpeBatch<R>* b, int iters, hipStream_t st) {
+    for (int k = 0; k < iters; ++k) {
+        launch_batched<PB>(OpPSpmv<R>{b->views}, b->wvec, PB, st);
+        launch_batched<1024>(OpPReduce<R>{b->views, RED_ALPHA}, b->wone, 1024, st);
+        launch_batched<PB>(OpPUpdate<R>{b->views}, b->wvec, PB, st);
+        launch_batched<1024>(OpPReduce<R>{b->views, RED_BETA}, b->wone, 1024, st);
+        launch_batched<PB>(OpPDirection<R>{b->views}, b->wvec, PB, st);
+    }
+}
+// assembly, warm start, first residual, the decisions on it, the PCG start and `iters` iterations
+template <typename R> void ppe_batch_head(PpeBatch<R>* b, const BatchLaunch<R>& bl, int iters, hipStream_t st) {
+    fill_int_batched(b->fills, b->wfill, st);
+    launch_batched<PB>(OpPFirstGe<R>{bl.devs, b->views}, b->wcells, PB, st);
+    launch_batched<SELL_BLOCK>(OpPSell<R>{b->views}, b->wwin, SELL_BLOCK, st);
+    exclusive_scan_batched(b->scans, b->wscan_t, b->wscan_1, st);
+    launch_batched<PB>(OpPFill<R>{bl.devs, b->views}, b->wvec, PB, st);
+    launch_batched<PB>(OpPWarm<R>{bl.devs, b->views}, b->wrow, PB, st);
+    launch_batched<PB>(OpPResid<R>{b->views}, b->wvec, PB, st);
+    launch_batched<1024>(OpPReduce<R>{b->views, RED_RESID}, b->wone, 1024, st);
+    launch_batched<64>(OpPGate<R>{b->views}, b->wone, 64, st);
+    launch_batched<PB>(OpPStart<R>{b->views}, b->wvec, PB, st);
+    launch_batched<1024>(OpPReduce<R>{b->views, RED_START}, b->wone, 1024, st);
+    ppe_batch_iterate(b, iters, st);
+}
+// whether every bucket's PCG has stopped (synchronises the stream)
+template <typename R> mpsr_status ppe_batch_converged(PpeBatch<R>* b, hipStream_t st, bool* all) {
+    launch_batched<64>(OpPFlags<R>{b->views, b->done}, b->wone, 64, st);
+    if (hipMemcpyAsync(b->done_h, b->done, sizeof(int) * b->n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return MPSR_GPUERR;
+    *all = true;
+    for (int j = 0; j < b->n; ++j) *all = *all && b->done_h[j] != 0;
+    return MPSR_NOERR;
+}
+// final residual, its checks, and the pressure written back where they passed
+template <typename R> void ppe_batch_tail(PpeBatch<R>* b, const BatchLaunch<R>& bl, hipStream_t st) {
+    launch_batched<PB>(OpPResid<R>{b->views}, b->wvec, PB, st);
+    launch_batched<1024>(OpPReduce<R>{b->views, RED_RESID}, b->wone, 1024, st);
+    launch_batched<64>(OpPVerdict<R>{b->views}, b->wone, 64, st);
+    launch_batched<PB>(OpPWriteback<R>{bl.devs, b->views}, b->wrow, PB, st);
+}
+// after a batched call: the solvers' host state, and the first bucket whose last solve failed
+template <typename R> mpsr_status ppe_batch_finish(PpeBatch<R>* b, int* failed, std::string* why) {
+    *failed = -1;
+    std::vector<int> f(2 * std::max(1, b->n));
+    if (hipMemcpy(f.data(), b->flags, sizeof(int) * f.size(), hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+    for (int j = 0; j < b->n; ++j) {
+        PpeSolver<R>* s = b->s[j];
+        double sc[S_COUNT];
+        if (hipMemcpy(sc, s->scal, sizeof(sc), hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
+        s->have_prev = f[2 * j];
+        s->iters = (int)sc[S_ITERS];
+        s->passes = 1;
+        s->relres = sc[S_BB] > 0 ? std::sqrt(sc[S_RR] / sc[S_BB]) : 0.0;
+        s->status = (f[2 * j + 1] || sc[S_OK] == 0.0) ? 1 : 0;
+        if (*failed < 0 && s->status) {
+            *failed = j;
+            char buf[256];
+            if (f[2 * j + 1])
+                std::snprintf(buf, sizeof(buf), "PPE matrix needs more than %d entries a row", NB_WIDTH);
+            else if (sc[S_DONE] == 3.0)
+                std::snprintf(buf, sizeof(buf), "PCG breakdown after %d iterations", (int)sc[S_ITERS]);
+            else
+                std::snprintf(buf, sizeof(buf), "not converged: relative residual %.3e after %d iterations",
+                              s->relres, (int)sc[S_ITERS]);
+            *why = buf;
+        }
+    }
+    return MPSR_NOERR;
+}
+
 #define MPSR_INSTANTIATE_P(R)                                                                                        \
     template mpsr_status ppe_get_matrix<R>(PpeSolver<R>*, const Dev<R>&, int*, int*, double*, double*, int, int*);   \
     template PpeSolver<R>* ppe_create<R>(const Dev<R>&, const mpsr_params&, const mpsr_ppe_options&, double, double, \
@@ -2294,7 +2748,17 @@ namespace mpsr {
     template mpsr_status ppe_mpnn_get<R>(PpeSolver<R>*, float*, hipStream_t);                                         \
     template mpsr_status ppe_mpnn_train<R>(PpeSolver<R>*, const Dev<R>&, double, double, int, float*, double*,       \
                                            hipStream_t);                                                              \
-    template mpsr_status ppe_mpnn_rows<R>(PpeSolver<R>*, const Dev<R>&, float*, float*, double*, hipStream_t);
+    template mpsr_status ppe_mpnn_rows<R>(PpeSolver<R>*, const Dev<R>&, float*, float*, double*, hipStream_t);     \
+    template PpeBatch<R>* ppe_batch_create<R>();                                                                    \
+    template void ppe_batch_destroy<R>(PpeBatch<R>*);                                                              \
+    template mpsr_status ppe_batch_build<R>(PpeBatch<R>*, PpeSolver<R>* const*, const int*, const Dev<R>*,          \
+                                            const mpsr_ppe_options*, int, int);                                    \
+    template void ppe_batch_pnd<R>(PpeBatch<R>*, const BatchLaunch<R>&, hipStream_t);                              \
+    template void ppe_batch_head<R>(PpeBatch<R>*, const BatchLaunch<R>&, int, hipStream_t);                        \
+    template void ppe_batch_iterate<R>(PpeBatch<R>*, int, hipStream_t);                                            \
+    template mpsr_status ppe_batch_converged<R>(PpeBatch<R>*, hipStream_t, bool*);                                 \
+    template void ppe_batch_tail<R>(PpeBatch<R>*, const BatchLaunch<R>&, hipStream_t);                             \
+    template mpsr_status ppe_batch_finish<R>(PpeBatch<R>*, int*, std::string*);
 MPSR_INSTANTIATE_P(float)
 MPSR_INSTANTIATE_P(double)
 }  // namespace mpsr

@@ -128,23 +128,62 @@ def test_batched_bucket_matches_oracle(small_buckets):
     b.close()
 
 
-def test_batch_rejects_ppe_buckets(lib):
-    """A bucket in PPE mode reads its convergence on the host every solve, so it cannot be part
-    of a recorded step: mpsr_create_batch refuses it with PARAMERR and names the reason."""
+def test_batch_rejects_ppe_buckets_with_other_solvers(lib):
+    """A PPE bucket of a batch solves with the FP64 Jacobi-PCG; mpsr_create_batch refuses one with
+    the mixed solver or another preconditioner with PARAMERR and names the bucket and the reason."""
     import ctypes as C
     eos = cases.buckets(lib, count=1, n_min=2e3, n_max=3e3)[0]
     ppe = cases.small_box(lib)
+    mixed = abi.PpeOptions(abi.MPSR_PCG_MIXED, 1e-8, 20000, abi.MPSR_WARM_PREVIOUS, 0.0, 0.0,
+                           abi.MPSR_PRECOND_CELL_VCYCLE)
     infos, keep = [], []
-    for case, mode in ((eos, abi.MPSR_PRESSURE_EOS), (ppe, abi.MPSR_PRESSURE_PPE)):
-        info, k = cases.create_info(case, abi.MPSR_FP32, mode, None, None, 0)
+    for case, mode, opt in ((eos, abi.MPSR_PRESSURE_EOS, None), (ppe, abi.MPSR_PRESSURE_PPE, mixed)):
+        info, k = cases.create_info(case, abi.MPSR_FP32, mode, opt, None, 0)
         infos.append(info)
         keep.append(k)
     arr = (abi.CreateInfo * 2)(*infos)
     h = C.c_void_p()
     st = lib.mpsr_create_batch(arr, 2, C.byref(h))
     assert st == abi.MPSR_PARAMERR
-    assert b"bucket 1" in lib.mpsr_last_error() and b"EOS" in lib.mpsr_last_error()
+    assert b"bucket 1" in lib.mpsr_last_error() and b"MPSR_PCG_FP64" in lib.mpsr_last_error()
     assert not h.value
+
+
+def ppe_singles(case_list, precs, modes, first, count):
+    out = []
+    for c, prec, mode in zip(case_list, precs, modes):
+        opt = abi.PpeOptions(abi.MPSR_PCG_FP64, 1e-8, 20000, abi.MPSR_WARM_PREVIOUS, 0.0, 0.0,
+                             abi.MPSR_PRECOND_JACOBI)
+        s = Solver(c, precision=prec, pressure_mode=mode, ppe=opt if mode == abi.MPSR_PRESSURE_PPE else None)
+        s.step_many(first, count)
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_ppe_buckets_equal_single_domains(lib, graph):
+    """Buckets with the PPE pressure (FP64 Jacobi-PCG to 1e-8 from the previous pressure) next to
+    EOS buckets, in both precisions: every bucket after 6 batched steps (graph replays, or the same
+    launches made directly) equals its single-domain run bit for bit, and its last solve reached
+    the tolerance."""
+    bl = [cases.small_box(lib), cases.dam_break(lib, explicit_pressure=False)] + \
+        cases.buckets(lib, count=2, n_min=2e3, n_max=4e3, seed=5)
+    precs = [abi.MPSR_FP32, abi.MPSR_FP64, abi.MPSR_FP32, abi.MPSR_FP64]
+    modes = [abi.MPSR_PRESSURE_PPE, abi.MPSR_PRESSURE_PPE, abi.MPSR_PRESSURE_EOS, abi.MPSR_PRESSURE_PPE]
+    print(f"[batch-ppe] graph={graph} n = {[c.n for c in bl]}", flush=True)
+    b = Batch(bl, precs, pressure_modes=modes)
+    if graph:
+        b.step_many(0, 3)
+        b.step_many(3, 3)
+    else:
+        b.step_many_eager(0, 6)
+    ref = ppe_singles(bl, precs, modes, 0, 6)
+    for i, (g, s) in enumerate(zip(b.buckets, ref)):
+        assert_same(g, s, f"bucket {i}")
+        if modes[i] == abi.MPSR_PRESSURE_PPE:
+            assert g.timings()["ppe_rel_residual"] <= 1e-8
+            assert np.any(g.get_particles(0).pressure > 0)
+    b.close()
 
 
 def test_graph_steps_with_heat_conduction_every_second_step(lib):
