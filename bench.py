@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--arm-steps A] [--l0 2.012e-3]
                     [--solver learned|mixed|fp64] [--precond vcycle|two|jacobi] [--no-cpu-baseline] [--no-arms]
-    python bench.py --workload batched [--buckets 64] [--steps K] [--warmup W] [--no-tune]
+    python bench.py --workload batched [--buckets 64] [--ppe-every k] [--steps K] [--warmup W] [--no-tune]
     python bench.py --workload precision --l0 1.6e-3 --steps 200
     python bench.py --workload radix [--steps K] [--warmup W]
 
@@ -26,7 +26,8 @@ One JSON line: value = particle-updates/s over the K timed steps; PPE ms/step an
 every arm; roofline of the dominant kernel (the PCG SpMV) from HIP events around every SpMV launch
 of the timed region, on the solver's stream; a whole-step roofline from the algorithmic bytes of a
 step; the CPU oracle timed on the host cores (median of three steady-state steps on all cores, and
-one step on one core).
+one step on one core); and under "batched" the 64-bucket batched form (about 3.15M particles), once
+with the EOS pressure in every bucket and once with every 4th bucket solving the PPE.
 """
 import argparse
 import json
@@ -148,7 +149,11 @@ def main():
                     help="drum: the 1M-particle drum (default); batched: 64 independent rotating buckets; "
                          "precision: the mixed-precision drum against the FP64 build after --steps steps")
     ap.add_argument("--buckets", type=int, default=64)
-    ap.add_argument("--no-tune", action="store_true", help="batched: keep the default launch shape (256/8)")
+    ap.add_argument("--no-tune", action="store_true", help="batched: keep the default launch shape (256/16)")
+    ap.add_argument("--ppe-every", type=int, default=0,
+                    help="batched: every k-th bucket solves the PPE (FP64 Jacobi-PCG), 0 = none")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="drum: skip the batched-bucket measurements added to the line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs one domain per GPU and no multi-GPU path is "
@@ -217,6 +222,12 @@ def main():
         learned_vs_plain = round(arms["plain_fp64_jacobi"]["ppe_ms_per_step"] /
                                  arms["learned_mixed_vcycle"]["ppe_ms_per_step"], 3)
 
+    batch = None
+    if not args.no_batched:
+        # the batched form (64 buckets, ~3.15M particles): EOS only, and every 4th bucket with the PPE
+        batch = {"eos": run_batched(args.buckets, 0, 10, 3, False),
+                 "ppe_every_4th": run_batched(args.buckets, 4, 10, 3, False)}
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -257,61 +268,84 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": SPMV_TRAFFIC_BYTES.get((args.l0, kind)),
                      "bytes_per_launch": spmv_b, "avg_launch_ms": spmv_avg_ms, "launches": launches},
+        "batched": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
-def batched(args):
-    """64 independent rotating buckets (boxes and drums, 5k-200k particles drawn log-uniformly,
-    10-60 rpm, l0 = 5e-3 m, EOS pressure, FP32): the launch shape is autotuned first (each shape
-    timed on the same saved state, which is then restored), then K steps run as one graph launch
-    per step and K more with the same kernels launched directly, bucket by bucket."""
-    from simulation_project_amd import Batch, cases, load_library
+def run_batched(count, ppe_every, steps, warmup, tune):
+    """`count` rotating buckets (boxes and drums, 5k-200k particles drawn log-uniformly, 10-60 rpm,
+    l0 = 5e-3 m, FP32); with ppe_every = k every k-th bucket solves the PPE (FP64 Jacobi-PCG to
+    1e-8 from the previous pressure), the rest use the EOS pressure.  The launch shape is autotuned
+    first when `tune` (each shape timed on the same saved state, which is then restored), then
+    `steps` steps run as graph launches and `steps` more with the same launches made directly."""
+    from simulation_project_amd import Batch, abi, cases, load_library
     lib = load_library()
     t0 = time.time()
-    bl = cases.buckets(lib, count=args.buckets)
+    bl = cases.buckets(lib, count=count)
     n = sum(c.n for c in bl)
     tris = sum(len(c.vertices) for c in bl)
-    b = Batch(bl)
-    log(f"[bench] {len(bl)} buckets, {n} particles, {tris} triangles, built in {time.time() - t0:.1f}s")
-    if args.no_tune:
-        blk, items, table = 256, 8, {}
-    else:
+    modes = [abi.MPSR_PRESSURE_PPE if ppe_every and i % ppe_every == 0 else abi.MPSR_PRESSURE_EOS
+             for i in range(len(bl))]
+    n_ppe = sum(c.n for c, m in zip(bl, modes) if m == abi.MPSR_PRESSURE_PPE)
+    b = Batch(bl, pressure_modes=modes)
+    log(f"[bench] {len(bl)} buckets, {n} particles ({n_ppe} in PPE buckets), {tris} triangles, "
+        f"built in {time.time() - t0:.1f}s")
+    if tune:
         blk, items, table = b.autotune(0, 2)
         log(f"[bench] autotune: block {blk}, sort items {items}; " +
             ", ".join(f"{k[0]}/{k[1]}: {v:.2f}" for k, v in sorted(table.items())))
-    if args.warmup:
-        b.step_many(0, args.warmup)
-    b.step_many(args.warmup, args.steps)      # timed on the batch stream with HIP events
+    else:
+        blk, items, table = 256, 16, {}
+        b.set_launch(blk, items)
+    if warmup:
+        b.step_many(0, warmup)
+    b.step_many(warmup, steps)      # timed on the batch stream with HIP events
     info = b.info()
     ms_graph = info["ms_per_step"]
-    b.step_many_eager(args.warmup + args.steps, args.steps)
+    iters = [bk.timings()["ppe_iterations"] for bk, m in zip(b.buckets, modes) if m == abi.MPSR_PRESSURE_PPE]
+    b.step_many_eager(warmup + steps, steps)
     ms_eager = b.info()["ms_per_step"]
-    log(f"[bench] graph {ms_graph:.3f} ms/step ({info['graph_nodes']} nodes), direct launches {ms_eager:.3f} ms/step")
+    log(f"[bench] batched{' with PPE buckets' if n_ppe else ''}: graph {ms_graph:.3f} ms/step "
+        f"({info['graph_nodes']} nodes), direct launches {ms_eager:.3f} ms/step")
     b.close()
+    return {
+        "particle_updates_per_s": round(n / (ms_graph * 1e-3), 1),
+        "ms_per_step": round(ms_graph, 4),
+        "direct_launch_ms_per_step": round(ms_eager, 4),
+        "buckets": len(bl), "particles": n, "triangles": tris,
+        "ppe_buckets": sum(1 for m in modes if m == abi.MPSR_PRESSURE_PPE), "ppe_particles": n_ppe,
+        "ppe_iterations_last_step": {"min": min(iters), "max": max(iters)} if iters else None,
+        "graph_nodes": info["graph_nodes"],
+        "launch": {"block": blk, "sort_items": items,
+                   "autotune_ms": {f"{k[0]}/{k[1]}": round(v, 4) for k, v in sorted(table.items())}},
+    }
+
+
+def batched(args):
+    """The batched buckets as the bench line (see run_batched)."""
+    r = run_batched(args.buckets, args.ppe_every, args.steps, args.warmup, not args.no_tune)
+    ppe = f", every {args.ppe_every}th bucket with the PPE (FP64 Jacobi-PCG to 1e-8)" if args.ppe_every else ""
     out = {
-        "metric": "particle-updates/s, 64 batched rotating buckets",
-        "value": round(n / (ms_graph * 1e-3), 1),
+        "metric": f"particle-updates/s, {r['buckets']} batched rotating buckets",
+        "value": r["particle_updates_per_s"],
         "unit": "particle-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_graph, 4),
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{len(bl)} rotating buckets (boxes of 12 and drums of up to 9984 triangles), "
-                               f"{n} particles (log-uniform 5k-200k per bucket), 10-60 rpm, l0=5e-3 m, "
-                               f"EOS pressure, one graph launch per step",
-                   "buckets": len(bl), "particles": n, "triangles": tris},
-        "direct_launch_ms_per_step": round(ms_eager, 4),
-        "graph_nodes": info["graph_nodes"],
-        "launch": {"block": blk, "sort_items": items,
-                   "autotune_ms": {f"{k[0]}/{k[1]}": round(v, 4) for k, v in sorted(table.items())}},
+        "config": {"workload": f"{r['buckets']} rotating buckets (boxes of 12 and drums of up to 9984 triangles), "
+                               f"{r['particles']} particles (log-uniform 5k-200k per bucket), 10-60 rpm, l0=5e-3 m, "
+                               f"EOS pressure{ppe}, every kernel one launch over all buckets",
+                   "buckets": r["buckets"], "particles": r["particles"], "triangles": r["triangles"]},
     }
+    out.update({k: v for k, v in r.items() if k not in ("particle_updates_per_s", "ms_per_step")})
     print(json.dumps(out), flush=True)
 
 
