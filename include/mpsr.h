@@ -507,11 +507,19 @@ mpsr_status mpsr_set_launch(mpsr_handle *h, int block, int sort_items);
 /* ---- batched buckets (no counterpart in the reference) ------------------------------------------
  * Many independent domains ("buckets": each its own particles, walls, grid and precision) advanced
  * together.  The reference steps one domain per CalculateNew call (Src/MultiGPUHandler.cpp:
- * 362-492); here one step of every bucket is recorded once into a HIP graph, each bucket's
- * kernels on its own stream forked from the batch stream, and every step is one graph launch.
- * Buckets use the explicit (EOS) pressure and carry no spheres or bucket inflow (rejected with
- * MPSR_PARAMERR); every bucket must name the same device.  A bucket's results are those of
- * mpsr_step_many on the same handle. */
+ * 362-492); here every kernel of a step is one launch over a work list of (bucket, block) items
+ * of all buckets of a precision, and the step is recorded once into a HIP graph (about 46
+ * launches per precision whatever the number of buckets), so each step is one graph launch.
+ * A bucket uses the explicit (EOS) pressure, or the PPE with the FP64 Jacobi-PCG (solver
+ * MPSR_PCG_FP64, preconditioner MPSR_PRECOND_JACOBI, not the learned start); the PCG of the PPE
+ * buckets runs in chunks of iterations that return at once after convergence, so a step with PPE
+ * buckets is three graph launches (up to the PCG with its first 32 iterations, 16-iteration
+ * chunks until every bucket's PCG has stopped, the rest of the step) with one flag read per
+ * chunk.  Buckets carry no spheres or bucket inflow; other PPE solvers, spheres and inflow are
+ * rejected with MPSR_PARAMERR naming the bucket; every bucket must name the same device.  A
+ * bucket's results are those of mpsr_step_many on the same handle, bit for bit.  A failed PPE
+ * solve (breakdown, not converged, more than 192 matrix entries a row) is the call's
+ * MPSR_CALCERR, naming the bucket; that bucket's pressure is then not written back. */
 typedef struct mpsr_batch mpsr_batch;
 #define MPSR_TUNE_BLOCKS 4   /* blocks 64, 128, 256, 512 */
 #define MPSR_TUNE_ITEMS 3    /* sort items 4, 8, 16 */
@@ -523,7 +531,7 @@ mpsr_status mpsr_batch_handle(mpsr_batch *b, int bucket, mpsr_handle **h);
 /* Steps first .. first+count-1 of every bucket: one graph launch per step (the graph is recorded
  * on the first call and again after a launch-shape change). */
 mpsr_status mpsr_step_batch(mpsr_batch *b, int first, int count);
-/* The same steps with the same kernels launched directly, bucket by bucket (no graph). */
+/* The same steps with the same launches made directly (no graph). */
 mpsr_status mpsr_step_batch_eager(mpsr_batch *b, int first, int count);
 mpsr_status mpsr_batch_set_launch(mpsr_batch *b, int block, int sort_items);
 /* Times `steps` graph-launched steps from `first` for every launch shape (MPSR_TUNE_BLOCKS x
