@@ -595,13 +595,13 @@ __device__ inline V spmv_row(V a, int base, int lane, int width, const ColSource
     return a;
 }
 
-// Copies the three slab ranges of v that the rows of this workgroup's SELL window read (k_win_box)
+// Copies the three slab ranges of v that the rows of SELL window blk read (k_win_box)
 // into LDS and sets lb[g] = LDS offset minus first index of range g; false (nothing staged) when
 // the window holds more than wcap values.  The same for every thread of the workgroup.
 template <typename V>
-__device__ inline bool stage_window(const int* __restrict__ wbox, int wcap, const V* __restrict__ v, V* win,
+__device__ inline bool stage_window(int blk, const int* __restrict__ wbox, int wcap, const V* __restrict__ v, V* win,
                                     int& lb0, int& lb1, int& lb2) {
-    const int* o = wbox + 8 * blockIdx.x;
+    const int* o = wbox + 8 * blk;
     if (o[6] > wcap) return false;
     const int B0 = o[0], B1 = o[1], B2 = o[2], E0 = o[3], E1 = o[4], E2 = o[5];
     const int S1 = E0 - B0, S2 = S1 + (E1 - B1);
@@ -632,7 +632,7 @@ __device__ inline void pcg_spmv_body(int blk, int nslices, const int* __restrict
     // LDS offset minus first p index of each slab range (0 when gathering from global memory)
     int lb0 = 0, lb1 = 0, lb2 = 0;
     bool staged = false;
-    if constexpr (WIN) staged = stage_window(wbox, wcap, p, win, lb0, lb1, lb2);
+    if constexpr (WIN) staged = stage_window(blk, wbox, wcap, p, win, lb0, lb1, lb2);
     const int P = blk * blockDim.x + threadIdx.x;   // SELL position
     const int slice = P >> 6, lane = lane_id();
     const int t = slice < nslices ? rowp[P] : -1;
@@ -693,7 +693,7 @@ __device__ inline void resid_body(int blk, int N, int nslices, const int* __rest
     double* win = reinterpret_cast<double*>(resid_dyn);
     int lb0 = 0, lb1 = 0, lb2 = 0;
     bool staged = false;
-    if constexpr (WIN) staged = stage_window(wbox, wcap, x, win, lb0, lb1, lb2);
+    if constexpr (WIN) staged = stage_window(blk, wbox, wcap, x, win, lb0, lb1, lb2);
     const int P = blk * blockDim.x + threadIdx.x;   // SELL position
     const int slice = P >> 6, lane = lane_id();
     const int t = slice < nslices ? rowp[P] : -1;
