@@ -72,9 +72,11 @@ __device__ inline R dist3(R x, R y, R z, R qx, R qy, R qz) {
 }
 
 // for_candidates with the records loaded ahead: every candidate t != self of the 27-cell stencil in
-// the canonical order, as f(t, spos[t]).  Inside a cell the records come in groups of four loads
-// issued together, so four gathers are in flight per lane instead of one; the calls keep the order.
-template <typename R, typename F>
+// the canonical order, as f(t, spos[t]).  Inside a cell the records come in groups of G (then 4)
+// loads issued together, so G gathers are in flight per lane instead of one; the calls keep the
+// order.  G = 8 suits a kernel whose registers allow it (the PND pass: 6 % faster than 4; the
+// predictor loses occupancy with it and keeps 4).
+template <int G = 4, typename R, typename F>
 __device__ inline void for_candidates_rec(const Consts<R>& c, const int* __restrict__ cs, const int* __restrict__ ce,
                                           const R4<R>* __restrict__ spos, R x, R y, R z, int self, F&& f) {
     int ci, cj, ck;
@@ -90,6 +92,16 @@ __device__ inline void for_candidates_rec(const Consts<R>& c, const int* __restr
                 if (a < 0) continue;
                 const int b = ce[m];
                 int t = a;
+                if constexpr (G == 8) {
+                    for (; t + 8 <= b; t += 8) {
+                        R4<R> Q[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) Q[u] = spos[t + u];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (t + u != self) f(t + u, Q[u]);
+                    }
+                }
                 for (; t + 4 <= b; t += 4) {
                     const R4<R> Q0 = spos[t], Q1 = spos[t + 1], Q2 = spos[t + 2], Q3 = spos[t + 3];
                     if (t != self) f(t, Q0);

@@ -306,7 +306,7 @@ __device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int*
         const int lbase = (t >> 6) * 64 * nb_width, lane = t & 63;
         int q0 = 0, q1 = 0, q2 = 0, q3 = 0;
         R w0 = 0, w1 = 0, w2 = 0, w3 = 0;
-        for_candidates_rec(c, cs, ce, spos, x, y, z, t, [&](int q, const R4<R>& Q) {
+        for_candidates_rec<8>(c, cs, ce, spos, x, y, z, t, [&](int q, const R4<R>& Q) {
             const R d0 = dist3(x, y, z, Q.x, Q.y, Q.z);
             if (d0 < re) {
                 R d = d0;
