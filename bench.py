@@ -43,8 +43,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per SpMV launch of the mixed-precision solver on the default drum (l0 = 2.012e-3 m),
 # from two rocprofv3 counter passes (FETCH_SIZE scaled by the factor a plain copy kernel of known
-# size reads, WRITE_SIZE unscaled): profiles/k_pcg_spmv_drum1M_traffic.md
-SPMV_TRAFFIC_BYTES = {(2.012e-3, "mixed"): 958.6e6}
+# size reads, WRITE_SIZE unscaled) over the working launches: profiles/k_pcg_spmv_drum1M_traffic.md
+SPMV_TRAFFIC_BYTES = {(2.012e-3, "mixed"): 745.1e6}
 WEIGHTS = os.path.join(ROOT, "simulation_project_amd", "weights", "mpnn_drum.npy")
 
 
