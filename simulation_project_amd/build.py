@@ -76,7 +76,7 @@ def build(force=False, verbose=False):
         # link next to the library and rename over it, so a reader never sees a half-written file
         tmp = LIB + ".tmp"
         cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + \
-            [f"-L{ROCM}/lib", "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{ROCM}/lib"]
+            [f"-Wl,-rpath,{ROCM}/lib"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError("libmpsr link failed:\n" + r.stdout + r.stderr)

@@ -54,10 +54,9 @@
 #include <algorithm>
 #include "mpsr_internal.h"
 #include "neighbors.h"
+#include <utility>
 #include "ppe.h"
 #include "mpnn.h"
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 namespace mpsr {
 
@@ -175,7 +174,6 @@ template <typename R> struct PpeSolver {
     double* rc = nullptr;        // [nc_cap] restricted residual
     double* cc = nullptr;        // [nc_cap] coarse correction
     int* info_dev = nullptr;
-    rocblas_handle blas = nullptr;
     SortScratch agg_scan{};
     int two_level_ok = 0;        // this solve applies a coarse correction
     int coarse_valid = 0;        // Ac holds the inverse built for the frozen aggregate map
@@ -1047,16 +1045,205 @@ template <typename T> __global__ __launch_bounds__(PB) void k_symmetrize(int nc,
     if (i < j) A[(size_t)j * nc + i] = A[(size_t)i * nc + j];
 }
 
-// FP32 copy of the assembled coarse operator, factorised and inverted in single precision
+// ---------------------------------------------------------------------------------------------
+// Dense inverse of the SPD coarse operator in FP64: blocked in-place Gauss-Jordan without pivoting
+// (the sweep of every diagonal block in turn).  For pivot block p with D = A_pp^-1:
+//   A_pp <- D;  A_pj <- D A_pj (j != p);  A_ij <- A_ij - A_ip A_pj (i, j != p);  A_ip <- -A_ip D (i != p)
+// After the last block A holds A^-1.  A non-positive or non-finite pivot (the matrix is not SPD)
+// sets *fail.  Column-major, leading dimension n; blocks of GJ_B rows and columns.
+// ---------------------------------------------------------------------------------------------
+static constexpr int GJ_B = 64;
+static constexpr int GJ_LD = GJ_B + 1;   // LDS row pitch (odd: column reads spread over banks)
+
+// the diagonal block: scalar Gauss-Jordan by one wave, lane j holding column j in registers.  At
+// step k every lane reads lane k's column with readlane (k is a constant: the steps are
+// instantiated one by one, gj_step<K>, so every register index is a constant too); the other lanes
+// scale their pivot-row element and update their column with it, and lane k sweeps its own column.
+// A partial last block is padded with the identity, which the sweep leaves as it is.
+template <int L> __device__ __attribute__((always_inline)) inline double readlane_d(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, L);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), L);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int K>
+__device__ __attribute__((always_inline)) inline void gj_step(double (&c)[GJ_B], int j, int* fail) {
+    double ck[GJ_B];
+#pragma unroll
+    for (int i = 0; i < GJ_B; ++i) ck[i] = readlane_d<K>(c[i]);
+    const double piv = ck[K];
+    if ((!(piv > 0.0) || !isfinite(piv)) && j == 0) *fail = 1;
+    const double inv = 1.0 / piv;
+    if (j != K) {
+        const double rk = c[K] * inv;   // the pivot row, scaled
+#pragma unroll
+        for (int i = 0; i < GJ_B; ++i)
+            if (i != K) c[i] = fma(-ck[i], rk, c[i]);
+        c[K] = rk;
+    } else {
+#pragma unroll
+        for (int i = 0; i < GJ_B; ++i)
+            if (i != K) c[i] = -c[i] * inv;
+        c[K] = inv;
+    }
+}
+template <int... K>
+__device__ __attribute__((always_inline)) inline void gj_steps(double (&c)[GJ_B], int j, int* fail,
+                                                             std::integer_sequence<int, K...>) {
+    (gj_step<K>(c, j, fail), ...);
+}
+__global__ __launch_bounds__(64) void k_gj_pivot(int n, int p0, double* A, int* fail) {
+    const int bp = min(GJ_B, n - p0);
+    const int j = threadIdx.x;
+    double c[GJ_B];
+#pragma unroll
+    for (int i = 0; i < GJ_B; ++i)
+        c[i] = (i < bp && j < bp) ? A[(size_t)(p0 + j) * n + p0 + i] : (i == j ? 1.0 : 0.0);
+    gj_steps(c, j, fail, std::make_integer_sequence<int, GJ_B>{});
+#pragma unroll
+    for (int i = 0; i < GJ_B; ++i)
+        if (i < bp && j < bp) A[(size_t)(p0 + j) * n + p0 + i] = c[i];
+}
+
+// loads block (r0.., c0..) of A (rows x cols, at most GJ_B each) into t[r][c], zero-padded
+__device__ inline void gj_load(const double* A, int n, int r0, int c0, int rows, int cols, double* t) {
+    for (int e = threadIdx.x; e < GJ_B * GJ_B; e += blockDim.x) {
+        const int r = e % GJ_B, c = e / GJ_B;
+        t[r * GJ_LD + c] = (r < rows && c < cols) ? A[(size_t)(c0 + c) * n + r0 + r] : 0.0;
+    }
+}
+// each of the 256 threads forms a 4 x 4 piece of the 64 x 64 product X Y (X, Y in LDS, K = GJ_B)
+__device__ inline void gj_mul(const double* X, const double* Y, double acc[4][4]) {
+    const int tr = (threadIdx.x % 16) * 4, tc = (threadIdx.x / 16) * 4;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int k = 0; k < GJ_B; ++k) {
+        double x[4], y[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) x[a] = X[(tr + a) * GJ_LD + k];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) y[b] = Y[k * GJ_LD + tc + b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+    }
+}
+
+// row panel: A_pj <- D A_pj for every column block j != p (blockIdx.x = j)
+__global__ __launch_bounds__(256) void k_gj_row(int n, int p0, double* A) {
+    __shared__ double D[GJ_B * GJ_LD], Y[GJ_B * GJ_LD];
+    const int j0 = blockIdx.x * GJ_B;
+    if (j0 == p0) return;
+    const int bp = min(GJ_B, n - p0), bj = min(GJ_B, n - j0);
+    gj_load(A, n, p0, p0, bp, bp, D);
+    gj_load(A, n, p0, j0, bp, bj, Y);
+    __syncthreads();
+    double acc[4][4];
+    gj_mul(D, Y, acc);
+    const int tr = (threadIdx.x % 16) * 4, tc = (threadIdx.x / 16) * 4;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (tr + a < bp && tc + b < bj) A[(size_t)(j0 + tc + b) * n + p0 + tr + a] = acc[a][b];
+}
+
+// trailing update: A_ij <- A_ij - A_ip A_pj for i, j != p.  A workgroup takes 64 rows (blockIdx.x,
+// one row block) and 128 columns (blockIdx.y), each thread a 4 x 8 piece; the columns of the pivot
+// block inside a tile are left alone.  The K = GJ_B sum runs in chunks of GJ_KC staged in LDS
+// (25 KB a workgroup), in ascending k.
+static constexpr int GJ_KC = 16;
+static constexpr int GJ_TN = 128;
+__global__ __launch_bounds__(256) void k_gj_update(int n, int p0, double* A) {
+    __shared__ double X[GJ_B * (GJ_KC + 1)], Y[GJ_KC * (GJ_TN + 1)];
+    const int i0 = blockIdx.x * GJ_B, j0 = blockIdx.y * GJ_TN;
+    if (i0 == p0) return;
+    const int bp = min(GJ_B, n - p0), bi = min(GJ_B, n - i0), bj = min(GJ_TN, n - j0);
+    if (j0 >= p0 && j0 + bj <= p0 + bp) return;   // the tile is all pivot columns
+    const int tr = (threadIdx.x % 16) * 4, tc = (threadIdx.x / 16) * 8;
+    double acc[4][8];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
+    for (int k0 = 0; k0 < GJ_B; k0 += GJ_KC) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < GJ_B * GJ_KC; e += blockDim.x) {
+            const int r = e % GJ_B, c = e / GJ_B;   // X[r][c] = A_ip(r, k0 + c)
+            X[r * (GJ_KC + 1) + c] = (r < bi && k0 + c < bp) ? A[(size_t)(p0 + k0 + c) * n + i0 + r] : 0.0;
+        }
+        for (int e = threadIdx.x; e < GJ_KC * GJ_TN; e += blockDim.x) {
+            const int kr = e % GJ_KC, cc = e / GJ_KC;   // Y[kr][cc] = A_pj(k0 + kr, cc)
+            Y[kr * (GJ_TN + 1) + cc] = (k0 + kr < bp && cc < bj) ? A[(size_t)(j0 + cc) * n + p0 + k0 + kr] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < GJ_KC; ++k) {
+            double x[4], y[8];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) x[a] = X[(tr + a) * (GJ_KC + 1) + k];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) y[b] = Y[k * (GJ_TN + 1) + tc + b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 8; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const int col = j0 + tc + b;
+        if (tc + b >= bj || (col >= p0 && col < p0 + bp)) continue;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            if (tr + a < bi) {
+                double* q = A + (size_t)col * n + i0 + tr + a;
+                *q = *q - acc[a][b];
+            }
+    }
+}
+
+// column panel: A_ip <- -A_ip D for every row block i != p (blockIdx.x = i)
+__global__ __launch_bounds__(256) void k_gj_col(int n, int p0, double* A) {
+    __shared__ double X[GJ_B * GJ_LD], D[GJ_B * GJ_LD];
+    const int i0 = blockIdx.x * GJ_B;
+    if (i0 == p0) return;
+    const int bp = min(GJ_B, n - p0), bi = min(GJ_B, n - i0);
+    gj_load(A, n, i0, p0, bi, bp, X);
+    gj_load(A, n, p0, p0, bp, bp, D);
+    __syncthreads();
+    double acc[4][4];
+    gj_mul(X, D, acc);
+    const int tr = (threadIdx.x % 16) * 4, tc = (threadIdx.x / 16) * 4;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (tr + a < bi && tc + b < bp) A[(size_t)(p0 + tc + b) * n + i0 + tr + a] = -acc[a][b];
+}
+
+// A (n x n, SPD, column-major) <- A^-1 on stream st; *fail (device) is set when a pivot is not
+// positive
+static void spd_inverse(int n, double* A, int* fail, hipStream_t st) {
+    const int nb = (n + GJ_B - 1) / GJ_B;
+    for (int p = 0; p < nb; ++p) {
+        const int p0 = p * GJ_B;
+        hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(64), 0, st, n, p0, A, fail);
+        hipLaunchKernelGGL(k_gj_row, dim3(nb), dim3(256), 0, st, n, p0, A);
+        hipLaunchKernelGGL(k_gj_update, dim3(nb, (n + GJ_TN - 1) / GJ_TN), dim3(256), 0, st, n, p0, A);
+        hipLaunchKernelGGL(k_gj_col, dim3(nb), dim3(256), 0, st, n, p0, A);
+    }
+}
+
+// FP32 copy of the coarse operator's inverse, the one the mixed solver's V-cycle applies
 __global__ __launch_bounds__(PB) void k_d2f(long long n, const double* __restrict__ a, float* b) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
 }
 // FP64 copy of the symmetric FP32 inverse, for the FP64 solver paths
-__global__ __launch_bounds__(PB) void k_f2d(long long n, const float* __restrict__ a, double* b) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (double)a[i];
-}
 
 __global__ __launch_bounds__(PB) void k_fill_d(double* a, long long n, double v) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1509,10 +1696,6 @@ static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hi
             return false;
         }
     }
-    if (!s->blas) {
-        if (rocblas_create_handle(&s->blas) != rocblas_status_success) return false;
-    }
-    if (rocblas_set_stream(s->blas, st) != rocblas_status_success) return false;
     hipLaunchKernelGGL(k_agg_rows, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, g, d.skeys, s->dir, s->agg_flag,
                        s->agg_map, s->agg_of_row);
     const long long n2 = (long long)nc * nc;
@@ -1534,18 +1717,15 @@ static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hi
                        s->cpart);
     hipLaunchKernelGGL(k_coarse_gather, dim3(grid_for((long long)nc * 27, PB)), dim3(PB), 0, st, nc, g, s->agg_full,
                        s->agg_flag, s->agg_map, s->cpart, s->Ac);
-    // Cholesky factorisation and inverse in single precision: the inverse only preconditions, the
-    // coarse operator is a well-conditioned aggregated Laplacian, and FP32 halves the set-up
-    hipLaunchKernelGGL(k_d2f, dim3(grid_for(n2, PB)), dim3(PB), 0, st, n2, s->Ac, s->Af);
-    if (rocsolver_spotrf(s->blas, rocblas_fill_lower, nc, s->Af, nc, s->info_dev) != rocblas_status_success) return false;
-    if (rocsolver_spotri(s->blas, rocblas_fill_lower, nc, s->Af, nc, s->info_dev + 1) != rocblas_status_success)
-        return false;
-    int info[2] = {0, 0};
-    if (hipMemcpyAsync(info, s->info_dev, 2 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return false;
+    // the dense inverse in FP64 (blocked Gauss-Jordan), made exactly symmetric, and its FP32 copy
+    fill_int(s->info_dev, 0, 1, st);
+    spd_inverse(nc, s->Ac, s->info_dev, st);
+    int info = 0;
+    if (hipMemcpyAsync(&info, s->info_dev, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return false;
     if (hipStreamSynchronize(st) != hipSuccess) return false;
-    if (info[0] != 0 || info[1] != 0) return false;
-    hipLaunchKernelGGL(k_symmetrize<float>, dim3(grid_for(n2, PB)), dim3(PB), 0, st, nc, s->Af);
-    hipLaunchKernelGGL(k_f2d, dim3(grid_for(n2, PB)), dim3(PB), 0, st, n2, s->Af, s->Ac);
+    if (info != 0) return false;
+    hipLaunchKernelGGL(k_symmetrize<double>, dim3(grid_for(n2, PB)), dim3(PB), 0, st, nc, s->Ac);
+    hipLaunchKernelGGL(k_d2f, dim3(grid_for(n2, PB)), dim3(PB), 0, st, n2, s->Ac, s->Af);
     return true;
 }
 
@@ -1728,7 +1908,6 @@ template <typename R> void ppe_destroy(PpeSolver<R>* s) {
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : s->setup_ev)
         if (e) (void)hipEventDestroy(e);
-    if (s->blas) (void)rocblas_destroy_handle(s->blas);
     delete s;
 }
 
