@@ -738,6 +738,27 @@ __global__ __launch_bounds__(PB) void k_resid(int N, int nslices, const int* __r
                              codes, skeys, fg, gz, gygz, wbox, wcap);
 }
 
+// The first residual of a solve, launched before the host has read the fill's counts: the 16-bit
+// codes and the staged FP64 window when the fill coded every column (nnz[2] == 0, read here on the
+// device), else the 32-bit columns.  Either form adds the same products in the same order.
+template <typename VA>
+__global__ __launch_bounds__(PB) void k_resid_first(int N, int nslices, const int* __restrict__ slice_off,
+                                                     const int* __restrict__ slice_w, const int* __restrict__ cols,
+                                                     const VA* __restrict__ vals, const double* __restrict__ diag,
+                                                     const double* __restrict__ x, const double* __restrict__ b,
+                                                     const int* __restrict__ rowp, const int* __restrict__ cnt,
+                                                     double* r, double* part, const uint16_t* __restrict__ codes,
+                                                     const uint32_t* __restrict__ skeys, const int* __restrict__ fg,
+                                                     int gz, int gygz, const int* __restrict__ wbox, int wcap,
+                                                     const int* __restrict__ nnz) {
+    if (nnz[2] == 0)
+        resid_body<VA, true, true>(blockIdx.x, N, nslices, slice_off, slice_w, cols, vals, diag, x, b, rowp, cnt, r,
+                                   part, codes, skeys, fg, gz, gygz, wbox, wcap);
+    else
+        resid_body<VA, false, false>(blockIdx.x, N, nslices, slice_off, slice_w, cols, vals, diag, x, b, rowp, cnt, r,
+                                     part, nullptr, nullptr, nullptr, 0, 0, nullptr, 0);
+}
+
 // x += alpha p; r -= alpha q; z = r / diag; partial r.z and r.r
 template <typename V>
 __device__ inline void pcg_update_body(int blk, int N, V* x, V* r, V* z, const V* __restrict__ p,
@@ -1983,10 +2004,18 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
     return done_iters;
 }
 
-// r = b - A x: with the 32-bit columns until the solve knows that every column has a 16-bit code
-// (the first residual of a solve), then with the codes and, for the FP32 matrix, the staged window
+// r = b - A x: the first residual of a solve decides between the 16-bit codes and the 32-bit
+// columns on the device (the FP32 matrix) or takes the 32-bit columns; later ones use the codes and,
+// for the FP32 matrix, the staged window when the solve found every column coded
 template <typename R, typename VA>
-static void launch_resid(PpeSolver<R>* s, int N, int nb, const VA* vals, hipStream_t st) {
+static void launch_resid(PpeSolver<R>* s, int N, int nb, const VA* vals, hipStream_t st, bool first) {
+    if (first && s->mixed && sizeof(VA) == 4 && !c16_off() && !win_off()) {
+        hipLaunchKernelGGL(k_resid_first<VA>, dim3(nb), dim3(PB), (size_t)RESID_WIN_BYTES, st, N, s->nslices,
+                           s->slice_off, s->slice_w, s->cols, vals, s->diag, s->x, s->rhs, s->rowp, s->cnt, s->r,
+                           s->part, s->cols16, s->skeys, s->first_ge, s->gz, s->gygz, s->wbox,
+                           RESID_WIN_BYTES / (int)sizeof(double), s->nnz_dev);
+        return;
+    }
     const int wcap = s->wcap > 0 ? std::min(s->wcap, RESID_WIN_BYTES / (int)sizeof(double)) : 0;
     if (s->c16 && wcap > 0 && sizeof(VA) == 4)
         hipLaunchKernelGGL((k_resid<VA, true, true>), dim3(nb), dim3(PB), sizeof(double) * (size_t)wcap, st, N,
@@ -2003,13 +2032,14 @@ static void launch_resid(PpeSolver<R>* s, int N, int nb, const VA* vals, hipStre
                            (const int*)nullptr, 0);
 }
 
-template <typename R> static double resid_pass(PpeSolver<R>* s, int N, hipStream_t st, double* bb_out) {
+template <typename R>
+static double resid_pass(PpeSolver<R>* s, int N, hipStream_t st, double* bb_out, bool first = false) {
     const int rows = s->nslices * 64;
     const int nb = (rows + PB - 1) / PB;
     if (s->mixed)
-        launch_resid(s, N, nb, s->vals32, st);
+        launch_resid(s, N, nb, s->vals32, st, first);
     else
-        launch_resid(s, N, nb, s->vals64, st);
+        launch_resid(s, N, nb, s->vals64, st, first);
     hipLaunchKernelGGL(k_pcg_reduce, dim3(1), dim3(1024), 0, st, s->part, nb, 2, s->scal, (int)RED_RESID);
     if (read_scal(s, st) != hipSuccess) return -1.0;
     *bb_out = s->scal_h[S_BB];
@@ -2154,7 +2184,7 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
         hipLaunchKernelGGL(k_warm, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, d.perm, s->dir, s->prev, use_prev, s->x);
     }
     double bb = 0.0;
-    double rr = resid_pass(s, N, st, &bb);
+    double rr = resid_pass(s, N, st, &bb, true);
     if (rr < 0) return MPSR_GPUERR;
     s->nnz_real = s->cnt_h[0];
     s->rows_real = s->cnt_h[1];

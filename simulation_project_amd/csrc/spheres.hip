@@ -13,7 +13,8 @@
 //   k_sph_move     per sphere: v += dt (F / m_s + (1 - rho_f / rho_s) g), x += dt v; a centre
 //                  closer than d/2 to the nearest non-drag wall is pushed out along the wall
 //                  normal and loses its inward normal velocity
-// The predictor adds dv to the fluid velocity.  One thread per sphere (2,000) or per particle.
+// The predictor adds dv to the fluid velocity.  One wave per sphere (sampling and moving) or one
+// thread per particle (the reaction).
 #include "mpsr_internal.h"
 #include "neighbors.h"
 #include "walls_geom.h"
@@ -157,13 +158,19 @@ __global__ __launch_bounds__(256) void k_sph_react(int N, Consts<R> c, const uin
     dv[t] = mk4<R>(ax, ay, az, R(0));
 }
 
+// One wave per sphere.  The nearest non-drag wall of the moved centre is nearest_triangle's: the
+// lanes take the triangles of each stencil cell's list in turn (lane L the L-th, (L+64)-th, ...),
+// each keeps its first strictly nearest, and the wave keeps the smallest distance at the smallest
+// position in the traversal: the triangle the sequential walk keeps (it replaces its best only when
+// strictly nearer, and a listing repeated in another cell is never strictly nearer).
 template <typename R>
 __global__ __launch_bounds__(256) void k_sph_move(int S, Consts<R> c, R4<R>* pos, R4<R>* vel,
                                                    const R4<R>* __restrict__ force, const int* __restrict__ tstart,
                                                    const int* __restrict__ tcount, const int* __restrict__ tids,
                                                    const TriD<R>* __restrict__ tris, R d, R rho_s, R rho_f) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= S) return;
+    const int k = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = lane_id();
+    if (k >= S) return;   // the same for the whole wave
     const R ms = rho_s * (R)M_PI * d * d * d / (R)6;
     const R inv = (R)1 / ms;
     const R buoy = (R)1 - rho_f / rho_s;
@@ -172,9 +179,48 @@ __global__ __launch_bounds__(256) void k_sph_move(int S, Consts<R> c, R4<R>* pos
     const R4<R> V0 = vel[k], X0 = pos[k];
     R vx = V0.x + ax * c.dt, vy = V0.y + ay * c.dt, vz = V0.z + az * c.dt;
     R x = X0.x + vx * c.dt, y = X0.y + vy * c.dt, z = X0.z + vz * c.dt;
-    R best;
-    P3<R> dir;
-    nearest_triangle(c, x, y, z, tstart, tcount, tids, tris, 0, &best, &dir);
+    // this lane's first strictly nearest triangle, and its position in the traversal
+    R best = (R)1e8;
+    P3<R> dir = p3<R>(0, 0, 0);
+    int at = 0x7fffffff;
+    int ci, cj, ck;
+    if (cell_coords(c, x, y, z, ci, cj, ck)) {
+        const long long cid = cell_of(c, ci, cj, ck);
+        if (cid >= 0 && cid < c.ncell) {
+            const P3<R> p = p3(x, y, z);
+            int seen = 0;   // traversal positions before this cell
+            for (int kk = -1; kk <= 1; ++kk)
+                for (int jj = -1; jj <= 1; ++jj)
+                    for (int ii = -1; ii <= 1; ++ii) {
+                        const long long m = cell_of(c, ci + ii, cj + jj, ck + kk);
+                        if (m < 0 || m >= c.ncell) continue;
+                        const int n = tcount[m], a0 = tstart[m];
+                        for (int e = lane; e < n; e += 64) {
+                            const TriD<R>& tr = tris[tids[a0 + e]];
+                            if (tr.is_drag != 0) continue;
+                            P3<R> dv;
+                            const R dm = tri_distance(tr, p, &dv);
+                            if (dm < best) {
+                                best = dm;
+                                dir = dv;
+                                at = seen + e;
+                            }
+                        }
+                        seen += n;
+                    }
+        }
+    }
+    // the wave's smallest distance, ties to the earliest position
+    for (int o = 32; o > 0; o >>= 1) {
+        const R ob = __shfl_xor(best, o, 64);
+        const int oa = __shfl_xor(at, o, 64);
+        const R odx = __shfl_xor(dir.x, o, 64), ody = __shfl_xor(dir.y, o, 64), odz = __shfl_xor(dir.z, o, 64);
+        if (ob < best || (ob == best && oa < at)) {
+            best = ob;
+            at = oa;
+            dir = p3<R>(odx, ody, odz);
+        }
+    }
     const R half = d / (R)2;
     if (best < half && best > 0) {
         const R nx = -dir.x / best, ny = -dir.y / best, nz = -dir.z / best;
@@ -189,8 +235,10 @@ __global__ __launch_bounds__(256) void k_sph_move(int S, Consts<R> c, R4<R>* pos
             vz = vz - nz * vn;
         }
     }
-    pos[k] = mk4<R>(x, y, z, R(0));
-    vel[k] = mk4<R>(vx, vy, vz, R(0));
+    if (lane == 0) {
+        pos[k] = mk4<R>(x, y, z, R(0));
+        vel[k] = mk4<R>(vx, vy, vz, R(0));
+    }
 }
 
 template <typename R> void launch_spheres(Dev<R>& d, SphDev<R>& s, hipStream_t st) {
@@ -209,7 +257,8 @@ template <typename R> void launch_spheres(Dev<R>& d, SphDev<R>& s, hipStream_t s
     if (d.N > 0)
         hipLaunchKernelGGL(k_sph_react<R>, dim3(grid_for(d.N, 256)), b, 0, st, d.N, d.c, d.skeys, d.spos, s.pos, s.perm,
                            s.cs, s.ce, s.force, s.wsum, s.mf, d.sph_dv);
-    hipLaunchKernelGGL(k_sph_move<R>, gs, b, 0, st, s.S, d.c, s.pos, s.vel, s.force, d.cell_tri_start,
+    hipLaunchKernelGGL(k_sph_move<R>, dim3(grid_for((long long)s.S * 64, 256)), b, 0, st, s.S, d.c, s.pos, s.vel,
+                       s.force, d.cell_tri_start,
                        d.cell_tri_count, d.uniq_tri, d.tris, s.d, s.rho_s, s.rho_f);
 }
 
