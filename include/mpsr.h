@@ -455,6 +455,11 @@ mpsr_status mpsr_get_ppe(mpsr_handle *h, double *rhs, double *solution, int *dir
  * (capacity cap), diagonal [n]; *nnz receives the off-diagonal count. */
 mpsr_status mpsr_get_ppe_matrix(mpsr_handle *h, int *row_ptr, int *col, double *val, double *diag,
                                 int cap, int *nnz);
+/* The inverse of a dense symmetric positive definite matrix (n x n, 1 <= n <= 16384, host
+ * arrays, column-major) by the blocked FP64 Gauss-Jordan sweep that inverts the V-cycle's coarse
+ * operator.  No counterpart in the reference (its solver has no PPE); it exposes that sweep for
+ * checking.  MPSR_CALCERR when a pivot is not positive (the matrix is not SPD). */
+mpsr_status mpsr_spd_inverse(int n, const double *a, double *inv);
 /* Per-phase HIP-event timing of the following steps (mpsr_get_timings); off by default. */
 mpsr_status mpsr_set_profiling(mpsr_handle *h, int on);
 

@@ -78,6 +78,7 @@ def load_library(path=None):
         "mpsr_get_ppe_matrix": (C.c_int, [vp, P(C.c_int), P(C.c_int), P(C.c_double), P(C.c_double), C.c_int,
                                           P(C.c_int)]),
         "mpsr_get_particle_count": (C.c_int, [vp, P(C.c_int)]),
+        "mpsr_spd_inverse": (C.c_int, [C.c_int, P(C.c_double), P(C.c_double)]),
         "mpsr_set_buckets": (C.c_int, [vp, P(abi.ParticlesC), P(C.c_int), P(C.c_int), C.c_int]),
         "mpsr_bucket_addition": (C.c_int, [vp, C.c_int, P(C.c_int)]),
         "mpsr_set_spheres": (C.c_int, [vp, P(abi.SpheresC)]),
@@ -137,6 +138,21 @@ def load_dt_schedule(path):
     if st != 0:
         raise MpsrError(st, "mpsr_dt_schedule_load", lib.mpsr_deck_last_error().decode(errors="replace"))
     return [(e.step, e.dt, e.courant, e.inflow_step, e.output_step) for e in arr[:n.value]]
+
+
+def spd_inverse(a):
+    """The inverse of a dense SPD matrix by the GPU's blocked FP64 Gauss-Jordan sweep (the one that
+    inverts the V-cycle's coarse operator); raises MpsrError when a pivot is not positive."""
+    import numpy as np
+    a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+    n = a.shape[0]
+    if a.shape != (n, n):
+        raise ValueError("spd_inverse needs a square matrix")
+    inv = np.zeros((n, n), dtype=np.float64, order="F")
+    st = load_library().mpsr_spd_inverse(n, _dp(a), _dp(inv))
+    if st != 0:
+        raise MpsrError(st, "mpsr_spd_inverse", "not positive definite" if st == 3 else "device error")
+    return inv
 
 
 def _ip(a):

@@ -114,4 +114,42 @@ __device__ inline void for_candidates_rec(const Consts<R>& c, const int* __restr
             }
 }
 
+// for_candidates_rec in groups: g(t0, Q, m) for runs of up to 4 consecutive candidates t0 ..
+// t0 + m - 1 of one stencil cell, in the canonical order, with Q[u] = spos[t0 + u] (Q[u] for u >= m
+// repeats the last record and is to be ignored; self is not skipped: the caller tests it).  A
+// caller that reads a second record of its neighbours loads them for the whole group before it
+// uses them, so one wait covers the group instead of one per neighbour.
+template <typename R, typename F>
+__device__ inline void for_candidate_groups(const Consts<R>& c, const int* __restrict__ cs,
+                                            const int* __restrict__ ce, const R4<R>* __restrict__ spos, R x, R y,
+                                            R z, F&& g) {
+    int ci, cj, ck;
+    if (!cell_coords(c, x, y, z, ci, cj, ck)) return;
+    const long long cid = cell_of(c, ci, cj, ck);
+    if (cid < 0 || cid >= c.ncell) return;
+    for (int k = -1; k <= 1; ++k)
+        for (int j = -1; j <= 1; ++j)
+            for (int i = -1; i <= 1; ++i) {
+                const long long mcell = cell_of(c, ci + i, cj + j, ck + k);
+                if (mcell < 0 || mcell >= c.ncell) continue;
+                const int a = cs[mcell];
+                if (a < 0) continue;
+                const int b = ce[mcell];
+                int t = a;
+                for (; t + 4 <= b; t += 4) {
+                    R4<R> Q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) Q[u] = spos[t + u];
+                    g(t, Q, 4);
+                }
+                if (t < b) {
+                    const int m = b - t;
+                    R4<R> Q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) Q[u] = spos[t + (u < m ? u : m - 1)];
+                    g(t, Q, m);
+                }
+            }
+}
+
 }  // namespace mpsr

@@ -121,25 +121,40 @@ __device__ inline void explicit_body(int blk, int N, Consts<R> c, const int* __r
     const R4<R> V0 = svel[t];
     R lx = 0, ly = 0, lz = 0, ax = 0, ay = 0, az = 0, qx = 0, qy = 0, qz = 0;
     const R re = c.re_lap, mind = c.mind_lap;
-    for_candidates_rec(c, cs, ce, spos, x, y, z, t, [&](int q, const R4<R>& Q) {
-        R d = dist3(x, y, z, Q.x, Q.y, Q.z);
-        if (d < re) {
+    // per group of 4 candidates: the distances first, then the velocities of the hits, loaded
+    // together, then the sums in the canonical order
+    for_candidate_groups(c, cs, ce, spos, x, y, z, [&](int t0, const R4<R> (&Q)[4], int m) {
+        R dd[4];
+        bool hit[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            dd[u] = dist3(x, y, z, Q[u].x, Q[u].y, Q[u].z);
+            hit[u] = u < m && t0 + u != t && dd[u] < re;
+        }
+        R4<R> V[4];   // read only where hit[u]
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (hit[u]) V[u] = svel[t0 + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (!hit[u]) continue;
+            R d = dd[u];
             if (d < mind) d = mind;
             const R w = weight(re, d);
-            const R4<R> V1 = svel[q];
+            const R4<R>& V1 = V[u];
             lx += (V1.x - V0.x) * w;
             ly += (V1.y - V0.y) * w;
             lz += (V1.z - V0.z) * w;
             if (c.expl) {
                 const R a = c.art_p * w / (d * d);
-                ax = ax + (Q.x - x) * a;
-                ay = ay + (Q.y - y) * a;
-                az = az + (Q.z - z) * a;
+                ax = ax + (Q[u].x - x) * a;
+                ay = ay + (Q[u].y - y) * a;
+                az = az + (Q[u].z - z) * a;
                 if (d < c.rc) {
                     const R cc = (c.rc - d) / d;
-                    qx = qx + (Q.x - x) * cc;
-                    qy = qy + (Q.y - y) * cc;
-                    qz = qz + (Q.z - z) * cc;
+                    qx = qx + (Q[u].x - x) * cc;
+                    qy = qy + (Q[u].y - y) * cc;
+                    qz = qz + (Q[u].z - z) * cc;
                 }
             }
         }
@@ -414,13 +429,12 @@ __device__ inline void gradient_body(int blk, int N, Consts<R> c, const int* __r
     R gx = 0, gy = 0, gz = 0, wx = 0, wy = 0, wz = 0;
     const R pi = in.p[s];
     const R re = c.re_grad, mind = c.mind_grad;
-    auto body = [&](int q) {
-        const R4<R> Q = spos[q];
+    auto term = [&](const R4<R>& Q, R pq) {
         R d = dist3(x, y, z, Q.x, Q.y, Q.z);
         if (d < re) {
             if (d < mind) d = mind;
             const R w = weight(re, d);
-            const R pc = (R)(0.5 * (double)(spres[q] + pi) * (double)w / (double)(d * d));
+            const R pc = (R)(0.5 * (double)(pq + pi) * (double)w / (double)(d * d));
             gx = gx + (Q.x - x) * pc;
             gy = gy + (Q.y - y) * pc;
             gz = gz + (Q.z - z) * pc;
@@ -429,16 +443,32 @@ __device__ inline void gradient_body(int blk, int N, Consts<R> c, const int* __r
     const int nn = nb.n ? nb.n[t] : -1;
     if (nn >= 0 && nn <= nb.width) {
         // every Laplacian-radius neighbour is listed, in the stencil walk's order; the gradient
-        // radius is the smaller one, so the same distances pick the same neighbours in the same order
+        // radius is the smaller one, so the same distances pick the same neighbours in the same order.
+        // A group's records (position and pressure of its 4 entries; the last group is padded with
+        // a valid index) are loaded together with the next group's indices and waited for once.
         const int lbase = (t >> 6) * 64 * nb.width, lane = t & 63;
+        int q4[4] = {t, t, t, t};
+        if (nn > 0) quad_load(nb.q + sell_at(lbase, lane, 0), q4);
         for (int j = 0; j < nn; j += 4) {
-            int q4[4];
-            quad_load(nb.q + sell_at(lbase, lane, j), q4);
+            R4<R> Q[4];
+            R pq[4];
+            int n4[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Q[u] = spos[q4[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pq[u] = spres[q4[u]];
+            quad_load(nb.q + sell_at(lbase, lane, j + 4 < nn ? j + 4 : j), n4);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) asm volatile("" ::"v"(Q[u].x), "v"(Q[u].y), "v"(Q[u].z), "v"(pq[u]));
             const int m = nn - j < 4 ? nn - j : 4;
-            for (int u = 0; u < m; ++u) body(q4[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (u < m) term(Q[u], pq[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q4[u] = n4[u];
         }
     } else {
-        for_candidates_within(c, cs, ce, x, y, z, t, re, body);
+        for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) { term(spos[q], spres[q]); });
     }
     gx *= c.c1; gy *= c.c1; gz *= c.c1;
     const int tid = stl_id[t];

@@ -390,16 +390,34 @@ __device__ inline void ppe_fill_body(int blk, int N, int nslices, Consts<R> c, c
         const int bound = cnt[t];   // Laplacian-radius neighbours, from the PND pass
         if (bound <= nb_width) {
             // the PND pass listed them with their weights, in the canonical order
+            // (a group's 4 active flags are loaded together with the next group's entries and waited
+            // for once; the last group is padded with a valid index)
             const int lbase = (t >> 6) * 64 * nb_width, ll = t & 63;
+            int q[4] = {t, t, t, t};
+            R w[4] = {0, 0, 0, 0};
+            if (bound > 0) {
+                quad_load(nbq + sell_at(lbase, ll, 0), q);
+                quad_load(nbw + sell_at(lbase, ll, 0), w);
+            }
             for (int j = 0; j < bound; j += 4) {
-                int q[4];
-                R w[4];
-                const int e = sell_at(lbase, ll, j);
-                quad_load(nbq + e, q);
-                quad_load(nbw + e, w);
+                int act[4], nq[4];
+                R nw[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) act[u] = active[q[u]];
+                const int e = sell_at(lbase, ll, j + 4 < bound ? j + 4 : j);
+                quad_load(nbq + e, nq);
+                quad_load(nbw + e, nw);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) asm volatile("" ::"v"(act[u]));
                 const int m = bound - j < 4 ? bound - j : 4;
-                for (int u = 0; u < m; ++u)
-                    if (active[q[u]] && k < width) push(q[u], (VA)(-(double)w[u]));
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (u < m && act[u] && k < width) push(q[u], (VA)(-(double)w[u]));
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    q[u] = nq[u];
+                    w[u] = nw[u];
+                }
             }
         } else {
             const R4<R> X = spos[t];
@@ -1076,54 +1094,75 @@ template <typename T> __global__ __launch_bounds__(PB) void k_symmetrize(int nc,
 static constexpr int GJ_B = 64;
 static constexpr int GJ_LD = GJ_B + 1;   // LDS row pitch (odd: column reads spread over banks)
 
-// the diagonal block: scalar Gauss-Jordan by one wave, lane j holding column j in registers.  At
-// step k every lane reads lane k's column with readlane (k is a constant: the steps are
-// instantiated one by one, gj_step<K>, so every register index is a constant too); the other lanes
-// scale their pivot-row element and update their column with it, and lane k sweeps its own column.
+// the diagonal block: scalar Gauss-Jordan by one workgroup of 256 threads, each holding a 4 x 4
+// piece of the 64 x 64 block in registers (rows 4 (t % 16) .., columns 4 (t / 16) ..).  Step K
+// needs the block's column K and row K as they stand after step K - 1: their owners leave them in
+// LDS (two buffers, alternating), so one barrier separates the steps.  With p the pivot and inv =
+// 1 / p, every element takes the same operations as in a column-by-column sweep:
+//   M_Kj <- M_Kj inv, M_ij <- M_ij - M_iK (M_Kj inv) (fma), M_iK <- -M_iK inv, M_KK <- inv.
 // A partial last block is padded with the identity, which the sweep leaves as it is.
-template <int L> __device__ __attribute__((always_inline)) inline double readlane_d(double v) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, L);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), L);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-template <int K>
-__device__ __attribute__((always_inline)) inline void gj_step(double (&c)[GJ_B], int j, int* fail) {
-    double ck[GJ_B];
-#pragma unroll
-    for (int i = 0; i < GJ_B; ++i) ck[i] = readlane_d<K>(c[i]);
-    const double piv = ck[K];
-    if ((!(piv > 0.0) || !isfinite(piv)) && j == 0) *fail = 1;
-    const double inv = 1.0 / piv;
-    if (j != K) {
-        const double rk = c[K] * inv;   // the pivot row, scaled
-#pragma unroll
-        for (int i = 0; i < GJ_B; ++i)
-            if (i != K) c[i] = fma(-ck[i], rk, c[i]);
-        c[K] = rk;
-    } else {
-#pragma unroll
-        for (int i = 0; i < GJ_B; ++i)
-            if (i != K) c[i] = -c[i] * inv;
-        c[K] = inv;
-    }
-}
-template <int... K>
-__device__ __attribute__((always_inline)) inline void gj_steps(double (&c)[GJ_B], int j, int* fail,
-                                                             std::integer_sequence<int, K...>) {
-    (gj_step<K>(c, j, fail), ...);
-}
-__global__ __launch_bounds__(64) void k_gj_pivot(int n, int p0, double* A, int* fail) {
+__global__ __launch_bounds__(256) void k_gj_pivot(int n, int p0, double* A, int* fail) {
+    __shared__ double colk[2][GJ_B], rowk[2][GJ_B];
     const int bp = min(GJ_B, n - p0);
-    const int j = threadIdx.x;
-    double c[GJ_B];
+    const int r0 = (threadIdx.x % 16) * 4, c0 = (threadIdx.x / 16) * 4;
+    double m[4][4];
 #pragma unroll
-    for (int i = 0; i < GJ_B; ++i)
-        c[i] = (i < bp && j < bp) ? A[(size_t)(p0 + j) * n + p0 + i] : (i == j ? 1.0 : 0.0);
-    gj_steps(c, j, fail, std::make_integer_sequence<int, GJ_B>{});
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int i = 0; i < GJ_B; ++i)
-        if (i < bp && j < bp) A[(size_t)(p0 + j) * n + p0 + i] = c[i];
+        for (int b = 0; b < 4; ++b) {
+            const int i = r0 + a, j = c0 + b;
+            m[a][b] = (i < bp && j < bp) ? A[(size_t)(p0 + j) * n + p0 + i] : (i == j ? 1.0 : 0.0);
+        }
+    // column 0 and row 0 for the first step
+    if (c0 == 0)
+#pragma unroll
+        for (int a = 0; a < 4; ++a) colk[0][r0 + a] = m[a][0];
+    if (r0 == 0)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) rowk[0][c0 + b] = m[0][b];
+    __syncthreads();
+    for (int K = 0; K < GJ_B; ++K) {
+        const int buf = K & 1;
+        const double piv = rowk[buf][K];
+        if (threadIdx.x == 0 && (!(piv > 0.0) || !isfinite(piv))) *fail = 1;
+        const double inv = 1.0 / piv;
+        double ck[4], rk[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) ck[a] = colk[buf][r0 + a];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) rk[b] = rowk[buf][c0 + b] * inv;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int i = r0 + a, j = c0 + b;
+                if (j != K) m[a][b] = i != K ? fma(-ck[a], rk[b], m[a][b]) : rk[b];
+                else m[a][b] = i != K ? -m[a][b] * inv : inv;
+            }
+        if (K + 1 < GJ_B) {
+            const int nb = buf ^ 1, k1 = K + 1;
+            if (k1 >= c0 && k1 < c0 + 4)
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        if (c0 + b == k1) colk[nb][r0 + a] = m[a][b];
+            if (k1 >= r0 && k1 < r0 + 4)
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        if (r0 + a == k1) rowk[nb][c0 + b] = m[a][b];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = r0 + a, j = c0 + b;
+            if (i < bp && j < bp) A[(size_t)(p0 + j) * n + p0 + i] = m[a][b];
+        }
 }
 
 // loads block (r0.., c0..) of A (rows x cols, at most GJ_B each) into t[r][c], zero-padded
@@ -1172,57 +1211,64 @@ __global__ __launch_bounds__(256) void k_gj_row(int n, int p0, double* A) {
             if (tr + a < bp && tc + b < bj) A[(size_t)(j0 + tc + b) * n + p0 + tr + a] = acc[a][b];
 }
 
-// trailing update: A_ij <- A_ij - A_ip A_pj for i, j != p.  A workgroup takes 64 rows (blockIdx.x,
-// one row block) and 128 columns (blockIdx.y), each thread a 4 x 8 piece; the columns of the pivot
-// block inside a tile are left alone.  The K = GJ_B sum runs in chunks of GJ_KC staged in LDS
-// (25 KB a workgroup), in ascending k.
-static constexpr int GJ_KC = 16;
+// trailing update: A_ij <- A_ij - A_ip A_pj for i, j != p, on the FP64 matrix cores
+// (v_mfma_f64_16x16x4: a wave multiplies a 16 x 4 piece of A_ip by a 4 x 16 piece of A_pj into a
+// 16 x 16 piece of the product; lane l brings A_ip(l % 16, l / 16) and A_pj(l / 16, l % 16) and
+// holds rows l / 16 + 4 v, v = 0 .. 3, of column l % 16 of the result).  A workgroup takes 64 rows
+// (blockIdx.x, one row block) and 128 columns (blockIdx.y); each of its 4 waves takes 32 of the
+// columns, as 4 x 2 pieces of 16 x 16, over K = GJ_B in steps of 4.  The pieces are read straight
+// from memory (A is column-major: the A_ip reads of a step are 4 runs of 16 rows, the A_pj reads
+// 16 runs of 4).  The columns of the pivot block inside a tile are left alone.
 static constexpr int GJ_TN = 128;
+typedef double gj_d4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_gj_update(int n, int p0, double* A) {
-    __shared__ double X[GJ_B * (GJ_KC + 1)], Y[GJ_KC * (GJ_TN + 1)];
     const int i0 = blockIdx.x * GJ_B, j0 = blockIdx.y * GJ_TN;
     if (i0 == p0) return;
-    const int bp = min(GJ_B, n - p0), bi = min(GJ_B, n - i0), bj = min(GJ_TN, n - j0);
-    if (j0 >= p0 && j0 + bj <= p0 + bp) return;   // the tile is all pivot columns
-    const int tr = (threadIdx.x % 16) * 4, tc = (threadIdx.x / 16) * 8;
-    double acc[4][8];
+    const int bp = min(GJ_B, n - p0);
+    if (j0 >= p0 && min(j0 + GJ_TN, n) <= p0 + bp) return;   // the tile is all pivot columns
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int jw = j0 + wave * 32;
+    gj_d4 acc[4][2];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-        for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
-    for (int k0 = 0; k0 < GJ_B; k0 += GJ_KC) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < GJ_B * GJ_KC; e += blockDim.x) {
-            const int r = e % GJ_B, c = e / GJ_B;   // X[r][c] = A_ip(r, k0 + c)
-            X[r * (GJ_KC + 1) + c] = (r < bi && k0 + c < bp) ? A[(size_t)(p0 + k0 + c) * n + i0 + r] : 0.0;
-        }
-        for (int e = threadIdx.x; e < GJ_KC * GJ_TN; e += blockDim.x) {
-            const int kr = e % GJ_KC, cc = e / GJ_KC;   // Y[kr][cc] = A_pj(k0 + kr, cc)
-            Y[kr * (GJ_TN + 1) + cc] = (k0 + kr < bp && cc < bj) ? A[(size_t)(j0 + cc) * n + p0 + k0 + kr] : 0.0;
-        }
-        __syncthreads();
+        for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = gj_d4{0.0, 0.0, 0.0, 0.0};
+    bool rok[4], cok[2];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) rok[rb] = i0 + rb * 16 + lr < n;
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) cok[cb] = jw + cb * 16 + lr < n;
 #pragma unroll 4
-        for (int k = 0; k < GJ_KC; ++k) {
-            double x[4], y[8];
+    for (int k = 0; k < GJ_B; k += 4) {
+        const int kk = k + lk;
+        const bool kok = kk < bp;
+        double x[4], y[2];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) x[a] = X[(tr + a) * (GJ_KC + 1) + k];
+        for (int rb = 0; rb < 4; ++rb)
+            x[rb] = (kok && rok[rb]) ? A[(size_t)(p0 + kk) * n + i0 + rb * 16 + lr] : 0.0;
 #pragma unroll
-            for (int b = 0; b < 8; ++b) y[b] = Y[k * (GJ_TN + 1) + tc + b];
+        for (int cb = 0; cb < 2; ++cb)
+            y[cb] = (kok && cok[cb]) ? A[(size_t)(jw + cb * 16 + lr) * n + p0 + kk] : 0.0;
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+        for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-                for (int b = 0; b < 8; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
-        }
+            for (int cb = 0; cb < 2; ++cb)
+                acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[rb], y[cb], acc[rb][cb], 0, 0, 0);
     }
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        const int col = j0 + tc + b;
-        if (tc + b >= bj || (col >= p0 && col < p0 + bp)) continue;
+    for (int cb = 0; cb < 2; ++cb) {
+        const int col = jw + cb * 16 + lr;
+        if (col >= n || (col >= p0 && col < p0 + bp)) continue;
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-            if (tr + a < bi) {
-                double* q = A + (size_t)col * n + i0 + tr + a;
-                *q = *q - acc[a][b];
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = i0 + rb * 16 + lk + 4 * v;
+                if (row < n) {
+                    double* q = A + (size_t)col * n + row;
+                    *q = *q - acc[rb][cb][v];
+                }
             }
     }
 }
@@ -1252,11 +1298,43 @@ static void spd_inverse(int n, double* A, int* fail, hipStream_t st) {
     const int nb = (n + GJ_B - 1) / GJ_B;
     for (int p = 0; p < nb; ++p) {
         const int p0 = p * GJ_B;
-        hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(64), 0, st, n, p0, A, fail);
+        hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, st, n, p0, A, fail);
         hipLaunchKernelGGL(k_gj_row, dim3(nb), dim3(256), 0, st, n, p0, A);
         hipLaunchKernelGGL(k_gj_update, dim3(nb, (n + GJ_TN - 1) / GJ_TN), dim3(256), 0, st, n, p0, A);
         hipLaunchKernelGGL(k_gj_col, dim3(nb), dim3(256), 0, st, n, p0, A);
     }
+}
+
+// the inverse of one SPD matrix given in host memory, on a stream of its own (the C entry point
+// mpsr_spd_inverse below)
+static int spd_inverse_host(int n, const double* a, double* inv) {
+    double* d = nullptr;
+    int* fail = nullptr;
+    hipStream_t st = nullptr;
+    int rc = 0;
+    const size_t bytes = (size_t)n * n * sizeof(double);
+    if (hipMalloc(&d, bytes) != hipSuccess || hipMalloc(&fail, sizeof(int)) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        rc = 2;
+    } else {
+        int h = 0;
+        if (hipMemcpyAsync(d, a, bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(fail, &h, sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess) {
+            rc = 3;
+        } else {
+            spd_inverse(n, d, fail, st);
+            if (hipMemcpyAsync(inv, d, bytes, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(&h, fail, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                rc = 3;
+            else if (h)
+                rc = 1;
+        }
+    }
+    if (st) (void)hipStreamDestroy(st);
+    if (fail) (void)hipFree(fail);
+    if (d) (void)hipFree(d);
+    return rc;
 }
 
 // FP32 copy of the coarse operator's inverse, the one the mixed solver's V-cycle applies
@@ -2561,6 +2639,16 @@ mpsr_status ppe_get_matrix(PpeSolver<R>* s, const Dev<R>& d, int* ptr, int* col,
 }
 
 }  // namespace mpsr
+
+extern "C" mpsr_status mpsr_spd_inverse(int n, const double* a, double* inv) {
+    if (n <= 0 || n > 16384 || !a || !inv) return MPSR_PARAMERR;
+    switch (mpsr::spd_inverse_host(n, a, inv)) {
+        case 0: return MPSR_NOERR;
+        case 1: return MPSR_CALCERR;
+        case 2: return MPSR_MEMERR;
+        default: return MPSR_GPUERR;
+    }
+}
 
 namespace mpsr {
 // ---------------------------------------------------------------------------------------------
