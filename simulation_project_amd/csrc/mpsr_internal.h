@@ -427,7 +427,7 @@ template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st);
 // Dirichlet flag from the same neighbour pass, and its first nb_width Laplacian-radius neighbours
 // and weights (SELL-64/4 by sorted index, row width nb_width)
 template <typename R>
-void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, int* nbq, R* nbw, int* nbn,
+void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, int* nbq, int* nbn,
                     int nb_width, hipStream_t st);
 // The Laplacian-radius neighbour list the PPE path's PND pass writes after the second binning:
 // per sorted row its first `width` neighbours in the canonical order (SELL-64/4 by sorted index)
@@ -509,7 +509,6 @@ template <typename R> struct PndRows {
     double* diag;
     int* dir;
     int* nbq;
-    R* nbw;
     int* nbn;
     int nb_width;
 };

@@ -272,8 +272,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_relocate(int N, Consts<R> c, PSta
 // entries: the Dirichlet ones are dropped when the row is filled) and sums their weights in FP64
 // for the diagonal.  A row that is not active or has no neighbour is a Dirichlet row (p = 0,
 // diagonal 1).  The PND sum sees the gradient-radius neighbours in the same order as without PPE,
-// so both give the same bits.  The first nbw_width Laplacian-radius neighbours and their weights go
-// to a per-row list (SELL-64/4 by sorted index) that the fill reads instead of scanning again.
+// so both give the same bits.  The sorted indices of the first nb_width Laplacian-radius neighbours
+// go to a per-row list (SELL-64/4 by sorted index) that the fill and the corrector read instead of
+// scanning again; the fill forms their weights again from the positions, with the same operations.
 template <typename R, bool PPE>
 __device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int* __restrict__ perm, PState<R> in,
                                 PState<R> out, const R4<R>* __restrict__ spos, R* __restrict__ spres,
@@ -281,7 +282,7 @@ __device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int*
                                 const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
                                 const uint32_t* __restrict__ skeys, int* __restrict__ active,
                                 int* __restrict__ rcnt, double* __restrict__ rdiag, int* __restrict__ rdir,
-                                int* __restrict__ nbq, R* __restrict__ nbw, int* __restrict__ nbn,
+                                int* __restrict__ nbq, int* __restrict__ nbn,
                                 int nb_width) {
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
@@ -320,7 +321,6 @@ __device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int*
         const R rl = c.re_lap, mindl = c.mind_lap;
         const int lbase = (t >> 6) * 64 * nb_width, lane = t & 63;
         int q0 = 0, q1 = 0, q2 = 0, q3 = 0;
-        R w0 = 0, w1 = 0, w2 = 0, w3 = 0;
         for_candidates_rec<8>(c, cs, ce, spos, x, y, z, t, [&](int q, const R4<R>& Q) {
             const R d0 = dist3(x, y, z, Q.x, Q.y, Q.z);
             if (d0 < re) {
@@ -335,11 +335,9 @@ __device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int*
                 wl += (double)w;
                 if (nl < nb_width) {
                     q0 = q1; q1 = q2; q2 = q3; q3 = q;
-                    w0 = w1; w1 = w2; w2 = w3; w3 = w;
                     if ((nl & 3) == 3) {
                         const int e = sell_at(lbase, lane, nl - 3);
                         quad_store(nbq + e, q0, q1, q2, q3);
-                        quad_store(nbw + e, w0, w1, w2, w3);
                     }
                 }
                 ++nl;
@@ -349,13 +347,10 @@ __device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int*
         if (kept & 3) {   // the last, partial group of the list
             const int used = kept & 3;
             int qq[4] = {q3, q3, q3, q3};
-            R ww[4] = {w3, w3, w3, w3};
-            if (used == 1) { qq[0] = q3; ww[0] = w3; }
-            if (used == 2) { qq[0] = q2; ww[0] = w2; qq[1] = q3; ww[1] = w3; }
-            if (used == 3) { qq[0] = q1; ww[0] = w1; qq[1] = q2; ww[1] = w2; qq[2] = q3; ww[2] = w3; }
+            if (used == 2) qq[0] = q2;
+            if (used == 3) { qq[0] = q1; qq[1] = q2; }
             const int e = sell_at(lbase, lane, kept - used);
             quad_store(nbq + e, qq[0], qq[1], qq[2], qq[3]);
-            quad_store(nbw + e, ww[0], ww[1], ww[2], ww[3]);
         }
     } else {
         for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
@@ -402,10 +397,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_pnd(int N, Consts<R> c, int eos, 
                                               const R* __restrict__ stl_mag, const int* __restrict__ stl_id,
                                               const uint32_t* __restrict__ skeys, int* __restrict__ active,
                                               int* __restrict__ rcnt, double* __restrict__ rdiag, int* __restrict__ rdir,
-                                              int* __restrict__ nbq, R* __restrict__ nbw, int* __restrict__ nbn,
+                                              int* __restrict__ nbq, int* __restrict__ nbn,
                                               int nb_width) {
     pnd_body<R, PPE>(blockIdx.x, N, c, eos, perm, in, out, spos, spres, cs, ce, stl_mag, stl_id, skeys, active, rcnt,
-                     rdiag, rdir, nbq, nbw, nbn, nb_width);
+                     rdiag, rdir, nbq, nbn, nb_width);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -763,18 +758,18 @@ template <typename R> void launch_relocate(const Dev<R>& d, hipStream_t st) {
     hipLaunchKernelGGL(k_relocate<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.in, d.out);
 }
 template <typename R>
-void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, int* nbq, R* nbw, int* nbn,
+void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* dir, int* nbq, int* nbn,
                     int nb_width, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL((k_pnd<R, true>), dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, 0, d.perm, d.in,
                        d.out, d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id, d.skeys, active, cnt, diag,
-                       dir, nbq, nbw, nbn, nb_width);
+                       dir, nbq, nbn, nb_width);
 }
 template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL((k_pnd<R, false>), dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, eos, d.perm, d.in, d.out,
                        d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+                       nullptr, nullptr, nullptr, nullptr, 0);
 }
 template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st, NbList nb) {
     if (d.N <= 0) return;
@@ -835,7 +830,7 @@ template <typename R> struct OpPndEos {
     __device__ void operator()(int k, int b) const {
         const Dev<R>& d = dv[k];
         pnd_body<R, false>(b, d.N, d.c, 1, d.perm, d.in, d.out, d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag,
-                           d.stl_id, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+                           d.stl_id, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
     }
 };
 template <typename R> struct OpGradient {
@@ -870,7 +865,7 @@ template <typename R> struct OpPndPpe {
         const PndRows<R>& q = rows[j];
         const Dev<R>& d = dv[q.dev];
         pnd_body<R, true>(b, d.N, d.c, 0, d.perm, d.in, d.out, d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag,
-                          d.stl_id, d.skeys, q.active, q.cnt, q.diag, q.dir, q.nbq, q.nbw, q.nbn, q.nb_width);
+                          d.stl_id, d.skeys, q.active, q.cnt, q.diag, q.dir, q.nbq, q.nbn, q.nb_width);
     }
 };
 template <typename R> void launch_pnd_ppe_b(const BatchLaunch<R>& b, const PndRows<R>* rows, WorkList p, hipStream_t st) {
@@ -907,7 +902,7 @@ template <typename R> void launch_turbulence_b(const BatchLaunch<R>& b, WorkList
     template void launch_explicit<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_relocate<R>(const Dev<R>&, hipStream_t);                             \
     template void launch_pnd<R>(const Dev<R>&, int, hipStream_t);                             \
-    template void launch_pnd_ppe<R>(const Dev<R>&, int*, int*, double*, int*, int*, R*, int*, int, hipStream_t); \
+    template void launch_pnd_ppe<R>(const Dev<R>&, int*, int*, double*, int*, int*, int*, int, hipStream_t); \
     template void launch_gradient<R>(const Dev<R>&, hipStream_t, NbList);                     \
     template void launch_thermal<R>(const Dev<R>&, const int*, const int*, int, hipStream_t);   \
     template void launch_turbulence<R>(const Dev<R>&, hipStream_t);                           \

@@ -99,7 +99,6 @@ template <typename R> struct PpeSolver {
     int hint[HINT_PASSES] = {};  // iterations of each refinement pass of the last solve (0: none)
     int* nbq = nullptr;          // [rows * NB_WIDTH] neighbour list of the PND pass (sorted index)
     int* nbn = nullptr;          // [rows] its neighbour count per row (-1: row skipped)
-    R* nbw = nullptr;            // ... and the weights
     int* slice_w = nullptr;
     int* pos = nullptr;          // [positions] SELL position of each row (rows ordered by length per window)
     int* rowp = nullptr;         // [positions] row at each SELL position, -1 when empty
@@ -360,7 +359,7 @@ __device__ inline void ppe_fill_body(int blk, int N, int nslices, Consts<R> c, c
                                      const int* __restrict__ perm, PState<R> out, double alpha, double n0,
                                      const int* __restrict__ rowp, int* cols, VA* vals, double* rhs,
                                      int* cnt, int* nnz, const int* __restrict__ nbq,
-                                     const R* __restrict__ nbw, int nb_width, uint16_t* cols16,
+                                     int nb_width, uint16_t* cols16,
                                      const uint32_t* __restrict__ skeys, const int* __restrict__ fg) {
     const int P = blk * blockDim.x + threadIdx.x;
     const int slice = P >> 6, lane = lane_id();
@@ -390,34 +389,34 @@ __device__ inline void ppe_fill_body(int blk, int N, int nslices, Consts<R> c, c
         const int bound = cnt[t];   // Laplacian-radius neighbours, from the PND pass
         if (bound <= nb_width) {
             // the PND pass listed them with their weights, in the canonical order
-            // (a group's 4 active flags are loaded together with the next group's entries and waited
-            // for once; the last group is padded with a valid index)
+            // (the weights are formed again from the positions, as the PND pass formed them; a
+            // group's 4 active flags and positions are loaded together with the next group's
+            // entries and waited for once; the last group is padded with a valid index)
             const int lbase = (t >> 6) * 64 * nb_width, ll = t & 63;
+            const R4<R> X = spos[t];
+            const R rl = c.re_lap, mindl = c.mind_lap;
             int q[4] = {t, t, t, t};
-            R w[4] = {0, 0, 0, 0};
-            if (bound > 0) {
-                quad_load(nbq + sell_at(lbase, ll, 0), q);
-                quad_load(nbw + sell_at(lbase, ll, 0), w);
-            }
+            if (bound > 0) quad_load(nbq + sell_at(lbase, ll, 0), q);
             for (int j = 0; j < bound; j += 4) {
                 int act[4], nq[4];
-                R nw[4];
+                R4<R> Q[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) act[u] = active[q[u]];
-                const int e = sell_at(lbase, ll, j + 4 < bound ? j + 4 : j);
-                quad_load(nbq + e, nq);
-                quad_load(nbw + e, nw);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) asm volatile("" ::"v"(act[u]));
+                for (int u = 0; u < 4; ++u) Q[u] = spos[q[u]];
+                quad_load(nbq + sell_at(lbase, ll, j + 4 < bound ? j + 4 : j), nq);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) asm volatile("" ::"v"(act[u]), "v"(Q[u].x), "v"(Q[u].y), "v"(Q[u].z));
                 const int m = bound - j < 4 ? bound - j : 4;
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    if (u < m && act[u] && k < width) push(q[u], (VA)(-(double)w[u]));
+                    if (u < m && act[u] && k < width) {
+                        R d = dist3(X.x, X.y, X.z, Q[u].x, Q[u].y, Q[u].z);
+                        if (d < mindl) d = mindl;
+                        push(q[u], (VA)(-(double)weight(rl, d)));
+                    }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    q[u] = nq[u];
-                    w[u] = nw[u];
-                }
+                for (int u = 0; u < 4; ++u) q[u] = nq[u];
             }
         } else {
             const R4<R> X = spos[t];
@@ -460,10 +459,10 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
                                                   const int* __restrict__ perm, PState<R> out, double alpha, double n0,
                                                   const int* __restrict__ rowp, int* cols, VA* vals, double* rhs,
                                                   int* cnt, int* nnz, const int* __restrict__ nbq,
-                                                  const R* __restrict__ nbw, int nb_width, uint16_t* cols16,
+                                                  int nb_width, uint16_t* cols16,
                                                   const uint32_t* __restrict__ skeys, const int* __restrict__ fg) {
     ppe_fill_body<R, VA>(blockIdx.x, N, nslices, c, active, dir, spos, cs, ce, slice_off, slice_w, perm, out, alpha,
-                         n0, rowp, cols, vals, rhs, cnt, nnz, nbq, nbw, nb_width, cols16, skeys, fg);
+                         n0, rowp, cols, vals, rhs, cnt, nnz, nbq, nb_width, cols16, skeys, fg);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1700,8 +1699,7 @@ PpeSolver<R>* ppe_create(const Dev<R>& d, const mpsr_params& P, const mpsr_ppe_o
     const int nb = (rows + PB - 1) / PB;
     bool ok = true;
     ok &= dev_alloc(&s->active, rows) && dev_alloc(&s->dir, rows) && dev_alloc(&s->cnt, rows);
-    ok &= dev_alloc(&s->nbq, (size_t)rows * NB_WIDTH) && dev_alloc(&s->nbw, (size_t)rows * NB_WIDTH) &&
-          dev_alloc(&s->nbn, rows) && dev_alloc(&s->first_ge, (size_t)d.c.ncell + 1);
+    ok &= dev_alloc(&s->nbq, (size_t)rows * NB_WIDTH) && dev_alloc(&s->nbn, rows) && dev_alloc(&s->first_ge, (size_t)d.c.ncell + 1);
     ok &= dev_alloc(&s->slice_w, s->nslices + 1) && dev_alloc(&s->slice_off, s->nslices + 1) && dev_alloc(&s->total, 1);
     const int positions = ((d.cap + SELL_SIGMA - 1) / SELL_SIGMA) * SELL_SIGMA;
     ok &= dev_alloc(&s->pos, positions) && dev_alloc(&s->rowp, positions);
@@ -1997,7 +1995,7 @@ template <typename R> void ppe_destroy(PpeSolver<R>* s) {
                   s->diag, s->rhs, s->x, s->prev, s->r, s->z, s->p0, s->p1, s->q, s->rf, s->zf, s->p0f,
                   s->p1f, s->qf, s->ef, s->part, s->scal, s->scan.scan_tmp, s->nnz_dev, s->agg_flag,
                   s->agg_map, s->agg_full, s->agg_of_row, s->nc_dev, s->info_dev, s->Ac, s->Af, s->agg_rng, s->cpart, s->rxyz, s->rc, s->cc,
-                  s->agg_scan.scan_tmp, s->pdiag_buf, s->omega, s->xprev_s, s->xl, s->nbq, s->nbw, s->nbn, s->cxyz,
+                  s->agg_scan.scan_tmp, s->pdiag_buf, s->omega, s->xprev_s, s->xl, s->nbq, s->nbn, s->cxyz,
                   s->cell_of_row, s->c_agg, s->c_op, s->c_dinv, s->c_rc, s->c_e, s->c_ef};
     if (s->net) mpnn_destroy(s->net);
     for (void* p : ps)
@@ -2157,7 +2155,7 @@ template <typename R> static MpnnGeo<R> mpnn_geo(const Dev<R>& d) {
 }
 
 template <typename R> void ppe_start(PpeSolver<R>* s, const Dev<R>& d, hipStream_t st) {
-    launch_pnd_ppe(d, s->active, s->cnt, s->diag, s->dir, s->nbq, s->nbw, s->nbn, NB_WIDTH, st);
+    launch_pnd_ppe(d, s->active, s->cnt, s->diag, s->dir, s->nbq, s->nbn, NB_WIDTH, st);
 }
 template <typename R> NbList ppe_neighbors(const PpeSolver<R>* s) {
     NbList nb;
@@ -2227,12 +2225,12 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     if (s->mixed)
         hipLaunchKernelGGL((k_ppe_fill<R, float>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
-                           s->rowp, s->cols, s->vals32, s->rhs, s->cnt, s->nnz_dev, s->nbq, s->nbw, NB_WIDTH,
+                           s->rowp, s->cols, s->vals32, s->rhs, s->cnt, s->nnz_dev, s->nbq, NB_WIDTH,
                            s->cols16, d.skeys, s->first_ge);
     else
         hipLaunchKernelGGL((k_ppe_fill<R, double>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
-                           s->rowp, s->cols, s->vals64, s->rhs, s->cnt, s->nnz_dev, s->nbq, s->nbw, NB_WIDTH,
+                           s->rowp, s->cols, s->vals64, s->rhs, s->cnt, s->nnz_dev, s->nbq, NB_WIDTH,
                            s->cols16, d.skeys, s->first_ge);
     // entry and row counts of the filled matrix, read after the first residual pass synchronises
     if (hipMemcpyAsync(s->cnt_h, s->nnz_dev, 4 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -2667,7 +2665,6 @@ template <typename R> struct PpeView {
     double tol = 1e-8, alpha_src = 0.0, n0 = 0.0;
     long long nnz_cap = 0;
     int *active, *dir, *cnt, *nbq, *nbn;
-    R* nbw;
     int *slice_w, *slice_off, *pos, *rowp, *total, *first_ge, *cols, *nnz;
     uint16_t* cols16;
     double *vals, *diag, *rhs, *x, *prev, *r, *z, *p, *q, *part, *scal;
@@ -2701,7 +2698,7 @@ template <typename R> struct OpPFill {
         }
         ppe_fill_body<R, double>(b, d.N, v.nslices, d.c, v.active, v.dir, d.spos, d.cell_start, d.cell_end,
                                  v.slice_off, v.slice_w, d.perm, d.out, v.alpha_src, v.n0, v.rowp, v.cols, v.vals,
-                                 v.rhs, v.cnt, v.nnz, v.nbq, v.nbw, NB_WIDTH, v.cols16, d.skeys, v.first_ge);
+                                 v.rhs, v.cnt, v.nnz, v.nbq, NB_WIDTH, v.cols16, d.skeys, v.first_ge);
     }
 };
 template <typename R> struct OpPWarm {
@@ -2897,14 +2894,14 @@ mpsr_status ppe_batch_build(PpeBatch<R>* b, PpeSolver<R>* const* solvers, const 
         v.alpha_src = s->alpha * (o.source_relaxation > 0 ? o.source_relaxation : 0.1);
         v.n0 = s->n0;
         v.nnz_cap = s->nnz_cap;
-        v.active = s->active; v.dir = s->dir; v.cnt = s->cnt; v.nbq = s->nbq; v.nbn = s->nbn; v.nbw = s->nbw;
+        v.active = s->active; v.dir = s->dir; v.cnt = s->cnt; v.nbq = s->nbq; v.nbn = s->nbn;
         v.slice_w = s->slice_w; v.slice_off = s->slice_off; v.pos = s->pos; v.rowp = s->rowp; v.total = s->total;
         v.first_ge = s->first_ge; v.cols = s->cols; v.nnz = s->nnz_dev; v.cols16 = s->cols16; v.vals = s->vals64;
         v.diag = s->diag; v.rhs = s->rhs; v.x = s->x; v.prev = s->prev; v.r = s->r; v.z = s->z; v.p = s->p0;
         v.q = s->q; v.part = s->part; v.scal = s->scal;
         v.flags = b->flags + 2 * j;
         b->hv[j] = v;
-        rows[j] = PndRows<R>{dev[j], s->active, s->cnt, s->diag, s->dir, s->nbq, s->nbw, s->nbn, NB_WIDTH};
+        rows[j] = PndRows<R>{dev[j], s->active, s->cnt, s->diag, s->dir, s->nbq, s->nbn, NB_WIDTH};
         for (int k = 0; k < (int)grid_for(N, block); ++k) pnd.push_back(make_int2(j, k));
         for (int k = 0; k < (int)grid_for((long long)d.c.ncell + 1, PB); ++k) cells.push_back(make_int2(j, k));
         for (int k = 0; k < (int)grid_for(N, SELL_SIGMA); ++k) win.push_back(make_int2(j, k));
