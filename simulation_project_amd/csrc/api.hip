@@ -546,6 +546,9 @@ template <typename R> struct Engine : EngineBase {
         HIPC(hipMemset(d.uniq_n, 0, sizeof(int)));
         ALLOC(d.cell_tri_start, d.c.ncell); ALLOC(d.cell_tri_count, d.c.ncell);
         HIPC(hipMemset(d.cell_tri_count, 0, sizeof(int) * d.c.ncell));
+        ALLOC(d.cell_utri, (size_t)d.c.ncell * kStencilTris); ALLOC(d.cell_utri_n, d.c.ncell);
+        ALLOC(d.cell_work, d.c.ncell); ALLOC(d.cell_work_n, 1);
+        HIPC(hipMemset(d.cell_utri_n, 0xff, sizeof(int) * d.c.ncell));   // -1: no walk yet
         // capacity of the (cell, triangle) pair buffer from the walls as given
         int pair_total = 0;
         if (d.ntri > 0) {

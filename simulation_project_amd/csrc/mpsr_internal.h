@@ -138,6 +138,9 @@ template <typename R> struct PState {
 
 // Triangle record on the device: three vertices, the stored normal, the model, the model's
 // drag flag.  64 B for float, 112 B for double.
+// length of the per-cell triangle walks (nearest_triangle_listed)
+static constexpr int kStencilTris = 32;
+
 template <typename R> struct alignas(16) TriD {
     R v[9];
     R nrm[3];
@@ -398,6 +401,10 @@ template <typename R> struct Dev {
     int* overflow = nullptr;          // set when a march emits more than pair_cap pairs
     int* cell_tri_start = nullptr;    // [ncell]
     int* cell_tri_count = nullptr;    // [ncell], at most 100
+    int* cell_utri = nullptr;         // [ncell * kStencilTris] each cell's triangle walk
+    int* cell_utri_n = nullptr;       // [ncell] its length, -1: none
+    int* cell_work = nullptr;         // [ncell] cells whose walk is being written
+    int* cell_work_n = nullptr;       // [1]
     const int* step_base = nullptr;   // batched step: device index of the first replayed step
     int thermal_step = 1;             // batched step: heat conduction every this many steps
     // nearest wall and drag, sorted order

@@ -161,10 +161,112 @@ template <typename R> void launch_tri_count_total(Dev<R>& d, hipStream_t st) {
     exclusive_scan(d.tri_emit, d.tri_off, nullptr, d.ntri, d.ntri, d.pair_n, d.tsort, st);
 }
 
+// The walk of every cell that holds particles, for nearest_triangle_listed, in two passes.
+// k_cell_walk_cells: one thread per cell reads the 27 list lengths; a cell without particles gets
+// -1, one whose stencil lists are all empty gets 0, the others go to a work list (in any order:
+// each cell's walk depends only on its own lists).
+__global__ __launch_bounds__(256) void k_cell_walk_cells(int ncell, int gy, int gz, const int* __restrict__ cs,
+                                                         const int* __restrict__ tcount, int* work, int* nwork,
+                                                         int* ucount) {
+    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= ncell) return;
+    if (cs[cell] < 0) {
+        ucount[cell] = -1;
+        return;
+    }
+    const long long gyz = (long long)gy * gz;
+    const int ci = (int)(cell / gyz), cj = (int)((cell / gz) % gy), ck = (int)(cell % gz);
+    int total = 0;
+#pragma unroll
+    for (int s = 0; s < 27; ++s) {
+        const long long m = gyz * (ci + s % 3 - 1) + (long long)gz * (cj + (s / 3) % 3 - 1) + (ck + s / 9 - 1);
+        total += (m >= 0 && m < ncell) ? tcount[m] : 0;
+    }
+    if (total == 0) {
+        ucount[cell] = 0;
+        return;
+    }
+    work[atomicAdd(nwork, 1)] = (int)cell;
+}
+// k_cell_walks: one wave per listed cell.  Lane s < 27 reads stencil cell s's list (z outer, x
+// inner); the ids are then taken 64 at a time in walk order, one per lane, and an id is kept when
+// neither the ids kept so far (LDS) nor an earlier lane of the same 64 holds it: its first meeting.
+// More than kStencilTris kept ids give -1 (the full walk).  Waves loop over the work list.
+__global__ __launch_bounds__(64) void k_cell_walks(int ncell, int gy, int gz, const int* __restrict__ work,
+                                                   const int* __restrict__ nwork, const int* __restrict__ tstart,
+                                                   const int* __restrict__ tcount, const int* __restrict__ tids,
+                                                   int* ulist, int* ucount) {
+    __shared__ int off[28], beg[27], kept[kStencilTris];
+    const int lane = threadIdx.x;
+    const int nw = *nwork;
+    const long long gyz = (long long)gy * gz;
+    for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+        const long long cell = work[w];
+        const int ci = (int)(cell / gyz), cj = (int)((cell / gz) % gy), ck = (int)(cell % gz);
+        int cnt = 0, b0 = 0;
+        if (lane < 27) {
+            const long long m = gyz * (ci + lane % 3 - 1) + (long long)gz * (cj + (lane / 3) % 3 - 1) +
+                                (ck + lane / 9 - 1);
+            if (m >= 0 && m < ncell) {
+                cnt = tcount[m];
+                b0 = tstart[m];
+            }
+        }
+        const int inc = wave_inclusive_scan(cnt);
+        const int total = __shfl(inc, 63, 64);
+        if (lane < 27) {
+            off[lane] = inc - cnt;
+            beg[lane] = b0;
+        }
+        if (lane == 0) off[27] = total;
+        __syncthreads();
+        int n = 0;
+        bool over = false;
+        for (int base = 0; base < total && !over; base += 64) {
+            const int f = base + lane;
+            int tid = -1;
+            if (f < total) {
+                int s = 0;   // the stencil cell of walk position f: off[s] <= f < off[s + 1]
+                for (int h = 16; h > 0; h >>= 1)
+                    if (s + h < 27 && off[s + h] <= f) s += h;
+                tid = tids[beg[s] + f - off[s]];
+            }
+            bool first = f < total;
+            for (int q = 0; q < n; ++q) first = first && kept[q] != tid;
+            for (int j = 0; j < 63; ++j) {
+                const int tj = __shfl(tid, j, 64);
+                if (j < lane && tj == tid) first = false;
+            }
+            const unsigned long long mask = __ballot(first);
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            const int add = __popcll(mask);
+            __syncthreads();
+            if (first && n + rank < kStencilTris) {
+                kept[n + rank] = tid;
+                ulist[cell * kStencilTris + n + rank] = tid;
+            }
+            __syncthreads();
+            if (n + add > kStencilTris) over = true;
+            n = min(n + add, kStencilTris);
+        }
+        if (lane == 0) ucount[cell] = over ? -1 : n;
+        __syncthreads();
+    }
+}
+
+template <typename R> static void launch_cell_stencil_tris(const Dev<R>& d, hipStream_t st) {
+    fill_int(d.cell_work_n, 0, 1, st);
+    hipLaunchKernelGGL(k_cell_walk_cells, dim3(grid_for(d.c.ncell, 256)), dim3(256), 0, st, (int)d.c.ncell, d.c.gy,
+                       d.c.gz, d.cell_start, d.cell_tri_count, d.cell_work, d.cell_work_n, d.cell_utri_n);
+    hipLaunchKernelGGL(k_cell_walks, dim3(2048), dim3(64), 0, st, (int)d.c.ncell, d.c.gy, d.c.gz, d.cell_work,
+                       d.cell_work_n, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.cell_utri, d.cell_utri_n);
+}
+
 template <typename R> void launch_tri_bin(Dev<R>& d, hipStream_t st) {
     fill_int(d.cell_tri_count, 0, d.c.ncell, st);
     if (d.ntri <= 0) {
         fill_int(d.uniq_n, 0, 1, st);
+        launch_cell_stencil_tris(d, st);
         return;
     }
     const int T = d.ntri;
@@ -183,6 +285,7 @@ template <typename R> void launch_tri_bin(Dev<R>& d, hipStream_t st) {
                        d.pair_pos, d.uniq_cell, d.uniq_tri);
     hipLaunchKernelGGL(k_pair_ranges, dim3(g), dim3(256), 0, st, d.uniq_n, d.uniq_cell, d.cell_tri_start,
                        d.cell_tri_count);
+    launch_cell_stencil_tris(d, st);
 }
 
 template <typename R> __device__ inline R4<R> mk4w(R x, R y, R z);
@@ -193,13 +296,15 @@ template <typename R>
 __device__ inline void stl_distance_body(int blk, int N, Consts<R> c, const R4<R>* __restrict__ spos,
                                          const int* __restrict__ tstart, const int* __restrict__ tcount,
                                          const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
-                                         R* stl_mag, R4<R>* stl_dir, int* stl_id) {
+                                         R* stl_mag, R4<R>* stl_dir, int* stl_id, const int* __restrict__ ulist,
+                                         const int* __restrict__ ucount) {
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const R4<R> P = spos[t];
     R best;
     P3<R> dir;
-    const int bid = nearest_triangle(c, P.x, P.y, P.z, tstart, tcount, tids, tris, 0, &best, &dir);
+    const int bid = nearest_triangle_listed(c, P.x, P.y, P.z, ulist, ucount, tstart, tcount, tids, tris, 0, &best,
+                                            &dir);
     stl_mag[t] = best;
     stl_dir[t] = mk4w<R>(dir.x, dir.y, dir.z);
     stl_id[t] = bid;
@@ -208,8 +313,9 @@ template <typename R>
 __global__ __launch_bounds__(kMaxBlock) void k_stl_distance(int N, Consts<R> c, const R4<R>* __restrict__ spos,
                                                        const int* __restrict__ tstart, const int* __restrict__ tcount,
                                                        const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
-                                                       R* stl_mag, R4<R>* stl_dir, int* stl_id) {
-    stl_distance_body<R>(blockIdx.x, N, c, spos, tstart, tcount, tids, tris, stl_mag, stl_dir, stl_id);
+                                                       R* stl_mag, R4<R>* stl_dir, int* stl_id,
+                                                       const int* __restrict__ ulist, const int* __restrict__ ucount) {
+    stl_distance_body<R>(blockIdx.x, N, c, spos, tstart, tcount, tids, tris, stl_mag, stl_dir, stl_id, ulist, ucount);
 }
 
 
@@ -220,7 +326,7 @@ __device__ inline void drag_body(int blk, int N, Consts<R> c, const int* __restr
                                  const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
                                  const mpsr_tri_params* __restrict__ models,
                                  const mpsr_drag_params* __restrict__ drag, R4<R>* drag_acc,
-                                 R* drag_temp) {
+                                 R* drag_temp, const int* __restrict__ ulist, const int* __restrict__ ucount) {
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
@@ -228,7 +334,8 @@ __device__ inline void drag_body(int blk, int N, Consts<R> c, const int* __restr
     const R4<R> P = spos[t];
     R best;
     P3<R> dir;
-    const int bid = nearest_triangle(c, P.x, P.y, P.z, tstart, tcount, tids, tris, 1, &best, &dir);
+    const int bid = nearest_triangle_listed(c, P.x, P.y, P.z, ulist, ucount, tstart, tcount, tids, tris, 1, &best,
+                                            &dir);
     drag_acc[t] = mk4w<R>(0, 0, 0);
     drag_temp[t] = 0;
     out.type[s] = 0;
@@ -280,9 +387,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_drag(int N, Consts<R> c, const in
                                                const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
                                                const mpsr_tri_params* __restrict__ models,
                                                const mpsr_drag_params* __restrict__ drag, R4<R>* drag_acc,
-                                               R* drag_temp) {
+                                               R* drag_temp, const int* __restrict__ ulist,
+                                               const int* __restrict__ ucount) {
     drag_body<R>(blockIdx.x, N, c, perm, in, out, spos, svel, tstart, tcount, tids, tris, models, drag, drag_acc,
-                 drag_temp);
+                 drag_temp, ulist, ucount);
 }
 
 template <typename R>
@@ -334,13 +442,14 @@ __global__ __launch_bounds__(256) void k_rotate(int T, TriD<R>* tris, const RotC
 template <typename R> void launch_stl_distance(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL(k_stl_distance<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.spos,
-                       d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.stl_mag, d.stl_dir, d.stl_id);
+                       d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.stl_mag, d.stl_dir, d.stl_id,
+                       d.cell_utri, d.cell_utri_n);
 }
 template <typename R> void launch_drag(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL(k_drag<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out, d.spos,
                        d.svel, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.models, d.drag, d.drag_acc,
-                       d.drag_temp);
+                       d.drag_temp, d.cell_utri, d.cell_utri_n);
 }
 template <typename R> void launch_translate(const Dev<R>& d, hipStream_t st) {
     if (d.ntri <= 0) return;
@@ -394,7 +503,7 @@ template <typename R> struct OpStl {
     __device__ void operator()(int k, int b) const {
         const Dev<R>& d = dv[k];
         stl_distance_body<R>(b, d.N, d.c, d.spos, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.stl_mag,
-                             d.stl_dir, d.stl_id);
+                             d.stl_dir, d.stl_id, nullptr, nullptr);
     }
 };
 template <typename R> struct OpDrag {
@@ -402,7 +511,7 @@ template <typename R> struct OpDrag {
     __device__ void operator()(int k, int b) const {
         const Dev<R>& d = dv[k];
         drag_body<R>(b, d.N, d.c, d.perm, d.in, d.out, d.spos, d.svel, d.cell_tri_start, d.cell_tri_count, d.uniq_tri,
-                     d.tris, d.models, d.drag, d.drag_acc, d.drag_temp);
+                     d.tris, d.models, d.drag, d.drag_acc, d.drag_temp, nullptr, nullptr);
     }
 };
 template <typename R> struct OpTranslate {

@@ -108,6 +108,45 @@ __device__ inline int nearest_triangle(const Consts<R>& c, R x, R y, R z, const 
     return bid;
 }
 
+// A triangle spans several cells, so the 27-cell walk of nearest_triangle meets it in several
+// lists.  Met again, its distance is the same and only a strictly nearer triangle replaces the
+// running nearest, so any walk that meets the same triangles in the same order of first meeting,
+// with or without repeats, gives nearest_triangle's result.  k_cell_stencil_tris writes such a
+// walk once per cell and triangle binning (repeats among the last 8 ids dropped); a cell without
+// one (count -1: no particles when the lists were written, or more than kStencilTris ids) takes
+// the full walk.
+template <typename R>
+__device__ inline int nearest_triangle_listed(const Consts<R>& c, R x, R y, R z, const int* __restrict__ ulist,
+                                              const int* __restrict__ ucount, const int* __restrict__ tstart,
+                                              const int* __restrict__ tcount, const int* __restrict__ tids,
+                                              const TriD<R>* __restrict__ tris, int want_drag, R* best, P3<R>* bdir) {
+    int ci, cj, ck;
+    long long cid = -1;
+    if (ucount && cell_coords(c, x, y, z, ci, cj, ck)) cid = cell_of(c, ci, cj, ck);
+    const int n = (cid >= 0 && cid < c.ncell) ? ucount[cid] : -1;
+    if (n < 0) return nearest_triangle(c, x, y, z, tstart, tcount, tids, tris, want_drag, best, bdir);
+    *best = (R)1e8;
+    *bdir = p3<R>(0, 0, 0);
+    int bid = -1;
+    const P3<R> p = p3(x, y, z);
+    const int* L = ulist + cid * kStencilTris;
+    for (int e = 0; e < n; ++e) {
+        const int tid = L[e];
+        const TriD<R>& tr = tris[tid];
+        if (tr.is_drag != want_drag) continue;
+        if (tid != bid) {
+            P3<R> dv;
+            const R dm = tri_distance(tr, p, &dv);
+            if (dm < *best) {
+                *best = dm;
+                *bdir = dv;
+                bid = tid;
+            }
+        }
+    }
+    return bid;
+}
+
 // The wall of the negative-pressure term (radix DeviceProcess_Kernel.h:1110-1134): every triangle
 // of the 27-cell stencil's lists, drag or not; the first listed triangle of each cell replaces the
 // running nearest (the reference's l == 0 branch) unless it is that triangle already, the others

@@ -599,3 +599,27 @@ def test_thermal_steps_with_drag_heat_identical(lib, precision):
     same_state(g, o, label="thermal steps")
     t = o.get_particles(0).temperature
     assert np.ptp(t) > 0 and not np.allclose(t, case.particles.temperature)
+
+
+@pytest.mark.parametrize("n_r,n_x", [(6, 12), (16, 36), (96, 200)])
+def test_stl_distance_walks_any_mesh_density(lib, n_r, n_x):
+    """CalcSTLDistance (Kern :635-699) on drums of 24.8 mm cells whose walls are meshed from coarse
+    (25 mm triangles: a cell's stored walk holds its stencil's few distinct triangles) to fine
+    (1.6 mm: more distinct triangles than a stored walk holds, so the cell takes the full walk),
+    before and after the particles move (cells that had no particles when the walks were
+    written): distances, directions and ids equal the oracle's bit for bit."""
+    case = cases.drum(lib, l0=8e-3, n_r=n_r, n_x=n_x)
+    print(f"[stl-walks] n_r={n_r} triangles={len(case.vertices)} particles={case.n}", flush=True)
+    g, o = pair(case)
+    run_phase("RegisterParticleTopology", g, o)
+    run_phase("RegisterTriangleTopology", g, o)
+    run_phase("CalcSTLDistance", g, o)
+    for a, b in zip(g.stl_distance(), o.stl_distance()):
+        np.testing.assert_array_equal(a, b)
+    run_phase("CalcExplicitly", g, o)
+    run_phase("RelocateAllParticleTopology", g, o)
+    run_phase("RegisterParticleTopology", g, o)
+    run_phase("CalcSTLDistance", g, o)
+    for a, b in zip(g.stl_distance(), o.stl_distance()):
+        np.testing.assert_array_equal(a, b)
+    assert np.any(g.stl_distance()[2] >= 0)
