@@ -70,27 +70,37 @@ def run_steps(s, first, count):
     return time.perf_counter() - t0, out
 
 
-def run_arm(Solver, abi, case, sph, kind, precond, warm, warmup, steps, weights=None):
+def run_arm(Solver, abi, case, sph, kind, precond, warm, warmup, steps, weights=None, phase_steps=0):
     """One solver arm on a fresh handle: `warmup` untimed steps, then `steps` steps in one
-    mpsr_step_many call timed on the host around a synchronised stream.  Returns the wall-clock
-    figures and the call's statistics (phase times and SpMV launch times from HIP events)."""
+    mpsr_step_many call timed on the host around a synchronised stream.  Inside the timed call only
+    the PPE phase and the SpMV launches of every fourth PCG chunk carry HIP events (each event
+    delays the stream by a few microseconds).  With phase_steps, that many more steps follow with
+    events around every phase and SpMV launch, for the per-phase breakdown.  Returns the wall-clock
+    time, the timed call's statistics (PPE time, SpMV launch times) and those of the breakdown
+    pass (or None)."""
     s = Solver(case, precision=abi.MPSR_FP32, pressure_mode=abi.MPSR_PRESSURE_PPE,
                ppe=ppe_options(abi, kind, precond, warm))
     if weights is not None:
         s.mpnn_set_weights(weights)
     if sph is not None:
         s.set_spheres(sph)
-    s.set_profiling(True)
     if warmup:
         s.step_many(0, warmup)
+    s.set_profiling(2)
     s.synchronize()
     t0 = time.perf_counter()
     s.step_many(warmup, steps)
     s.synchronize()
     elapsed = time.perf_counter() - t0
     t = s.timings()
+    tp = None
+    if phase_steps:
+        s.set_profiling(1)
+        s.step_many(warmup + steps, phase_steps)
+        s.synchronize()
+        tp = s.timings()
     s.close()
-    return elapsed, t
+    return elapsed, t, tp
 
 
 def arm_summary(elapsed, t, steps):
@@ -180,8 +190,8 @@ def main():
     warm = abi.MPSR_WARM_LEARNED if solver == "learned" else abi.MPSR_WARM_PREVIOUS
     kind = "fp64" if solver == "fp64" else "mixed"
     log(f"[bench] timed region: {args.steps} steps after {args.warmup} warm-up steps")
-    elapsed, t = run_arm(Solver, abi, case, sph, kind, args.precond, warm, args.warmup, args.steps,
-                         weights if solver == "learned" else None)
+    elapsed, t, tp = run_arm(Solver, abi, case, sph, kind, args.precond, warm, args.warmup, args.steps,
+                             weights if solver == "learned" else None, phase_steps=5)
     value = n * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     iters = t["ppe_iterations_total"] / args.steps
@@ -189,9 +199,11 @@ def main():
     spmv_avg_ms = t["spmv_ms"]
     spmv_b = t["spmv_bytes"]
     achieved = spmv_b / (spmv_avg_ms * 1e-3) / 1e9 if spmv_avg_ms > 0 else 0.0
-    phases = {k: round(float(t[k]), 4) for k in
+    phases = {k: round(float(tp[k]), 4) for k in
               ("bin_ms", "tri_ms", "stl_ms", "explicit_ms", "relocate_ms", "pnd_ms", "ppe_ms", "gradient_ms",
                "wall_ms", "sphere_ms")}
+    phases["note"] = ("5 steps after the timed ones, every phase and SpMV launch between HIP events: "
+                      f"{tp['step_ms']:.3f} ms/step there")
     # whole-step algorithmic bytes: the particle phases (FP32: 442 B per particle) plus the PPE
     # assembly and every PCG iteration (SpMV values and columns, 19 vector words per row)
     nnz = float(t["ppe_nnz"])
@@ -212,7 +224,7 @@ def main():
             if name in arms:
                 continue
             log(f"[bench] arm {name}: {args.warmup} + {args.arm_steps} steps")
-            e2, t2 = run_arm(Solver, abi, case, sph, k2, pre2, warm2, args.warmup, args.arm_steps, w2)
+            e2, t2, _ = run_arm(Solver, abi, case, sph, k2, pre2, warm2, args.warmup, args.arm_steps, w2)
             arms[name] = arm_summary(e2, t2, args.arm_steps)
             if warm2 == abi.MPSR_WARM_LEARNED:
                 arms[name]["learned_fallbacks_last_step"] = t2["ppe_learned_fallback"]

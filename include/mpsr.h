@@ -460,7 +460,9 @@ mpsr_status mpsr_get_ppe_matrix(mpsr_handle *h, int *row_ptr, int *col, double *
  * operator.  No counterpart in the reference (its solver has no PPE); it exposes that sweep for
  * checking.  MPSR_CALCERR when a pivot is not positive (the matrix is not SPD). */
 mpsr_status mpsr_spd_inverse(int n, const double *a, double *inv);
-/* Per-phase HIP-event timing of the following steps (mpsr_get_timings); off by default. */
+/* HIP-event timing of the following steps (mpsr_get_timings); off (0) by default.  1: every phase
+ * and every PPE SpMV launch.  2: the PPE phase and the SpMV launches of every fourth PCG chunk
+ * (each event delays the stream by a few microseconds; 2 keeps timing runs close to untimed ones). */
 mpsr_status mpsr_set_profiling(mpsr_handle *h, int on);
 
 /* ---- turbulence and negative pressure (the radix solver variant's terms) ------------------------

@@ -381,7 +381,10 @@ class Solver:
         self._check(self.lib.mpsr_set_dt(self.h, dt, courant), "mpsr_set_dt")
 
     def set_profiling(self, on=True):
-        self._check(self.lib.mpsr_set_profiling(self.h, 1 if on else 0), "mpsr_set_profiling")
+        """True or 1: events around every phase and SpMV launch; 2: around the PPE phase and a
+        quarter of the SpMV launches (for timing runs); False or 0: off."""
+        level = on if isinstance(on, int) and not isinstance(on, bool) else (1 if on else 0)
+        self._check(self.lib.mpsr_set_profiling(self.h, level), "mpsr_set_profiling")
 
     def set_ppe_options(self, opt):
         self._check(self.lib.mpsr_set_ppe_options(self.h, C.byref(opt)), "mpsr_set_ppe_options")

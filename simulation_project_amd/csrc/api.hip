@@ -583,8 +583,12 @@ template <typename R> struct Engine : EngineBase {
     }
 
     // ---- timing -----------------------------------------------------------------------------
+    // profiling 1 brackets every phase with events, 2 only the PPE phase (each event costs the
+    // stream a few microseconds)
+    bool pev_open = false;
     void mark_begin(int ph) {
-        if (!profiling) return;
+        pev_open = false;
+        if (!profiling || (profiling == 2 && ph != PH_PPE)) return;
         if (pev_used + 2 > pev.size()) {
             for (int k = 0; k < 64; ++k) {
                 hipEvent_t e;
@@ -595,11 +599,13 @@ template <typename R> struct Engine : EngineBase {
         }
         pev_phase[pev_used] = ph;
         (void)hipEventRecord(pev[pev_used], st);
+        pev_open = true;
     }
     void mark_end() {
-        if (!profiling) return;
+        if (!pev_open) return;
         (void)hipEventRecord(pev[pev_used + 1], st);
         pev_used += 2;
+        pev_open = false;
     }
 
     // ---- phases -------------------------------------------------------------------------------
@@ -1901,6 +1907,7 @@ mpsr_status mpsr_get_timings(mpsr_handle* h, mpsr_timings* o) {
 mpsr_status mpsr_set_ppe_options(mpsr_handle* h, const mpsr_ppe_options* o) { GUARD(h); return h->e->set_ppe(o); }
 mpsr_status mpsr_set_profiling(mpsr_handle* h, int on) {
     GUARD(h);
+    if (on < 0 || on > 2) return MPSR_PARAMERR;
     h->e->profiling = on;
     return MPSR_NOERR;
 }

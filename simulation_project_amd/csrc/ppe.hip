@@ -137,7 +137,8 @@ template <typename R> struct PpeSolver {
     long long nnz_stored = 0;     // entries kept, SELL padding included
     int rows_real = 0;
     // SpMV timing (profiling)
-    int profile = 0;
+    int profile = 0;             // 1: events around every SpMV launch; 2: around those of every 4th chunk
+    unsigned chunk_tick = 0;
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
     // two-level preconditioner: aggregates of a x a x a cells, dense inverse of Ac = P^T A P
@@ -1953,7 +1954,8 @@ static int run_pcg2(PpeSolver<R>* s, const Dev<R>& d, int N, const VA* vals, V* 
     for (int it = 0; it < maxit;) {
         const int len = chunk_len(it, hint, probe);
         const size_t chunk_ev0 = s->ev_used;
-        const bool timed = s->profile && s->ev_used + 2 * (size_t)len <= s->ev.size();
+        const bool timed = s->profile && (s->profile == 1 || (s->chunk_tick++ & 3) == 0) &&
+                           s->ev_used + 2 * (size_t)len <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
             launch_spmv<R, VA, V>(s, vals, p, q, st);
@@ -2046,7 +2048,8 @@ static int run_pcg(PpeSolver<R>* s, int N, const VA* vals, V* x, V* r, V* z, V* 
         // `probe` iterations long and r.r is checked after it.
         const int len = chunk_len(it, hint, probe);
         const size_t chunk_ev0 = s->ev_used;
-        const bool timed = s->profile && s->ev_used + 2 * (size_t)len <= s->ev.size();
+        const bool timed = s->profile && (s->profile == 1 || (s->chunk_tick++ & 3) == 0) &&
+                           s->ev_used + 2 * (size_t)len <= s->ev.size();
         for (int k = 0; k < len; ++k) {
             if (timed) (void)hipEventRecord(s->ev[s->ev_used], st);
             launch_spmv<R, VA, V>(s, vals, pa, q, st);
