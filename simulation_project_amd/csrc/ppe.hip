@@ -695,7 +695,8 @@ __global__ __launch_bounds__(PB) void k_pcg_spmv(int nslices, const int* __restr
 // Columns and the staged window as in k_pcg_spmv (the window holds FP64 values of x); the sum runs
 // entry by entry in either form.
 template <typename VA, bool C16, bool WIN>
-__device__ inline void resid_body(int blk, int N, int nslices, const int* __restrict__ slice_off,
+__device__ __attribute__((always_inline)) inline void resid_body(int blk, int N, int nslices,
+                                                                  const int* __restrict__ slice_off,
                                   const int* __restrict__ slice_w, const int* __restrict__ cols,
                                   const VA* __restrict__ vals, const double* __restrict__ diag,
                                   const double* __restrict__ x, const double* __restrict__ b,

@@ -181,7 +181,8 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_upsweep(const uint32_t* keys, c
 }
 
 template <int RS_ITEMS>
-__device__ inline void rs_downsweep_body(int b, const uint32_t* __restrict__ kin, const int* __restrict__ vin,
+__device__ __attribute__((always_inline)) inline void rs_downsweep_body(int b, const uint32_t* __restrict__ kin,
+                                                                         const int* __restrict__ vin,
                                          uint32_t* __restrict__ kout, int* __restrict__ vout, const int* d_n,
                                          int n_host, int n_cap, int shift, int nbits, const int* __restrict__ hist) {
     constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
