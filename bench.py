@@ -55,7 +55,7 @@ def log(msg):
 def ppe_options(abi, kind, precond, warm):
     solver = abi.MPSR_PCG_FP64 if kind == "fp64" else abi.MPSR_PCG_MIXED
     pre = {"vcycle": abi.MPSR_PRECOND_CELL_VCYCLE, "two": abi.MPSR_PRECOND_TWO_LEVEL}.get(precond, abi.MPSR_PRECOND_JACOBI)
-    return abi.PpeOptions(solver, 1e-8, 20000, warm, 0.0, 0.0, pre, 2, 50)
+    return abi.PpeOptions(solver, 1e-8, 20000, warm, 0.0, 0.0, pre, 2, 200)
 
 
 def run_steps(s, first, count):
@@ -145,7 +145,8 @@ def cpu_baseline(abi, case, sph, n):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200,
+                    help="timed steps; 200 holds one scheduled rebuild of the coarse operator")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--arm-steps", type=int, default=10, help="timed steps of each comparison arm")
     ap.add_argument("--l0", type=float, default=2.012e-3)

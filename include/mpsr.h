@@ -233,12 +233,15 @@ typedef struct mpsr_ppe_options {
                           edge grows while the aggregates outnumber 8192) */
     int coarse_rebuild;/* rebuild the coarse operator every this many solves, or sooner when the
                           iteration count grows by a quarter over the solve that rebuilt it (not
-                          within 10 solves of a rebuild); 0 = 50, 1 = every solve.  In between, the
+                          within 10 solves of a rebuild; a growth rebuild that leaves the
+                          iterations within 10 % of the solve before it stops these until the
+                          next scheduled one); 0 = 200, 1 = every solve.  In between, the
                           last coarse inverse is applied to the current aggregates: the
                           preconditioner stays symmetric positive definite.  With
                           MPSR_PRECOND_CELL_VCYCLE the cell operator is also rebuilt with the
                           aggregates, every 50 solves, and from 5 solves after its build when a
-                          solve takes over 1.2 x the fewest iterations since. */
+                          solve takes over 1.2 x the fewest iterations since (stopped the same way
+                          when such a rebuild does not help). */
     int learned_omega; /* with MPSR_WARM_LEARNED: 1 = also scale the preconditioner's diagonal part
                           by the network's omega (clamped to [0.5, 1.5]), 0 = warm start only */
     double source_relaxation; /* gamma in the source gamma * alpha * max(n* - n0s, 0) / n0s: the share

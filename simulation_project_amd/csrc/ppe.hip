@@ -181,6 +181,11 @@ template <typename R> struct PpeSolver {
     int coarse_rebuilt = 0;      // this solve rebuilt it
     int iters_at_build = 0;      // iterations of the first solve after the last rebuild
     int iters_last = 0;
+    // A rebuild for iteration growth that did not cut the iterations (the system got harder, the
+    // operator was not stale) turns that trigger off until the next scheduled rebuild
+    int growth_off = 0, cell_growth_off = 0;
+    int growth_check = 0, cell_growth_check = 0;   // this solve rebuilt for growth: compare after it
+    int iters_before_growth = 0, iters_before_cell_growth = 0;
     hipEvent_t setup_ev[2] = {nullptr, nullptr};
     double setup_ms = 0.0;
     // learned warm start and preconditioner weights
@@ -1856,9 +1861,13 @@ static int setup_cells(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hipStre
                        s->cell_of_row);
     hipLaunchKernelGGL(k_cell_agg_map, dim3(grid_for(ncell, PB)), dim3(PB), 0, st, ncell, g, s->agg_flag, s->agg_map,
                        s->c_agg);
-    s->cell_rebuilt = s->cell_age < 0 || s->cell_age + 1 >= CELL_REBUILD || s->coarse_rebuilt ||
-                      (s->iters_at_cell > 0 && s->cell_age + 1 >= CELL_MIN_AGE &&
-                       5 * s->iters_last > 6 * s->iters_at_cell);
+    const bool cell_due = s->cell_age < 0 || s->cell_age + 1 >= CELL_REBUILD || s->coarse_rebuilt;
+    const bool cell_grew = !s->cell_growth_off && s->iters_at_cell > 0 && s->cell_age + 1 >= CELL_MIN_AGE &&
+                           5 * s->iters_last > 6 * s->iters_at_cell;
+    s->cell_rebuilt = cell_due || cell_grew;
+    if (cell_due) s->cell_growth_off = 0;
+    s->cell_growth_check = !cell_due && cell_grew;
+    if (s->cell_growth_check) s->iters_before_cell_growth = s->iters_last;
     if (s->cell_rebuilt) {
         hipLaunchKernelGGL((k_cell_op<VA>), dim3(ncell), dim3(64), 0, st, ncell, g, d.cell_start, d.cell_end, s->dir,
                            s->cnt, s->pos, s->slice_off, s->cols, vals, s->diag, s->cxyz, s->c_op, s->c_dinv);
@@ -2283,10 +2292,14 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     s->setup_ms = 0.0;
     const bool coarse_pre = opt.precond == MPSR_PRECOND_TWO_LEVEL || opt.precond == MPSR_PRECOND_CELL_VCYCLE;
     if (coarse_pre && bb > 0 && rr > target) {
-        const int every = opt.coarse_rebuild > 0 ? opt.coarse_rebuild : 50;
-        const bool rebuild = !s->coarse_valid || s->coarse_age + 1 >= every ||
-                             (s->iters_at_build > 0 && s->coarse_age + 1 >= GROWTH_MIN_AGE &&
-                              4 * s->iters_last > 5 * s->iters_at_build);
+        const int every = opt.coarse_rebuild > 0 ? opt.coarse_rebuild : 200;
+        const bool due = !s->coarse_valid || s->coarse_age + 1 >= every;
+        const bool grew = !s->growth_off && s->iters_at_build > 0 && s->coarse_age + 1 >= GROWTH_MIN_AGE &&
+                          4 * s->iters_last > 5 * s->iters_at_build;
+        const bool rebuild = due || grew;
+        if (due) s->growth_off = 0;
+        s->growth_check = !due && grew;
+        if (s->growth_check) s->iters_before_growth = s->iters_last;
         if (rebuild) {
             if (profile) (void)hipEventRecord(s->setup_ev[0], st);
             s->two_level_ok = s->mixed ? setup_two_level<R, float>(s, d, s->vals32, st)
@@ -2435,6 +2448,9 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     }
     s->relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
     if (s->coarse_rebuilt && s->two_level_ok) s->iters_at_build = s->iters;
+    if (s->growth_check && 10 * s->iters >= 9 * s->iters_before_growth) s->growth_off = 1;
+    if (s->cell_growth_check && 10 * s->iters >= 9 * s->iters_before_cell_growth) s->cell_growth_off = 1;
+    s->growth_check = s->cell_growth_check = 0;
     if (s->vcycle) s->iters_at_cell = s->cell_rebuilt ? s->iters : std::min(s->iters_at_cell, s->iters);
     s->iters_last = s->iters;
     if (profile && s->coarse_rebuilt) {
