@@ -1204,7 +1204,10 @@ template <typename R> struct BatchGroup {
         const SortJob* jobs = nullptr;
         WorkList tiles, singles;
     };
-    WorkList wp, wt64, wt256, wpairs, wone, wdrag, wtherm, wturb;
+    WorkList wp, wt64, wt256, wpairs, wone, wdrag, wtherm, wturb, wc256;
+    int2* walk_items = nullptr;        // (bucket, cell) of the cell walks, one per cell of the buckets
+    int* walk_n = nullptr;
+    size_t walk_cap = 0;
     std::vector<Pass> psort, tsort;
     const FillJob* fill_bin = nullptr;
     const FillJob* fill_tri = nullptr;
@@ -1218,6 +1221,8 @@ template <typename R> struct BatchGroup {
     ~BatchGroup() {
         if (devs) (void)hipFree(devs);
         if (tab) (void)hipFree(tab);
+        if (walk_items) (void)hipFree(walk_items);
+        if (walk_n) (void)hipFree(walk_n);
         ppe_batch_destroy(pb);
     }
     bool has_ppe() const {
@@ -1282,8 +1287,9 @@ template <typename R> struct BatchGroup {
             int n;
         };
         auto items_of = [&](const std::vector<int2>& v) { return Ref{put(v.data(), v.size() * sizeof(int2)), (int)v.size()}; };
-        std::vector<int2> p, t64, t256, pairs, one, drag, therm, turb, peos;
+        std::vector<int2> p, t64, t256, pairs, one, drag, therm, turb, peos, c256;
         const int n = (int)eng.size();
+        size_t ncells = 0;
         for (int k = 0; k < n; ++k) {
             const Dev<R>& d = host[k];
             const int nb = (int)grid_for(d.N, block);
@@ -1299,11 +1305,15 @@ template <typename R> struct BatchGroup {
                 for (int j = 0; j < (int)grid_for(d.ntri, 256); ++j) t256.push_back(make_int2(k, j));
                 for (int j = 0; j < (int)grid_for(d.pair_cap, 256); ++j) pairs.push_back(make_int2(k, j));
             }
+            if (d.N > 0) {
+                for (int j = 0; j < (int)grid_for(d.c.ncell, 256); ++j) c256.push_back(make_int2(k, j));
+                ncells += (size_t)d.c.ncell;
+            }
             one.push_back(make_int2(k, 0));
         }
         const Ref rp = items_of(p), r64 = items_of(t64), r256 = items_of(t256), rpairs = items_of(pairs),
                   rone = items_of(one), rdrag = items_of(drag), rtherm = items_of(therm), rturb = items_of(turb),
-                  rpeos = items_of(peos);
+                  rpeos = items_of(peos), rc256 = items_of(c256);
         // fills: cell ranges of the particle binning; triangle counts per cell (and the pair count of
         // a bucket without triangles)
         std::vector<FillJob> fb, ft;
@@ -1397,7 +1407,15 @@ template <typename R> struct BatchGroup {
         HIPC(hipMemcpy(tab, img.data(), img.size(), hipMemcpyHostToDevice));
         auto wl = [&](Ref r) { return WorkList{reinterpret_cast<const int2*>(tab + r.off), r.n}; };
         wp = wl(rp); wt64 = wl(r64); wt256 = wl(r256); wpairs = wl(rpairs); wone = wl(rone);
-        wdrag = wl(rdrag); wtherm = wl(rtherm); wturb = wl(rturb); wp_eos = wl(rpeos);
+        wdrag = wl(rdrag); wtherm = wl(rtherm); wturb = wl(rturb); wp_eos = wl(rpeos); wc256 = wl(rc256);
+        if (ncells > walk_cap) {
+            if (walk_items) (void)hipFree(walk_items);
+            walk_items = nullptr;
+            walk_cap = 0;
+            HIPC(hipMalloc((void**)&walk_items, sizeof(int2) * ncells));
+            walk_cap = ncells;
+        }
+        if (!walk_n) HIPC(hipMalloc((void**)&walk_n, sizeof(int)));
         fill_bin = reinterpret_cast<const FillJob*>(tab + ofb);
         fill_tri = reinterpret_cast<const FillJob*>(tab + oft);
         fill_bin_w = wl(rfbw);
@@ -1447,6 +1465,7 @@ template <typename R> struct BatchGroup {
         exclusive_scan_batched(scan_flag, scan_flag_t, scan_flag_1, st);
         launch_pairs_b(b, 1, wpairs, st);
         launch_pairs_b(b, 2, wpairs, st);
+        launch_cell_walks_b(b, wc256, walk_items, walk_n, st);
     }
     // one step of every bucket in the order of Engine::one_step, the rotation row of each bucket
     // from its device step counter, which the step then advances.  head: up to the pressure (with

@@ -139,7 +139,7 @@ template <typename R> struct PState {
 // Triangle record on the device: three vertices, the stored normal, the model, the model's
 // drag flag.  64 B for float, 112 B for double.
 // length of the per-cell triangle walks (nearest_triangle_listed)
-static constexpr int kStencilTris = 32;
+static constexpr int kStencilTris = 256;
 
 template <typename R> struct alignas(16) TriD {
     R v[9];
@@ -524,6 +524,10 @@ template <typename R> void launch_tri_count_b(const BatchLaunch<R>& b, WorkList 
 template <typename R> void launch_tri_fill_b(const BatchLaunch<R>& b, WorkList t64, hipStream_t st);
 // the pair kernels of launch_tri_bin after the pair sort; `pass` = 0 flags, 1 compacts, 2 ranges
 template <typename R> void launch_pairs_b(const BatchLaunch<R>& b, int pass, WorkList pairs, hipStream_t st);
+// each bucket's cell walks (c256: (bucket, block of 256 cells) of the buckets with particles);
+// work holds one int2 per such cell, nwork one int
+template <typename R>
+void launch_cell_walks_b(const BatchLaunch<R>& b, WorkList c256, int2* work, int* nwork, hipStream_t st);
 template <typename R> void launch_stl_distance_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
 template <typename R> void launch_drag_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st);
 template <typename R> void launch_translate_b(const BatchLaunch<R>& b, WorkList t256, hipStream_t st);

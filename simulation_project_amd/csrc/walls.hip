@@ -162,17 +162,15 @@ template <typename R> void launch_tri_count_total(Dev<R>& d, hipStream_t st) {
 }
 
 // The walk of every cell that holds particles, for nearest_triangle_listed, in two passes.
-// k_cell_walk_cells: one thread per cell reads the 27 list lengths; a cell without particles gets
-// -1, one whose stencil lists are all empty gets 0, the others go to a work list (in any order:
-// each cell's walk depends only on its own lists).
-__global__ __launch_bounds__(256) void k_cell_walk_cells(int ncell, int gy, int gz, const int* __restrict__ cs,
-                                                         const int* __restrict__ tcount, int* work, int* nwork,
-                                                         int* ucount) {
-    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= ncell) return;
+// First, per cell: a cell without particles gets -1, one whose stencil lists are all empty gets 0;
+// true for the others, which go to a work list (in any order: each cell's walk depends only on
+// its own lists).
+__device__ inline bool cell_walk_needed(long long cell, int ncell, int gy, int gz, const int* __restrict__ cs,
+                                        const int* __restrict__ tcount, int* ucount) {
+    if (cell >= ncell) return false;
     if (cs[cell] < 0) {
         ucount[cell] = -1;
-        return;
+        return false;
     }
     const long long gyz = (long long)gy * gz;
     const int ci = (int)(cell / gyz), cj = (int)((cell / gz) % gy), ck = (int)(cell % gz);
@@ -184,74 +182,90 @@ __global__ __launch_bounds__(256) void k_cell_walk_cells(int ncell, int gy, int 
     }
     if (total == 0) {
         ucount[cell] = 0;
-        return;
+        return false;
     }
-    work[atomicAdd(nwork, 1)] = (int)cell;
+    return true;
 }
-// k_cell_walks: one wave per listed cell.  Lane s < 27 reads stencil cell s's list (z outer, x
-// inner); the ids are then taken 64 at a time in walk order, one per lane, and an id is kept when
-// neither the ids kept so far (LDS) nor an earlier lane of the same 64 holds it: its first meeting.
-// More than kStencilTris kept ids give -1 (the full walk).  Waves loop over the work list.
+__global__ __launch_bounds__(256) void k_cell_walk_cells(int ncell, int gy, int gz, const int* __restrict__ cs,
+                                                         const int* __restrict__ tcount, int* work, int* nwork,
+                                                         int* ucount) {
+    const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell_walk_needed(cell, ncell, gy, gz, cs, tcount, ucount)) work[atomicAdd(nwork, 1)] = (int)cell;
+}
+// Then one wave per listed cell (a workgroup of 64).  Lane s < 27 reads stencil cell s's list (z
+// outer, x inner); the ids are then taken 64 at a time in walk order, one per lane, and an id is
+// kept when neither the ids kept so far (LDS, read 4 at a time) nor an earlier lane of the same 64
+// holds it: its first meeting.  More than kStencilTris kept ids give -1 (the full walk).
+struct WalkLds {
+    int off[28], beg[28];
+    __align__(16) int kept[kStencilTris];
+};
+__device__ inline void cell_walk_wave(long long cell, int ncell, int gy, int gz, const int* __restrict__ tstart,
+                                      const int* __restrict__ tcount, const int* __restrict__ tids, int* ulist,
+                                      int* ucount, WalkLds& L) {
+    const int lane = threadIdx.x;
+    const long long gyz = (long long)gy * gz;
+    const int ci = (int)(cell / gyz), cj = (int)((cell / gz) % gy), ck = (int)(cell % gz);
+    int cnt = 0, b0 = 0;
+    if (lane < 27) {
+        const long long m = gyz * (ci + lane % 3 - 1) + (long long)gz * (cj + (lane / 3) % 3 - 1) + (ck + lane / 9 - 1);
+        if (m >= 0 && m < ncell) {
+            cnt = tcount[m];
+            b0 = tstart[m];
+        }
+    }
+    const int inc = wave_inclusive_scan(cnt);
+    const int total = __shfl(inc, 63, 64);
+    if (lane < 27) {
+        L.off[lane] = inc - cnt;
+        L.beg[lane] = b0;
+    }
+    if (lane == 0) L.off[27] = total;
+    for (int q = lane; q < kStencilTris; q += 64) L.kept[q] = -1;
+    __syncthreads();
+    int n = 0;
+    bool over = false;
+    for (int base = 0; base < total && !over; base += 64) {
+        const int f = base + lane;
+        int tid = -1;
+        if (f < total) {
+            int s = 0;   // the stencil cell of walk position f: off[s] <= f < off[s + 1]
+            for (int h = 16; h > 0; h >>= 1)
+                if (s + h < 27 && L.off[s + h] <= f) s += h;
+            tid = tids[L.beg[s] + f - L.off[s]];
+        }
+        bool first = f < total;
+        for (int q = 0; q < n; q += 4) {   // unused slots hold -1, never an id
+            const int4 k4 = *reinterpret_cast<const int4*>(&L.kept[q]);
+            first = first && k4.x != tid && k4.y != tid && k4.z != tid && k4.w != tid;
+        }
+        for (int j = 0; j < 63; ++j) {
+            const int tj = __shfl(tid, j, 64);
+            if (j < lane && tj == tid) first = false;
+        }
+        const unsigned long long mask = __ballot(first);
+        const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        const int add = __popcll(mask);
+        __syncthreads();
+        if (first && n + rank < kStencilTris) {
+            L.kept[n + rank] = tid;
+            ulist[cell * kStencilTris + n + rank] = tid;
+        }
+        __syncthreads();
+        if (n + add > kStencilTris) over = true;
+        n = min(n + add, kStencilTris);
+    }
+    if (lane == 0) ucount[cell] = over ? -1 : n;
+    __syncthreads();
+}
 __global__ __launch_bounds__(64) void k_cell_walks(int ncell, int gy, int gz, const int* __restrict__ work,
                                                    const int* __restrict__ nwork, const int* __restrict__ tstart,
                                                    const int* __restrict__ tcount, const int* __restrict__ tids,
                                                    int* ulist, int* ucount) {
-    __shared__ int off[28], beg[27], kept[kStencilTris];
-    const int lane = threadIdx.x;
+    __shared__ WalkLds L;
     const int nw = *nwork;
-    const long long gyz = (long long)gy * gz;
-    for (int w = blockIdx.x; w < nw; w += gridDim.x) {
-        const long long cell = work[w];
-        const int ci = (int)(cell / gyz), cj = (int)((cell / gz) % gy), ck = (int)(cell % gz);
-        int cnt = 0, b0 = 0;
-        if (lane < 27) {
-            const long long m = gyz * (ci + lane % 3 - 1) + (long long)gz * (cj + (lane / 3) % 3 - 1) +
-                                (ck + lane / 9 - 1);
-            if (m >= 0 && m < ncell) {
-                cnt = tcount[m];
-                b0 = tstart[m];
-            }
-        }
-        const int inc = wave_inclusive_scan(cnt);
-        const int total = __shfl(inc, 63, 64);
-        if (lane < 27) {
-            off[lane] = inc - cnt;
-            beg[lane] = b0;
-        }
-        if (lane == 0) off[27] = total;
-        __syncthreads();
-        int n = 0;
-        bool over = false;
-        for (int base = 0; base < total && !over; base += 64) {
-            const int f = base + lane;
-            int tid = -1;
-            if (f < total) {
-                int s = 0;   // the stencil cell of walk position f: off[s] <= f < off[s + 1]
-                for (int h = 16; h > 0; h >>= 1)
-                    if (s + h < 27 && off[s + h] <= f) s += h;
-                tid = tids[beg[s] + f - off[s]];
-            }
-            bool first = f < total;
-            for (int q = 0; q < n; ++q) first = first && kept[q] != tid;
-            for (int j = 0; j < 63; ++j) {
-                const int tj = __shfl(tid, j, 64);
-                if (j < lane && tj == tid) first = false;
-            }
-            const unsigned long long mask = __ballot(first);
-            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-            const int add = __popcll(mask);
-            __syncthreads();
-            if (first && n + rank < kStencilTris) {
-                kept[n + rank] = tid;
-                ulist[cell * kStencilTris + n + rank] = tid;
-            }
-            __syncthreads();
-            if (n + add > kStencilTris) over = true;
-            n = min(n + add, kStencilTris);
-        }
-        if (lane == 0) ucount[cell] = over ? -1 : n;
-        __syncthreads();
-    }
+    for (int w = blockIdx.x; w < nw; w += gridDim.x)
+        cell_walk_wave(work[w], ncell, gy, gz, tstart, tcount, tids, ulist, ucount, L);
 }
 
 template <typename R> static void launch_cell_stencil_tris(const Dev<R>& d, hipStream_t st) {
@@ -503,7 +517,7 @@ template <typename R> struct OpStl {
     __device__ void operator()(int k, int b) const {
         const Dev<R>& d = dv[k];
         stl_distance_body<R>(b, d.N, d.c, d.spos, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.stl_mag,
-                             d.stl_dir, d.stl_id, nullptr, nullptr);
+                             d.stl_dir, d.stl_id, d.cell_utri, d.cell_utri_n);
     }
 };
 template <typename R> struct OpDrag {
@@ -511,9 +525,34 @@ template <typename R> struct OpDrag {
     __device__ void operator()(int k, int b) const {
         const Dev<R>& d = dv[k];
         drag_body<R>(b, d.N, d.c, d.perm, d.in, d.out, d.spos, d.svel, d.cell_tri_start, d.cell_tri_count, d.uniq_tri,
-                     d.tris, d.models, d.drag, d.drag_acc, d.drag_temp, nullptr, nullptr);
+                     d.tris, d.models, d.drag, d.drag_acc, d.drag_temp, d.cell_utri, d.cell_utri_n);
     }
 };
+// the walks of the batched step: (bucket, block of 256 cells) items sort the cells as
+// k_cell_walk_cells does and append (bucket, cell) to one list; one wave per listed cell then
+template <typename R> struct OpCellWalkCells {
+    const Dev<R>* dv;
+    int2* work;
+    int* nwork;
+    __device__ void operator()(int k, int b) const {
+        const Dev<R>& d = dv[k];
+        const long long cell = (long long)b * 256 + threadIdx.x;
+        if (cell_walk_needed(cell, (int)d.c.ncell, d.c.gy, d.c.gz, d.cell_start, d.cell_tri_count, d.cell_utri_n))
+            work[atomicAdd(nwork, 1)] = make_int2(k, (int)cell);
+    }
+};
+template <typename R>
+__global__ __launch_bounds__(64) void k_cell_walks_b(const Dev<R>* __restrict__ dv, const int2* __restrict__ work,
+                                                     const int* __restrict__ nwork) {
+    __shared__ WalkLds L;
+    const int nw = *nwork;
+    for (int w = blockIdx.x; w < nw; w += gridDim.x) {
+        const int2 it = work[w];
+        const Dev<R>& d = dv[it.x];
+        cell_walk_wave(it.y, (int)d.c.ncell, d.c.gy, d.c.gz, d.cell_tri_start, d.cell_tri_count, d.uniq_tri,
+                       d.cell_utri, d.cell_utri_n, L);
+    }
+}
 template <typename R> struct OpTranslate {
     const Dev<R>* dv;
     __device__ void operator()(int k, int b) const {
@@ -544,6 +583,12 @@ template <typename R> void launch_tri_fill_b(const BatchLaunch<R>& b, WorkList t
 template <typename R> void launch_pairs_b(const BatchLaunch<R>& b, int pass, WorkList pairs, hipStream_t st) {
     launch_batched<256>(OpPairs<R>{b.devs, pass}, pairs, 256, st);
 }
+template <typename R>
+void launch_cell_walks_b(const BatchLaunch<R>& b, WorkList c256, int2* work, int* nwork, hipStream_t st) {
+    fill_int(nwork, 0, 1, st);
+    launch_batched<256>(OpCellWalkCells<R>{b.devs, work, nwork}, c256, 256, st);
+    hipLaunchKernelGGL(k_cell_walks_b<R>, dim3(2048), dim3(64), 0, st, b.devs, work, nwork);
+}
 template <typename R> void launch_stl_distance_b(const BatchLaunch<R>& b, WorkList p, hipStream_t st) {
     launch_batched<kMaxBlock>(OpStl<R>{b.devs}, p, b.block, st);
 }
@@ -570,6 +615,7 @@ template <typename R> void launch_step_advance_b(const BatchLaunch<R>& b, WorkLi
     template void launch_tri_count_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);           \
     template void launch_tri_fill_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);            \
     template void launch_pairs_b<R>(const BatchLaunch<R>&, int, WorkList, hipStream_t);          \
+    template void launch_cell_walks_b<R>(const BatchLaunch<R>&, WorkList, int2*, int*, hipStream_t); \
     template void launch_stl_distance_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);        \
     template void launch_drag_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);                \
     template void launch_translate_b<R>(const BatchLaunch<R>&, WorkList, hipStream_t);           \
