@@ -577,6 +577,9 @@ template <typename R> struct Engine : EngineBase {
         if (pressure_mode == MPSR_PRESSURE_PPE) {
             ppe = ppe_create<R>(d, P, ppe_opt, n0_lap, lambda_lap, n0_src);
             if (!ppe) return fail(MPSR_MEMERR, "PPE solver allocation failed");
+        } else {
+            ALLOC(d.gnb_q, ((size_t)std::max(cap, 1) + 63) / 64 * 64 * kGradWidth);
+            ALLOC(d.gnb_n, std::max(cap, 1));
         }
         HIPC(hipDeviceSynchronize());
         return MPSR_NOERR;
@@ -614,6 +617,7 @@ template <typename R> struct Engine : EngineBase {
     double call_relres_max = 0.0;      // ... and its worst final relative residual
     int bin_epoch = 0;                 // binnings so far
     int nb_epoch = -1;                 // the binning the PPE neighbour list was written for
+    int gnb_epoch = -1;                // ... and the EOS pressure's gradient list
     void do_bin() {
         ++bin_epoch;
         mark_begin(PH_BIN);
@@ -686,14 +690,19 @@ template <typename R> struct Engine : EngineBase {
         }
         mark_begin(PH_PND);
         launch_pnd(d, 1, st);
+        gnb_epoch = bin_epoch;
         mark_end();
         return MPSR_NOERR;
     }
     void do_gradient() {
         mark_begin(PH_GRAD);
-        // the PPE path's PND pass listed every row's neighbours; the list serves while the binning
-        // it was written for is the current one
-        launch_gradient(d, st, ppe && nb_epoch == bin_epoch ? ppe_neighbors(ppe) : NbList{});
+        // the PND pass listed every row's neighbours (the PPE path's over the Laplacian radius, the
+        // EOS path's over the gradient radius); a list serves while the binning it was written for
+        // is the current one
+        NbList nb{};
+        if (ppe && nb_epoch == bin_epoch) nb = ppe_neighbors(ppe);
+        else if (d.gnb_q && gnb_epoch == bin_epoch) nb = NbList{d.gnb_q, d.gnb_n, kGradWidth};
+        launch_gradient(d, st, nb);
         mark_end();
     }
     mpsr_status upload_rot(int first, int count) {
@@ -967,7 +976,10 @@ template <typename R> struct Engine : EngineBase {
             case 3: if (d.ndrag > 0) do_drag(); break;
             case 4: do_explicit(); break;
             case 5: do_relocate(); break;
-            case 6: launch_pnd(d, 1, st); break;
+            case 6:
+                launch_pnd(d, 1, st);
+                gnb_epoch = bin_epoch;
+                break;
             case 7:
                 if (!ppe) {
                     ppe = ppe_create<R>(d, P, ppe_opt, n0_lap, lambda_lap, n0_src);
@@ -1262,6 +1274,9 @@ template <typename R> struct BatchGroup {
             x.step_ctr = e->step_ctr_buf;
             x.step_base = e->step_base_buf;
             x.thermal_step = e->P.ThermalStep > 0 ? e->P.ThermalStep : 1;
+            // the corrector walks the list of the step's own PND pass
+            if (e->pressure_mode == MPSR_PRESSURE_PPE) x.nbl = e->ppe ? ppe_neighbors(e->ppe) : NbList{};
+            else x.nbl = d.gnb_q ? NbList{d.gnb_q, d.gnb_n, kGradWidth} : NbList{};
             host[k] = x;
         }
         if (!devs) HIPC(hipMalloc((void**)&devs, sizeof(Dev<R>) * std::max<size_t>(1, eng.size())));

@@ -361,6 +361,18 @@ namespace mpsr {
 // Largest block of the per-particle kernels; the launch block (64-512) is a runtime choice.
 static constexpr int kMaxBlock = 512;
 
+// A per-row neighbour list in the SELL-64/4 layout (sorted index t): row t's entries at
+// sell_at(64 * width * (t / 64), t % 64, k), n[t] of them (-1: the row was skipped; above width:
+// only the first width are listed).
+struct NbList {
+    const int* q = nullptr;
+    const int* n = nullptr;
+    int width = 0;
+};
+// gradient-radius neighbours listed per row by the PND pass of the EOS pressure (about 32 on the
+// initial lattice at 2.1 l0; rows with more walk the stencil)
+static constexpr int kGradWidth = 64;
+
 template <typename R> struct Dev {
     int N = 0, cap = 0;
     int block = 256;                  // threads per block of the per-particle kernels (64..512)
@@ -417,6 +429,11 @@ template <typename R> struct Dev {
     R* turb_k = nullptr;              // [cap] slot order: turbulent kinetic energy
     NegModel<R>* negm = nullptr;      // [nmodel] negative-pressure fields
     R* drag_temp = nullptr;
+    // EOS pressure: the PND pass lists each row's gradient-radius neighbours in walk order for the
+    // corrector (SELL-64/4, kGradWidth per row)
+    int* gnb_q = nullptr;             // [cap rounded up to 64, * kGradWidth]
+    int* gnb_n = nullptr;             // [cap]
+    NbList nbl{};                     // batched step: the list the corrector walks (none: the stencil)
     // radix-sort ping-pong buffers: one set for the particle keys (the sorted keys and the
     // permutation may end up in them and must live until the next binning), one for the
     // triangle pairs
@@ -439,11 +456,6 @@ void launch_pnd_ppe(const Dev<R>& d, int* active, int* cnt, double* diag, int* d
 // The Laplacian-radius neighbour list the PPE path's PND pass writes after the second binning:
 // per sorted row its first `width` neighbours in the canonical order (SELL-64/4 by sorted index)
 // and n[t] = the number of neighbours (all of them, listed or not), -1 for a row the pass skipped
-struct NbList {
-    const int* q = nullptr;
-    const int* n = nullptr;
-    int width = 0;
-};
 // The corrector; with a neighbour list of the same binning, rows whose neighbours are all listed
 // walk the list instead of the 27-cell stencil (the same neighbours in the same order)
 template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st, NbList nb = NbList{});

@@ -263,6 +263,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_relocate(int N, Consts<R> c, PSta
 }
 
 // ---------------------------------------------------------------------------------------------
+// the last, partial group of a row's neighbour list, padded with its last index
+__device__ inline void list_tail(int* nbq, int lbase, int lane, int kept, int q1, int q2, int q3) {
+    if (kept & 3) {
+        const int used = kept & 3;
+        int qq[4] = {q3, q3, q3, q3};
+        if (used == 2) qq[0] = q2;
+        if (used == 3) { qq[0] = q1; qq[1] = q2; }
+        quad_store(nbq + sell_at(lbase, lane, kept - used), qq[0], qq[1], qq[2], qq[3]);
+    }
+}
 // PND (and the EOS pressure when eos != 0).  spres[t] receives the slot's out-pressure after the
 // kernel, which is what the gradient kernel reads for neighbours (Kern :860).
 //
@@ -275,6 +285,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_relocate(int N, Consts<R> c, PSta
 // so both give the same bits.  The sorted indices of the first nb_width Laplacian-radius neighbours
 // go to a per-row list (SELL-64/4 by sorted index) that the fill and the corrector read instead of
 // scanning again; the fill forms their weights again from the positions, with the same operations.
+// Without PPE and with nbq given, the gradient-radius neighbours go to the list instead (nb_width
+// of them, nbn[t] their count, -1 for a skipped row), in walk order, for the corrector.
 template <typename R, bool PPE>
 __device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int* __restrict__ perm, PState<R> in,
                                 PState<R> out, const R4<R>* __restrict__ spos, R* __restrict__ spres,
@@ -295,6 +307,8 @@ __device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int*
             rcnt[t] = 0;
             rdiag[t] = 1.0;
             rdir[t] = 1;
+            nbn[t] = -1;
+        } else if (nbn) {
             nbn[t] = -1;
         }
     };
@@ -343,15 +357,25 @@ __device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int*
                 ++nl;
             }
         });
-        const int kept = nl < nb_width ? nl : nb_width;
-        if (kept & 3) {   // the last, partial group of the list
-            const int used = kept & 3;
-            int qq[4] = {q3, q3, q3, q3};
-            if (used == 2) qq[0] = q2;
-            if (used == 3) { qq[0] = q1; qq[1] = q2; }
-            const int e = sell_at(lbase, lane, kept - used);
-            quad_store(nbq + e, qq[0], qq[1], qq[2], qq[3]);
-        }
+        list_tail(nbq, lbase, lane, nl < nb_width ? nl : nb_width, q1, q2, q3);
+    } else if (nbq) {
+        const int lbase = (t >> 6) * 64 * nb_width, lane = t & 63;
+        int q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+        for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
+            const R4<R> Q = spos[q];
+            R d = dist3(x, y, z, Q.x, Q.y, Q.z);
+            if (d < re) {
+                if (d < mind) d = mind;
+                dens = dens + weight(re, d);
+                if (nl < nb_width) {
+                    q0 = q1; q1 = q2; q2 = q3; q3 = q;
+                    if ((nl & 3) == 3) quad_store(nbq + sell_at(lbase, lane, nl - 3), q0, q1, q2, q3);
+                }
+                ++nl;
+            }
+        });
+        list_tail(nbq, lbase, lane, nl < nb_width ? nl : nb_width, q1, q2, q3);
+        nbn[t] = nl;
     } else {
         for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
             const R4<R> Q = spos[q];
@@ -437,8 +461,9 @@ __device__ inline void gradient_body(int blk, int N, Consts<R> c, const int* __r
     };
     const int nn = nb.n ? nb.n[t] : -1;
     if (nn >= 0 && nn <= nb.width) {
-        // every Laplacian-radius neighbour is listed, in the stencil walk's order; the gradient
-        // radius is the smaller one, so the same distances pick the same neighbours in the same order.
+        // every neighbour within the gradient radius is listed (by the PPE path's PND pass within
+        // the Laplacian radius, which is not smaller), in the stencil walk's order, so the same
+        // distances pick the same neighbours in the same order.
         // A group's records (position and pressure of its 4 entries; the last group is padded with
         // a valid index) are loaded together with the next group's indices and waited for once.
         const int lbase = (t >> 6) * 64 * nb.width, lane = t & 63;
@@ -769,7 +794,7 @@ template <typename R> void launch_pnd(const Dev<R>& d, int eos, hipStream_t st) 
     if (d.N <= 0) return;
     hipLaunchKernelGGL((k_pnd<R, false>), dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, eos, d.perm, d.in, d.out,
                        d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag, d.stl_id, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, nullptr, 0);
+                       nullptr, nullptr, d.gnb_q, d.gnb_n, kGradWidth);
 }
 template <typename R> void launch_gradient(const Dev<R>& d, hipStream_t st, NbList nb) {
     if (d.N <= 0) return;
@@ -830,7 +855,7 @@ template <typename R> struct OpPndEos {
     __device__ void operator()(int k, int b) const {
         const Dev<R>& d = dv[k];
         pnd_body<R, false>(b, d.N, d.c, 1, d.perm, d.in, d.out, d.spos, d.spres, d.cell_start, d.cell_end, d.stl_mag,
-                           d.stl_id, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+                           d.stl_id, nullptr, nullptr, nullptr, nullptr, nullptr, d.gnb_q, d.gnb_n, kGradWidth);
     }
 };
 template <typename R> struct OpGradient {
@@ -839,7 +864,7 @@ template <typename R> struct OpGradient {
         const Dev<R>& d = dv[k];
         gradient_body<R>(b, d.N, d.c, d.perm, d.in, d.out, d.spos, d.svel, d.spres, d.cell_start, d.cell_end,
                          d.stl_mag, d.stl_dir, d.stl_id, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris,
-                         d.negm, NbList{});
+                         d.negm, d.nbl);
     }
 };
 template <typename R> struct OpThermal {
