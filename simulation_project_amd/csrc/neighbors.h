@@ -71,11 +71,36 @@ __device__ inline R dist3(R x, R y, R z, R qx, R qy, R qz) {
     return build_sqrt((x - qx) * (x - qx) + (y - qy) * (y - qy) + (z - qz) * (z - qz));
 }
 
+// one stencil cell's candidates t in [a, b), t != self, as f(t, spos[t]) in order, the records
+// loaded in groups of G (then 4) issued together, so G gathers are in flight per lane instead of one
+template <int G, typename R, typename F>
+__device__ inline void cell_records(const R4<R>* __restrict__ spos, int a, int b, int self, F& f) {
+    int t = a;
+    if constexpr (G == 8) {
+        for (; t + 8 <= b; t += 8) {
+            R4<R> Q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) Q[u] = spos[t + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (t + u != self) f(t + u, Q[u]);
+        }
+    }
+    for (; t + 4 <= b; t += 4) {
+        const R4<R> Q0 = spos[t], Q1 = spos[t + 1], Q2 = spos[t + 2], Q3 = spos[t + 3];
+        if (t != self) f(t, Q0);
+        if (t + 1 != self) f(t + 1, Q1);
+        if (t + 2 != self) f(t + 2, Q2);
+        if (t + 3 != self) f(t + 3, Q3);
+    }
+    for (; t < b; ++t)
+        if (t != self) f(t, spos[t]);
+}
+
 // for_candidates with the records loaded ahead: every candidate t != self of the 27-cell stencil in
-// the canonical order, as f(t, spos[t]).  Inside a cell the records come in groups of G (then 4)
-// loads issued together, so G gathers are in flight per lane instead of one; the calls keep the
-// order.  G = 8 suits a kernel whose registers allow it (the PND pass: 6 % faster than 4; the
-// predictor loses occupancy with it and keeps 4).
+// the canonical order, as f(t, spos[t]) (cell_records per cell).  G = 8 suits a kernel whose
+// registers allow it (the PND pass: 6 % faster than 4; the predictor loses occupancy with it and
+// keeps 4).
 template <int G = 4, typename R, typename F>
 __device__ inline void for_candidates_rec(const Consts<R>& c, const int* __restrict__ cs, const int* __restrict__ ce,
                                           const R4<R>* __restrict__ spos, R x, R y, R z, int self, F&& f) {
@@ -90,28 +115,40 @@ __device__ inline void for_candidates_rec(const Consts<R>& c, const int* __restr
                 if (m < 0 || m >= c.ncell) continue;
                 const int a = cs[m];
                 if (a < 0) continue;
-                const int b = ce[m];
-                int t = a;
-                if constexpr (G == 8) {
-                    for (; t + 8 <= b; t += 8) {
-                        R4<R> Q[8];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) Q[u] = spos[t + u];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u)
-                            if (t + u != self) f(t + u, Q[u]);
-                    }
-                }
-                for (; t + 4 <= b; t += 4) {
-                    const R4<R> Q0 = spos[t], Q1 = spos[t + 1], Q2 = spos[t + 2], Q3 = spos[t + 3];
-                    if (t != self) f(t, Q0);
-                    if (t + 1 != self) f(t + 1, Q1);
-                    if (t + 2 != self) f(t + 2, Q2);
-                    if (t + 3 != self) f(t + 3, Q3);
-                }
-                for (; t < b; ++t)
-                    if (t != self) f(t, spos[t]);
+                cell_records<G, R>(spos, a, ce[m], self, f);
             }
+}
+
+// for_candidates_within with the records loaded ahead as in for_candidates_rec: the stencil cells
+// within `reach` (the same test, so the same cells in the same order), f(t, spos[t])
+template <int G = 4, typename R, typename F>
+__device__ inline void for_candidates_rec_within(const Consts<R>& c, const int* __restrict__ cs,
+                                                 const int* __restrict__ ce, const R4<R>* __restrict__ spos, R x,
+                                                 R y, R z, int self, R reach, F&& f) {
+    int ci, cj, ck;
+    if (!cell_coords(c, x, y, z, ci, cj, ck)) return;
+    const long long cid = cell_of(c, ci, cj, ck);
+    if (cid < 0 || cid >= c.ncell) return;
+    const R m = reach * (R)1.01;
+    const R m2 = m * m;
+    const R lx = c.bminx + (R)ci * c.cell, ly = c.bminy + (R)cj * c.cell, lz = c.bminz + (R)ck * c.cell;
+    const R xm = x - lx, xp = lx + c.cell - x, ym = y - ly, yp = ly + c.cell - y, zm = z - lz, zp = lz + c.cell - z;
+    const R xm2 = xm * xm, xp2 = xp * xp, ym2 = ym * ym, yp2 = yp * yp, zm2 = zm * zm, zp2 = zp * zp;
+    for (int k = -1; k <= 1; ++k) {
+        const R dz2 = k < 0 ? zm2 : (k > 0 ? zp2 : (R)0);
+        for (int j = -1; j <= 1; ++j) {
+            const R dyz2 = dz2 + (j < 0 ? ym2 : (j > 0 ? yp2 : (R)0));
+            if (dyz2 > m2) continue;
+            for (int i = -1; i <= 1; ++i) {
+                if (dyz2 + (i < 0 ? xm2 : (i > 0 ? xp2 : (R)0)) > m2) continue;
+                const long long mm = cell_of(c, ci + i, cj + j, ck + k);
+                if (mm < 0 || mm >= c.ncell) continue;
+                const int a = cs[mm];
+                if (a < 0) continue;
+                cell_records<G, R>(spos, a, ce[mm], self, f);
+            }
+        }
+    }
 }
 
 // for_candidates_rec in groups: g(t0, Q, m) for runs of up to 4 consecutive candidates t0 ..

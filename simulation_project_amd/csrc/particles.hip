@@ -361,8 +361,7 @@ __device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int*
     } else if (nbq) {
         const int lbase = (t >> 6) * 64 * nb_width, lane = t & 63;
         int q0 = 0, q1 = 0, q2 = 0, q3 = 0;
-        for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
-            const R4<R> Q = spos[q];
+        for_candidates_rec_within<8>(c, cs, ce, spos, x, y, z, t, re, [&](int q, const R4<R>& Q) {
             R d = dist3(x, y, z, Q.x, Q.y, Q.z);
             if (d < re) {
                 if (d < mind) d = mind;
