@@ -361,7 +361,8 @@ __device__ inline void pnd_body(int blk, int N, Consts<R> c, int eos, const int*
     } else if (nbq) {
         const int lbase = (t >> 6) * 64 * nb_width, lane = t & 63;
         int q0 = 0, q1 = 0, q2 = 0, q3 = 0;
-        for_candidates_rec_within<8>(c, cs, ce, spos, x, y, z, t, re, [&](int q, const R4<R>& Q) {
+        for_near_candidates(c, cs, ce, spos, x, y, z, t, re, [&](int q) {
+            const R4<R> Q = spos[q];
             R d = dist3(x, y, z, Q.x, Q.y, Q.z);
             if (d < re) {
                 if (d < mind) d = mind;
@@ -612,8 +613,7 @@ __device__ inline void thermal_body(int blk, int N, Consts<R> c, const int* __re
     const R t0 = in.temp[s];
     const R re = c.re_temp, mind = c.mind_temp;
     R hf = 0, hw = 0;
-    for_candidates_within(c, cs, ce, P.x, P.y, P.z, t, re, [&](int q) {
-        const R4<R> Q = spos[q];
+    for_candidates_rec_within<4>(c, cs, ce, spos, P.x, P.y, P.z, t, re, [&](int q, const R4<R>& Q) {
         R d = dist3(P.x, P.y, P.z, Q.x, Q.y, Q.z);
         if (d < re) {
             if (d < mind) d = mind;
@@ -710,8 +710,7 @@ __device__ inline void turbulence_body(int blk, int N, Consts<R> c, const int* _
     const R4<R> V0 = svel[t];
     R txx = 0, txy = 0, txz = 0, tyx = 0, tyy = 0, tyz = 0, tzx = 0, tzy = 0, tzz = 0;
     const R re = c.re_grad, mind = c.mind_grad;
-    for_candidates_within(c, cs, ce, x, y, z, t, re, [&](int q) {
-        const R4<R> Q = spos[q];
+    for_candidates_rec_within<4>(c, cs, ce, spos, x, y, z, t, re, [&](int q, const R4<R>& Q) {
         R d = dist3(x, y, z, Q.x, Q.y, Q.z);
         if (d < re) {
             if (d < mind) d = mind;
