@@ -546,9 +546,9 @@ template <typename R> struct Engine : EngineBase {
         HIPC(hipMemset(d.uniq_n, 0, sizeof(int)));
         ALLOC(d.cell_tri_start, d.c.ncell); ALLOC(d.cell_tri_count, d.c.ncell);
         HIPC(hipMemset(d.cell_tri_count, 0, sizeof(int) * d.c.ncell));
-        ALLOC(d.cell_utri, (size_t)d.c.ncell * kStencilTris); ALLOC(d.cell_utri_n, d.c.ncell);
-        ALLOC(d.cell_work, d.c.ncell); ALLOC(d.cell_work_n, 1);
-        HIPC(hipMemset(d.cell_utri_n, 0xff, sizeof(int) * d.c.ncell));   // -1: no walk yet
+        ALLOC(d.cell_uhead, d.c.ncell);
+        ALLOC(d.cell_work, d.c.ncell); ALLOC(d.cell_work_n, 2);
+        HIPC(hipMemset(d.cell_uhead, 0xff, sizeof(int2) * d.c.ncell));   // count -1: no walk yet
         // capacity of the (cell, triangle) pair buffer from the walls as given
         int pair_total = 0;
         if (d.ntri > 0) {
@@ -559,6 +559,11 @@ template <typename R> struct Engine : EngineBase {
             HIPC(hipMemcpy(&pair_total, d.pair_n, sizeof(int), hipMemcpyDeviceToHost));
         }
         d.pair_cap = std::max(64, pair_total * 2 + 65536);
+        // the cell walks' pool: a cell reserves at most the length of its 27 stencil lists, so 16
+        // pair_cap entries hold every reservation while the pairs stay below 16/27 of pair_cap (more
+        // than the initial count); a cell that finds the pool full takes the full walk
+        d.utri_cap = (int)std::min<long long>(16LL * d.pair_cap, 1LL << 30);
+        ALLOC(d.cell_utri, d.utri_cap);
         ALLOC(d.pair_cell, d.pair_cap); ALLOC(d.pair_tri, d.pair_cap);
         ALLOC(d.pair_flag, d.pair_cap); ALLOC(d.pair_pos, d.pair_cap);
         ALLOC(d.uniq_cell, d.pair_cap); ALLOC(d.uniq_tri, d.pair_cap);
@@ -1345,6 +1350,7 @@ template <typename R> struct BatchGroup {
                 add(fb, fbw, d.cell_end, -1, (int)d.c.ncell);
             }
             add(ft, ftw, d.cell_tri_count, 0, (int)d.c.ncell);
+            add(ft, ftw, d.cell_work_n, 0, 2);
             if (d.ntri <= 0) add(ft, ftw, d.uniq_n, 0, 1);
         }
         const size_t ofb = put(fb.data(), fb.size() * sizeof(FillJob)), oft = put(ft.data(), ft.size() * sizeof(FillJob));

@@ -136,10 +136,13 @@ template <typename R> struct PState {
     R *p, *n, *temp, *nu, *solid;
 };
 
+// most triangles of a per-cell triangle walk (nearest_triangle_listed); a walk entry's key keeps
+// the walk position in its low 10 bits
+static constexpr int kStencilTris = 1024;
+static constexpr unsigned kWalkPosMask = 1023u;
+
 // Triangle record on the device: three vertices, the stored normal, the model, the model's
 // drag flag.  64 B for float, 112 B for double.
-// length of the per-cell triangle walks (nearest_triangle_listed)
-static constexpr int kStencilTris = 256;
 
 template <typename R> struct alignas(16) TriD {
     R v[9];
@@ -413,10 +416,11 @@ template <typename R> struct Dev {
     int* overflow = nullptr;          // set when a march emits more than pair_cap pairs
     int* cell_tri_start = nullptr;    // [ncell]
     int* cell_tri_count = nullptr;    // [ncell], at most 100
-    int* cell_utri = nullptr;         // [ncell * kStencilTris] each cell's triangle walk
-    int* cell_utri_n = nullptr;       // [ncell] its length, -1: none
+    int2* cell_utri = nullptr;        // [utri_cap] pool of the cells' triangle walks: (id, bound key)
+    int2* cell_uhead = nullptr;       // [ncell] (pool offset, count) of each cell's walk, count -1: none
     int* cell_work = nullptr;         // [ncell] cells whose walk is being written
-    int* cell_work_n = nullptr;       // [1]
+    int* cell_work_n = nullptr;       // [2] work-list length, pool entries reserved
+    int utri_cap = 0;
     const int* step_base = nullptr;   // batched step: device index of the first replayed step
     int thermal_step = 1;             // batched step: heat conduction every this many steps
     // nearest wall and drag, sorted order

@@ -161,49 +161,80 @@ template <typename R> void launch_tri_count_total(Dev<R>& d, hipStream_t st) {
     exclusive_scan(d.tri_emit, d.tri_off, nullptr, d.ntri, d.ntri, d.pair_n, d.tsort, st);
 }
 
-// The walk of every cell that holds particles, for nearest_triangle_listed, in two passes.
+// The walk of every cell that holds particles, for nearest_triangle_listed, in two passes, into a
+// pool shared by the cells (head[cell] = (offset, count); count -1: no walk, the full one is taken).
 // First, per cell: a cell without particles gets -1, one whose stencil lists are all empty gets 0;
-// true for the others, which go to a work list (in any order: each cell's walk depends only on
-// its own lists).
+// the others reserve min(total, kStencilTris) pool entries (total: the stencil lists' lengths,
+// repeats included; one reservation per wave) and go to a work list (in any order: each cell's
+// walk depends only on its own lists), or get -1 when the pool is full.  ctr[0] counts the work
+// list, ctr[1] the pool.  All 64 lanes of the wave call it; it returns whether the cell is listed.
 __device__ inline bool cell_walk_needed(long long cell, int ncell, int gy, int gz, const int* __restrict__ cs,
-                                        const int* __restrict__ tcount, int* ucount) {
-    if (cell >= ncell) return false;
-    if (cs[cell] < 0) {
-        ucount[cell] = -1;
-        return false;
-    }
-    const long long gyz = (long long)gy * gz;
-    const int ci = (int)(cell / gyz), cj = (int)((cell / gz) % gy), ck = (int)(cell % gz);
-    int total = 0;
+                                        const int* __restrict__ tcount, int2* head, int* ctr, int cap) {
+    int need = 0;
+    if (cell < ncell) {
+        if (cs[cell] < 0) {
+            head[cell] = make_int2(-1, -1);
+        } else {
+            const long long gyz = (long long)gy * gz;
+            const int ci = (int)(cell / gyz), cj = (int)((cell / gz) % gy), ck = (int)(cell % gz);
+            int total = 0;
 #pragma unroll
-    for (int s = 0; s < 27; ++s) {
-        const long long m = gyz * (ci + s % 3 - 1) + (long long)gz * (cj + (s / 3) % 3 - 1) + (ck + s / 9 - 1);
-        total += (m >= 0 && m < ncell) ? tcount[m] : 0;
+            for (int s = 0; s < 27; ++s) {
+                const long long m =
+                    gyz * (ci + s % 3 - 1) + (long long)gz * (cj + (s / 3) % 3 - 1) + (ck + s / 9 - 1);
+                total += (m >= 0 && m < ncell) ? tcount[m] : 0;
+            }
+            if (total == 0) head[cell] = make_int2(0, 0);
+            else need = total < kStencilTris ? total : kStencilTris;
+        }
     }
-    if (total == 0) {
-        ucount[cell] = 0;
+    const int inc = wave_inclusive_scan(need);
+    const int sum = __shfl(inc, 63, 64);
+    int base = 0;
+    if (lane_id() == 63 && sum > 0) base = atomicAdd(&ctr[1], sum);
+    base = __shfl(base, 63, 64);
+    if (need == 0) return false;
+    const int off = base + inc - need;
+    if (off > cap - need) {
+        head[cell] = make_int2(-1, -1);
         return false;
     }
+    head[cell] = make_int2(off, -1);
     return true;
 }
+// the work-list slot of each listed lane: one atomic per wave
+__device__ inline int wave_append(bool listed, int* counter) {
+    const unsigned long long mask = __ballot(listed);
+    int base = 0;
+    if (lane_id() == 0 && mask) base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, 0, 64);
+    return base + __popcll(mask & ((1ull << lane_id()) - 1ull));
+}
 __global__ __launch_bounds__(256) void k_cell_walk_cells(int ncell, int gy, int gz, const int* __restrict__ cs,
-                                                         const int* __restrict__ tcount, int* work, int* nwork,
-                                                         int* ucount) {
+                                                         const int* __restrict__ tcount, int* work, int* ctr,
+                                                         int2* head, int cap) {
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell_walk_needed(cell, ncell, gy, gz, cs, tcount, ucount)) work[atomicAdd(nwork, 1)] = (int)cell;
+    const bool listed = cell_walk_needed(cell, ncell, gy, gz, cs, tcount, head, ctr, cap);
+    const int slot = wave_append(listed, &ctr[0]);
+    if (listed) work[slot] = (int)cell;
 }
 // Then one wave per listed cell (a workgroup of 64).  Lane s < 27 reads stencil cell s's list (z
 // outer, x inner); the ids are then taken 64 at a time in walk order, one per lane, and an id is
 // kept when neither the ids kept so far (LDS, read 4 at a time) nor an earlier lane of the same 64
-// holds it: its first meeting.  More than kStencilTris kept ids give -1 (the full walk).
+// holds it: its first meeting.  More than kStencilTris kept ids give -1 (the full walk).  The kept
+// ids are then ordered by tri_box_bound over the cell's box (bitonic sort in LDS), each with its
+// position in the walk, and written at the cell's pool offset for nearest_triangle_listed.
 struct WalkLds {
     int off[28], beg[28];
     __align__(16) int kept[kStencilTris];
+    unsigned key[kStencilTris];
 };
-__device__ inline void cell_walk_wave(long long cell, int ncell, int gy, int gz, const int* __restrict__ tstart,
-                                      const int* __restrict__ tcount, const int* __restrict__ tids, int* ulist,
-                                      int* ucount, WalkLds& L) {
+template <typename R>
+__device__ inline void cell_walk_wave(long long cell, const Consts<R>& c, const int* __restrict__ tstart,
+                                      const int* __restrict__ tcount, const int* __restrict__ tids,
+                                      const TriD<R>* __restrict__ tris, int2* ulist, int2* head, WalkLds& L) {
     const int lane = threadIdx.x;
+    const int ncell = c.ncell, gy = c.gy, gz = c.gz;
     const long long gyz = (long long)gy * gz;
     const int ci = (int)(cell / gyz), cj = (int)((cell / gz) % gy), ck = (int)(cell % gz);
     int cnt = 0, b0 = 0;
@@ -247,33 +278,64 @@ __device__ inline void cell_walk_wave(long long cell, int ncell, int gy, int gz,
         const int rank = __popcll(mask & ((1ull << lane) - 1ull));
         const int add = __popcll(mask);
         __syncthreads();
-        if (first && n + rank < kStencilTris) {
-            L.kept[n + rank] = tid;
-            ulist[cell * kStencilTris + n + rank] = tid;
-        }
+        if (first && n + rank < kStencilTris) L.kept[n + rank] = tid;
         __syncthreads();
         if (n + add > kStencilTris) over = true;
         n = min(n + add, kStencilTris);
     }
-    if (lane == 0) ucount[cell] = over ? -1 : n;
+    if (over) {
+        if (lane == 0) head[cell].y = -1;
+        __syncthreads();
+        return;
+    }
+    // sort keys: the bound's float bits with the walk position in the low 10 bits (clearing them
+    // lowers a non-negative bound), padding after the n ids up to a power of two
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    const R lox = c.bminx + (R)ci * c.cell, loy = c.bminy + (R)cj * c.cell, loz = c.bminz + (R)ck * c.cell;
+    for (int q = lane; q < n2; q += 64) {
+        unsigned key = 0xffffffffu;
+        if (q < n) key = (__float_as_uint(tri_box_bound(tris[L.kept[q]], lox, loy, loz, c.cell)) & ~kWalkPosMask) | (unsigned)q;
+        L.key[q] = key;
+    }
+    __syncthreads();
+    for (int size = 2; size <= n2; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = lane; i < (n2 >> 1); i += 64) {
+                const int lo = 2 * stride * (i / stride) + (i % stride), hi = lo + stride;
+                const unsigned ka = L.key[lo], kb = L.key[hi];
+                if ((ka > kb) == ((lo & size) == 0)) {
+                    L.key[lo] = kb;
+                    L.key[hi] = ka;
+                    const int t = L.kept[lo];
+                    L.kept[lo] = L.kept[hi];
+                    L.kept[hi] = t;
+                }
+            }
+            __syncthreads();
+        }
+    const int off = head[cell].x;
+    for (int q = lane; q < n; q += 64) ulist[off + q] = make_int2(L.kept[q], (int)L.key[q]);
+    if (lane == 0) head[cell].y = n;
     __syncthreads();
 }
-__global__ __launch_bounds__(64) void k_cell_walks(int ncell, int gy, int gz, const int* __restrict__ work,
+template <typename R>
+__global__ __launch_bounds__(64) void k_cell_walks(Consts<R> c, const int* __restrict__ work,
                                                    const int* __restrict__ nwork, const int* __restrict__ tstart,
                                                    const int* __restrict__ tcount, const int* __restrict__ tids,
-                                                   int* ulist, int* ucount) {
+                                                   const TriD<R>* __restrict__ tris, int2* ulist, int2* head) {
     __shared__ WalkLds L;
     const int nw = *nwork;
     for (int w = blockIdx.x; w < nw; w += gridDim.x)
-        cell_walk_wave(work[w], ncell, gy, gz, tstart, tcount, tids, ulist, ucount, L);
+        cell_walk_wave<R>(work[w], c, tstart, tcount, tids, tris, ulist, head, L);
 }
 
 template <typename R> static void launch_cell_stencil_tris(const Dev<R>& d, hipStream_t st) {
-    fill_int(d.cell_work_n, 0, 1, st);
+    fill_int(d.cell_work_n, 0, 2, st);
     hipLaunchKernelGGL(k_cell_walk_cells, dim3(grid_for(d.c.ncell, 256)), dim3(256), 0, st, (int)d.c.ncell, d.c.gy,
-                       d.c.gz, d.cell_start, d.cell_tri_count, d.cell_work, d.cell_work_n, d.cell_utri_n);
-    hipLaunchKernelGGL(k_cell_walks, dim3(2048), dim3(64), 0, st, (int)d.c.ncell, d.c.gy, d.c.gz, d.cell_work,
-                       d.cell_work_n, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.cell_utri, d.cell_utri_n);
+                       d.c.gz, d.cell_start, d.cell_tri_count, d.cell_work, d.cell_work_n, d.cell_uhead, d.utri_cap);
+    hipLaunchKernelGGL(k_cell_walks<R>, dim3(2048), dim3(64), 0, st, d.c, d.cell_work, d.cell_work_n,
+                       d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.cell_utri, d.cell_uhead);
 }
 
 template <typename R> void launch_tri_bin(Dev<R>& d, hipStream_t st) {
@@ -310,14 +372,14 @@ template <typename R>
 __device__ inline void stl_distance_body(int blk, int N, Consts<R> c, const R4<R>* __restrict__ spos,
                                          const int* __restrict__ tstart, const int* __restrict__ tcount,
                                          const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
-                                         R* stl_mag, R4<R>* stl_dir, int* stl_id, const int* __restrict__ ulist,
-                                         const int* __restrict__ ucount) {
+                                         R* stl_mag, R4<R>* stl_dir, int* stl_id, const int2* __restrict__ ulist,
+                                         const int2* __restrict__ uhead) {
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const R4<R> P = spos[t];
     R best;
     P3<R> dir;
-    const int bid = nearest_triangle_listed(c, P.x, P.y, P.z, ulist, ucount, tstart, tcount, tids, tris, 0, &best,
+    const int bid = nearest_triangle_listed(c, P.x, P.y, P.z, ulist, uhead, tstart, tcount, tids, tris, 0, &best,
                                             &dir);
     stl_mag[t] = best;
     stl_dir[t] = mk4w<R>(dir.x, dir.y, dir.z);
@@ -328,8 +390,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_stl_distance(int N, Consts<R> c, 
                                                        const int* __restrict__ tstart, const int* __restrict__ tcount,
                                                        const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
                                                        R* stl_mag, R4<R>* stl_dir, int* stl_id,
-                                                       const int* __restrict__ ulist, const int* __restrict__ ucount) {
-    stl_distance_body<R>(blockIdx.x, N, c, spos, tstart, tcount, tids, tris, stl_mag, stl_dir, stl_id, ulist, ucount);
+                                                       const int2* __restrict__ ulist, const int2* __restrict__ uhead) {
+    stl_distance_body<R>(blockIdx.x, N, c, spos, tstart, tcount, tids, tris, stl_mag, stl_dir, stl_id, ulist, uhead);
 }
 
 
@@ -340,7 +402,7 @@ __device__ inline void drag_body(int blk, int N, Consts<R> c, const int* __restr
                                  const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
                                  const mpsr_tri_params* __restrict__ models,
                                  const mpsr_drag_params* __restrict__ drag, R4<R>* drag_acc,
-                                 R* drag_temp, const int* __restrict__ ulist, const int* __restrict__ ucount) {
+                                 R* drag_temp, const int2* __restrict__ ulist, const int2* __restrict__ uhead) {
     const int t = blk * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const int s = perm[t];
@@ -348,7 +410,7 @@ __device__ inline void drag_body(int blk, int N, Consts<R> c, const int* __restr
     const R4<R> P = spos[t];
     R best;
     P3<R> dir;
-    const int bid = nearest_triangle_listed(c, P.x, P.y, P.z, ulist, ucount, tstart, tcount, tids, tris, 1, &best,
+    const int bid = nearest_triangle_listed(c, P.x, P.y, P.z, ulist, uhead, tstart, tcount, tids, tris, 1, &best,
                                             &dir);
     drag_acc[t] = mk4w<R>(0, 0, 0);
     drag_temp[t] = 0;
@@ -401,10 +463,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_drag(int N, Consts<R> c, const in
                                                const int* __restrict__ tids, const TriD<R>* __restrict__ tris,
                                                const mpsr_tri_params* __restrict__ models,
                                                const mpsr_drag_params* __restrict__ drag, R4<R>* drag_acc,
-                                               R* drag_temp, const int* __restrict__ ulist,
-                                               const int* __restrict__ ucount) {
+                                               R* drag_temp, const int2* __restrict__ ulist,
+                                               const int2* __restrict__ uhead) {
     drag_body<R>(blockIdx.x, N, c, perm, in, out, spos, svel, tstart, tcount, tids, tris, models, drag, drag_acc,
-                 drag_temp, ulist, ucount);
+                 drag_temp, ulist, uhead);
 }
 
 template <typename R>
@@ -457,13 +519,13 @@ template <typename R> void launch_stl_distance(const Dev<R>& d, hipStream_t st) 
     if (d.N <= 0) return;
     hipLaunchKernelGGL(k_stl_distance<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.spos,
                        d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.stl_mag, d.stl_dir, d.stl_id,
-                       d.cell_utri, d.cell_utri_n);
+                       d.cell_utri, d.cell_uhead);
 }
 template <typename R> void launch_drag(const Dev<R>& d, hipStream_t st) {
     if (d.N <= 0) return;
     hipLaunchKernelGGL(k_drag<R>, dim3(grid_for(d.N, d.block)), dim3(d.block), 0, st, d.N, d.c, d.perm, d.in, d.out, d.spos,
                        d.svel, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.models, d.drag, d.drag_acc,
-                       d.drag_temp, d.cell_utri, d.cell_utri_n);
+                       d.drag_temp, d.cell_utri, d.cell_uhead);
 }
 template <typename R> void launch_translate(const Dev<R>& d, hipStream_t st) {
     if (d.ntri <= 0) return;
@@ -517,7 +579,7 @@ template <typename R> struct OpStl {
     __device__ void operator()(int k, int b) const {
         const Dev<R>& d = dv[k];
         stl_distance_body<R>(b, d.N, d.c, d.spos, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.stl_mag,
-                             d.stl_dir, d.stl_id, d.cell_utri, d.cell_utri_n);
+                             d.stl_dir, d.stl_id, d.cell_utri, d.cell_uhead);
     }
 };
 template <typename R> struct OpDrag {
@@ -525,7 +587,7 @@ template <typename R> struct OpDrag {
     __device__ void operator()(int k, int b) const {
         const Dev<R>& d = dv[k];
         drag_body<R>(b, d.N, d.c, d.perm, d.in, d.out, d.spos, d.svel, d.cell_tri_start, d.cell_tri_count, d.uniq_tri,
-                     d.tris, d.models, d.drag, d.drag_acc, d.drag_temp, d.cell_utri, d.cell_utri_n);
+                     d.tris, d.models, d.drag, d.drag_acc, d.drag_temp, d.cell_utri, d.cell_uhead);
     }
 };
 // the walks of the batched step: (bucket, block of 256 cells) items sort the cells as
@@ -537,8 +599,10 @@ template <typename R> struct OpCellWalkCells {
     __device__ void operator()(int k, int b) const {
         const Dev<R>& d = dv[k];
         const long long cell = (long long)b * 256 + threadIdx.x;
-        if (cell_walk_needed(cell, (int)d.c.ncell, d.c.gy, d.c.gz, d.cell_start, d.cell_tri_count, d.cell_utri_n))
-            work[atomicAdd(nwork, 1)] = make_int2(k, (int)cell);
+        const bool listed = cell_walk_needed(cell, (int)d.c.ncell, d.c.gy, d.c.gz, d.cell_start, d.cell_tri_count,
+                                             d.cell_uhead, d.cell_work_n, d.utri_cap);
+        const int slot = wave_append(listed, nwork);
+        if (listed) work[slot] = make_int2(k, (int)cell);
     }
 };
 template <typename R>
@@ -549,8 +613,8 @@ __global__ __launch_bounds__(64) void k_cell_walks_b(const Dev<R>* __restrict__ 
     for (int w = blockIdx.x; w < nw; w += gridDim.x) {
         const int2 it = work[w];
         const Dev<R>& d = dv[it.x];
-        cell_walk_wave(it.y, (int)d.c.ncell, d.c.gy, d.c.gz, d.cell_tri_start, d.cell_tri_count, d.uniq_tri,
-                       d.cell_utri, d.cell_utri_n, L);
+        cell_walk_wave<R>(it.y, d.c, d.cell_tri_start, d.cell_tri_count, d.uniq_tri, d.tris, d.cell_utri,
+                          d.cell_uhead, L);
     }
 }
 template <typename R> struct OpTranslate {

@@ -108,40 +108,87 @@ __device__ inline int nearest_triangle(const Consts<R>& c, R x, R y, R z, const 
     return bid;
 }
 
+// A lower bound of tri_distance(t, p) over the points p of the box [lo, lo + size]^3, less a margin
+// of size / 100 (0 where no bound holds), as a float no larger than the bound.  tri_distance ends
+// on a point of an edge, a vertex, or the point P = p - (distance along the stored normal) when the
+// barycentric coordinates of P put it inside the triangle.  The first two lie in the triangle's
+// bounding box.  P lies in the triangle too when the stored normal is the triangle's own (to within
+// an angle whose sine times the longest edge is at most size / 1000) and the triangle is not
+// degenerate (twice its area at least 1 % of the longest edge squared); rounding moves it by far
+// less than the margin.  The bound is then the distance between the box and the bounding box;
+// for other triangles it is 0.
+template <typename R>
+__device__ inline float tri_box_bound(const TriD<R>& t, R lox, R loy, R loz, R size) {
+    const double ax = t.v[0], ay = t.v[1], az = t.v[2], bx = t.v[3], by = t.v[4], bz = t.v[5], cx = t.v[6],
+                 cy = t.v[7], cz = t.v[8];
+    const double e1x = bx - ax, e1y = by - ay, e1z = bz - az, e2x = cx - ax, e2y = cy - ay, e2z = cz - az;
+    const double e3x = cx - bx, e3y = cy - by, e3z = cz - bz;
+    const double gx = e1y * e2z - e1z * e2y, gy = e1z * e2x - e1x * e2z, gz = e1x * e2y - e1y * e2x;
+    const double nx = t.nrm[0], ny = t.nrm[1], nz = t.nrm[2];
+    const double hx = ny * gz - nz * gy, hy = nz * gx - nx * gz, hz = nx * gy - ny * gx;   // n x g
+    const double gg = gx * gx + gy * gy + gz * gz, nn = nx * nx + ny * ny + nz * nz, hh = hx * hx + hy * hy + hz * hz;
+    const double L2 = fmax(e1x * e1x + e1y * e1y + e1z * e1z,
+                           fmax(e2x * e2x + e2y * e2y + e2z * e2z, e3x * e3x + e3y * e3y + e3z * e3z));
+    const double sz = (double)size;
+    // sin^2(n, g) L2 <= (size / 1000)^2, and |g|^2 >= (L2 / 100)^2
+    const bool normal_ok = hh * L2 <= 1e-6 * sz * sz * nn * gg && nn > 0.0;
+    const bool shape_ok = gg >= 1e-4 * L2 * L2 && L2 > 0.0;
+    if (!(normal_ok && shape_ok)) return 0.0f;
+    const double lx = lox, ly = loy, lz = loz;
+    auto gap = [&](double a, double b, double c, double lo) {
+        const double tmin = fmin(a, fmin(b, c)), tmax = fmax(a, fmax(b, c));
+        return fmax(0.0, fmax(tmin - (lo + sz), lo - tmax));
+    };
+    const double dx = gap(ax, bx, cx, lx), dy = gap(ay, by, cy, ly), dz = gap(az, bz, cz, lz);
+    const double lb = sqrt(dx * dx + dy * dy + dz * dz) - 0.01 * sz;
+    if (!(lb > 0.0)) return 0.0f;
+    return (float)(lb * 0.999999);
+}
+
 // A triangle spans several cells, so the 27-cell walk of nearest_triangle meets it in several
 // lists.  Met again, its distance is the same and only a strictly nearer triangle replaces the
-// running nearest, so any walk that meets the same triangles in the same order of first meeting,
-// with or without repeats, gives nearest_triangle's result.  k_cell_stencil_tris writes such a
-// walk once per cell and triangle binning (repeats among the last 8 ids dropped); a cell without
-// one (count -1: no particles when the lists were written, or more than kStencilTris ids) takes
-// the full walk.
+// running nearest, so the result is the triangle of least distance, the first met of those of
+// equal distance: the least (distance, position of first meeting).  The cell walk kernels write
+// each cell's triangles once (head[cell] = (pool offset, count)), with their positions of first
+// meeting, ordered by tri_box_bound over the cell's box; a particle inside that box (to within half the bound's margin) stops at the
+// first triangle whose bound exceeds its nearest distance so far, as no later one can be nearer or
+// as near.  A particle outside it (the grid's truncating cell index puts particles up to one cell
+// below the grid into the first cell) takes every listed triangle.  A cell without a list (count
+// -1: no particles when the lists were written, or more than kStencilTris triangles) takes the
+// full walk.
 template <typename R>
-__device__ inline int nearest_triangle_listed(const Consts<R>& c, R x, R y, R z, const int* __restrict__ ulist,
-                                              const int* __restrict__ ucount, const int* __restrict__ tstart,
+__device__ inline int nearest_triangle_listed(const Consts<R>& c, R x, R y, R z, const int2* __restrict__ ulist,
+                                              const int2* __restrict__ uhead, const int* __restrict__ tstart,
                                               const int* __restrict__ tcount, const int* __restrict__ tids,
                                               const TriD<R>* __restrict__ tris, int want_drag, R* best, P3<R>* bdir) {
-    int ci, cj, ck;
+    int ci = 0, cj = 0, ck = 0;
     long long cid = -1;
-    if (ucount && cell_coords(c, x, y, z, ci, cj, ck)) cid = cell_of(c, ci, cj, ck);
-    const int n = (cid >= 0 && cid < c.ncell) ? ucount[cid] : -1;
+    if (uhead && cell_coords(c, x, y, z, ci, cj, ck)) cid = cell_of(c, ci, cj, ck);
+    const int2 h = (cid >= 0 && cid < c.ncell) ? uhead[cid] : make_int2(0, -1);
+    const int n = h.y;
     if (n < 0) return nearest_triangle(c, x, y, z, tstart, tcount, tids, tris, want_drag, best, bdir);
     *best = (R)1e8;
     *bdir = p3<R>(0, 0, 0);
-    int bid = -1;
+    int bid = -1, bpos = 0;
     const P3<R> p = p3(x, y, z);
-    const int* L = ulist + cid * kStencilTris;
+    const R slack = c.cell * (R)0.005;
+    const R lox = c.bminx + (R)ci * c.cell, loy = c.bminy + (R)cj * c.cell, loz = c.bminz + (R)ck * c.cell;
+    const bool cut = x >= lox - slack && x <= lox + c.cell + slack && y >= loy - slack &&
+                     y <= loy + c.cell + slack && z >= loz - slack && z <= loz + c.cell + slack;
+    const int2* L = ulist + h.x;
     for (int e = 0; e < n; ++e) {
-        const int tid = L[e];
-        const TriD<R>& tr = tris[tid];
+        const int2 en = L[e];
+        if (cut && (R)__uint_as_float((unsigned)en.y & ~kWalkPosMask) > *best) break;
+        const TriD<R>& tr = tris[en.x];
         if (tr.is_drag != want_drag) continue;
-        if (tid != bid) {
-            P3<R> dv;
-            const R dm = tri_distance(tr, p, &dv);
-            if (dm < *best) {
-                *best = dm;
-                *bdir = dv;
-                bid = tid;
-            }
+        P3<R> dv;
+        const R dm = tri_distance(tr, p, &dv);
+        const int pos = (int)((unsigned)en.y & kWalkPosMask);
+        if (dm < *best || (dm == *best && bid >= 0 && pos < bpos)) {
+            *best = dm;
+            *bdir = dv;
+            bid = en.x;
+            bpos = pos;
         }
     }
     return bid;

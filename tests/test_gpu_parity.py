@@ -601,6 +601,40 @@ def test_thermal_steps_with_drag_heat_identical(lib, precision):
     assert np.ptp(t) > 0 and not np.allclose(t, case.particles.temperature)
 
 
+def test_stl_distance_sorted_walks_hold_for_any_normals(lib):
+    """CalcSTLDistance (Kern :635-699) where the per-cell walks' box bounds do not hold: every third
+    triangle's stored normal turned away from its winding (the distance then ends off the triangle),
+    some normals zero, one triangle collapsed to a segment, and particles moved just below the
+    compute zone and far outside it (their truncated cell index names a cell whose box does not hold
+    them).  Distances, directions and ids equal the oracle's bit for bit."""
+    case = cases.drum(lib, l0=8e-3, n_r=16, n_x=36)
+    base = case.triangles
+
+    def odd_normals():
+        arr = base()
+        for t in range(0, len(case.vertices), 3):
+            n = arr[t].Normal
+            arr[t].Normal = abi.Float3(n.y, -n.x + 0.3 * n.z, 0.5 * n.z) if t % 9 else abi.Float3(0.0, 0.0, 0.0)
+        return arr
+
+    case.vertices = case.vertices.copy()
+    case.vertices[7, 2] = case.vertices[7, 0] + 0.5 * (case.vertices[7, 1] - case.vertices[7, 0])
+    case.triangles = odd_normals
+    zmin = case.zone.m_MinBound
+    pos = case.particles.position
+    cell = 3.1 * 8e-3
+    pos[:16, 0] = zmin.x - 0.3 * cell
+    pos[16:24] = pos[16:24] + np.array([0.0, 0.0, 40 * cell])
+    print(f"[stl-normals] triangles={len(case.vertices)} particles={case.n}", flush=True)
+    g, o = pair(case)
+    run_phase("RegisterParticleTopology", g, o)
+    run_phase("RegisterTriangleTopology", g, o)
+    run_phase("CalcSTLDistance", g, o)
+    for a, b in zip(g.stl_distance(), o.stl_distance()):
+        np.testing.assert_array_equal(a, b)
+    assert np.any(g.stl_distance()[2] >= 0)
+
+
 @pytest.mark.parametrize("n_r,n_x", [(6, 12), (16, 36), (96, 200)])
 def test_stl_distance_walks_any_mesh_density(lib, n_r, n_x):
     """CalcSTLDistance (Kern :635-699) on drums of 24.8 mm cells whose walls are meshed from coarse
