@@ -110,6 +110,8 @@ template <typename R> struct PpeSolver {
     int* wbox = nullptr;         // [windows * 8] per SELL window: the p ranges its rows read (k_win_box)
     int wcap = 0;                // p values a window stages in LDS in this solve's SpMV (0: none)
     int c16 = 0;                 // this solve's SpMV reads the 16-bit codes (every column fitted one)
+    int cols_valid = 0;          // the 32-bit columns of the current matrix are written
+    int cols_wide = 0;           // a matrix had columns beyond the codes: the fill writes 32-bit columns
     const uint32_t* skeys = nullptr;  // this solve's sorted cell keys
     int gz = 0, gygz = 0;        // cell-key strides of one grid row and one x-slab
     double* vals64 = nullptr;
@@ -366,7 +368,7 @@ __device__ inline void ppe_fill_body(int blk, int N, int nslices, Consts<R> c, c
                                      const int* __restrict__ rowp, int* cols, VA* vals, double* rhs,
                                      int* cnt, int* nnz, const int* __restrict__ nbq,
                                      int nb_width, uint16_t* cols16,
-                                     const uint32_t* __restrict__ skeys, const int* __restrict__ fg) {
+                                     const uint32_t* __restrict__ skeys, const int* __restrict__ fg, int wc = 1) {
     const int P = blk * blockDim.x + threadIdx.x;
     const int slice = P >> 6, lane = lane_id();
     if (slice >= nslices) return;
@@ -385,7 +387,7 @@ __device__ inline void ppe_fill_body(int blk, int N, int nslices, Consts<R> c, c
         ++k;
         if ((k & 3) == 0) {
             const int e = sell_at(base, lane, k - 4);
-            quad_store(cols + e, c0, c1, c2, c3);
+            if (wc) quad_store(cols + e, c0, c1, c2, c3);
             quad_store(cols16 + e, h0, h1, h2, h3);
             quad_store(vals + e, v0, v1, v2, v3);
         }
@@ -466,9 +468,47 @@ __global__ __launch_bounds__(PB) void k_ppe_fill(int N, int nslices, Consts<R> c
                                                   const int* __restrict__ rowp, int* cols, VA* vals, double* rhs,
                                                   int* cnt, int* nnz, const int* __restrict__ nbq,
                                                   int nb_width, uint16_t* cols16,
-                                                  const uint32_t* __restrict__ skeys, const int* __restrict__ fg) {
+                                                  const uint32_t* __restrict__ skeys, const int* __restrict__ fg, int wc) {
     ppe_fill_body<R, VA>(blockIdx.x, N, nslices, c, active, dir, spos, cs, ce, slice_off, slice_w, perm, out, alpha,
-                         n0, rowp, cols, vals, rhs, cnt, nnz, nbq, nb_width, cols16, skeys, fg);
+                         n0, rowp, cols, vals, rhs, cnt, nnz, nbq, nb_width, cols16, skeys, fg, wc);
+}
+// The 32-bit columns of a matrix whose columns all fitted the 16-bit codes, from the codes, for the
+// readers that take that form (the coarse and cell operator assembly, the network, the matrix
+// getter): the mixed solver's fill leaves them out.  Entries up to each row's count rounded up to
+// a group of 4, as the fill writes them.
+__global__ __launch_bounds__(PB) void k_decode_cols(int nslices, const int* __restrict__ slice_off,
+                                                     const int* __restrict__ rowp, const int* __restrict__ cnt,
+                                                     const int* __restrict__ dir, const uint16_t* __restrict__ codes,
+                                                     const uint32_t* __restrict__ skeys, const int* __restrict__ fg,
+                                                     int gz, int gygz, int ncell, int* cols) {
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = P >> 6, lane = lane_id();
+    if (slice >= nslices) return;
+    const int t = rowp[P];
+    if (t < 0 || dir[t]) return;
+    C16Row cr;
+    if (!c16_row(skeys, fg, t, gz, gygz, ncell, cr)) return;
+    const int base = slice_off[slice];
+    const int n4 = (cnt[t] + 3) & ~3;
+    for (int k = 0; k < n4; k += 4) {
+        const int e = sell_at(base, lane, k);
+        int h[4], o[4];
+        quad_load(codes + e, h);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int g = h[u] >> 14, off = h[u] & 0x3fff;
+            o[u] = (g == 0 ? cr.b0 : g == 1 ? cr.b1 : cr.b2) + off;
+        }
+        quad_store(cols + e, o[0], o[1], o[2], o[3]);
+    }
+}
+// the 32-bit columns of the current matrix, decoded when its fill left them out
+template <typename R> static void ensure_cols(PpeSolver<R>* s, const Dev<R>& d, hipStream_t st) {
+    if (s->cols_valid || s->nslices <= 0) return;
+    hipLaunchKernelGGL(k_decode_cols, dim3((s->nslices * 64 + PB - 1) / PB), dim3(PB), 0, st, s->nslices,
+                       s->slice_off, s->rowp, s->cnt, s->dir, s->cols16, d.skeys, s->first_ge, d.c.gz,
+                       d.c.gy * d.c.gz, d.c.ncell, s->cols);
+    s->cols_valid = 1;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -774,11 +814,11 @@ __global__ __launch_bounds__(PB) void k_resid_first(int N, int nslices, const in
                                                      double* r, double* part, const uint16_t* __restrict__ codes,
                                                      const uint32_t* __restrict__ skeys, const int* __restrict__ fg,
                                                      int gz, int gygz, const int* __restrict__ wbox, int wcap,
-                                                     const int* __restrict__ nnz) {
+                                                     const int* __restrict__ nnz, int have_cols) {
     if (nnz[2] == 0)
         resid_body<VA, true, true>(blockIdx.x, N, nslices, slice_off, slice_w, cols, vals, diag, x, b, rowp, cnt, r,
                                    part, codes, skeys, fg, gz, gygz, wbox, wcap);
-    else
+    else if (have_cols)   // without them the solve fills the matrix again with its 32-bit columns
         resid_body<VA, false, false>(blockIdx.x, N, nslices, slice_off, slice_w, cols, vals, diag, x, b, rowp, cnt, r,
                                      part, nullptr, nullptr, nullptr, 0, 0, nullptr, 0);
 }
@@ -1815,6 +1855,7 @@ static bool setup_two_level(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hi
             return false;
         }
     }
+    ensure_cols(s, d, st);
     hipLaunchKernelGGL(k_row_aggxyz, dim3(grid_for(N, PB)), dim3(PB), 0, st, N, g, s->agg_of_row, s->agg_full, s->rxyz);
     hipLaunchKernelGGL((k_coarse_cells<VA>), dim3((unsigned)(nc * cells)), dim3(64), 0, st, nc, g, s->agg_full,
                        d.cell_start, d.cell_end, s->dir, s->cnt, s->diag, s->slice_off, s->pos, s->cols, vals, s->rxyz,
@@ -1869,6 +1910,7 @@ static int setup_cells(PpeSolver<R>* s, const Dev<R>& d, const VA* vals, hipStre
     s->cell_growth_check = !cell_due && cell_grew;
     if (s->cell_growth_check) s->iters_before_cell_growth = s->iters_last;
     if (s->cell_rebuilt) {
+        ensure_cols(s, d, st);
         hipLaunchKernelGGL((k_cell_op<VA>), dim3(ncell), dim3(64), 0, st, ncell, g, d.cell_start, d.cell_end, s->dir,
                            s->cnt, s->pos, s->slice_off, s->cols, vals, s->diag, s->cxyz, s->c_op, s->c_dinv);
         hipLaunchKernelGGL(k_cell_sym, dim3(grid_for((long long)ncell * 27, PB)), dim3(PB), 0, st, ncell, g, s->c_op);
@@ -2102,7 +2144,7 @@ static void launch_resid(PpeSolver<R>* s, int N, int nb, const VA* vals, hipStre
         hipLaunchKernelGGL(k_resid_first<VA>, dim3(nb), dim3(PB), (size_t)RESID_WIN_BYTES, st, N, s->nslices,
                            s->slice_off, s->slice_w, s->cols, vals, s->diag, s->x, s->rhs, s->rowp, s->cnt, s->r,
                            s->part, s->cols16, s->skeys, s->first_ge, s->gz, s->gygz, s->wbox,
-                           RESID_WIN_BYTES / (int)sizeof(double), s->nnz_dev);
+                           RESID_WIN_BYTES / (int)sizeof(double), s->nnz_dev, s->cols_valid);
         return;
     }
     const int wcap = s->wcap > 0 ? std::min(s->wcap, RESID_WIN_BYTES / (int)sizeof(double)) : 0;
@@ -2235,16 +2277,22 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     // the source's share of the density error corrected per step, 0.1 unless set (a double product,
     // evaluated alike by the oracle)
     const double alpha_src = s->alpha * (opt.source_relaxation > 0 ? opt.source_relaxation : 0.1);
+    // the mixed solver's SpMV and residuals read the 16-bit codes, so its fill leaves the 32-bit
+    // columns out (a tenth of the fill's writes at 1M) unless a matrix of this solver had columns
+    // beyond the codes, or the codes or the window are switched off; the readers of the 32-bit
+    // form decode them first (ensure_cols)
+    const int wc = (!s->mixed || s->cols_wide || c16_off() || win_off()) ? 1 : 0;
     if (s->mixed)
         hipLaunchKernelGGL((k_ppe_fill<R, float>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
                            s->rowp, s->cols, s->vals32, s->rhs, s->cnt, s->nnz_dev, s->nbq, NB_WIDTH,
-                           s->cols16, d.skeys, s->first_ge);
+                           s->cols16, d.skeys, s->first_ge, wc);
     else
         hipLaunchKernelGGL((k_ppe_fill<R, double>), dim3(nb), dim3(PB), 0, st, N, nsl, d.c, s->active, s->dir, d.spos,
                            d.cell_start, d.cell_end, s->slice_off, s->slice_w, d.perm, d.out, alpha_src, s->n0,
                            s->rowp, s->cols, s->vals64, s->rhs, s->cnt, s->nnz_dev, s->nbq, NB_WIDTH,
-                           s->cols16, d.skeys, s->first_ge);
+                           s->cols16, d.skeys, s->first_ge, 1);
+    s->cols_valid = wc;
     // entry and row counts of the filled matrix, read after the first residual pass synchronises
     if (hipMemcpyAsync(s->cnt_h, s->nnz_dev, 4 * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess)
         return MPSR_GPUERR;
@@ -2260,6 +2308,7 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     // the plain warm start: the previous pressure (learned and previous modes) or zero
     const int use_prev = (opt.warm_start != MPSR_WARM_ZERO && s->have_prev) ? 1 : 0;
     if (s->learned) {
+        ensure_cols(s, d, st);
         const MpnnSys sys = mpnn_sys(s, d);
         mpnn_features<R>(s->net, sys, mpnn_geo(d), st);
         mpnn_forward(s->net, sys, st);
@@ -2275,6 +2324,13 @@ mpsr_status ppe_solve(PpeSolver<R>* s, Dev<R>& d, const mpsr_ppe_options& opt, i
     double bb = 0.0;
     double rr = resid_pass(s, N, st, &bb, true);
     if (rr < 0) return MPSR_GPUERR;
+    if (s->cnt_h[2] != 0 && !s->cols_valid) {
+        // columns beyond the codes and no 32-bit columns: the first residual pass skipped its work;
+        // this solver fills them from now on, and the solve starts again from the PND pass
+        s->cols_wide = 1;
+        ppe_start(s, d, st);
+        return ppe_solve(s, d, opt, profile, st);
+    }
     s->nnz_real = s->cnt_h[0];
     s->rows_real = s->cnt_h[1];
     s->c16 = (s->cnt_h[2] == 0 && !c16_off()) ? 1 : 0;
@@ -2550,6 +2606,7 @@ mpsr_status ppe_mpnn_train(PpeSolver<R>* s, const Dev<R>& d, double lr, double b
                            double* parts, hipStream_t st) {
     if (!s->net || !mpnn_has_weights(s->net)) return MPSR_PARAMERR;
     if (s->nslices <= 0 || d.N <= 0) return MPSR_PARAMERR;
+    ensure_cols(s, d, st);
     const MpnnSys sys = mpnn_sys(s, d);
     mpnn_features<R>(s->net, sys, mpnn_geo(d), st);
     mpnn_forward(s->net, sys, st);
@@ -2567,6 +2624,7 @@ mpsr_status ppe_mpnn_train(PpeSolver<R>* s, const Dev<R>& d, double lr, double b
 template <typename R>
 mpsr_status ppe_mpnn_rows(PpeSolver<R>* s, const Dev<R>& d, float* X, float* Y, double* scale, hipStream_t st) {
     if (!s->net || !mpnn_has_weights(s->net)) return MPSR_PARAMERR;
+    ensure_cols(s, d, st);
     const MpnnSys sys = mpnn_sys(s, d);
     mpnn_features<R>(s->net, sys, mpnn_geo(d), st);
     mpnn_forward(s->net, sys, st);
@@ -2612,6 +2670,10 @@ mpsr_status ppe_get_matrix(PpeSolver<R>* s, const Dev<R>& d, int* ptr, int* col,
                            int* nnz) {
     const int N = d.N;
     const int nsl = (N + 63) / 64;
+    if (!s->cols_valid) {
+        ensure_cols(s, d, nullptr);
+        if (hipDeviceSynchronize() != hipSuccess) return MPSR_GPUERR;
+    }
     std::vector<int> sw(nsl), so(nsl), cn(N), pp(N), dd(N), po(N);
     std::vector<double> dg(N);
     if (hipMemcpy(sw.data(), s->slice_w, sizeof(int) * nsl, hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
@@ -3021,6 +3083,7 @@ template <typename R> mpsr_status ppe_batch_finish(PpeBatch<R>* b, int* failed, 
     if (hipMemcpy(f.data(), b->flags, sizeof(int) * f.size(), hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
     for (int j = 0; j < b->n; ++j) {
         PpeSolver<R>* s = b->s[j];
+        s->cols_valid = 1;   // the batched fill writes the 32-bit columns
         double sc[S_COUNT];
         if (hipMemcpy(sc, s->scal, sizeof(sc), hipMemcpyDeviceToHost) != hipSuccess) return MPSR_GPUERR;
         s->have_prev = f[2 * j];

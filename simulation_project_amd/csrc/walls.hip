@@ -165,11 +165,13 @@ template <typename R> void launch_tri_count_total(Dev<R>& d, hipStream_t st) {
 // pool shared by the cells (head[cell] = (offset, count); count -1: no walk, the full one is taken).
 // First, per cell: a cell without particles gets -1, one whose stencil lists are all empty gets 0;
 // the others reserve min(total, kStencilTris) pool entries (total: the stencil lists' lengths,
-// repeats included; one reservation per wave) and go to a work list (in any order: each cell's
-// walk depends only on its own lists), or get -1 when the pool is full.  ctr[0] counts the work
-// list, ctr[1] the pool.  All 64 lanes of the wave call it; it returns whether the cell is listed.
-__device__ inline bool cell_walk_needed(long long cell, int ncell, int gy, int gz, const int* __restrict__ cs,
-                                        const int* __restrict__ tcount, int2* head, int* ctr, int cap) {
+// repeats included) and go to a work list (in any order: each cell's walk depends only on its own
+// lists); a cell that finds the pool full gets -1 and its work item is skipped.  One wave makes one
+// reservation in *pool_ctr and one append to *work_ctr, issued together.  All 64 lanes of the wave
+// call it; it returns the cell's work-list slot, or -1.
+__device__ inline int cell_walk_needed(long long cell, int ncell, int gy, int gz, const int* __restrict__ cs,
+                                       const int* __restrict__ tcount, int2* head, int* pool_ctr, int cap,
+                                       int* work_ctr) {
     int need = 0;
     if (cell < ncell) {
         if (cs[cell] < 0) {
@@ -190,33 +192,25 @@ __device__ inline bool cell_walk_needed(long long cell, int ncell, int gy, int g
     }
     const int inc = wave_inclusive_scan(need);
     const int sum = __shfl(inc, 63, 64);
-    int base = 0;
-    if (lane_id() == 63 && sum > 0) base = atomicAdd(&ctr[1], sum);
-    base = __shfl(base, 63, 64);
-    if (need == 0) return false;
-    const int off = base + inc - need;
-    if (off > cap - need) {
-        head[cell] = make_int2(-1, -1);
-        return false;
+    const unsigned long long mask = __ballot(need > 0);
+    int pbase = 0, wbase = 0;
+    if (lane_id() == 63 && sum > 0) {
+        pbase = atomicAdd(pool_ctr, sum);
+        wbase = atomicAdd(work_ctr, __popcll(mask));
     }
-    head[cell] = make_int2(off, -1);
-    return true;
-}
-// the work-list slot of each listed lane: one atomic per wave
-__device__ inline int wave_append(bool listed, int* counter) {
-    const unsigned long long mask = __ballot(listed);
-    int base = 0;
-    if (lane_id() == 0 && mask) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, 0, 64);
-    return base + __popcll(mask & ((1ull << lane_id()) - 1ull));
+    pbase = __shfl(pbase, 63, 64);
+    wbase = __shfl(wbase, 63, 64);
+    if (need == 0) return -1;
+    const int off = pbase + inc - need;
+    head[cell] = off > cap - need ? make_int2(-1, -1) : make_int2(off, -1);
+    return wbase + __popcll(mask & ((1ull << lane_id()) - 1ull));
 }
 __global__ __launch_bounds__(256) void k_cell_walk_cells(int ncell, int gy, int gz, const int* __restrict__ cs,
                                                          const int* __restrict__ tcount, int* work, int* ctr,
                                                          int2* head, int cap) {
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool listed = cell_walk_needed(cell, ncell, gy, gz, cs, tcount, head, ctr, cap);
-    const int slot = wave_append(listed, &ctr[0]);
-    if (listed) work[slot] = (int)cell;
+    const int slot = cell_walk_needed(cell, ncell, gy, gz, cs, tcount, head, &ctr[1], cap, &ctr[0]);
+    if (slot >= 0) work[slot] = (int)cell;
 }
 // Then one wave per listed cell (a workgroup of 64).  Lane s < 27 reads stencil cell s's list (z
 // outer, x inner); the ids are then taken 64 at a time in walk order, one per lane, and an id is
@@ -233,6 +227,7 @@ template <typename R>
 __device__ inline void cell_walk_wave(long long cell, const Consts<R>& c, const int* __restrict__ tstart,
                                       const int* __restrict__ tcount, const int* __restrict__ tids,
                                       const TriD<R>* __restrict__ tris, int2* ulist, int2* head, WalkLds& L) {
+    if (head[cell].x < 0) return;   // the pool was full: the cell keeps the full walk
     const int lane = threadIdx.x;
     const int ncell = c.ncell, gy = c.gy, gz = c.gz;
     const long long gyz = (long long)gy * gz;
@@ -599,10 +594,9 @@ template <typename R> struct OpCellWalkCells {
     __device__ void operator()(int k, int b) const {
         const Dev<R>& d = dv[k];
         const long long cell = (long long)b * 256 + threadIdx.x;
-        const bool listed = cell_walk_needed(cell, (int)d.c.ncell, d.c.gy, d.c.gz, d.cell_start, d.cell_tri_count,
-                                             d.cell_uhead, d.cell_work_n, d.utri_cap);
-        const int slot = wave_append(listed, nwork);
-        if (listed) work[slot] = make_int2(k, (int)cell);
+        const int slot = cell_walk_needed(cell, (int)d.c.ncell, d.c.gy, d.c.gz, d.cell_start, d.cell_tri_count,
+                                          d.cell_uhead, &d.cell_work_n[1], d.utri_cap, nwork);
+        if (slot >= 0) work[slot] = make_int2(k, (int)cell);
     }
 };
 template <typename R>

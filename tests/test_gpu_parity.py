@@ -136,6 +136,67 @@ def test_ppe_system_identical_and_solution_close(lib, precision):
     assert np.linalg.norm(gx - ox) <= 1e-6 * np.linalg.norm(ox)
 
 
+def _mixed_and_fp64(case):
+    """The same system assembled by the mixed solver (16-bit column codes in its fill) and by the
+    FP64 solver (32-bit columns), each after the phases up to the PPE solve."""
+    mixed = abi.PpeOptions(abi.MPSR_PCG_MIXED, 1e-8, 20000, abi.MPSR_WARM_ZERO, 0.0, 0.0,
+                           abi.MPSR_PRECOND_CELL_VCYCLE, 0, 0, 0, 0.0)
+    fp64 = abi.PpeOptions(abi.MPSR_PCG_FP64, 1e-8, 20000, abi.MPSR_WARM_ZERO, 0.0, 0.0)
+    out = []
+    for opt in (mixed, fp64):
+        g = Solver(case, pressure_mode=abi.MPSR_PRESSURE_PPE, ppe=opt)
+        for name in ("RegisterParticleTopology", "RegisterTriangleTopology", "CalcSTLDistance", "CalcExplicitly",
+                     "RelocateAllParticleTopology", "RegisterParticleTopology", "CalcSTLDistance",
+                     "CalcPressurePoisson"):
+            getattr(g, name)()
+        out.append(g)
+    return out
+
+
+def _spmv_columns(g, n):
+    """The column form of the mixed solver's SpMV, from its byte count: 4 + 2 bytes an entry and
+    the row's cell key with the 16-bit codes, 4 + 4 bytes an entry with the 32-bit columns."""
+    t = g.timings()
+    if abs(t["spmv_bytes"] - (6.0 * t["ppe_nnz"] + 20.0 * n)) < 1.0:
+        return 16
+    if abs(t["spmv_bytes"] - (8.0 * t["ppe_nnz"] + 16.0 * n)) < 1.0:
+        return 32
+    return None
+
+
+def test_ppe_mixed_fill_columns_decoded_for_readers(lib):
+    """The mixed solver's fill writes the 16-bit column codes only; the matrix getter (and the
+    operator assembly and the network) decode the 32-bit columns from them.  The dam break's
+    matrix from the mixed solver: the same columns as the FP64 solver's, the values its FP32
+    rounding, and the SpMV on the codes."""
+    case = cases.dam_break(lib, explicit_pressure=False)
+    gm, gd = _mixed_and_fp64(case)
+    assert _spmv_columns(gm, case.n) == 16
+    (pm, cm, vm, dm), (pd, cd, vd, dd) = gm.ppe_matrix(), gd.ppe_matrix()
+    np.testing.assert_array_equal(pm, pd)
+    np.testing.assert_array_equal(cm, cd)
+    np.testing.assert_array_equal(vm, vd.astype(np.float32).astype(np.float64))
+    np.testing.assert_array_equal(dm, dd)
+    assert gm.timings()["ppe_rel_residual"] <= 1e-8
+
+
+def test_ppe_mixed_columns_beyond_the_codes(lib):
+    """A water column 2,000 layers tall in one cell column: the rows near its top have neighbours
+    more than 16,384 sorted rows into their column range, which the 16-bit codes cannot hold.  The
+    mixed solver's fill left the 32-bit columns out, so the solve starts again from the PND pass
+    with them and runs on the 32-bit form: converged, and the matrix the FP64 solver's."""
+    case = cases.small_box(lib, nx=3, ny=3, nz=2000, explicit_pressure=False, velocity_jitter=0.05)
+    print(f"[ppe-wide] particles={case.n}", flush=True)
+    gm, gd = _mixed_and_fp64(case)
+    assert _spmv_columns(gm, case.n) == 32
+    (pm, cm, vm, dm), (pd, cd, vd, dd) = gm.ppe_matrix(), gd.ppe_matrix()
+    np.testing.assert_array_equal(pm, pd)
+    np.testing.assert_array_equal(cm, cd)
+    np.testing.assert_array_equal(vm, vd.astype(np.float32).astype(np.float64))
+    assert gm.timings()["ppe_rel_residual"] <= 1e-8
+    assert gd.timings()["ppe_rel_residual"] <= 1e-8
+
+
 def ppe_errors(a, b):
     """Per-field errors of state a against state b: the norm-wise relative error and the largest
     per-particle error relative to max(|b_i|, 1e-2 RMS |b|) (the floor keeps particles at rest or at
