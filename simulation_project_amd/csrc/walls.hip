@@ -166,9 +166,9 @@ template <typename R> void launch_tri_count_total(Dev<R>& d, hipStream_t st) {
 // First, per cell: a cell without particles gets -1, one whose stencil lists are all empty gets 0;
 // the others reserve min(total, kStencilTris) pool entries (total: the stencil lists' lengths,
 // repeats included) and go to a work list (in any order: each cell's walk depends only on its own
-// lists); a cell that finds the pool full gets -1 and its work item is skipped.  One wave makes one
-// reservation in *pool_ctr and one append to *work_ctr, issued together.  All 64 lanes of the wave
-// call it; it returns the cell's work-list slot, or -1.
+// lists); a cell that finds the pool full gets -1 and its work item is skipped.  One workgroup makes
+// one reservation in *pool_ctr and one append to *work_ctr.  Every thread of the workgroup calls it;
+// it returns the cell's work-list slot, or -1.
 __device__ inline int cell_walk_needed(long long cell, int ncell, int gy, int gz, const int* __restrict__ cs,
                                        const int* __restrict__ tcount, int2* head, int* pool_ctr, int cap,
                                        int* work_ctr) {
@@ -190,16 +190,31 @@ __device__ inline int cell_walk_needed(long long cell, int ncell, int gy, int gz
             else need = total < kStencilTris ? total : kStencilTris;
         }
     }
+    // the workgroup's reservation and append: wave prefix sums, then one atomic pair per workgroup
+    __shared__ int wsum[kMaxBlock / 64], wcnt[kMaxBlock / 64], wbase2[2];
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int inc = wave_inclusive_scan(need);
-    const int sum = __shfl(inc, 63, 64);
     const unsigned long long mask = __ballot(need > 0);
-    int pbase = 0, wbase = 0;
-    if (lane_id() == 63 && sum > 0) {
-        pbase = atomicAdd(pool_ctr, sum);
-        wbase = atomicAdd(work_ctr, __popcll(mask));
+    if (lane_id() == 63) {
+        wsum[w] = inc;
+        wcnt[w] = __popcll(mask);
     }
-    pbase = __shfl(pbase, 63, 64);
-    wbase = __shfl(wbase, 63, 64);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ps = 0, pc = 0;
+        for (int i = 0; i < nw; ++i) {
+            const int a = wsum[i], b = wcnt[i];
+            wsum[i] = ps;
+            wcnt[i] = pc;
+            ps += a;
+            pc += b;
+        }
+        wbase2[0] = ps > 0 ? atomicAdd(pool_ctr, ps) : 0;
+        wbase2[1] = pc > 0 ? atomicAdd(work_ctr, pc) : 0;
+    }
+    __syncthreads();
+    const int pbase = wbase2[0] + wsum[w], wbase = wbase2[1] + wcnt[w];
+    __syncthreads();   // the next call of the workgroup reuses the arrays
     if (need == 0) return -1;
     const int off = pbase + inc - need;
     head[cell] = off > cap - need ? make_int2(-1, -1) : make_int2(off, -1);
