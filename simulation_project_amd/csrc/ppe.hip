@@ -2864,7 +2864,7 @@ template <typename R> struct OpPDirection {
 };
 // ppe_solve's checks of the final residual: no breakdown, a finite relative residual, and the
 // residual within the tolerance
-template <typename R> struct OpPVerdict {
+template <typename R> struct OpPOutcome {
     const PpeView<R>* pv;
     __device__ void operator()(int j, int) const {
         const PpeView<R>& v = pv[j];
@@ -3073,7 +3073,7 @@ template <typename R> mpsr_status ppe_batch_converged(PpeBatch<R>* b, hipStream_
 template <typename R> void ppe_batch_tail(PpeBatch<R>* b, const BatchLaunch<R>& bl, hipStream_t st) {
     launch_batched<PB>(OpPResid<R>{b->views}, b->wvec, PB, st);
     launch_batched<1024>(OpPReduce<R>{b->views, RED_RESID}, b->wone, 1024, st);
-    launch_batched<64>(OpPVerdict<R>{b->views}, b->wone, 64, st);
+    launch_batched<64>(OpPOutcome<R>{b->views}, b->wone, 64, st);
     launch_batched<PB>(OpPWriteback<R>{bl.devs, b->views}, b->wrow, PB, st);
 }
 // after a batched call: the solvers' host state, and the first bucket whose last solve failed
