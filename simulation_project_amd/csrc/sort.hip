@@ -72,36 +72,55 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tile_sums(const int* in, co
     scan_tile_sums_body(blockIdx.x, in, d_n, n_host, n_cap, sums);
 }
 
-// In-place exclusive scan of m = ceil(n / tile) ints in one workgroup; total to sums[m] and to
-// *d_total.  Used for tile sums (tile = SCAN_TILE) and for radix histograms (tile = 1 with the
-// element count given directly).
+// In-place exclusive scan of m = ceil(n / tile) * mult ints in one workgroup; the total to
+// *d_total.  Used for tile sums (tile = SCAN_TILE) and for radix histograms (tile = the sort tile,
+// mult = the radix).  Each thread owns one run of k consecutive ints (k = m / SCAN_BLOCK rounded
+// up to 4): the runs' sums (16-byte loads, issued together), one block scan of them, then every run
+// rewritten from its start.  Integer sums, so the result does not depend on the split.
 __device__ inline void scan_single_body(int* a, const int* d_n, int n_host, int n_cap, int tile, int mult, int* d_total) {
     const int n = dev_n(d_n, n_host, n_cap);
     const int m = ((n + tile - 1) / tile) * mult;
     __shared__ int lds[SCAN_BLOCK / 64 + 1];
-    int carry = 0;
-    for (int base = 0; base < m; base += SCAN_BLOCK * SCAN_ITEMS) {
-        // each thread owns SCAN_ITEMS consecutive elements
-        int v[SCAN_ITEMS];
-        int local = 0;
-        const int i0 = base + threadIdx.x * SCAN_ITEMS;
-#pragma unroll
-        for (int r = 0; r < SCAN_ITEMS; ++r) {
-            v[r] = (i0 + r < m) ? a[i0 + r] : 0;
-            local += v[r];
+    const int k = ((m + SCAN_BLOCK - 1) / SCAN_BLOCK + 3) & ~3;
+    const int i0 = threadIdx.x * k;
+    const bool vec = (reinterpret_cast<uintptr_t>(a) & 15) == 0;
+    int local = 0;
+#pragma unroll 8
+    for (int r = 0; r < k; r += 4) {
+        const int i = i0 + r;
+        if (vec && i + 3 < m) {
+            const int4 v = *reinterpret_cast<const int4*>(a + i);
+            local += v.x + v.y + v.z + v.w;
+        } else {
+            for (int u = 0; u < 4; ++u)
+                if (i + u < m) local += a[i + u];
         }
-        int total;
-        int pre = block_exclusive_scan<SCAN_BLOCK / 64>(local, lds, &total);
-        int run = carry + pre;
-#pragma unroll
-        for (int r = 0; r < SCAN_ITEMS; ++r) {
-            if (i0 + r < m) a[i0 + r] = run;
-            run += v[r];
+    }
+    int total;
+    int run = block_exclusive_scan<SCAN_BLOCK / 64>(local, lds, &total);
+#pragma unroll 8
+    for (int r = 0; r < k; r += 4) {
+        const int i = i0 + r;
+        if (vec && i + 3 < m) {
+            const int4 v = *reinterpret_cast<const int4*>(a + i);
+            int4 o;
+            o.x = run;
+            o.y = run + v.x;
+            o.z = o.y + v.y;
+            o.w = o.z + v.z;
+            *reinterpret_cast<int4*>(a + i) = o;
+            run = o.w + v.w;
+        } else {
+            for (int u = 0; u < 4; ++u)
+                if (i + u < m) {
+                    const int v = a[i + u];
+                    a[i + u] = run;
+                    run += v;
+                }
         }
-        carry += total;
     }
     if (threadIdx.x == 0) {
-        if (d_total) *d_total = carry;
+        if (d_total) *d_total = total;
     }
 }
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(int* a, const int* d_n, int n_host, int n_cap, int tile, int mult,
